@@ -1,0 +1,295 @@
+"""ctypes binding of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Loads ``oracle/liboracle.so`` (built by ``oracle/Makefile``) and exposes
+numpy-level wrappers.  Only tests/, ``__graft_entry__.smoke()`` and the CPU
+baseline leg of ``bench.py`` import this module; the product package never
+does.
+"""
+
+from __future__ import annotations
+
+import ctypes as ct
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_d = ct.POINTER(ct.c_double)
+_f = ct.POINTER(ct.c_float)
+_i = ct.POINTER(ct.c_int)
+_l = ct.POINTER(ct.c_long)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        _LIB = ct.CDLL(path)
+        L = _LIB
+        L.or_spline_integral.restype = ct.c_double
+        L.or_proj_interpolate.restype = ct.c_double
+        L.or_legendre.restype = ct.c_double
+        L.or_legendre.argtypes = [ct.c_int, ct.c_int, ct.c_double]
+        L.or_ylm.argtypes = [ct.c_int, ct.c_int, ct.c_double, ct.c_double, _d]
+        L.or_proj_interpolate.argtypes = [ct.c_double, ct.c_double, ct.c_int, _d, _d, _d, _d, _d]
+        L.or_site_capacity.argtypes = [_d, _i, ct.c_double]
+        L.or_site_table.argtypes = [_d, _i, _d, ct.c_double, ct.c_int, _d, ct.c_int, _i, _d, _d, _d, _d,
+                                    ct.c_int, _i, _d, _d]
+    return _LIB
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+# ---------------------------------------------------------------- splines
+def spline_coeff(x, y):
+    x = _c(x, np.float64)
+    y = _c(y, np.float64)
+    n = len(x)
+    b, c, d = (np.zeros(n) for _ in range(3))
+    lib().or_spline_coeff(_p(x, _d), _p(y, _d), ct.c_int(n), _p(b, _d), _p(c, _d), _p(d, _d))
+    return b, c, d
+
+
+def spline_integral(x, a, b, c, d):
+    x, a, b, c, d = (_c(v, np.float64) for v in (x, a, b, c, d))
+    return lib().or_spline_integral(_p(x, _d), _p(a, _d), _p(b, _d), _p(c, _d), _p(d, _d), ct.c_int(len(x)))
+
+
+def proj_interpolate(r, rmax, x, f, coeffs):
+    x = _c(x, np.float64)
+    f = _c(f, np.float64)
+    b, c, d = (_c(v, np.float64) for v in coeffs)
+    return lib().or_proj_interpolate(float(r), float(rmax), len(x), _p(x, _d), _p(f, _d), _p(b, _d), _p(c, _d),
+                                     _p(d, _d))
+
+
+def legendre(l, m, x):
+    return lib().or_legendre(int(l), int(m), float(x))
+
+
+def ylm(l, m, theta, phi):
+    out = np.zeros(2)
+    lib().or_ylm(int(l), int(m), float(theta), float(phi), _p(out, _d))
+    return complex(out[0], out[1])
+
+
+# ---------------------------------------------------------------- sphere tables
+class SiteTables:
+    """CSR sphere tables of a whole structure (all sites, one species set)."""
+
+    def __init__(self, site_pt, idx, path, values, val_off, nproj, proj_off, meta):
+        self.site_pt = site_pt
+        self.idx = idx
+        self.path = path
+        self.values = values          # complex128, per site [P_s, n_s] flattened
+        self.val_off = val_off
+        self.nproj = nproj
+        self.proj_off = proj_off      # length nsites + 1
+        self.meta = meta              # per species (n, l, m) arrays
+
+    @property
+    def nsites(self):
+        return len(self.nproj)
+
+    @property
+    def ptot(self):
+        return int(self.proj_off[-1])
+
+
+def species_spline(sp):
+    """(x, f[nchan, n], b, c, d) spline data of a species' projectors."""
+    x = np.asarray(sp.projgrid, np.float64)
+    f = np.stack([np.asarray(p, np.float64) for p in sp.realprojs])
+    bcd = [spline_coeff(x, fi) for fi in f]
+    b = np.stack([t[0] for t in bcd])
+    c = np.stack([t[1] for t in bcd])
+    d = np.stack([t[2] for t in bcd])
+    return x, f, b, c, d
+
+
+def projector_meta(ls):
+    ns, lls, ms = [], [], []
+    for ch, l in enumerate(ls):
+        for m in range(-l, l + 1):
+            ns.append(ch)
+            lls.append(l)
+            ms.append(m)
+    return np.array(ns, np.int32), np.array(lls, np.int32), np.array(ms, np.int32)
+
+
+def site_tables(lattice, fftg, frac, species, labels=None):
+    """utils.c:590-696 for every site; species is one Species or a list
+    indexed by ``labels``."""
+    L = lib()
+    lattice = _c(lattice, np.float64)
+    fftg = _c(fftg, np.int32)
+    frac = _c(frac, np.float64)
+    sps = species if isinstance(species, (list, tuple)) else [species]
+    labels = np.zeros(len(frac), np.int32) if labels is None else _c(labels, np.int32)
+    spl = [species_spline(sp) for sp in sps]
+    idxs, paths, vals, npts, nproj = [], [], [], [], []
+    for s in range(len(frac)):
+        sp = sps[labels[s]]
+        x, f, b, c, d = spl[labels[s]]
+        ls = _c(sp.ls, np.int32)
+        P = int(sum(2 * l + 1 for l in sp.ls))
+        cap = L.or_site_capacity(_p(lattice, _d), _p(fftg, _i), ct.c_double(sp.rmax))
+        idx = np.zeros(cap, np.int32)
+        path = np.zeros(3 * cap)
+        values = np.zeros(2 * P * cap)
+        coord = _c(frac[s], np.float64)
+        n = L.or_site_table(_p(lattice, _d), _p(fftg, _i), _p(coord, _d), ct.c_double(sp.rmax), len(x),
+                            _p(x, _d), len(ls), _p(ls, _i), _p(f, _d), _p(b, _d), _p(c, _d), _p(d, _d), cap,
+                            _p(idx, _i), _p(path, _d), _p(values, _d))
+        assert n >= 0
+        v = values.view(np.complex128).reshape(P, cap)[:, :n]
+        idxs.append(idx[:n])
+        paths.append(path[: 3 * n].reshape(n, 3))
+        vals.append(np.ascontiguousarray(v).ravel())
+        npts.append(n)
+        nproj.append(P)
+    site_pt = np.concatenate([[0], np.cumsum(npts)]).astype(np.int32)
+    val_off = np.concatenate([[0], np.cumsum([len(v) for v in vals])[:-1]]).astype(np.int64)
+    proj_off = np.concatenate([[0], np.cumsum(nproj)]).astype(np.int32)
+    meta = [projector_meta(sp.ls) for sp in sps]
+    return SiteTables(site_pt, np.concatenate(idxs).astype(np.int32), np.concatenate(paths),
+                      np.concatenate(vals), val_off, np.array(nproj, np.int32), proj_off, meta)
+
+
+# ---------------------------------------------------------------- FFT
+def fft3d(lattice, G, C, fftg):
+    """linalg.c:14-45 for one band: c64 coefficients -> c128 grid [N0,N1,N2]."""
+    fftg = _c(fftg, np.int32)
+    G = _c(G, np.int32)
+    C = _c(C, np.complex64)
+    x = np.zeros(int(np.prod(fftg)), np.complex128)
+    lib().or_fft3d(_p(x, _d), _p(_c(lattice, np.float64), _d), _p(G, _i), _p(C, _f), ct.c_int(len(G)),
+                   _p(fftg, _i))
+    return x.reshape(tuple(fftg))
+
+
+def ifft3d(x):
+    """Unscaled backward DFT (e^{+i}) of a c128 grid."""
+    y = np.ascontiguousarray(x, np.complex128).copy()
+    fftg = np.array(y.shape, np.int32)
+    lib().or_ifft3d_inplace(_p(y, _d), _p(fftg, _i))
+    return y
+
+
+# ---------------------------------------------------------------- sphere sums
+def onto_projector(grid, lattice, kfrac, fftg, T: SiteTables):
+    grid = _c(grid, np.complex128).ravel()
+    out = np.zeros(T.ptot, np.complex128)
+    lib().or_onto_projector(_p(grid, _d), _p(_c(lattice, np.float64), _d), _p(_c(kfrac, np.float64), _d),
+                            _p(_c(fftg, np.int32), _i), ct.c_int(T.nsites), _p(T.site_pt, _i), _p(T.nproj, _i),
+                            _p(T.val_off, _l), _p(T.proj_off, _i), _p(T.idx, _i), _p(T.path, _d),
+                            _p(T.values, _d), _p(out, _d))
+    return out
+
+
+def setup_projections(lattice, fftg, kpts, Gs, Cs, T: SiteTables, pairs=None, nthreads=0):
+    """projector.c:560-602 hot loop.  Cs: per k complex64 [nband, W_k].
+    Returns c128 [nk, nband, ptot] (rows not in ``pairs`` stay zero)."""
+    nk = len(Gs)
+    nband = Cs[0].shape[0]
+    ldc = max(c.shape[1] for c in Cs)
+    C = np.zeros((nk, nband, ldc), np.complex64)
+    for k in range(nk):
+        C[k, :, : Cs[k].shape[1]] = Cs[k]
+    nw = np.array([len(g) for g in Gs], np.int32)
+    Gc = [_c(g, np.int32) for g in Gs]
+    Gp = (ct.POINTER(ct.c_int) * nk)(*[_p(g, _i) for g in Gc])
+    kp = _c(kpts, np.float64)
+    out = np.zeros((nk, nband, T.ptot), np.complex128)
+    if pairs is not None:
+        pr = _c(pairs, np.int32).ravel()
+        npairs = len(pr) // 2
+        prp = _p(pr, _i)
+    else:
+        npairs, prp = 0, None
+    lib().or_setup_projections(_p(_c(lattice, np.float64), _d), _p(_c(fftg, np.int32), _i), ct.c_int(nk),
+                               _p(kp, _d), _p(nw, _i), Gp, _p(C, _f), ct.c_long(ldc), ct.c_int(nband),
+                               ct.c_int(T.nsites), _p(T.site_pt, _i), _p(T.nproj, _i), _p(T.val_off, _l),
+                               _p(T.proj_off, _i), ct.c_int(T.ptot), _p(T.idx, _i), _p(T.path, _d),
+                               _p(T.values, _d), ct.c_int(npairs), prp, _p(out, _d), ct.c_int(nthreads))
+    return out
+
+
+# ---------------------------------------------------------------- overlaps
+def pseudoprojection(C_ref, C_proj, band, nwk, nspin, flip=False, fp32acc=False):
+    """pseudoprojector.c:63-90.  C_ref/C_proj: per k-index lists of
+    complex64 [nband, W_k] (k-index = k + spin*nwk).  Returns the
+    reference-ordered c128 vector res[b*nkt + k]."""
+    nkt = nwk * nspin
+    nbr = C_ref[0].shape[0]
+    nbp = C_proj[0].shape[0]
+    ldc = max(c.shape[1] for c in C_ref)
+    A = np.zeros((nkt, nbr, ldc), np.complex64)
+    B = np.zeros((nkt, nbp, ldc), np.complex64)
+    for k in range(nkt):
+        A[k, :, : C_ref[k].shape[1]] = C_ref[k]
+        B[k, :, : C_proj[k].shape[1]] = C_proj[k]
+    nw = np.array([c.shape[1] for c in C_ref], np.int32)
+    out = np.zeros(nbr * nkt, np.complex128)
+    lib().or_pseudoprojection(_p(out, _d), _p(A, _f), _p(B, _f), ct.c_long(ldc), nbr, nbp, nwk, nspin,
+                              _p(nw, _i), int(band), int(bool(flip)), int(bool(fp32acc)))
+    return out
+
+
+def compensation_terms_m(overlap, band, nwk, nspin, proj_R, T_R: SiteTables, labels_R, proj_S_band, T_S,
+                         M_R, M_S, species_dO, species_ls, flip=False):
+    """O_M of projector.c:850-910 added into ``overlap`` (c128 [nbR*nkt]).
+
+    proj_R: c128 [nkt, nbR, ptotR]; proj_S_band: c128 [nkt, ptotS] (the
+    projected band's rows); species_dO: list of channel matrices
+    (aepw - pspw) per species; species_ls: list of channel l lists."""
+    nkt = nwk * nspin
+    nbR = proj_R.shape[1]
+    metas = [projector_meta(ls) for ls in species_ls]
+    meta_n = np.concatenate([m[0] for m in metas]).astype(np.int32)
+    meta_l = np.concatenate([m[1] for m in metas]).astype(np.int32)
+    meta_m = np.concatenate([m[2] for m in metas]).astype(np.int32)
+    sp_meta_off = np.concatenate([[0], np.cumsum([len(m[0]) for m in metas])[:-1]]).astype(np.int32)
+    sp_nchan = np.array([len(ls) for ls in species_ls], np.int32)
+    dO = np.concatenate([np.asarray(d, np.float64).ravel() for d in species_dO])
+    sp_do_off = np.concatenate([[0], np.cumsum([np.asarray(d).size for d in species_dO])[:-1]]).astype(np.int32)
+    ov = _c(overlap, np.complex128)
+    pr = _c(proj_R, np.complex128)
+    ps = _c(proj_S_band, np.complex128)
+    M_R = _c(M_R, np.int32)
+    M_S = _c(M_S, np.int32)
+    lab = _c(labels_R, np.int32)
+    lib().or_compensation_terms_m(_p(ov, _d), int(band), nwk, nspin, nbR, _p(pr, _d), T_R.ptot,
+                                  _p(T_R.proj_off, _i), _p(lab, _i), _p(ps, _d), T_S.ptot, _p(T_S.proj_off, _i),
+                                  len(M_R), _p(M_R, _i), _p(M_S, _i), _p(sp_nchan, _i), _p(sp_meta_off, _i),
+                                  _p(meta_n, _i), _p(meta_l, _i), _p(meta_m, _i), _p(sp_do_off, _i), _p(dO, _d),
+                                  int(bool(flip)))
+    return ov
+
+
+def pwave_overlaps(grid, ls, aewaves, pswaves):
+    grid = _c(grid, np.float64)
+    n = len(grid)
+    nch = len(ls)
+    ae = _c(np.stack(aewaves), np.float64)
+    ps = _c(np.stack(pswaves), np.float64)
+    lsa = _c(ls, np.int32)
+    psov, aeov, diov = (np.zeros(nch * nch) for _ in range(3))
+    lib().or_pwave_overlaps(_p(grid, _d), n, nch, _p(lsa, _i), _p(ae, _d), _p(ps, _d), _p(psov, _d),
+                            _p(aeov, _d), _p(diov, _d))
+    return psov.reshape(nch, nch), aeov.reshape(nch, nch), diov.reshape(nch, nch)
