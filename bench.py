@@ -1,0 +1,356 @@
+#!/usr/bin/env python3
+"""Benchmark of the pawpyseed projection hot path on one MI355X.
+
+Workload (216-atom defect supercell): diamond-lattice cubic cell a = 16.29 A
+with one vacancy and its 4 neighbours moved 0.15 A inward (S, 215 sites)
+projected onto the bulk cell (R, 216 sites); 520 eV cutoff (116,329 plane
+waves per k), 108^3 FFT grid, 2x2x1 Monkhorst-Pack (4 k), 512 bands per k,
+18 projectors per site (l = 0,1,2 x 2 radial channels, rmax 1.6 A).
+Synthetic seeded inputs (no VASP data can be shipped).
+
+One step = the full projection of S onto R:
+  (1) setup_projections of S: sphere tables, then scatter -> 3D FFT ->
+      sphere sums for all 512 x 4 (band, k);      -> band-projections/s
+  (2) band-overlap matrix <R b,k | S b',k> of all pairs as one GEMM per k
+                                                    -> overlap TFLOP/s
+  (3) O_M augmentation of all pairs (FP64).
+The basis R is set up once before timing (one basis serves many projected
+structures).  Paths: "fp32" (FFT, sphere sums and overlap GEMM in FP32 on
+MFMA, FP64 reductions and augmentation) is the headline `value`; "fp64" is
+the plain FP64 path; "bf16x3" re-runs the overlap GEMM as split-bf16 MFMA.
+
+Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM = 8.0e12
+PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16": 2.5e15}
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def make_coefficients(torch, wl, Gs, seed, nband, ldc):
+    """c_G ~ CN(0,1) exp(-|k+G|^2 / (2 (Gcut/2)^2)), normalised per band,
+    generated on the GPU with a seeded generator."""
+    from pawpyseed_amd import synthetic
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    rec = synthetic.reciprocal_lattice(wl.lattice)
+    gcut = math.sqrt(synthetic.C_EV * wl.encut)
+    C = torch.zeros((len(Gs), nband, ldc), dtype=torch.complex64, device="cuda")
+    for k, (kp, G) in enumerate(zip(wl.kpts, Gs)):
+        kg = (G.astype(np.float64) + kp[None, :]) @ rec
+        env = np.exp(-np.einsum("ij,ij->i", kg, kg) / (2.0 * (0.5 * gcut) ** 2)).astype(np.float32)
+        envt = torch.from_numpy(env).cuda()
+        re = torch.randn((nband, len(G)), generator=gen, device="cuda")
+        im = torch.randn((nband, len(G)), generator=gen, device="cuda")
+        c = torch.complex(re * envt, im * envt)
+        c /= torch.linalg.vector_norm(c, dim=1, keepdim=True)
+        C[k, :, : len(G)] = c
+        del re, im, c
+    return C
+
+
+def cpu_cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except Exception:
+        n = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit():
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nband", type=int, default=512)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline work")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle checks")
+    ap.add_argument("--paths", default="fp32,fp64", help="comma list of fp32, fp64 setup/overlap paths")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the projection path shards by k-point, see the design notes)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from pawpyseed_amd import synthetic
+    from pawpyseed_amd.core import batched as B
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    wl = synthetic.config1(nband=args.nband)
+    Gs = wl.ensure_gvecs()
+    nk = len(Gs)
+    nb = args.nband
+    nw = np.array([len(g) for g in Gs], np.int32)
+    ldc = int(-(-nw.max() // 64) * 64)
+    gofs = np.concatenate([[0], np.cumsum(nw)]).astype(np.int32)
+    Gdev = torch.from_numpy(np.ascontiguousarray(np.concatenate(Gs), np.int32)).cuda()
+    log(f"workload: {len(wl.frac_R)}/{len(wl.frac_S)} sites, W={nw.tolist()}, grid={wl.fftg.tolist()}, "
+        f"nk={nk}, nband={nb}")
+    CR = make_coefficients(torch, wl, Gs, 1001, nb, ldc)
+    CS = make_coefficients(torch, wl, Gs, 2002, nb, ldc)
+    torch.cuda.synchronize()
+
+    stream = torch.cuda.Stream()
+    ctx = B.Context(0, stream=stream)
+    labels_R = np.zeros(len(wl.frac_R), np.int32)
+    labels_S = np.zeros(len(wl.frac_S), np.int32)
+    paths = [p for p in args.paths.split(",") if p]
+    plans, projR, projS, Om = {}, {}, {}, {}
+    for p in paths:
+        plans[p] = (B.SpherePlan(ctx, wl.lattice, wl.fftg, wl.frac_R, labels_R, [wl.species], p),
+                    B.SpherePlan(ctx, wl.lattice, wl.fftg, wl.frac_S, labels_S, [wl.species], p))
+        pR, pS = plans[p]
+        projR[p] = torch.zeros((nk, nb, pR.ptot), dtype=torch.complex128, device="cuda")
+        projS[p] = torch.zeros((nk, nb, pS.ptot), dtype=torch.complex128, device="cuda")
+        Om[p] = torch.zeros((nk, nb, nb), dtype=torch.complex128, device="cuda")
+        with torch.cuda.stream(stream):
+            B.setup_projections(ctx, pR, wl.kpts, gofs, Gdev, CR, projR[p], p)  # basis, once
+    ptotS = plans[paths[0]][1].ptot
+    ptotR = plans[paths[0]][0].ptot
+
+    def step(p, ev):
+        pR, pS = plans[p]
+        with torch.cuda.stream(stream):
+            ev[0].record(stream)
+            pS.rebuild(wl.frac_S)
+            B.setup_projections(ctx, pS, wl.kpts, gofs, Gdev, CS, projS[p], p, sync=False)
+            ev[1].record(stream)
+            B.overlap_gemm(ctx, nk, 1, False, nw, CR, CS, Om[p], p, sync=False)
+            ev[2].record(stream)
+            B.augmentation(ctx, nk, 1, False, projR[p], pR.proj_off, labels_R, projS[p], pS.proj_off, labels_S,
+                           wl.M_R, wl.M_S, [wl.species], [wl.delta_o], Om[p], sync=False)
+            ev[3].record(stream)
+
+    results = {}
+    for p in paths:
+        for _ in range(args.warmup):
+            evs = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+            step(p, evs)
+        torch.cuda.synchronize()
+        ctx.set_timing(True)
+        ctx.timing_reset()
+        rec = []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            evs = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+            step(p, evs)
+            rec.append(evs)
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / args.steps
+        kt = ctx.timing()
+        ctx.set_timing(False)
+        ts = np.mean([e[0].elapsed_time(e[1]) for e in rec]) / 1e3
+        to = np.mean([e[1].elapsed_time(e[2]) for e in rec]) / 1e3
+        ta = np.mean([e[2].elapsed_time(e[3]) for e in rec]) / 1e3
+        nproj = nb * nk * ptotS
+        ovl_flops = 8.0 * nb * nb * float(nw.sum())
+        results[p] = dict(setup_s=ts, overlap_s=to, aug_s=ta, step_wall_s=wall,
+                          band_projections_per_s=nproj / ts, overlap_tflops=ovl_flops / to / 1e12,
+                          kernels={k: {"ms_total": v[0] / args.steps, "launches": v[1] // args.steps}
+                                   for k, v in kt.items()})
+        log(p, json.dumps({k: v for k, v in results[p].items() if k != "kernels"}))
+        log(p, "kernels", json.dumps(results[p]["kernels"]))
+
+    # overlap as 3-term split bf16 on MFMA (same inputs)
+    ob = torch.zeros((nk, nb, nb), dtype=torch.complex128, device="cuda")
+    with torch.cuda.stream(stream):
+        B.overlap_gemm(ctx, nk, 1, False, nw, CR, CS, ob, "bf16x3")  # warm
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.steps):
+            B.overlap_gemm(ctx, nk, 1, False, nw, CR, CS, ob, "bf16x3", sync=False)
+        e1.record(stream)
+    torch.cuda.synchronize()
+    t_bf = e0.elapsed_time(e1) / 1e3 / args.steps
+    results["bf16x3"] = dict(overlap_s=t_bf, overlap_tflops=8.0 * nb * nb * float(nw.sum()) / t_bf / 1e12)
+    log("bf16x3 overlap", results["bf16x3"])
+
+    head = "fp32" if "fp32" in results else paths[0]
+    R = results[head]
+    # dominant kernel of the headline path and its roofline
+    prec_bytes = 8 if head != "fp64" else 16
+    npts_grid = int(np.prod(wl.fftg))
+    NB = 64
+    groups = {"k_fft_pass": ["fft_x", "fft_y", "fft_z"], "k_sphere_sum": ["sphere_sum"], "gemm": ["gemm"]}
+    tot = {g: sum(R["kernels"].get(c, {}).get("ms_total", 0.0) for c in cs) for g, cs in groups.items()}
+    launches = {g: sum(R["kernels"].get(c, {}).get("launches", 0) for c in cs) for g, cs in groups.items()}
+    dom = max(tot, key=tot.get)
+    per_launch_ms = {g: tot[g] / max(1, launches[g]) for g in groups}
+    fft_bytes = 2.0 * npts_grid * NB * prec_bytes
+    planS = plans[head][1]
+    sphere_bytes = planS.npts * NB * prec_bytes + planS.npts * 18 * prec_bytes
+    gemm_flops = 8.0 * nb * nb * float(nw.mean())
+    rl = {
+        "k_fft_pass": {"bound": "hbm", "achieved": fft_bytes / (per_launch_ms["k_fft_pass"] * 1e-3) / 1e9,
+                       "peak": PEAK_HBM / 1e9, "unit": "GB/s", "per_launch": f"{fft_bytes:.4g} B"},
+        "k_sphere_sum": {"bound": "hbm",
+                         "achieved": sphere_bytes / (per_launch_ms["k_sphere_sum"] * 1e-3) / 1e9,
+                         "peak": PEAK_HBM / 1e9, "unit": "GB/s", "per_launch": f"{sphere_bytes:.4g} B"},
+        "gemm": {"bound": "mfma", "achieved": gemm_flops / (per_launch_ms["gemm"] * 1e-3) / 1e12,
+                 "peak": PEAK["fp32" if head != "fp64" else "fp64"] / 1e12, "unit": "TFLOP/s",
+                 "per_launch": f"{gemm_flops:.4g} flop"},
+    }
+    for v in rl.values():
+        v["frac"] = v["achieved"] / v["peak"]
+    roof = dict(rl[dom])
+    roof["kernel"] = dom
+    roof["avg_launch_ms"] = per_launch_ms[dom]
+    roof["traffic"] = None
+    roof.pop("per_launch", None)
+
+    out = {
+        "metric": "band_projections_per_sec",
+        "value": R["band_projections_per_s"],
+        "unit": "band-projections/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": (R["setup_s"] + R["overlap_s"] + R["aug_s"]) * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 (FFT, sphere sums, overlap GEMM on MFMA; fp64 reductions and augmentation)"
+        if head == "fp32" else head,
+        "data": "synthetic (seeded)",
+        "config": {"workload": "216-atom diamond defect supercell, a=16.29 A, 520 eV (W=116329/k), 108^3 grid, "
+                               "2x2x1 MP (4 k), 512 bands/k, 18 projectors/site, S (215 sites) onto R (216)",
+                   "nband": nb, "nk": nk, "fftg": wl.fftg.tolist(), "plane_waves": nw.tolist(),
+                   "projectors_S": ptotS, "projectors_R": ptotR},
+        "overlap_tflops": R["overlap_tflops"],
+        "paths": {p: {k: v for k, v in r.items() if k != "kernels"} for p, r in results.items()},
+        "kernels": rl,
+        "roofline": roof,
+    }
+    if "fp64" in results:
+        out["speedup_vs_fp64_gpu"] = R["band_projections_per_s"] / results["fp64"]["band_projections_per_s"]
+
+    if not args.no_cpu:
+        out.update(cpu_and_errors(args, torch, wl, Gs, CR, CS, projR, projS, Om, ob, plans, results, head))
+    print(json.dumps(out), flush=True)
+
+
+def cpu_and_errors(args, torch, wl, Gs, CR, CS, projR, projS, Om, ob, plans, results, head):
+    """CPU baseline (the oracle on the host cores, a bounded sample) and
+    errors of every path against the FP64 oracle on sampled entries."""
+    from oracle import pyoracle as orc
+    nk = len(Gs)
+    nb = CR.shape[1]
+    cores = cpu_cores()
+    # S sphere tables (the per-structure setup of the CPU path)
+    t0 = time.perf_counter()
+    TS = orc.site_tables(wl.lattice, wl.fftg, wl.frac_S, wl.species)
+    t_tab = time.perf_counter() - t0
+    SB = sorted({0, 1, nb // 2, nb - 1})
+    RB = sorted({0, 7, nb // 3})
+    CSh = [CS[k, :, : len(Gs[k])].cpu().numpy() for k in range(nk)]
+    # calibrate, then run a sample of (k, band) pairs sized to ~cpu-seconds
+    pairs = [(k, b) for b in SB for k in range(nk)]
+    t0 = time.perf_counter()
+    P_or = orc.setup_projections(wl.lattice, wl.fftg, wl.kpts, Gs, CSh, TS, pairs=np.array(pairs, np.int32),
+                                 nthreads=cores)
+    t_first = time.perf_counter() - t0
+    per_pair = t_first / len(pairs)
+    extra = int(max(0, (args.cpu_seconds - t_first - t_tab) / max(per_pair, 1e-3)))
+    extra = min(extra, nk * nb - len(pairs))
+    rng = np.random.default_rng(5)
+    done = set(pairs)
+    more = [p for p in ((int(rng.integers(nk)), int(rng.integers(nb))) for _ in range(4 * extra + 8))
+            if p not in done][:extra]
+    t_more = 0.0
+    if more:
+        t0 = time.perf_counter()
+        orc.setup_projections(wl.lattice, wl.fftg, wl.kpts, Gs, CSh, TS, pairs=np.array(more, np.int32),
+                              nthreads=cores)
+        t_more = time.perf_counter() - t0
+    npairs = len(pairs) + len(more)
+    t_pairs = t_first + t_more
+    t_cpu_all = t_tab + t_pairs / npairs * nk * nb
+    cpu_value = nb * nk * TS.ptot / t_cpu_all
+    log(f"cpu: tables {t_tab:.2f}s, {npairs} pairs in {t_pairs:.2f}s on {cores} threads -> "
+        f"{cpu_value:.4g} band-projections/s")
+
+    # CPU overlap: the reference's FP32-accumulating cdotc, one projected band
+    CRh = [CR[k, :, : len(Gs[k])].cpu().numpy() for k in range(nk)]
+    t0 = time.perf_counter()
+    ov_or = orc.pseudoprojection(CRh, CSh, SB[1], nk, 1, fp32acc=True)
+    t_ov = time.perf_counter() - t0
+    cpu_ov_tflops = 8.0 * nb * float(sum(len(g) for g in Gs)) / t_ov / 1e12
+
+    # errors vs the FP64 oracle
+    errs = {}
+    for p in projS:
+        g = projS[p].cpu().numpy()
+        e = {}
+        for k in range(nk):
+            ref = P_or[k, SB]
+            d = g[k, SB] - ref
+            e[k] = dict(max_rel=float(np.abs(d).max() / np.abs(ref).max()),
+                        rms_rel=float(np.sqrt(np.mean(np.abs(d) ** 2)) / np.sqrt(np.mean(np.abs(ref) ** 2))))
+        errs[f"projections_{p}"] = e
+    # overlap: FP64 oracle on sampled rows/columns
+    CRs = [c[RB] for c in CRh]
+    CSs = [c[SB] for c in CSh]
+    ovref = np.zeros((nk, len(RB), len(SB)), np.complex128)
+    for j in range(len(SB)):
+        ovref[:, :, j] = orc.pseudoprojection(CRs, CSs, j, nk, 1).reshape(len(RB), nk).T
+    # augmentation of the sampled pairs by the oracle, on FP64 oracle projections
+    TR = orc.site_tables(wl.lattice, wl.fftg, wl.frac_R, wl.species)
+    prR = orc.setup_projections(wl.lattice, wl.fftg, wl.kpts, Gs, [c[RB] for c in CRh], TR, nthreads=cores)
+    prS = P_or[:, SB]
+    aug = np.zeros((nk, len(RB), len(SB)), np.complex128)
+    for j in range(len(SB)):
+        a = orc.compensation_terms_m(np.zeros(len(RB) * nk, complex), 0, nk, 1, prR, TR,
+                                     np.zeros(len(wl.frac_R), np.int32), prS[:, j], TS, wl.M_R, wl.M_S,
+                                     [wl.delta_o], [wl.species.ls])
+        aug[:, :, j] = a.reshape(len(RB), nk).T
+    total_ref = ovref + aug
+    obh = ob.cpu().numpy()[:, RB][:, :, SB]
+    errs["overlap_bf16x3_pseudo"] = {k: float(np.abs(obh[k] - ovref[k]).max() / np.abs(ovref[k]).max())
+                                     for k in range(nk)}
+    for p in Om:
+        o = Om[p].cpu().numpy()[:, RB][:, :, SB]
+        errs[f"overlap_total_{p}"] = {
+            k: dict(max_rel=float(np.abs(o[k] - total_ref[k]).max() / np.abs(total_ref[k]).max()),
+                    rms_rel=float(np.sqrt(np.mean(np.abs(o[k] - total_ref[k]) ** 2))
+                                  / np.sqrt(np.mean(np.abs(total_ref[k]) ** 2))))
+            for k in range(nk)}
+    worst = {name: max((v["max_rel"] if isinstance(v, dict) else v) for v in e.values()) for name, e in errs.items()}
+    log("errors (max over k of max|d|/max|ref|):", json.dumps(worst))
+    return {
+        "cpu_baseline": {"value": cpu_value, "unit": "band-projections/s", "cores": cores, "kind": "port",
+                         "sample": f"oracle on the host: S sphere tables (215 sites, {t_tab:.1f}s) + scatter/FFT/"
+                                   f"sphere sums of {npairs} of {nk * nb} (band,k) pairs in {t_pairs:.1f}s, "
+                                   f"extrapolated to all pairs",
+                         "overlap_tflops": cpu_ov_tflops,
+                         "overlap_sample": f"FP32-accumulated dots of 1 projected band vs {nb}x{nk} basis bands"},
+        "max_err_vs_fp64_oracle": worst,
+        "errors_per_k": errs,
+        "speedup_vs_cpu": results[head]["band_projections_per_s"] / cpu_value,
+    }
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,223 @@
+/*
+ * pawpy_abi.h -- C-ABI of libpawpy_amd.so, the MI355X (gfx950) projection
+ * path of pawpyseed.
+ *
+ * Two layers:
+ *  1. Reference-named entry points.  Same symbol names and argument
+ *     meaning as the reference's C core, so the reference's Python layer
+ *     (through its Cython externs, pawpyc_extern.pxd, or the ctypes
+ *     binding in pawpyseed_amd.core) can call them unchanged.  Functions
+ *     the reference declares `void` return an int status here (0 = OK);
+ *     on x86-64 a caller that ignores the return value is unaffected.
+ *     Errors never exit the process: the status is returned and
+ *     pp_last_error() gives the message.
+ *  2. Batched device-resident entry points (pp_*): plain device pointers
+ *     and sizes, asynchronous on the context's stream.
+ *
+ * Complex data are interleaved (re, im) pairs; c64 = two floats, c128 =
+ * two doubles.
+ */
+#ifndef PAWPY_ABI_H
+#define PAWPY_ABI_H
+
+#include "pawpy_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define PP_OK 0
+#define PP_ERR_ARG 1
+#define PP_ERR_HIP 2
+#define PP_ERR_NOMEM 3
+#define PP_ERR_UNSUPPORTED 4
+#define PP_ERR_IO 5
+
+/* arithmetic of the hot path */
+#define PP_FP64 0   /* FP64 everywhere (the plain path, the parity path) */
+#define PP_FP32 1   /* FFT, sphere sums and overlap GEMM in FP32 (MFMA), FP64 reductions */
+#define PP_BF16X3 2 /* as PP_FP32, overlap GEMM as 3-term split bf16 MFMA products */
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *pp_last_error(void);
+int pp_last_status(void);
+/* Library version string. */
+const char *pp_version(void);
+/* Precision used by the reference-named entry points (default PP_FP64). */
+int pp_set_precision(int precision);
+int pp_get_precision(void);
+
+/* ===================================================================== */
+/* 1. Reference-named entry points                                       */
+/* ===================================================================== */
+
+/* replaces reader.c:317-323 read_wavefunctions (WAVECAR reader; G vectors
+ * enumerated as reader.c:231-271). Returns NULL on error. */
+pswf_t *read_wavefunctions(char *filename, double *kpt_weights);
+/* replaces reader.c:325-330 read_wavefunctions_from_str (in-memory WAVECAR) */
+pswf_t *read_wavefunctions_from_str(char *start, double *kpt_weights);
+
+/* No reference counterpart: builds a pswf_t from in-memory arrays (the
+ * data a WAVECAR holds).  kpts [nwk][3]; weights [nwk]; nplane [nspin*nwk];
+ * Gs: concatenated int32 [sum nplane][3]; coeffs: concatenated c64
+ * [sum over (spin,k) of nband*nplane] (band-major per k);
+ * occ, energy: [nspin*nwk][nband].  k-index = k + spin*nwk. */
+pswf_t *pp_pswf_from_arrays(int nspin, int nwk, int nband, double encut,
+                            const double *lattice, const double *kpts,
+                            const double *weights, const int *nplane,
+                            const int *Gs, const pp_cfloat *coeffs,
+                            const double *occ, const double *energy);
+
+/* replaces utils.c:275-301 free_pswf */
+void free_pswf(pswf_t *wf);
+/* replaces utils.c:322-327 free_ppot_list */
+void free_ppot_list(ppot_t *pps, int length);
+/* replace the accessors of utils.c:343-373 */
+double *get_occs(pswf_t *wf);
+int get_nband(pswf_t *wf);
+int get_nwk(pswf_t *wf);
+int get_nspin(pswf_t *wf);
+int is_ncl(pswf_t *wf);
+double get_encut(pswf_t *wf);
+double get_energy(pswf_t *wf, int band, int kpt, int spin);
+double get_occ(pswf_t *wf, int band, int kpt, int spin);
+void set_num_sites(pswf_t *wf, int nsites);
+void free_ptr(void *ptr);
+
+/* replaces projector.c:20-171 get_projector_list: projector splines
+ * (utils.c:699-749) and PAW partial-wave overlap matrices
+ * (projector.c:507-558).  The SBT-filtered smooth_diffwave used only by the
+ * O_R/O_S terms is not built (left NULL). */
+ppot_t *get_projector_list(int num_els, int *labels, int *ls,
+                           double *wave_grids, double *projectors,
+                           double *aewaves, double *pswaves, double *rmaxs,
+                           double grid_encut);
+
+/* replaces projector.c:560-602 setup_projections: sphere tables
+ * (utils.c:590-696), then scatter -> 3D FFT -> sphere sums for every
+ * (band, k) on the GPU.  Takes ownership of pps as the reference does.
+ * Fills band->projections of every band (host copies) and keeps the
+ * projections resident in HBM for compensation_terms. */
+int setup_projections(pswf_t *wf, ppot_t *pps, int num_elems, int num_sites,
+                      int *fftg, int *labels, double *coords);
+
+/* replaces pseudoprojector.c:63-90 pseudoprojection:
+ * projections[b*NK + k] = <C_ref[k'][b] | C_proj[k][BAND_NUM]>.
+ * The first call for a (wf_ref, wf_proj, flip_spin) triple computes the
+ * whole band-overlap matrix of every k as one GEMM in HBM; later calls
+ * read their column from it. */
+int pseudoprojection(pp_cdouble *projections, pswf_t *wf_ref, pswf_t *wf_proj,
+                     int BAND_NUM, int flip_spin);
+
+/* replaces projector.c:850-963 compensation_terms (adds into overlap).
+ * The O_M term is computed for all band pairs at once (FP64); the O_R,
+ * O_S and O_N terms need overlap_setup_real, which this build does not
+ * provide: nonzero num_N_R/num_N_S/num_N_RS returns PP_ERR_UNSUPPORTED. */
+int compensation_terms(pp_cdouble *overlap, int BAND_NUM, pswf_t *wf_S,
+                       pswf_t *wf_R, int num_M, int num_N_R, int num_N_S,
+                       int num_N_RS, int *M_R, int *M_S, int *N_R, int *N_S,
+                       int *N_RS_R, int *N_RS_S, int *proj_labels,
+                       double *proj_coords, int *ref_labels, double *ref_coords,
+                       int *fft_grid, int flip_spin);
+
+/* replaces linalg.c:14-45 fft3d for one band (host buffers). */
+int fft3d(pp_cdouble *x, int *G_bounds, double *lattice, double *kpt, int *Gs,
+          pp_cfloat *Cs, int num_waves, int *fftg);
+
+/* No reference counterpart: copies projections of one (band, k-index)
+ * into out (c128 [total projectors]) from the device-resident table. */
+int pp_get_projections(pswf_t *wf, int band, int kidx, pp_cdouble *out);
+
+/* ===================================================================== */
+/* 2. Batched device-resident entry points                               */
+/* ===================================================================== */
+
+typedef struct pp_ctx pp_ctx;
+typedef struct pp_sphere pp_sphere;
+
+/* Context: one HIP stream (default priority) on `device`, workspaces. */
+int pp_ctx_create(pp_ctx **out, int device);
+void pp_ctx_destroy(pp_ctx *ctx);
+/* Use an existing stream (e.g. the caller's); NULL = the context's own. */
+int pp_ctx_set_stream(pp_ctx *ctx, void *hip_stream);
+int pp_ctx_synchronize(pp_ctx *ctx);
+/* Per-kernel-class timing with HIP events on the context's stream.
+ * pp_ctx_timing_read fills up to n entries of names/ms_total/launches and
+ * returns the number of classes. */
+int pp_ctx_set_timing(pp_ctx *ctx, int enable);
+int pp_ctx_timing_read(pp_ctx *ctx, int n, const char **names,
+                       double *ms_total, long *launches);
+int pp_ctx_timing_reset(pp_ctx *ctx);
+/* Tuning knobs (0 = automatic): bands per FFT chunk, sphere-sum parts,
+ * GEMM split-K factor. */
+int pp_ctx_set_tuning(pp_ctx *ctx, int fft_bands, int sphere_parts,
+                      int gemm_splitk);
+
+/* Sphere plan of one structure (replaces the table build of
+ * projector.c:193-209 / utils.c:590-696, done on the GPU).
+ * Species e: nchan[e] channels with l = ls[...] (concatenated), radius
+ * rmax[e], ngrid[e] points of the linear grid x_j = j*rmax/ngrid with
+ * projector values proj[...] (concatenated [nchan][ngrid] per species).
+ * precision selects the table type (PP_FP64 or PP_FP32/PP_BF16X3). */
+int pp_sphere_create(pp_ctx *ctx, pp_sphere **out, const double *lattice,
+                     const int *fftg, int num_sites, const int *labels,
+                     const double *frac_coords, int num_elems, const int *nchan,
+                     const int *ls, const double *rmax, const int *ngrid,
+                     const double *proj, int precision);
+void pp_sphere_destroy(pp_sphere *sp);
+/* total projectors, total sphere points, and per-site projector offsets
+ * (proj_off[num_sites+1], may be NULL) */
+int pp_sphere_info(pp_sphere *sp, int *total_projs, long *total_points,
+                   int *proj_off);
+/* Rebuild the tables of an existing plan for new coordinates (same sites
+ * and species): the per-structure setup step of setup_projections. */
+int pp_sphere_rebuild(pp_sphere *sp, const double *frac_coords);
+
+/* <p_i|psi_nk> for all bands and k-points of one structure:
+ * kfrac [nk][3] (host); gofs [nk+1] (host) offsets into G_dev (device
+ * int32 [sum W][3]); C_dev device c64 [nk][nband][ldc]; proj_dev device c128
+ * [nk][nband][total_projs]. */
+int pp_setup_projections_batched(pp_ctx *ctx, pp_sphere *sp, int nk,
+                                 const double *kfrac, const int *gofs,
+                                 const int *G_dev, const void *C_dev, long ldc,
+                                 int nband, void *proj_dev, int precision);
+
+/* Band overlaps of every k as one complex GEMM each:
+ * out[k][r][s] = sum_G conj(CR[kR(k)][r][G]) CS[k][s][G], G < nw[k],
+ * kR(k) = k, or the other spin's k when flip && nspin == 2
+ * (k-index = k + spin*nwk).  CR, CS device c64 with row stride ld and
+ * k stride ld*nb; out device c128 [nkt][nbR][nbS]. */
+int pp_overlap_gemm(pp_ctx *ctx, int nwk, int nspin, int flip, const int *nw,
+                    const void *CR_dev, long ldR, int nbR, const void *CS_dev,
+                    long ldS, int nbS, void *out_dev, int precision);
+
+/* O_M augmentation of every band pair (projector.c:890-910), FP64:
+ * out[k][r][s] += sum_{t in M} sum_{i,j: (l,m) equal}
+ *     conj(PR[kR][r][offR(M_R[t])+i]) dO_e[n_i][n_j] PS[k][s][offS(M_S[t])+j].
+ * projR_dev c128 [nkt][nbR][ptotR], projS_dev c128 [nkt][nbS][ptotS];
+ * proj_off_R/S (host) per-site offsets; labels_R/S (host) site species
+ * (dO and n_i from the R site's species, n_j from the S site's);
+ * species tables as in pp_sphere_create plus dO (concatenated nchan^2,
+ * host).  out_dev c128 [nkt][nbR][nbS]. */
+int pp_augmentation(pp_ctx *ctx, int nwk, int nspin, int flip,
+                    const void *projR_dev, int nbR, int ptotR,
+                    const int *proj_off_R, const int *labels_R,
+                    const void *projS_dev, int nbS, int ptotS,
+                    const int *proj_off_S, const int *labels_S, int num_M,
+                    const int *M_R, const int *M_S, int num_elems,
+                    const int *nchan, const int *ls, const double *dO,
+                    void *out_dev);
+
+/* Plane-wave coefficients of nband (<= 64) bands to their real-space
+ * grids with the reference's 1/sqrt(V) scale (FP64 or FP32 FFT), the
+ * device form of fft3d: G_dev int32 [nw][3], C_dev c64 [nband][ldc],
+ * grid_out_dev c128 [nband][N0][N1][N2]. */
+int pp_fft3d_batched(pp_ctx *ctx, const double *lattice, const int *fftg,
+                     int nw, const int *G_dev, const void *C_dev, long ldc,
+                     int nband, void *grid_out_dev, int precision);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -289,136 +289,169 @@ int or_site_table(const double *lattice, const int *fftg, const double *coord,
 /* FFT                                                                  */
 /* ------------------------------------------------------------------ */
 
-/* One 1D plan: factors of n (4,2,3,5,7, then any prime) and the table
- * w[j] = exp(+2 pi i j / n). */
+/* One 1D plan: radices (4 first, then primes) and per-stage twiddles
+ * w^(k r) = exp(+2 pi i k r / (p R)) for k < p, r < R, plus the R-th roots. */
 typedef struct {
   int n, nf, f[32];
-  double *w;
+  double *tw[32];   /* [p][R] complex per stage */
+  double *root[32]; /* [R] complex per stage */
 } fft_plan;
 
-static void plan_make(fft_plan *p, int n) {
-  p->n = n;
-  p->nf = 0;
+static void plan_make(fft_plan *pl, int n) {
+  pl->n = n;
+  pl->nf = 0;
   int m = n;
   while (m % 4 == 0) {
-    p->f[p->nf++] = 4;
+    pl->f[pl->nf++] = 4;
     m /= 4;
   }
   for (int q = 2; m > 1 && q <= m; q++)
     while (m % q == 0) {
-      p->f[p->nf++] = q;
+      pl->f[pl->nf++] = q;
       m /= q;
     }
-  p->w = (double *)malloc(sizeof(double) * 2 * n);
-  for (int j = 0; j < n; j++) {
-    p->w[2 * j] = cos(2.0 * OR_PI * j / n);
-    p->w[2 * j + 1] = sin(2.0 * OR_PI * j / n);
-  }
-}
-
-static void plan_free(fft_plan *p) { free(p->w); }
-
-/* Recursive decimation in time: out[k] = sum_j in[j*stride] e^{+2 pi i jk/n}.
- * level selects the radix; tmp has room for n complex values. */
-static void fft_rec(const fft_plan *p, int level, int n, const double *in,
-                    long stride, double *out, double *tmp) {
-  if (n == 1) {
-    out[0] = in[0];
-    out[1] = in[1];
-    return;
-  }
-  int r = p->f[level];
-  int m = n / r;
-  /* sub-transforms of the r decimated sequences into out[q*m ...] */
-  for (int q = 0; q < r; q++)
-    fft_rec(p, level + 1, m, in + 2 * q * stride, stride * r, out + 2 * q * m,
-            tmp);
-  /* combine: X[k + s m] = sum_q W_n^{q k} W_r^{q s} Y_q[k] */
-  long wstep = p->n / n; /* table stride for W_n */
-  for (int k = 0; k < m; k++) {
-    double y[2 * 64];
-    for (int q = 0; q < r; q++) {
-      long e = (wstep * (long)q * k) % p->n;
-      double wr = p->w[2 * e], wi = p->w[2 * e + 1];
-      double ar = out[2 * (q * m + k)], ai = out[2 * (q * m + k) + 1];
-      y[2 * q] = ar * wr - ai * wi;
-      y[2 * q + 1] = ar * wi + ai * wr;
-    }
-    for (int s = 0; s < r; s++) {
-      double sr = 0.0, si = 0.0;
-      for (int q = 0; q < r; q++) {
-        long e = (wstep * (long)m * ((long)q * s % r)) % p->n;
-        double wr = p->w[2 * e], wi = p->w[2 * e + 1];
-        sr += y[2 * q] * wr - y[2 * q + 1] * wi;
-        si += y[2 * q] * wi + y[2 * q + 1] * wr;
+  int p = 1;
+  for (int s = 0; s < pl->nf; s++) {
+    int R = pl->f[s];
+    pl->tw[s] = (double *)malloc(sizeof(double) * 2 * p * R);
+    pl->root[s] = (double *)malloc(sizeof(double) * 2 * R);
+    for (int k = 0; k < p; k++)
+      for (int r = 0; r < R; r++) {
+        double ang = 2.0 * OR_PI * (double)(k * r) / (double)(p * R);
+        pl->tw[s][2 * (k * R + r)] = cos(ang);
+        pl->tw[s][2 * (k * R + r) + 1] = sin(ang);
       }
-      tmp[2 * (k + s * m)] = sr;
-      tmp[2 * (k + s * m) + 1] = si;
+    for (int r = 0; r < R; r++) {
+      pl->root[s][2 * r] = cos(2.0 * OR_PI * r / R);
+      pl->root[s][2 * r + 1] = sin(2.0 * OR_PI * r / R);
     }
+    p *= R;
   }
-  memcpy(out, tmp, sizeof(double) * 2 * n);
 }
 
-static void fft_line(const fft_plan *p, double *line, double *work) {
-  /* work: 3n complex (input copy, output, combine scratch) */
-  int n = p->n;
-  double *in = work, *out = work + 2 * n, *tmp = work + 4 * n;
-  memcpy(in, line, sizeof(double) * 2 * n);
-  fft_rec(p, 0, n, in, 1, out, tmp);
-  memcpy(line, out, sizeof(double) * 2 * n);
+static void plan_free(fft_plan *pl) {
+  for (int s = 0; s < pl->nf; s++) {
+    free(pl->tw[s]);
+    free(pl->root[s]);
+  }
 }
 
-static void ifft3d_with(double *x, const int *fftg, fft_plan *pl,
-                        double *work) {
+/* Backward transform of nb sequences stored [e][b] (b fastest, length n
+ * each) in a, using b as scratch of the same size; result left in a.
+ * Stockham autosort: stage (R, p): for butterfly j = (q, k), k = j mod p,
+ * inputs a[j + r n/R] * w^(k r), outputs b[(j/p) p R + k + r p]. */
+static void fft_batch(const fft_plan *pl, double *a, double *b, int nb) {
+  int n = pl->n, p = 1;
+  double *in = a, *out = b;
+  double v[2 * 64];
+  for (int s = 0; s < pl->nf; s++) {
+    int R = pl->f[s], T = n / R;
+    const double *tw = pl->tw[s], *rt = pl->root[s];
+    for (int j = 0; j < T; j++) {
+      int k = j % p;
+      int o = (j / p) * p * R + k;
+      for (int c = 0; c < nb; c++) {
+        for (int r = 0; r < R; r++) {
+          const double *x = in + 2 * ((long)(j + r * T) * nb + c);
+          double wr = tw[2 * (k * R + r)], wi = tw[2 * (k * R + r) + 1];
+          v[2 * r] = x[0] * wr - x[1] * wi;
+          v[2 * r + 1] = x[0] * wi + x[1] * wr;
+        }
+        if (R == 4) {
+          double a0r = v[0] + v[4], a0i = v[1] + v[5], a1r = v[0] - v[4], a1i = v[1] - v[5];
+          double b0r = v[2] + v[6], b0i = v[3] + v[7], b1r = -(v[3] - v[7]), b1i = v[2] - v[6];
+          double *y0 = out + 2 * ((long)(o + 0 * p) * nb + c), *y1 = out + 2 * ((long)(o + 1 * p) * nb + c);
+          double *y2 = out + 2 * ((long)(o + 2 * p) * nb + c), *y3 = out + 2 * ((long)(o + 3 * p) * nb + c);
+          y0[0] = a0r + b0r;
+          y0[1] = a0i + b0i;
+          y2[0] = a0r - b0r;
+          y2[1] = a0i - b0i;
+          y1[0] = a1r + b1r;
+          y1[1] = a1i + b1i;
+          y3[0] = a1r - b1r;
+          y3[1] = a1i - b1i;
+        } else if (R == 2) {
+          double *y0 = out + 2 * ((long)o * nb + c), *y1 = out + 2 * ((long)(o + p) * nb + c);
+          y0[0] = v[0] + v[2];
+          y0[1] = v[1] + v[3];
+          y1[0] = v[0] - v[2];
+          y1[1] = v[1] - v[3];
+        } else {
+          for (int q = 0; q < R; q++) {
+            double sr = 0.0, si = 0.0;
+            for (int r = 0; r < R; r++) {
+              int e = (r * q) % R;
+              sr += v[2 * r] * rt[2 * e] - v[2 * r + 1] * rt[2 * e + 1];
+              si += v[2 * r] * rt[2 * e + 1] + v[2 * r + 1] * rt[2 * e];
+            }
+            double *y = out + 2 * ((long)(o + q * p) * nb + c);
+            y[0] = sr;
+            y[1] = si;
+          }
+        }
+      }
+    }
+    double *t = in;
+    in = out;
+    out = t;
+    p *= R;
+  }
+  if (in != a)
+    memcpy(a, in, sizeof(double) * 2 * (size_t)n * nb);
+}
+
+/* work: 4 * nmax * nmax doubles (two [n][nb] complex buffers) */
+static void ifft3d_with(double *x, const int *fftg, fft_plan *pl, double *work) {
   int n0 = fftg[0], n1 = fftg[1], n2 = fftg[2];
   int nmax = n0 > n1 ? n0 : n1;
   nmax = nmax > n2 ? nmax : n2;
-  double *line = work + 6 * nmax;
-  /* z lines (contiguous) */
-  for (long a = 0; a < (long)n0 * n1; a++)
-    fft_line(&pl[2], x + 2 * a * n2, work);
-  /* y lines */
-  for (int i = 0; i < n0; i++)
-    for (int k = 0; k < n2; k++) {
-      for (int j = 0; j < n1; j++) {
-        long o = 2 * (((long)i * n1 + j) * n2 + k);
-        line[2 * j] = x[o];
-        line[2 * j + 1] = x[o + 1];
+  double *A = work, *B = work + 2 * (size_t)nmax * nmax;
+  /* z transforms: for each x, sequences along z, batch over y (transpose in) */
+  for (int i = 0; i < n0; i++) {
+    double *pl0 = x + 2 * (size_t)i * n1 * n2;
+    for (int j = 0; j < n1; j++)
+      for (int k = 0; k < n2; k++) {
+        A[2 * ((size_t)k * n1 + j)] = pl0[2 * ((size_t)j * n2 + k)];
+        A[2 * ((size_t)k * n1 + j) + 1] = pl0[2 * ((size_t)j * n2 + k) + 1];
       }
-      fft_line(&pl[1], line, work);
-      for (int j = 0; j < n1; j++) {
-        long o = 2 * (((long)i * n1 + j) * n2 + k);
-        x[o] = line[2 * j];
-        x[o + 1] = line[2 * j + 1];
+    fft_batch(&pl[2], A, B, n1);
+    for (int j = 0; j < n1; j++)
+      for (int k = 0; k < n2; k++) {
+        pl0[2 * ((size_t)j * n2 + k)] = A[2 * ((size_t)k * n1 + j)];
+        pl0[2 * ((size_t)j * n2 + k) + 1] = A[2 * ((size_t)k * n1 + j) + 1];
       }
-    }
-  /* x lines */
-  for (int j = 0; j < n1; j++)
-    for (int k = 0; k < n2; k++) {
-      for (int i = 0; i < n0; i++) {
-        long o = 2 * (((long)i * n1 + j) * n2 + k);
-        line[2 * i] = x[o];
-        line[2 * i + 1] = x[o + 1];
-      }
-      fft_line(&pl[0], line, work);
-      for (int i = 0; i < n0; i++) {
-        long o = 2 * (((long)i * n1 + j) * n2 + k);
-        x[o] = line[2 * i];
-        x[o + 1] = line[2 * i + 1];
-      }
-    }
+  }
+  /* y transforms: for each x the plane is already [j][k] = [e][b] */
+  for (int i = 0; i < n0; i++) {
+    double *pl0 = x + 2 * (size_t)i * n1 * n2;
+    memcpy(A, pl0, sizeof(double) * 2 * (size_t)n1 * n2);
+    fft_batch(&pl[1], A, B, n2);
+    memcpy(pl0, A, sizeof(double) * 2 * (size_t)n1 * n2);
+  }
+  /* x transforms: for each y, [i][k] with stride n1*n2 between i */
+  for (int j = 0; j < n1; j++) {
+    for (int i = 0; i < n0; i++)
+      memcpy(A + 2 * (size_t)i * n2, x + 2 * (((size_t)i * n1 + j) * n2), sizeof(double) * 2 * n2);
+    fft_batch(&pl[0], A, B, n2);
+    for (int i = 0; i < n0; i++)
+      memcpy(x + 2 * (((size_t)i * n1 + j) * n2), A + 2 * (size_t)i * n2, sizeof(double) * 2 * n2);
+  }
+}
+
+static double *fft_work(const int *fftg) {
+  int nmax = fftg[0];
+  if (fftg[1] > nmax)
+    nmax = fftg[1];
+  if (fftg[2] > nmax)
+    nmax = fftg[2];
+  return (double *)malloc(sizeof(double) * 4 * (size_t)nmax * nmax);
 }
 
 void or_ifft3d_inplace(double *x, const int *fftg) {
   fft_plan pl[3];
   for (int a = 0; a < 3; a++)
     plan_make(&pl[a], fftg[a]);
-  int nmax = fftg[0];
-  if (fftg[1] > nmax)
-    nmax = fftg[1];
-  if (fftg[2] > nmax)
-    nmax = fftg[2];
-  double *work = (double *)malloc(sizeof(double) * 2 * 4 * nmax);
+  double *work = fft_work(fftg);
   ifft3d_with(x, fftg, pl, work);
   free(work);
   for (int a = 0; a < 3; a++)
@@ -451,12 +484,7 @@ void or_fft3d(double *x, const double *lattice, const int *Gs,
   fft_plan pl[3];
   for (int a = 0; a < 3; a++)
     plan_make(&pl[a], fftg[a]);
-  int nmax = fftg[0];
-  if (fftg[1] > nmax)
-    nmax = fftg[1];
-  if (fftg[2] > nmax)
-    nmax = fftg[2];
-  double *work = (double *)malloc(sizeof(double) * 2 * 4 * nmax);
+  double *work = fft_work(fftg);
   fft3d_with(x, lattice, Gs, Cs_c64, nw, fftg, pl, work);
   free(work);
   for (int a = 0; a < 3; a++)
@@ -476,8 +504,8 @@ static void reclat(const double *lat, double *rec) {
     rec[i] *= 2.0 * OR_PI / v;
 }
 
-/* projector.c:223-274: xvals = x[idx] dv e^{i k.path};
- * <p_j|psi> = sum conj(values_j) xvals (zdotc). */
+/* projector.c:223-274: xvals = x[idx] dv e^{i k.path} once per point,
+ * then <p_j|psi> = sum conj(values_j) xvals (zdotc) for each projector. */
 void or_onto_projector(const double *grid, const double *lattice,
                        const double *kfrac, const int *fftg, int nsites,
                        const int *site_pt, const int *site_nproj,
@@ -488,26 +516,35 @@ void or_onto_projector(const double *grid, const double *lattice,
   double rec[9], kc[3];
   reclat(lattice, rec);
   frac_to_cart(kfrac, rec, kc);
+  int maxn = 0;
+  for (int s = 0; s < nsites; s++)
+    if (site_pt[s + 1] - site_pt[s] > maxn)
+      maxn = site_pt[s + 1] - site_pt[s];
+  double *xv = (double *)malloc(sizeof(double) * 2 * (size_t)(maxn > 0 ? maxn : 1));
   for (int s = 0; s < nsites; s++) {
     int p0 = site_pt[s], np = site_pt[s + 1] - site_pt[s];
+    for (int i = 0; i < np; i++) {
+      const double *pa = path + 3 * (long)(p0 + i);
+      double kr = kc[0] * pa[0] + kc[1] * pa[1] + kc[2] * pa[2];
+      double cr = cos(kr), ci = sin(kr);
+      long g = idx[p0 + i];
+      double xr = grid[2 * g] * dv, xi = grid[2 * g + 1] * dv;
+      xv[2 * i] = xr * cr - xi * ci;
+      xv[2 * i + 1] = xr * ci + xi * cr;
+    }
     for (int p = 0; p < site_nproj[s]; p++) {
       double sr = 0.0, si = 0.0;
       const double *val = values + 2 * (val_off[s] + (long)p * np);
       for (int i = 0; i < np; i++) {
-        const double *pa = path + 3 * (long)(p0 + i);
-        double kr = kc[0] * pa[0] + kc[1] * pa[1] + kc[2] * pa[2];
-        double cr = cos(kr), ci = sin(kr);
-        long g = idx[p0 + i];
-        double xr = grid[2 * g] * dv, xi = grid[2 * g + 1] * dv;
-        double yr = xr * cr - xi * ci, yi = xr * ci + xi * cr;
         double vr = val[2 * i], vi = val[2 * i + 1];
-        sr += vr * yr + vi * yi;
-        si += vr * yi - vi * yr;
+        sr += vr * xv[2 * i] + vi * xv[2 * i + 1];
+        si += vr * xv[2 * i + 1] - vi * xv[2 * i];
       }
       proj_out[2 * (proj_off[s] + p)] = sr;
       proj_out[2 * (proj_off[s] + p) + 1] = si;
     }
   }
+  free(xv);
 }
 
 void or_setup_projections(const double *lattice, const int *fftg, int nk,
@@ -521,11 +558,6 @@ void or_setup_projections(const double *lattice, const int *fftg, int nk,
                           int nthreads) {
   long ng = (long)fftg[0] * fftg[1] * fftg[2];
   int total = pairs ? npairs : nk * nband;
-  int nmax = fftg[0];
-  if (fftg[1] > nmax)
-    nmax = fftg[1];
-  if (fftg[2] > nmax)
-    nmax = fftg[2];
 #ifdef _OPENMP
   if (nthreads > 0)
     omp_set_num_threads(nthreads);
@@ -535,7 +567,7 @@ void or_setup_projections(const double *lattice, const int *fftg, int nk,
     fft_plan pl[3];
     for (int a = 0; a < 3; a++)
       plan_make(&pl[a], fftg[a]);
-    double *work = (double *)malloc(sizeof(double) * 2 * 4 * nmax);
+    double *work = fft_work(fftg);
     double *x = (double *)malloc(sizeof(double) * 2 * ng);
 #pragma omp for schedule(dynamic, 1)
     for (int w = 0; w < total; w++) {

@@ -1,0 +1,193 @@
+"""Python wrapper of the batched device-resident entry points.
+
+Device memory comes from torch tensors (plumbing only: every computation
+runs in the HIP library); pointers and sizes go through the C-ABI.
+"""
+
+from __future__ import annotations
+
+import ctypes as ct
+
+import numpy as np
+
+from ._lib import (PRECISIONS, as_f64, as_i32, c_double_p, c_int_p, check, lib, ptr)
+
+
+def _prec(p):
+    return PRECISIONS[p] if isinstance(p, str) else int(p)
+
+
+def species_arrays(species_list):
+    """Flatten species (objects with ls, realprojs, rmax) into the
+    pp_sphere_create arrays."""
+    nchan = as_i32([len(sp.ls) for sp in species_list])
+    ls = as_i32([l for sp in species_list for l in sp.ls])
+    rmax = as_f64([sp.rmax for sp in species_list])
+    ngrid = as_i32([len(sp.realprojs[0]) for sp in species_list])
+    proj = as_f64(np.concatenate([np.asarray(p, np.float64) for sp in species_list for p in sp.realprojs]))
+    return nchan, ls, rmax, ngrid, proj
+
+
+class Context:
+    """A pp_ctx: one HIP stream on one device plus workspaces."""
+
+    def __init__(self, device: int = 0, stream=None):
+        """stream: optional torch.cuda.Stream to launch on (its handle must
+        be non-null); by default the context owns a stream and the wrapper
+        functions below synchronize it before returning (sync=True)."""
+        h = ct.c_void_p()
+        check(lib().pp_ctx_create(ct.byref(h), int(device)), "pp_ctx_create")
+        self.h = h
+        self.device = device
+        self.stream = None
+        if stream is not None:
+            if not stream.cuda_stream:
+                raise ValueError("pass a non-default torch.cuda.Stream")
+            self.set_stream(stream.cuda_stream)
+            self.stream = stream
+
+    def set_stream(self, stream_handle):
+        """Launch on an external HIP stream (e.g. torch.cuda.current_stream().cuda_stream)."""
+        check(lib().pp_ctx_set_stream(self.h, ct.c_void_p(stream_handle) if stream_handle else None),
+              "pp_ctx_set_stream")
+
+    def synchronize(self):
+        check(lib().pp_ctx_synchronize(self.h), "pp_ctx_synchronize")
+
+    def set_timing(self, on: bool):
+        check(lib().pp_ctx_set_timing(self.h, int(bool(on))), "pp_ctx_set_timing")
+
+    def timing_reset(self):
+        check(lib().pp_ctx_timing_reset(self.h), "pp_ctx_timing_reset")
+
+    def timing(self):
+        n = 32
+        names = (ct.c_char_p * n)()
+        ms = np.zeros(n)
+        cnt = np.zeros(n, np.int64)
+        k = lib().pp_ctx_timing_read(self.h, n, names, ptr(ms, c_double_p), cnt.ctypes.data_as(ct.POINTER(ct.c_long)))
+        return {names[i].decode(): (float(ms[i]), int(cnt[i])) for i in range(k) if cnt[i] > 0}
+
+    def set_tuning(self, fft_bands=0, sphere_parts=0, gemm_splitk=0):
+        check(lib().pp_ctx_set_tuning(self.h, fft_bands, sphere_parts, gemm_splitk), "pp_ctx_set_tuning")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pp_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class SpherePlan:
+    """pp_sphere: sphere tables of one structure on the device."""
+
+    def __init__(self, ctx: Context, lattice, fftg, frac_coords, labels, species_list, precision="fp64"):
+        self.ctx = ctx
+        self._keep = []
+        lat = as_f64(lattice).ravel()
+        fg = as_i32(fftg)
+        fr = as_f64(frac_coords).reshape(-1, 3)
+        lab = as_i32(labels)
+        nchan, ls, rmax, ngrid, proj = species_arrays(species_list)
+        h = ct.c_void_p()
+        check(lib().pp_sphere_create(ctx.h, ct.byref(h), ptr(lat, c_double_p), ptr(fg, c_int_p), len(fr),
+                                     ptr(lab, c_int_p), ptr(fr, c_double_p), len(species_list),
+                                     ptr(nchan, c_int_p), ptr(ls, c_int_p), ptr(rmax, c_double_p),
+                                     ptr(ngrid, c_int_p), ptr(proj, c_double_p), _prec(precision)),
+              "pp_sphere_create")
+        self.h = h
+        self.nsites = len(fr)
+        self.precision = _prec(precision)
+        self._info()
+
+    def _info(self):
+        tp = ct.c_int()
+        npt = ct.c_long()
+        off = np.zeros(self.nsites + 1, np.int32)
+        check(lib().pp_sphere_info(self.h, ct.byref(tp), ct.byref(npt), ptr(off, c_int_p)), "pp_sphere_info")
+        self.ptot = tp.value
+        self.npts = npt.value
+        self.proj_off = off
+
+    def rebuild(self, frac_coords):
+        fr = as_f64(frac_coords).reshape(-1, 3)
+        check(lib().pp_sphere_rebuild(self.h, ptr(fr, c_double_p)), "pp_sphere_rebuild")
+        self._info()
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pp_sphere_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _dptr(t):
+    return ct.c_void_p(t.data_ptr())
+
+
+def _done(ctx, sync):
+    if sync:
+        ctx.synchronize()
+
+
+def setup_projections(ctx: Context, plan: SpherePlan, kfrac, gofs, G_dev, C_dev, proj_out, precision="fp64", sync=True):
+    """<p|psi> of every (k, band): C_dev complex64 [nk, nband, ldc] (cuda),
+    G_dev int32 [sum W, 3] (cuda), proj_out complex128 [nk, nband, ptot]."""
+    nk, nband, ldc = C_dev.shape
+    kf = as_f64(kfrac).reshape(-1, 3)
+    go = as_i32(gofs)
+    assert proj_out.shape == (nk, nband, plan.ptot)
+    check(lib().pp_setup_projections_batched(ctx.h, plan.h, nk, ptr(kf, c_double_p), ptr(go, c_int_p),
+                                             _dptr(G_dev), _dptr(C_dev), ldc, nband, _dptr(proj_out),
+                                             _prec(precision)), "pp_setup_projections_batched")
+    _done(ctx, sync)
+
+
+def overlap_gemm(ctx: Context, nwk, nspin, flip, nw, CR, CS, out, precision="fp64", sync=True):
+    """out[k, r, s] = <CR[kR(k), r] | CS[k, s]> over the first nw[k] plane waves."""
+    nkt, nbR, ldR = CR.shape
+    _, nbS, ldS = CS.shape
+    nwa = as_i32(nw)
+    assert out.shape == (nkt, nbR, nbS)
+    check(lib().pp_overlap_gemm(ctx.h, nwk, nspin, int(bool(flip)), ptr(nwa, c_int_p), _dptr(CR), ldR, nbR,
+                                _dptr(CS), ldS, nbS, _dptr(out), _prec(precision)), "pp_overlap_gemm")
+    _done(ctx, sync)
+
+
+def augmentation(ctx: Context, nwk, nspin, flip, projR, proj_off_R, labels_R, projS, proj_off_S, labels_S,
+                 M_R, M_S, species_list, dO_list, out, sync=True):
+    """out[k, r, s] += O_M of projector.c:890-910 (FP64)."""
+    nkt, nbR, ptotR = projR.shape
+    _, nbS, ptotS = projS.shape
+    offR, offS = as_i32(proj_off_R), as_i32(proj_off_S)
+    labR, labS = as_i32(labels_R), as_i32(labels_S)
+    mr, ms = as_i32(M_R), as_i32(M_S)
+    nchan = as_i32([len(sp.ls) for sp in species_list])
+    ls = as_i32([l for sp in species_list for l in sp.ls])
+    dO = as_f64(np.concatenate([np.asarray(d, np.float64).ravel() for d in dO_list]))
+    check(lib().pp_augmentation(ctx.h, nwk, nspin, int(bool(flip)), _dptr(projR), nbR, ptotR, ptr(offR, c_int_p),
+                                ptr(labR, c_int_p), _dptr(projS), nbS, ptotS, ptr(offS, c_int_p),
+                                ptr(labS, c_int_p), len(mr), ptr(mr, c_int_p), ptr(ms, c_int_p), len(species_list),
+                                ptr(nchan, c_int_p), ptr(ls, c_int_p), ptr(dO, c_double_p), _dptr(out)),
+          "pp_augmentation")
+    _done(ctx, sync)
+
+
+def fft3d(ctx: Context, lattice, fftg, G_dev, C_dev, out, precision="fp64", sync=True):
+    """Real-space grids (c128 [nband, N0, N1, N2]) of up to 64 bands."""
+    nband, ldc = C_dev.shape
+    lat = as_f64(lattice).ravel()
+    fg = as_i32(fftg)
+    check(lib().pp_fft3d_batched(ctx.h, ptr(lat, c_double_p), ptr(fg, c_int_p), G_dev.shape[0], _dptr(G_dev),
+                                 _dptr(C_dev), ldc, nband, _dptr(out), _prec(precision)), "pp_fft3d_batched")
+    _done(ctx, sync)

@@ -1,0 +1,165 @@
+"""Projector: band projections of one structure onto another's bands.
+
+Mirrors pawpyseed/core/projector.py (Projector, make_site_lists,
+single_band_projection, proportion_conduction, defect_band_analysis) with
+the 'aug_real' and 'pseudo' methods.  The pseudo-wavefunction overlap and
+the O_M augmentation run in libpawpy_amd.so; the O_R/O_S/O_N terms for
+sites that differ between the structures need overlap_setup_real, which
+this build does not provide, so a Projector whose site lists leave sites
+unmatched raises PAWpyError unless explicit ``site_lists`` are given.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from ._lib import PAWpyError
+
+
+class Projector:
+    METHODS = ["pseudo", "aug_real"]
+
+    def __init__(self, wf, basis, unsym_basis=False, unsym_wf=False, method="aug_real", site_lists=None):
+        self.method = method
+        if method == "pseudo":
+            self._single_band_projection = self._single_band_projection_pseudo
+        elif method == "aug_real":
+            self._single_band_projection = self._single_band_projection_aug_real
+        elif method in ("aug_recip", "realspace"):
+            raise PAWpyError(f"method {method} is not provided by this build")
+        else:
+            raise PAWpyError("method not recognized for Projector")
+        if wf.ncl or basis.ncl:
+            raise PAWpyError("Projection not supported for noncollinear case!")
+        if unsym_basis or unsym_wf:
+            raise PAWpyError("k-point desymmetrisation is not provided by this build")
+        # projector.py:97-104
+        if np.linalg.norm(basis.kpts - wf.kpts) > 1e-10:
+            raise PAWpyError("k-point grids for projection are not matched.")
+        if np.linalg.norm(basis.kws - wf.kws) > 1e-10:
+            raise PAWpyError("k-point weights for projection are not matched.")
+        if not (wf.structure.lattice == basis.structure.lattice):
+            raise PAWpyError("Need the lattice to be the same for projections, and they are not")
+        self.wf = wf
+        self.basis = basis
+        self._site_lists = site_lists
+        if method != "pseudo":
+            basis.check_c_projectors()
+            wf.check_c_projectors()
+            self.setup_overlap()
+
+    def make_site_lists(self):
+        """projector.py:115-160: M_R/M_S identical sites (same element,
+        within 0.02 A), N_R/N_S the rest, N_RS pairs of unmatched sites whose
+        augmentation spheres overlap."""
+        ref, cur = self.basis.structure, self.wf.structure
+        d = ref.distance_matrix_to(cur)
+        same_el = np.array(ref.species)[:, None] == np.array(cur.species)[None, :]
+        mr, ms = np.nonzero((d <= 0.02) & same_el)
+        M_R, M_S = mr.tolist(), ms.tolist()
+        N_R = [i for i in range(len(ref)) if i not in set(M_R)]
+        N_S = [j for j in range(len(cur)) if j not in set(M_S)]
+        N_RS = []
+        for i in N_R:
+            for j in N_S:
+                rmax1 = self.basis.cr.pps[ref.species[i]].rmax
+                rmax2 = self.wf.cr.pps[cur.species[j]].rmax
+                if d[i, j] < rmax1 + rmax2:
+                    N_RS.append((i, j))
+        return M_R, M_S, N_R, N_S, N_RS
+
+    def setup_overlap(self):
+        if self._site_lists is not None:
+            M_R, M_S = self._site_lists
+            N_R, N_S, N_RS = [], [], []
+        else:
+            M_R, M_S, N_R, N_S, N_RS = self.make_site_lists()
+        if N_R or N_S or N_RS:
+            raise PAWpyError("the structures differ at %d/%d sites; the O_R/O_S/O_N terms need "
+                             "overlap_setup_real, which this build does not provide (pass site_lists "
+                             "to use the on-site O_M term only)" % (len(N_R), len(N_S)))
+        self.M_R = np.asarray(M_R, np.int32)
+        self.M_S = np.asarray(M_S, np.int32)
+        self.site_cat = [self.M_R, self.M_S, [], [], [], []]
+
+    def _single_band_projection_pseudo(self, band_num, flip_spin=False):
+        return self.wf.pseudoprojection(band_num, self.basis, flip_spin)
+
+    def _single_band_projection_aug_real(self, band_num, flip_spin=False):
+        res = self.wf.pseudoprojection(band_num, self.basis, flip_spin)
+        self._add_augmentation_terms(res, band_num, flip_spin)
+        return res
+
+    def _add_augmentation_terms(self, res, band_num, flip_spin):
+        """pawpyc.pyx:685-702 -> compensation_terms."""
+        import ctypes as ct
+
+        from ._lib import as_i32, c_double_p, c_int_p, check, lib, ptr
+        mr, ms = as_i32(self.M_R), as_i32(self.M_S)
+        check(lib().compensation_terms(res.ctypes.data_as(ct.c_void_p), int(band_num), self.wf.wf_ptr,
+                                       self.basis.wf_ptr, len(mr), 0, 0, 0, ptr(mr, c_int_p), ptr(ms, c_int_p),
+                                       None, None, None, None, ptr(self.wf.nums, c_int_p),
+                                       ptr(self.wf.coords, c_double_p), ptr(self.basis.nums, c_int_p),
+                                       ptr(self.basis.coords, c_double_p), ptr(self.wf.dimv, c_int_p),
+                                       int(bool(flip_spin))), "compensation_terms")
+
+    def single_band_projection(self, band_num, **kwargs):
+        """projector.py:238-272: overlaps <basis; b,k,s | wf; band_num,k,s>
+        ordered res[b*nwk*nspin + s*nwk + k]."""
+        if band_num >= self.wf.nband or band_num < 0:
+            raise ValueError("Band index out of range (0-indexed)")
+        return self._single_band_projection(band_num, **kwargs)
+
+    def proportion_conduction(self, band_num, spinpol=False):
+        """projector.py:384-435."""
+        basis = self.basis
+        nband, nwk, nspin = basis.nband, basis.nwk, basis.nspin
+        occs = basis._get_occs()
+        res = self.single_band_projection(band_num)
+        if spinpol:
+            c, v = np.zeros(nspin), np.zeros(nspin)
+            for b in range(nband):
+                for s in range(nspin):
+                    ind = b * nspin + s
+                    prop = np.absolute(res[ind * nwk:(ind + 1) * nwk]) ** 2
+                    c[s] += np.dot(prop, (1 - occs[ind * nwk:(ind + 1) * nwk]) * self.wf.kws)
+                    v[s] += np.dot(prop, occs[ind * nwk:(ind + 1) * nwk] * self.wf.kws)
+        else:
+            c, v = 0, 0
+            for i in range(nband * nwk * nspin):
+                if occs[i] > 0.5:
+                    v += np.absolute(res[i]) ** 2 * self.wf.kws[i % nwk] / nspin
+                else:
+                    c += np.absolute(res[i]) ** 2 * self.wf.kws[i % nwk] / nspin
+        if self.method == "pseudo":
+            t = v + c
+            v /= t
+            c /= t
+        if spinpol:
+            v, c = v.tolist(), c.tolist()
+        return v, c
+
+    def defect_band_analysis(self, num_below_ef=20, num_above_ef=20, spinpol=False, return_energies=False,
+                             vbmband=None, band_list=None, analyze_all=False):
+        """projector.py:437-503."""
+        if num_below_ef < 0 or num_above_ef < 0:
+            raise ValueError("num_above_ef and num_below_ef must both be nonnegative.")
+        nband = self.basis.nband
+        occs = self.wf._get_occs()
+        if analyze_all:
+            totest = list(range(nband))
+        elif band_list:
+            totest = band_list[:]
+        else:
+            vbm = 0
+            for i in range(self.wf.nband):
+                if occs[i * self.wf.nwk * self.wf.nspin] > 0.5:
+                    vbm = i
+            if vbmband is not None:
+                vbm = vbmband
+            lo, hi = max(vbm - num_below_ef, 0), min(vbm + num_above_ef, self.wf.nband - 1)
+            totest = list(range(lo, hi + 1))
+        results = {b: self.proportion_conduction(b, spinpol=spinpol) for b in totest}
+        if return_energies:
+            return results, self.wf._get_energy_list(totest)
+        return results

@@ -1,0 +1,278 @@
+// Context, errors, timing and the batched device-resident entry points.
+#include <cstring>
+#include <mutex>
+
+#include "pp_internal.h"
+
+struct pp_sphere;
+
+namespace pp {
+
+static thread_local std::string g_err;
+static thread_local int g_code = PP_OK;
+
+void set_error(int code, const std::string &msg) {
+  g_code = code;
+  g_err = msg;
+}
+int last_code() { return g_code; }
+
+const char *kclass_name(int c) {
+  static const char *names[K_NCLASS] = {"zero",          "scatter",    "fft_z",       "fft_y",    "fft_x",
+                                        "phase",         "sphere_sum", "sphere_reduce", "table",  "gemm",
+                                        "gemm_reduce",   "aug_prep",   "aug_gemm", "misc"};
+  return (c >= 0 && c < K_NCLASS) ? names[c] : "?";
+}
+
+int DevBuf::ensure(size_t need) {
+  if (need <= bytes && p)
+    return PP_OK;
+  release();
+  size_t alloc = need < 256 ? 256 : need;
+  hipError_t e = hipMalloc(&p, alloc);
+  if (e != hipSuccess) {
+    p = nullptr;
+    bytes = 0;
+    set_error(PP_ERR_NOMEM, std::string("hipMalloc(") + std::to_string(alloc) + "): " + hipGetErrorString(e));
+    return PP_ERR_NOMEM;
+  }
+  bytes = alloc;
+  return PP_OK;
+}
+
+void DevBuf::release() {
+  if (p)
+    (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
+void tbegin(pp_ctx *ctx, int cls, hipEvent_t *start) {
+  *start = nullptr;
+  if (!ctx->timer.on)
+    return;
+  hipEvent_t e;
+  if (!ctx->timer.pool.empty()) {
+    e = ctx->timer.pool.back();
+    ctx->timer.pool.pop_back();
+  } else if (hipEventCreate(&e) != hipSuccess) {
+    return;
+  }
+  (void)hipEventRecord(e, ctx->stream);
+  *start = e;
+  (void)cls;
+}
+
+void tend(pp_ctx *ctx, int cls, hipEvent_t start) {
+  if (!ctx->timer.on || !start)
+    return;
+  hipEvent_t e;
+  if (!ctx->timer.pool.empty()) {
+    e = ctx->timer.pool.back();
+    ctx->timer.pool.pop_back();
+  } else if (hipEventCreate(&e) != hipSuccess) {
+    return;
+  }
+  (void)hipEventRecord(e, ctx->stream);
+  ctx->timer.pending.push_back({cls, {start, e}});
+}
+
+static void timer_collect(pp_ctx *ctx) {
+  if (ctx->timer.pending.empty())
+    return;
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto &pe : ctx->timer.pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, pe.second.first, pe.second.second) == hipSuccess) {
+      ctx->timer.ms[pe.first] += ms;
+      ctx->timer.launches[pe.first] += 1;
+    }
+    ctx->timer.pool.push_back(pe.second.first);
+    ctx->timer.pool.push_back(pe.second.second);
+  }
+  ctx->timer.pending.clear();
+}
+
+// sphere helpers implemented in pp_sphere.hip
+template <typename T>
+int sphere_sums(pp_ctx *ctx, pp_sphere *sp, const cplx<T> *grid, int NB, int nbc, const cplx<T> *phase,
+                cplx<double> *out, long ptot);
+template <typename T> int make_phase(pp_ctx *ctx, pp_sphere *sp, const double *kfrac, cplx<T> *phase);
+long sphere_npts(pp_sphere *sp);
+int sphere_ptot(pp_sphere *sp);
+int sphere_precision(pp_sphere *sp);
+const int *sphere_fftg(pp_sphere *sp);
+
+template <typename T>
+static int setup_batched(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac, const int *gofs, const int *G_dev,
+                         const cplx<float> *C, long ldc, int nband, cplx<double> *proj) {
+  const int *fftg = sphere_fftg(sp);
+  FftPlan *pl = nullptr;
+  PP_TRY(get_fft_plan(ctx, fftg, &pl));
+  int NB = ctx->tune_fft_bands > 0 ? ctx->tune_fft_bands : 64;
+  if (NB > 64)
+    NB = 64;
+  if (NB > nband)
+    NB = nband < 1 ? 1 : nband;
+  const long npts_grid = (long)fftg[0] * fftg[1] * fftg[2];
+  const long ptot = sphere_ptot(sp);
+  int maxw = 0;
+  for (int k = 0; k < nk; k++)
+    maxw = std::max(maxw, gofs[k + 1] - gofs[k]);
+  PP_TRY(ctx->grid.ensure(sizeof(cplx<T>) * (size_t)npts_grid * NB));
+  PP_TRY(ctx->gidx.ensure(sizeof(int) * (size_t)std::max(1, maxw)));
+  PP_TRY(ctx->phase.ensure(sizeof(cplx<T>) * (size_t)std::max<long>(1, sphere_npts(sp))));
+  cplx<T> *grid = (cplx<T> *)ctx->grid.p;
+  cplx<T> *phase = (cplx<T> *)ctx->phase.p;
+  int *gidx = (int *)ctx->gidx.p;
+  for (int k = 0; k < nk; k++) {
+    int nw = gofs[k + 1] - gofs[k];
+    PP_TRY(compute_gidx(ctx, fftg, nw, G_dev + 3 * (long)gofs[k], gidx));
+    PP_TRY(make_phase<T>(ctx, sp, kfrac + 3 * k, phase));
+    for (int b0 = 0; b0 < nband; b0 += NB) {
+      int nbc = std::min(NB, nband - b0);
+      PP_TRY(fft_chunk<T>(ctx, pl, grid, NB, nw, gidx, C + ((long)k * nband + b0) * ldc, ldc, nbc));
+      PP_TRY(sphere_sums<T>(ctx, sp, grid, NB, nbc, phase, proj + ((long)k * nband + b0) * ptot, ptot));
+    }
+  }
+  return PP_OK;
+}
+
+}  // namespace pp
+
+using namespace pp;
+
+extern "C" {
+
+const char *pp_last_error(void) { return g_err.c_str(); }
+int pp_last_status(void) { return g_code; }
+const char *pp_version(void) { return "pawpyseed_amd 0.1.0 (gfx950)"; }
+
+int pp_ctx_create(pp_ctx **out, int device) {
+  PP_REQUIRE(out, PP_ERR_ARG, "pp_ctx_create: null out");
+  int ndev = 0;
+  PP_CHECK_HIP(hipGetDeviceCount(&ndev));
+  PP_REQUIRE(device >= 0 && device < ndev, PP_ERR_ARG, "pp_ctx_create: no such device");
+  PP_CHECK_HIP(hipSetDevice(device));
+  pp_ctx *c = new pp_ctx();
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    PP_CHECK_HIP(e);
+  }
+  c->stream = c->own;
+  *out = c;
+  return PP_OK;
+}
+
+void pp_ctx_destroy(pp_ctx *ctx) {
+  if (!ctx)
+    return;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto &kv : ctx->fft_plans)
+    delete kv.second;
+  for (auto &pe : ctx->timer.pending) {
+    (void)hipEventDestroy(pe.second.first);
+    (void)hipEventDestroy(pe.second.second);
+  }
+  for (auto e : ctx->timer.pool)
+    (void)hipEventDestroy(e);
+  if (ctx->own)
+    (void)hipStreamDestroy(ctx->own);
+  delete ctx;
+}
+
+int pp_ctx_set_stream(pp_ctx *ctx, void *stream) {
+  PP_REQUIRE(ctx, PP_ERR_ARG, "pp_ctx_set_stream: null ctx");
+  ctx->stream = stream ? (hipStream_t)stream : ctx->own;
+  return PP_OK;
+}
+
+int pp_ctx_synchronize(pp_ctx *ctx) {
+  PP_REQUIRE(ctx, PP_ERR_ARG, "pp_ctx_synchronize: null ctx");
+  PP_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  return PP_OK;
+}
+
+int pp_ctx_set_timing(pp_ctx *ctx, int enable) {
+  PP_REQUIRE(ctx, PP_ERR_ARG, "pp_ctx_set_timing: null ctx");
+  timer_collect(ctx);
+  ctx->timer.on = enable != 0;
+  return PP_OK;
+}
+
+int pp_ctx_timing_reset(pp_ctx *ctx) {
+  PP_REQUIRE(ctx, PP_ERR_ARG, "pp_ctx_timing_reset: null ctx");
+  timer_collect(ctx);
+  for (int c = 0; c < K_NCLASS; c++) {
+    ctx->timer.ms[c] = 0;
+    ctx->timer.launches[c] = 0;
+  }
+  return PP_OK;
+}
+
+int pp_ctx_timing_read(pp_ctx *ctx, int n, const char **names, double *ms_total, long *launches) {
+  PP_REQUIRE(ctx, PP_ERR_ARG, "pp_ctx_timing_read: null ctx");
+  timer_collect(ctx);
+  for (int c = 0; c < K_NCLASS && c < n; c++) {
+    if (names)
+      names[c] = kclass_name(c);
+    if (ms_total)
+      ms_total[c] = ctx->timer.ms[c];
+    if (launches)
+      launches[c] = ctx->timer.launches[c];
+  }
+  return K_NCLASS;
+}
+
+int pp_ctx_set_tuning(pp_ctx *ctx, int fft_bands, int sphere_parts, int gemm_splitk) {
+  PP_REQUIRE(ctx, PP_ERR_ARG, "pp_ctx_set_tuning: null ctx");
+  PP_REQUIRE(fft_bands >= 0 && fft_bands <= 64, PP_ERR_ARG, "fft_bands must be 0..64");
+  ctx->tune_fft_bands = fft_bands;
+  ctx->tune_sphere_parts = sphere_parts;
+  ctx->tune_splitk = gemm_splitk;
+  return PP_OK;
+}
+
+int pp_setup_projections_batched(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac, const int *gofs,
+                                 const int *G_dev, const void *C_dev, long ldc, int nband, void *proj_dev,
+                                 int precision) {
+  PP_REQUIRE(ctx && sp && kfrac && gofs && nk >= 0 && nband >= 0, PP_ERR_ARG,
+             "pp_setup_projections_batched: bad arguments");
+  PP_REQUIRE(nband == 0 || (C_dev && proj_dev && G_dev), PP_ERR_ARG, "pp_setup_projections_batched: null buffer");
+  for (int k = 0; k < nk; k++)
+    PP_REQUIRE(gofs[k + 1] >= gofs[k] && gofs[k + 1] - gofs[k] <= ldc, PP_ERR_ARG,
+               "pp_setup_projections_batched: plane-wave count exceeds ldc");
+  int want = precision == PP_FP64 ? PP_FP64 : PP_FP32;
+  PP_REQUIRE(sphere_precision(sp) == want, PP_ERR_ARG,
+             "pp_setup_projections_batched: sphere plan built for another precision");
+  PP_CHECK_HIP(hipSetDevice(ctx->device));
+  if (nband == 0 || nk == 0)
+    return PP_OK;
+  if (want == PP_FP64)
+    return setup_batched<double>(ctx, sp, nk, kfrac, gofs, G_dev, (const cplx<float> *)C_dev, ldc, nband,
+                                 (cplx<double> *)proj_dev);
+  return setup_batched<float>(ctx, sp, nk, kfrac, gofs, G_dev, (const cplx<float> *)C_dev, ldc, nband,
+                              (cplx<double> *)proj_dev);
+}
+
+int pp_overlap_gemm(pp_ctx *ctx, int nwk, int nspin, int flip, const int *nw, const void *CR_dev, long ldR, int nbR,
+                    const void *CS_dev, long ldS, int nbS, void *out_dev, int precision) {
+  PP_REQUIRE(ctx && nw && nwk > 0 && (nspin == 1 || nspin == 2) && nbR >= 0 && nbS >= 0, PP_ERR_ARG,
+             "pp_overlap_gemm: bad arguments");
+  PP_REQUIRE(precision == PP_FP64 || precision == PP_FP32 || precision == PP_BF16X3, PP_ERR_ARG,
+             "pp_overlap_gemm: unknown precision");
+  const int nkt = nwk * nspin;
+  for (int k = 0; k < nkt; k++)
+    PP_REQUIRE(nw[k] >= 0 && nw[k] <= ldR && nw[k] <= ldS, PP_ERR_ARG, "pp_overlap_gemm: nw exceeds ld");
+  PP_CHECK_HIP(hipSetDevice(ctx->device));
+  std::vector<int> amap(nkt);
+  for (int k = 0; k < nkt; k++)
+    amap[k] = (flip && nspin == 2) ? (k < nwk ? k + nwk : k - nwk) : k;
+  return zgemm_nt(ctx, precision, nbR, nbS, 0, 1, CR_dev, ldR, (long)nbR * ldR, 1, CS_dev, ldS, (long)nbS * ldS,
+                  out_dev, nbS, (long)nbR * nbS, nkt, amap.data(), nullptr, nw, false);
+}
+
+}  // extern "C"

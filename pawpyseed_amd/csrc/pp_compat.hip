@@ -1,0 +1,797 @@
+// Reference-named C entry points over the device-resident data model.
+//
+// pswf_t keeps the reference's pointer-tree layout (pawpyseed/core/utils.h)
+// for callers that read it; the library additionally keeps each
+// structure's coefficients, G vectors and projections in HBM (WfDev,
+// pswf_t::dev) and answers the per-band calls of the reference
+// (pseudoprojection, compensation_terms) from all-band matrices computed
+// once on the GPU.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
+
+#include "pp_internal.h"
+
+namespace pp {
+
+struct PairCache {
+  std::vector<pp_cdouble> m;  // [nkt][nbR][nbS]
+  int nbR = 0, nbS = 0;
+};
+
+struct WfDev {
+  int nkt = 0, nband = 0;
+  long ldc = 0;
+  std::vector<int> nw, gofs;
+  DevBuf C, G;  // c64 [nkt][nband][ldc]; int32 [sum nw][3]
+  bool uploaded = false;
+  DevBuf proj;  // c128 [nkt][nband][ptot]
+  int ptot = 0, nsites = 0, proj_precision = -1;
+  std::vector<int> proj_off, labels;
+  std::vector<int> sp_nchan, sp_ls;
+  std::vector<double> sp_dO;  // aepw - pspw per species
+  std::map<std::tuple<const void *, int, int>, PairCache> overlap_cache;
+  std::map<std::tuple<const void *, int, unsigned long long>, PairCache> aug_cache;
+};
+
+static std::mutex g_mu;
+static pp_ctx *g_ctx = nullptr;
+static int g_precision = PP_FP64;
+
+static int global_ctx(pp_ctx **out) {
+  if (!g_ctx) {
+    int dev = 0;
+    const char *e = getenv("PAWPY_AMD_DEVICE");
+    if (e)
+      dev = atoi(e);
+    PP_TRY(pp_ctx_create(&g_ctx, dev));
+  }
+  *out = g_ctx;
+  return PP_OK;
+}
+
+static WfDev *dev_of(pswf_t *wf) {
+  if (!wf->dev)
+    wf->dev = new WfDev();
+  return (WfDev *)wf->dev;
+}
+
+static int ensure_upload(pp_ctx *ctx, pswf_t *wf) {
+  WfDev *d = dev_of(wf);
+  if (d->uploaded)
+    return PP_OK;
+  d->nkt = wf->nwk * wf->nspin;
+  d->nband = wf->nband;
+  d->nw.assign(d->nkt, 0);
+  d->gofs.assign(d->nkt + 1, 0);
+  int maxw = 0;
+  for (int k = 0; k < d->nkt; k++) {
+    d->nw[k] = wf->kpts[k]->num_waves;
+    d->gofs[k + 1] = d->gofs[k] + d->nw[k];
+    maxw = std::max(maxw, d->nw[k]);
+  }
+  d->ldc = std::max(64, (maxw + 63) / 64 * 64);
+  size_t cbytes = sizeof(pp_cfloat) * (size_t)d->nkt * d->nband * d->ldc;
+  PP_TRY(d->C.ensure(cbytes));
+  PP_TRY(d->G.ensure(sizeof(int) * 3 * (size_t)std::max(1, d->gofs[d->nkt])));
+  std::vector<pp_cfloat> stage((size_t)d->nband * d->ldc);
+  for (int k = 0; k < d->nkt; k++) {
+    std::fill(stage.begin(), stage.end(), pp_cfloat(0.f, 0.f));
+    for (int b = 0; b < d->nband; b++)
+      memcpy(&stage[(size_t)b * d->ldc], wf->kpts[k]->bands[b]->Cs, sizeof(pp_cfloat) * d->nw[k]);
+    PP_CHECK_HIP(hipMemcpy((pp_cfloat *)d->C.p + (size_t)k * d->nband * d->ldc, stage.data(),
+                           sizeof(pp_cfloat) * stage.size(), hipMemcpyHostToDevice));
+    if (d->nw[k])
+      PP_CHECK_HIP(hipMemcpy((int *)d->G.p + 3 * (size_t)d->gofs[k], wf->kpts[k]->Gs, sizeof(int) * 3 * d->nw[k],
+                             hipMemcpyHostToDevice));
+  }
+  d->uploaded = true;
+  return PP_OK;
+}
+
+static void free_projection_list(projection_t *pl, int num) {
+  if (!pl)
+    return;
+  for (int i = 0; i < num; i++) {
+    free(pl[i].ns);
+    free(pl[i].ls);
+    free(pl[i].ms);
+    free(pl[i].overlaps);
+  }
+  free(pl);
+}
+
+static double **spline3(const double *x, const double *y, int n) {
+  double **c = (double **)malloc(3 * sizeof(double *));
+  for (int i = 0; i < 3; i++)
+    c[i] = (double *)malloc(sizeof(double) * n);
+  spline_coeff_host(x, y, n, c[0], c[1], c[2]);
+  return c;
+}
+
+static void free_spline(double **s) {
+  if (!s)
+    return;
+  for (int i = 0; i < 3; i++)
+    free(s[i]);
+  free(s);
+}
+
+// utils.c:751-764
+static double spline_integral_host(const double *x, const double *a, double **s, int n) {
+  double r = 0.0;
+  for (int i = 0; i < n - 1; i++) {
+    double h = x[i + 1] - x[i];
+    r += h * (a[i] + h * (s[0][i] / 2 + h * (s[1][i] / 3 + s[2][i] * h / 4)));
+  }
+  return r;
+}
+
+// ------------------------------------------------------------------ WAVECAR
+struct Src {
+  FILE *fp = nullptr;
+  const char *mem = nullptr;
+  bool read(long off, void *dst, size_t n) {
+    if (fp) {
+      if (fseek(fp, off, SEEK_SET) != 0)
+        return false;
+      return fread(dst, 1, n, fp) == n;
+    }
+    memcpy(dst, mem + off, n);
+    return true;
+  }
+};
+
+static void cross3(const double *a, const double *b, double *o) {
+  o[0] = a[1] * b[2] - a[2] * b[1];
+  o[1] = a[2] * b[0] - a[0] * b[2];
+  o[2] = a[0] * b[1] - a[1] * b[0];
+}
+static double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static double mag3(const double *a) { return sqrt(dot3(a, a)); }
+
+// Half-extents of the G box from the cutoff and the reciprocal lattice
+// (the geometric bound of reader.c:55-127).
+static void gbox(const double *rec, double encut, double *nb) {
+  const double c = 0.262465831;
+  double m[3] = {mag3(rec), mag3(rec + 3), mag3(rec + 6)};
+  double g = sqrt(encut * c), v[3];
+  auto sinb = [&](const double *u, const double *w) { return fabs(sin(acos(dot3(u, w) / (mag3(u) * mag3(w))))); };
+  auto sinp = [&](const double *w, const double *u, const double *x) {
+    cross3(u, x, v);
+    return fabs(dot3(w, v) / (mag3(v) * mag3(w)));
+  };
+  const double *b1 = rec, *b2 = rec + 3, *b3 = rec + 6;
+  double A[3] = {g / (m[0] * sinb(b1, b2)) + 1, g / (m[1] * sinb(b1, b2)) + 1, g / (m[2] * sinp(b3, b1, b2)) + 1};
+  double B[3] = {g / (m[0] * sinb(b1, b3)) + 1, g / (m[1] * sinp(b2, b1, b3)) + 1, g / (m[2] * sinb(b1, b3)) + 1};
+  double C[3] = {g / (m[0] * sinp(b1, b2, b3)) + 1, g / (m[1] * sinb(b2, b3)) + 1, g / (m[2] * sinb(b2, b3)) + 1};
+  for (int i = 0; i < 3; i++)
+    nb[i] = std::max(A[i], std::max(B[i], C[i]));
+}
+
+static pswf_t *new_pswf(int nspin, int nwk, int nband, double encut, const double *lattice) {
+  pswf_t *wf = (pswf_t *)calloc(1, sizeof(pswf_t));
+  wf->nspin = nspin;
+  wf->nwk = nwk;
+  wf->nband = nband;
+  wf->encut = encut;
+  wf->lattice = (double *)malloc(9 * sizeof(double));
+  wf->reclattice = (double *)malloc(9 * sizeof(double));
+  memcpy(wf->lattice, lattice, 9 * sizeof(double));
+  double v = det3(lattice);
+  cross3(lattice + 3, lattice + 6, wf->reclattice);
+  cross3(lattice + 6, lattice + 0, wf->reclattice + 3);
+  cross3(lattice + 0, lattice + 3, wf->reclattice + 6);
+  for (int i = 0; i < 9; i++)
+    wf->reclattice[i] *= 2.0 * M_PI / v;
+  wf->G_bounds = (int *)calloc(6, sizeof(int));
+  wf->kpts = (kpoint_t **)calloc((size_t)nwk * nspin, sizeof(kpoint_t *));
+  return wf;
+}
+
+static kpoint_t *new_kpoint(int nband, int nplane) {
+  kpoint_t *kp = (kpoint_t *)calloc(1, sizeof(kpoint_t));
+  kp->num_bands = nband;
+  kp->num_waves = nplane;
+  kp->bands = (band_t **)calloc(nband, sizeof(band_t *));
+  kp->k = (double *)calloc(3, sizeof(double));
+  kp->Gs = (int *)malloc(sizeof(int) * 3 * (size_t)std::max(1, nplane));
+  for (int b = 0; b < nband; b++) {
+    band_t *bd = (band_t *)calloc(1, sizeof(band_t));
+    bd->n = b;
+    bd->num_waves = nplane;
+    bd->Cs = (pp_cfloat *)malloc(sizeof(pp_cfloat) * (size_t)std::max(1, nplane));
+    kp->bands[b] = bd;
+  }
+  return kp;
+}
+
+static pswf_t *read_wavecar_src(Src &src, const double *kpt_weights) {
+  double head[3];
+  if (!src.read(0, head, sizeof(head))) {
+    set_error(PP_ERR_IO, "WAVECAR: cannot read header");
+    return nullptr;
+  }
+  long nrecl = lround(head[0]);
+  int nspin = (int)lround(head[1]);
+  int nprec = (int)lround(head[2]);
+  if (nrecl <= 0 || (nspin != 1 && nspin != 2) || nprec != 45200) {
+    set_error(PP_ERR_IO, "WAVECAR: unsupported header (only single-precision WAVECARs)");
+    return nullptr;
+  }
+  std::vector<double> rec(nrecl / 8);
+  if (!src.read(nrecl, rec.data(), nrecl)) {
+    set_error(PP_ERR_IO, "WAVECAR: cannot read record 1");
+    return nullptr;
+  }
+  int nwk = (int)lround(rec[0]), nband = (int)lround(rec[1]);
+  double encut = rec[2];
+  pswf_t *wf = new_pswf(nspin, nwk, nband, encut, &rec[3]);
+  double nb[3];
+  gbox(wf->reclattice, encut, nb);
+  const double *b1 = wf->reclattice, *b2 = wf->reclattice + 3, *b3 = wf->reclattice + 6;
+  std::vector<float> cbuf(nrecl / 4);
+  for (int iwk = 0; iwk < nwk * nspin; iwk++) {
+    long irec = (long)iwk * (1 + nband);
+    if (!src.read((irec + 2) * nrecl, rec.data(), nrecl)) {
+      set_error(PP_ERR_IO, "WAVECAR: truncated k-point record");
+      free_pswf(wf);
+      return nullptr;
+    }
+    int nplane = (int)lround(rec[0]);
+    kpoint_t *kp = new_kpoint(nband, nplane);
+    wf->kpts[iwk] = kp;
+    kp->up = (short)(iwk / nwk);
+    kp->k[0] = rec[1];
+    kp->k[1] = rec[2];
+    kp->k[2] = rec[3];
+    for (int b = 0; b < nband; b++) {
+      kp->bands[b]->energy = rec[4 + 3 * b];
+      kp->bands[b]->occ = rec[6 + 3 * b];
+    }
+    // G enumeration in VASP order (reader.c:231-271)
+    int ncnt = 0;
+    for (int ig3 = 0; ig3 <= 2 * nb[2]; ig3++) {
+      int g3 = ig3 > nb[2] ? (int)(ig3 - 2 * nb[2] - 1) : ig3;
+      for (int ig2 = 0; ig2 <= 2 * nb[1]; ig2++) {
+        int g2 = ig2 > nb[1] ? (int)(ig2 - 2 * nb[1] - 1) : ig2;
+        for (int ig1 = 0; ig1 <= 2 * nb[0]; ig1++) {
+          int g1 = ig1 > nb[0] ? (int)(ig1 - 2 * nb[0] - 1) : ig1;
+          double s[3];
+          for (int j = 0; j < 3; j++)
+            s[j] = (kp->k[0] + g1) * b1[j] + (kp->k[1] + g2) * b2[j] + (kp->k[2] + g3) * b3[j];
+          double gt = mag3(s);
+          if (gt * gt / 0.262465831 <= encut) {
+            if (ncnt < nplane) {
+              kp->Gs[3 * ncnt] = g1;
+              kp->Gs[3 * ncnt + 1] = g2;
+              kp->Gs[3 * ncnt + 2] = g3;
+            }
+            ncnt++;
+            wf->G_bounds[0] = std::min(wf->G_bounds[0], g1);
+            wf->G_bounds[1] = std::max(wf->G_bounds[1], g1);
+            wf->G_bounds[2] = std::min(wf->G_bounds[2], g2);
+            wf->G_bounds[3] = std::max(wf->G_bounds[3], g2);
+            wf->G_bounds[4] = std::min(wf->G_bounds[4], g3);
+            wf->G_bounds[5] = std::max(wf->G_bounds[5], g3);
+          }
+        }
+      }
+    }
+    if (ncnt * 2 == nplane) {
+      // spinor: the G list repeats for the second component (reader.c:274-281)
+      wf->is_ncl = 1;
+      memcpy(kp->Gs + 3 * ncnt, kp->Gs, sizeof(int) * 3 * ncnt);
+    } else if (ncnt != nplane) {
+      set_error(PP_ERR_IO, "WAVECAR: plane-wave count " + std::to_string(nplane) + " differs from the " +
+                               std::to_string(ncnt) + " G vectors inside the cutoff");
+      free_pswf(wf);
+      return nullptr;
+    }
+    for (int b = 0; b < nband; b++) {
+      irec++;
+      if (!src.read((irec + 2) * nrecl, cbuf.data(), (size_t)nplane * 8)) {
+        set_error(PP_ERR_IO, "WAVECAR: truncated band record");
+        free_pswf(wf);
+        return nullptr;
+      }
+      memcpy(kp->bands[b]->Cs, cbuf.data(), (size_t)nplane * 8);
+    }
+    kp->weight = kpt_weights ? kpt_weights[iwk % nwk] : 1.0 / nwk;
+  }
+  return wf;
+}
+
+}  // namespace pp
+
+using namespace pp;
+
+extern "C" {
+
+int pp_set_precision(int precision) {
+  PP_REQUIRE(precision == PP_FP64 || precision == PP_FP32 || precision == PP_BF16X3, PP_ERR_ARG,
+             "pp_set_precision: unknown precision");
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_precision = precision;
+  return PP_OK;
+}
+int pp_get_precision(void) { return g_precision; }
+
+pswf_t *read_wavefunctions(char *filename, double *kpt_weights) {
+  Src s;
+  s.fp = fopen(filename, "rb");
+  if (!s.fp) {
+    set_error(PP_ERR_IO, std::string("cannot open ") + filename);
+    return nullptr;
+  }
+  pswf_t *wf = read_wavecar_src(s, kpt_weights);
+  fclose(s.fp);
+  return wf;
+}
+
+pswf_t *read_wavefunctions_from_str(char *start, double *kpt_weights) {
+  if (!start) {
+    set_error(PP_ERR_ARG, "read_wavefunctions_from_str: null buffer");
+    return nullptr;
+  }
+  Src s;
+  s.mem = start;
+  return read_wavecar_src(s, kpt_weights);
+}
+
+pswf_t *pp_pswf_from_arrays(int nspin, int nwk, int nband, double encut, const double *lattice, const double *kpts,
+                            const double *weights, const int *nplane, const int *Gs, const pp_cfloat *coeffs,
+                            const double *occ, const double *energy) {
+  if ((nspin != 1 && nspin != 2) || nwk <= 0 || nband <= 0 || !lattice || !kpts || !nplane || !Gs || !coeffs) {
+    set_error(PP_ERR_ARG, "pp_pswf_from_arrays: bad arguments");
+    return nullptr;
+  }
+  pswf_t *wf = new_pswf(nspin, nwk, nband, encut, lattice);
+  long go = 0, co = 0;
+  for (int iwk = 0; iwk < nwk * nspin; iwk++) {
+    int np = nplane[iwk];
+    kpoint_t *kp = new_kpoint(nband, np);
+    wf->kpts[iwk] = kp;
+    kp->up = (short)(iwk / nwk);
+    for (int j = 0; j < 3; j++)
+      kp->k[j] = kpts[3 * (iwk % nwk) + j];
+    kp->weight = weights ? weights[iwk % nwk] : 1.0 / nwk;
+    memcpy(kp->Gs, Gs + 3 * go, sizeof(int) * 3 * (size_t)np);
+    for (int w = 0; w < np; w++) {
+      int g[3] = {Gs[3 * (go + w)], Gs[3 * (go + w) + 1], Gs[3 * (go + w) + 2]};
+      for (int a = 0; a < 3; a++) {
+        wf->G_bounds[2 * a] = std::min(wf->G_bounds[2 * a], g[a]);
+        wf->G_bounds[2 * a + 1] = std::max(wf->G_bounds[2 * a + 1], g[a]);
+      }
+    }
+    go += np;
+    for (int b = 0; b < nband; b++) {
+      memcpy(kp->bands[b]->Cs, coeffs + co, sizeof(pp_cfloat) * (size_t)np);
+      co += np;
+      kp->bands[b]->occ = occ ? occ[(long)iwk * nband + b] : 0.0;
+      kp->bands[b]->energy = energy ? energy[(long)iwk * nband + b] : 0.0;
+    }
+  }
+  return wf;
+}
+
+void free_ppot_list(ppot_t *pps, int length) {
+  if (!pps)
+    return;
+  for (int i = 0; i < length; i++) {
+    ppot_t *pp = pps + i;
+    for (int k = 0; pp->funcs && k < pp->num_projs; k++) {
+      funcset_t *f = pp->funcs + k;
+      free(f->proj);
+      free(f->aewave);
+      free(f->pswave);
+      free(f->diffwave);
+      free(f->kwave);
+      free(f->smooth_diffwave);
+      free_spline(f->proj_spline);
+      free_spline(f->aewave_spline);
+      free_spline(f->pswave_spline);
+      free_spline(f->diffwave_spline);
+      free_spline(f->kwave_spline);
+      free_spline(f->smooth_diffwave_spline);
+    }
+    free(pp->funcs);
+    free(pp->wave_grid);
+    free(pp->kwave_grid);
+    free(pp->proj_grid);
+    free(pp->smooth_grid);
+    free(pp->pspw_overlap_matrix);
+    free(pp->aepw_overlap_matrix);
+    free(pp->diff_overlap_matrix);
+  }
+  free(pps);
+}
+
+void free_pswf(pswf_t *wf) {
+  if (!wf)
+    return;
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (int i = 0; wf->kpts && i < wf->nwk * wf->nspin; i++) {
+    kpoint_t *kp = wf->kpts[i];
+    if (!kp)
+      continue;
+    for (int b = 0; b < kp->num_bands; b++) {
+      band_t *bd = kp->bands[b];
+      if (!bd)
+        continue;
+      free(bd->Cs);
+      free_projection_list(bd->projections, wf->num_sites);
+      free_projection_list(bd->wave_projections, wf->wp_num);
+      free_projection_list(bd->up_projections, wf->num_sites);
+      free_projection_list(bd->down_projections, wf->num_sites);
+      free(bd->CRs);
+      free(bd->CAs);
+      free(bd);
+    }
+    free(kp->Gs);
+    free(kp->bands);
+    free(kp->k);
+    free(kp);
+  }
+  if (wf->overlaps) {
+    for (int i = 0; i < wf->num_aug_overlap_sites; i++)
+      free(wf->overlaps[i]);
+    free(wf->overlaps);
+  }
+  free(wf->num_projs);
+  free(wf->kpts);
+  free(wf->G_bounds);
+  free(wf->lattice);
+  free(wf->reclattice);
+  if (wf->pps)
+    free_ppot_list(wf->pps, wf->num_elems);
+  free(wf->dcoords);
+  free(wf->fftg);
+  delete (WfDev *)wf->dev;
+  free(wf);
+}
+
+double *get_occs(pswf_t *wf) {
+  int nk = wf->nwk * wf->nspin;
+  double *o = (double *)malloc(sizeof(double) * (size_t)nk * wf->nband);
+  for (int k = 0; k < nk; k++)
+    for (int b = 0; b < wf->nband; b++)
+      o[(long)b * nk + k] = wf->kpts[k]->bands[b]->occ;
+  return o;
+}
+int get_nband(pswf_t *wf) { return wf->nband; }
+int get_nwk(pswf_t *wf) { return wf->nwk; }
+int get_nspin(pswf_t *wf) { return wf->nspin; }
+int is_ncl(pswf_t *wf) { return wf->is_ncl; }
+double get_encut(pswf_t *wf) { return wf->encut; }
+double get_energy(pswf_t *wf, int band, int kpt, int spin) { return wf->kpts[kpt + spin * wf->nwk]->bands[band]->energy; }
+double get_occ(pswf_t *wf, int band, int kpt, int spin) { return wf->kpts[kpt + spin * wf->nwk]->bands[band]->occ; }
+void set_num_sites(pswf_t *wf, int nsites) { wf->num_sites = nsites; }
+void free_ptr(void *ptr) { free(ptr); }
+
+ppot_t *get_projector_list(int num_els, int *labels, int *ls, double *wave_grids, double *projectors,
+                           double *aewaves, double *pswaves, double *rmaxs, double grid_encut) {
+  (void)grid_encut;  // used only by the SBT filter of the O_R/O_S terms
+  if (num_els <= 0 || !labels || !ls || !wave_grids || !projectors || !aewaves || !pswaves || !rmaxs) {
+    set_error(PP_ERR_ARG, "get_projector_list: bad arguments");
+    return nullptr;
+  }
+  ppot_t *pps = (ppot_t *)calloc(num_els, sizeof(ppot_t));
+  long wgt = 0, wt = 0, pt = 0, lnum = 0;
+  for (int i = 0; i < num_els; i++) {
+    ppot_t *pp = pps + i;
+    pp->num_projs = labels[4 * i + 1];
+    pp->rmax = rmaxs[i];
+    pp->proj_gridsize = labels[4 * i + 2];
+    pp->wave_gridsize = labels[4 * i + 3];
+    int nw = pp->wave_gridsize, np = pp->proj_gridsize;
+    pp->wave_grid = (double *)malloc(sizeof(double) * nw);
+    pp->kwave_grid = (double *)calloc(nw, sizeof(double));
+    memcpy(pp->wave_grid, wave_grids + wgt, sizeof(double) * nw);
+    wgt += nw;
+    pp->proj_grid = (double *)malloc(sizeof(double) * np);
+    for (int j = 0; j < np; j++)
+      pp->proj_grid[j] = pp->rmax / np * j;  // projector.c:56
+    pp->wave_rmax = pp->wave_grid[nw - 1];
+    pp->smooth_grid = (double *)malloc(sizeof(double) * np);
+    for (int j = 0; j < np; j++)
+      pp->smooth_grid[j] = pp->wave_rmax / np * j;
+    pp->funcs = (funcset_t *)calloc(pp->num_projs, sizeof(funcset_t));
+    for (int k = 0; k < pp->num_projs; k++) {
+      funcset_t *f = pp->funcs + k;
+      f->l = ls[lnum++];
+      if (f->l > pp->lmax)
+        pp->lmax = f->l;
+      pp->total_projs += 2 * f->l + 1;
+      f->proj = (double *)malloc(sizeof(double) * np);
+      f->aewave = (double *)malloc(sizeof(double) * nw);
+      f->pswave = (double *)malloc(sizeof(double) * nw);
+      f->diffwave = (double *)malloc(sizeof(double) * nw);
+      memcpy(f->proj, projectors + pt, sizeof(double) * np);
+      pt += np;
+      for (int j = 0; j < nw; j++) {
+        f->aewave[j] = aewaves[wt + j];
+        f->pswave[j] = pswaves[wt + j];
+        f->diffwave[j] = aewaves[wt + j] - pswaves[wt + j];
+      }
+      wt += nw;
+      f->proj_spline = spline3(pp->proj_grid, f->proj, np);
+      f->aewave_spline = spline3(pp->wave_grid, f->aewave, nw);
+      f->pswave_spline = spline3(pp->wave_grid, f->pswave, nw);
+      f->diffwave_spline = spline3(pp->wave_grid, f->diffwave, nw);
+    }
+    // projector.c:507-558: same-l spline integrals, symmetric
+    int n = pp->num_projs;
+    pp->pspw_overlap_matrix = (double *)calloc((size_t)n * n, sizeof(double));
+    pp->aepw_overlap_matrix = (double *)calloc((size_t)n * n, sizeof(double));
+    pp->diff_overlap_matrix = (double *)calloc((size_t)n * n, sizeof(double));
+    std::vector<double> prod(nw);
+    for (int a = 0; a < n; a++)
+      for (int b = a; b < n; b++) {
+        if (pp->funcs[a].l != pp->funcs[b].l)
+          continue;
+        for (int t = 0; t < 3; t++) {
+          const funcset_t *fa = pp->funcs + a, *fb = pp->funcs + b;
+          for (int j = 0; j < nw; j++)
+            prod[j] = t == 0   ? fa->pswave[j] * fb->pswave[j]
+                      : t == 1 ? fa->aewave[j] * fb->aewave[j]
+                               : fa->diffwave[j] * fb->diffwave[j];
+          double **s = spline3(pp->wave_grid, prod.data(), nw);
+          double v = spline_integral_host(pp->wave_grid, prod.data(), s, nw);
+          free_spline(s);
+          double *M = t == 0 ? pp->pspw_overlap_matrix : t == 1 ? pp->aepw_overlap_matrix : pp->diff_overlap_matrix;
+          M[a * n + b] = v;
+          M[b * n + a] = v;
+        }
+      }
+  }
+  return pps;
+}
+
+int setup_projections(pswf_t *wf, ppot_t *pps, int num_elems, int num_sites, int *fftg, int *labels,
+                      double *coords) {
+  PP_REQUIRE(wf && pps && num_elems > 0 && num_sites >= 0 && fftg && (num_sites == 0 || (labels && coords)),
+             PP_ERR_ARG, "setup_projections: bad arguments");
+  PP_REQUIRE(!wf->is_ncl, PP_ERR_UNSUPPORTED, "setup_projections: noncollinear wavefunctions are not supported");
+  std::lock_guard<std::mutex> lk(g_mu);
+  pp_ctx *ctx;
+  PP_TRY(global_ctx(&ctx));
+  for (int s = 0; s < num_sites; s++)
+    PP_REQUIRE(labels[s] >= 0 && labels[s] < num_elems, PP_ERR_ARG, "setup_projections: label out of range");
+  // release projections of an earlier setup
+  for (int k = 0; k < wf->nwk * wf->nspin; k++)
+    for (int b = 0; b < wf->nband; b++) {
+      free_projection_list(wf->kpts[k]->bands[b]->projections, wf->num_sites);
+      wf->kpts[k]->bands[b]->projections = nullptr;
+    }
+  if (wf->pps && wf->pps != pps)
+    free_ppot_list(wf->pps, wf->num_elems);
+  wf->pps = pps;
+  wf->num_elems = num_elems;
+  wf->num_sites = num_sites;
+  if (!wf->fftg)
+    wf->fftg = (int *)malloc(3 * sizeof(int));
+  memcpy(wf->fftg, fftg, 3 * sizeof(int));
+  PP_TRY(ensure_upload(ctx, wf));
+  WfDev *d = dev_of(wf);
+  // species tables
+  std::vector<int> nchan(num_elems), ngrid(num_elems), lsv;
+  std::vector<double> rmax(num_elems), proj, dO;
+  for (int e = 0; e < num_elems; e++) {
+    nchan[e] = pps[e].num_projs;
+    ngrid[e] = pps[e].proj_gridsize;
+    rmax[e] = pps[e].rmax;
+    for (int ch = 0; ch < nchan[e]; ch++) {
+      lsv.push_back(pps[e].funcs[ch].l);
+      proj.insert(proj.end(), pps[e].funcs[ch].proj, pps[e].funcs[ch].proj + ngrid[e]);
+    }
+    for (int i = 0; i < nchan[e] * nchan[e]; i++)
+      dO.push_back(pps[e].aepw_overlap_matrix[i] - pps[e].pspw_overlap_matrix[i]);
+  }
+  int prec = g_precision == PP_FP64 ? PP_FP64 : PP_FP32;
+  pp_sphere *sp = nullptr;
+  PP_TRY(pp_sphere_create(ctx, &sp, wf->lattice, fftg, num_sites, labels, coords, num_elems, nchan.data(),
+                          lsv.data(), rmax.data(), ngrid.data(), proj.data(), prec));
+  int ptot = 0;
+  std::vector<int> proj_off(num_sites + 1);
+  pp_sphere_info(sp, &ptot, nullptr, proj_off.data());
+  int st = d->proj.ensure(sizeof(pp_cdouble) * (size_t)d->nkt * d->nband * std::max(1, ptot));
+  std::vector<double> kf(3 * (size_t)d->nkt);
+  for (int k = 0; k < d->nkt; k++)
+    for (int j = 0; j < 3; j++)
+      kf[3 * k + j] = wf->kpts[k]->k[j];
+  if (st == PP_OK)
+    st = pp_setup_projections_batched(ctx, sp, d->nkt, kf.data(), d->gofs.data(), (const int *)d->G.p, d->C.p,
+                                      d->ldc, d->nband, d->proj.p, prec);
+  if (st == PP_OK)
+    st = pp_ctx_synchronize(ctx);
+  pp_sphere_destroy(sp);
+  PP_TRY(st);
+  d->ptot = ptot;
+  d->nsites = num_sites;
+  d->proj_off = proj_off;
+  d->labels.assign(labels, labels + num_sites);
+  d->sp_nchan = nchan;
+  d->sp_ls = lsv;
+  d->sp_dO = dO;
+  d->proj_precision = prec;
+  d->aug_cache.clear();
+  // host copies in the reference's per-band projection_t lists
+  std::vector<pp_cdouble> host((size_t)d->nkt * d->nband * std::max(1, ptot));
+  PP_CHECK_HIP(hipMemcpy(host.data(), d->proj.p, sizeof(pp_cdouble) * host.size(), hipMemcpyDeviceToHost));
+  std::vector<int> lofs(num_elems + 1, 0);
+  for (int e = 0; e < num_elems; e++)
+    lofs[e + 1] = lofs[e] + nchan[e];
+  for (int k = 0; k < d->nkt; k++)
+    for (int b = 0; b < d->nband; b++) {
+      projection_t *pl = (projection_t *)calloc(std::max(1, num_sites), sizeof(projection_t));
+      const pp_cdouble *row = host.data() + ((size_t)k * d->nband + b) * ptot;
+      for (int s = 0; s < num_sites; s++) {
+        int e = labels[s], P = proj_off[s + 1] - proj_off[s];
+        pl[s].num_projs = nchan[e];
+        pl[s].total_projs = P;
+        pl[s].ns = (int *)malloc(sizeof(int) * P);
+        pl[s].ls = (int *)malloc(sizeof(int) * P);
+        pl[s].ms = (int *)malloc(sizeof(int) * P);
+        pl[s].overlaps = (pp_cdouble *)malloc(sizeof(pp_cdouble) * P);
+        int p = 0;
+        for (int ch = 0; ch < nchan[e]; ch++) {
+          int l = lsv[lofs[e] + ch];
+          for (int m = -l; m <= l; m++, p++) {
+            pl[s].ns[p] = ch;
+            pl[s].ls[p] = l;
+            pl[s].ms[p] = m;
+            pl[s].overlaps[p] = row[proj_off[s] + p];
+          }
+        }
+      }
+      wf->kpts[k]->bands[b]->projections = pl;
+    }
+  return PP_OK;
+}
+
+int pp_get_projections(pswf_t *wf, int band, int kidx, pp_cdouble *out) {
+  PP_REQUIRE(wf && wf->dev && out, PP_ERR_ARG, "pp_get_projections: no projections (call setup_projections)");
+  WfDev *d = (WfDev *)wf->dev;
+  PP_REQUIRE(d->proj_precision >= 0, PP_ERR_ARG, "pp_get_projections: no projections (call setup_projections)");
+  PP_REQUIRE(band >= 0 && band < d->nband && kidx >= 0 && kidx < d->nkt, PP_ERR_ARG,
+             "pp_get_projections: index out of range");
+  PP_CHECK_HIP(hipMemcpy(out, (pp_cdouble *)d->proj.p + ((size_t)kidx * d->nband + band) * d->ptot,
+                         sizeof(pp_cdouble) * d->ptot, hipMemcpyDeviceToHost));
+  return PP_OK;
+}
+
+int pseudoprojection(pp_cdouble *projections, pswf_t *wf_ref, pswf_t *wf_proj, int BAND_NUM, int flip_spin) {
+  PP_REQUIRE(projections && wf_ref && wf_proj, PP_ERR_ARG, "pseudoprojection: null argument");
+  PP_REQUIRE(BAND_NUM >= 0 && BAND_NUM < wf_proj->nband, PP_ERR_ARG, "pseudoprojection: band index out of range");
+  PP_REQUIRE(wf_ref->nwk == wf_proj->nwk && wf_ref->nspin == wf_proj->nspin, PP_ERR_ARG,
+             "pseudoprojection: k-point sets differ");
+  std::lock_guard<std::mutex> lk(g_mu);
+  pp_ctx *ctx;
+  PP_TRY(global_ctx(&ctx));
+  PP_TRY(ensure_upload(ctx, wf_ref));
+  PP_TRY(ensure_upload(ctx, wf_proj));
+  WfDev *dr = dev_of(wf_ref), *dp = dev_of(wf_proj);
+  const int nkt = dr->nkt;
+  int flip = (flip_spin && wf_ref->nspin == 2) ? 1 : 0;
+  auto key = std::make_tuple((const void *)wf_ref, flip, g_precision);
+  auto it = dp->overlap_cache.find(key);
+  if (it == dp->overlap_cache.end()) {
+    // pseudoprojector.c:85: length of the reference band at the unflipped k
+    for (int k = 0; k < nkt; k++)
+      PP_REQUIRE(dr->nw[k] <= dp->ldc, PP_ERR_ARG, "pseudoprojection: plane-wave bases differ");
+    DevBuf out;
+    PP_TRY(out.ensure(sizeof(pp_cdouble) * (size_t)nkt * dr->nband * dp->nband));
+    PP_TRY(pp_overlap_gemm(ctx, wf_ref->nwk, wf_ref->nspin, flip, dr->nw.data(), dr->C.p, dr->ldc, dr->nband,
+                           dp->C.p, dp->ldc, dp->nband, out.p, g_precision));
+    PairCache pc;
+    pc.nbR = dr->nband;
+    pc.nbS = dp->nband;
+    pc.m.resize((size_t)nkt * pc.nbR * pc.nbS);
+    PP_CHECK_HIP(hipMemcpyAsync(pc.m.data(), out.p, sizeof(pp_cdouble) * pc.m.size(), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    PP_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    it = dp->overlap_cache.emplace(key, std::move(pc)).first;
+  }
+  const PairCache &pc = it->second;
+  for (int b = 0; b < pc.nbR; b++)
+    for (int k = 0; k < nkt; k++)
+      projections[(long)b * nkt + k] = pc.m[((size_t)k * pc.nbR + b) * pc.nbS + BAND_NUM];
+  return PP_OK;
+}
+
+int compensation_terms(pp_cdouble *overlap, int BAND_NUM, pswf_t *wf_S, pswf_t *wf_R, int num_M, int num_N_R,
+                       int num_N_S, int num_N_RS, int *M_R, int *M_S, int *N_R, int *N_S, int *N_RS_R, int *N_RS_S,
+                       int *proj_labels, double *proj_coords, int *ref_labels, double *ref_coords, int *fft_grid,
+                       int flip_spin) {
+  (void)N_R;
+  (void)N_S;
+  (void)N_RS_R;
+  (void)N_RS_S;
+  (void)proj_labels;
+  (void)proj_coords;
+  (void)ref_labels;
+  (void)ref_coords;
+  (void)fft_grid;
+  PP_REQUIRE(overlap && wf_S && wf_R, PP_ERR_ARG, "compensation_terms: null argument");
+  PP_REQUIRE(num_N_R == 0 && num_N_S == 0 && num_N_RS == 0, PP_ERR_UNSUPPORTED,
+             "compensation_terms: the O_R/O_S/O_N terms (sites that differ between the structures) need "
+             "overlap_setup_real, which this build does not provide");
+  PP_REQUIRE(BAND_NUM >= 0 && BAND_NUM < wf_S->nband, PP_ERR_ARG, "compensation_terms: band index out of range");
+  PP_REQUIRE(num_M == 0 || (M_R && M_S), PP_ERR_ARG, "compensation_terms: null site list");
+  std::lock_guard<std::mutex> lk(g_mu);
+  WfDev *dS = (WfDev *)wf_S->dev, *dR = (WfDev *)wf_R->dev;
+  PP_REQUIRE(dS && dR && dS->proj_precision >= 0 && dR->proj_precision >= 0, PP_ERR_ARG,
+             "compensation_terms: call setup_projections on both wavefunctions first");
+  const int nkt = dR->nkt;
+  for (int t = 0; t < num_M; t++)
+    PP_REQUIRE(M_R[t] >= 0 && M_R[t] < dR->nsites && M_S[t] >= 0 && M_S[t] < dS->nsites, PP_ERR_ARG,
+               "compensation_terms: site index out of range");
+  if (num_M == 0)
+    return PP_OK;
+  pp_ctx *ctx;
+  PP_TRY(global_ctx(&ctx));
+  int flip = (flip_spin && wf_R->nspin == 2) ? 1 : 0;
+  unsigned long long h = 1469598103934665603ull;
+  for (int t = 0; t < num_M; t++) {
+    h = (h ^ (unsigned)M_R[t]) * 1099511628211ull;
+    h = (h ^ (unsigned)M_S[t]) * 1099511628211ull;
+  }
+  h ^= (unsigned long long)num_M << 32;
+  auto key = std::make_tuple((const void *)wf_R, flip, h);
+  auto it = dS->aug_cache.find(key);
+  if (it == dS->aug_cache.end()) {
+    DevBuf out;
+    size_t n = (size_t)nkt * dR->nband * dS->nband;
+    PP_TRY(out.ensure(sizeof(pp_cdouble) * n));
+    PP_TRY(zero_async(ctx, out.p, sizeof(pp_cdouble) * n));
+    // dO and channel tables of the reference structure's species
+    PP_TRY(pp_augmentation(ctx, wf_R->nwk, wf_R->nspin, flip, dR->proj.p, dR->nband, dR->ptot, dR->proj_off.data(),
+                           dR->labels.data(), dS->proj.p, dS->nband, dS->ptot, dS->proj_off.data(),
+                           dS->labels.data(), num_M, M_R, M_S, (int)dR->sp_nchan.size(), dR->sp_nchan.data(),
+                           dR->sp_ls.data(), dR->sp_dO.data(), out.p));
+    PairCache pc;
+    pc.nbR = dR->nband;
+    pc.nbS = dS->nband;
+    pc.m.resize(n);
+    PP_CHECK_HIP(hipMemcpyAsync(pc.m.data(), out.p, sizeof(pp_cdouble) * n, hipMemcpyDeviceToHost, ctx->stream));
+    PP_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    it = dS->aug_cache.emplace(key, std::move(pc)).first;
+  }
+  const PairCache &pc = it->second;
+  for (int b = 0; b < pc.nbR; b++)
+    for (int k = 0; k < nkt; k++)
+      overlap[(long)b * nkt + k] += pc.m[((size_t)k * pc.nbR + b) * pc.nbS + BAND_NUM];
+  return PP_OK;
+}
+
+int fft3d(pp_cdouble *x, int *G_bounds, double *lattice, double *kpt, int *Gs, pp_cfloat *Cs, int num_waves,
+          int *fftg) {
+  (void)G_bounds;
+  (void)kpt;
+  PP_REQUIRE(x && lattice && fftg && num_waves >= 0 && (num_waves == 0 || (Gs && Cs)), PP_ERR_ARG,
+             "fft3d: bad arguments");
+  std::lock_guard<std::mutex> lk(g_mu);
+  pp_ctx *ctx;
+  PP_TRY(global_ctx(&ctx));
+  long npts = (long)fftg[0] * fftg[1] * fftg[2];
+  DevBuf dG, dC, dX;
+  PP_TRY(dG.ensure(sizeof(int) * 3 * (size_t)std::max(1, num_waves)));
+  PP_TRY(dC.ensure(sizeof(pp_cfloat) * (size_t)std::max(1, num_waves)));
+  PP_TRY(dX.ensure(sizeof(pp_cdouble) * (size_t)npts));
+  if (num_waves) {
+    PP_CHECK_HIP(hipMemcpy(dG.p, Gs, sizeof(int) * 3 * num_waves, hipMemcpyHostToDevice));
+    PP_CHECK_HIP(hipMemcpy(dC.p, Cs, sizeof(pp_cfloat) * num_waves, hipMemcpyHostToDevice));
+  }
+  PP_TRY(pp_fft3d_batched(ctx, lattice, fftg, num_waves, (const int *)dG.p, dC.p, std::max(1, num_waves), 1, dX.p,
+                          g_precision == PP_FP64 ? PP_FP64 : PP_FP32));
+  PP_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  PP_CHECK_HIP(hipMemcpy(x, dX.p, sizeof(pp_cdouble) * npts, hipMemcpyDeviceToHost));
+  return PP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,414 @@
+// Plane-wave scatter and batched 3D backward FFT for gfx950.
+//
+// Replaces the per-band work of linalg.c:14-45 (fft3d): zero the grid,
+// place C_G at ((G+N) mod N) with z fastest, backward (e^{+i}) transform.
+// The 1/sqrt(V) scale of the reference is folded into the per-point phase
+// of the sphere sums, so these kernels compute the unscaled transform.
+//
+// Layout in HBM: a chunk of NB bands is stored band-fastest,
+//   grid[((x*N1 + y)*N2 + z)*NB + b],
+// so every FFT pass moves runs of NB contiguous complex values per grid
+// point and the sphere-sum gather reads one contiguous run per point.
+// Each 1D pass loads one line x NBW bands into LDS, runs a mixed-radix
+// Stockham transform (radices 4,2,3,5 and any other small prime) there,
+// and writes the line back in place.
+#include <cmath>
+
+#include "pp_internal.h"
+
+namespace pp {
+
+// ------------------------------------------------------------------ plan
+static int factor_axis(int n, AxisPlan &a) {
+  a.n = n;
+  a.nst = 0;
+  int m = n;
+  while (m % 4 == 0 && a.nst < 12) {
+    a.radix[a.nst++] = 4;
+    m /= 4;
+  }
+  for (int q = 2; m > 1 && q <= m; q++)
+    while (m % q == 0) {
+      if (a.nst >= 12 || q > 13)
+        return PP_ERR_UNSUPPORTED;
+      a.radix[a.nst++] = q;
+      m /= q;
+    }
+  return PP_OK;
+}
+
+int get_fft_plan(pp_ctx *ctx, const int *fftg, FftPlan **out) {
+  long long key = ((long long)fftg[0] << 40) | ((long long)fftg[1] << 20) | fftg[2];
+  auto it = ctx->fft_plans.find(key);
+  if (it != ctx->fft_plans.end()) {
+    *out = it->second;
+    return PP_OK;
+  }
+  FftPlan *pl = new FftPlan();
+  std::vector<double> tw;  // interleaved c128
+  for (int a = 0; a < 3; a++) {
+    pl->fftg[a] = fftg[a];
+    AxisPlan &ap = pl->ax[a];
+    if (factor_axis(fftg[a], ap) != PP_OK) {
+      delete pl;
+      PP_REQUIRE(false, PP_ERR_UNSUPPORTED,
+                 "FFT length " + std::to_string(fftg[a]) + " has a prime factor above 13");
+    }
+    pl->twbase[a] = (int)(tw.size() / 2);
+    int p = 1;
+    for (int s = 0; s < ap.nst; s++) {
+      int R = ap.radix[s];
+      ap.pprev[s] = p;
+      ap.twoff[s] = (int)(tw.size() / 2) - pl->twbase[a];
+      // twiddles exp(+2 pi i k r / (p R)), k < p, r = 1..R-1
+      for (int k = 0; k < p; k++)
+        for (int r = 1; r < R; r++) {
+          long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)(k * r) /
+                            (long double)(p * R);
+          tw.push_back((double)cosl(ang));
+          tw.push_back((double)sinl(ang));
+        }
+      // roots exp(+2 pi i m / R) for the generic small DFT
+      for (int m = 0; m < R; m++) {
+        long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)R;
+        tw.push_back((double)cosl(ang));
+        tw.push_back((double)sinl(ang));
+      }
+      p *= R;
+    }
+  }
+  size_t ntw = tw.size() / 2;
+  std::vector<float> tw32(tw.size());
+  for (size_t i = 0; i < tw.size(); i++)
+    tw32[i] = (float)tw[i];
+  if (pl->tw64.ensure(ntw * 16) != PP_OK || pl->tw32.ensure(ntw * 8) != PP_OK) {
+    delete pl;
+    return PP_ERR_NOMEM;
+  }
+  PP_CHECK_HIP(hipMemcpy(pl->tw64.p, tw.data(), ntw * 16, hipMemcpyHostToDevice));
+  PP_CHECK_HIP(hipMemcpy(pl->tw32.p, tw32.data(), ntw * 8, hipMemcpyHostToDevice));
+  ctx->fft_plans[key] = pl;
+  *out = pl;
+  return PP_OK;
+}
+
+// ------------------------------------------------------------------ kernels
+__global__ __launch_bounds__(256) void k_zero(uint4 *__restrict__ p, size_t n16) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t st = (size_t)gridDim.x * blockDim.x;
+  uint4 z = {0u, 0u, 0u, 0u};
+  for (; i < n16; i += st)
+    p[i] = z;
+}
+
+__global__ __launch_bounds__(256) void k_zero_bytes(unsigned char *__restrict__ p, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n)
+    p[i] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_gidx(int n0, int n1, int n2, int nw, const int *__restrict__ G,
+                                              int *__restrict__ gidx) {
+  int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nw)
+    return;
+  int a = ((G[3 * w] % n0) + n0) % n0;
+  int b = ((G[3 * w + 1] % n1) + n1) % n1;
+  int c = ((G[3 * w + 2] % n2) + n2) % n2;
+  gidx[w] = (a * n1 + b) * n2 + c;
+}
+
+// Scatter a 32-coefficient x NB-band tile through LDS: coalesced reads
+// along G of each band row, contiguous NB-band writes per grid point.
+template <typename T>
+__global__ __launch_bounds__(256) void k_scatter(cplx<T> *__restrict__ g, int NB, const int *__restrict__ gidx,
+                                                 int nw, const cplx<float> *__restrict__ C, long ldc, int nbc) {
+  __shared__ cplx<T> tile[64][33];
+  int g0 = blockIdx.x * 32;
+  for (int e = threadIdx.x; e < 64 * 32; e += 256) {
+    int b = e >> 5, gg = e & 31;
+    cplx<T> v = mk<T>(0, 0);
+    if (b < nbc && g0 + gg < nw)
+      v = cvt<T>(C[(long)b * ldc + g0 + gg]);
+    tile[b][gg] = v;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 64 * 32; e += 256) {
+    int gg = e >> 6, b = e & 63;
+    if (b < nbc && b < NB && g0 + gg < nw)
+      g[(long)gidx[g0 + gg] * NB + b] = tile[b][gg];
+  }
+}
+
+template <typename T> struct Consts;
+template <> struct Consts<double> {
+  static constexpr double s3 = 0.86602540378443864676;
+  static constexpr double c51 = 0.30901699437494742410, c52 = -0.80901699437494742410;
+  static constexpr double s51 = 0.95105651629515357212, s52 = 0.58778525229247312917;
+};
+template <> struct Consts<float> {
+  static constexpr float s3 = 0.86602540378443864676f;
+  static constexpr float c51 = 0.30901699437494742410f, c52 = -0.80901699437494742410f;
+  static constexpr float s51 = 0.95105651629515357212f, s52 = 0.58778525229247312917f;
+};
+
+// Backward (e^{+2 pi i rq/R}) DFT of R in {2,3,4,5} values in registers.
+template <typename T>
+__device__ __forceinline__ void small_dft(int R, cplx<T> *v) {
+  if (R == 4) {
+    cplx<T> a0 = v[0] + v[2], a1 = v[0] - v[2], b0 = v[1] + v[3], b1 = mul_i(v[1] - v[3]);
+    v[0] = a0 + b0;
+    v[2] = a0 - b0;
+    v[1] = a1 + b1;
+    v[3] = a1 - b1;
+  } else if (R == 2) {
+    cplx<T> a = v[0], b = v[1];
+    v[0] = a + b;
+    v[1] = a - b;
+  } else if (R == 3) {
+    cplx<T> s = v[1] + v[2], d = v[1] - v[2];
+    cplx<T> m = v[0] - scale(s, (T)0.5);
+    cplx<T> t = mul_i(scale(d, Consts<T>::s3));
+    v[0] = v[0] + s;
+    v[1] = m + t;
+    v[2] = m - t;
+  } else if (R == 5) {
+    cplx<T> a1 = v[1] + v[4], b1 = v[1] - v[4], a2 = v[2] + v[3], b2 = v[2] - v[3];
+    cplx<T> m1 = v[0] + scale(a1, Consts<T>::c51) + scale(a2, Consts<T>::c52);
+    cplx<T> m2 = v[0] + scale(a1, Consts<T>::c52) + scale(a2, Consts<T>::c51);
+    cplx<T> t1 = mul_i(scale(b1, Consts<T>::s51) + scale(b2, Consts<T>::s52));
+    cplx<T> t2 = mul_i(scale(b1, Consts<T>::s52) - scale(b2, Consts<T>::s51));
+    v[0] = v[0] + a1 + a2;
+    v[1] = m1 + t1;
+    v[4] = m1 - t1;
+    v[2] = m2 + t2;
+    v[3] = m2 - t2;
+  }
+}
+
+// One Stockham stage of radix R over a line tile in LDS:
+// v[r] = in[j + r*Tn] * w^(k r) (k = j mod p), small DFT, then
+// out[(j/p)*p*R + k + r*p] = V[r].
+template <typename T, int NBW, int R>
+__device__ __forceinline__ void stage(const cplx<T> *__restrict__ in, cplx<T> *__restrict__ out,
+                                      const cplx<T> *__restrict__ tws, int p, int Tn) {
+  for (int t = threadIdx.x; t < Tn * NBW; t += 256) {
+    const int j = t / NBW, b = t % NBW;
+    const int k = j % p;
+    cplx<T> v[R];
+    v[0] = in[j * NBW + b];
+#pragma unroll
+    for (int r = 1; r < R; r++)
+      v[r] = in[(j + r * Tn) * NBW + b] * tws[k * (R - 1) + (r - 1)];
+    small_dft<T>(R, v);
+    const int o = (j / p) * p * R + k;
+#pragma unroll
+    for (int r = 0; r < R; r++)
+      out[(o + r * p) * NBW + b] = v[r];
+  }
+}
+
+// Any other small prime radix: each output computed directly from LDS
+// (O(R^2) work, no local arrays).
+template <typename T, int NBW>
+__device__ void stage_generic(const cplx<T> *__restrict__ in, cplx<T> *__restrict__ out,
+                              const cplx<T> *__restrict__ tws, int p, int Tn, int R) {
+  const cplx<T> *roots = tws + p * (R - 1);
+  for (int t = threadIdx.x; t < Tn * NBW; t += 256) {
+    const int j = t / NBW, b = t % NBW;
+    const int k = j % p;
+    const int o = (j / p) * p * R + k;
+    for (int q = 0; q < R; q++) {
+      cplx<T> acc = in[j * NBW + b];
+      for (int r = 1; r < R; r++) {
+        cplx<T> x = in[(j + r * Tn) * NBW + b] * tws[k * (R - 1) + (r - 1)];
+        acc = acc + x * roots[(r * q) % R];
+      }
+      out[(o + q * p) * NBW + b] = acc;
+    }
+  }
+}
+
+// One pass of 1D transforms along an axis.  Work-group: one line (fixed
+// other two coordinates) x NBW bands.  Element e of band slot b lives at
+// g[base + e*es + b].
+template <typename T, int NBW>
+__global__ __launch_bounds__(256) void k_fft_pass(cplx<T> *__restrict__ g, AxisPlan pl,
+                                                  const cplx<T> *__restrict__ tw, long es, long s1, long s2,
+                                                  int n2, int nslices) {
+  extern __shared__ unsigned char smem_raw[];
+  cplx<T> *buf0 = reinterpret_cast<cplx<T> *>(smem_raw);
+  const int n = pl.n;
+  cplx<T> *buf1 = buf0 + n * NBW;
+  const int blk = blockIdx.x;
+  const int slice = blk % nslices;
+  const int line = blk / nslices;
+  const long base = (long)(line / n2) * s1 + (long)(line % n2) * s2 + (long)slice * NBW;
+  const int tot = n * NBW;
+  for (int t = threadIdx.x; t < tot; t += 256) {
+    int e = t / NBW, b = t % NBW;
+    buf0[t] = g[base + (long)e * es + b];
+  }
+  __syncthreads();
+  cplx<T> *in = buf0, *out = buf1;
+  for (int s = 0; s < pl.nst; s++) {
+    const int R = pl.radix[s], p = pl.pprev[s], Tn = n / R;
+    const cplx<T> *tws = tw + pl.twoff[s];
+    switch (R) {
+    case 4:
+      stage<T, NBW, 4>(in, out, tws, p, Tn);
+      break;
+    case 3:
+      stage<T, NBW, 3>(in, out, tws, p, Tn);
+      break;
+    case 2:
+      stage<T, NBW, 2>(in, out, tws, p, Tn);
+      break;
+    case 5:
+      stage<T, NBW, 5>(in, out, tws, p, Tn);
+      break;
+    default:
+      stage_generic<T, NBW>(in, out, tws, p, Tn, R);
+    }
+    __syncthreads();
+    cplx<T> *tmp = in;
+    in = out;
+    out = tmp;
+  }
+  for (int t = threadIdx.x; t < tot; t += 256) {
+    int e = t / NBW, b = t % NBW;
+    g[base + (long)e * es + b] = in[t];
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_to_bandmajor(const cplx<T> *__restrict__ g, int NB, int nbc, long npts,
+                                                      double scl, cplx<double> *__restrict__ out) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npts * nbc)
+    return;
+  long pt = i / nbc;
+  int b = (int)(i % nbc);
+  out[(long)b * npts + pt] = scale(cvt<double>(g[pt * NB + b]), scl);
+}
+
+// ------------------------------------------------------------------ host
+int zero_async(pp_ctx *ctx, void *p, size_t bytes) {
+  hipEvent_t ev;
+  tbegin(ctx, K_ZERO, &ev);
+  size_t n16 = bytes / 16;
+  if (n16) {
+    size_t blocks = (n16 + 255) / 256;
+    if (blocks > 65536)
+      blocks = 65536;
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, (uint4 *)p, n16);
+  }
+  size_t tail = bytes - n16 * 16;
+  if (tail)
+    hipLaunchKernelGGL(k_zero_bytes, dim3(1), dim3(256), 0, ctx->stream, (unsigned char *)p + n16 * 16, tail);
+  tend(ctx, K_ZERO, ev);
+  PP_CHECK_HIP(hipGetLastError());
+  return PP_OK;
+}
+
+int compute_gidx(pp_ctx *ctx, const int *fftg, int nw, const int *G_dev, int *gidx_dev) {
+  if (nw <= 0)
+    return PP_OK;
+  hipLaunchKernelGGL(k_gidx, dim3((nw + 255) / 256), dim3(256), 0, ctx->stream, fftg[0], fftg[1], fftg[2], nw,
+                     G_dev, gidx_dev);
+  PP_CHECK_HIP(hipGetLastError());
+  return PP_OK;
+}
+
+template <typename T, int NBW>
+static int launch_pass(pp_ctx *ctx, cplx<T> *grid, const AxisPlan &ap, const cplx<T> *tw, long es, long s1,
+                       long s2, int n1, int n2, int NB, int cls) {
+  int nslices = NB / NBW;
+  size_t lds = (size_t)2 * ap.n * NBW * sizeof(cplx<T>);
+  long nblk = (long)n1 * n2 * nslices;
+  hipEvent_t ev;
+  tbegin(ctx, cls, &ev);
+  hipLaunchKernelGGL((k_fft_pass<T, NBW>), dim3((unsigned)nblk), dim3(256), lds, ctx->stream, grid, ap, tw, es,
+                     s1, s2, n2, nslices);
+  tend(ctx, cls, ev);
+  PP_CHECK_HIP(hipGetLastError());
+  return PP_OK;
+}
+
+template <typename T>
+static int run_pass(pp_ctx *ctx, cplx<T> *grid, const AxisPlan &ap, const cplx<T> *tw, long es, long s1, long s2,
+                    int n1, int n2, int NB, int cls) {
+  const size_t ldsmax = 64 * 1024;
+  int nbw = 32;
+  while (nbw > 1 && (NB % nbw != 0 || (size_t)2 * ap.n * nbw * sizeof(cplx<T>) > ldsmax))
+    nbw >>= 1;
+  PP_REQUIRE((size_t)2 * ap.n * nbw * sizeof(cplx<T>) <= ldsmax, PP_ERR_UNSUPPORTED,
+             "FFT line too long for LDS: " + std::to_string(ap.n));
+  switch (nbw) {
+  case 32:
+    return launch_pass<T, 32>(ctx, grid, ap, tw, es, s1, s2, n1, n2, NB, cls);
+  case 16:
+    return launch_pass<T, 16>(ctx, grid, ap, tw, es, s1, s2, n1, n2, NB, cls);
+  case 8:
+    return launch_pass<T, 8>(ctx, grid, ap, tw, es, s1, s2, n1, n2, NB, cls);
+  case 4:
+    return launch_pass<T, 4>(ctx, grid, ap, tw, es, s1, s2, n1, n2, NB, cls);
+  case 2:
+    return launch_pass<T, 2>(ctx, grid, ap, tw, es, s1, s2, n1, n2, NB, cls);
+  default:
+    return launch_pass<T, 1>(ctx, grid, ap, tw, es, s1, s2, n1, n2, NB, cls);
+  }
+}
+
+template <typename T> static const cplx<T> *tw_of(FftPlan *pl, int axis);
+template <> const cplx<double> *tw_of<double>(FftPlan *pl, int axis) {
+  return (const cplx<double> *)pl->tw64.p + pl->twbase[axis];
+}
+template <> const cplx<float> *tw_of<float>(FftPlan *pl, int axis) {
+  return (const cplx<float> *)pl->tw32.p + pl->twbase[axis];
+}
+
+template <typename T>
+int fft_chunk(pp_ctx *ctx, FftPlan *pl, cplx<T> *grid, int NB, int nw, const int *gidx_dev,
+              const cplx<float> *C_dev, long ldc, int nbc) {
+  const long n0 = pl->fftg[0], n1 = pl->fftg[1], n2 = pl->fftg[2];
+  const long npts = n0 * n1 * n2;
+  PP_TRY(zero_async(ctx, grid, (size_t)npts * NB * sizeof(cplx<T>)));
+  if (nw > 0) {
+    hipEvent_t ev;
+    tbegin(ctx, K_SCATTER, &ev);
+    hipLaunchKernelGGL(k_scatter<T>, dim3((nw + 31) / 32), dim3(256), 0, ctx->stream, grid, NB, gidx_dev, nw, C_dev,
+                       ldc, nbc);
+    tend(ctx, K_SCATTER, ev);
+    PP_CHECK_HIP(hipGetLastError());
+  }
+  // z lines: (x, y); y lines: (x, z); x lines: (y, z)
+  PP_TRY(run_pass<T>(ctx, grid, pl->ax[2], tw_of<T>(pl, 2), NB, n1 * n2 * NB, n2 * NB, (int)n0, (int)n1, NB,
+                     K_FFT_Z));
+  PP_TRY(run_pass<T>(ctx, grid, pl->ax[1], tw_of<T>(pl, 1), n2 * NB, n1 * n2 * NB, NB, (int)n0, (int)n2, NB,
+                     K_FFT_Y));
+  PP_TRY(run_pass<T>(ctx, grid, pl->ax[0], tw_of<T>(pl, 0), n1 * n2 * NB, n2 * NB, NB, (int)n1, (int)n2, NB,
+                     K_FFT_X));
+  return PP_OK;
+}
+
+template int fft_chunk<double>(pp_ctx *, FftPlan *, cplx<double> *, int, int, const int *, const cplx<float> *,
+                               long, int);
+template int fft_chunk<float>(pp_ctx *, FftPlan *, cplx<float> *, int, int, const int *, const cplx<float> *, long,
+                              int);
+
+int grid_to_bandmajor(pp_ctx *ctx, int precision, const void *grid, int NB, int nbc, long npts, double scl,
+                      void *out) {
+  long n = npts * nbc;
+  unsigned blocks = (unsigned)((n + 255) / 256);
+  if (precision == PP_FP64)
+    hipLaunchKernelGGL(k_to_bandmajor<double>, dim3(blocks), dim3(256), 0, ctx->stream,
+                       (const cplx<double> *)grid, NB, nbc, npts, scl, (cplx<double> *)out);
+  else
+    hipLaunchKernelGGL(k_to_bandmajor<float>, dim3(blocks), dim3(256), 0, ctx->stream, (const cplx<float> *)grid,
+                       NB, nbc, npts, scl, (cplx<double> *)out);
+  PP_CHECK_HIP(hipGetLastError());
+  return PP_OK;
+}
+
+}  // namespace pp

@@ -1,0 +1,169 @@
+// Complex "NT" GEMM: C[m][n] (+)= sum_k conj(A[m][k]) B[n][k].
+//
+// The band-overlap matrix of pseudoprojector.c:63-90 (one cdotc per band
+// pair) and the O_M augmentation sum of projector.c:890-910 are both this
+// contraction with K contiguous in both operands.  FP64 path: 64x64 output
+// tiles, 256 threads with 4x4 complex accumulators each, K tiles of 16
+// staged through LDS; split-K with a deterministic reduction when the tile
+// grid alone cannot fill the 256 CUs.
+#include "pp_internal.h"
+
+namespace pp {
+
+template <typename TA, typename TB>
+__global__ __launch_bounds__(256) void k_zgemm_f64(int M, int N, int kbeg_stride, int K, const cplx<TA> *__restrict__ A,
+                                                   long lda, const cplx<TB> *__restrict__ B, long ldb,
+                                                   cplx<double> *__restrict__ C, long ldc, long sSplit) {
+  __shared__ cplx<double> As[16][64 + 1];
+  __shared__ cplx<double> Bs[16][64 + 1];
+  const int split = blockIdx.y;
+  const int tilesN = (N + 63) / 64;
+  const int tm = blockIdx.x / tilesN, tn = blockIdx.x % tilesN;
+  const int k0 = split * kbeg_stride;
+  const int k1 = min(K, k0 + kbeg_stride);
+  C += (long)split * sSplit;
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  double ar[4][4], ai[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      ar[i][j] = ai[i][j] = 0.0;
+  for (int kb = k0; kb < k1; kb += 16) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      int e = tid + q * 256;
+      int r = e >> 4, kk = e & 15;
+      int gm = tm * 64 + r, gn = tn * 64 + r, gk = kb + kk;
+      cplx<double> va = mk<double>(0, 0), vb = mk<double>(0, 0);
+      if (gk < k1) {
+        if (gm < M)
+          va = cvt<double>(A[(long)gm * lda + gk]);
+        if (gn < N)
+          vb = cvt<double>(B[(long)gn * ldb + gk]);
+      }
+      As[kk][r] = va;
+      Bs[kk][r] = vb;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int kk = 0; kk < 16; kk++) {
+      cplx<double> a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+        a[i] = As[kk][ty + 16 * i];
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        b[j] = Bs[kk][tx + 16 * j];
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          // conj(a) * b
+          ar[i][j] = fma(a[i].x, b[j].x, ar[i][j]);
+          ar[i][j] = fma(a[i].y, b[j].y, ar[i][j]);
+          ai[i][j] = fma(a[i].x, b[j].y, ai[i][j]);
+          ai[i][j] = fma(-a[i].y, b[j].x, ai[i][j]);
+        }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      int gm = tm * 64 + ty + 16 * i, gn = tn * 64 + tx + 16 * j;
+      if (gm < M && gn < N)
+        C[(long)gm * ldc + gn] = mk<double>(ar[i][j], ai[i][j]);
+    }
+}
+
+// C[m][n] (+)= sum over splits of partial[s][m][n]
+__global__ __launch_bounds__(256) void k_splitk_reduce(int M, int N, int nsplit, const cplx<double> *__restrict__ P,
+                                                       cplx<double> *__restrict__ C, long ldc, int accumulate) {
+  long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (long)M * N)
+    return;
+  int m = (int)(t / N), n = (int)(t % N);
+  cplx<double> s = accumulate ? C[(long)m * ldc + n] : mk<double>(0, 0);
+  for (int q = 0; q < nsplit; q++)
+    s = s + P[(long)q * M * N + t];
+  C[(long)m * ldc + n] = s;
+}
+
+int gemm_fp32_mfma(pp_ctx *ctx, int precision, int M, int N, int K, const void *A, long lda, const void *B, long ldb,
+                   cplx<double> *C, long ldc, bool accumulate);
+
+static int zgemm_one(pp_ctx *ctx, int precision, int M, int N, int K, int atype, const void *A, long lda, int btype,
+                     const void *B, long ldb, cplx<double> *C, long ldc, bool accumulate, int cls) {
+  if (M <= 0 || N <= 0)
+    return PP_OK;
+  if (precision != PP_FP64 && atype == 1 && btype == 1)
+    return gemm_fp32_mfma(ctx, precision, M, N, K, A, lda, B, ldb, C, ldc, accumulate);
+  int tiles = ((M + 63) / 64) * ((N + 63) / 64);
+  int nsplit = ctx->tune_splitk;
+  if (nsplit <= 0) {
+    nsplit = (2048 + tiles - 1) / tiles;
+    int maxs = std::max(1, K / 512);
+    if (nsplit > maxs)
+      nsplit = maxs;
+  }
+  if (nsplit < 1)
+    nsplit = 1;
+  int kchunk = (K + nsplit - 1) / nsplit;
+  kchunk = (kchunk + 15) / 16 * 16;
+  nsplit = std::max(1, (K + kchunk - 1) / std::max(1, kchunk));
+  bool direct = nsplit == 1 && !accumulate;
+  cplx<double> *dst = C;
+  long dld = ldc, sSplit = 0;
+  if (!direct) {
+    PP_TRY(ctx->gemm_ws.ensure(sizeof(cplx<double>) * (size_t)nsplit * M * N));
+    dst = (cplx<double> *)ctx->gemm_ws.p;
+    dld = N;
+    sSplit = (long)M * N;
+  }
+  dim3 grd(tiles, nsplit);
+  hipEvent_t ev;
+  tbegin(ctx, cls, &ev);
+  if (atype == 1 && btype == 1)
+    hipLaunchKernelGGL((k_zgemm_f64<float, float>), grd, dim3(256), 0, ctx->stream, M, N, kchunk, K,
+                       (const cplx<float> *)A, lda, (const cplx<float> *)B, ldb, dst, dld, sSplit);
+  else if (atype == 0 && btype == 0)
+    hipLaunchKernelGGL((k_zgemm_f64<double, double>), grd, dim3(256), 0, ctx->stream, M, N, kchunk, K,
+                       (const cplx<double> *)A, lda, (const cplx<double> *)B, ldb, dst, dld, sSplit);
+  else if (atype == 0 && btype == 1)
+    hipLaunchKernelGGL((k_zgemm_f64<double, float>), grd, dim3(256), 0, ctx->stream, M, N, kchunk, K,
+                       (const cplx<double> *)A, lda, (const cplx<float> *)B, ldb, dst, dld, sSplit);
+  else
+    hipLaunchKernelGGL((k_zgemm_f64<float, double>), grd, dim3(256), 0, ctx->stream, M, N, kchunk, K,
+                       (const cplx<float> *)A, lda, (const cplx<double> *)B, ldb, dst, dld, sSplit);
+  tend(ctx, cls, ev);
+  PP_CHECK_HIP(hipGetLastError());
+  if (!direct) {
+    long tot = (long)M * N;
+    tbegin(ctx, K_GEMM_REDUCE, &ev);
+    hipLaunchKernelGGL(k_splitk_reduce, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream, M, N, nsplit,
+                       (const cplx<double> *)ctx->gemm_ws.p, C, ldc, accumulate ? 1 : 0);
+    tend(ctx, K_GEMM_REDUCE, ev);
+    PP_CHECK_HIP(hipGetLastError());
+  }
+  return PP_OK;
+}
+
+int zgemm_nt(pp_ctx *ctx, int precision, int M, int N, int K, int atype, const void *A, long lda, long sA, int btype,
+             const void *B, long ldb, long sB, void *C, long ldc, long sC, int nbatch, const int *amap,
+             const int *bmap, const int *klen, bool accumulate) {
+  size_t ea = atype == 0 ? 16 : 8, eb = btype == 0 ? 16 : 8;
+  for (int b = 0; b < nbatch; b++) {
+    int ia = amap ? amap[b] : b, ib = bmap ? bmap[b] : b;
+    int kk = klen ? klen[b] : K;
+    const char *pa = (const char *)A + (size_t)ia * sA * ea;
+    const char *pb = (const char *)B + (size_t)ib * sB * eb;
+    cplx<double> *pc = (cplx<double> *)C + (size_t)b * sC;
+    PP_TRY(zgemm_one(ctx, precision, M, N, kk, atype, pa, lda, btype, pb, ldb, pc, ldc, accumulate,
+                     atype == 0 ? K_AUG_GEMM : K_GEMM));
+  }
+  return PP_OK;
+}
+
+}  // namespace pp

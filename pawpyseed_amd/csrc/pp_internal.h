@@ -1,0 +1,106 @@
+// Internal host-side interfaces shared by the library's translation units.
+#pragma once
+#include <map>
+#include <string>
+#include <vector>
+
+#include "pp_common.h"
+
+namespace pp {
+
+// Kernel classes timed by the context's optional event timer.
+enum KClass {
+  K_ZERO = 0,
+  K_SCATTER,
+  K_FFT_Z,
+  K_FFT_Y,
+  K_FFT_X,
+  K_PHASE,
+  K_SPHERE_SUM,
+  K_SPHERE_REDUCE,
+  K_TABLE,
+  K_GEMM,
+  K_GEMM_REDUCE,
+  K_AUG_PREP,
+  K_AUG_GEMM,
+  K_MISC,
+  K_NCLASS
+};
+const char *kclass_name(int c);
+
+// Device buffer owned by the library (grows, never shrinks).
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t need);
+  void release();
+  ~DevBuf() { release(); }
+};
+
+// Length-n 1D FFT plan of one axis: radices and twiddle offsets.
+struct AxisPlan {
+  int n = 0, nst = 0;
+  int radix[12];
+  int pprev[12];
+  int twoff[12];
+};
+
+struct FftPlan {
+  int fftg[3] = {0, 0, 0};
+  AxisPlan ax[3];
+  DevBuf tw64, tw32;  // concatenated per-axis twiddle tables (c128 / c64)
+  int twbase[3] = {0, 0, 0};
+};
+
+struct Timer {
+  bool on = false;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  std::vector<hipEvent_t> pool;
+  double ms[K_NCLASS] = {0};
+  long launches[K_NCLASS] = {0};
+};
+
+}  // namespace pp
+
+struct pp_ctx {
+  int device = 0;
+  hipStream_t own = nullptr;
+  hipStream_t stream = nullptr;
+  int tune_fft_bands = 0, tune_sphere_parts = 0, tune_splitk = 0;
+  pp::DevBuf grid;      // band-fastest FFT chunk
+  pp::DevBuf partial;   // sphere-sum partials
+  pp::DevBuf gidx;      // wrapped grid index per G
+  pp::DevBuf phase;     // per-point Bloch phase of the current k
+  pp::DevBuf gemm_ws;   // split-K partials
+  pp::DevBuf aug_ws;    // gathered / dO-applied projections
+  std::map<long long, pp::FftPlan *> fft_plans;
+  pp::Timer timer;
+};
+
+namespace pp {
+// timing helpers: call begin before a launch and end after it
+void tbegin(pp_ctx *ctx, int cls, hipEvent_t *start);
+void tend(pp_ctx *ctx, int cls, hipEvent_t start);
+
+int get_fft_plan(pp_ctx *ctx, const int *fftg, FftPlan **out);
+
+// Fill a device buffer with zeros (a kernel; no memset node).
+int zero_async(pp_ctx *ctx, void *p, size_t bytes);
+
+// Band-fastest grid chunk pipeline pieces (pp_fft.hip).
+int compute_gidx(pp_ctx *ctx, const int *fftg, int nw, const int *G_dev, int *gidx_dev);
+template <typename T>
+int fft_chunk(pp_ctx *ctx, FftPlan *pl, cplx<T> *grid, int NB, int nw, const int *gidx_dev,
+              const cplx<float> *C_dev, long ldc, int nbc);
+int grid_to_bandmajor(pp_ctx *ctx, int precision, const void *grid, int NB, int nbc, long npts,
+                      double scl, void *out);
+double det3(const double *m);
+void spline_coeff_host(const double *x, const double *y, int n, double *b, double *c, double *d);
+
+// Complex GEMM out[b][m][n] (+)= sum_k conj(A[b][m][k]) B[b][n][k] (pp_gemm.hip).
+// type codes: 0 = c128, 1 = c64.  Batch strides in elements; amap/bmap
+// optional per-batch index remaps (host arrays of length nbatch).
+int zgemm_nt(pp_ctx *ctx, int precision, int M, int N, int K, int atype, const void *A, long lda,
+             long sA, int btype, const void *B, long ldb, long sB, void *C, long ldc, long sC,
+             int nbatch, const int *amap, const int *bmap, const int *klen, bool accumulate);
+}  // namespace pp

@@ -1,0 +1,325 @@
+// Mixed-precision band-overlap GEMM on the CDNA4 matrix cores.
+//
+// Complex conj(A) B^T with c64 operands is two real GEMMs over the
+// interleaved (re, im) storage taken as a real K' = 2K dimension:
+//   Re = sum_k' A'[m][k'] B'[n][k'],   B' = (b_re, b_im) as stored,
+//   Im = sum_k' A'[m][k'] B''[n][k'],  B'' = (b_im, -b_re),
+// so no operand repacking is needed for the real part and B'' is the
+// stored pair swapped with one sign flip.
+//
+// PP_FP32: v_mfma_f32_16x16x4_f32, FP32 accumulation over at most 512
+// complex terms, then FP64 (per thread, then across split-K chunks).  PP_BF16X3: each FP32 operand is split into
+// hi = bf16(x) and lo = bf16(x - hi); hi*hi + hi*lo + lo*hi on
+// v_mfma_f32_16x16x32_bf16 with FP32 accumulation.
+//
+// Work-group: 4 wavefronts, 64x64 complex outputs, each wavefront a 32x32
+// quadrant as 2x2 16x16 MFMA tiles (real and imaginary accumulators).
+#include "pp_internal.h"
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+namespace pp {
+
+// MFMA 16x16 output layout: lane l, register v holds row 4*(l/16)+v,
+// column l%16 of the 16x16 tile.
+__device__ __forceinline__ void store_tile(cplx<double> *C, long ldc, int M, int N, int r0, int c0, floatx4 re,
+                                           floatx4 im, int lane) {
+  const int j = c0 + (lane & 15);
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const int i = r0 + 4 * (lane >> 4) + v;
+    if (i < M && j < N)
+      C[(long)i * ldc + j] = mk<double>((double)re[v], (double)im[v]);
+  }
+}
+
+// As store_tile, adding FP64 partial sums of earlier K blocks.
+__device__ __forceinline__ void store_tile64(cplx<double> *C, long ldc, int M, int N, int r0, int c0,
+                                             const double *dre, const double *dim, floatx4 re, floatx4 im,
+                                             int lane) {
+  const int j = c0 + (lane & 15);
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const int i = r0 + 4 * (lane >> 4) + v;
+    if (i < M && j < N)
+      C[(long)i * ldc + j] = mk<double>(dre[v] + (double)re[v], dim[v] + (double)im[v]);
+  }
+}
+
+// Load 4 consecutive complex (8 floats) of row `row` starting at complex
+// column gk, zero outside [0, M) x [.., k1).
+template <bool VEC>
+__device__ __forceinline__ void load8(const float *__restrict__ P, long ld, int row, int nrows, int gk, int k1,
+                                      float *v) {
+  if (VEC && row < nrows && gk + 4 <= k1) {
+    const float4 *p = reinterpret_cast<const float4 *>(P + ((long)row * ld + gk) * 2);
+    float4 x = p[0], y = p[1];
+    v[0] = x.x;
+    v[1] = x.y;
+    v[2] = x.z;
+    v[3] = x.w;
+    v[4] = y.x;
+    v[5] = y.y;
+    v[6] = y.z;
+    v[7] = y.w;
+  } else {
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      bool ok = row < nrows && gk + c < k1;
+      v[2 * c] = ok ? P[((long)row * ld + gk + c) * 2] : 0.f;
+      v[2 * c + 1] = ok ? P[((long)row * ld + gk + c) * 2 + 1] : 0.f;
+    }
+  }
+}
+
+template <bool VEC>
+__global__ __launch_bounds__(256) void k_cgemm_f32(int M, int N, int kchunk, int K, const float *__restrict__ A,
+                                                   long lda, const float *__restrict__ B, long ldb,
+                                                   cplx<double> *__restrict__ C, long ldc, long sSplit) {
+  __shared__ float As[64][36];
+  __shared__ float Bs[64][36];
+  const int split = blockIdx.y;
+  const int tilesN = (N + 63) / 64;
+  const int tm = blockIdx.x / tilesN, tn = blockIdx.x % tilesN;
+  const int k0 = split * kchunk;
+  const int k1 = min(K, k0 + kchunk);
+  C += (long)split * sSplit;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  floatx4 cr[2][2], ci[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      cr[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+      ci[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+    }
+  double dr[2][2][4], di[2][2][4];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int v = 0; v < 4; v++)
+        dr[a][b][v] = di[a][b][v] = 0.0;
+  const int lrow = threadIdx.x >> 2, part = threadIdx.x & 3;
+  int since_flush = 0;
+  for (int kb = k0; kb < k1; kb += 16) {
+    float va[8], vb[8];
+    load8<VEC>(A, lda, tm * 64 + lrow, M, kb + part * 4, k1, va);
+    load8<VEC>(B, ldb, tn * 64 + lrow, N, kb + part * 4, k1, vb);
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+      As[lrow][part * 8 + c] = va[c];
+      Bs[lrow][part * 8 + c] = vb[c];
+    }
+    __syncthreads();
+    const int sub = lane & 15, kq = lane >> 4;
+#pragma unroll
+    for (int kk = 0; kk < 32; kk += 4) {
+      const int kr = kk + kq;
+      const float a0 = As[wm * 32 + sub][kr];
+      const float a1 = As[wm * 32 + 16 + sub][kr];
+#pragma unroll
+      for (int nj = 0; nj < 2; nj++) {
+        const int col = wn * 32 + nj * 16 + sub;
+        const float b = Bs[col][kr];
+        const float b2 = (kq & 1) ? -Bs[col][kr - 1] : Bs[col][kr + 1];
+        cr[0][nj] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b, cr[0][nj], 0, 0, 0);
+        ci[0][nj] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b2, ci[0][nj], 0, 0, 0);
+        cr[1][nj] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b, cr[1][nj], 0, 0, 0);
+        ci[1][nj] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b2, ci[1][nj], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+    // bound the FP32 accumulation length: fold into FP64 every 512 terms
+    if (++since_flush == 32) {
+      since_flush = 0;
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            dr[a][b][v] += (double)cr[a][b][v];
+            di[a][b][v] += (double)ci[a][b][v];
+          }
+          cr[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+          ci[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+        }
+    }
+  }
+#pragma unroll
+  for (int mi = 0; mi < 2; mi++)
+#pragma unroll
+    for (int nj = 0; nj < 2; nj++)
+      store_tile64(C, ldc, M, N, tm * 64 + wm * 32 + mi * 16, tn * 64 + wn * 32 + nj * 16, dr[mi][nj], di[mi][nj],
+                   cr[mi][nj], ci[mi][nj], lane);
+}
+
+// round-to-nearest-even float -> bf16 bits
+__device__ __forceinline__ unsigned short bf16_bits(float x) {
+  unsigned u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf16_val(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+
+__device__ __forceinline__ bf16x8 ld8(const unsigned short *p) {
+  uint4 u = *reinterpret_cast<const uint4 *>(p);
+  return __builtin_bit_cast(bf16x8, u);
+}
+
+template <bool VEC>
+__global__ __launch_bounds__(256) void k_cgemm_bf16x3(int M, int N, int kchunk, int K, const float *__restrict__ A,
+                                                      long lda, const float *__restrict__ B, long ldb,
+                                                      cplx<double> *__restrict__ C, long ldc, long sSplit) {
+  // 32 complex = 64 real k' per stage; rows padded to 72 (144 B)
+  __shared__ __attribute__((aligned(16))) unsigned short Ah[64][72], Al[64][72];
+  __shared__ __attribute__((aligned(16))) unsigned short Bh[64][72], Bl[64][72];
+  __shared__ __attribute__((aligned(16))) unsigned short Qh[64][72], Ql[64][72];
+  const int split = blockIdx.y;
+  const int tilesN = (N + 63) / 64;
+  const int tm = blockIdx.x / tilesN, tn = blockIdx.x % tilesN;
+  const int k0 = split * kchunk;
+  const int k1 = min(K, k0 + kchunk);
+  C += (long)split * sSplit;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  floatx4 cr[2][2], ci[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      cr[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+      ci[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+    }
+  const int lrow = threadIdx.x >> 2, part = threadIdx.x & 3;
+  for (int kb = k0; kb < k1; kb += 32) {
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      float va[8], vb[8];
+      const int gk = kb + part * 8 + h * 4;
+      load8<VEC>(A, lda, tm * 64 + lrow, M, gk, k1, va);
+      load8<VEC>(B, ldb, tn * 64 + lrow, N, gk, k1, vb);
+      const int o = part * 16 + h * 8;
+#pragma unroll
+      for (int c = 0; c < 8; c++) {
+        unsigned short ah = bf16_bits(va[c]);
+        Ah[lrow][o + c] = ah;
+        Al[lrow][o + c] = bf16_bits(va[c] - bf16_val(ah));
+        unsigned short bh = bf16_bits(vb[c]);
+        Bh[lrow][o + c] = bh;
+        Bl[lrow][o + c] = bf16_bits(vb[c] - bf16_val(bh));
+        // B'' = (b_im, -b_re)
+        const float q = (c & 1) ? -vb[c - 1] : vb[c + 1];
+        unsigned short qh = bf16_bits(q);
+        Qh[lrow][o + c] = qh;
+        Ql[lrow][o + c] = bf16_bits(q - bf16_val(qh));
+      }
+    }
+    __syncthreads();
+    const int sub = lane & 15, kq = lane >> 4;
+#pragma unroll
+    for (int kk = 0; kk < 64; kk += 32) {
+      const int kc = kk + kq * 8;
+      bf16x8 ah[2], al[2];
+#pragma unroll
+      for (int mi = 0; mi < 2; mi++) {
+        ah[mi] = ld8(&Ah[wm * 32 + mi * 16 + sub][kc]);
+        al[mi] = ld8(&Al[wm * 32 + mi * 16 + sub][kc]);
+      }
+#pragma unroll
+      for (int nj = 0; nj < 2; nj++) {
+        const int col = wn * 32 + nj * 16 + sub;
+        const bf16x8 bh = ld8(&Bh[col][kc]), bl = ld8(&Bl[col][kc]);
+        const bf16x8 qh = ld8(&Qh[col][kc]), ql = ld8(&Ql[col][kc]);
+#pragma unroll
+        for (int mi = 0; mi < 2; mi++) {
+          cr[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[mi], bh, cr[mi][nj], 0, 0, 0);
+          cr[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mi], bl, cr[mi][nj], 0, 0, 0);
+          cr[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mi], bh, cr[mi][nj], 0, 0, 0);
+          ci[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[mi], qh, ci[mi][nj], 0, 0, 0);
+          ci[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mi], ql, ci[mi][nj], 0, 0, 0);
+          ci[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mi], qh, ci[mi][nj], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int mi = 0; mi < 2; mi++)
+#pragma unroll
+    for (int nj = 0; nj < 2; nj++)
+      store_tile(C, ldc, M, N, tm * 64 + wm * 32 + mi * 16, tn * 64 + wn * 32 + nj * 16, cr[mi][nj], ci[mi][nj],
+                 lane);
+}
+
+__global__ __launch_bounds__(256) void k_splitk_reduce2(int M, int N, int nsplit, const cplx<double> *__restrict__ P,
+                                                        cplx<double> *__restrict__ C, long ldc, int accumulate) {
+  long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (long)M * N)
+    return;
+  int m = (int)(t / N), n = (int)(t % N);
+  cplx<double> s = accumulate ? C[(long)m * ldc + n] : mk<double>(0, 0);
+  for (int q = 0; q < nsplit; q++)
+    s = s + P[(long)q * M * N + t];
+  C[(long)m * ldc + n] = s;
+}
+
+int gemm_fp32_mfma(pp_ctx *ctx, int precision, int M, int N, int K, const void *A, long lda, const void *B, long ldb,
+                   cplx<double> *C, long ldc, bool accumulate) {
+  const int kstep = precision == PP_BF16X3 ? 32 : 16;
+  int tiles = ((M + 63) / 64) * ((N + 63) / 64);
+  int nsplit = ctx->tune_splitk;
+  if (nsplit <= 0) {
+    nsplit = (2048 + tiles - 1) / tiles;
+    // FP32 accumulation: keep chunks at most ~8k complex terms
+    int minsplit = (K + 8191) / 8192;
+    int maxs = std::max(1, K / 256);
+    if (nsplit > maxs)
+      nsplit = maxs;
+    if (nsplit < minsplit)
+      nsplit = minsplit;
+  }
+  if (nsplit < 1)
+    nsplit = 1;
+  int kchunk = (K + nsplit - 1) / std::max(1, nsplit);
+  kchunk = (kchunk + kstep - 1) / kstep * kstep;
+  if (kchunk == 0)
+    kchunk = kstep;
+  nsplit = std::max(1, (K + kchunk - 1) / kchunk);
+  PP_TRY(ctx->gemm_ws.ensure(sizeof(cplx<double>) * (size_t)nsplit * M * N));
+  cplx<double> *P = (cplx<double> *)ctx->gemm_ws.p;
+  bool vec = (lda % 2 == 0) && (ldb % 2 == 0) && ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0);
+  dim3 grd(tiles, nsplit);
+  hipEvent_t ev;
+  tbegin(ctx, K_GEMM, &ev);
+  if (precision == PP_BF16X3) {
+    if (vec)
+      hipLaunchKernelGGL(k_cgemm_bf16x3<true>, grd, dim3(256), 0, ctx->stream, M, N, kchunk, K, (const float *)A,
+                         lda, (const float *)B, ldb, P, (long)N, (long)M * N);
+    else
+      hipLaunchKernelGGL(k_cgemm_bf16x3<false>, grd, dim3(256), 0, ctx->stream, M, N, kchunk, K, (const float *)A,
+                         lda, (const float *)B, ldb, P, (long)N, (long)M * N);
+  } else {
+    if (vec)
+      hipLaunchKernelGGL(k_cgemm_f32<true>, grd, dim3(256), 0, ctx->stream, M, N, kchunk, K, (const float *)A, lda,
+                         (const float *)B, ldb, P, (long)N, (long)M * N);
+    else
+      hipLaunchKernelGGL(k_cgemm_f32<false>, grd, dim3(256), 0, ctx->stream, M, N, kchunk, K, (const float *)A, lda,
+                         (const float *)B, ldb, P, (long)N, (long)M * N);
+  }
+  tend(ctx, K_GEMM, ev);
+  PP_CHECK_HIP(hipGetLastError());
+  long tot = (long)M * N;
+  tbegin(ctx, K_GEMM_REDUCE, &ev);
+  hipLaunchKernelGGL(k_splitk_reduce2, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream, M, N, nsplit,
+                     (const cplx<double> *)P, C, ldc, accumulate ? 1 : 0);
+  tend(ctx, K_GEMM_REDUCE, ev);
+  PP_CHECK_HIP(hipGetLastError());
+  return PP_OK;
+}
+
+}  // namespace pp

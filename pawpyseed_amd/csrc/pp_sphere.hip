@@ -1,0 +1,696 @@
+// Projector spheres: table build and the <p_i|psi_nk> sphere sums.
+//
+// Table build replaces projector.c:193-209 / utils.c:590-696 (serial in the
+// reference): per site, every grid point of the bounding box
+// +-(int(|a_j x a_k| rmax / V * N_i) + 1) around round(frac * N) with
+// |r - R| < (n-1)/n * rmax, in the reference's i-j-k loop order.  Each
+// point stores its wrapped grid index, its unwrapped path r - R (for the
+// Bloch phase) and conj(p_j(r - R)) for every projector j of the site,
+// where p_j = spline(|r|) Y_lm evaluated on the minimum-image vector
+// (utils.c:547-576).
+//
+// Sphere sums replace projector.c:223-274 (onto_projector_helper):
+// <p_j|psi> = sum_i conj(p_j(i)) psi(idx_i) dv exp(i k.path_i).  One
+// wavefront per (site, part of its points); lane = band of the chunk, so
+// each point's NB bands are one contiguous load from the band-fastest
+// grid while the projector values and phase are uniform across the wave.
+#include <cmath>
+#include <cstring>
+
+#include "pp_internal.h"
+
+struct pp_sphere {
+  pp_ctx *ctx = nullptr;
+  int precision = PP_FP64;
+  double lattice[9];
+  int fftg[3];
+  int nsites = 0, nelems = 0;
+  std::vector<int> labels;
+  std::vector<double> frac;
+  std::vector<int> nchan, ls_off, ngrid, grid_off;
+  std::vector<int> ls;
+  std::vector<double> rmax;
+  std::vector<int> nproj_site, proj_off;  // host
+  std::vector<int> site_pt_h;
+  int ptot = 0, pmax = 0;
+  long npts = 0;
+  // device
+  pp::DevBuf d_spline;  // per species: x, f, b, c, d [nchan][ngrid]
+  pp::DevBuf d_sitemeta;
+  pp::DevBuf d_site_pt, d_site_np, d_tab_off, d_proj_off, d_counts;
+  pp::DevBuf d_pidx, d_path, d_table;
+  pp::DevBuf d_chan;  // per species channel l list
+};
+
+namespace pp {
+
+// per-site geometry handed to the table kernels
+struct SiteGeo {
+  double coord[3];
+  double rmax;
+  double R0;
+  int g[3];
+  int c[3];
+  int elem;
+  int nproj;
+  int nchan;
+  int ngrid;
+  long spl_off;  // offset (doubles) of the species spline block
+  int chan_off;  // offset into the channel-l array
+};
+
+__device__ __forceinline__ void d_frac_to_cart(const double *f, const double *lat, double *o) {
+  o[0] = f[0] * lat[0] + f[1] * lat[3] + f[2] * lat[6];
+  o[1] = f[0] * lat[1] + f[1] * lat[4] + f[2] * lat[7];
+  o[2] = f[0] * lat[2] + f[1] * lat[5] + f[2] * lat[8];
+}
+
+struct Lat9 {
+  double m[9];
+};
+
+// decode candidate c of a site's box into (i, j, k), reference loop order
+__device__ __forceinline__ void box_ijk(const SiteGeo &s, int c, int &i, int &j, int &k) {
+  int w2 = 2 * s.g[2] + 1, w1 = 2 * s.g[1] + 1;
+  k = c % w2 - s.g[2] + s.c[2];
+  int t = c / w2;
+  j = t % w1 - s.g[1] + s.c[1];
+  i = t / w1 - s.g[0] + s.c[0];
+}
+
+__device__ __forceinline__ bool inside(const SiteGeo &s, const Lat9 &L, const int *fftg, int i, int j, int k,
+                                       double *cv) {
+  double fv[3] = {(double)i / fftg[0] - s.coord[0], (double)j / fftg[1] - s.coord[1],
+                  (double)k / fftg[2] - s.coord[2]};
+  d_frac_to_cart(fv, L.m, cv);
+  return sqrt(cv[0] * cv[0] + cv[1] * cv[1] + cv[2] * cv[2]) < s.R0;
+}
+
+__global__ __launch_bounds__(256) void k_site_count(const SiteGeo *__restrict__ geo, Lat9 L, int f0, int f1, int f2,
+                                                    int *__restrict__ counts) {
+  const SiteGeo s = geo[blockIdx.x];
+  const int fftg[3] = {f0, f1, f2};
+  int ncand = (2 * s.g[0] + 1) * (2 * s.g[1] + 1) * (2 * s.g[2] + 1);
+  int cnt = 0;
+  for (int c = threadIdx.x; c < ncand; c += 256) {
+    int i, j, k;
+    box_ijk(s, c, i, j, k);
+    double cv[3];
+    cnt += inside(s, L, fftg, i, j, k, cv) ? 1 : 0;
+  }
+  __shared__ int red[256];
+  red[threadIdx.x] = cnt;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o)
+      red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    counts[blockIdx.x] = red[0];
+}
+
+// Ordered compaction of the inside candidates (block scan per 256 chunk).
+__global__ __launch_bounds__(256) void k_site_fill(const SiteGeo *__restrict__ geo, Lat9 L, int f0, int f1, int f2,
+                                                   const int *__restrict__ site_pt, int *__restrict__ pidx,
+                                                   double *__restrict__ path) {
+  const SiteGeo s = geo[blockIdx.x];
+  const int fftg[3] = {f0, f1, f2};
+  int ncand = (2 * s.g[0] + 1) * (2 * s.g[1] + 1) * (2 * s.g[2] + 1);
+  __shared__ int scan[256];
+  __shared__ int running;
+  if (threadIdx.x == 0)
+    running = site_pt[blockIdx.x];
+  __syncthreads();
+  for (int c0 = 0; c0 < ncand; c0 += 256) {
+    int c = c0 + threadIdx.x;
+    int i = 0, j = 0, k = 0;
+    double cv[3] = {0, 0, 0};
+    int in = 0;
+    if (c < ncand) {
+      box_ijk(s, c, i, j, k);
+      in = inside(s, L, fftg, i, j, k, cv) ? 1 : 0;
+    }
+    scan[threadIdx.x] = in;
+    __syncthreads();
+    // Hillis-Steele inclusive scan
+    for (int o = 1; o < 256; o <<= 1) {
+      int v = threadIdx.x >= o ? scan[threadIdx.x - o] : 0;
+      __syncthreads();
+      scan[threadIdx.x] += v;
+      __syncthreads();
+    }
+    int pos = running + scan[threadIdx.x] - in;
+    if (in) {
+      int ii = (i % fftg[0] + fftg[0]) % fftg[0];
+      int jj = (j % fftg[1] + fftg[1]) % fftg[1];
+      int kk = (k % fftg[2] + fftg[2]) % fftg[2];
+      pidx[pos] = (ii * fftg[1] + jj) * fftg[2] + kk;
+      path[3 * (long)pos + 0] = cv[0];
+      path[3 * (long)pos + 1] = cv[1];
+      path[3 * (long)pos + 2] = cv[2];
+    }
+    __syncthreads();
+    if (threadIdx.x == 255)
+      running += scan[255];
+    __syncthreads();
+  }
+}
+
+__device__ double d_fact(int n) {
+  double r = 1.0;
+  for (int i = 2; i <= n; i++)
+    r *= i;
+  return r;
+}
+
+// associated Legendre P_l^m(x), Condon-Shortley phase (utils.c:377-386)
+__device__ double d_legendre(int l, int m, double x) {
+  double sgn = 1.0, fac = 1.0;
+  if (m < 0) {
+    m = -m;
+    sgn = (m & 1) ? -1.0 : 1.0;
+    fac = d_fact(l - m) / d_fact(l + m);
+  }
+  if (m > l)
+    return 0.0;
+  double pmm = 1.0, s = sqrt((1.0 - x) * (1.0 + x)), odd = 1.0;
+  for (int i = 1; i <= m; i++) {
+    pmm *= -odd * s;
+    odd += 2.0;
+  }
+  double res;
+  if (l == m) {
+    res = pmm;
+  } else {
+    double pm1 = x * (2 * m + 1) * pmm;
+    if (l == m + 1) {
+      res = pm1;
+    } else {
+      double pl = 0.0;
+      for (int ll = m + 2; ll <= l; ll++) {
+        pl = (x * (2 * ll - 1) * pm1 - (ll + m - 1) * pmm) / (ll - m);
+        pmm = pm1;
+        pm1 = pl;
+      }
+      res = pl;
+    }
+  }
+  return sgn * fac * res;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_site_values(const SiteGeo *__restrict__ geo, Lat9 L, int f0, int f1, int f2,
+                                                     const int *__restrict__ site_pt, const long *__restrict__ tab_off,
+                                                     const int *__restrict__ pidx, const double *__restrict__ spl,
+                                                     const int *__restrict__ chan_l, cplx<T> *__restrict__ table) {
+  const int s_id = blockIdx.y;
+  const SiteGeo s = geo[s_id];
+  const int p0 = site_pt[s_id], np = site_pt[s_id + 1] - p0;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= np)
+    return;
+  const int fftg[3] = {f0, f1, f2};
+  int g = pidx[p0 + i];
+  int kk = g % fftg[2];
+  int jj = (g / fftg[2]) % fftg[1];
+  int ii = g / (fftg[1] * fftg[2]);
+  double pf[3] = {(double)ii / fftg[0], (double)jj / fftg[1], (double)kk / fftg[2]};
+  // minimum image of the wrapped point (utils.c:51-73, 566-576)
+  double best = INFINITY, v[3] = {0, 0, 0};
+  for (int a = -1; a <= 1; a++)
+    for (int b = -1; b <= 1; b++)
+      for (int c = -1; c <= 1; c++) {
+        double fv[3] = {pf[0] + a - s.coord[0], pf[1] + b - s.coord[1], pf[2] + c - s.coord[2]}, cv[3];
+        d_frac_to_cart(fv, L.m, cv);
+        double d = sqrt(cv[0] * cv[0] + cv[1] * cv[1] + cv[2] * cv[2]);
+        if (d < best) {
+          best = d;
+          v[0] = cv[0];
+          v[1] = cv[1];
+          v[2] = cv[2];
+        }
+      }
+  const double r = best;
+  double theta = 0.0, phi = 0.0;
+  if (r != 0.0) {
+    theta = acos(v[2] / r);
+    if (r - fabs(v[2]) != 0.0)
+      phi = acos(v[0] / sqrt(v[0] * v[0] + v[1] * v[1]));
+    if (v[1] < 0)
+      phi = 2.0 * M_PI - phi;
+  }
+  const double ct = cos(theta);
+  const int n = s.ngrid;
+  const double *x = spl + s.spl_off;
+  cplx<T> *out = table + tab_off[s_id] + (long)i * s.nproj;
+  int p = 0;
+  for (int ch = 0; ch < s.nchan; ch++) {
+    const int l = chan_l[s.chan_off + ch];
+    const double *f = x + (long)(1 + ch) * n;
+    const double *bb = x + (long)(1 + s.nchan + ch) * n;
+    const double *cc = x + (long)(1 + 2 * s.nchan + ch) * n;
+    const double *dd = x + (long)(1 + 3 * s.nchan + ch) * n;
+    // utils.c:461-475
+    double radial;
+    if (r > x[n - 1]) {
+      radial = 0.0;
+    } else if (r < x[0]) {
+      radial = f[0];
+    } else {
+      int id = (int)(r / s.rmax * n);
+      if (id > n - 2)
+        id = n - 2;
+      double t = r - x[id];
+      radial = f[id] + t * (bb[id] + t * (cc[id] + t * dd[id]));
+    }
+    for (int m = -l; m <= l; m++, p++) {
+      double norm = sqrt((2 * l + 1) / (4.0 * M_PI) * d_fact(l - m) / d_fact(l + m));
+      double pl = norm * d_legendre(l, m, ct) * radial;
+      double sn, cs;
+      sincos(m * phi, &sn, &cs);
+      // store conj(p) so the sphere sum is a plain product
+      out[p] = mk<T>((T)(pl * cs), (T)(-pl * sn));
+    }
+  }
+}
+
+// phase[pt] = dv / sqrt(V) * exp(i kc . path)
+template <typename T>
+__global__ __launch_bounds__(256) void k_phase(long npts, const double *__restrict__ path, double k0, double k1,
+                                               double k2, double scl, cplx<T> *__restrict__ phase) {
+  long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= npts)
+    return;
+  double kr = k0 * path[3 * i] + k1 * path[3 * i + 1] + k2 * path[3 * i + 2];
+  double sn, cs;
+  sincos(kr, &sn, &cs);
+  phase[i] = mk<T>((T)(scl * cs), (T)(scl * sn));
+}
+
+// One wavefront per (site, part); lane b accumulates the site's
+// projectors for band b of the chunk.
+template <typename T, int PMAX>
+__global__ __launch_bounds__(64) void k_sphere_sum(const cplx<T> *__restrict__ g, int NB,
+                                                   const int *__restrict__ site_pt, const int *__restrict__ site_np,
+                                                   const long *__restrict__ tab_off, const int *__restrict__ pidx,
+                                                   const cplx<T> *__restrict__ phase,
+                                                   const cplx<T> *__restrict__ table, int nparts,
+                                                   cplx<double> *__restrict__ partial) {
+  const int s = blockIdx.x / nparts, part = blockIdx.x % nparts;
+  const int lane = threadIdx.x;
+  const int p0 = site_pt[s], p1 = site_pt[s + 1];
+  const int np = site_np[s];
+  const int len = p1 - p0;
+  const int chunk = (len + nparts - 1) / nparts;
+  const int a = p0 + part * chunk;
+  const int e = min(p1, a + chunk);
+  cplx<T> acc[PMAX];
+#pragma unroll
+  for (int j = 0; j < PMAX; j++)
+    acc[j] = mk<T>(0, 0);
+  const bool active = lane < NB;
+  const cplx<T> *tb = table + tab_off[s] + (long)(a - p0) * np;
+  for (int i = a; i < e; i++, tb += np) {
+    cplx<T> x = active ? g[(long)pidx[i] * NB + lane] : mk<T>(0, 0);
+    x = x * phase[i];
+#pragma unroll
+    for (int j = 0; j < PMAX; j++)
+      if (j < np)
+        cmac(acc[j], tb[j], x);
+  }
+  if (active) {
+    cplx<double> *o = partial + ((long)blockIdx.x * NB + lane) * PMAX;
+#pragma unroll
+    for (int j = 0; j < PMAX; j++)
+      if (j < np)
+        o[j] = cvt<double>(acc[j]);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_sphere_reduce(const cplx<double> *__restrict__ partial, int NB, int nbc,
+                                                       int PMAX, int nparts, int nsites,
+                                                       const int *__restrict__ site_np,
+                                                       const int *__restrict__ proj_off, long ptot,
+                                                       cplx<double> *__restrict__ out) {
+  long t = (long)blockIdx.x * 256 + threadIdx.x;
+  long total = (long)nsites * nbc * PMAX;
+  if (t >= total)
+    return;
+  int j = (int)(t % PMAX);
+  long r = t / PMAX;
+  int b = (int)(r % nbc);
+  int s = (int)(r / nbc);
+  if (j >= site_np[s])
+    return;
+  cplx<double> acc = mk<double>(0, 0);
+  for (int q = 0; q < nparts; q++)
+    acc = acc + partial[(((long)s * nparts + q) * NB + b) * PMAX + j];
+  out[(long)b * ptot + proj_off[s] + j] = acc;
+}
+
+// ------------------------------------------------------------------ host
+static void cross(const double *a, const double *b, double *o) {
+  o[0] = a[1] * b[2] - a[2] * b[1];
+  o[1] = a[2] * b[0] - a[0] * b[2];
+  o[2] = a[0] * b[1] - a[1] * b[0];
+}
+static double nrm(const double *v) { return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+double det3(const double *m) {
+  return m[0] * m[4] * m[8] + m[1] * m[5] * m[6] + m[2] * m[3] * m[7] - m[2] * m[4] * m[6] -
+         m[1] * m[3] * m[8] - m[0] * m[5] * m[7];
+}
+
+// Host spline coefficients with the reference's boundary conditions
+// (utils.c:699-749): y'(x0) = secant slope, natural end.
+void spline_coeff_host(const double *x, const double *y, int n, double *b, double *c, double *d) {
+  std::vector<double> diag(n), up(n), rhs(n), M(n);
+  double h0 = x[1] - x[0];
+  diag[0] = 2.0 * h0;
+  up[0] = h0;
+  rhs[0] = 0.0;  // 6((y1-y0)/h0 - slope0) with slope0 the secant slope
+  for (int i = 1; i < n - 1; i++) {
+    double hl = x[i] - x[i - 1], hr = x[i + 1] - x[i];
+    double w = hl / diag[i - 1];
+    diag[i] = 2.0 * (hl + hr) - w * up[i - 1];
+    up[i] = hr;
+    rhs[i] = 6.0 * ((y[i + 1] - y[i]) / hr - (y[i] - y[i - 1]) / hl) - w * rhs[i - 1];
+  }
+  M[n - 1] = 0.0;
+  for (int i = n - 2; i >= 0; i--)
+    M[i] = (rhs[i] - up[i] * M[i + 1]) / diag[i];
+  for (int i = 0; i < n - 1; i++) {
+    double h = x[i + 1] - x[i];
+    b[i] = (y[i + 1] - y[i]) / h - h * (2.0 * M[i] + M[i + 1]) / 6.0;
+    c[i] = 0.5 * M[i];
+    d[i] = (M[i + 1] - M[i]) / (6.0 * h);
+  }
+  b[n - 1] = c[n - 1] = d[n - 1] = 0.0;
+}
+
+static int sphere_build(pp_sphere *sp) {
+  pp_ctx *ctx = sp->ctx;
+  const int ns = sp->nsites;
+  // per-site geometry
+  std::vector<SiteGeo> geo(ns);
+  double vol = det3(sp->lattice);
+  std::vector<long> spl_off(sp->nelems);
+  long so = 0;
+  for (int e = 0; e < sp->nelems; e++) {
+    spl_off[e] = so;
+    so += (long)(1 + 4 * sp->nchan[e]) * sp->ngrid[e];
+  }
+  sp->nproj_site.assign(ns, 0);
+  sp->proj_off.assign(ns + 1, 0);
+  sp->pmax = 0;
+  for (int s = 0; s < ns; s++) {
+    SiteGeo &g = geo[s];
+    int e = sp->labels[s];
+    for (int a = 0; a < 3; a++)
+      g.coord[a] = sp->frac[3 * s + a];
+    g.rmax = sp->rmax[e];
+    g.ngrid = sp->ngrid[e];
+    // utils.c:651-652 (radius from the site's own species)
+    g.R0 = (g.ngrid - 1) * g.rmax / g.ngrid;
+    double cr[3];
+    cross(sp->lattice + 3, sp->lattice + 6, cr);
+    g.g[0] = (int)(nrm(cr) * g.rmax / vol * sp->fftg[0]) + 1;
+    cross(sp->lattice + 0, sp->lattice + 6, cr);
+    g.g[1] = (int)(nrm(cr) * g.rmax / vol * sp->fftg[1]) + 1;
+    cross(sp->lattice + 0, sp->lattice + 3, cr);
+    g.g[2] = (int)(nrm(cr) * g.rmax / vol * sp->fftg[2]) + 1;
+    for (int a = 0; a < 3; a++)
+      g.c[a] = (int)round(g.coord[a] * sp->fftg[a]);
+    g.elem = e;
+    g.nchan = sp->nchan[e];
+    int np = 0;
+    for (int ch = 0; ch < g.nchan; ch++)
+      np += 2 * sp->ls[sp->ls_off[e] + ch] + 1;
+    g.nproj = np;
+    g.spl_off = spl_off[e];
+    g.chan_off = sp->ls_off[e];
+    sp->nproj_site[s] = np;
+    sp->proj_off[s + 1] = sp->proj_off[s] + np;
+    if (np > sp->pmax)
+      sp->pmax = np;
+  }
+  sp->ptot = sp->proj_off[ns];
+  PP_REQUIRE(sp->pmax <= 64, PP_ERR_UNSUPPORTED, "more than 64 projectors on one site");
+  Lat9 L;
+  memcpy(L.m, sp->lattice, sizeof(L.m));
+  PP_TRY(sp->d_sitemeta.ensure(sizeof(SiteGeo) * ns));
+  PP_CHECK_HIP(hipMemcpyAsync(sp->d_sitemeta.p, geo.data(), sizeof(SiteGeo) * ns, hipMemcpyHostToDevice, ctx->stream));
+  PP_TRY(sp->d_counts.ensure(sizeof(int) * ns));
+  hipEvent_t ev;
+  tbegin(ctx, K_TABLE, &ev);
+  hipLaunchKernelGGL(k_site_count, dim3(ns), dim3(256), 0, ctx->stream, (const SiteGeo *)sp->d_sitemeta.p, L,
+                     sp->fftg[0], sp->fftg[1], sp->fftg[2], (int *)sp->d_counts.p);
+  tend(ctx, K_TABLE, ev);
+  PP_CHECK_HIP(hipGetLastError());
+  std::vector<int> counts(ns);
+  PP_CHECK_HIP(hipMemcpyAsync(counts.data(), sp->d_counts.p, sizeof(int) * ns, hipMemcpyDeviceToHost, ctx->stream));
+  PP_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  sp->site_pt_h.assign(ns + 1, 0);
+  std::vector<long> tab_off(ns);
+  long to = 0;
+  int maxcount = 0;
+  for (int s = 0; s < ns; s++) {
+    sp->site_pt_h[s + 1] = sp->site_pt_h[s] + counts[s];
+    tab_off[s] = to;
+    to += (long)counts[s] * sp->nproj_site[s];
+    if (counts[s] > maxcount)
+      maxcount = counts[s];
+  }
+  sp->npts = sp->site_pt_h[ns];
+  size_t csz = sp->precision == PP_FP64 ? sizeof(cplx<double>) : sizeof(cplx<float>);
+  PP_TRY(sp->d_site_pt.ensure(sizeof(int) * (ns + 1)));
+  PP_TRY(sp->d_site_np.ensure(sizeof(int) * ns));
+  PP_TRY(sp->d_tab_off.ensure(sizeof(long) * ns));
+  PP_TRY(sp->d_proj_off.ensure(sizeof(int) * (ns + 1)));
+  PP_TRY(sp->d_pidx.ensure(sizeof(int) * std::max<long>(1, sp->npts)));
+  PP_TRY(sp->d_path.ensure(sizeof(double) * 3 * std::max<long>(1, sp->npts)));
+  PP_TRY(sp->d_table.ensure(csz * std::max<long>(1, to)));
+  PP_CHECK_HIP(hipMemcpyAsync(sp->d_site_pt.p, sp->site_pt_h.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice,
+                              ctx->stream));
+  PP_CHECK_HIP(hipMemcpyAsync(sp->d_site_np.p, sp->nproj_site.data(), sizeof(int) * ns, hipMemcpyHostToDevice,
+                              ctx->stream));
+  PP_CHECK_HIP(hipMemcpyAsync(sp->d_tab_off.p, tab_off.data(), sizeof(long) * ns, hipMemcpyHostToDevice, ctx->stream));
+  PP_CHECK_HIP(hipMemcpyAsync(sp->d_proj_off.p, sp->proj_off.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice,
+                              ctx->stream));
+  tbegin(ctx, K_TABLE, &ev);
+  hipLaunchKernelGGL(k_site_fill, dim3(ns), dim3(256), 0, ctx->stream, (const SiteGeo *)sp->d_sitemeta.p, L,
+                     sp->fftg[0], sp->fftg[1], sp->fftg[2], (const int *)sp->d_site_pt.p, (int *)sp->d_pidx.p,
+                     (double *)sp->d_path.p);
+  tend(ctx, K_TABLE, ev);
+  PP_CHECK_HIP(hipGetLastError());
+  if (maxcount > 0) {
+    dim3 grd((maxcount + 255) / 256, ns);
+    tbegin(ctx, K_TABLE, &ev);
+    if (sp->precision == PP_FP64)
+      hipLaunchKernelGGL(k_site_values<double>, grd, dim3(256), 0, ctx->stream, (const SiteGeo *)sp->d_sitemeta.p,
+                         L, sp->fftg[0], sp->fftg[1], sp->fftg[2], (const int *)sp->d_site_pt.p,
+                         (const long *)sp->d_tab_off.p, (const int *)sp->d_pidx.p, (const double *)sp->d_spline.p,
+                         (const int *)sp->d_chan.p, (cplx<double> *)sp->d_table.p);
+    else
+      hipLaunchKernelGGL(k_site_values<float>, grd, dim3(256), 0, ctx->stream, (const SiteGeo *)sp->d_sitemeta.p, L,
+                         sp->fftg[0], sp->fftg[1], sp->fftg[2], (const int *)sp->d_site_pt.p,
+                         (const long *)sp->d_tab_off.p, (const int *)sp->d_pidx.p, (const double *)sp->d_spline.p,
+                         (const int *)sp->d_chan.p, (cplx<float> *)sp->d_table.p);
+    tend(ctx, K_TABLE, ev);
+    PP_CHECK_HIP(hipGetLastError());
+  }
+  return PP_OK;
+}
+
+template <typename T, int PMAX>
+static void launch_sum(pp_ctx *ctx, pp_sphere *sp, const cplx<T> *grid, int NB, const cplx<T> *phase, int nparts,
+                       cplx<double> *partial) {
+  hipLaunchKernelGGL((k_sphere_sum<T, PMAX>), dim3(sp->nsites * nparts), dim3(64), 0, ctx->stream, grid, NB,
+                     (const int *)sp->d_site_pt.p, (const int *)sp->d_site_np.p, (const long *)sp->d_tab_off.p,
+                     (const int *)sp->d_pidx.p, phase, (const cplx<T> *)sp->d_table.p, nparts, partial);
+}
+
+template <typename T>
+int sphere_sums(pp_ctx *ctx, pp_sphere *sp, const cplx<T> *grid, int NB, int nbc, const cplx<T> *phase,
+                cplx<double> *out, long ptot) {
+  if (sp->nsites == 0 || nbc == 0)
+    return PP_OK;
+  int nparts = ctx->tune_sphere_parts;
+  if (nparts <= 0) {
+    // aim for ~4 waves per SIMD across 256 CUs
+    nparts = (4096 + sp->nsites - 1) / sp->nsites;
+    long avg = sp->npts / std::max(1, sp->nsites);
+    int maxp = (int)std::max<long>(1, avg / 64);
+    if (nparts > maxp)
+      nparts = maxp;
+    if (nparts < 1)
+      nparts = 1;
+  }
+  int PM = sp->pmax <= 8 ? 8 : sp->pmax <= 18 ? 18 : sp->pmax <= 32 ? 32 : 64;
+  PP_TRY(ctx->partial.ensure(sizeof(cplx<double>) * (size_t)sp->nsites * nparts * NB * PM));
+  cplx<double> *partial = (cplx<double> *)ctx->partial.p;
+  hipEvent_t ev;
+  tbegin(ctx, K_SPHERE_SUM, &ev);
+  switch (PM) {
+  case 8:
+    launch_sum<T, 8>(ctx, sp, grid, NB, phase, nparts, partial);
+    break;
+  case 18:
+    launch_sum<T, 18>(ctx, sp, grid, NB, phase, nparts, partial);
+    break;
+  case 32:
+    launch_sum<T, 32>(ctx, sp, grid, NB, phase, nparts, partial);
+    break;
+  default:
+    launch_sum<T, 64>(ctx, sp, grid, NB, phase, nparts, partial);
+  }
+  tend(ctx, K_SPHERE_SUM, ev);
+  PP_CHECK_HIP(hipGetLastError());
+  long total = (long)sp->nsites * nbc * PM;
+  tbegin(ctx, K_SPHERE_REDUCE, &ev);
+  hipLaunchKernelGGL(k_sphere_reduce, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, ctx->stream, partial, NB,
+                     nbc, PM, nparts, sp->nsites, (const int *)sp->d_site_np.p, (const int *)sp->d_proj_off.p, ptot,
+                     out);
+  tend(ctx, K_SPHERE_REDUCE, ev);
+  PP_CHECK_HIP(hipGetLastError());
+  return PP_OK;
+}
+
+template <typename T>
+int make_phase(pp_ctx *ctx, pp_sphere *sp, const double *kfrac, cplx<T> *phase) {
+  if (sp->npts == 0)
+    return PP_OK;
+  // reciprocal lattice with 2 pi, rows b_i (utils.h:131)
+  double rec[9], v = det3(sp->lattice);
+  cross(sp->lattice + 3, sp->lattice + 6, rec + 0);
+  cross(sp->lattice + 6, sp->lattice + 0, rec + 3);
+  cross(sp->lattice + 0, sp->lattice + 3, rec + 6);
+  for (int i = 0; i < 9; i++)
+    rec[i] *= 2.0 * M_PI / v;
+  double kc[3];
+  for (int j = 0; j < 3; j++)
+    kc[j] = kfrac[0] * rec[j] + kfrac[1] * rec[3 + j] + kfrac[2] * rec[6 + j];
+  double dv = v / sp->fftg[0] / sp->fftg[1] / sp->fftg[2];
+  double scl = dv / sqrt(v);
+  hipEvent_t ev;
+  tbegin(ctx, K_PHASE, &ev);
+  hipLaunchKernelGGL(k_phase<T>, dim3((unsigned)((sp->npts + 255) / 256)), dim3(256), 0, ctx->stream, sp->npts,
+                     (const double *)sp->d_path.p, kc[0], kc[1], kc[2], scl, phase);
+  tend(ctx, K_PHASE, ev);
+  PP_CHECK_HIP(hipGetLastError());
+  return PP_OK;
+}
+
+template int sphere_sums<double>(pp_ctx *, pp_sphere *, const cplx<double> *, int, int, const cplx<double> *,
+                                 cplx<double> *, long);
+template int sphere_sums<float>(pp_ctx *, pp_sphere *, const cplx<float> *, int, int, const cplx<float> *,
+                                cplx<double> *, long);
+template int make_phase<double>(pp_ctx *, pp_sphere *, const double *, cplx<double> *);
+template int make_phase<float>(pp_ctx *, pp_sphere *, const double *, cplx<float> *);
+
+long sphere_npts(pp_sphere *sp) { return sp->npts; }
+int sphere_ptot(pp_sphere *sp) { return sp->ptot; }
+int sphere_precision(pp_sphere *sp) { return sp->precision; }
+const int *sphere_fftg(pp_sphere *sp) { return sp->fftg; }
+const double *sphere_lattice(pp_sphere *sp) { return sp->lattice; }
+
+}  // namespace pp
+
+using namespace pp;
+
+extern "C" int pp_sphere_create(pp_ctx *ctx, pp_sphere **out, const double *lattice, const int *fftg,
+                                int num_sites, const int *labels, const double *frac_coords, int num_elems,
+                                const int *nchan, const int *ls, const double *rmax, const int *ngrid,
+                                const double *proj, int precision) {
+  PP_REQUIRE(ctx && out && lattice && fftg && num_sites >= 0 && num_elems > 0, PP_ERR_ARG,
+             "pp_sphere_create: bad arguments");
+  PP_REQUIRE(precision == PP_FP64 || precision == PP_FP32 || precision == PP_BF16X3, PP_ERR_ARG,
+             "pp_sphere_create: unknown precision");
+  PP_CHECK_HIP(hipSetDevice(ctx->device));
+  pp_sphere *sp = new pp_sphere();
+  sp->ctx = ctx;
+  sp->precision = precision == PP_FP64 ? PP_FP64 : PP_FP32;
+  memcpy(sp->lattice, lattice, sizeof(sp->lattice));
+  memcpy(sp->fftg, fftg, sizeof(sp->fftg));
+  sp->nsites = num_sites;
+  sp->nelems = num_elems;
+  sp->labels.assign(labels, labels + num_sites);
+  sp->frac.assign(frac_coords, frac_coords + 3 * num_sites);
+  sp->nchan.assign(nchan, nchan + num_elems);
+  sp->ngrid.assign(ngrid, ngrid + num_elems);
+  sp->rmax.assign(rmax, rmax + num_elems);
+  for (int s = 0; s < num_sites; s++)
+    if (labels[s] < 0 || labels[s] >= num_elems) {
+      delete sp;
+      PP_REQUIRE(false, PP_ERR_ARG, "pp_sphere_create: site label out of range");
+    }
+  // species spline blocks: x, f[nchan], b[nchan], c[nchan], d[nchan]
+  std::vector<double> spl;
+  int lo = 0;
+  long po = 0;
+  for (int e = 0; e < num_elems; e++) {
+    sp->ls_off.push_back(lo);
+    int n = ngrid[e];
+    if (n < 3) {
+      delete sp;
+      PP_REQUIRE(false, PP_ERR_ARG, "pp_sphere_create: projector grid needs >= 3 points");
+    }
+    std::vector<double> x(n);
+    for (int j = 0; j < n; j++)
+      x[j] = rmax[e] / n * j;  // projector.c:56
+    size_t base = spl.size();
+    spl.resize(base + (size_t)(1 + 4 * nchan[e]) * n);
+    memcpy(&spl[base], x.data(), sizeof(double) * n);
+    for (int ch = 0; ch < nchan[e]; ch++) {
+      sp->ls.push_back(ls[lo + ch]);
+      if (ls[lo + ch] < 0 || ls[lo + ch] > 3) {
+        delete sp;
+        PP_REQUIRE(false, PP_ERR_UNSUPPORTED, "pp_sphere_create: l must be 0..3");
+      }
+      const double *f = proj + po + (long)ch * n;
+      double *F = &spl[base + (size_t)(1 + ch) * n];
+      double *B = &spl[base + (size_t)(1 + nchan[e] + ch) * n];
+      double *C = &spl[base + (size_t)(1 + 2 * nchan[e] + ch) * n];
+      double *D = &spl[base + (size_t)(1 + 3 * nchan[e] + ch) * n];
+      memcpy(F, f, sizeof(double) * n);
+      spline_coeff_host(x.data(), f, n, B, C, D);
+    }
+    lo += nchan[e];
+    po += (long)nchan[e] * n;
+  }
+  int st = sp->d_spline.ensure(sizeof(double) * spl.size());
+  if (st == PP_OK)
+    st = sp->d_chan.ensure(sizeof(int) * std::max<size_t>(1, sp->ls.size()));
+  if (st != PP_OK) {
+    delete sp;
+    return st;
+  }
+  PP_CHECK_HIP(hipMemcpy(sp->d_spline.p, spl.data(), sizeof(double) * spl.size(), hipMemcpyHostToDevice));
+  PP_CHECK_HIP(hipMemcpy(sp->d_chan.p, sp->ls.data(), sizeof(int) * sp->ls.size(), hipMemcpyHostToDevice));
+  st = sphere_build(sp);
+  if (st != PP_OK) {
+    delete sp;
+    return st;
+  }
+  *out = sp;
+  return PP_OK;
+}
+
+extern "C" int pp_sphere_rebuild(pp_sphere *sp, const double *frac_coords) {
+  PP_REQUIRE(sp && frac_coords, PP_ERR_ARG, "pp_sphere_rebuild: bad arguments");
+  sp->frac.assign(frac_coords, frac_coords + 3 * sp->nsites);
+  return sphere_build(sp);
+}
+
+extern "C" void pp_sphere_destroy(pp_sphere *sp) { delete sp; }
+
+extern "C" int pp_sphere_info(pp_sphere *sp, int *total_projs, long *total_points, int *proj_off) {
+  PP_REQUIRE(sp, PP_ERR_ARG, "pp_sphere_info: null plan");
+  if (total_projs)
+    *total_projs = sp->ptot;
+  if (total_points)
+    *total_points = sp->npts;
+  if (proj_off)
+    memcpy(proj_off, sp->proj_off.data(), sizeof(int) * (sp->nsites + 1));
+  return PP_OK;
+}

@@ -1,0 +1,243 @@
+"""GPU parity of the HIP kernels against the CPU oracle, through the C-ABI.
+
+Tolerances (written per test): FP64 path <= 1e-9 relative (normwise,
+max|d| / max|ref|) against the FP64 oracle -- in practice ~1e-14; FP32
+path <= 1e-5 for FFT / projections and <= 1e-6 for the overlap GEMM;
+3xBF16 overlap <= 2e-5.
+"""
+
+import numpy as np
+import pytest
+
+from helpers import device_inputs, normwise, oracle_projections
+from oracle import pyoracle as orc
+from pawpyseed_amd import synthetic
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from pawpyseed_amd.core.batched import Context
+    c = Context(0)
+    yield c
+    c.close()
+
+
+def _fft_case(ctx, wl, nb, precision):
+    from pawpyseed_amd.core import batched as B
+    G = wl.ensure_gvecs()[0]
+    C = wl.coefficients("R", nband=nb)[0]
+    Cd, Gd, gofs, nw = device_inputs([G], [C])
+    out = torch.zeros((nb,) + tuple(int(n) for n in wl.fftg), dtype=torch.complex128, device="cuda")
+    B.fft3d(ctx, wl.lattice, wl.fftg, Gd, Cd[0], out, precision)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    errs = []
+    for b in range(nb):
+        want = orc.fft3d(wl.lattice, G, C[b], wl.fftg)
+        errs.append(normwise(got[b], want))
+    return max(errs)
+
+
+@pytest.mark.parametrize("fftg,encut", [((20, 20, 20), 150.0), ((24, 18, 30), 120.0), ((28, 14, 21), 60.0),
+                                        ((60, 60, 60), 400.0), ((26, 22, 33), 90.0)])
+def test_fft3d_fp64(ctx, fftg, encut):
+    wl = synthetic.tiny_config(encut=encut, fftg=fftg)
+    print("fft3d fp64", fftg, flush=True)
+    assert _fft_case(ctx, wl, 5, "fp64") < 1e-12
+
+
+def test_fft3d_fp32(ctx):
+    wl = synthetic.tiny_config(encut=400.0, fftg=(60, 60, 60))
+    err = _fft_case(ctx, wl, 3, "fp32")
+    print("fft3d fp32 normwise error", err)
+    assert err < 1e-5
+
+
+def test_fft3d_reference_wavecar(ctx, wavecar):
+    """tests.c:28-80 fft_check on test_files/WAVECAR (k0, band 0, 20^3)."""
+    from pawpyseed_amd.core import batched as B
+    G = wavecar["G_0"]
+    C = wavecar["C_0"]
+    Cd, Gd, gofs, nw = device_inputs([G], [C])
+    out = torch.zeros((C.shape[0], 20, 20, 20), dtype=torch.complex128, device="cuda")
+    B.fft3d(ctx, wavecar["lattice"], wavecar["grid_fftg"], Gd, Cd[0], out, "fp64")
+    got = out.cpu().numpy()
+    want = wavecar["grid_k0_b0"]
+    assert np.abs(got[0] - want).max() < 1e-10 * np.abs(want).max()
+    for b in range(1, C.shape[0]):
+        assert normwise(got[b], orc.fft3d(wavecar["lattice"], G, C[b], wavecar["grid_fftg"])) < 1e-12
+
+
+def _proj_case(ctx, wl, frac, nb, precision, labels=None):
+    from pawpyseed_amd.core import batched as B
+    Gs = wl.ensure_gvecs()
+    Cs = wl.coefficients("S", nband=nb)
+    plan = B.SpherePlan(ctx, wl.lattice, wl.fftg, frac, np.zeros(len(frac), np.int32) if labels is None else labels,
+                        [wl.species], precision)
+    Cd, Gd, gofs, nw = device_inputs(Gs, Cs)
+    out = torch.zeros((len(Gs), nb, plan.ptot), dtype=torch.complex128, device="cuda")
+    B.setup_projections(ctx, plan, wl.kpts, gofs, Gd, Cd, out, precision)
+    torch.cuda.synchronize()
+    want, T = oracle_projections(wl, frac, Cs)
+    assert plan.npts == len(T.idx)
+    assert np.array_equal(plan.proj_off, T.proj_off)
+    got = out.cpu().numpy()
+    plan.close()
+    return got, want
+
+
+def test_projections_fp64_tiny_two_k(ctx):
+    wl = synthetic.tiny_config(encut=150.0, fftg=(24, 24, 24), nband=8, kpts=[[0, 0, 0], [0.25, -0.25, 0.1]],
+                               ls=(0, 1, 2), alphas=(1.5, 3.0, 1.5))
+    got, want = _proj_case(ctx, wl, wl.frac_S, 8, "fp64")
+    err = normwise(got, want)
+    print("projections fp64 tiny", err)
+    assert err < 1e-9
+
+
+def test_projections_fp64_ragged_band_chunk(ctx):
+    """nband not a multiple of the 64-band chunk, and > 64."""
+    wl = synthetic.tiny_config(encut=100.0, fftg=(20, 20, 20), nband=70, kpts=[[0.1, 0.2, 0.3]], ls=(0, 1),
+                               alphas=(1.5, 3.0))
+    got, want = _proj_case(ctx, wl, wl.frac_S, 70, "fp64")
+    assert normwise(got, want) < 1e-9
+
+
+def test_projections_fp64_cfg0(ctx):
+    """64-site cell of the 400 eV / 60^3 workload, Gamma, 16 bands."""
+    wl = synthetic.config0()
+    got, want = _proj_case(ctx, wl, wl.frac_S, 16, "fp64")
+    err = normwise(got, want)
+    print("projections fp64 cfg0", err)
+    assert err < 1e-9
+
+
+def test_projections_nonorthogonal_cell(ctx, wavecar):
+    """Hexagonal Ga4 cell of test_files (CONTCAR positions), its WAVECAR
+    coefficients, synthetic l=0..2 projectors."""
+    frac = np.array([[0.1560427174376784, 0.8439572825631387, 0.9166540580595424],
+                     [0.3439572825623216, 0.6560427174368613, 0.4166540580595424],
+                     [0.8439572825623216, 0.1560427174368613, 0.0833459419404576],
+                     [0.6560427174376784, 0.3439572825631387, 0.5833459419404576]])
+    sp = synthetic.make_species([0, 1, 2], [1.5, 2.0, 2.5], 1.2)
+    wl = synthetic.Workload("ga4", wavecar["lattice"], frac, frac, sp, np.array([20, 20, 20], np.int32),
+                            float(wavecar["encut"]), wavecar["kpts"], np.array([0.5, 0.5]), 6, np.eye(3),
+                            np.arange(4), np.arange(4), gvecs=[wavecar["G_0"], wavecar["G_1"]])
+    from pawpyseed_amd.core import batched as B
+    Cs = [wavecar["C_0"], wavecar["C_1"]]
+    plan = B.SpherePlan(ctx, wl.lattice, wl.fftg, frac, np.zeros(4, np.int32), [sp], "fp64")
+    Cd, Gd, gofs, nw = device_inputs(wl.gvecs, Cs)
+    out = torch.zeros((2, 6, plan.ptot), dtype=torch.complex128, device="cuda")
+    B.setup_projections(ctx, plan, wl.kpts, gofs, Gd, Cd, out, "fp64")
+    want, T = oracle_projections(wl, frac, Cs)
+    assert normwise(out.cpu().numpy(), want) < 1e-9
+
+
+def test_projections_fp32(ctx):
+    wl = synthetic.config0()
+    got, want = _proj_case(ctx, wl, wl.frac_S, 8, "fp32")
+    err = normwise(got, want)
+    print("projections fp32 cfg0 normwise error", err)
+    assert err < 1e-5
+
+
+def _overlap_case(ctx, precision, nbR=70, nbS=45, nspin=1, flip=False):
+    from pawpyseed_amd.core import batched as B
+    kp = [[0, 0, 0], [0.25, 0.25, 0.0]]
+    wl = synthetic.tiny_config(encut=250.0, nband=nbR, kpts=kp)
+    rng = np.random.default_rng(7)
+    CR = [c for c in wl.coefficients("R", rng=rng, nband=nbR)] * nspin
+    CS = [c for c in wl.coefficients("S", rng=rng, nband=nbS)] * nspin
+    if nspin == 2:
+        CR = CR[:2] + [c * np.exp(0.3j) for c in CR[2:]]
+    nwk = len(kp)
+    CRd, _, _, nw = device_inputs(wl.ensure_gvecs() * nspin, CR)
+    CSd, _, _, _ = device_inputs(wl.ensure_gvecs() * nspin, CS)
+    out = torch.zeros((nwk * nspin, nbR, nbS), dtype=torch.complex128, device="cuda")
+    B.overlap_gemm(ctx, nwk, nspin, flip, nw, CRd, CSd, out, precision)
+    got = out.cpu().numpy()
+    nkt = nwk * nspin
+    want = np.zeros_like(got)
+    for band in range(nbS):
+        res = orc.pseudoprojection(CR, CS, band, nwk, nspin, flip=flip)
+        want[:, :, band] = res.reshape(nbR, nkt).T
+    return got, want
+
+
+def test_overlap_fp64(ctx):
+    got, want = _overlap_case(ctx, "fp64")
+    assert np.abs(got - want).max() < 1e-12
+
+
+def test_overlap_fp64_spin_flip(ctx):
+    got, want = _overlap_case(ctx, "fp64", nbR=33, nbS=20, nspin=2, flip=True)
+    assert np.abs(got - want).max() < 1e-12
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32", 1e-6), ("bf16x3", 2e-5)])
+def test_overlap_mixed(ctx, precision, tol):
+    got, want = _overlap_case(ctx, precision, nbR=130, nbS=67)
+    err = normwise(got, want)
+    print("overlap", precision, "normwise error", err)
+    assert err < tol
+
+
+def test_augmentation_matches_oracle(ctx):
+    from pawpyseed_amd.core import batched as B
+    rng = np.random.default_rng(3)
+    sp = synthetic.make_species([0, 0, 1, 1, 2], [1.5, 3.0, 1.5, 3.0, 2.0], 1.2)
+    P = sp.total_projs
+    nsR, nsS, nwk, nspin, nbR, nbS = 6, 5, 2, 2, 40, 9
+    nkt = nwk * nspin
+    pR = rng.standard_normal((nkt, nbR, nsR * P)) + 1j * rng.standard_normal((nkt, nbR, nsR * P))
+    pS = rng.standard_normal((nkt, nbS, nsS * P)) + 1j * rng.standard_normal((nkt, nbS, nsS * P))
+    dO = synthetic.channel_delta_o(rng, sp.ls)
+    M_R = np.array([0, 2, 3, 5], np.int32)
+    M_S = np.array([1, 0, 4, 2], np.int32)
+    offR = np.arange(nsR + 1, dtype=np.int32) * P
+    offS = np.arange(nsS + 1, dtype=np.int32) * P
+    for flip in (False, True):
+        out = torch.zeros((nkt, nbR, nbS), dtype=torch.complex128, device="cuda")
+        B.augmentation(ctx, nwk, nspin, flip, torch.from_numpy(pR).cuda(), offR, np.zeros(nsR, np.int32),
+                       torch.from_numpy(pS).cuda(), offS, np.zeros(nsS, np.int32), M_R, M_S, [sp], [dO], out)
+        got = out.cpu().numpy()
+
+        class T:
+            pass
+
+        TR, TS = T(), T()
+        TR.ptot, TR.proj_off = nsR * P, offR
+        TS.ptot, TS.proj_off = nsS * P, offS
+        for band in range(nbS):
+            want = orc.compensation_terms_m(np.zeros(nbR * nkt, complex), band, nwk, nspin, pR, TR,
+                                            np.zeros(nsR, np.int32), pS[:, band], TS, M_R, M_S, [dO], [sp.ls],
+                                            flip=flip)
+            np.testing.assert_allclose(got[:, :, band], want.reshape(nbR, nkt).T, atol=1e-11)
+
+
+def test_empty_and_invalid_inputs(ctx):
+    from pawpyseed_amd.core import batched as B
+    from pawpyseed_amd.core._lib import PAWpyError
+    wl = synthetic.tiny_config()
+    plan = B.SpherePlan(ctx, wl.lattice, wl.fftg, wl.frac_S, np.zeros(len(wl.frac_S), np.int32), [wl.species])
+    Cd = torch.zeros((1, 0, 64), dtype=torch.complex64, device="cuda")
+    out = torch.zeros((1, 0, plan.ptot), dtype=torch.complex128, device="cuda")
+    Gd = torch.zeros((1, 3), dtype=torch.int32, device="cuda")
+    B.setup_projections(ctx, plan, wl.kpts, np.array([0, 0], np.int32), Gd, Cd, out)  # nband = 0: no-op
+    with pytest.raises(PAWpyError):
+        B.SpherePlan(ctx, wl.lattice, wl.fftg, wl.frac_S, np.full(len(wl.frac_S), 3, np.int32), [wl.species])
+    with pytest.raises(PAWpyError):  # table precision differs from the call's
+        C1 = torch.zeros((1, 1, 64), dtype=torch.complex64, device="cuda")
+        o1 = torch.zeros((1, 1, plan.ptot), dtype=torch.complex128, device="cuda")
+        B.setup_projections(ctx, plan, wl.kpts, np.array([0, 1], np.int32), Gd, C1, o1, "fp32")
+    # zero plane waves: overlap is zero
+    o = torch.ones((1, 3, 2), dtype=torch.complex128, device="cuda")
+    B.overlap_gemm(ctx, 1, 1, False, [0], torch.zeros((1, 3, 64), dtype=torch.complex64, device="cuda"),
+                   torch.zeros((1, 2, 64), dtype=torch.complex64, device="cuda"), o)
+    assert torch.count_nonzero(o).item() == 0
+    plan.close()
