@@ -86,6 +86,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle checks")
     ap.add_argument("--paths", default="fp32,fp64", help="comma list of fp32, fp64 setup/overlap paths")
+    ap.add_argument("--profile-step", action="store_true",
+                    help="for rocprofv3 --pmc: a 1 GiB device copy (counter calibration) and one fp32 setup step")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -142,6 +144,19 @@ def main():
             B.augmentation(ctx, nk, 1, False, projR[p], pR.proj_off, labels_R, projS[p], pS.proj_off, labels_S,
                            wl.M_R, wl.M_S, [wl.species], [wl.delta_o], Om[p], sync=False)
             ev[3].record(stream)
+
+    if args.profile_step:
+        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        torch.cuda.synchronize()
+        b.copy_(a)  # calibration: reads and writes exactly 1 GiB
+        torch.cuda.synchronize()
+        evs = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        step(paths[0], evs)
+        torch.cuda.synchronize()
+        log("profile step done")
+        return
 
     results = {}
     for p in paths:

@@ -149,10 +149,15 @@ int pp_ctx_set_timing(pp_ctx *ctx, int enable);
 int pp_ctx_timing_read(pp_ctx *ctx, int n, const char **names,
                        double *ms_total, long *launches);
 int pp_ctx_timing_reset(pp_ctx *ctx);
-/* Tuning knobs (0 = automatic): bands per FFT chunk, sphere-sum parts,
- * GEMM split-K factor. */
-int pp_ctx_set_tuning(pp_ctx *ctx, int fft_bands, int sphere_parts,
-                      int gemm_splitk);
+/* Tuning knobs (0 = automatic): bands per FFT chunk (<= 64), bands per
+ * FFT work-group tile (power of two <= 32), FFT pruning to the plane-wave
+ * sphere (1 on, 2 off), sphere-sum parts per site, GEMM split-K factor. */
+int pp_ctx_set_tuning(pp_ctx *ctx, int fft_bands, int fft_tile, int fft_prune,
+                      int sphere_parts, int gemm_splitk);
+/* One tuning knob by name: "fft_bands", "fft_tile", "fft_prune",
+ * "fft_pingpong" (1 = two-buffer LDS FFT stages instead of in-place),
+ * "sphere_parts", "gemm_splitk". */
+int pp_ctx_set_param(pp_ctx *ctx, const char *name, int value);
 
 /* Sphere plan of one structure (replaces the table build of
  * projector.c:193-209 / utils.c:590-696, done on the GPU).

@@ -73,7 +73,8 @@ SIGNATURES = {
     "pp_ctx_timing_read": (ct.c_int, [c_void_p, ct.c_int, ct.POINTER(ct.c_char_p), c_double_p,
                                       ct.POINTER(ct.c_long)]),
     "pp_ctx_timing_reset": (ct.c_int, [c_void_p]),
-    "pp_ctx_set_tuning": (ct.c_int, [c_void_p, ct.c_int, ct.c_int, ct.c_int]),
+    "pp_ctx_set_tuning": (ct.c_int, [c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_int]),
+    "pp_ctx_set_param": (ct.c_int, [c_void_p, ct.c_char_p, ct.c_int]),
     "pp_sphere_create": (ct.c_int, [c_void_p, ct.POINTER(c_void_p), c_double_p, c_int_p, ct.c_int, c_int_p,
                                     c_double_p, ct.c_int, c_int_p, c_int_p, c_double_p, c_int_p, c_double_p,
                                     ct.c_int]),

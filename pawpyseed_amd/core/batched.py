@@ -68,8 +68,12 @@ class Context:
         k = lib().pp_ctx_timing_read(self.h, n, names, ptr(ms, c_double_p), cnt.ctypes.data_as(ct.POINTER(ct.c_long)))
         return {names[i].decode(): (float(ms[i]), int(cnt[i])) for i in range(k) if cnt[i] > 0}
 
-    def set_tuning(self, fft_bands=0, sphere_parts=0, gemm_splitk=0):
-        check(lib().pp_ctx_set_tuning(self.h, fft_bands, sphere_parts, gemm_splitk), "pp_ctx_set_tuning")
+    def set_tuning(self, fft_bands=0, fft_tile=0, fft_prune=0, sphere_parts=0, gemm_splitk=0):
+        check(lib().pp_ctx_set_tuning(self.h, fft_bands, fft_tile, fft_prune, sphere_parts, gemm_splitk),
+              "pp_ctx_set_tuning")
+
+    def set_param(self, name: str, value: int):
+        check(lib().pp_ctx_set_param(self.h, name.encode(), int(value)), "pp_ctx_set_param")
 
     def close(self):
         if getattr(self, "h", None):

@@ -125,13 +125,29 @@ static int setup_batched(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac
   cplx<T> *grid = (cplx<T> *)ctx->grid.p;
   cplx<T> *phase = (cplx<T> *)ctx->phase.p;
   int *gidx = (int *)ctx->gidx.p;
+  // plane-wave sphere bounds of every k (one host sync for the alias flags)
+  const int nbnd = 2 * fftg[0] * fftg[1] + 2 * fftg[0] + 3;
+  PP_TRY(ctx->bounds.ensure(sizeof(int) * (size_t)nbnd * nk));
+  int *bnd = (int *)ctx->bounds.p;
+  for (int k = 0; k < nk; k++)
+    PP_TRY(sphere_bounds(ctx, fftg, gofs[k + 1] - gofs[k], G_dev + 3 * (long)gofs[k], bnd + (size_t)k * nbnd));
+  std::vector<int> alias(nk, 1);
+  for (int k = 0; k < nk; k++)
+    PP_CHECK_HIP(hipMemcpyAsync(&alias[k], bnd + (size_t)k * nbnd + nbnd - 1, sizeof(int), hipMemcpyDeviceToHost,
+                                ctx->stream));
+  PP_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   for (int k = 0; k < nk; k++) {
     int nw = gofs[k + 1] - gofs[k];
     PP_TRY(compute_gidx(ctx, fftg, nw, G_dev + 3 * (long)gofs[k], gidx));
     PP_TRY(make_phase<T>(ctx, sp, kfrac + 3 * k, phase));
+    const int *kb = bnd + (size_t)k * nbnd;
+    SphereBounds bd = {kb, kb + 2 * fftg[0] * fftg[1], kb + 2 * fftg[0] * fftg[1] + 2 * fftg[0], fftg[0], fftg[1],
+                       fftg[2]};
+    const bool prune = alias[k] == 0 && ctx->tune_fft_prune != 2;
     for (int b0 = 0; b0 < nband; b0 += NB) {
       int nbc = std::min(NB, nband - b0);
-      PP_TRY(fft_chunk<T>(ctx, pl, grid, NB, nw, gidx, C + ((long)k * nband + b0) * ldc, ldc, nbc));
+      PP_TRY(fft_chunk<T>(ctx, pl, grid, NB, nw, gidx, C + ((long)k * nband + b0) * ldc, ldc, nbc,
+                          prune ? &bd : nullptr));
       PP_TRY(sphere_sums<T>(ctx, sp, grid, NB, nbc, phase, proj + ((long)k * nband + b0) * ptot, ptot));
     }
   }
@@ -227,12 +243,44 @@ int pp_ctx_timing_read(pp_ctx *ctx, int n, const char **names, double *ms_total,
   return K_NCLASS;
 }
 
-int pp_ctx_set_tuning(pp_ctx *ctx, int fft_bands, int sphere_parts, int gemm_splitk) {
+int pp_ctx_set_tuning(pp_ctx *ctx, int fft_bands, int fft_tile, int fft_prune, int sphere_parts, int gemm_splitk) {
   PP_REQUIRE(ctx, PP_ERR_ARG, "pp_ctx_set_tuning: null ctx");
   PP_REQUIRE(fft_bands >= 0 && fft_bands <= 64, PP_ERR_ARG, "fft_bands must be 0..64");
+  PP_REQUIRE(fft_tile == 0 || fft_tile == 1 || fft_tile == 2 || fft_tile == 4 || fft_tile == 8 || fft_tile == 16 ||
+                 fft_tile == 32,
+             PP_ERR_ARG, "fft_tile must be 0 or a power of two <= 32");
+  PP_REQUIRE(fft_prune >= 0 && fft_prune <= 2, PP_ERR_ARG, "fft_prune must be 0 (auto), 1 (on) or 2 (off)");
+  PP_REQUIRE(sphere_parts >= 0 && gemm_splitk >= 0, PP_ERR_ARG, "negative tuning value");
   ctx->tune_fft_bands = fft_bands;
+  ctx->tune_fft_tile = fft_tile;
+  ctx->tune_fft_prune = fft_prune;
   ctx->tune_sphere_parts = sphere_parts;
   ctx->tune_splitk = gemm_splitk;
+  return PP_OK;
+}
+
+int pp_ctx_set_param(pp_ctx *ctx, const char *name, int value) {
+  PP_REQUIRE(ctx && name, PP_ERR_ARG, "pp_ctx_set_param: null argument");
+  std::string n(name);
+  if (n == "fft_pingpong")
+    ctx->tune_fft_pingpong = value != 0;
+  else if (n == "fft_tile")
+    return pp_ctx_set_tuning(ctx, ctx->tune_fft_bands, value, ctx->tune_fft_prune, ctx->tune_sphere_parts,
+                             ctx->tune_splitk);
+  else if (n == "fft_prune")
+    return pp_ctx_set_tuning(ctx, ctx->tune_fft_bands, ctx->tune_fft_tile, value, ctx->tune_sphere_parts,
+                             ctx->tune_splitk);
+  else if (n == "fft_bands")
+    return pp_ctx_set_tuning(ctx, value, ctx->tune_fft_tile, ctx->tune_fft_prune, ctx->tune_sphere_parts,
+                             ctx->tune_splitk);
+  else if (n == "sphere_parts")
+    return pp_ctx_set_tuning(ctx, ctx->tune_fft_bands, ctx->tune_fft_tile, ctx->tune_fft_prune, value,
+                             ctx->tune_splitk);
+  else if (n == "gemm_splitk")
+    return pp_ctx_set_tuning(ctx, ctx->tune_fft_bands, ctx->tune_fft_tile, ctx->tune_fft_prune,
+                             ctx->tune_sphere_parts, value);
+  else
+    PP_REQUIRE(false, PP_ERR_ARG, "pp_ctx_set_param: unknown parameter " + n);
   return PP_OK;
 }
 

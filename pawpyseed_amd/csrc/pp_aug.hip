@@ -172,10 +172,10 @@ extern "C" int pp_fft3d_batched(pp_ctx *ctx, const double *lattice, const int *f
   PP_TRY(compute_gidx(ctx, fftg, nw, G_dev, (int *)ctx->gidx.p));
   if (f64)
     PP_TRY(fft_chunk<double>(ctx, pl, (cplx<double> *)ctx->grid.p, NB, nw, (const int *)ctx->gidx.p,
-                             (const cplx<float> *)C_dev, ldc, nband));
+                             (const cplx<float> *)C_dev, ldc, nband, nullptr));
   else
     PP_TRY(fft_chunk<float>(ctx, pl, (cplx<float> *)ctx->grid.p, NB, nw, (const int *)ctx->gidx.p,
-                            (const cplx<float> *)C_dev, ldc, nband));
+                            (const cplx<float> *)C_dev, ldc, nband, nullptr));
   return grid_to_bandmajor(ctx, f64 ? PP_FP64 : PP_FP32, ctx->grid.p, NB, nband, npts, 1.0 / sqrt(det3(lattice)),
                            grid_out_dev);
 }

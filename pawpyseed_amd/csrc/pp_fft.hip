@@ -229,13 +229,25 @@ __device__ void stage_generic(const cplx<T> *__restrict__ in, cplx<T> *__restric
   }
 }
 
+// Signed frequency of wrapped index i on an axis of length n.
+__device__ __forceinline__ int sgn_idx(int i, int n) { return i < n - n / 2 ? i : i - n; }
+
 // One pass of 1D transforms along an axis.  Work-group: one line (fixed
 // other two coordinates) x NBW bands.  Element e of band slot b lives at
 // g[base + e*es + b].
-template <typename T, int NBW>
+//
+// With bounds (SphereBounds, from the G vectors of the k-point) the pass
+// only touches the part of the grid the plane-wave sphere can reach: the
+// sphere is convex, so on every z-line its coefficients fill one signed
+// interval [zlo, zhi]; the populated z-lines of a plane x lie in
+// [ylo(x), yhi(x)] and the populated planes in [xlo, xhi].  Elements
+// outside those intervals are zero by construction and are not loaded, and
+// lines whose input is entirely outside are skipped (z and y passes), so no
+// zero fill of the grid is needed.
+template <typename T, int NBW, int AXIS>
 __global__ __launch_bounds__(256) void k_fft_pass(cplx<T> *__restrict__ g, AxisPlan pl,
                                                   const cplx<T> *__restrict__ tw, long es, long s1, long s2,
-                                                  int n2, int nslices) {
+                                                  int n2, int nslices, SphereBounds bd) {
   extern __shared__ unsigned char smem_raw[];
   cplx<T> *buf0 = reinterpret_cast<cplx<T> *>(smem_raw);
   const int n = pl.n;
@@ -243,11 +255,32 @@ __global__ __launch_bounds__(256) void k_fft_pass(cplx<T> *__restrict__ g, AxisP
   const int blk = blockIdx.x;
   const int slice = blk % nslices;
   const int line = blk / nslices;
-  const long base = (long)(line / n2) * s1 + (long)(line % n2) * s2 + (long)slice * NBW;
+  const int l1 = line / n2, l2 = line % n2;
+  int lo = -(1 << 30), hi = 1 << 30;
+  if (bd.xb) {
+    if (AXIS == 2) {  // line (x, y)
+      const int sx = sgn_idx(l1, bd.n0), sy = sgn_idx(l2, bd.n1);
+      if (sx < bd.xb[0] || sx > bd.xb[1] || sy < bd.yb[2 * l1] || sy > bd.yb[2 * l1 + 1])
+        return;
+      lo = bd.zb[2 * (l1 * bd.n1 + l2)];
+      hi = bd.zb[2 * (l1 * bd.n1 + l2) + 1];
+    } else if (AXIS == 1) {  // line (x, z)
+      const int sx = sgn_idx(l1, bd.n0);
+      if (sx < bd.xb[0] || sx > bd.xb[1])
+        return;
+      lo = bd.yb[2 * l1];
+      hi = bd.yb[2 * l1 + 1];
+    } else {  // line (y, z)
+      lo = bd.xb[0];
+      hi = bd.xb[1];
+    }
+  }
+  const long base = (long)l1 * s1 + (long)l2 * s2 + (long)slice * NBW;
   const int tot = n * NBW;
   for (int t = threadIdx.x; t < tot; t += 256) {
     int e = t / NBW, b = t % NBW;
-    buf0[t] = g[base + (long)e * es + b];
+    const int se = sgn_idx(e, n);
+    buf0[t] = (se >= lo && se <= hi) ? g[base + (long)e * es + b] : mk<T>(0, 0);
   }
   __syncthreads();
   cplx<T> *in = buf0, *out = buf1;
@@ -279,6 +312,129 @@ __global__ __launch_bounds__(256) void k_fft_pass(cplx<T> *__restrict__ g, AxisP
     int e = t / NBW, b = t % NBW;
     g[base + (long)e * es + b] = in[t];
   }
+}
+
+// In-place Stockham stage: every thread keeps the R inputs of up to MAXB
+// butterflies in registers across one barrier, so a single LDS buffer
+// serves as input and output (half the LDS of the ping-pong form, twice
+// the work-groups per CU).
+template <typename T, int NBW, int R, int MAXB>
+__device__ __forceinline__ void stage_ip(cplx<T> *__restrict__ buf, const cplx<T> *__restrict__ tws, int p,
+                                         int Tn) {
+  cplx<T> v[MAXB][R];
+  const int tot = Tn * NBW;
+#pragma unroll
+  for (int q = 0; q < MAXB; q++) {
+    const int t = threadIdx.x + q * 256;
+    if (t < tot) {
+      const int j = t / NBW, b = t % NBW;
+      const int k = j % p;
+      v[q][0] = buf[j * NBW + b];
+#pragma unroll
+      for (int r = 1; r < R; r++)
+        v[q][r] = buf[(j + r * Tn) * NBW + b] * tws[k * (R - 1) + (r - 1)];
+      small_dft<T>(R, v[q]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < MAXB; q++) {
+    const int t = threadIdx.x + q * 256;
+    if (t < tot) {
+      const int j = t / NBW, b = t % NBW;
+      const int k = j % p;
+      const int o = (j / p) * p * R + k;
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        buf[(o + r * p) * NBW + b] = v[q][r];
+    }
+  }
+  __syncthreads();
+}
+
+// As k_fft_pass with in-place stages; radices 2, 3, 4, 5 only, and at most
+// MAXB butterflies per thread per stage.
+template <typename T, int NBW, int AXIS, int MAXB>
+__global__ __launch_bounds__(256) void k_fft_pass_ip(cplx<T> *__restrict__ g, AxisPlan pl,
+                                                     const cplx<T> *__restrict__ tw, long es, long s1, long s2,
+                                                     int n2, int nslices, SphereBounds bd) {
+  extern __shared__ unsigned char smem_raw[];
+  cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
+  const int n = pl.n;
+  const int blk = blockIdx.x;
+  const int slice = blk % nslices;
+  const int line = blk / nslices;
+  const int l1 = line / n2, l2 = line % n2;
+  int lo = -(1 << 30), hi = 1 << 30;
+  if (bd.xb) {
+    if (AXIS == 2) {
+      const int sx = sgn_idx(l1, bd.n0), sy = sgn_idx(l2, bd.n1);
+      if (sx < bd.xb[0] || sx > bd.xb[1] || sy < bd.yb[2 * l1] || sy > bd.yb[2 * l1 + 1])
+        return;
+      lo = bd.zb[2 * (l1 * bd.n1 + l2)];
+      hi = bd.zb[2 * (l1 * bd.n1 + l2) + 1];
+    } else if (AXIS == 1) {
+      const int sx = sgn_idx(l1, bd.n0);
+      if (sx < bd.xb[0] || sx > bd.xb[1])
+        return;
+      lo = bd.yb[2 * l1];
+      hi = bd.yb[2 * l1 + 1];
+    } else {
+      lo = bd.xb[0];
+      hi = bd.xb[1];
+    }
+  }
+  const long base = (long)l1 * s1 + (long)l2 * s2 + (long)slice * NBW;
+  const int tot = n * NBW;
+  for (int t = threadIdx.x; t < tot; t += 256) {
+    int e = t / NBW, b = t % NBW;
+    const int se = sgn_idx(e, n);
+    buf[t] = (se >= lo && se <= hi) ? g[base + (long)e * es + b] : mk<T>(0, 0);
+  }
+  __syncthreads();
+  for (int s = 0; s < pl.nst; s++) {
+    const int R = pl.radix[s], p = pl.pprev[s], Tn = n / R;
+    const cplx<T> *tws = tw + pl.twoff[s];
+    if (R == 4)
+      stage_ip<T, NBW, 4, MAXB>(buf, tws, p, Tn);
+    else if (R == 3)
+      stage_ip<T, NBW, 3, MAXB>(buf, tws, p, Tn);
+    else if (R == 2)
+      stage_ip<T, NBW, 2, MAXB>(buf, tws, p, Tn);
+    else
+      stage_ip<T, NBW, 5, MAXB>(buf, tws, p, Tn);
+  }
+  for (int t = threadIdx.x; t < tot; t += 256) {
+    int e = t / NBW, b = t % NBW;
+    g[base + (long)e * es + b] = buf[t];
+  }
+}
+
+// Bounds of the plane-wave sphere of one k-point in signed grid indices.
+// bnd layout: zb[2*n0*n1] | yb[2*n0] | xb[2] | alias flag.
+__global__ __launch_bounds__(256) void k_bounds_init(int *__restrict__ bnd, int n) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n)
+    bnd[i] = (i == n - 1) ? 0 : ((i & 1) ? -(1 << 30) : (1 << 30));
+}
+
+__global__ __launch_bounds__(256) void k_bounds(int n0, int n1, int n2, int nw, const int *__restrict__ G,
+                                                int *__restrict__ bnd) {
+  int w = blockIdx.x * 256 + threadIdx.x;
+  if (w >= nw)
+    return;
+  int *zb = bnd, *yb = bnd + 2 * n0 * n1, *xb = yb + 2 * n0, *flag = xb + 2;
+  const int g0 = G[3 * w], g1 = G[3 * w + 1], g2 = G[3 * w + 2];
+  const int x = ((g0 % n0) + n0) % n0, y = ((g1 % n1) + n1) % n1, z = ((g2 % n2) + n2) % n2;
+  const int sx = sgn_idx(x, n0), sy = sgn_idx(y, n1), sz = sgn_idx(z, n2);
+  if (sx != g0 || sy != g1 || sz != g2)
+    atomicOr(flag, 1);  // the grid aliases this G: no pruning
+  atomicMin(&zb[2 * (x * n1 + y)], sz);
+  atomicMax(&zb[2 * (x * n1 + y) + 1], sz);
+  atomicMin(&yb[2 * x], sy);
+  atomicMax(&yb[2 * x + 1], sy);
+  atomicMin(&xb[0], sx);
+  atomicMax(&xb[1], sx);
 }
 
 template <typename T>
@@ -320,43 +476,72 @@ int compute_gidx(pp_ctx *ctx, const int *fftg, int nw, const int *G_dev, int *gi
   return PP_OK;
 }
 
-template <typename T, int NBW>
+template <typename T, int NBW, int AXIS>
 static int launch_pass(pp_ctx *ctx, cplx<T> *grid, const AxisPlan &ap, const cplx<T> *tw, long es, long s1,
-                       long s2, int n1, int n2, int NB, int cls) {
+                       long s2, int n1, int n2, int NB, int cls, const SphereBounds &bd) {
   int nslices = NB / NBW;
-  size_t lds = (size_t)2 * ap.n * NBW * sizeof(cplx<T>);
   long nblk = (long)n1 * n2 * nslices;
+  // in-place form when every radix is 2..5 and each thread holds <= 8 butterflies
+  int maxb = 0;
+  bool ip = ctx->tune_fft_pingpong == 0 && NBW >= 8;
+  for (int s = 0; s < ap.nst; s++) {
+    if (ap.radix[s] > 5)
+      ip = false;
+    maxb = std::max(maxb, ((ap.n / ap.radix[s]) * NBW + 255) / 256);
+  }
+  if (maxb > 8)
+    ip = false;
   hipEvent_t ev;
   tbegin(ctx, cls, &ev);
-  hipLaunchKernelGGL((k_fft_pass<T, NBW>), dim3((unsigned)nblk), dim3(256), lds, ctx->stream, grid, ap, tw, es,
-                     s1, s2, n2, nslices);
+  bool launched = false;
+  if constexpr (NBW >= 8) {
+    if (ip) {
+      size_t lds = (size_t)ap.n * NBW * sizeof(cplx<T>);
+#define PP_IP(MB)                                                                                              \
+  hipLaunchKernelGGL((k_fft_pass_ip<T, NBW, AXIS, MB>), dim3((unsigned)nblk), dim3(256), lds, ctx->stream, grid, ap, \
+                     tw, es, s1, s2, n2, nslices, bd)
+      if (maxb <= 2)
+        PP_IP(2);
+      else if (maxb <= 4)
+        PP_IP(4);
+      else
+        PP_IP(8);
+#undef PP_IP
+      launched = true;
+    }
+  }
+  if (!launched) {
+    size_t lds = (size_t)2 * ap.n * NBW * sizeof(cplx<T>);
+    hipLaunchKernelGGL((k_fft_pass<T, NBW, AXIS>), dim3((unsigned)nblk), dim3(256), lds, ctx->stream, grid, ap, tw,
+                       es, s1, s2, n2, nslices, bd);
+  }
   tend(ctx, cls, ev);
   PP_CHECK_HIP(hipGetLastError());
   return PP_OK;
 }
 
-template <typename T>
+template <typename T, int AXIS>
 static int run_pass(pp_ctx *ctx, cplx<T> *grid, const AxisPlan &ap, const cplx<T> *tw, long es, long s1, long s2,
-                    int n1, int n2, int NB, int cls) {
+                    int n1, int n2, int NB, int cls, const SphereBounds &bd) {
   const size_t ldsmax = 64 * 1024;
-  int nbw = 32;
+  int nbw = ctx->tune_fft_tile > 0 ? ctx->tune_fft_tile : (sizeof(T) == 4 ? 16 : 8);
   while (nbw > 1 && (NB % nbw != 0 || (size_t)2 * ap.n * nbw * sizeof(cplx<T>) > ldsmax))
     nbw >>= 1;
   PP_REQUIRE((size_t)2 * ap.n * nbw * sizeof(cplx<T>) <= ldsmax, PP_ERR_UNSUPPORTED,
              "FFT line too long for LDS: " + std::to_string(ap.n));
   switch (nbw) {
   case 32:
-    return launch_pass<T, 32>(ctx, grid, ap, tw, es, s1, s2, n1, n2, NB, cls);
+    return launch_pass<T, 32, AXIS>(ctx, grid, ap, tw, es, s1, s2, n1, n2, NB, cls, bd);
   case 16:
-    return launch_pass<T, 16>(ctx, grid, ap, tw, es, s1, s2, n1, n2, NB, cls);
+    return launch_pass<T, 16, AXIS>(ctx, grid, ap, tw, es, s1, s2, n1, n2, NB, cls, bd);
   case 8:
-    return launch_pass<T, 8>(ctx, grid, ap, tw, es, s1, s2, n1, n2, NB, cls);
+    return launch_pass<T, 8, AXIS>(ctx, grid, ap, tw, es, s1, s2, n1, n2, NB, cls, bd);
   case 4:
-    return launch_pass<T, 4>(ctx, grid, ap, tw, es, s1, s2, n1, n2, NB, cls);
+    return launch_pass<T, 4, AXIS>(ctx, grid, ap, tw, es, s1, s2, n1, n2, NB, cls, bd);
   case 2:
-    return launch_pass<T, 2>(ctx, grid, ap, tw, es, s1, s2, n1, n2, NB, cls);
+    return launch_pass<T, 2, AXIS>(ctx, grid, ap, tw, es, s1, s2, n1, n2, NB, cls, bd);
   default:
-    return launch_pass<T, 1>(ctx, grid, ap, tw, es, s1, s2, n1, n2, NB, cls);
+    return launch_pass<T, 1, AXIS>(ctx, grid, ap, tw, es, s1, s2, n1, n2, NB, cls, bd);
   }
 }
 
@@ -370,10 +555,14 @@ template <> const cplx<float> *tw_of<float>(FftPlan *pl, int axis) {
 
 template <typename T>
 int fft_chunk(pp_ctx *ctx, FftPlan *pl, cplx<T> *grid, int NB, int nw, const int *gidx_dev,
-              const cplx<float> *C_dev, long ldc, int nbc) {
+              const cplx<float> *C_dev, long ldc, int nbc, const SphereBounds *bounds) {
   const long n0 = pl->fftg[0], n1 = pl->fftg[1], n2 = pl->fftg[2];
   const long npts = n0 * n1 * n2;
-  PP_TRY(zero_async(ctx, grid, (size_t)npts * NB * sizeof(cplx<T>)));
+  SphereBounds bd = {nullptr, nullptr, nullptr, (int)n0, (int)n1, (int)n2};
+  if (bounds)
+    bd = *bounds;
+  else
+    PP_TRY(zero_async(ctx, grid, (size_t)npts * NB * sizeof(cplx<T>)));
   if (nw > 0) {
     hipEvent_t ev;
     tbegin(ctx, K_SCATTER, &ev);
@@ -383,19 +572,29 @@ int fft_chunk(pp_ctx *ctx, FftPlan *pl, cplx<T> *grid, int NB, int nw, const int
     PP_CHECK_HIP(hipGetLastError());
   }
   // z lines: (x, y); y lines: (x, z); x lines: (y, z)
-  PP_TRY(run_pass<T>(ctx, grid, pl->ax[2], tw_of<T>(pl, 2), NB, n1 * n2 * NB, n2 * NB, (int)n0, (int)n1, NB,
-                     K_FFT_Z));
-  PP_TRY(run_pass<T>(ctx, grid, pl->ax[1], tw_of<T>(pl, 1), n2 * NB, n1 * n2 * NB, NB, (int)n0, (int)n2, NB,
-                     K_FFT_Y));
-  PP_TRY(run_pass<T>(ctx, grid, pl->ax[0], tw_of<T>(pl, 0), n1 * n2 * NB, n2 * NB, NB, (int)n1, (int)n2, NB,
-                     K_FFT_X));
+  PP_TRY((run_pass<T, 2>(ctx, grid, pl->ax[2], tw_of<T>(pl, 2), NB, n1 * n2 * NB, n2 * NB, (int)n0, (int)n1, NB,
+                         K_FFT_Z, bd)));
+  PP_TRY((run_pass<T, 1>(ctx, grid, pl->ax[1], tw_of<T>(pl, 1), n2 * NB, n1 * n2 * NB, NB, (int)n0, (int)n2, NB,
+                         K_FFT_Y, bd)));
+  PP_TRY((run_pass<T, 0>(ctx, grid, pl->ax[0], tw_of<T>(pl, 0), n1 * n2 * NB, n2 * NB, NB, (int)n1, (int)n2, NB,
+                         K_FFT_X, bd)));
+  return PP_OK;
+}
+
+int sphere_bounds(pp_ctx *ctx, const int *fftg, int nw, const int *G_dev, int *bnd_dev) {
+  const int n = 2 * fftg[0] * fftg[1] + 2 * fftg[0] + 2 + 1;
+  hipLaunchKernelGGL(k_bounds_init, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, bnd_dev, n);
+  if (nw > 0)
+    hipLaunchKernelGGL(k_bounds, dim3((nw + 255) / 256), dim3(256), 0, ctx->stream, fftg[0], fftg[1], fftg[2], nw,
+                       G_dev, bnd_dev);
+  PP_CHECK_HIP(hipGetLastError());
   return PP_OK;
 }
 
 template int fft_chunk<double>(pp_ctx *, FftPlan *, cplx<double> *, int, int, const int *, const cplx<float> *,
-                               long, int);
+                               long, int, const SphereBounds *);
 template int fft_chunk<float>(pp_ctx *, FftPlan *, cplx<float> *, int, int, const int *, const cplx<float> *, long,
-                              int);
+                              int, const SphereBounds *);
 
 int grid_to_bandmajor(pp_ctx *ctx, int precision, const void *grid, int NB, int nbc, long npts, double scl,
                       void *out) {

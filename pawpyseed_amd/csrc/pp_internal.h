@@ -45,6 +45,13 @@ struct AxisPlan {
   int twoff[12];
 };
 
+// Plane-wave sphere bounds of one k-point in signed grid indices (device
+// pointers; xb == nullptr means "no pruning").
+struct SphereBounds {
+  const int *zb, *yb, *xb;
+  int n0, n1, n2;
+};
+
 struct FftPlan {
   int fftg[3] = {0, 0, 0};
   AxisPlan ax[3];
@@ -66,13 +73,15 @@ struct pp_ctx {
   int device = 0;
   hipStream_t own = nullptr;
   hipStream_t stream = nullptr;
-  int tune_fft_bands = 0, tune_sphere_parts = 0, tune_splitk = 0;
+  int tune_fft_bands = 0, tune_sphere_parts = 0, tune_splitk = 0, tune_fft_tile = 0, tune_fft_prune = 0;
+  int tune_fft_pingpong = 0;
   pp::DevBuf grid;      // band-fastest FFT chunk
   pp::DevBuf partial;   // sphere-sum partials
   pp::DevBuf gidx;      // wrapped grid index per G
   pp::DevBuf phase;     // per-point Bloch phase of the current k
   pp::DevBuf gemm_ws;   // split-K partials
   pp::DevBuf aug_ws;    // gathered / dO-applied projections
+  pp::DevBuf bounds;    // plane-wave sphere bounds per k
   std::map<long long, pp::FftPlan *> fft_plans;
   pp::Timer timer;
 };
@@ -91,7 +100,9 @@ int zero_async(pp_ctx *ctx, void *p, size_t bytes);
 int compute_gidx(pp_ctx *ctx, const int *fftg, int nw, const int *G_dev, int *gidx_dev);
 template <typename T>
 int fft_chunk(pp_ctx *ctx, FftPlan *pl, cplx<T> *grid, int NB, int nw, const int *gidx_dev,
-              const cplx<float> *C_dev, long ldc, int nbc);
+              const cplx<float> *C_dev, long ldc, int nbc, const SphereBounds *bounds);
+// bnd_dev: 2*n0*n1 + 2*n0 + 3 ints; the last is set when the grid aliases a G.
+int sphere_bounds(pp_ctx *ctx, const int *fftg, int nw, const int *G_dev, int *bnd_dev);
 int grid_to_bandmajor(pp_ctx *ctx, int precision, const void *grid, int NB, int nbc, long npts,
                       double scl, void *out);
 double det3(const double *m);

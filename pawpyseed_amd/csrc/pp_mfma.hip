@@ -7,8 +7,8 @@
 // so no operand repacking is needed for the real part and B'' is the
 // stored pair swapped with one sign flip.
 //
-// PP_FP32: v_mfma_f32_16x16x4_f32, FP32 accumulation over at most 512
-// complex terms, then FP64 (per thread, then across split-K chunks).  PP_BF16X3: each FP32 operand is split into
+// PP_FP32: v_mfma_f32_16x16x4_f32, FP32 accumulation over split-K chunks
+// of at most 512 complex terms, chunks summed in FP64.  PP_BF16X3: each FP32 operand is split into
 // hi = bf16(x) and lo = bf16(x - hi); hi*hi + hi*lo + lo*hi on
 // v_mfma_f32_16x16x32_bf16 with FP32 accumulation.
 //
@@ -73,7 +73,7 @@ __device__ __forceinline__ void load8(const float *__restrict__ P, long ld, int 
   }
 }
 
-template <bool VEC>
+template <bool VEC, bool FOLD>
 __global__ __launch_bounds__(256) void k_cgemm_f32(int M, int N, int kchunk, int K, const float *__restrict__ A,
                                                    long lda, const float *__restrict__ B, long ldb,
                                                    cplx<double> *__restrict__ C, long ldc, long sSplit) {
@@ -134,7 +134,7 @@ __global__ __launch_bounds__(256) void k_cgemm_f32(int M, int N, int kchunk, int
     }
     __syncthreads();
     // bound the FP32 accumulation length: fold into FP64 every 512 terms
-    if (++since_flush == 32) {
+    if (FOLD && ++since_flush == 32) {
       since_flush = 0;
 #pragma unroll
       for (int a = 0; a < 2; a++)
@@ -272,16 +272,14 @@ int gemm_fp32_mfma(pp_ctx *ctx, int precision, int M, int N, int K, const void *
                    cplx<double> *C, long ldc, bool accumulate) {
   const int kstep = precision == PP_BF16X3 ? 32 : 16;
   int tiles = ((M + 63) / 64) * ((N + 63) / 64);
+  // FP32 accumulation runs over one split-K chunk of at most 512 complex
+  // terms; chunks are summed in FP64 by the reduction kernel.
   int nsplit = ctx->tune_splitk;
   if (nsplit <= 0) {
-    nsplit = (2048 + tiles - 1) / tiles;
-    // FP32 accumulation: keep chunks at most ~8k complex terms
-    int minsplit = (K + 8191) / 8192;
+    nsplit = std::max((2048 + tiles - 1) / tiles, (K + 511) / 512);
     int maxs = std::max(1, K / 256);
     if (nsplit > maxs)
       nsplit = maxs;
-    if (nsplit < minsplit)
-      nsplit = minsplit;
   }
   if (nsplit < 1)
     nsplit = 1;
@@ -304,12 +302,20 @@ int gemm_fp32_mfma(pp_ctx *ctx, int precision, int M, int N, int K, const void *
       hipLaunchKernelGGL(k_cgemm_bf16x3<false>, grd, dim3(256), 0, ctx->stream, M, N, kchunk, K, (const float *)A,
                          lda, (const float *)B, ldb, P, (long)N, (long)M * N);
   } else {
-    if (vec)
-      hipLaunchKernelGGL(k_cgemm_f32<true>, grd, dim3(256), 0, ctx->stream, M, N, kchunk, K, (const float *)A, lda,
-                         (const float *)B, ldb, P, (long)N, (long)M * N);
+    // FP64 folds inside the tile only when a caller forces long chunks
+    const bool fold = ctx->tune_splitk > 0 && (K + ctx->tune_splitk - 1) / ctx->tune_splitk > 1024;
+    if (vec && fold)
+      hipLaunchKernelGGL((k_cgemm_f32<true, true>), grd, dim3(256), 0, ctx->stream, M, N, kchunk, K,
+                         (const float *)A, lda, (const float *)B, ldb, P, (long)N, (long)M * N);
+    else if (vec)
+      hipLaunchKernelGGL((k_cgemm_f32<true, false>), grd, dim3(256), 0, ctx->stream, M, N, kchunk, K,
+                         (const float *)A, lda, (const float *)B, ldb, P, (long)N, (long)M * N);
+    else if (fold)
+      hipLaunchKernelGGL((k_cgemm_f32<false, true>), grd, dim3(256), 0, ctx->stream, M, N, kchunk, K,
+                         (const float *)A, lda, (const float *)B, ldb, P, (long)N, (long)M * N);
     else
-      hipLaunchKernelGGL(k_cgemm_f32<false>, grd, dim3(256), 0, ctx->stream, M, N, kchunk, K, (const float *)A, lda,
-                         (const float *)B, ldb, P, (long)N, (long)M * N);
+      hipLaunchKernelGGL((k_cgemm_f32<false, false>), grd, dim3(256), 0, ctx->stream, M, N, kchunk, K,
+                         (const float *)A, lda, (const float *)B, ldb, P, (long)N, (long)M * N);
   }
   tend(ctx, K_GEMM, ev);
   PP_CHECK_HIP(hipGetLastError());

@@ -11,7 +11,7 @@
 //
 // Sphere sums replace projector.c:223-274 (onto_projector_helper):
 // <p_j|psi> = sum_i conj(p_j(i)) psi(idx_i) dv exp(i k.path_i).  One
-// wavefront per (site, part of its points); lane = band of the chunk, so
+// work-group per (site, part of its points); lane = band of the chunk, so
 // each point's NB bands are one contiguous load from the band-fastest
 // grid while the projector values and phase are uniform across the wave.
 #include <cmath>
@@ -288,38 +288,96 @@ __global__ __launch_bounds__(256) void k_phase(long npts, const double *__restri
   phase[i] = mk<T>((T)(scl * cs), (T)(scl * sn));
 }
 
-// One wavefront per (site, part); lane b accumulates the site's
-// projectors for band b of the chunk.
+// One work-group (4 wavefronts) per (site, part of its points); lane b
+// of every wavefront accumulates the site's projectors for band b of the
+// chunk.  Tiles of TP points stage their projector values, phases and grid
+// indices in LDS with coalesced loads; each wavefront then takes every
+// fourth point of the tile, four points per step with their grid loads in
+// flight together, and reads the projector values as LDS broadcasts.  The
+// four wavefronts' sums are added in LDS at the end.
 template <typename T, int PMAX>
-__global__ __launch_bounds__(64) void k_sphere_sum(const cplx<T> *__restrict__ g, int NB,
-                                                   const int *__restrict__ site_pt, const int *__restrict__ site_np,
-                                                   const long *__restrict__ tab_off, const int *__restrict__ pidx,
-                                                   const cplx<T> *__restrict__ phase,
-                                                   const cplx<T> *__restrict__ table, int nparts,
-                                                   cplx<double> *__restrict__ partial) {
+__global__ __launch_bounds__(256) void k_sphere_sum(const cplx<T> *__restrict__ g, int NB,
+                                                    const int *__restrict__ site_pt, const int *__restrict__ site_np,
+                                                    const long *__restrict__ tab_off, const int *__restrict__ pidx,
+                                                    const cplx<T> *__restrict__ phase,
+                                                    const cplx<T> *__restrict__ table, int nparts,
+                                                    cplx<double> *__restrict__ partial) {
+  constexpr int TP = 64;
+  __shared__ cplx<T> s_tab[TP * PMAX];
+  __shared__ cplx<T> s_ph[TP];
+  __shared__ int s_idx[TP];
   const int s = blockIdx.x / nparts, part = blockIdx.x % nparts;
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int p0 = site_pt[s], p1 = site_pt[s + 1];
   const int np = site_np[s];
   const int len = p1 - p0;
   const int chunk = (len + nparts - 1) / nparts;
   const int a = p0 + part * chunk;
   const int e = min(p1, a + chunk);
+  const bool active = lane < NB;
   cplx<T> acc[PMAX];
 #pragma unroll
   for (int j = 0; j < PMAX; j++)
     acc[j] = mk<T>(0, 0);
-  const bool active = lane < NB;
-  const cplx<T> *tb = table + tab_off[s] + (long)(a - p0) * np;
-  for (int i = a; i < e; i++, tb += np) {
-    cplx<T> x = active ? g[(long)pidx[i] * NB + lane] : mk<T>(0, 0);
-    x = x * phase[i];
+  const cplx<T> *tsite = table + tab_off[s];
+  for (int t0 = a; t0 < e; t0 += TP) {
+    const int n = min(TP, e - t0);
+    const cplx<T> *tsrc = tsite + (long)(t0 - p0) * np;
+    for (int q = tid; q < n * np; q += 256)
+      s_tab[q] = tsrc[q];
+    if (tid < n) {
+      s_idx[tid] = pidx[t0 + tid];
+      s_ph[tid] = phase[t0 + tid];
+    }
+    __syncthreads();
+    int i = wave;
+    for (; i + 12 < n; i += 16) {
+      cplx<T> x[4];
 #pragma unroll
-    for (int j = 0; j < PMAX; j++)
-      if (j < np)
-        cmac(acc[j], tb[j], x);
+      for (int u = 0; u < 4; u++)
+        x[u] = active ? g[(long)s_idx[i + 4 * u] * NB + lane] : mk<T>(0, 0);
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const cplx<T> xv = x[u] * s_ph[i + 4 * u];
+        const cplx<T> *tp = s_tab + (i + 4 * u) * np;
+#pragma unroll
+        for (int j = 0; j < PMAX; j++)
+          if (j < np)
+            cmac(acc[j], tp[j], xv);
+      }
+    }
+    for (; i < n; i += 4) {
+      cplx<T> xv = active ? g[(long)s_idx[i] * NB + lane] : mk<T>(0, 0);
+      xv = xv * s_ph[i];
+      const cplx<T> *tp = s_tab + i * np;
+#pragma unroll
+      for (int j = 0; j < PMAX; j++)
+        if (j < np)
+          cmac(acc[j], tp[j], xv);
+    }
+    __syncthreads();
   }
-  if (active) {
+  // add the four wavefronts' sums (reuse the table tile as scratch)
+  cplx<T> *red = s_tab;
+  for (int w = 1; w < 4; w++) {
+    for (int j0 = 0; j0 < PMAX; j0 += TP * PMAX / 64) {
+      if (wave == w) {
+#pragma unroll
+        for (int j = 0; j < PMAX; j++)
+          if (j >= j0 && j < j0 + TP * PMAX / 64)
+            red[(j - j0) * 64 + lane] = acc[j];
+      }
+      __syncthreads();
+      if (wave == 0) {
+#pragma unroll
+        for (int j = 0; j < PMAX; j++)
+          if (j >= j0 && j < j0 + TP * PMAX / 64)
+            acc[j] = acc[j] + red[(j - j0) * 64 + lane];
+      }
+      __syncthreads();
+    }
+  }
+  if (wave == 0 && active) {
     cplx<double> *o = partial + ((long)blockIdx.x * NB + lane) * PMAX;
 #pragma unroll
     for (int j = 0; j < PMAX; j++)
@@ -505,7 +563,7 @@ static int sphere_build(pp_sphere *sp) {
 template <typename T, int PMAX>
 static void launch_sum(pp_ctx *ctx, pp_sphere *sp, const cplx<T> *grid, int NB, const cplx<T> *phase, int nparts,
                        cplx<double> *partial) {
-  hipLaunchKernelGGL((k_sphere_sum<T, PMAX>), dim3(sp->nsites * nparts), dim3(64), 0, ctx->stream, grid, NB,
+  hipLaunchKernelGGL((k_sphere_sum<T, PMAX>), dim3(sp->nsites * nparts), dim3(256), 0, ctx->stream, grid, NB,
                      (const int *)sp->d_site_pt.p, (const int *)sp->d_site_np.p, (const long *)sp->d_tab_off.p,
                      (const int *)sp->d_pidx.p, phase, (const cplx<T> *)sp->d_table.p, nparts, partial);
 }
@@ -517,10 +575,10 @@ int sphere_sums(pp_ctx *ctx, pp_sphere *sp, const cplx<T> *grid, int NB, int nbc
     return PP_OK;
   int nparts = ctx->tune_sphere_parts;
   if (nparts <= 0) {
-    // aim for ~4 waves per SIMD across 256 CUs
-    nparts = (4096 + sp->nsites - 1) / sp->nsites;
+    // aim for ~8 four-wave work-groups per CU (256 CUs)
+    nparts = (2048 + sp->nsites - 1) / sp->nsites;
     long avg = sp->npts / std::max(1, sp->nsites);
-    int maxp = (int)std::max<long>(1, avg / 64);
+    int maxp = (int)std::max<long>(1, avg / 256);
     if (nparts > maxp)
       nparts = maxp;
     if (nparts < 1)
