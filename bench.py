@@ -77,6 +77,23 @@ def cpu_cores():
     return max(1, n)
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_216atom.md")
+
+
+def read_traffic(path_name):
+    """HBM bytes per launch of each kernel class, from the rocprofv3 --pmc
+    summary kept with the profiles (FETCH_SIZE and WRITE_SIZE per dispatch,
+    scaled by the factors a plain 1 GiB device copy reads on the same
+    counters).  Empty when the summary is absent or covers another path."""
+    try:
+        txt = open(TRAFFIC_FILE).read()
+        blk = txt.split("```json", 1)[1].split("```", 1)[0]
+        d = json.loads(blk)
+    except Exception:
+        return {}
+    return {k: float(v) for k, v in d.get("traffic_bytes_per_launch", {}).get(path_name, {}).items()}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -205,36 +222,61 @@ def main():
 
     head = "fp32" if "fp32" in results else paths[0]
     R = results[head]
-    # dominant kernel of the headline path and its roofline
+    # dominant kernel of the headline path and its roofline.  Algorithmic
+    # bytes per launch (one 64-band chunk of one k-point):
+    #   fft_x (last 1D pass): read the populated x-slab of the grid
+    #     (the plane-wave sphere spans xs_k planes), write the whole grid:
+    #     N1 N2 NB prec (xs_k + N0), averaged over k;
+    #   fft_y: read xs_k planes restricted to the populated y-rows ys_k(x),
+    #     write the xs_k planes in full;
+    #   fft_z: read the plane-wave coefficients in place (W_k points), write
+    #     every populated (x, y) z-line in full;
+    #   sphere_sum: gather NB values per sphere point, one real table row,
+    #     one phase and one grid index per point;  gemm: 8 M N K flop.
     prec_bytes = 8 if head != "fp64" else 16
-    npts_grid = int(np.prod(wl.fftg))
+    n0, n1, n2 = (int(v) for v in wl.fftg)
     NB = 64
-    groups = {"k_fft_pass": ["fft_x", "fft_y", "fft_z"], "k_sphere_sum": ["sphere_sum"], "gemm": ["gemm"]}
-    tot = {g: sum(R["kernels"].get(c, {}).get("ms_total", 0.0) for c in cs) for g, cs in groups.items()}
-    launches = {g: sum(R["kernels"].get(c, {}).get("launches", 0) for c in cs) for g, cs in groups.items()}
-    dom = max(tot, key=tot.get)
-    per_launch_ms = {g: tot[g] / max(1, launches[g]) for g in groups}
-    fft_bytes = 2.0 * npts_grid * NB * prec_bytes
+    bx, by, bz = [], [], []
+    for G in Gs:
+        gx, gy = G[:, 0].astype(np.int64), G[:, 1].astype(np.int64)
+        xs = int(gx.max() - gx.min() + 1)
+        rows = np.unique(gx * 4096 + gy).size  # populated (x, y) z-lines
+        yrow = 0
+        for x in np.unique(gx):
+            sel = gy[gx == x]
+            yrow += int(sel.max() - sel.min() + 1)
+        bx.append(n1 * n2 * NB * prec_bytes * (xs + n0))
+        by.append(NB * prec_bytes * n2 * (yrow + xs * n1))
+        bz.append(NB * prec_bytes * (len(G) + rows * n2))
     planS = plans[head][1]
-    sphere_bytes = planS.npts * NB * prec_bytes + planS.npts * 18 * prec_bytes
+    # grid gather (NB complex), real table row (P reals), phase (complex), index (int32) per point
+    sphere_bytes = planS.npts * (NB * prec_bytes + (ptotS // len(wl.frac_S)) * prec_bytes // 2 + prec_bytes + 4)
+    alg = {"fft_x": float(np.mean(bx)), "fft_y": float(np.mean(by)), "fft_z": float(np.mean(bz)),
+           "sphere_sum": float(sphere_bytes)}
     gemm_flops = 8.0 * nb * nb * float(nw.mean())
-    rl = {
-        "k_fft_pass": {"bound": "hbm", "achieved": fft_bytes / (per_launch_ms["k_fft_pass"] * 1e-3) / 1e9,
-                       "peak": PEAK_HBM / 1e9, "unit": "GB/s", "per_launch": f"{fft_bytes:.4g} B"},
-        "k_sphere_sum": {"bound": "hbm",
-                         "achieved": sphere_bytes / (per_launch_ms["k_sphere_sum"] * 1e-3) / 1e9,
-                         "peak": PEAK_HBM / 1e9, "unit": "GB/s", "per_launch": f"{sphere_bytes:.4g} B"},
-        "gemm": {"bound": "mfma", "achieved": gemm_flops / (per_launch_ms["gemm"] * 1e-3) / 1e12,
-                 "peak": PEAK["fp32" if head != "fp64" else "fp64"] / 1e12, "unit": "TFLOP/s",
-                 "per_launch": f"{gemm_flops:.4g} flop"},
-    }
+    traffic = read_traffic(head)
+    rl = {}
+    for c, b in alg.items():
+        kc = R["kernels"].get(c)
+        if not kc or not kc["launches"]:
+            continue
+        ms = kc["ms_total"] / kc["launches"]
+        rl[c] = {"bound": "hbm", "achieved": b / (ms * 1e-3) / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                 "alg_bytes": b, "avg_launch_ms": ms, "ms_per_step": kc["ms_total"],
+                 "traffic": traffic.get(c)}
+    kc = R["kernels"].get("gemm")
+    if kc and kc["launches"]:
+        ms = kc["ms_total"] / kc["launches"]
+        rl["gemm"] = {"bound": "mfma", "achieved": gemm_flops / (ms * 1e-3) / 1e12,
+                      "peak": PEAK["fp32" if head != "fp64" else "fp64"] / 1e12, "unit": "TFLOP/s",
+                      "alg_flops": gemm_flops, "avg_launch_ms": ms, "ms_per_step": kc["ms_total"],
+                      "traffic": traffic.get("gemm")}
     for v in rl.values():
         v["frac"] = v["achieved"] / v["peak"]
-    roof = dict(rl[dom])
+    dom = max(rl, key=lambda c: rl[c]["ms_per_step"])
+    roof = {k: rl[dom][k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")}
     roof["kernel"] = dom
-    roof["avg_launch_ms"] = per_launch_ms[dom]
-    roof["traffic"] = None
-    roof.pop("per_launch", None)
+    roof["avg_launch_ms"] = rl[dom]["avg_launch_ms"]
 
     out = {
         "metric": "band_projections_per_sec",

@@ -110,10 +110,25 @@ int setup_projections(pswf_t *wf, ppot_t *pps, int num_elems, int num_sites,
 int pseudoprojection(pp_cdouble *projections, pswf_t *wf_ref, pswf_t *wf_proj,
                      int BAND_NUM, int flip_spin);
 
+/* replaces projector.c:604-720 overlap_setup_real: S's bands projected
+ * onto R's plane-wave-filtered partial-wave differences at the N_R sites
+ * (on S's grid), and R's onto S's at the N_S sites, on the GPU (same
+ * scatter/FFT/sphere-sum path as setup_projections, with the
+ * smooth_diffwave tables of get_projector_list, radius wave_rmax); fills
+ * band->wave_projections (wf_S: num_N_R sites, wf_R: num_N_S sites) and
+ * keeps them in HBM.  For the N_RS pairs, the off-site partial-wave
+ * overlaps (radial.c:116-196) into wf_S->overlaps / wf_S->dcoords as the
+ * reference stores them.  Call setup_projections on both wavefunctions
+ * first. */
+int overlap_setup_real(pswf_t *wf_R, pswf_t *wf_S, int *labels_R, int *labels_S,
+                       double *coords_R, double *coords_S, int *N_R, int *N_S,
+                       int *N_RS_R, int *N_RS_S, int num_N_R, int num_N_S,
+                       int num_N_RS);
+
 /* replaces projector.c:850-963 compensation_terms (adds into overlap).
- * The O_M term is computed for all band pairs at once (FP64); the O_R,
- * O_S and O_N terms need overlap_setup_real, which this build does not
- * provide: nonzero num_N_R/num_N_S/num_N_RS returns PP_ERR_UNSUPPORTED. */
+ * O_M, O_R, O_S and O_N are computed for all band pairs at once (FP64) and
+ * cached per (wf_R, flip_spin, site lists); O_R/O_S/O_N need a preceding
+ * overlap_setup_real with the same N_R/N_S/N_RS lists. */
 int compensation_terms(pp_cdouble *overlap, int BAND_NUM, pswf_t *wf_S,
                        pswf_t *wf_R, int num_M, int num_N_R, int num_N_S,
                        int num_N_RS, int *M_R, int *M_S, int *N_R, int *N_S,
@@ -156,7 +171,10 @@ int pp_ctx_set_tuning(pp_ctx *ctx, int fft_bands, int fft_tile, int fft_prune,
                       int sphere_parts, int gemm_splitk);
 /* One tuning knob by name: "fft_bands", "fft_tile", "fft_prune",
  * "fft_pingpong" (1 = two-buffer LDS FFT stages instead of in-place),
- * "sphere_parts", "gemm_splitk". */
+ * "fft_persistent" (0 off, 1 on with 2048 work-groups, > 1 work-group
+ * count: FFT passes as a fixed grid that prefetches the next line),
+ * "sphere_parts", "gemm_splitk", "gemm_f64_valu" (1 = FP64 GEMM on the
+ * VALU instead of the FP64 matrix cores). */
 int pp_ctx_set_param(pp_ctx *ctx, const char *name, int value);
 
 /* Sphere plan of one structure (replaces the table build of
@@ -214,6 +232,24 @@ int pp_augmentation(pp_ctx *ctx, int nwk, int nspin, int flip,
                     const int *nchan, const int *ls, const double *dO,
                     void *out_dev);
 
+/* The off-site O_N term of projector.c:940-959 for every band pair:
+ * out[k][r][s] += sum over pairs t of
+ *   sum_ij conj(P_R[kR][r][off_R(N_RS_R[t]) + i]) ov_t[i][j]
+ *          P_S[k][s][off_S(N_RS_S[t]) + j] exp(2 pi i kpts[kR] . dcoords[t]),
+ * kR the spin-flipped k when flip (the reference multiplies the fractional
+ * k by the cartesian pair vector).  ov: host c128 blocks P_R x P_S at
+ * ov_off[t] (num_pairs + 1 offsets), as overlap_setup_real computes them
+ * (conj of the partial-wave overlaps, radial.c:116-196); dcoords [3 t];
+ * kpts [nkt][3].  FP64 on the GPU. */
+int pp_augmentation_offsite(pp_ctx *ctx, int nwk, int nspin, int flip,
+                            const void *projR_dev, int nbR, int ptotR,
+                            const int *proj_off_R, const void *projS_dev,
+                            int nbS, int ptotS, const int *proj_off_S,
+                            int num_pairs, const int *N_RS_R,
+                            const int *N_RS_S, const pp_cdouble *ov,
+                            const long *ov_off, const double *dcoords,
+                            const double *kpts, void *out_dev);
+
 /* Plane-wave coefficients of nband (<= 64) bands to their real-space
  * grids with the reference's 1/sqrt(V) scale (FP64 or FP32 FFT), the
  * device form of fft3d: G_dev int32 [nw][3], C_dev c64 [nband][ldc],
@@ -221,6 +257,26 @@ int pp_augmentation(pp_ctx *ctx, int nwk, int nspin, int flip,
 int pp_fft3d_batched(pp_ctx *ctx, const double *lattice, const int *fftg,
                      int nw, const int *G_dev, const void *C_dev, long ldc,
                      int nband, void *grid_out_dev, int precision);
+
+/* ---- per-species radial setup (host) ---- */
+
+/* Spherical Bessel transform g(k) = int r^2 j_l(k r) R(r) dr of
+ * f = r R(r) given on the n-point log grid r, on the k grid
+ * k_q = sqrt((encut + enbuf) c) e^{(q - n + 1) drho} (q < n).  Replaces
+ * spherical_bessel_transform_setup + wave_spherical_bessel_transform
+ * (sbt.c:24-153, bound as pawpyc.spherical_bessel_transform,
+ * pawpyc.pyx:130-150).  Writes ks[n] and g[n]. */
+int pp_sbt_forward(double encut, double enbuf, int lmax, int n, const double *r,
+                   const double *f, int l, double *ks, double *g);
+
+/* The plane-wave-filtered partial-wave difference of one channel
+ * (projector.c:114-163): diffwave = r (phi - phi~) on the n-point log
+ * wave grid, transformed (encut 1e7 eV), cut at k < sqrt(c grid_encut),
+ * transformed back and sampled on the linear grid j wave_rmax / np
+ * (sdw[np]); kgrid[n], kwave[n] get the unfiltered transform. */
+int pp_smooth_diffwave(int n, const double *wave_grid, const double *diffwave,
+                       int l, int lmax, double grid_encut, int np, double *sdw,
+                       double *kgrid, double *kwave);
 
 #ifdef __cplusplus
 }

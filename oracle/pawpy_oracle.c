@@ -720,3 +720,433 @@ void or_pwave_overlaps(const double *grid, int n, int nchan, const int *ls,
   free(c);
   free(d);
 }
+
+/* ------------------------------------------------------------------ */
+/* Spherical Bessel transform and the filtered partial-wave difference  */
+/* ------------------------------------------------------------------ */
+
+#define OR_C_EV 0.262465831
+
+/* x <- sum_j x[j] exp(+2 pi i j k / n), unnormalised (the DFTI backward
+ * transform of sbt.c:128-144); direct O(n^2) sum with exact twiddles. */
+static void dft_backward(double *x, int n) {
+  double *y = (double *)calloc(2 * (size_t)n, sizeof(double));
+  for (int k = 0; k < n; k++) {
+    long double re = 0.0L, im = 0.0L;
+    for (int j = 0; j < n; j++) {
+      long q = ((long)j * k) % n;
+      long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)q / (long double)n;
+      long double cs = cosl(a), sn = sinl(a);
+      re += x[2 * j] * cs - x[2 * j + 1] * sn;
+      im += x[2 * j] * sn + x[2 * j + 1] * cs;
+    }
+    y[2 * k] = (double)re;
+    y[2 * k + 1] = (double)im;
+  }
+  memcpy(x, y, 2 * (size_t)n * sizeof(double));
+  free(y);
+}
+
+typedef struct {
+  int N, lmax;
+  double *ks, *rs; /* N each */
+  double *mult;    /* (lmax+1) x N complex */
+} sbt_desc;
+
+/* sbt.c:24-96: log grids r_i = r0 e^{(i-N/2) drho}, k_i = kmin e^{i drho}
+ * (N = 2n, kmin = sqrt((encut+enbuf) c) e^{-(N/2-1) drho}) and the
+ * multiplier table M_l(t_i), t_i = i 2 pi / (N drho), of Talman's
+ * algorithm (J. Talman, CPC 180 (2009) 332): the phase of the Gamma-function
+ * ratio from a Stirling series with ten shift terms, then the
+ * recurrence M_{l+1} = e^{-2i atan(2t/(2l+1))} M_{l-1}. */
+static void sbt_setup(sbt_desc *d, double encut, double enbuf, int lmax, int n, const double *r) {
+  const int N = 2 * n;
+  if (lmax == 0)
+    lmax = 1;
+  d->N = N;
+  d->lmax = lmax;
+  d->ks = (double *)malloc(sizeof(double) * N);
+  d->rs = (double *)malloc(sizeof(double) * N);
+  d->mult = (double *)calloc(2 * (size_t)(lmax + 1) * N, sizeof(double));
+  const double drho = log(r[1] / r[0]);
+  const double dt = 2.0 * OR_PI / N / drho;
+  const double kmin = sqrt((encut + enbuf) * OR_C_EV) * exp(-(N / 2 - 1) * drho);
+  const double kappamin = log(kmin);
+  for (int i = 0; i < N; i++) {
+    d->ks[i] = kmin * exp(i * drho);
+    d->rs[i] = r[0] * exp((i - N / 2) * drho);
+  }
+  const double rhomin = log(d->rs[0]);
+  for (int i = 0; i < N; i++) {
+    const double t = i * dt;
+    const double rad = sqrt(10.5 * 10.5 + t * t);
+    const double ph = atan(2.0 * t / 21.0);
+    double p1 = -10.0 * ph - t * log(rad) + t + sin(ph) / (12.0 * rad) - sin(3.0 * ph) / (360.0 * pow(rad, 3)) +
+                sin(5.0 * ph) / (1260.0 * pow(rad, 5)) - sin(7.0 * ph) / (1680.0 * pow(rad, 7));
+    for (int j = 1; j <= 10; j++)
+      p1 += atan(2.0 * t / (2 * j - 1));
+    const double p2 = -atan(tanh(OR_PI * t / 2.0));
+    const double p3 = (kappamin + rhomin) * t;
+    const double phase = p1 + p2 + p3;
+    double amp = sqrt(OR_PI / 2.0) / N;
+    if (i == 0)
+      amp *= 0.5;
+    double *m0 = d->mult + 2 * (size_t)i;
+    m0[0] = amp * cos(phase);
+    m0[1] = amp * sin(phase);
+    /* M_1 = e^{2i(-p2 - atan 2t)} M_0 */
+    const double a1 = 2.0 * (-p2 - atan(2.0 * t));
+    double *m1 = d->mult + 2 * ((size_t)N + i);
+    m1[0] = cos(a1) * m0[0] - sin(a1) * m0[1];
+    m1[1] = sin(a1) * m0[0] + cos(a1) * m0[1];
+    for (int l = 1; l < lmax; l++) {
+      const double al = -2.0 * atan(2.0 * t / (2 * l + 1));
+      const double *mp = d->mult + 2 * ((size_t)(l - 1) * N + i);
+      double *mn = d->mult + 2 * ((size_t)(l + 1) * N + i);
+      mn[0] = cos(al) * mp[0] - sin(al) * mp[1];
+      mn[1] = sin(al) * mp[0] + cos(al) * mp[1];
+    }
+  }
+}
+
+static void sbt_free(sbt_desc *d) {
+  free(d->ks);
+  free(d->rs);
+  free(d->mult);
+}
+
+/* Two backward DFTs around the multiplier: in[m] -> DFT -> x M_l, upper
+ * half zeroed -> DFT (sbt.c:133-144, 190-200). */
+static void sbt_core(const sbt_desc *d, double *x, int l) {
+  const int N = d->N;
+  dft_backward(x, N);
+  const double *M = d->mult + 2 * (size_t)l * N;
+  for (int i = 0; i < N; i++) {
+    if (i >= N / 2) {
+      x[2 * i] = x[2 * i + 1] = 0.0;
+    } else {
+      const double re = x[2 * i] * M[2 * i] - x[2 * i + 1] * M[2 * i + 1];
+      const double im = x[2 * i] * M[2 * i + 1] + x[2 * i + 1] * M[2 * i];
+      x[2 * i] = re;
+      x[2 * i + 1] = im;
+    }
+  }
+  dft_backward(x, N);
+}
+
+/* sbt.c:98-153: g(k_p) = int r^2 j_l(k_p r) R(r) dr for f = r R(r)
+ * sampled on the n-point log grid r (below r[0], f is continued as
+ * f[0] (r/r[0])^{l+1}).  Writes ks[n] and g[n]. */
+void or_sbt_forward(double encut, double enbuf, int lmax, int n, const double *r, const double *f, int l,
+                    double *ks_out, double *g) {
+  sbt_desc d;
+  sbt_setup(&d, encut, enbuf, lmax, n, r);
+  const int N = d.N;
+  double *x = (double *)calloc(2 * (size_t)N, sizeof(double));
+  const double C = f[0] / pow(d.rs[N / 2], l + 1);
+  for (int i = 0; i < N; i++) {
+    const double fi = i < N / 2 ? C * pow(d.rs[i], l + 1) : f[i - N / 2];
+    x[2 * i] = sqrt(d.rs[i]) * fi;
+  }
+  sbt_core(&d, x, l);
+  for (int p = 0; p < n; p++) {
+    ks_out[p] = d.ks[p];
+    g[p] = x[2 * p] * 2.0 / pow(d.ks[p], 1.5);
+  }
+  free(x);
+  sbt_free(&d);
+}
+
+/* sbt.c:155-213: R(r_p) = (2/pi) int k^2 j_l(k r_p) g(k) dk for g on the
+ * k grid of the same descriptor; r_p = rs[n + p] (the input r grid). */
+void or_sbt_inverse(double encut, double enbuf, int lmax, int n, const double *r, const double *g, int l,
+                    double *rs_out, double *R) {
+  sbt_desc d;
+  sbt_setup(&d, encut, enbuf, lmax, n, r);
+  const int N = d.N;
+  double *x = (double *)calloc(2 * (size_t)N, sizeof(double));
+  for (int i = 0; i < N / 2; i++)
+    x[2 * i] = pow(d.ks[i], 1.5) * g[i];
+  sbt_core(&d, x, l);
+  for (int p = 0; p < n; p++) {
+    rs_out[p] = d.rs[p + N / 2];
+    R[p] = x[2 * (p + N / 2)] / OR_PI * 2.0 * 2.0 / pow(d.rs[p + N / 2], 1.5);
+  }
+  free(x);
+  sbt_free(&d);
+}
+
+/* utils.c:477-489 (wave_interpolate): cubic spline on a log grid. */
+double or_wave_interpolate(double r, int n, const double *x, const double *f, const double *b, const double *c,
+                           const double *d) {
+  if (r > x[n - 1])
+    return 0.0;
+  if (r < x[0])
+    return f[0];
+  int i = (int)(log(r / x[0]) / log(x[1] / x[0]));
+  if (i > n - 2)
+    i = n - 2;
+  const double t = r - x[i];
+  return f[i] + t * (b[i] + t * (c[i] + t * d[i]));
+}
+
+/* projector.c:114-163 for one partial-wave difference f = r (phi - phi~)
+ * on the n-point log grid wave_grid: SBT onto the k grid (encut 1e7 eV),
+ * keep k < sqrt(c grid_encut), transform back, and sample the result on
+ * the linear grid j wave_rmax / np (j < np) of the sphere tables
+ * (sdw[0] = 0 for l > 0, sdw[0] = sdw[1] for l = 0).  Also returns the
+ * k grid and the unfiltered transform (kwave, used by the off-site terms). */
+void or_smooth_diffwave(int n, const double *wave_grid, const double *diffwave, int l, int lmax,
+                        double grid_encut, int np, double *sdw, double *kgrid, double *kwave) {
+  or_sbt_forward(1e7, 0.0, lmax, n, wave_grid, diffwave, l, kgrid, kwave);
+  double *kd = (double *)calloc((size_t)n, sizeof(double));
+  const double kcut = sqrt(OR_C_EV * grid_encut);
+  for (int q = 0; q < n && kgrid[q] < kcut; q++)
+    kd[q] = kwave[q];
+  double *rs = (double *)malloc(sizeof(double) * n);
+  double *sm = (double *)malloc(sizeof(double) * n);
+  or_sbt_inverse(1e7, 0.0, lmax, n, wave_grid, kd, l, rs, sm);
+  /* the dense grid of projector.c:62-69 (scale 1): wave_grid[0] times
+   * powers of wave_grid[1] / wave_grid[0] */
+  double *dg = (double *)malloc(sizeof(double) * n);
+  dg[0] = wave_grid[0];
+  const double fac = wave_grid[1] / wave_grid[0];
+  for (int p = 1; p < n; p++)
+    dg[p] = dg[p - 1] * fac;
+  double *b = (double *)malloc(sizeof(double) * n);
+  double *c = (double *)malloc(sizeof(double) * n);
+  double *d = (double *)malloc(sizeof(double) * n);
+  or_spline_coeff(dg, sm, n, b, c, d);
+  const double wave_rmax = wave_grid[n - 1];
+  if (l > 0)
+    sdw[0] = 0.0;
+  for (int p = 1; p < np; p++)
+    sdw[p] = or_wave_interpolate(wave_rmax / np * p, n, dg, sm, b, c, d);
+  if (l == 0)
+    sdw[0] = np > 1 ? sdw[1] : 0.0;
+  free(kd);
+  free(rs);
+  free(sm);
+  free(dg);
+  free(b);
+  free(c);
+  free(d);
+}
+
+/* projector.c:912-938 (O_R and O_S): for w = b_R * nkt + k,
+ * overlap[w] += sum_{s < num_N_R} sum_i WP_S[s][i] conj(P_R[N_R[s]][i])
+ *             + sum_{s < num_N_S} sum_i conj(WP_R[s][i]) P_S[N_S[s]][i].
+ * P_R, WP_R: [nkt][nbR][ptot]; WP_S, P_S: the projected band's rows
+ * [nkt][ptot].  WP_* site s occupies wp_off[s] .. wp_off[s+1]. */
+void or_compensation_terms_rs(double *overlap, int nwk, int nspin, int nbR,
+                              const double *proj_R, int ptotR,
+                              const int *proj_off_R, int num_N_R,
+                              const int *N_R, const double *wp_S,
+                              int wp_ptot_S, const int *wp_off_S,
+                              const double *wp_R, int wp_ptot_R,
+                              const int *wp_off_R, const double *proj_S,
+                              int ptotS, const int *proj_off_S, int num_N_S,
+                              const int *N_S, int flip) {
+  int nkt = nwk * nspin;
+#pragma omp parallel for
+  for (int w = 0; w < nbR * nkt; w++) {
+    int k = w % nkt;
+    int kR = k;
+    if (nspin == 2 && flip)
+      kR = (kR < nwk) ? kR + nwk : kR - nwk;
+    int bR = w / nkt;
+    double tr = 0.0, ti = 0.0;
+    const double *pr = proj_R + 2 * (((long)kR * nbR + bR) * ptotR);
+    const double *ws = wp_S + 2 * ((long)k * wp_ptot_S);
+    for (int s = 0; s < num_N_R; s++) {
+      int site = N_R[s];
+      int np = wp_off_S[s + 1] - wp_off_S[s];
+      for (int i = 0; i < np; i++) {
+        double ar = ws[2 * (wp_off_S[s] + i)], ai = ws[2 * (wp_off_S[s] + i) + 1];
+        double br = pr[2 * (proj_off_R[site] + i)], bi = -pr[2 * (proj_off_R[site] + i) + 1];
+        tr += ar * br - ai * bi;
+        ti += ar * bi + ai * br;
+      }
+    }
+    const double *wr = wp_R + 2 * (((long)kR * nbR + bR) * wp_ptot_R);
+    const double *ps = proj_S + 2 * ((long)k * ptotS);
+    for (int s = 0; s < num_N_S; s++) {
+      int site = N_S[s];
+      int np = wp_off_R[s + 1] - wp_off_R[s];
+      for (int i = 0; i < np; i++) {
+        double ar = wr[2 * (wp_off_R[s] + i)], ai = -wr[2 * (wp_off_R[s] + i) + 1];
+        double br = ps[2 * (proj_off_S[site] + i)], bi = ps[2 * (proj_off_S[site] + i) + 1];
+        tr += ar * br - ai * bi;
+        ti += ar * bi + ai * br;
+      }
+    }
+    overlap[2 * w] += tr;
+    overlap[2 * w + 1] += ti;
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* Off-site partial-wave overlaps (O_N)                                 */
+/* ------------------------------------------------------------------ */
+
+/* Wigner 3j symbol (j1 j2 j3; m1 m2 m3) for integer arguments, Racah's
+ * closed form. */
+double or_wigner3j(int j1, int j2, int j3, int m1, int m2, int m3) {
+  if (m1 + m2 + m3 != 0 || j3 < abs(j1 - j2) || j3 > j1 + j2 || abs(m1) > j1 || abs(m2) > j2 || abs(m3) > j3)
+    return 0.0;
+  double tri = fact(j1 + j2 - j3) * fact(j1 - j2 + j3) * fact(-j1 + j2 + j3) / fact(j1 + j2 + j3 + 1);
+  double pre = sqrt(tri * fact(j1 + m1) * fact(j1 - m1) * fact(j2 + m2) * fact(j2 - m2) * fact(j3 + m3) *
+                    fact(j3 - m3));
+  int kmin = 0;
+  if (j2 - j3 - m1 > kmin)
+    kmin = j2 - j3 - m1;
+  if (j1 - j3 + m2 > kmin)
+    kmin = j1 - j3 + m2;
+  int kmax = j1 + j2 - j3;
+  if (j1 - m1 < kmax)
+    kmax = j1 - m1;
+  if (j2 + m2 < kmax)
+    kmax = j2 + m2;
+  double sum = 0.0;
+  for (int k = kmin; k <= kmax; k++) {
+    double den = fact(k) * fact(j1 + j2 - j3 - k) * fact(j1 - m1 - k) * fact(j2 + m2 - k) * fact(j3 - j2 + m1 + k) *
+                 fact(j3 - j1 - m2 + k);
+    sum += ((k % 2) ? -1.0 : 1.0) / den;
+  }
+  int ph = j1 - j2 - m3;
+  return (((ph % 2) + 2) % 2 ? -1.0 : 1.0) * pre * sum;
+}
+
+/* gaunt.py:17-27: SBTFACS[l1][l2][(l3-|l1-l2|)/2][l1+m1][m2] =
+ * (l1 l2 l3; 0 0 0)(l1 l2 l3; -m1 m2 m1-m2) sqrt((2l1+1)(2l2+1)(2l3+1)/4pi). */
+double or_sbtfac(int l1, int l2, int l3, int m1, int m2) {
+  return or_wigner3j(l1, l2, l3, 0, 0, 0) * or_wigner3j(l1, l2, l3, -m1, m2, m1 - m2) *
+         sqrt((2 * l1 + 1) * (2 * l2 + 1) * (2 * l3 + 1) / 4.0 / OR_PI);
+}
+
+/* utils.c:807-827: j_l(x) by upward recurrence, 1 or 0 below 1e-5. */
+static double sbf(double x, int l) {
+  if (x < 10e-6)
+    return l == 0 ? 1.0 : 0.0;
+  double jm = sin(x) / x, j = sin(x) / (x * x) - cos(x) / x, jp = 0.0;
+  if (l == 0)
+    return jm;
+  if (l == 1)
+    return j;
+  for (int ll = 1; ll < l; ll++) {
+    jp = (2 * ll + 1) / x * j - jm;
+    jm = j;
+    j = jp;
+  }
+  return jp;
+}
+
+/* radial.c:116-196 (reciprocal_offsite_wave_overlap): overlap of the
+ * partial-wave differences (l1, m1) at the origin and (l2, m2) at dcoord
+ * (cartesian) from their transforms f1, f2 on the log k grids k1, k2
+ * (spline coefficients b, c, d):
+ *   sum_{L = |l1-l2|, step 2}^{l1+l2} I_L F(L) Y_L^{m1-m2}(dcoord) i^{l2+L-l1} 8 (-1)^m1,
+ * I_L = int k^2 f1 f2 j_L(k R) dk by spline integration over 500 log
+ * points of [max(k1[0], k2[0]), min(k1[-1], k2[-1])]; at R = 0 only
+ * L = 0, l1 = l2, m1 = m2 contributes, as (2/pi) I_0.  out2 = (re, im). */
+void or_offsite_overlap(const double *dcoord, const double *k1, const double *f1, const double *b1,
+                        const double *c1, const double *d1, int n1, const double *k2, const double *f2,
+                        const double *b2, const double *c2, const double *d2, int n2, int l1, int m1, int l2,
+                        int m2, double *out2) {
+  enum { NK = 500 };
+  int lx = l1, ly = l2, mx = m1, my = m2;
+  if (l1 < l2) {
+    lx = l2;
+    ly = l1;
+    mx = m2;
+    my = m1;
+  }
+  if (my < 0) {
+    mx = -mx;
+    my = -my;
+  }
+  double kmax = k1[n1 - 1] < k2[n2 - 1] ? k1[n1 - 1] : k2[n2 - 1];
+  double kmin = k1[0] > k2[0] ? k1[0] : k2[0];
+  double R = sqrt(dcoord[0] * dcoord[0] + dcoord[1] * dcoord[1] + dcoord[2] * dcoord[2]);
+  double theta = 0.0, phi = 0.0;
+  if (R < 10e-12) {
+    R = 0.0;
+  } else {
+    theta = acos(dcoord[2] / R);
+    if (R - fabs(dcoord[2]) >= 10e-12)
+      phi = acos(dcoord[0] / sqrt(dcoord[0] * dcoord[0] + dcoord[1] * dcoord[1]));
+    if (dcoord[1] < 0)
+      phi = 2.0 * OR_PI - phi;
+  }
+  double kg[NK], fn[NK], bb[NK], cc[NK], dd[NK];
+  double tr = 0.0, ti = 0.0;
+  const double mult = ((m1 % 2) ? -8.0 : 8.0);
+  for (int L = abs(l1 - l2); L <= l1 + l2; L += 2) {
+    for (int q = 0; q < NK; q++) {
+      kg[q] = kmin * pow(kmax / kmin, (double)q / NK);
+      fn[q] = or_wave_interpolate(kg[q], n1, k1, f1, b1, c1, d1) * or_wave_interpolate(kg[q], n2, k2, f2, b2, c2, d2) *
+              kg[q] * kg[q] * sbf(kg[q] * R, L);
+    }
+    or_spline_coeff(kg, fn, NK, bb, cc, dd);
+    double I = or_spline_integral(kg, fn, bb, cc, dd, NK);
+    if (R > 10e-10) {
+      double y[2];
+      or_ylm(L, m1 - m2, theta, phi, y);
+      double f = I * or_sbtfac(lx, ly, L, mx, my) * mult;
+      /* i^(l2 + L - l1) */
+      int p = (((l2 + L - l1) % 4) + 4) % 4;
+      double ir = p == 0 ? 1.0 : p == 2 ? -1.0 : 0.0, ii = p == 1 ? 1.0 : p == 3 ? -1.0 : 0.0;
+      tr += f * (y[0] * ir - y[1] * ii);
+      ti += f * (y[0] * ii + y[1] * ir);
+    } else if (L == 0 && l1 == l2 && m1 == m2) {
+      tr += I * 2.0 / OR_PI;
+    }
+  }
+  out2[0] = tr;
+  out2[1] = ti;
+}
+
+/* projector.c:940-959 (O_N): for w = b_R * nkt + k,
+ * overlap[w] += sum_s sum_ij conj(P_R[N_RS_R[s]][i]) Ov_s[i][j]
+ *                         P_S[N_RS_S[s]][j] exp(2 pi i kR . dcoord_s),
+ * kR the (possibly spin-flipped) k of R in the units the reference mixes.
+ * Ov: per pair s, P_R_s x P_S_s complex at ov_off[s]; kpts [nkt][3]. */
+void or_compensation_terms_n(double *overlap, int nwk, int nspin, int nbR, const double *proj_R, int ptotR,
+                             const int *proj_off_R, const double *proj_S, int ptotS, const int *proj_off_S,
+                             int num_N_RS, const int *N_RS_R, const int *N_RS_S, const double *ov,
+                             const long *ov_off, const double *dcoords, const double *kpts, int flip) {
+  int nkt = nwk * nspin;
+#pragma omp parallel for
+  for (int w = 0; w < nbR * nkt; w++) {
+    int k = w % nkt;
+    int kR = k;
+    if (nspin == 2 && flip)
+      kR = (kR < nwk) ? kR + nwk : kR - nwk;
+    int bR = w / nkt;
+    const double *pr = proj_R + 2 * (((long)kR * nbR + bR) * ptotR);
+    const double *ps = proj_S + 2 * ((long)k * ptotS);
+    double tr = 0.0, ti = 0.0;
+    for (int s = 0; s < num_N_RS; s++) {
+      int s1 = N_RS_R[s], s2 = N_RS_S[s];
+      int n1 = proj_off_R[s1 + 1] - proj_off_R[s1], n2 = proj_off_S[s2 + 1] - proj_off_S[s2];
+      const double *dc = dcoords + 3 * s;
+      double a = 2.0 * OR_PI * (kpts[3 * kR] * dc[0] + kpts[3 * kR + 1] * dc[1] + kpts[3 * kR + 2] * dc[2]);
+      double er = cos(a), ei = sin(a);
+      const double *O = ov + 2 * ov_off[s];
+      for (int i = 0; i < n1; i++) {
+        double ar = pr[2 * (proj_off_R[s1] + i)], ai = -pr[2 * (proj_off_R[s1] + i) + 1];
+        for (int j = 0; j < n2; j++) {
+          double orr = O[2 * (i * n2 + j)], oi = O[2 * (i * n2 + j) + 1];
+          double br = ps[2 * (proj_off_S[s2] + j)], bi = ps[2 * (proj_off_S[s2] + j) + 1];
+          /* conj(P_R) Ov P_S e */
+          double xr = ar * orr - ai * oi, xi = ar * oi + ai * orr;
+          double yr = xr * br - xi * bi, yi = xr * bi + xi * br;
+          tr += yr * er - yi * ei;
+          ti += yr * ei + yi * er;
+        }
+      }
+    }
+    overlap[2 * w] += tr;
+    overlap[2 * w + 1] += ti;
+  }
+}

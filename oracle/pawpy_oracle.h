@@ -111,6 +111,45 @@ void or_pwave_overlaps(const double *grid, int n, int nchan, const int *ls,
                        const double *aewaves, const double *pswaves,
                        double *psov, double *aeov, double *diov);
 
+/* projector.c:912-938: the O_R and O_S terms added into overlap
+ * (c128 [nbR * nkt]) from the wave projections of overlap_setup_real. */
+void or_compensation_terms_rs(double *overlap, int nwk, int nspin, int nbR,
+                              const double *proj_R, int ptotR,
+                              const int *proj_off_R, int num_N_R,
+                              const int *N_R, const double *wp_S,
+                              int wp_ptot_S, const int *wp_off_S,
+                              const double *wp_R, int wp_ptot_R,
+                              const int *wp_off_R, const double *proj_S,
+                              int ptotS, const int *proj_off_S, int num_N_S,
+                              const int *N_S, int flip);
+
+/* ---- spherical Bessel transform: sbt.c:24-213 (Talman's NumSBT) ----
+ * forward: g(k) = int r^2 j_l(k r) R(r) dr from f = r R(r) on the n-point
+ * log grid r; inverse: R(r) = (2/pi) int k^2 j_l(k r) g(k) dk. */
+void or_sbt_forward(double encut, double enbuf, int lmax, int n, const double *r, const double *f, int l,
+                    double *ks_out, double *g);
+void or_sbt_inverse(double encut, double enbuf, int lmax, int n, const double *r, const double *g, int l,
+                    double *rs_out, double *R);
+/* utils.c:477-489 */
+double or_wave_interpolate(double r, int n, const double *x, const double *f, const double *b, const double *c,
+                           const double *d);
+/* projector.c:114-163: SBT-filtered partial-wave difference on the
+ * np-point linear grid j wave_rmax / np (sdw[np]); kgrid[n], kwave[n]. */
+void or_smooth_diffwave(int n, const double *wave_grid, const double *diffwave, int l, int lmax,
+                        double grid_encut, int np, double *sdw, double *kgrid, double *kwave);
+
+/* ---- off-site overlaps: gaunt.py:17-27, radial.c:116-196, projector.c:940-959 ---- */
+double or_wigner3j(int j1, int j2, int j3, int m1, int m2, int m3);
+double or_sbtfac(int l1, int l2, int l3, int m1, int m2);
+void or_offsite_overlap(const double *dcoord, const double *k1, const double *f1, const double *b1,
+                        const double *c1, const double *d1, int n1, const double *k2, const double *f2,
+                        const double *b2, const double *c2, const double *d2, int n2, int l1, int m1, int l2,
+                        int m2, double *out2);
+void or_compensation_terms_n(double *overlap, int nwk, int nspin, int nbR, const double *proj_R, int ptotR,
+                             const int *proj_off_R, const double *proj_S, int ptotS, const int *proj_off_S,
+                             int num_N_RS, const int *N_RS_R, const int *N_RS_S, const double *ov,
+                             const long *ov_off, const double *dcoords, const double *kpts, int flip);
+
 #ifdef __cplusplus
 }
 #endif

@@ -44,6 +44,11 @@ def lib():
         L.or_site_capacity.argtypes = [_d, _i, ct.c_double]
         L.or_site_table.argtypes = [_d, _i, _d, ct.c_double, ct.c_int, _d, ct.c_int, _i, _d, _d, _d, _d,
                                     ct.c_int, _i, _d, _d]
+        L.or_wigner3j.restype = ct.c_double
+        L.or_wigner3j.argtypes = [ct.c_int] * 6
+        L.or_sbtfac.restype = ct.c_double
+        L.or_sbtfac.argtypes = [ct.c_int] * 5
+        L.or_wave_interpolate.restype = ct.c_double
     return _LIB
 
 
@@ -293,3 +298,116 @@ def pwave_overlaps(grid, ls, aewaves, pswaves):
     lib().or_pwave_overlaps(_p(grid, _d), n, nch, _p(lsa, _i), _p(ae, _d), _p(ps, _d), _p(psov, _d),
                             _p(aeov, _d), _p(diov, _d))
     return psov.reshape(nch, nch), aeov.reshape(nch, nch), diov.reshape(nch, nch)
+
+
+def sbt_forward(r, f, l, lmax=None, encut=1e7, enbuf=0.0):
+    """sbt.c:98-153.  Returns (k grid, g(k))."""
+    r = _c(r, np.float64)
+    f = _c(f, np.float64)
+    n = len(r)
+    ks, g = np.zeros(n), np.zeros(n)
+    lib().or_sbt_forward(ct.c_double(encut), ct.c_double(enbuf), int(l if lmax is None else lmax), n, _p(r, _d),
+                         _p(f, _d), int(l), _p(ks, _d), _p(g, _d))
+    return ks, g
+
+
+def sbt_inverse(r, g, l, lmax=None, encut=1e7, enbuf=0.0):
+    """sbt.c:155-213 with the descriptor of the r grid.  Returns (r grid, R(r))."""
+    r = _c(r, np.float64)
+    g = _c(g, np.float64)
+    n = len(r)
+    rs, R = np.zeros(n), np.zeros(n)
+    lib().or_sbt_inverse(ct.c_double(encut), ct.c_double(enbuf), int(l if lmax is None else lmax), n, _p(r, _d),
+                         _p(g, _d), int(l), _p(rs, _d), _p(R, _d))
+    return rs, R
+
+
+def smooth_diffwave(wave_grid, diffwave, l, lmax, grid_encut, np_proj):
+    """projector.c:114-163.  Returns (sdw on j*wave_rmax/np_proj, k grid, kwave)."""
+    wg = _c(wave_grid, np.float64)
+    dw = _c(diffwave, np.float64)
+    n = len(wg)
+    sdw, kg, kw = np.zeros(np_proj), np.zeros(n), np.zeros(n)
+    lib().or_smooth_diffwave(n, _p(wg, _d), _p(dw, _d), int(l), int(lmax), ct.c_double(grid_encut), int(np_proj),
+                             _p(sdw, _d), _p(kg, _d), _p(kw, _d))
+    return sdw, kg, kw
+
+
+def compensation_terms_rs(overlap, nwk, nspin, proj_R, T_R, N_R, wp_S_band, WT_S, wp_R, WT_R, proj_S_band, T_S, N_S,
+                          flip=False):
+    """O_R + O_S of projector.c:912-938 added into ``overlap`` (c128
+    [nbR*nkt]).  proj_R, wp_R: c128 [nkt, nbR, ptot]; wp_S_band,
+    proj_S_band: c128 [nkt, ptot] (the projected band's rows).  T_*/WT_*:
+    SiteTables (only ptot and proj_off are read); WT_S holds the N_R sites in
+    list order, WT_R the N_S sites."""
+    nbR = proj_R.shape[1]
+    ov = _c(overlap, np.complex128)
+    pr = _c(proj_R, np.complex128)
+    ws = _c(wp_S_band, np.complex128)
+    wr = _c(wp_R, np.complex128)
+    ps = _c(proj_S_band, np.complex128)
+    nr = _c(N_R, np.int32)
+    ns = _c(N_S, np.int32)
+    lib().or_compensation_terms_rs(_p(ov, _d), nwk, nspin, nbR, _p(pr, _d), T_R.ptot, _p(T_R.proj_off, _i), len(nr),
+                                   _p(nr, _i), _p(ws, _d), WT_S.ptot, _p(WT_S.proj_off, _i), _p(wr, _d), WT_R.ptot,
+                                   _p(WT_R.proj_off, _i), _p(ps, _d), T_S.ptot, _p(T_S.proj_off, _i), len(ns),
+                                   _p(ns, _i), int(bool(flip)))
+    return ov
+
+
+def smooth_species(sp, grid_encut):
+    """The species' filtered partial-wave differences (projector.c:114-163)
+    as a projector set on the smooth grid j * wave_rmax / npts with radius
+    wave_rmax (the tables of smooth_pw_values, projector.c:211-221)."""
+    import dataclasses
+    wave_rmax = float(sp.grid[-1])
+    lmax = max(sp.ls)
+    vals = [smooth_diffwave(sp.grid, np.asarray(ae) - np.asarray(ps), l, lmax, grid_encut, sp.npts)[0]
+            for l, ae, ps in zip(sp.ls, sp.aewaves, sp.pswaves)]
+    return dataclasses.replace(sp, rmax=wave_rmax, realprojs=vals,
+                               projgrid=np.arange(sp.npts) * wave_rmax / sp.npts)
+
+
+def wigner3j(j1, j2, j3, m1, m2, m3):
+    return lib().or_wigner3j(int(j1), int(j2), int(j3), int(m1), int(m2), int(m3))
+
+
+def sbtfac(l1, l2, l3, m1, m2):
+    return lib().or_sbtfac(int(l1), int(l2), int(l3), int(m1), int(m2))
+
+
+def offsite_overlap(dcoord, k1, f1, k2, f2, l1, m1, l2, m2):
+    """radial.c:116-196 for transforms f1, f2 on log k grids k1, k2."""
+    k1, f1, k2, f2 = (_c(a, np.float64) for a in (k1, f1, k2, f2))
+    b1, c1, d1 = spline_coeff(k1, f1)
+    b2, c2, d2 = spline_coeff(k2, f2)
+    dc = _c(dcoord, np.float64)
+    out = np.zeros(2)
+    lib().or_offsite_overlap(_p(dc, _d), _p(k1, _d), _p(f1, _d), _p(b1, _d), _p(c1, _d), _p(d1, _d), len(k1),
+                             _p(k2, _d), _p(f2, _d), _p(b2, _d), _p(c2, _d), _p(d2, _d), len(k2), int(l1), int(m1),
+                             int(l2), int(m2), _p(out, _d))
+    return complex(out[0], out[1])
+
+
+def compensation_terms_n(overlap, nwk, nspin, proj_R, T_R, proj_S_band, T_S, N_RS_R, N_RS_S, ovs, dcoords, kpts,
+                         flip=False):
+    """O_N of projector.c:940-959 added into ``overlap`` (c128 [nbR*nkt]);
+    ovs: list of complex [P_R_s, P_S_s] matrices (the conj of the off-site
+    overlaps, as overlap_setup_real stores them); dcoords [n, 3]; kpts
+    [nkt, 3] fractional."""
+    nbR = proj_R.shape[1]
+    ov = _c(overlap, np.complex128)
+    pr = _c(proj_R, np.complex128)
+    ps = _c(proj_S_band, np.complex128)
+    flat = _c(np.concatenate([np.asarray(o, np.complex128).ravel() for o in ovs]) if len(ovs) else np.zeros(1),
+              np.complex128)
+    off = _c(np.concatenate([[0], np.cumsum([np.asarray(o).size for o in ovs])]), np.int64)
+    dc = _c(np.asarray(dcoords, np.float64).reshape(-1, 3) if len(ovs) else np.zeros((1, 3)), np.float64)
+    kp = _c(kpts, np.float64)
+    nr = _c(N_RS_R if len(ovs) else [0], np.int32)
+    ns = _c(N_RS_S if len(ovs) else [0], np.int32)
+    lib().or_compensation_terms_n(_p(ov, _d), nwk, nspin, nbR, _p(pr, _d), T_R.ptot, _p(T_R.proj_off, _i),
+                                  _p(ps, _d), T_S.ptot, _p(T_S.proj_off, _i), len(ovs), _p(nr, _i), _p(ns, _i),
+                                  _p(flat, _d), off.ctypes.data_as(ct.POINTER(ct.c_long)), _p(dc, _d), _p(kp, _d),
+                                  int(bool(flip)))
+    return ov

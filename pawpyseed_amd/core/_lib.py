@@ -60,6 +60,8 @@ SIGNATURES = {
                                       c_double_p, c_double_p, ct.c_double]),
     "setup_projections": (ct.c_int, [c_void_p, c_void_p, ct.c_int, ct.c_int, c_int_p, c_int_p, c_double_p]),
     "pseudoprojection": (ct.c_int, [c_void_p, c_void_p, c_void_p, ct.c_int, ct.c_int]),
+    "overlap_setup_real": (ct.c_int, [c_void_p, c_void_p, c_int_p, c_int_p, c_double_p, c_double_p, c_int_p, c_int_p,
+                                      c_int_p, c_int_p, ct.c_int, ct.c_int, ct.c_int]),
     "compensation_terms": (ct.c_int, [c_void_p, ct.c_int, c_void_p, c_void_p, ct.c_int, ct.c_int, ct.c_int,
                                       ct.c_int, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p,
                                       c_double_p, c_int_p, c_double_p, c_int_p, ct.c_int]),
@@ -90,6 +92,14 @@ SIGNATURES = {
                                    c_int_p, ct.c_int, c_int_p, c_int_p, c_double_p, c_void_p]),
     "pp_fft3d_batched": (ct.c_int, [c_void_p, c_double_p, c_int_p, ct.c_int, c_void_p, c_void_p, ct.c_long,
                                     ct.c_int, c_void_p, ct.c_int]),
+    "pp_augmentation_offsite": (ct.c_int, [c_void_p, ct.c_int, ct.c_int, ct.c_int, c_void_p, ct.c_int, ct.c_int,
+                                           c_int_p, c_void_p, ct.c_int, ct.c_int, c_int_p, ct.c_int, c_int_p,
+                                           c_int_p, c_void_p, ct.POINTER(ct.c_long), c_double_p, c_double_p,
+                                           c_void_p]),
+    "pp_sbt_forward": (ct.c_int, [ct.c_double, ct.c_double, ct.c_int, ct.c_int, c_double_p, c_double_p, ct.c_int,
+                                  c_double_p, c_double_p]),
+    "pp_smooth_diffwave": (ct.c_int, [ct.c_int, c_double_p, c_double_p, ct.c_int, ct.c_int, ct.c_double, ct.c_int,
+                                      c_double_p, c_double_p, c_double_p]),
 }
 
 

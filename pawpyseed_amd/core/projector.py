@@ -1,12 +1,14 @@
 """Projector: band projections of one structure onto another's bands.
 
 Mirrors pawpyseed/core/projector.py (Projector, make_site_lists,
-single_band_projection, proportion_conduction, defect_band_analysis) with
-the 'aug_real' and 'pseudo' methods.  The pseudo-wavefunction overlap and
-the O_M augmentation run in libpawpy_amd.so; the O_R/O_S/O_N terms for
-sites that differ between the structures need overlap_setup_real, which
-this build does not provide, so a Projector whose site lists leave sites
-unmatched raises PAWpyError unless explicit ``site_lists`` are given.
+setup_overlap, single_band_projection, proportion_conduction,
+defect_band_analysis) with the 'aug_real' and 'pseudo' methods.  The
+pseudo-wavefunction overlap, the on-site O_M term and the O_R/O_S terms of
+sites that differ between the structures run in libpawpy_amd.so
+(overlap_setup_real, compensation_terms).  The off-site O_N term of
+N_RS pairs (overlapping augmentation spheres of unmatched sites) is not
+provided: a Projector with N_RS pairs raises PAWpyError unless
+``site_lists`` are given.
 """
 
 from __future__ import annotations
@@ -69,18 +71,33 @@ class Projector:
         return M_R, M_S, N_R, N_S, N_RS
 
     def setup_overlap(self):
+        """projector.py:162-187 and pawpyc.pyx:651-683.  ``site_lists`` may
+        be (M_R, M_S) (on-site term only) or (M_R, M_S, N_R, N_S, N_RS)."""
         if self._site_lists is not None:
-            M_R, M_S = self._site_lists
-            N_R, N_S, N_RS = [], [], []
+            sl = list(self._site_lists)
+            M_R, M_S = sl[0], sl[1]
+            N_R, N_S, N_RS = (sl[2], sl[3], sl[4]) if len(sl) == 5 else ([], [], [])
         else:
             M_R, M_S, N_R, N_S, N_RS = self.make_site_lists()
-        if N_R or N_S or N_RS:
-            raise PAWpyError("the structures differ at %d/%d sites; the O_R/O_S/O_N terms need "
-                             "overlap_setup_real, which this build does not provide (pass site_lists "
-                             "to use the on-site O_M term only)" % (len(N_R), len(N_S)))
+        if len(N_RS):
+            raise PAWpyError("%d pairs of unmatched sites have overlapping augmentation spheres; the off-site "
+                             "O_N term they need is not provided by this build (pass site_lists to select "
+                             "the terms)" % len(N_RS))
         self.M_R = np.asarray(M_R, np.int32)
         self.M_S = np.asarray(M_S, np.int32)
-        self.site_cat = [self.M_R, self.M_S, [], [], [], []]
+        self.N_R = np.asarray(N_R, np.int32)
+        self.N_S = np.asarray(N_S, np.int32)
+        self.site_cat = [self.M_R, self.M_S, self.N_R, self.N_S, [], []]
+        if len(self.N_R) or len(self.N_S):
+            from ._lib import c_double_p, c_int_p, check, lib, ptr
+            nr = self.N_R if len(self.N_R) else None
+            ns = self.N_S if len(self.N_S) else None
+            check(lib().overlap_setup_real(self.basis.wf_ptr, self.wf.wf_ptr, ptr(self.basis.nums, c_int_p),
+                                           ptr(self.wf.nums, c_int_p), ptr(self.basis.coords, c_double_p),
+                                           ptr(self.wf.coords, c_double_p),
+                                           None if nr is None else ptr(nr, c_int_p),
+                                           None if ns is None else ptr(ns, c_int_p), None, None,
+                                           len(self.N_R), len(self.N_S), 0), "overlap_setup_real")
 
     def _single_band_projection_pseudo(self, band_num, flip_spin=False):
         return self.wf.pseudoprojection(band_num, self.basis, flip_spin)
@@ -91,17 +108,20 @@ class Projector:
         return res
 
     def _add_augmentation_terms(self, res, band_num, flip_spin):
-        """pawpyc.pyx:685-702 -> compensation_terms."""
+        """pawpyc.pyx:685-702 -> compensation_terms (O_M, O_R, O_S)."""
         import ctypes as ct
 
-        from ._lib import as_i32, c_double_p, c_int_p, check, lib, ptr
-        mr, ms = as_i32(self.M_R), as_i32(self.M_S)
+        from ._lib import c_double_p, c_int_p, check, lib, ptr
+
+        def arr(a):
+            return ptr(a, c_int_p) if len(a) else None
+
         check(lib().compensation_terms(res.ctypes.data_as(ct.c_void_p), int(band_num), self.wf.wf_ptr,
-                                       self.basis.wf_ptr, len(mr), 0, 0, 0, ptr(mr, c_int_p), ptr(ms, c_int_p),
-                                       None, None, None, None, ptr(self.wf.nums, c_int_p),
-                                       ptr(self.wf.coords, c_double_p), ptr(self.basis.nums, c_int_p),
-                                       ptr(self.basis.coords, c_double_p), ptr(self.wf.dimv, c_int_p),
-                                       int(bool(flip_spin))), "compensation_terms")
+                                       self.basis.wf_ptr, len(self.M_R), len(self.N_R), len(self.N_S), 0,
+                                       arr(self.M_R), arr(self.M_S), arr(self.N_R), arr(self.N_S), None, None,
+                                       ptr(self.wf.nums, c_int_p), ptr(self.wf.coords, c_double_p),
+                                       ptr(self.basis.nums, c_int_p), ptr(self.basis.coords, c_double_p),
+                                       ptr(self.wf.dimv, c_int_p), int(bool(flip_spin))), "compensation_terms")
 
     def single_band_projection(self, band_num, **kwargs):
         """projector.py:238-272: overlaps <basis; b,k,s | wf; band_num,k,s>

@@ -264,6 +264,10 @@ int pp_ctx_set_param(pp_ctx *ctx, const char *name, int value) {
   std::string n(name);
   if (n == "fft_pingpong")
     ctx->tune_fft_pingpong = value != 0;
+  else if (n == "gemm_f64_valu")
+    ctx->tune_gemm_f64_valu = value != 0;
+  else if (n == "fft_persistent")
+    ctx->tune_fft_persistent = value < 0 ? 0 : value;
   else if (n == "fft_tile")
     return pp_ctx_set_tuning(ctx, ctx->tune_fft_bands, value, ctx->tune_fft_prune, ctx->tune_sphere_parts,
                              ctx->tune_splitk);

@@ -1,4 +1,5 @@
-// PAW augmentation O_M for every band pair (projector.c:890-910), FP64.
+// PAW augmentation terms for every band pair, FP64: the on-site O_M
+// (projector.c:890-910) and the off-site O_N (projector.c:940-959).
 //
 // O_M[k][r][s] = sum_{t in M} sum_{i,j: (l_i,m_i) = (l_j,m_j)}
 //     conj(P_R[kR][r][offR(t)+i]) dO[n_i][n_j] P_S[k][s][offS(t)+j].
@@ -27,9 +28,10 @@ __global__ __launch_bounds__(256) void k_aug_gather(int nkt, int nbR, int KM, co
   X[t] = PR[((long)kmap[k] * nbR + b) * ptotR + gR[i]];
 }
 
-__global__ __launch_bounds__(256) void k_aug_apply(int nkt, int nbS, int KM, const cplx<double> *__restrict__ PS,
-                                                   long ptotS, const int *__restrict__ rowptr,
-                                                   const int *__restrict__ colS, const double *__restrict__ coef,
+__global__ __launch_bounds__(256) void k_aug_apply(int nkt, int nbS, int KM, int nnz,
+                                                   const cplx<double> *__restrict__ PS, long ptotS,
+                                                   const int *__restrict__ rowptr, const int *__restrict__ colS,
+                                                   const cplx<double> *__restrict__ coef,
                                                    cplx<double> *__restrict__ Y) {
   long t = (long)blockIdx.x * 256 + threadIdx.x;
   long total = (long)nkt * nbS * KM;
@@ -37,11 +39,69 @@ __global__ __launch_bounds__(256) void k_aug_apply(int nkt, int nbS, int KM, con
     return;
   int i = (int)(t % KM);
   long r = t / KM;  // = k * nbS + s
+  const int k = (int)(r / nbS);
   const cplx<double> *ps = PS + r * ptotS;
+  const cplx<double> *ck = coef + (long)k * nnz;
   cplx<double> acc = mk<double>(0, 0);
   for (int q = rowptr[i]; q < rowptr[i + 1]; q++)
-    acc = acc + scale(ps[colS[q]], coef[q]);
+    acc = acc + ck[q] * ps[colS[q]];
   Y[t] = acc;
+}
+
+// out[k] += conj(X[k]) Y[k]^T over the KM gathered R projector rows gR,
+// with Y[k][s][i] = sum_q coef[k][q] PS[k][s][colS[q]] (CSR rows rowptr).
+static int aug_contract(pp_ctx *ctx, int nwk, int nspin, int flip, const void *projR_dev, int nbR, int ptotR,
+                        const void *projS_dev, int nbS, int ptotS, const std::vector<int> &gR,
+                        const std::vector<int> &rowptr, const std::vector<int> &colS,
+                        const std::vector<cplx<double>> &coef, void *out_dev) {
+  const int KM = (int)gR.size();
+  const int nkt = nwk * nspin;
+  const int nnz = (int)colS.size();
+  if (KM == 0)
+    return PP_OK;
+  std::vector<int> kmap(nkt);
+  for (int k = 0; k < nkt; k++)
+    kmap[k] = (flip && nspin == 2) ? (k < nwk ? k + nwk : k - nwk) : k;
+  // device staging: ints, then coefficients, then X and Y
+  size_t nints = gR.size() + rowptr.size() + std::max<size_t>(1, colS.size()) + kmap.size();
+  size_t ibytes = (nints * sizeof(int) + 255) / 256 * 256;
+  size_t cbytes = (std::max<size_t>(1, coef.size()) * sizeof(cplx<double>) + 255) / 256 * 256;
+  size_t xbytes = sizeof(cplx<double>) * (size_t)nkt * nbR * KM;
+  size_t ybytes = sizeof(cplx<double>) * (size_t)nkt * nbS * KM;
+  PP_TRY(ctx->aug_ws.ensure(ibytes + cbytes + xbytes + ybytes));
+  char *base = (char *)ctx->aug_ws.p;
+  std::vector<int> ints;
+  ints.insert(ints.end(), gR.begin(), gR.end());
+  ints.insert(ints.end(), rowptr.begin(), rowptr.end());
+  ints.insert(ints.end(), colS.begin(), colS.end());
+  if (colS.empty())
+    ints.push_back(0);
+  ints.insert(ints.end(), kmap.begin(), kmap.end());
+  int *d_gR = (int *)base;
+  int *d_rowptr = d_gR + gR.size();
+  int *d_colS = d_rowptr + rowptr.size();
+  int *d_kmap = d_colS + std::max<size_t>(1, colS.size());
+  cplx<double> *d_coef = (cplx<double> *)(base + ibytes);
+  cplx<double> *X = (cplx<double> *)(base + ibytes + cbytes);
+  cplx<double> *Y = (cplx<double> *)(base + ibytes + cbytes + xbytes);
+  PP_CHECK_HIP(hipMemcpyAsync(base, ints.data(), ints.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  if (!coef.empty())
+    PP_CHECK_HIP(hipMemcpyAsync(d_coef, coef.data(), coef.size() * sizeof(cplx<double>), hipMemcpyHostToDevice,
+                                ctx->stream));
+  hipEvent_t ev;
+  long tx = (long)nkt * nbR * KM, ty = (long)nkt * nbS * KM;
+  tbegin(ctx, K_AUG_PREP, &ev);
+  hipLaunchKernelGGL(k_aug_gather, dim3((unsigned)((tx + 255) / 256)), dim3(256), 0, ctx->stream, nkt, nbR, KM,
+                     d_kmap, (const cplx<double> *)projR_dev, (long)ptotR, d_gR, X);
+  hipLaunchKernelGGL(k_aug_apply, dim3((unsigned)((ty + 255) / 256)), dim3(256), 0, ctx->stream, nkt, nbS, KM, nnz,
+                     (const cplx<double> *)projS_dev, (long)ptotS, d_rowptr, d_colS, d_coef, Y);
+  tend(ctx, K_AUG_PREP, ev);
+  PP_CHECK_HIP(hipGetLastError());
+  PP_TRY(zgemm_nt(ctx, PP_FP64, nbR, nbS, KM, 0, X, KM, (long)nbR * KM, 0, Y, KM, (long)nbS * KM, out_dev, nbS,
+                  (long)nbR * nbS, nkt, nullptr, nullptr, nullptr, true));
+  // the host staging vectors must outlive the async copies
+  PP_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  return PP_OK;
 }
 
 }  // namespace pp
@@ -105,52 +165,61 @@ extern "C" int pp_augmentation(pp_ctx *ctx, int nwk, int nspin, int flip, const 
       rowptr.push_back((int)colS.size());
     }
   }
-  const int KM = (int)gR.size();
+  // the same real coefficients at every k
   const int nkt = nwk * nspin;
-  std::vector<int> kmap(nkt);
+  std::vector<cplx<double>> cf((size_t)nkt * coef.size());
   for (int k = 0; k < nkt; k++)
-    kmap[k] = (flip && nspin == 2) ? (k < nwk ? k + nwk : k - nwk) : k;
-  // device staging: ints then doubles, then X and Y
-  size_t nints = gR.size() + rowptr.size() + std::max<size_t>(1, colS.size()) + kmap.size();
-  size_t ibytes = (nints * sizeof(int) + 255) / 256 * 256;
-  size_t dbytes = (std::max<size_t>(1, coef.size()) * sizeof(double) + 255) / 256 * 256;
-  size_t xbytes = sizeof(cplx<double>) * (size_t)nkt * nbR * KM;
-  size_t ybytes = sizeof(cplx<double>) * (size_t)nkt * nbS * KM;
-  PP_TRY(ctx->aug_ws.ensure(ibytes + dbytes + xbytes + ybytes));
-  char *base = (char *)ctx->aug_ws.p;
-  std::vector<int> ints;
-  ints.insert(ints.end(), gR.begin(), gR.end());
-  ints.insert(ints.end(), rowptr.begin(), rowptr.end());
-  ints.insert(ints.end(), colS.begin(), colS.end());
-  if (colS.empty())
-    ints.push_back(0);
-  ints.insert(ints.end(), kmap.begin(), kmap.end());
-  int *d_gR = (int *)base;
-  int *d_rowptr = d_gR + gR.size();
-  int *d_colS = d_rowptr + rowptr.size();
-  int *d_kmap = d_colS + std::max<size_t>(1, colS.size());
-  double *d_coef = (double *)(base + ibytes);
-  cplx<double> *X = (cplx<double> *)(base + ibytes + dbytes);
-  cplx<double> *Y = (cplx<double> *)(base + ibytes + dbytes + xbytes);
-  PP_CHECK_HIP(hipMemcpyAsync(base, ints.data(), ints.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
-  if (!coef.empty())
-    PP_CHECK_HIP(hipMemcpyAsync(d_coef, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice,
-                                ctx->stream));
-  hipEvent_t ev;
-  long tx = (long)nkt * nbR * KM, ty = (long)nkt * nbS * KM;
-  tbegin(ctx, K_AUG_PREP, &ev);
-  hipLaunchKernelGGL(k_aug_gather, dim3((unsigned)((tx + 255) / 256)), dim3(256), 0, ctx->stream, nkt, nbR, KM,
-                     d_kmap, (const cplx<double> *)projR_dev, (long)ptotR, d_gR, X);
-  hipLaunchKernelGGL(k_aug_apply, dim3((unsigned)((ty + 255) / 256)), dim3(256), 0, ctx->stream, nkt, nbS, KM,
-                     (const cplx<double> *)projS_dev, (long)ptotS, d_rowptr, d_colS, d_coef, Y);
-  tend(ctx, K_AUG_PREP, ev);
-  PP_CHECK_HIP(hipGetLastError());
-  // out[k] += conj(X[k]) Y[k]^T  (FP64)
-  PP_TRY(zgemm_nt(ctx, PP_FP64, nbR, nbS, KM, 0, X, KM, (long)nbR * KM, 0, Y, KM, (long)nbS * KM, out_dev, nbS,
-                  (long)nbR * nbS, nkt, nullptr, nullptr, nullptr, true));
-  // the host staging vectors must outlive the async copies
-  PP_CHECK_HIP(hipStreamSynchronize(ctx->stream));
-  return PP_OK;
+    for (size_t q = 0; q < coef.size(); q++)
+      cf[(size_t)k * coef.size() + q] = mk<double>(coef[q], 0.0);
+  return aug_contract(ctx, nwk, nspin, flip, projR_dev, nbR, ptotR, projS_dev, nbS, ptotS, gR, rowptr, colS, cf,
+                      out_dev);
+}
+
+extern "C" int pp_augmentation_offsite(pp_ctx *ctx, int nwk, int nspin, int flip, const void *projR_dev, int nbR,
+                                       int ptotR, const int *proj_off_R, const void *projS_dev, int nbS, int ptotS,
+                                       const int *proj_off_S, int num_pairs, const int *N_RS_R, const int *N_RS_S,
+                                       const pp_cdouble *ov, const long *ov_off, const double *dcoords,
+                                       const double *kpts, void *out_dev) {
+  PP_REQUIRE(ctx && nwk > 0 && (nspin == 1 || nspin == 2) && nbR >= 0 && nbS >= 0 && num_pairs >= 0, PP_ERR_ARG,
+             "pp_augmentation_offsite: bad arguments");
+  if (num_pairs == 0 || nbR == 0 || nbS == 0)
+    return PP_OK;
+  PP_REQUIRE(projR_dev && projS_dev && out_dev && proj_off_R && proj_off_S && N_RS_R && N_RS_S && ov && ov_off &&
+                 dcoords && kpts,
+             PP_ERR_ARG, "pp_augmentation_offsite: null argument");
+  PP_CHECK_HIP(hipSetDevice(ctx->device));
+  const int nkt = nwk * nspin;
+  std::vector<int> gR, rowptr(1, 0), colS;
+  std::vector<std::pair<int, long>> src;  // (pair, entry of its overlap block) per nonzero
+  for (int t = 0; t < num_pairs; t++) {
+    const int s1 = N_RS_R[t], s2 = N_RS_S[t];
+    const int n1 = proj_off_R[s1 + 1] - proj_off_R[s1], n2 = proj_off_S[s2 + 1] - proj_off_S[s2];
+    PP_REQUIRE(n1 >= 0 && n2 >= 0 && proj_off_R[s1] + n1 <= ptotR && proj_off_S[s2] + n2 <= ptotS, PP_ERR_ARG,
+               "pp_augmentation_offsite: projector offsets exceed the projection table");
+    PP_REQUIRE(ov_off[t + 1] - ov_off[t] == (long)n1 * n2, PP_ERR_ARG,
+               "pp_augmentation_offsite: overlap block size differs from the projector counts");
+    for (int i = 0; i < n1; i++) {
+      gR.push_back(proj_off_R[s1] + i);
+      for (int j = 0; j < n2; j++) {
+        colS.push_back(proj_off_S[s2] + j);
+        src.emplace_back(t, ov_off[t] + (long)i * n2 + j);
+      }
+      rowptr.push_back((int)colS.size());
+    }
+  }
+  // coef[k][q] = Ov e^{2 pi i kR . dcoord} with kR the (spin-flipped) k of R
+  std::vector<cplx<double>> cf((size_t)nkt * src.size());
+  for (int k = 0; k < nkt; k++) {
+    const int kR = (flip && nspin == 2) ? (k < nwk ? k + nwk : k - nwk) : k;
+    for (size_t q = 0; q < src.size(); q++) {
+      const double *dc = dcoords + 3 * src[q].first;
+      const double a = 2.0 * M_PI * (kpts[3 * kR] * dc[0] + kpts[3 * kR + 1] * dc[1] + kpts[3 * kR + 2] * dc[2]);
+      const pp_cdouble o = ov[src[q].second];
+      cf[(size_t)k * src.size() + q] = mk<double>(o.real(), o.imag()) * mk<double>(cos(a), sin(a));
+    }
+  }
+  return aug_contract(ctx, nwk, nspin, flip, projR_dev, nbR, ptotR, projS_dev, nbS, ptotS, gR, rowptr, colS, cf,
+                      out_dev);
 }
 
 extern "C" int pp_fft3d_batched(pp_ctx *ctx, const double *lattice, const int *fftg, int nw, const int *G_dev,

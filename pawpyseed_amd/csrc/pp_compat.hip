@@ -36,6 +36,23 @@ struct WfDev {
   std::vector<double> sp_dO;  // aepw - pspw per species
   std::map<std::tuple<const void *, int, int>, PairCache> overlap_cache;
   std::map<std::tuple<const void *, int, unsigned long long>, PairCache> aug_cache;
+  // <(phi - phi~)_smooth | psi~> at the other structure's unmatched sites
+  // (overlap_setup_real): c128 [nkt][nband][wp_ptot]; wp_labels index the
+  // other structure's species
+  DevBuf wp;
+  int wp_n = 0, wp_ptot = 0;
+  const void *wp_of = nullptr;  // the pswf_t whose sites they are
+  std::vector<int> wp_off, wp_labels, wp_nchan, wp_ls;
+  unsigned long long wp_hash = 0;
+  // off-site overlap blocks of the N_RS pairs (overlap_setup_real), kept on
+  // the projected structure: conj of the partial-wave overlaps, [P_R x P_S]
+  // per pair at on_off[t], and the minimum-image pair vectors
+  int on_n = 0;
+  const void *on_of = nullptr;
+  std::vector<int> on_R, on_S;
+  std::vector<pp_cdouble> on_ov;
+  std::vector<long> on_off;
+  std::vector<double> on_dc;
 };
 
 static std::mutex g_mu;
@@ -475,7 +492,6 @@ void free_ptr(void *ptr) { free(ptr); }
 
 ppot_t *get_projector_list(int num_els, int *labels, int *ls, double *wave_grids, double *projectors,
                            double *aewaves, double *pswaves, double *rmaxs, double grid_encut) {
-  (void)grid_encut;  // used only by the SBT filter of the O_R/O_S terms
   if (num_els <= 0 || !labels || !ls || !wave_grids || !projectors || !aewaves || !pswaves || !rmaxs) {
     set_error(PP_ERR_ARG, "get_projector_list: bad arguments");
     return nullptr;
@@ -524,6 +540,21 @@ ppot_t *get_projector_list(int num_els, int *labels, int *ls, double *wave_grids
       f->pswave_spline = spline3(pp->wave_grid, f->pswave, nw);
       f->diffwave_spline = spline3(pp->wave_grid, f->diffwave, nw);
     }
+    // projector.c:114-163: transform of every partial-wave difference and
+    // its plane-wave-filtered form on the smooth grid (the O_R/O_S tables)
+    for (int k = 0; k < pp->num_projs; k++) {
+      funcset_t *f = pp->funcs + k;
+      f->kwave = (double *)malloc(sizeof(double) * nw);
+      f->smooth_diffwave = (double *)malloc(sizeof(double) * np);
+      if (smooth_diffwave_host(nw, pp->wave_grid, f->diffwave, f->l, pp->lmax, grid_encut, np, f->smooth_diffwave,
+                               pp->kwave_grid, f->kwave) != PP_OK) {
+        free_ppot_list(pps, i + 1);
+        set_error(PP_ERR_ARG, "get_projector_list: partial-wave transform failed");
+        return nullptr;
+      }
+      f->kwave_spline = spline3(pp->kwave_grid, f->kwave, nw);
+      f->smooth_diffwave_spline = spline3(pp->smooth_grid, f->smooth_diffwave, np);
+    }
     // projector.c:507-558: same-l spline integrals, symmetric
     int n = pp->num_projs;
     pp->pspw_overlap_matrix = (double *)calloc((size_t)n * n, sizeof(double));
@@ -552,6 +583,114 @@ ppot_t *get_projector_list(int num_els, int *labels, int *ls, double *wave_grids
   return pps;
 }
 
+}  // extern "C"
+
+namespace pp {
+
+// Species arrays of a ppot_t list for pp_sphere_create: the projectors
+// (smooth = 0) or the filtered partial-wave differences on the smooth grid
+// with radius wave_rmax (smooth = 1, utils.c:601-604 and :674-679).
+struct SpeciesArrays {
+  std::vector<int> nchan, ngrid, ls;
+  std::vector<double> rmax, vals, dO, ident;
+};
+
+static void species_arrays(const ppot_t *pps, int num_elems, int smooth, SpeciesArrays &a) {
+  a.nchan.assign(num_elems, 0);
+  a.ngrid.assign(num_elems, 0);
+  a.rmax.assign(num_elems, 0.0);
+  for (int e = 0; e < num_elems; e++) {
+    const ppot_t &pp = pps[e];
+    a.nchan[e] = pp.num_projs;
+    a.ngrid[e] = pp.proj_gridsize;
+    a.rmax[e] = smooth ? pp.wave_rmax : pp.rmax;
+    for (int ch = 0; ch < pp.num_projs; ch++) {
+      a.ls.push_back(pp.funcs[ch].l);
+      const double *v = smooth ? pp.funcs[ch].smooth_diffwave : pp.funcs[ch].proj;
+      a.vals.insert(a.vals.end(), v, v + pp.proj_gridsize);
+    }
+    for (int i = 0; i < pp.num_projs * pp.num_projs; i++) {
+      a.dO.push_back(pp.aepw_overlap_matrix[i] - pp.pspw_overlap_matrix[i]);
+      a.ident.push_back(i % (pp.num_projs + 1) == 0 ? 1.0 : 0.0);
+    }
+  }
+}
+
+// <table_j | psi~_nk> for every band and k of wf at the given sites (grid
+// and lattice of wf), into out c128 [nkt][nband][ptot].
+static int project_sites(pp_ctx *ctx, pswf_t *wf, const SpeciesArrays &a, int num_elems, int nsites,
+                         const int *labels, const double *coords, int prec, DevBuf &out, int &ptot,
+                         std::vector<int> &proj_off) {
+  WfDev *d = dev_of(wf);
+  pp_sphere *sp = nullptr;
+  PP_TRY(pp_sphere_create(ctx, &sp, wf->lattice, wf->fftg, nsites, labels, coords, num_elems, a.nchan.data(),
+                          a.ls.data(), a.rmax.data(), a.ngrid.data(), a.vals.data(), prec));
+  ptot = 0;
+  proj_off.assign(nsites + 1, 0);
+  pp_sphere_info(sp, &ptot, nullptr, proj_off.data());
+  int st = out.ensure(sizeof(pp_cdouble) * (size_t)d->nkt * d->nband * std::max(1, ptot));
+  std::vector<double> kf(3 * (size_t)d->nkt);
+  for (int k = 0; k < d->nkt; k++)
+    for (int j = 0; j < 3; j++)
+      kf[3 * k + j] = wf->kpts[k]->k[j];
+  if (st == PP_OK)
+    st = pp_setup_projections_batched(ctx, sp, d->nkt, kf.data(), d->gofs.data(), (const int *)d->G.p, d->C.p,
+                                      d->ldc, d->nband, out.p, prec);
+  if (st == PP_OK)
+    st = pp_ctx_synchronize(ctx);
+  pp_sphere_destroy(sp);
+  return st;
+}
+
+// Host projection_t lists of the reference (per band and k) from a device
+// table [nkt][nband][ptot].
+static int fill_host_lists(pswf_t *wf, const DevBuf &tab, int ptot, int nsites, const int *labels,
+                           const std::vector<int> &proj_off, const ppot_t *pps, bool wave) {
+  WfDev *d = dev_of(wf);
+  std::vector<pp_cdouble> host((size_t)d->nkt * d->nband * std::max(1, ptot));
+  PP_CHECK_HIP(hipMemcpy(host.data(), tab.p, sizeof(pp_cdouble) * host.size(), hipMemcpyDeviceToHost));
+  for (int k = 0; k < d->nkt; k++)
+    for (int b = 0; b < d->nband; b++) {
+      projection_t *pl = (projection_t *)calloc(std::max(1, nsites), sizeof(projection_t));
+      const pp_cdouble *row = host.data() + ((size_t)k * d->nband + b) * ptot;
+      for (int s = 0; s < nsites; s++) {
+        const ppot_t &pp = pps[labels[s]];
+        int P = proj_off[s + 1] - proj_off[s];
+        pl[s].num_projs = pp.num_projs;
+        pl[s].total_projs = P;
+        pl[s].ns = (int *)malloc(sizeof(int) * P);
+        pl[s].ls = (int *)malloc(sizeof(int) * P);
+        pl[s].ms = (int *)malloc(sizeof(int) * P);
+        pl[s].overlaps = (pp_cdouble *)malloc(sizeof(pp_cdouble) * P);
+        int p = 0;
+        for (int ch = 0; ch < pp.num_projs; ch++) {
+          int l = pp.funcs[ch].l;
+          for (int m = -l; m <= l; m++, p++) {
+            pl[s].ns[p] = ch;
+            pl[s].ls[p] = l;
+            pl[s].ms[p] = m;
+            pl[s].overlaps[p] = row[proj_off[s] + p];
+          }
+        }
+      }
+      if (wave)
+        wf->kpts[k]->bands[b]->wave_projections = pl;
+      else
+        wf->kpts[k]->bands[b]->projections = pl;
+    }
+  return PP_OK;
+}
+
+static unsigned long long hash_ints(unsigned long long h, const int *v, int n) {
+  for (int i = 0; i < n; i++)
+    h = (h ^ (unsigned)v[i]) * 1099511628211ull;
+  return (h ^ (unsigned long long)n) * 1099511628211ull;
+}
+
+}  // namespace pp
+
+extern "C" {
+
 int setup_projections(pswf_t *wf, ppot_t *pps, int num_elems, int num_sites, int *fftg, int *labels,
                       double *coords) {
   PP_REQUIRE(wf && pps && num_elems > 0 && num_sites >= 0 && fftg && (num_sites == 0 || (labels && coords)),
@@ -578,79 +717,174 @@ int setup_projections(pswf_t *wf, ppot_t *pps, int num_elems, int num_sites, int
   memcpy(wf->fftg, fftg, 3 * sizeof(int));
   PP_TRY(ensure_upload(ctx, wf));
   WfDev *d = dev_of(wf);
-  // species tables
-  std::vector<int> nchan(num_elems), ngrid(num_elems), lsv;
-  std::vector<double> rmax(num_elems), proj, dO;
-  for (int e = 0; e < num_elems; e++) {
-    nchan[e] = pps[e].num_projs;
-    ngrid[e] = pps[e].proj_gridsize;
-    rmax[e] = pps[e].rmax;
-    for (int ch = 0; ch < nchan[e]; ch++) {
-      lsv.push_back(pps[e].funcs[ch].l);
-      proj.insert(proj.end(), pps[e].funcs[ch].proj, pps[e].funcs[ch].proj + ngrid[e]);
-    }
-    for (int i = 0; i < nchan[e] * nchan[e]; i++)
-      dO.push_back(pps[e].aepw_overlap_matrix[i] - pps[e].pspw_overlap_matrix[i]);
-  }
+  SpeciesArrays a;
+  species_arrays(pps, num_elems, 0, a);
   int prec = g_precision == PP_FP64 ? PP_FP64 : PP_FP32;
-  pp_sphere *sp = nullptr;
-  PP_TRY(pp_sphere_create(ctx, &sp, wf->lattice, fftg, num_sites, labels, coords, num_elems, nchan.data(),
-                          lsv.data(), rmax.data(), ngrid.data(), proj.data(), prec));
   int ptot = 0;
-  std::vector<int> proj_off(num_sites + 1);
-  pp_sphere_info(sp, &ptot, nullptr, proj_off.data());
-  int st = d->proj.ensure(sizeof(pp_cdouble) * (size_t)d->nkt * d->nband * std::max(1, ptot));
-  std::vector<double> kf(3 * (size_t)d->nkt);
-  for (int k = 0; k < d->nkt; k++)
-    for (int j = 0; j < 3; j++)
-      kf[3 * k + j] = wf->kpts[k]->k[j];
-  if (st == PP_OK)
-    st = pp_setup_projections_batched(ctx, sp, d->nkt, kf.data(), d->gofs.data(), (const int *)d->G.p, d->C.p,
-                                      d->ldc, d->nband, d->proj.p, prec);
-  if (st == PP_OK)
-    st = pp_ctx_synchronize(ctx);
-  pp_sphere_destroy(sp);
-  PP_TRY(st);
+  std::vector<int> proj_off;
+  PP_TRY(project_sites(ctx, wf, a, num_elems, num_sites, labels, coords, prec, d->proj, ptot, proj_off));
   d->ptot = ptot;
   d->nsites = num_sites;
   d->proj_off = proj_off;
   d->labels.assign(labels, labels + num_sites);
-  d->sp_nchan = nchan;
-  d->sp_ls = lsv;
-  d->sp_dO = dO;
+  d->sp_nchan = a.nchan;
+  d->sp_ls = a.ls;
+  d->sp_dO = a.dO;
   d->proj_precision = prec;
   d->aug_cache.clear();
-  // host copies in the reference's per-band projection_t lists
-  std::vector<pp_cdouble> host((size_t)d->nkt * d->nband * std::max(1, ptot));
-  PP_CHECK_HIP(hipMemcpy(host.data(), d->proj.p, sizeof(pp_cdouble) * host.size(), hipMemcpyDeviceToHost));
-  std::vector<int> lofs(num_elems + 1, 0);
-  for (int e = 0; e < num_elems; e++)
-    lofs[e + 1] = lofs[e] + nchan[e];
-  for (int k = 0; k < d->nkt; k++)
-    for (int b = 0; b < d->nband; b++) {
-      projection_t *pl = (projection_t *)calloc(std::max(1, num_sites), sizeof(projection_t));
-      const pp_cdouble *row = host.data() + ((size_t)k * d->nband + b) * ptot;
-      for (int s = 0; s < num_sites; s++) {
-        int e = labels[s], P = proj_off[s + 1] - proj_off[s];
-        pl[s].num_projs = nchan[e];
-        pl[s].total_projs = P;
-        pl[s].ns = (int *)malloc(sizeof(int) * P);
-        pl[s].ls = (int *)malloc(sizeof(int) * P);
-        pl[s].ms = (int *)malloc(sizeof(int) * P);
-        pl[s].overlaps = (pp_cdouble *)malloc(sizeof(pp_cdouble) * P);
-        int p = 0;
-        for (int ch = 0; ch < nchan[e]; ch++) {
-          int l = lsv[lofs[e] + ch];
-          for (int m = -l; m <= l; m++, p++) {
-            pl[s].ns[p] = ch;
-            pl[s].ls[p] = l;
-            pl[s].ms[p] = m;
-            pl[s].overlaps[p] = row[proj_off[s] + p];
+  return fill_host_lists(wf, d->proj, ptot, num_sites, labels, proj_off, pps, false);
+}
+
+int overlap_setup_real(pswf_t *wf_R, pswf_t *wf_S, int *labels_R, int *labels_S, double *coords_R,
+                       double *coords_S, int *N_R, int *N_S, int *N_RS_R, int *N_RS_S, int num_N_R, int num_N_S,
+                       int num_N_RS) {
+  PP_REQUIRE(wf_R && wf_S && num_N_R >= 0 && num_N_S >= 0 && num_N_RS >= 0, PP_ERR_ARG,
+             "overlap_setup_real: bad arguments");
+  PP_REQUIRE(num_N_RS == 0 || (N_RS_R && N_RS_S && labels_R && labels_S && coords_R && coords_S), PP_ERR_ARG,
+             "overlap_setup_real: null N_RS data");
+  PP_REQUIRE(wf_R->pps && wf_S->pps && wf_R->fftg && wf_S->fftg, PP_ERR_ARG,
+             "overlap_setup_real: call setup_projections on both wavefunctions first");
+  PP_REQUIRE(num_N_R == 0 || (N_R && labels_R && coords_R), PP_ERR_ARG, "overlap_setup_real: null N_R data");
+  PP_REQUIRE(num_N_S == 0 || (N_S && labels_S && coords_S), PP_ERR_ARG, "overlap_setup_real: null N_S data");
+  for (int s = 0; s < num_N_R; s++)
+    PP_REQUIRE(N_R[s] >= 0 && N_R[s] < wf_R->num_sites && labels_R[N_R[s]] >= 0 &&
+                   labels_R[N_R[s]] < wf_R->num_elems,
+               PP_ERR_ARG, "overlap_setup_real: N_R site out of range");
+  for (int s = 0; s < num_N_S; s++)
+    PP_REQUIRE(N_S[s] >= 0 && N_S[s] < wf_S->num_sites && labels_S[N_S[s]] >= 0 &&
+                   labels_S[N_S[s]] < wf_S->num_elems,
+               PP_ERR_ARG, "overlap_setup_real: N_S site out of range");
+  for (int e = 0; e < wf_R->num_elems; e++)
+    PP_REQUIRE(num_N_R == 0 || wf_R->pps[e].funcs[0].smooth_diffwave, PP_ERR_ARG,
+               "overlap_setup_real: projector list without filtered partial waves");
+  for (int e = 0; e < wf_S->num_elems; e++)
+    PP_REQUIRE(num_N_S == 0 || wf_S->pps[e].funcs[0].smooth_diffwave, PP_ERR_ARG,
+               "overlap_setup_real: projector list without filtered partial waves");
+  std::lock_guard<std::mutex> lk(g_mu);
+  pp_ctx *ctx;
+  PP_TRY(global_ctx(&ctx));
+  PP_TRY(ensure_upload(ctx, wf_R));
+  PP_TRY(ensure_upload(ctx, wf_S));
+  int prec = g_precision == PP_FP64 ? PP_FP64 : PP_FP32;
+  // projector.c:604-672: S's bands onto R's filtered partial waves at the
+  // N_R sites, and R's bands onto S's at the N_S sites
+  struct Side {
+    pswf_t *wf;        // whose bands are projected
+    pswf_t *owner;     // whose sites and species
+    int n;
+    const int *sites;
+    const int *labels;
+    const double *coords;
+  } sides[2] = {{wf_S, wf_R, num_N_R, N_R, labels_R, coords_R}, {wf_R, wf_S, num_N_S, N_S, labels_S, coords_S}};
+  for (auto &sd : sides) {
+    WfDev *d = dev_of(sd.wf);
+    for (int k = 0; k < sd.wf->nwk * sd.wf->nspin; k++)
+      for (int b = 0; b < sd.wf->nband; b++) {
+        free_projection_list(sd.wf->kpts[k]->bands[b]->wave_projections, sd.wf->wp_num);
+        sd.wf->kpts[k]->bands[b]->wave_projections = nullptr;
+      }
+    sd.wf->wp_num = sd.n;
+    d->wp_n = sd.n;
+    d->wp_of = sd.owner;
+    d->aug_cache.clear();
+    d->wp_hash = hash_ints(1469598103934665603ull, sd.sites, sd.n);
+    std::vector<int> lab(sd.n);
+    std::vector<double> crd(3 * (size_t)sd.n);
+    for (int s = 0; s < sd.n; s++) {
+      lab[s] = sd.labels[sd.sites[s]];
+      for (int j = 0; j < 3; j++)
+        crd[3 * s + j] = sd.coords[3 * sd.sites[s] + j];
+    }
+    SpeciesArrays a;
+    species_arrays(sd.owner->pps, sd.owner->num_elems, 1, a);
+    d->wp_labels = lab;
+    d->wp_nchan = a.nchan;
+    d->wp_ls = a.ls;
+    if (sd.n == 0) {
+      d->wp_ptot = 0;
+      d->wp_off.assign(1, 0);
+      continue;
+    }
+    // the smooth tables live on the projected structure's grid
+    PP_TRY(project_sites(ctx, sd.wf, a, sd.owner->num_elems, sd.n, lab.data(), crd.data(), prec, d->wp, d->wp_ptot,
+                         d->wp_off));
+    PP_TRY(fill_host_lists(sd.wf, d->wp, d->wp_ptot, sd.n, lab.data(), d->wp_off, sd.owner->pps, true));
+  }
+  // projector.c:674-712: off-site overlaps of the N_RS pairs from the
+  // transforms of the partial-wave differences (radial.c:116-196)
+  for (int t = 0; t < num_N_RS; t++)
+    PP_REQUIRE(N_RS_R[t] >= 0 && N_RS_R[t] < wf_R->num_sites && N_RS_S[t] >= 0 && N_RS_S[t] < wf_S->num_sites,
+               PP_ERR_ARG, "overlap_setup_real: N_RS site out of range");
+  if (wf_S->overlaps) {
+    for (int i = 0; i < wf_S->num_aug_overlap_sites; i++)
+      free(wf_S->overlaps[i]);
+    free(wf_S->overlaps);
+    wf_S->overlaps = nullptr;
+  }
+  free(wf_S->dcoords);
+  wf_S->dcoords = nullptr;
+  WfDev *dS = dev_of(wf_S);
+  dS->on_n = num_N_RS;
+  dS->on_of = wf_R;
+  dS->on_R.assign(N_RS_R, N_RS_R + num_N_RS);
+  dS->on_S.assign(N_RS_S, N_RS_S + num_N_RS);
+  dS->on_ov.clear();
+  dS->on_off.assign(1, 0);
+  dS->on_dc.assign(3 * (size_t)num_N_RS, 0.0);
+  dS->aug_cache.clear();
+  if (num_N_RS > 0) {
+    wf_S->overlaps = (pp_cdouble **)calloc(num_N_RS, sizeof(pp_cdouble *));
+    wf_S->dcoords = (double *)malloc(3 * sizeof(double) * num_N_RS);
+  }
+  for (int t = 0; t < num_N_RS; t++) {
+    const int s1 = N_RS_R[t], s2 = N_RS_S[t];
+    const ppot_t &p1 = wf_R->pps[labels_R[s1]];
+    const ppot_t &p2 = wf_S->pps[labels_S[s2]];
+    PP_REQUIRE(p1.funcs[0].kwave && p2.funcs[0].kwave, PP_ERR_ARG,
+               "overlap_setup_real: projector list without partial-wave transforms");
+    // minimum-image cartesian vector from the R site to the S site
+    double best = INFINITY, dc[3] = {0, 0, 0};
+    for (int a = -1; a <= 1; a++)
+      for (int b = -1; b <= 1; b++)
+        for (int c = -1; c <= 1; c++) {
+          const double f[3] = {coords_S[3 * s2] + a - coords_R[3 * s1], coords_S[3 * s2 + 1] + b - coords_R[3 * s1 + 1],
+                               coords_S[3 * s2 + 2] + c - coords_R[3 * s1 + 2]};
+          double v[3];
+          for (int j = 0; j < 3; j++)
+            v[j] = f[0] * wf_R->lattice[j] + f[1] * wf_R->lattice[3 + j] + f[2] * wf_R->lattice[6 + j];
+          const double r = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+          if (r < best) {
+            best = r;
+            dc[0] = v[0];
+            dc[1] = v[1];
+            dc[2] = v[2];
           }
         }
+    memcpy(wf_S->dcoords + 3 * t, dc, sizeof(dc));
+    memcpy(dS->on_dc.data() + 3 * t, dc, sizeof(dc));
+    std::vector<pp_cdouble> blk((size_t)p1.total_projs * p2.total_projs);
+    int tj = 0;
+    for (int j = 0; j < p1.num_projs; j++) {
+      const funcset_t &f1 = p1.funcs[j];
+      for (int m1 = -f1.l; m1 <= f1.l; m1++, tj++) {
+        int tk = 0;
+        for (int k = 0; k < p2.num_projs; k++) {
+          const funcset_t &f2 = p2.funcs[k];
+          for (int m2 = -f2.l; m2 <= f2.l; m2++, tk++)
+            blk[(size_t)tj * p2.total_projs + tk] =
+                std::conj(offsite_overlap_host(dc, p1.kwave_grid, f1.kwave, f1.kwave_spline, p1.wave_gridsize,
+                                               p2.kwave_grid, f2.kwave, f2.kwave_spline, p2.wave_gridsize, f1.l, m1,
+                                               f2.l, m2));
+        }
       }
-      wf->kpts[k]->bands[b]->projections = pl;
     }
+    wf_S->overlaps[t] = (pp_cdouble *)malloc(sizeof(pp_cdouble) * blk.size());
+    memcpy(wf_S->overlaps[t], blk.data(), sizeof(pp_cdouble) * blk.size());
+    dS->on_ov.insert(dS->on_ov.end(), blk.begin(), blk.end());
+    dS->on_off.push_back((long)dS->on_ov.size());
+  }
+  wf_S->num_aug_overlap_sites = num_N_RS;
+  wf_R->num_aug_overlap_sites = num_N_RS;
   return PP_OK;
 }
 
@@ -708,21 +942,19 @@ int compensation_terms(pp_cdouble *overlap, int BAND_NUM, pswf_t *wf_S, pswf_t *
                        int num_N_S, int num_N_RS, int *M_R, int *M_S, int *N_R, int *N_S, int *N_RS_R, int *N_RS_S,
                        int *proj_labels, double *proj_coords, int *ref_labels, double *ref_coords, int *fft_grid,
                        int flip_spin) {
-  (void)N_R;
-  (void)N_S;
-  (void)N_RS_R;
-  (void)N_RS_S;
   (void)proj_labels;
   (void)proj_coords;
   (void)ref_labels;
   (void)ref_coords;
   (void)fft_grid;
-  PP_REQUIRE(overlap && wf_S && wf_R, PP_ERR_ARG, "compensation_terms: null argument");
-  PP_REQUIRE(num_N_R == 0 && num_N_S == 0 && num_N_RS == 0, PP_ERR_UNSUPPORTED,
-             "compensation_terms: the O_R/O_S/O_N terms (sites that differ between the structures) need "
-             "overlap_setup_real, which this build does not provide");
+  PP_REQUIRE(overlap && wf_S && wf_R && num_M >= 0 && num_N_R >= 0 && num_N_S >= 0, PP_ERR_ARG,
+             "compensation_terms: bad arguments");
+  PP_REQUIRE(num_N_RS >= 0 && (num_N_RS == 0 || (N_RS_R && N_RS_S)), PP_ERR_ARG,
+             "compensation_terms: bad N_RS lists");
   PP_REQUIRE(BAND_NUM >= 0 && BAND_NUM < wf_S->nband, PP_ERR_ARG, "compensation_terms: band index out of range");
   PP_REQUIRE(num_M == 0 || (M_R && M_S), PP_ERR_ARG, "compensation_terms: null site list");
+  PP_REQUIRE(num_N_R == 0 || N_R, PP_ERR_ARG, "compensation_terms: null N_R list");
+  PP_REQUIRE(num_N_S == 0 || N_S, PP_ERR_ARG, "compensation_terms: null N_S list");
   std::lock_guard<std::mutex> lk(g_mu);
   WfDev *dS = (WfDev *)wf_S->dev, *dR = (WfDev *)wf_R->dev;
   PP_REQUIRE(dS && dR && dS->proj_precision >= 0 && dR->proj_precision >= 0, PP_ERR_ARG,
@@ -731,7 +963,24 @@ int compensation_terms(pp_cdouble *overlap, int BAND_NUM, pswf_t *wf_S, pswf_t *
   for (int t = 0; t < num_M; t++)
     PP_REQUIRE(M_R[t] >= 0 && M_R[t] < dR->nsites && M_S[t] >= 0 && M_S[t] < dS->nsites, PP_ERR_ARG,
                "compensation_terms: site index out of range");
-  if (num_M == 0)
+  for (int t = 0; t < num_N_R; t++)
+    PP_REQUIRE(N_R[t] >= 0 && N_R[t] < dR->nsites, PP_ERR_ARG, "compensation_terms: N_R site out of range");
+  for (int t = 0; t < num_N_S; t++)
+    PP_REQUIRE(N_S[t] >= 0 && N_S[t] < dS->nsites, PP_ERR_ARG, "compensation_terms: N_S site out of range");
+  // the wave projections of overlap_setup_real must be those of these lists
+  const unsigned long long hR = hash_ints(1469598103934665603ull, N_R, num_N_R);
+  const unsigned long long hS = hash_ints(1469598103934665603ull, N_S, num_N_S);
+  PP_REQUIRE(num_N_R == 0 || (dS->wp_n == num_N_R && dS->wp_of == wf_R && dS->wp_hash == hR), PP_ERR_ARG,
+             "compensation_terms: call overlap_setup_real with this N_R list first");
+  PP_REQUIRE(num_N_S == 0 || (dR->wp_n == num_N_S && dR->wp_of == wf_S && dR->wp_hash == hS), PP_ERR_ARG,
+             "compensation_terms: call overlap_setup_real with this N_S list first");
+  if (num_N_RS > 0) {
+    bool same = dS->on_n == num_N_RS && dS->on_of == wf_R;
+    for (int t = 0; same && t < num_N_RS; t++)
+      same = dS->on_R[t] == N_RS_R[t] && dS->on_S[t] == N_RS_S[t];
+    PP_REQUIRE(same, PP_ERR_ARG, "compensation_terms: call overlap_setup_real with these N_RS pairs first");
+  }
+  if (num_M + num_N_R + num_N_S + num_N_RS == 0)
     return PP_OK;
   pp_ctx *ctx;
   PP_TRY(global_ctx(&ctx));
@@ -742,6 +991,12 @@ int compensation_terms(pp_cdouble *overlap, int BAND_NUM, pswf_t *wf_S, pswf_t *
     h = (h ^ (unsigned)M_S[t]) * 1099511628211ull;
   }
   h ^= (unsigned long long)num_M << 32;
+  h = hash_ints(h, N_R, num_N_R);
+  h = hash_ints(h, N_S, num_N_S);
+  if (num_N_RS > 0) {
+    h = hash_ints(h, N_RS_R, num_N_RS);
+    h = hash_ints(h, N_RS_S, num_N_RS);
+  }
   auto key = std::make_tuple((const void *)wf_R, flip, h);
   auto it = dS->aug_cache.find(key);
   if (it == dS->aug_cache.end()) {
@@ -749,11 +1004,53 @@ int compensation_terms(pp_cdouble *overlap, int BAND_NUM, pswf_t *wf_S, pswf_t *
     size_t n = (size_t)nkt * dR->nband * dS->nband;
     PP_TRY(out.ensure(sizeof(pp_cdouble) * n));
     PP_TRY(zero_async(ctx, out.p, sizeof(pp_cdouble) * n));
-    // dO and channel tables of the reference structure's species
+    // O_M (projector.c:890-910): dO and channel tables of R's species
     PP_TRY(pp_augmentation(ctx, wf_R->nwk, wf_R->nspin, flip, dR->proj.p, dR->nband, dR->ptot, dR->proj_off.data(),
                            dR->labels.data(), dS->proj.p, dS->nband, dS->ptot, dS->proj_off.data(),
                            dS->labels.data(), num_M, M_R, M_S, (int)dR->sp_nchan.size(), dR->sp_nchan.data(),
                            dR->sp_ls.data(), dR->sp_dO.data(), out.p));
+    // O_R (:915-924) = sum_i conj(<p_i|R>) <(phi_i - phi~_i)|S> over the N_R
+    // sites, and O_S (:929-938) = sum_i conj(<(phi_i - phi~_i)|R>) <p_i|S>
+    // over the N_S sites: the same contraction with identity channel
+    // matrices (only i = j, equal (n, l, m), contributes)
+    auto ident = [](const std::vector<int> &nchan) {
+      std::vector<double> I;
+      for (int nc : nchan)
+        for (int i = 0; i < nc * nc; i++)
+          I.push_back(i % (nc + 1) == 0 ? 1.0 : 0.0);
+      return I;
+    };
+    if (num_N_R > 0) {
+      std::vector<int> seq(num_N_R);
+      for (int t = 0; t < num_N_R; t++)
+        seq[t] = t;
+      std::vector<double> I = ident(dR->sp_nchan);
+      PP_TRY(pp_augmentation(ctx, wf_R->nwk, wf_R->nspin, flip, dR->proj.p, dR->nband, dR->ptot,
+                             dR->proj_off.data(), dR->labels.data(), dS->wp.p, dS->nband, dS->wp_ptot,
+                             dS->wp_off.data(), dS->wp_labels.data(), num_N_R, N_R, seq.data(),
+                             (int)dR->sp_nchan.size(), dR->sp_nchan.data(), dR->sp_ls.data(), I.data(), out.p));
+    }
+    if (num_N_S > 0) {
+      std::vector<int> seq(num_N_S);
+      for (int t = 0; t < num_N_S; t++)
+        seq[t] = t;
+      std::vector<double> I = ident(dS->sp_nchan);
+      PP_TRY(pp_augmentation(ctx, wf_R->nwk, wf_R->nspin, flip, dR->wp.p, dR->nband, dR->wp_ptot,
+                             dR->wp_off.data(), dR->wp_labels.data(), dS->proj.p, dS->nband, dS->ptot,
+                             dS->proj_off.data(), dS->labels.data(), num_N_S, seq.data(), N_S,
+                             (int)dS->sp_nchan.size(), dS->sp_nchan.data(), dS->sp_ls.data(), I.data(), out.p));
+    }
+    // O_N (:940-959) = sum over N_RS pairs of conj(P_R) Ov P_S e^{2 pi i kR . dcoord}
+    if (num_N_RS > 0) {
+      std::vector<double> kp(3 * (size_t)nkt);
+      for (int k = 0; k < nkt; k++)
+        for (int j = 0; j < 3; j++)
+          kp[3 * k + j] = wf_R->kpts[k]->k[j];
+      PP_TRY(pp_augmentation_offsite(ctx, wf_R->nwk, wf_R->nspin, flip, dR->proj.p, dR->nband, dR->ptot,
+                                     dR->proj_off.data(), dS->proj.p, dS->nband, dS->ptot, dS->proj_off.data(),
+                                     num_N_RS, N_RS_R, N_RS_S, dS->on_ov.data(), dS->on_off.data(),
+                                     dS->on_dc.data(), kp.data(), out.p));
+    }
     PairCache pc;
     pc.nbR = dR->nband;
     pc.nbS = dS->nband;

@@ -59,6 +59,7 @@ int get_fft_plan(pp_ctx *ctx, const int *fftg, FftPlan **out) {
     for (int s = 0; s < ap.nst; s++) {
       int R = ap.radix[s];
       ap.pprev[s] = p;
+      ap.pmag[s] = p > 1 ? (unsigned)(((1ull << 32) + (unsigned long long)p - 1) / (unsigned long long)p) : 0u;
       ap.twoff[s] = (int)(tw.size() / 2) - pl->twbase[a];
       // twiddles exp(+2 pi i k r / (p R)), k < p, r = 1..R-1
       for (int k = 0; k < p; k++)
@@ -76,6 +77,7 @@ int get_fft_plan(pp_ctx *ctx, const int *fftg, FftPlan **out) {
       }
       p *= R;
     }
+    ap.twlen = (int)(tw.size() / 2) - pl->twbase[a];
   }
   size_t ntw = tw.size() / 2;
   std::vector<float> tw32(tw.size());
@@ -318,17 +320,25 @@ __global__ __launch_bounds__(256) void k_fft_pass(cplx<T> *__restrict__ g, AxisP
 // butterflies in registers across one barrier, so a single LDS buffer
 // serves as input and output (half the LDS of the ping-pong form, twice
 // the work-groups per CU).
+//
+// Thread t owns band slot b = t % NBW and butterflies j = t / NBW + q *
+// (256 / NBW); j / p comes from the multiplier pmag = ceil(2^32 / p)
+// (exact for j < 2^16), and the twiddles are read from the LDS copy of
+// the axis table.
 template <typename T, int NBW, int R, int MAXB>
-__device__ __forceinline__ void stage_ip(cplx<T> *__restrict__ buf, const cplx<T> *__restrict__ tws, int p,
-                                         int Tn) {
+__device__ __forceinline__ void stage_ip(cplx<T> *__restrict__ buf, const cplx<T> *__restrict__ tws, unsigned p,
+                                         unsigned pmag, unsigned Tn) {
+  constexpr unsigned JS = 256 / NBW;
+  const unsigned b = threadIdx.x % NBW, j0 = threadIdx.x / NBW;
   cplx<T> v[MAXB][R];
-  const int tot = Tn * NBW;
+  unsigned o[MAXB];
 #pragma unroll
   for (int q = 0; q < MAXB; q++) {
-    const int t = threadIdx.x + q * 256;
-    if (t < tot) {
-      const int j = t / NBW, b = t % NBW;
-      const int k = j % p;
+    const unsigned j = j0 + q * JS;
+    if (j < Tn) {
+      const unsigned h = p > 1 ? __umulhi(j, pmag) : j;
+      const unsigned k = j - h * p;
+      o[q] = h * p * R + k;
       v[q][0] = buf[j * NBW + b];
 #pragma unroll
       for (int r = 1; r < R; r++)
@@ -339,17 +349,22 @@ __device__ __forceinline__ void stage_ip(cplx<T> *__restrict__ buf, const cplx<T
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < MAXB; q++) {
-    const int t = threadIdx.x + q * 256;
-    if (t < tot) {
-      const int j = t / NBW, b = t % NBW;
-      const int k = j % p;
-      const int o = (j / p) * p * R + k;
+    const unsigned j = j0 + q * JS;
+    if (j < Tn) {
 #pragma unroll
       for (int r = 0; r < R; r++)
-        buf[(o + r * p) * NBW + b] = v[q][r];
+        buf[(o[q] + r * p) * NBW + b] = v[q][r];
     }
   }
   __syncthreads();
+}
+
+// Copy the twiddle/root table of one axis into LDS (visible after the
+// caller's next barrier).
+template <typename T>
+__device__ __forceinline__ void stage_twiddles(cplx<T> *__restrict__ dst, const cplx<T> *__restrict__ tw, int len) {
+  for (int i = threadIdx.x; i < len; i += 256)
+    dst[i] = tw[i];
 }
 
 // As k_fft_pass with in-place stages; radices 2, 3, 4, 5 only, and at most
@@ -386,6 +401,8 @@ __global__ __launch_bounds__(256) void k_fft_pass_ip(cplx<T> *__restrict__ g, Ax
   }
   const long base = (long)l1 * s1 + (long)l2 * s2 + (long)slice * NBW;
   const int tot = n * NBW;
+  cplx<T> *twl = buf + tot;
+  stage_twiddles<T>(twl, tw, pl.twlen);
   for (int t = threadIdx.x; t < tot; t += 256) {
     int e = t / NBW, b = t % NBW;
     const int se = sgn_idx(e, n);
@@ -393,20 +410,132 @@ __global__ __launch_bounds__(256) void k_fft_pass_ip(cplx<T> *__restrict__ g, Ax
   }
   __syncthreads();
   for (int s = 0; s < pl.nst; s++) {
-    const int R = pl.radix[s], p = pl.pprev[s], Tn = n / R;
-    const cplx<T> *tws = tw + pl.twoff[s];
+    const unsigned R = pl.radix[s], p = pl.pprev[s], pm = pl.pmag[s], Tn = n / R;
+    const cplx<T> *tws = twl + pl.twoff[s];
     if (R == 4)
-      stage_ip<T, NBW, 4, MAXB>(buf, tws, p, Tn);
+      stage_ip<T, NBW, 4, MAXB>(buf, tws, p, pm, Tn);
     else if (R == 3)
-      stage_ip<T, NBW, 3, MAXB>(buf, tws, p, Tn);
+      stage_ip<T, NBW, 3, MAXB>(buf, tws, p, pm, Tn);
     else if (R == 2)
-      stage_ip<T, NBW, 2, MAXB>(buf, tws, p, Tn);
+      stage_ip<T, NBW, 2, MAXB>(buf, tws, p, pm, Tn);
     else
-      stage_ip<T, NBW, 5, MAXB>(buf, tws, p, Tn);
+      stage_ip<T, NBW, 5, MAXB>(buf, tws, p, pm, Tn);
   }
   for (int t = threadIdx.x; t < tot; t += 256) {
     int e = t / NBW, b = t % NBW;
     g[base + (long)e * es + b] = buf[t];
+  }
+}
+
+// Line bounds of a pass (see k_fft_pass); false when the line is skipped.
+template <int AXIS>
+__device__ __forceinline__ bool line_range(const SphereBounds &bd, int l1, int l2, int &lo, int &hi) {
+  lo = -(1 << 30);
+  hi = 1 << 30;
+  if (!bd.xb)
+    return true;
+  if (AXIS == 2) {
+    const int sx = sgn_idx(l1, bd.n0), sy = sgn_idx(l2, bd.n1);
+    if (sx < bd.xb[0] || sx > bd.xb[1] || sy < bd.yb[2 * l1] || sy > bd.yb[2 * l1 + 1])
+      return false;
+    lo = bd.zb[2 * (l1 * bd.n1 + l2)];
+    hi = bd.zb[2 * (l1 * bd.n1 + l2) + 1];
+  } else if (AXIS == 1) {
+    const int sx = sgn_idx(l1, bd.n0);
+    if (sx < bd.xb[0] || sx > bd.xb[1])
+      return false;
+    lo = bd.yb[2 * l1];
+    hi = bd.yb[2 * l1 + 1];
+  } else {
+    lo = bd.xb[0];
+    hi = bd.xb[1];
+  }
+  return true;
+}
+
+// Persistent, software-pipelined form of k_fft_pass_ip: a fixed grid of
+// work-groups walks the lines; each thread holds its EPT elements of the
+// next line in registers while the current line is transformed in LDS, so
+// the global loads of line i+1 overlap the LDS stages of line i.
+template <typename T, int NBW, int AXIS, int MAXB, int EPT>
+__global__ __launch_bounds__(256) void k_fft_pass_pl(cplx<T> *__restrict__ g, AxisPlan pl,
+                                                     const cplx<T> *__restrict__ tw, long es, long s1, long s2,
+                                                     int n2, int nslices, long nlines, SphereBounds bd) {
+  extern __shared__ unsigned char smem_raw[];
+  cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
+  const int n = pl.n;
+  const int tot = n * NBW;
+  cplx<T> *twl = buf + tot;
+  stage_twiddles<T>(twl, tw, pl.twlen);
+  cplx<T> reg[EPT];
+  // find the first line of this work-group that has work
+  long line = blockIdx.x;
+  int lo = 0, hi = 0;
+  for (; line < nlines; line += gridDim.x) {
+    const long ln = line / nslices;
+    if (line_range<AXIS>(bd, (int)(ln / n2), (int)(ln % n2), lo, hi))
+      break;
+  }
+  auto load = [&](long L, int llo, int lhi) {
+    const long ln = L / nslices;
+    const int slice = (int)(L % nslices);
+    const long base = (ln / n2) * s1 + (ln % n2) * s2 + (long)slice * NBW;
+#pragma unroll
+    for (int q = 0; q < EPT; q++) {
+      const int t = threadIdx.x + q * 256;
+      cplx<T> v = mk<T>(0, 0);
+      if (t < tot) {
+        const int e = t / NBW, b = t % NBW;
+        const int se = sgn_idx(e, n);
+        if (se >= llo && se <= lhi)
+          v = g[base + (long)e * es + b];
+      }
+      reg[q] = v;
+    }
+  };
+  if (line < nlines)
+    load(line, lo, hi);
+  while (line < nlines) {
+#pragma unroll
+    for (int q = 0; q < EPT; q++) {
+      const int t = threadIdx.x + q * 256;
+      if (t < tot)
+        buf[t] = reg[q];
+    }
+    __syncthreads();
+    // prefetch the next line with work
+    long next = line + gridDim.x;
+    int nlo = 0, nhi = 0;
+    for (; next < nlines; next += gridDim.x) {
+      const long ln = next / nslices;
+      if (line_range<AXIS>(bd, (int)(ln / n2), (int)(ln % n2), nlo, nhi))
+        break;
+    }
+    if (next < nlines)
+      load(next, nlo, nhi);
+    for (int s = 0; s < pl.nst; s++) {
+      const unsigned R = pl.radix[s], p = pl.pprev[s], pm = pl.pmag[s], Tn = n / R;
+      const cplx<T> *tws = twl + pl.twoff[s];
+      if (R == 4)
+        stage_ip<T, NBW, 4, MAXB>(buf, tws, p, pm, Tn);
+      else if (R == 3)
+        stage_ip<T, NBW, 3, MAXB>(buf, tws, p, pm, Tn);
+      else if (R == 2)
+        stage_ip<T, NBW, 2, MAXB>(buf, tws, p, pm, Tn);
+      else
+        stage_ip<T, NBW, 5, MAXB>(buf, tws, p, pm, Tn);
+    }
+    {
+      const long ln = line / nslices;
+      const int slice = (int)(line % nslices);
+      const long base = (ln / n2) * s1 + (ln % n2) * s2 + (long)slice * NBW;
+      for (int t = threadIdx.x; t < tot; t += 256) {
+        const int e = t / NBW, b = t % NBW;
+        g[base + (long)e * es + b] = buf[t];
+      }
+    }
+    __syncthreads();
+    line = next;
   }
 }
 
@@ -494,9 +623,28 @@ static int launch_pass(pp_ctx *ctx, cplx<T> *grid, const AxisPlan &ap, const cpl
   hipEvent_t ev;
   tbegin(ctx, cls, &ev);
   bool launched = false;
+  const int ept = (ap.n * NBW + 255) / 256;
   if constexpr (NBW >= 8) {
-    if (ip) {
-      size_t lds = (size_t)ap.n * NBW * sizeof(cplx<T>);
+    if (ip && ctx->tune_fft_persistent != 0 && ept <= 16) {
+      size_t lds = ((size_t)ap.n * NBW + ap.twlen) * sizeof(cplx<T>);
+      long nlines = nblk;
+      long grid_wg = std::min<long>(nlines, (long)ctx->tune_fft_persistent > 1 ? ctx->tune_fft_persistent : 2048);
+#define PP_PL(MB, EP)                                                                                          \
+  hipLaunchKernelGGL((k_fft_pass_pl<T, NBW, AXIS, MB, EP>), dim3((unsigned)grid_wg), dim3(256), lds, ctx->stream,   \
+                     grid, ap, tw, es, s1, s2, n2, nslices, nlines, bd)
+      if (maxb <= 4 && ept <= 8)
+        PP_PL(4, 8);
+      else if (ept <= 8)
+        PP_PL(8, 8);
+      else if (maxb <= 4)
+        PP_PL(4, 16);
+      else
+        PP_PL(8, 16);
+#undef PP_PL
+      launched = true;
+    }
+    if (!launched && ip) {
+      size_t lds = ((size_t)ap.n * NBW + ap.twlen) * sizeof(cplx<T>);
 #define PP_IP(MB)                                                                                              \
   hipLaunchKernelGGL((k_fft_pass_ip<T, NBW, AXIS, MB>), dim3((unsigned)nblk), dim3(256), lds, ctx->stream, grid, ap, \
                      tw, es, s1, s2, n2, nslices, bd)

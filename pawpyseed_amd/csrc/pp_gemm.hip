@@ -2,10 +2,11 @@
 //
 // The band-overlap matrix of pseudoprojector.c:63-90 (one cdotc per band
 // pair) and the O_M augmentation sum of projector.c:890-910 are both this
-// contraction with K contiguous in both operands.  FP64 path: 64x64 output
-// tiles, 256 threads with 4x4 complex accumulators each, K tiles of 16
-// staged through LDS; split-K with a deterministic reduction when the tile
-// grid alone cannot fill the 256 CUs.
+// contraction with K contiguous in both operands.  FP64 path: FP64 matrix
+// cores (k_zgemm_mfma64) or, selectable, a VALU tile GEMM (64x64 output
+// tiles, 4x4 complex accumulators per thread); K tiles of 16 staged through
+// LDS; split-K with a deterministic reduction when the tile grid alone
+// cannot fill the 256 CUs.
 #include "pp_internal.h"
 
 namespace pp {
@@ -78,6 +79,89 @@ __global__ __launch_bounds__(256) void k_zgemm_f64(int M, int N, int kbeg_stride
     }
 }
 
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+// FP64 matrix-core form: conj(A) B^T as two real GEMMs over the
+// interleaved (re, im) storage (K' = 2K; Im uses B'' = (b_im, -b_re)) on
+// v_mfma_f64_16x16x4_f64.  Work-group: 4 wavefronts, 64x64 complex
+// outputs, each wavefront a 32x32 quadrant of 2x2 16x16 tiles.  Output
+// layout of a 16x16 tile: lane l, register v holds row l/16 + 4v, column
+// l%16 (unlike the FP32 16x16x4 form, whose register v is row 4(l/16)+v).
+template <typename TA, typename TB>
+__global__ __launch_bounds__(256) void k_zgemm_mfma64(int M, int N, int kchunk, int K,
+                                                      const cplx<TA> *__restrict__ A, long lda,
+                                                      const cplx<TB> *__restrict__ B, long ldb,
+                                                      cplx<double> *__restrict__ C, long ldc, long sSplit) {
+  __shared__ double As[64][36];
+  __shared__ double Bs[64][36];
+  const int split = blockIdx.y;
+  const int tilesN = (N + 63) / 64;
+  const int tm = blockIdx.x / tilesN, tn = blockIdx.x % tilesN;
+  const int k0 = split * kchunk;
+  const int k1 = min(K, k0 + kchunk);
+  C += (long)split * sSplit;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  doublex4 cr[2][2], ci[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      cr[a][b] = doublex4{0.0, 0.0, 0.0, 0.0};
+      ci[a][b] = doublex4{0.0, 0.0, 0.0, 0.0};
+    }
+  const int lrow = threadIdx.x >> 2, part = threadIdx.x & 3;
+  const int gm = tm * 64 + lrow, gn = tn * 64 + lrow;
+  for (int kb = k0; kb < k1; kb += 16) {
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const int gk = kb + part * 4 + c;
+      cplx<double> va = mk<double>(0, 0), vb = mk<double>(0, 0);
+      if (gk < k1) {
+        if (gm < M)
+          va = cvt<double>(A[(long)gm * lda + gk]);
+        if (gn < N)
+          vb = cvt<double>(B[(long)gn * ldb + gk]);
+      }
+      As[lrow][part * 8 + 2 * c] = va.x;
+      As[lrow][part * 8 + 2 * c + 1] = va.y;
+      Bs[lrow][part * 8 + 2 * c] = vb.x;
+      Bs[lrow][part * 8 + 2 * c + 1] = vb.y;
+    }
+    __syncthreads();
+    const int sub = lane & 15, kq = lane >> 4;
+#pragma unroll
+    for (int kk = 0; kk < 32; kk += 4) {
+      const int kr = kk + kq;
+      const double a0 = As[wm * 32 + sub][kr];
+      const double a1 = As[wm * 32 + 16 + sub][kr];
+#pragma unroll
+      for (int nj = 0; nj < 2; nj++) {
+        const int col = wn * 32 + nj * 16 + sub;
+        const double b = Bs[col][kr];
+        const double b2 = (kq & 1) ? -Bs[col][kr - 1] : Bs[col][kr + 1];
+        cr[0][nj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b, cr[0][nj], 0, 0, 0);
+        ci[0][nj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b2, ci[0][nj], 0, 0, 0);
+        cr[1][nj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b, cr[1][nj], 0, 0, 0);
+        ci[1][nj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b2, ci[1][nj], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int mi = 0; mi < 2; mi++)
+#pragma unroll
+    for (int nj = 0; nj < 2; nj++) {
+      const int j = tn * 64 + wn * 32 + nj * 16 + (lane & 15);
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int i = tm * 64 + wm * 32 + mi * 16 + (lane >> 4) + 4 * v;
+        if (i < M && j < N)
+          C[(long)i * ldc + j] = mk<double>(cr[mi][nj][v], ci[mi][nj][v]);
+      }
+    }
+}
+
 // C[m][n] (+)= sum over splits of partial[s][m][n]
 __global__ __launch_bounds__(256) void k_splitk_reduce(int M, int N, int nsplit, const cplx<double> *__restrict__ P,
                                                        cplx<double> *__restrict__ C, long ldc, int accumulate) {
@@ -125,7 +209,20 @@ static int zgemm_one(pp_ctx *ctx, int precision, int M, int N, int K, int atype,
   dim3 grd(tiles, nsplit);
   hipEvent_t ev;
   tbegin(ctx, cls, &ev);
-  if (atype == 1 && btype == 1)
+  if (ctx->tune_gemm_f64_valu == 0) {
+#define PP_MF(TA, TB)                                                                                          \
+  hipLaunchKernelGGL((k_zgemm_mfma64<TA, TB>), grd, dim3(256), 0, ctx->stream, M, N, kchunk, K, (const cplx<TA> *)A, \
+                     lda, (const cplx<TB> *)B, ldb, dst, dld, sSplit)
+    if (atype == 1 && btype == 1)
+      PP_MF(float, float);
+    else if (atype == 0 && btype == 0)
+      PP_MF(double, double);
+    else if (atype == 0 && btype == 1)
+      PP_MF(double, float);
+    else
+      PP_MF(float, double);
+#undef PP_MF
+  } else if (atype == 1 && btype == 1)
     hipLaunchKernelGGL((k_zgemm_f64<float, float>), grd, dim3(256), 0, ctx->stream, M, N, kchunk, K,
                        (const cplx<float> *)A, lda, (const cplx<float> *)B, ldb, dst, dld, sSplit);
   else if (atype == 0 && btype == 0)

@@ -2,6 +2,7 @@
 #pragma once
 #include <map>
 #include <string>
+#include <complex>
 #include <vector>
 
 #include "pp_common.h"
@@ -43,6 +44,8 @@ struct AxisPlan {
   int radix[12];
   int pprev[12];
   int twoff[12];
+  unsigned pmag[12];  // ceil(2^32 / pprev): j / pprev == umulhi(j, pmag) for j < 2^16 (pprev > 1)
+  int twlen = 0;      // twiddle + root entries of this axis
 };
 
 // Plane-wave sphere bounds of one k-point in signed grid indices (device
@@ -75,6 +78,8 @@ struct pp_ctx {
   hipStream_t stream = nullptr;
   int tune_fft_bands = 0, tune_sphere_parts = 0, tune_splitk = 0, tune_fft_tile = 0, tune_fft_prune = 0;
   int tune_fft_pingpong = 0;
+  int tune_fft_persistent = 0;
+  int tune_gemm_f64_valu = 0;  // 1: FP64 GEMM on the VALU instead of the matrix cores  // 0 off, 1 on (2048 work-groups), >1 work-group count
   pp::DevBuf grid;      // band-fastest FFT chunk
   pp::DevBuf partial;   // sphere-sum partials
   pp::DevBuf gidx;      // wrapped grid index per G
@@ -107,6 +112,12 @@ int grid_to_bandmajor(pp_ctx *ctx, int precision, const void *grid, int NB, int 
                       double scl, void *out);
 double det3(const double *m);
 void spline_coeff_host(const double *x, const double *y, int n, double *b, double *c, double *d);
+int smooth_diffwave_host(int n, const double *wave_grid, const double *diffwave, int l, int lmax, double grid_encut,
+                         int np, double *sdw, double *kgrid, double *kwave);
+double wigner3j_host(int j1, int j2, int j3, int m1, int m2, int m3);
+std::complex<double> offsite_overlap_host(const double *dcoord, const double *k1, const double *f1,
+                                          const double *const *s1, int n1, const double *k2, const double *f2,
+                                          const double *const *s2, int n2, int l1, int m1, int l2, int m2);
 
 // Complex GEMM out[b][m][n] (+)= sum_k conj(A[b][m][k]) B[b][n][k] (pp_gemm.hip).
 // type codes: 0 = c128, 1 = c64.  Batch strides in elements; amap/bmap

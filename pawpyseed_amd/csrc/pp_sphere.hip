@@ -5,14 +5,23 @@
 // +-(int(|a_j x a_k| rmax / V * N_i) + 1) around round(frac * N) with
 // |r - R| < (n-1)/n * rmax, in the reference's i-j-k loop order.  Each
 // point stores its wrapped grid index, its unwrapped path r - R (for the
-// Bloch phase) and conj(p_j(r - R)) for every projector j of the site,
-// where p_j = spline(|r|) Y_lm evaluated on the minimum-image vector
-// (utils.c:547-576).
+// Bloch phase) and, for every projector j = (channel, l, m) of the site,
+// the real value t_j of the projector in real-harmonic form:
+//   t_(l,m)  = A_lm cos(m phi)   (m >= 0),
+//   t_(l,-m) = A_lm sin(m phi)   (m > 0),
+// A_lm = N_lm P_l^m(cos theta) spline_l(|r|), evaluated on the
+// minimum-image vector (utils.c:547-576).
 //
 // Sphere sums replace projector.c:223-274 (onto_projector_helper):
-// <p_j|psi> = sum_i conj(p_j(i)) psi(idx_i) dv exp(i k.path_i).  One
-// work-group per (site, part of its points); lane = band of the chunk, so
-// each point's NB bands are one contiguous load from the band-fastest
+// <p_j|psi> = sum_i conj(p_j(i)) psi(idx_i) dv exp(i k.path_i).  With
+// s_j = sum_i t_j(i) psi(idx_i) dv exp(i k.path_i) and
+// Y_l,-m = (-1)^m conj(Y_lm) (utils.c:377-386, 441-449):
+//   <p_(l,m)|psi>  = s_(l,m) - i s_(l,-m),
+//   <p_(l,-m)|psi> = (-1)^m (s_(l,m) + i s_(l,-m)),
+// so the sums run over real table values (half the flops and bytes of a
+// complex table) and the reduction kernel forms the complex projections.
+// One work-group per (site, part of its points); lane = band of the chunk,
+// so each point's NB bands are one contiguous load from the band-fastest
 // grid while the projector values and phase are uniform across the wave.
 #include <cmath>
 #include <cstring>
@@ -32,12 +41,13 @@ struct pp_sphere {
   std::vector<double> rmax;
   std::vector<int> nproj_site, proj_off;  // host
   std::vector<int> site_pt_h;
-  int ptot = 0, pmax = 0;
+  int ptot = 0, pmax = 0, pmin = 0;
   long npts = 0;
   // device
   pp::DevBuf d_spline;  // per species: x, f, b, c, d [nchan][ngrid]
   pp::DevBuf d_sitemeta;
   pp::DevBuf d_site_pt, d_site_np, d_tab_off, d_proj_off, d_counts;
+  pp::DevBuf d_proj_m;  // m of every projector, [ptot]
   pp::DevBuf d_pidx, d_path, d_table;
   pp::DevBuf d_chan;  // per species channel l list
 };
@@ -203,7 +213,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_site_values(const SiteGeo *__restrict__ geo, Lat9 L, int f0, int f1, int f2,
                                                      const int *__restrict__ site_pt, const long *__restrict__ tab_off,
                                                      const int *__restrict__ pidx, const double *__restrict__ spl,
-                                                     const int *__restrict__ chan_l, cplx<T> *__restrict__ table) {
+                                                     const int *__restrict__ chan_l, T *__restrict__ table) {
   const int s_id = blockIdx.y;
   const SiteGeo s = geo[s_id];
   const int p0 = site_pt[s_id], np = site_pt[s_id + 1] - p0;
@@ -243,7 +253,7 @@ __global__ __launch_bounds__(256) void k_site_values(const SiteGeo *__restrict__
   const double ct = cos(theta);
   const int n = s.ngrid;
   const double *x = spl + s.spl_off;
-  cplx<T> *out = table + tab_off[s_id] + (long)i * s.nproj;
+  T *out = table + tab_off[s_id] + (long)i * s.nproj;
   int p = 0;
   for (int ch = 0; ch < s.nchan; ch++) {
     const int l = chan_l[s.chan_off + ch];
@@ -265,12 +275,14 @@ __global__ __launch_bounds__(256) void k_site_values(const SiteGeo *__restrict__
       radial = f[id] + t * (bb[id] + t * (cc[id] + t * dd[id]));
     }
     for (int m = -l; m <= l; m++, p++) {
-      double norm = sqrt((2 * l + 1) / (4.0 * M_PI) * d_fact(l - m) / d_fact(l + m));
-      double pl = norm * d_legendre(l, m, ct) * radial;
+      // real form: slot m >= 0 holds A_|m| cos(|m| phi), slot -|m| holds
+      // A_|m| sin(|m| phi), with A_|m| = N_l|m| P_l^|m|(cos theta) radial
+      const int am = m < 0 ? -m : m;
+      double norm = sqrt((2 * l + 1) / (4.0 * M_PI) * d_fact(l - am) / d_fact(l + am));
+      double pl = norm * d_legendre(l, am, ct) * radial;
       double sn, cs;
-      sincos(m * phi, &sn, &cs);
-      // store conj(p) so the sphere sum is a plain product
-      out[p] = mk<T>((T)(pl * cs), (T)(-pl * sn));
+      sincos(am * phi, &sn, &cs);
+      out[p] = (T)(m < 0 ? pl * sn : pl * cs);
     }
   }
 }
@@ -295,21 +307,20 @@ __global__ __launch_bounds__(256) void k_phase(long npts, const double *__restri
 // fourth point of the tile, four points per step with their grid loads in
 // flight together, and reads the projector values as LDS broadcasts.  The
 // four wavefronts' sums are added in LDS at the end.
-template <typename T, int PMAX>
+template <typename T, int PMAX, bool FULL>
 __global__ __launch_bounds__(256) void k_sphere_sum(const cplx<T> *__restrict__ g, int NB,
                                                     const int *__restrict__ site_pt, const int *__restrict__ site_np,
                                                     const long *__restrict__ tab_off, const int *__restrict__ pidx,
-                                                    const cplx<T> *__restrict__ phase,
-                                                    const cplx<T> *__restrict__ table, int nparts,
-                                                    cplx<double> *__restrict__ partial) {
+                                                    const cplx<T> *__restrict__ phase, const T *__restrict__ table,
+                                                    int nparts, cplx<double> *__restrict__ partial) {
   constexpr int TP = 64;
-  __shared__ cplx<T> s_tab[TP * PMAX];
+  __shared__ __align__(16) T s_tab[TP * PMAX];
   __shared__ cplx<T> s_ph[TP];
   __shared__ int s_idx[TP];
   const int s = blockIdx.x / nparts, part = blockIdx.x % nparts;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int p0 = site_pt[s], p1 = site_pt[s + 1];
-  const int np = site_np[s];
+  const int np = FULL ? PMAX : site_np[s];  // FULL: every site has PMAX projectors
   const int len = p1 - p0;
   const int chunk = (len + nparts - 1) / nparts;
   const int a = p0 + part * chunk;
@@ -319,10 +330,10 @@ __global__ __launch_bounds__(256) void k_sphere_sum(const cplx<T> *__restrict__ 
 #pragma unroll
   for (int j = 0; j < PMAX; j++)
     acc[j] = mk<T>(0, 0);
-  const cplx<T> *tsite = table + tab_off[s];
+  const T *tsite = table + tab_off[s];
   for (int t0 = a; t0 < e; t0 += TP) {
     const int n = min(TP, e - t0);
-    const cplx<T> *tsrc = tsite + (long)(t0 - p0) * np;
+    const T *tsrc = tsite + (long)(t0 - p0) * np;
     for (int q = tid; q < n * np; q += 256)
       s_tab[q] = tsrc[q];
     if (tid < n) {
@@ -339,40 +350,45 @@ __global__ __launch_bounds__(256) void k_sphere_sum(const cplx<T> *__restrict__ 
 #pragma unroll
       for (int u = 0; u < 4; u++) {
         const cplx<T> xv = x[u] * s_ph[i + 4 * u];
-        const cplx<T> *tp = s_tab + (i + 4 * u) * np;
+        const T *tp = s_tab + (i + 4 * u) * np;
 #pragma unroll
         for (int j = 0; j < PMAX; j++)
-          if (j < np)
-            cmac(acc[j], tp[j], xv);
+          if (FULL || j < np) {
+            acc[j].x += tp[j] * xv.x;
+            acc[j].y += tp[j] * xv.y;
+          }
       }
     }
     for (; i < n; i += 4) {
       cplx<T> xv = active ? g[(long)s_idx[i] * NB + lane] : mk<T>(0, 0);
       xv = xv * s_ph[i];
-      const cplx<T> *tp = s_tab + i * np;
+      const T *tp = s_tab + i * np;
 #pragma unroll
       for (int j = 0; j < PMAX; j++)
-        if (j < np)
-          cmac(acc[j], tp[j], xv);
+        if (FULL || j < np) {
+          acc[j].x += tp[j] * xv.x;
+          acc[j].y += tp[j] * xv.y;
+        }
     }
     __syncthreads();
   }
-  // add the four wavefronts' sums (reuse the table tile as scratch)
-  cplx<T> *red = s_tab;
+  // add the four wavefronts' sums, PMAX/2 projectors at a time through the
+  // table tile (TP * PMAX reals = 64 lanes x PMAX/2 complex)
+  cplx<T> *red = reinterpret_cast<cplx<T> *>(s_tab);
+  constexpr int JH = PMAX / 2;
   for (int w = 1; w < 4; w++) {
-    for (int j0 = 0; j0 < PMAX; j0 += TP * PMAX / 64) {
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
       if (wave == w) {
 #pragma unroll
-        for (int j = 0; j < PMAX; j++)
-          if (j >= j0 && j < j0 + TP * PMAX / 64)
-            red[(j - j0) * 64 + lane] = acc[j];
+        for (int j = 0; j < JH; j++)
+          red[j * 64 + lane] = acc[h * JH + j];
       }
       __syncthreads();
       if (wave == 0) {
 #pragma unroll
-        for (int j = 0; j < PMAX; j++)
-          if (j >= j0 && j < j0 + TP * PMAX / 64)
-            acc[j] = acc[j] + red[(j - j0) * 64 + lane];
+        for (int j = 0; j < JH; j++)
+          acc[h * JH + j] = acc[h * JH + j] + red[j * 64 + lane];
       }
       __syncthreads();
     }
@@ -381,15 +397,19 @@ __global__ __launch_bounds__(256) void k_sphere_sum(const cplx<T> *__restrict__ 
     cplx<double> *o = partial + ((long)blockIdx.x * NB + lane) * PMAX;
 #pragma unroll
     for (int j = 0; j < PMAX; j++)
-      if (j < np)
+      if (FULL || j < np)
         o[j] = cvt<double>(acc[j]);
   }
 }
 
+// Sum the parts of every (site, band, projector) in FP64 and form the
+// complex projections from the real-harmonic sums: thread (s, b, j) with
+// m_j >= 0 writes slot j and, for m_j > 0, its partner slot j - 2 m_j.
 __global__ __launch_bounds__(256) void k_sphere_reduce(const cplx<double> *__restrict__ partial, int NB, int nbc,
                                                        int PMAX, int nparts, int nsites,
                                                        const int *__restrict__ site_np,
-                                                       const int *__restrict__ proj_off, long ptot,
+                                                       const int *__restrict__ proj_off,
+                                                       const int *__restrict__ proj_m, long ptot,
                                                        cplx<double> *__restrict__ out) {
   long t = (long)blockIdx.x * 256 + threadIdx.x;
   long total = (long)nsites * nbc * PMAX;
@@ -401,10 +421,24 @@ __global__ __launch_bounds__(256) void k_sphere_reduce(const cplx<double> *__res
   int s = (int)(r / nbc);
   if (j >= site_np[s])
     return;
-  cplx<double> acc = mk<double>(0, 0);
-  for (int q = 0; q < nparts; q++)
-    acc = acc + partial[(((long)s * nparts + q) * NB + b) * PMAX + j];
-  out[(long)b * ptot + proj_off[s] + j] = acc;
+  const int m = proj_m[proj_off[s] + j];
+  if (m < 0)
+    return;
+  cplx<double> c = mk<double>(0, 0), sn = mk<double>(0, 0);
+  const cplx<double> *ps = partial + ((long)s * nparts * NB + b) * PMAX;
+  for (int q = 0; q < nparts; q++) {
+    c = c + ps[(long)q * NB * PMAX + j];
+    if (m > 0)
+      sn = sn + ps[(long)q * NB * PMAX + j - 2 * m];
+  }
+  cplx<double> *o = out + (long)b * ptot + proj_off[s];
+  if (m == 0) {
+    o[j] = c;
+  } else {
+    const double sg = (m & 1) ? -1.0 : 1.0;
+    o[j] = mk<double>(c.x + sn.y, c.y - sn.x);
+    o[j - 2 * m] = mk<double>(sg * (c.x - sn.y), sg * (c.y + sn.x));
+  }
 }
 
 // ------------------------------------------------------------------ host
@@ -491,9 +525,23 @@ static int sphere_build(pp_sphere *sp) {
     sp->proj_off[s + 1] = sp->proj_off[s] + np;
     if (np > sp->pmax)
       sp->pmax = np;
+    if (s == 0 || np < sp->pmin)
+      sp->pmin = np;
   }
   sp->ptot = sp->proj_off[ns];
   PP_REQUIRE(sp->pmax <= 64, PP_ERR_UNSUPPORTED, "more than 64 projectors on one site");
+  std::vector<int> proj_m(std::max(1, sp->ptot));
+  for (int s = 0, q = 0; s < ns; s++) {
+    int e = sp->labels[s];
+    for (int ch = 0; ch < sp->nchan[e]; ch++) {
+      int l = sp->ls[sp->ls_off[e] + ch];
+      for (int m = -l; m <= l; m++)
+        proj_m[q++] = m;
+    }
+  }
+  PP_TRY(sp->d_proj_m.ensure(sizeof(int) * proj_m.size()));
+  PP_CHECK_HIP(hipMemcpyAsync(sp->d_proj_m.p, proj_m.data(), sizeof(int) * proj_m.size(), hipMemcpyHostToDevice,
+                              ctx->stream));
   Lat9 L;
   memcpy(L.m, sp->lattice, sizeof(L.m));
   PP_TRY(sp->d_sitemeta.ensure(sizeof(SiteGeo) * ns));
@@ -520,7 +568,7 @@ static int sphere_build(pp_sphere *sp) {
       maxcount = counts[s];
   }
   sp->npts = sp->site_pt_h[ns];
-  size_t csz = sp->precision == PP_FP64 ? sizeof(cplx<double>) : sizeof(cplx<float>);
+  size_t csz = sp->precision == PP_FP64 ? sizeof(double) : sizeof(float);
   PP_TRY(sp->d_site_pt.ensure(sizeof(int) * (ns + 1)));
   PP_TRY(sp->d_site_np.ensure(sizeof(int) * ns));
   PP_TRY(sp->d_tab_off.ensure(sizeof(long) * ns));
@@ -548,12 +596,12 @@ static int sphere_build(pp_sphere *sp) {
       hipLaunchKernelGGL(k_site_values<double>, grd, dim3(256), 0, ctx->stream, (const SiteGeo *)sp->d_sitemeta.p,
                          L, sp->fftg[0], sp->fftg[1], sp->fftg[2], (const int *)sp->d_site_pt.p,
                          (const long *)sp->d_tab_off.p, (const int *)sp->d_pidx.p, (const double *)sp->d_spline.p,
-                         (const int *)sp->d_chan.p, (cplx<double> *)sp->d_table.p);
+                         (const int *)sp->d_chan.p, (double *)sp->d_table.p);
     else
       hipLaunchKernelGGL(k_site_values<float>, grd, dim3(256), 0, ctx->stream, (const SiteGeo *)sp->d_sitemeta.p, L,
                          sp->fftg[0], sp->fftg[1], sp->fftg[2], (const int *)sp->d_site_pt.p,
                          (const long *)sp->d_tab_off.p, (const int *)sp->d_pidx.p, (const double *)sp->d_spline.p,
-                         (const int *)sp->d_chan.p, (cplx<float> *)sp->d_table.p);
+                         (const int *)sp->d_chan.p, (float *)sp->d_table.p);
     tend(ctx, K_TABLE, ev);
     PP_CHECK_HIP(hipGetLastError());
   }
@@ -563,9 +611,14 @@ static int sphere_build(pp_sphere *sp) {
 template <typename T, int PMAX>
 static void launch_sum(pp_ctx *ctx, pp_sphere *sp, const cplx<T> *grid, int NB, const cplx<T> *phase, int nparts,
                        cplx<double> *partial) {
-  hipLaunchKernelGGL((k_sphere_sum<T, PMAX>), dim3(sp->nsites * nparts), dim3(256), 0, ctx->stream, grid, NB,
-                     (const int *)sp->d_site_pt.p, (const int *)sp->d_site_np.p, (const long *)sp->d_tab_off.p,
-                     (const int *)sp->d_pidx.p, phase, (const cplx<T> *)sp->d_table.p, nparts, partial);
+  if (sp->pmin == PMAX)
+    hipLaunchKernelGGL((k_sphere_sum<T, PMAX, true>), dim3(sp->nsites * nparts), dim3(256), 0, ctx->stream, grid,
+                       NB, (const int *)sp->d_site_pt.p, (const int *)sp->d_site_np.p, (const long *)sp->d_tab_off.p,
+                       (const int *)sp->d_pidx.p, phase, (const T *)sp->d_table.p, nparts, partial);
+  else
+    hipLaunchKernelGGL((k_sphere_sum<T, PMAX, false>), dim3(sp->nsites * nparts), dim3(256), 0, ctx->stream, grid,
+                       NB, (const int *)sp->d_site_pt.p, (const int *)sp->d_site_np.p, (const long *)sp->d_tab_off.p,
+                       (const int *)sp->d_pidx.p, phase, (const T *)sp->d_table.p, nparts, partial);
 }
 
 template <typename T>
@@ -607,8 +660,8 @@ int sphere_sums(pp_ctx *ctx, pp_sphere *sp, const cplx<T> *grid, int NB, int nbc
   long total = (long)sp->nsites * nbc * PM;
   tbegin(ctx, K_SPHERE_REDUCE, &ev);
   hipLaunchKernelGGL(k_sphere_reduce, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, ctx->stream, partial, NB,
-                     nbc, PM, nparts, sp->nsites, (const int *)sp->d_site_np.p, (const int *)sp->d_proj_off.p, ptot,
-                     out);
+                     nbc, PM, nparts, sp->nsites, (const int *)sp->d_site_np.p, (const int *)sp->d_proj_off.p,
+                     (const int *)sp->d_proj_m.p, ptot, out);
   tend(ctx, K_SPHERE_REDUCE, ev);
   PP_CHECK_HIP(hipGetLastError());
   return PP_OK;

@@ -127,15 +127,66 @@ def test_compat_fft3d(gpu):
 
 
 def test_compensation_terms_rejects_unmatched_lists(gpu):
+    """N_RS pairs (off-site O_N) are not provided; N_R lists need a
+    preceding overlap_setup_real with the same list."""
     from pawpyseed_amd.core import wavefunction_from_workload
-    from pawpyseed_amd.core._lib import PP_ERR_UNSUPPORTED, lib
+    from pawpyseed_amd.core._lib import PP_ERR_ARG, PP_ERR_UNSUPPORTED, lib
     wl, CR, CS = _pair(nband=3, kpts=((0, 0, 0),))
     a = wavefunction_from_workload(wl, "R", coeffs=CR, setup_projectors=True)
     b = wavefunction_from_workload(wl, "S", coeffs=CS, setup_projectors=True)
     res = np.zeros(3, np.complex128)
     one = np.array([0], np.int32)
-    st = lib().compensation_terms(res.ctypes.data_as(ct.c_void_p), 0, b.wf_ptr, a.wf_ptr, 0, 1, 0, 0, None, None,
-                                  one.ctypes.data_as(ct.POINTER(ct.c_int)), None, None, None, None, None, None, None,
-                                  None, 0)
+    ip = one.ctypes.data_as(ct.POINTER(ct.c_int))
+    st = lib().compensation_terms(res.ctypes.data_as(ct.c_void_p), 0, b.wf_ptr, a.wf_ptr, 0, 0, 0, 1, None, None,
+                                  None, None, ip, ip, None, None, None, None, None, 0)
     assert st == PP_ERR_UNSUPPORTED
+    st = lib().compensation_terms(res.ctypes.data_as(ct.c_void_p), 0, b.wf_ptr, a.wf_ptr, 0, 1, 0, 0, None, None,
+                                  ip, None, None, None, None, None, None, None, None, 0)
+    assert st == PP_ERR_ARG
     assert b"overlap_setup_real" in lib().pp_last_error()
+    assert not np.any(res)
+
+
+def _grid_encut(wl):
+    abc = np.linalg.norm(wl.lattice, axis=1)
+    return float(np.max((np.pi * np.asarray(wl.fftg) / abc) ** 2 / 0.262))
+
+
+@pytest.mark.parametrize("kpts", [((0, 0, 0),), ((0, 0, 0), (0.25, -0.25, 0.0))])
+def test_projector_unmatched_sites_or_os(gpu, kpts):
+    """O_M over the matched sites plus O_R/O_S (projector.c:604-672,
+    912-938) over the unmatched ones, through Projector and the
+    reference-named overlap_setup_real / compensation_terms, against the
+    oracle: the filtered partial waves (SBT, projector.c:114-163) as sphere
+    tables of radius wave_rmax on the other structure's grid, projected by
+    the oracle's setup_projections."""
+    from pawpyseed_amd.core import Projector, wavefunction_from_workload
+    wl, CR, CS = _pair(nband=6, kpts=kpts)
+    basis = wavefunction_from_workload(wl, "R", coeffs=CR)
+    wf = wavefunction_from_workload(wl, "S", coeffs=CS)
+    M = np.arange(4, dtype=np.int32)
+    N_R = np.array([4, 5, 6, 7], np.int32)
+    N_S = np.array([7, 4, 6, 5], np.int32)
+    pr = Projector(wf, basis, site_lists=(M, M, N_R, N_S, []))
+    nwk = len(wl.kpts)
+    PR, TR = oracle_projections(wl, wl.frac_R, CR)
+    PS, TS = oracle_projections(wl, wl.frac_S, CS)
+    smooth = orc.smooth_species(wl.species, _grid_encut(wl))
+    Gs = wl.ensure_gvecs()
+    WT_S = orc.site_tables(wl.lattice, wl.fftg, wl.frac_R[N_R], smooth)
+    WP_S = orc.setup_projections(wl.lattice, wl.fftg, wl.kpts, Gs, CS, WT_S)
+    WT_R = orc.site_tables(wl.lattice, wl.fftg, wl.frac_S[N_S], smooth)
+    WP_R = orc.setup_projections(wl.lattice, wl.fftg, wl.kpts, Gs, CR, WT_R)
+    assert np.abs(WP_S).max() > 1e-3 * np.abs(PS).max()  # the term is not negligible
+    for band in (0, 4):
+        got = pr.single_band_projection(band)
+        want = orc.pseudoprojection(CR, CS, band, nwk, 1)
+        want = orc.compensation_terms_m(want, band, nwk, 1, PR, TR, np.zeros(len(wl.frac_R), np.int32),
+                                        PS[:, band], TS, M, M, [wl.delta_o], [wl.species.ls])
+        want = orc.compensation_terms_rs(want, nwk, 1, PR, TR, N_R, WP_S[:, band], WT_S, WP_R, WT_R, PS[:, band],
+                                         TS, N_S)
+        err = normwise(got, want)
+        print("O_M + O_R + O_S band", band, "normwise", err, flush=True)
+        assert err < 1e-9
+
+

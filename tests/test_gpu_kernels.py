@@ -108,6 +108,32 @@ def test_projections_fp64_ragged_band_chunk(ctx):
     assert normwise(got, want) < 1e-9
 
 
+def test_projections_two_species(ctx):
+    """Sites of two elements with different projector sets (9 and 18
+    projectors, different radii) in one plan; FP64 and FP32 paths."""
+    from pawpyseed_amd.core import batched as B
+    wl = synthetic.tiny_config(encut=150.0, fftg=(24, 24, 24), nband=8, kpts=[[0, 0, 0], [0.25, -0.25, 0.1]],
+                               ls=(0, 1, 2), alphas=(1.5, 3.0, 1.5))
+    sp2 = synthetic.make_species([0, 0, 1, 1, 2, 2], [1.2, 2.5, 1.4, 2.8, 1.6, 3.0], 0.95)
+    species = [wl.species, sp2]
+    labels = (np.arange(len(wl.frac_S)) % 2).astype(np.int32)
+    Gs = wl.ensure_gvecs()
+    Cs = wl.coefficients("S", nband=8)
+    T = orc.site_tables(wl.lattice, wl.fftg, wl.frac_S, species, labels)
+    want = orc.setup_projections(wl.lattice, wl.fftg, wl.kpts, Gs, Cs, T)
+    Cd, Gd, gofs, nw = device_inputs(Gs, Cs)
+    for precision, tol in (("fp64", 1e-9), ("fp32", 1e-5)):
+        plan = B.SpherePlan(ctx, wl.lattice, wl.fftg, wl.frac_S, labels, species, precision)
+        assert np.array_equal(plan.proj_off, T.proj_off)
+        out = torch.zeros((len(Gs), 8, plan.ptot), dtype=torch.complex128, device="cuda")
+        B.setup_projections(ctx, plan, wl.kpts, gofs, Gd, Cd, out, precision)
+        torch.cuda.synchronize()
+        err = normwise(out.cpu().numpy(), want)
+        print("projections two species", precision, err, flush=True)
+        plan.close()
+        assert err < tol
+
+
 def test_projections_fp64_cfg0(ctx):
     """64-site cell of the 400 eV / 60^3 workload, Gamma, 16 bands."""
     wl = synthetic.config0()
@@ -185,6 +211,94 @@ def test_overlap_mixed(ctx, precision, tol):
     err = normwise(got, want)
     print("overlap", precision, "normwise error", err)
     assert err < tol
+
+
+def test_overlap_near_orthogonal_pairs(ctx):
+    """Pairs of nearly orthogonal bands: S band i = d_i + eps_i c_i with d_i
+    orthogonal to R band c_i, eps_i from 1e-8 to 1, so overlap entries span
+    eight decades.  FP64: elementwise relative error <= 1e-6 on every entry
+    with |O| > 1e-8; FP32 / 3xBF16 are held to their normwise bounds."""
+    from pawpyseed_amd.core import batched as B
+    wl = synthetic.tiny_config(encut=250.0, nband=9, kpts=[[0.1, -0.2, 0.3]])
+    rng = np.random.default_rng(11)
+    G = wl.ensure_gvecs()[0]
+    c = wl.coefficients("R", rng=rng, nband=9)[0].astype(np.complex128)
+    d = wl.coefficients("S", rng=rng, nband=9)[0].astype(np.complex128)
+    eps = 10.0 ** -np.arange(9.0)[::-1]  # 1e-8 ... 1
+    S = np.empty_like(d)
+    for i in range(9):
+        di = d[i] - np.vdot(c[i], d[i]) / np.vdot(c[i], c[i]) * c[i]
+        di /= np.linalg.norm(di)
+        S[i] = di + eps[i] * c[i] / np.linalg.norm(c[i])
+    CR, CS = [c.astype(np.complex64)], [S.astype(np.complex64)]
+    CRd, _, _, nw = device_inputs([G], CR)
+    CSd, _, _, _ = device_inputs([G], CS)
+    want = np.stack([orc.pseudoprojection(CR, CS, b, 1, 1) for b in range(9)], axis=1)  # [bR, bS]
+    diag = np.abs(np.diag(want))
+    assert diag.min() < 1e-7 and diag.max() > 0.5  # the stored pairs really span the range
+    for precision, tol in (("fp64", None), ("fp32", 1e-6), ("bf16x3", 2e-5)):
+        out = torch.zeros((1, 9, 9), dtype=torch.complex128, device="cuda")
+        B.overlap_gemm(ctx, 1, 1, False, nw, CRd, CSd, out, precision)
+        got = out.cpu().numpy()[0]
+        big = np.abs(want) > 1e-8
+        rel = float((np.abs(got - want)[big] / np.abs(want)[big]).max())
+        print("near-orthogonal overlap", precision, "max elementwise rel err (|O|>1e-8)", rel,
+              "normwise", normwise(got, want), flush=True)
+        if tol is None:
+            assert rel < 1e-6
+        else:
+            assert normwise(got, want) < tol
+
+
+def test_uneven_structures_onto_one_basis(ctx):
+    """Three perturbed structures with different band counts onto one
+    basis, at random (non-grid) k-points whose plane-wave counts differ:
+    projections of every structure, band overlaps (FP64) and O_M, checked
+    against the oracle's setup_projections, pseudoprojection and
+    compensation_terms."""
+    from pawpyseed_amd.core import batched as B
+    rng = np.random.default_rng(5)
+    kp = rng.uniform(-0.5, 0.5, (3, 3))
+    wl = synthetic.tiny_config(encut=140.0, fftg=(24, 24, 24), nband=10, kpts=kp, ls=(0, 1, 2),
+                               alphas=(1.5, 3.0, 2.0))
+    Gs = wl.ensure_gvecs()
+    nk = len(Gs)
+    assert len({len(g) for g in Gs}) > 1
+    nbR = 10
+    CR = wl.coefficients("R", rng=rng, nband=nbR)
+    frac_R = wl.frac_R
+    planR = B.SpherePlan(ctx, wl.lattice, wl.fftg, frac_R, np.zeros(len(frac_R), np.int32), [wl.species], "fp64")
+    CRd, Gd, gofs, nw = device_inputs(Gs, CR)
+    projR = torch.zeros((nk, nbR, planR.ptot), dtype=torch.complex128, device="cuda")
+    B.setup_projections(ctx, planR, wl.kpts, gofs, Gd, CRd, projR, "fp64")
+    wantR, TR = oracle_projections(wl, frac_R, CR)
+    assert normwise(projR.cpu().numpy(), wantR) < 1e-9
+    sites = np.arange(len(frac_R), dtype=np.int32)
+    labels = np.zeros(len(frac_R), np.int32)
+    for j, nbS in enumerate((7, 12, 9)):
+        cart = frac_R @ wl.lattice + rng.normal(0.0, 0.03, frac_R.shape)
+        frac_S = np.mod(cart @ np.linalg.inv(wl.lattice), 1.0)
+        CS = wl.coefficients("S", rng=rng, nband=nbS)
+        planS = B.SpherePlan(ctx, wl.lattice, wl.fftg, frac_S, labels, [wl.species], "fp64")
+        CSd, _, _, _ = device_inputs(Gs, CS)
+        projS = torch.zeros((nk, nbS, planS.ptot), dtype=torch.complex128, device="cuda")
+        B.setup_projections(ctx, planS, wl.kpts, gofs, Gd, CSd, projS, "fp64")
+        wantS, TS = oracle_projections(wl, frac_S, CS)
+        assert normwise(projS.cpu().numpy(), wantS) < 1e-9
+        O = torch.zeros((nk, nbR, nbS), dtype=torch.complex128, device="cuda")
+        B.overlap_gemm(ctx, nk, 1, False, nw, CRd, CSd, O, "fp64")
+        B.augmentation(ctx, nk, 1, False, projR, planR.proj_off, labels, projS, planS.proj_off, labels, sites,
+                       sites, [wl.species], [wl.delta_o], O)
+        got = O.cpu().numpy()
+        for band in range(nbS):
+            res = orc.pseudoprojection(CR, CS, band, nk, 1)
+            res = orc.compensation_terms_m(res, band, nk, 1, wantR, TR, labels, wantS[:, band], TS, sites, sites,
+                                           [wl.delta_o], [wl.species.ls])
+            want = res.reshape(nbR, nk).T
+            assert normwise(got[:, :, band], want) < 1e-9, (j, band)
+        print("structure", j, "nbS", nbS, "ok", flush=True)
+        planS.close()
+    planR.close()
 
 
 def test_augmentation_matches_oracle(ctx):
