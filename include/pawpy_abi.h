@@ -278,6 +278,16 @@ int pp_smooth_diffwave(int n, const double *wave_grid, const double *diffwave,
                        int l, int lmax, double grid_encut, int np, double *sdw,
                        double *kgrid, double *kwave);
 
+/* Overlap of the functions f1/r1 Y_{l1 m1} at the origin and f2/r2 Y_{l2 m2}
+ * at dcoord (cartesian), as int phi_1 conj(phi_2) d^3r, from their
+ * spherical Bessel transforms (encut 1e5 eV) on the log grids r1[n1],
+ * r2[n2]; l <= 3.  Replaces pawpyc.reciprocal_offsite_wave_overlap
+ * (pawpyc.pyx:151-190) over radial.c:116-196.  out2 = (re, im). */
+int pp_offsite_overlap(const double *dcoord, int n1, const double *r1,
+                       const double *f1, int n2, const double *r2,
+                       const double *f2, int l1, int m1, int l2, int m2,
+                       double *out2);
+
 #ifdef __cplusplus
 }
 #endif

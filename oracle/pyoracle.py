@@ -411,3 +411,39 @@ def compensation_terms_n(overlap, nwk, nspin, proj_R, T_R, proj_S_band, T_S, N_R
                                   _p(flat, _d), off.ctypes.data_as(ct.POINTER(ct.c_long)), _p(dc, _d), _p(kp, _d),
                                   int(bool(flip)))
     return ov
+
+
+def min_cart_path(coord, center, lattice):
+    """utils.c:51-73: minimum-image cartesian vector coord - center (fractional inputs) and its length."""
+    c = _c(coord, np.float64)
+    z = _c(center, np.float64)
+    lat = _c(lattice, np.float64)
+    path = np.zeros(3)
+    r = ct.c_double(0.0)
+    lib().or_min_cart_path(_p(c, _d), _p(z, _d), _p(lat, _d), _p(path, _d), ct.byref(r))
+    return path, r.value
+
+
+def offsite_blocks(species_R, species_S, lattice, frac_R, frac_S, N_RS_R, N_RS_S, grid_encut):
+    """projector.c:674-712: per pair the [P_R, P_S] block conj(overlap) of the
+    partial-wave differences (transforms from smooth_diffwave's kwave) and
+    the minimum-image vectors from the R site to the S site."""
+    def kw(sp):
+        lmax = max(sp.ls)
+        return [smooth_diffwave(sp.grid, np.asarray(ae) - np.asarray(ps), l, lmax, grid_encut, sp.npts)[1:]
+                for l, ae, ps in zip(sp.ls, sp.aewaves, sp.pswaves)]
+    kR, kS = kw(species_R), kw(species_S)
+    ovs, dcs = [], []
+    for s1, s2 in zip(N_RS_R, N_RS_S):
+        dc, _ = min_cart_path(frac_S[s2], frac_R[s1], lattice)
+        rows = []
+        for (kg1, kw1), l1 in zip(kR, species_R.ls):
+            for m1 in range(-l1, l1 + 1):
+                row = []
+                for (kg2, kw2), l2 in zip(kS, species_S.ls):
+                    for m2 in range(-l2, l2 + 1):
+                        row.append(np.conj(offsite_overlap(dc, kg1, kw1, kg2, kw2, l1, m1, l2, m2)))
+                rows.append(row)
+        ovs.append(np.array(rows))
+        dcs.append(dc)
+    return ovs, np.array(dcs)

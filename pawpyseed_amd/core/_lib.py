@@ -100,6 +100,8 @@ SIGNATURES = {
                                   c_double_p, c_double_p]),
     "pp_smooth_diffwave": (ct.c_int, [ct.c_int, c_double_p, c_double_p, ct.c_int, ct.c_int, ct.c_double, ct.c_int,
                                       c_double_p, c_double_p, c_double_p]),
+    "pp_offsite_overlap": (ct.c_int, [c_double_p, ct.c_int, c_double_p, c_double_p, ct.c_int, c_double_p,
+                                      c_double_p, ct.c_int, ct.c_int, ct.c_int, ct.c_int, c_double_p]),
 }
 
 

@@ -3,19 +3,23 @@
 Mirrors pawpyseed/core/projector.py (Projector, make_site_lists,
 setup_overlap, single_band_projection, proportion_conduction,
 defect_band_analysis) with the 'aug_real' and 'pseudo' methods.  The
-pseudo-wavefunction overlap, the on-site O_M term and the O_R/O_S terms of
-sites that differ between the structures run in libpawpy_amd.so
-(overlap_setup_real, compensation_terms).  The off-site O_N term of
-N_RS pairs (overlapping augmentation spheres of unmatched sites) is not
-provided: a Projector with N_RS pairs raises PAWpyError unless
-``site_lists`` are given.
+pseudo-wavefunction overlap and every augmentation term run in
+libpawpy_amd.so: the on-site O_M term, the O_R/O_S terms of sites that
+differ between the structures and the off-site O_N term of unmatched sites
+with overlapping augmentation spheres (overlap_setup_real,
+compensation_terms).
 """
 
 from __future__ import annotations
 
 import numpy as np
 
-from ._lib import PAWpyError
+from ._lib import PAWpyError, c_int_p as _int_p, ptr as _ptr
+
+
+def _arr(a):
+    """int32 array as a C pointer, NULL when empty (pawpyc.pyx:666-671)."""
+    return _ptr(a, _int_p) if len(a) else None
 
 
 class Projector:
@@ -79,25 +83,21 @@ class Projector:
             N_R, N_S, N_RS = (sl[2], sl[3], sl[4]) if len(sl) == 5 else ([], [], [])
         else:
             M_R, M_S, N_R, N_S, N_RS = self.make_site_lists()
-        if len(N_RS):
-            raise PAWpyError("%d pairs of unmatched sites have overlapping augmentation spheres; the off-site "
-                             "O_N term they need is not provided by this build (pass site_lists to select "
-                             "the terms)" % len(N_RS))
         self.M_R = np.asarray(M_R, np.int32)
         self.M_S = np.asarray(M_S, np.int32)
         self.N_R = np.asarray(N_R, np.int32)
         self.N_S = np.asarray(N_S, np.int32)
-        self.site_cat = [self.M_R, self.M_S, self.N_R, self.N_S, [], []]
-        if len(self.N_R) or len(self.N_S):
-            from ._lib import c_double_p, c_int_p, check, lib, ptr
-            nr = self.N_R if len(self.N_R) else None
-            ns = self.N_S if len(self.N_S) else None
-            check(lib().overlap_setup_real(self.basis.wf_ptr, self.wf.wf_ptr, ptr(self.basis.nums, c_int_p),
-                                           ptr(self.wf.nums, c_int_p), ptr(self.basis.coords, c_double_p),
-                                           ptr(self.wf.coords, c_double_p),
-                                           None if nr is None else ptr(nr, c_int_p),
-                                           None if ns is None else ptr(ns, c_int_p), None, None,
-                                           len(self.N_R), len(self.N_S), 0), "overlap_setup_real")
+        N_RS = [tuple(p) for p in N_RS]
+        self.N_RS_R = np.asarray([p[0] for p in N_RS], np.int32)
+        self.N_RS_S = np.asarray([p[1] for p in N_RS], np.int32)
+        self.site_cat = [self.M_R, self.M_S, self.N_R, self.N_S, self.N_RS_R, self.N_RS_S]
+        if len(self.N_R) or len(self.N_S) or len(self.N_RS_R):
+            from ._lib import c_double_p, check, lib, ptr
+            check(lib().overlap_setup_real(self.basis.wf_ptr, self.wf.wf_ptr, ptr(self.basis.nums, _int_p),
+                                           ptr(self.wf.nums, _int_p), ptr(self.basis.coords, c_double_p),
+                                           ptr(self.wf.coords, c_double_p), _arr(self.N_R), _arr(self.N_S),
+                                           _arr(self.N_RS_R), _arr(self.N_RS_S), len(self.N_R), len(self.N_S),
+                                           len(self.N_RS_R)), "overlap_setup_real")
 
     def _single_band_projection_pseudo(self, band_num, flip_spin=False):
         return self.wf.pseudoprojection(band_num, self.basis, flip_spin)
@@ -111,17 +111,14 @@ class Projector:
         """pawpyc.pyx:685-702 -> compensation_terms (O_M, O_R, O_S)."""
         import ctypes as ct
 
-        from ._lib import c_double_p, c_int_p, check, lib, ptr
-
-        def arr(a):
-            return ptr(a, c_int_p) if len(a) else None
-
+        from ._lib import c_double_p, check, lib, ptr
         check(lib().compensation_terms(res.ctypes.data_as(ct.c_void_p), int(band_num), self.wf.wf_ptr,
-                                       self.basis.wf_ptr, len(self.M_R), len(self.N_R), len(self.N_S), 0,
-                                       arr(self.M_R), arr(self.M_S), arr(self.N_R), arr(self.N_S), None, None,
-                                       ptr(self.wf.nums, c_int_p), ptr(self.wf.coords, c_double_p),
-                                       ptr(self.basis.nums, c_int_p), ptr(self.basis.coords, c_double_p),
-                                       ptr(self.wf.dimv, c_int_p), int(bool(flip_spin))), "compensation_terms")
+                                       self.basis.wf_ptr, len(self.M_R), len(self.N_R), len(self.N_S),
+                                       len(self.N_RS_R), _arr(self.M_R), _arr(self.M_S), _arr(self.N_R),
+                                       _arr(self.N_S), _arr(self.N_RS_R), _arr(self.N_RS_S),
+                                       ptr(self.wf.nums, _int_p), ptr(self.wf.coords, c_double_p),
+                                       ptr(self.basis.nums, _int_p), ptr(self.basis.coords, c_double_p),
+                                       ptr(self.wf.dimv, _int_p), int(bool(flip_spin))), "compensation_terms")
 
     def single_band_projection(self, band_num, **kwargs):
         """projector.py:238-272: overlaps <basis; b,k,s | wf; band_num,k,s>

@@ -119,10 +119,17 @@ static int setup_batched(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac
   int maxw = 0;
   for (int k = 0; k < nk; k++)
     maxw = std::max(maxw, gofs[k + 1] - gofs[k]);
+  const long npts_sph = std::max<long>(1, sphere_npts(sp));
+  const bool two = ctx->tune_streams > 1 && (long)nk * ((nband + NB - 1) / NB) > 1;
   PP_TRY(ctx->grid.ensure(sizeof(cplx<T>) * (size_t)npts_grid * NB));
-  PP_TRY(ctx->gidx.ensure(sizeof(int) * (size_t)std::max(1, maxw)));
-  PP_TRY(ctx->phase.ensure(sizeof(cplx<T>) * (size_t)std::max<long>(1, sphere_npts(sp))));
-  cplx<T> *grid = (cplx<T> *)ctx->grid.p;
+  if (two) {
+    PP_TRY(ctx->grid2.ensure(sizeof(cplx<T>) * (size_t)npts_grid * NB));
+    if (!ctx->aux)
+      PP_CHECK_HIP(hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking));
+  }
+  PP_TRY(ctx->gidx.ensure(sizeof(int) * (size_t)std::max(1, gofs[nk])));
+  PP_TRY(ctx->phase.ensure(sizeof(cplx<T>) * (size_t)npts_sph * nk));
+  cplx<T> *grids[2] = {(cplx<T> *)ctx->grid.p, two ? (cplx<T> *)ctx->grid2.p : (cplx<T> *)ctx->grid.p};
   cplx<T> *phase = (cplx<T> *)ctx->phase.p;
   int *gidx = (int *)ctx->gidx.p;
   // plane-wave sphere bounds of every k (one host sync for the alias flags)
@@ -136,22 +143,52 @@ static int setup_batched(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac
     PP_CHECK_HIP(hipMemcpyAsync(&alias[k], bnd + (size_t)k * nbnd + nbnd - 1, sizeof(int), hipMemcpyDeviceToHost,
                                 ctx->stream));
   PP_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  // grid indices and Bloch phases of every k, so the chunks of all k can be
+  // in flight on both streams
   for (int k = 0; k < nk; k++) {
-    int nw = gofs[k + 1] - gofs[k];
-    PP_TRY(compute_gidx(ctx, fftg, nw, G_dev + 3 * (long)gofs[k], gidx));
-    PP_TRY(make_phase<T>(ctx, sp, kfrac + 3 * k, phase));
+    PP_TRY(compute_gidx(ctx, fftg, gofs[k + 1] - gofs[k], G_dev + 3 * (long)gofs[k], gidx + gofs[k]));
+    PP_TRY(make_phase<T>(ctx, sp, kfrac + 3 * k, phase + (size_t)k * npts_sph));
+  }
+  // Chunk pipeline: chunk c runs on stream c % 2 with buffer set c % 2, so
+  // one chunk's sphere sums overlap the next chunk's FFT passes.
+  hipStream_t streams[2] = {ctx->stream, two ? ctx->aux : ctx->stream};
+  hipEvent_t fork = nullptr, join = nullptr;
+  if (two) {
+    PP_CHECK_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    PP_CHECK_HIP(hipEventCreateWithFlags(&join, hipEventDisableTiming));
+    PP_CHECK_HIP(hipEventRecord(fork, ctx->stream));
+    PP_CHECK_HIP(hipStreamWaitEvent(ctx->aux, fork, 0));
+  }
+  int st = PP_OK;
+  long c = 0;
+  for (int k = 0; k < nk && st == PP_OK; k++) {
+    const int nw = gofs[k + 1] - gofs[k];
     const int *kb = bnd + (size_t)k * nbnd;
     SphereBounds bd = {kb, kb + 2 * fftg[0] * fftg[1], kb + 2 * fftg[0] * fftg[1] + 2 * fftg[0], fftg[0], fftg[1],
                        fftg[2]};
     const bool prune = alias[k] == 0 && ctx->tune_fft_prune != 2;
-    for (int b0 = 0; b0 < nband; b0 += NB) {
-      int nbc = std::min(NB, nband - b0);
-      PP_TRY(fft_chunk<T>(ctx, pl, grid, NB, nw, gidx, C + ((long)k * nband + b0) * ldc, ldc, nbc,
-                          prune ? &bd : nullptr));
-      PP_TRY(sphere_sums<T>(ctx, sp, grid, NB, nbc, phase, proj + ((long)k * nband + b0) * ptot, ptot));
+    for (int b0 = 0; b0 < nband && st == PP_OK; b0 += NB, c++) {
+      const int nbc = std::min(NB, nband - b0);
+      const int s = two ? (int)(c & 1) : 0;
+      ctx->stream = streams[s];
+      ctx->slot = s;
+      st = fft_chunk<T>(ctx, pl, grids[s], NB, nw, gidx + gofs[k], C + ((long)k * nband + b0) * ldc, ldc, nbc,
+                        prune ? &bd : nullptr);
+      if (st == PP_OK)
+        st = sphere_sums<T>(ctx, sp, grids[s], NB, nbc, phase + (size_t)k * npts_sph,
+                            proj + ((long)k * nband + b0) * ptot, ptot);
     }
   }
-  return PP_OK;
+  ctx->stream = streams[0];
+  ctx->slot = 0;
+  if (two) {
+    // the caller's stream sees everything the second stream did
+    (void)hipEventRecord(join, ctx->aux);
+    (void)hipStreamWaitEvent(ctx->stream, join, 0);
+    (void)hipEventDestroy(fork);
+    (void)hipEventDestroy(join);
+  }
+  return st;
 }
 
 }  // namespace pp
@@ -195,6 +232,10 @@ void pp_ctx_destroy(pp_ctx *ctx) {
   }
   for (auto e : ctx->timer.pool)
     (void)hipEventDestroy(e);
+  if (ctx->aux) {
+    (void)hipStreamSynchronize(ctx->aux);
+    (void)hipStreamDestroy(ctx->aux);
+  }
   if (ctx->own)
     (void)hipStreamDestroy(ctx->own);
   delete ctx;
@@ -268,6 +309,8 @@ int pp_ctx_set_param(pp_ctx *ctx, const char *name, int value) {
     ctx->tune_gemm_f64_valu = value != 0;
   else if (n == "fft_persistent")
     ctx->tune_fft_persistent = value < 0 ? 0 : value;
+  else if (n == "streams")
+    ctx->tune_streams = value > 1 ? 2 : 1;
   else if (n == "fft_tile")
     return pp_ctx_set_tuning(ctx, ctx->tune_fft_bands, value, ctx->tune_fft_prune, ctx->tune_sphere_parts,
                              ctx->tune_splitk);

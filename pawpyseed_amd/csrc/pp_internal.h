@@ -78,12 +78,15 @@ struct pp_ctx {
   hipStream_t stream = nullptr;
   int tune_fft_bands = 0, tune_sphere_parts = 0, tune_splitk = 0, tune_fft_tile = 0, tune_fft_prune = 0;
   int tune_fft_pingpong = 0;
-  int tune_fft_persistent = 0;
-  int tune_gemm_f64_valu = 0;  // 1: FP64 GEMM on the VALU instead of the matrix cores  // 0 off, 1 on (2048 work-groups), >1 work-group count
-  pp::DevBuf grid;      // band-fastest FFT chunk
-  pp::DevBuf partial;   // sphere-sum partials
+  int tune_fft_persistent = 0;  // 0 off, 1 on (2048 work-groups), > 1 work-group count
+  int tune_gemm_f64_valu = 0;   // 1: FP64 GEMM on the VALU instead of the matrix cores
+  int tune_streams = 2;         // band chunks in flight (1 or 2 streams)
+  hipStream_t aux = nullptr;    // second stream of the chunk pipeline (default priority)
+  int slot = 0;                 // chunk buffer set in use (0 or 1)
+  pp::DevBuf grid, grid2;       // band-fastest FFT chunk, one per slot
+  pp::DevBuf partial, partial2; // sphere-sum partials, one per slot
   pp::DevBuf gidx;      // wrapped grid index per G
-  pp::DevBuf phase;     // per-point Bloch phase of the current k
+  pp::DevBuf phase;     // per-point Bloch phase of every k
   pp::DevBuf gemm_ws;   // split-K partials
   pp::DevBuf aug_ws;    // gathered / dO-applied projections
   pp::DevBuf bounds;    // plane-wave sphere bounds per k

@@ -354,4 +354,32 @@ int pp_smooth_diffwave(int n, const double *wave_grid, const double *diffwave, i
   return smooth_diffwave_host(n, wave_grid, diffwave, l, lmax, grid_encut, np, sdw, kgrid, kwave);
 }
 
+int pp_offsite_overlap(const double *dcoord, int n1, const double *r1, const double *f1, int n2, const double *r2,
+                       const double *f2, int l1, int m1, int l2, int m2, double *out2) {
+  PP_REQUIRE(dcoord && r1 && f1 && r2 && f2 && out2 && n1 >= 2 && n2 >= 2 && l1 >= 0 && l2 >= 0 && l1 <= 3 &&
+                 l2 <= 3 && std::abs(m1) <= l1 && std::abs(m2) <= l2 && r1[0] > 0 && r1[1] > r1[0] && r2[0] > 0 &&
+                 r2[1] > r2[0],
+             PP_ERR_ARG, "pp_offsite_overlap: bad arguments");
+  // pawpyc.pyx:151-190: both functions transformed at encut 1e5 eV
+  std::vector<double> k1(n1), g1(n1), k2(n2), g2(n2);
+  SbtPlan p1, p2;
+  sbt_plan(p1, 1e5, 0.0, l1, n1, r1);
+  sbt_forward(p1, f1, l1, g1.data());
+  sbt_plan(p2, 1e5, 0.0, l2, n2, r2);
+  sbt_forward(p2, f2, l2, g2.data());
+  for (int q = 0; q < n1; q++)
+    k1[q] = p1.ks[q];
+  for (int q = 0; q < n2; q++)
+    k2[q] = p2.ks[q];
+  std::vector<double> b1(n1), c1(n1), d1(n1), b2(n2), c2(n2), d2(n2);
+  spline_coeff_host(k1.data(), g1.data(), n1, b1.data(), c1.data(), d1.data());
+  spline_coeff_host(k2.data(), g2.data(), n2, b2.data(), c2.data(), d2.data());
+  const double *s1[3] = {b1.data(), c1.data(), d1.data()};
+  const double *s2[3] = {b2.data(), c2.data(), d2.data()};
+  const cd v = offsite_overlap_host(dcoord, k1.data(), g1.data(), s1, n1, k2.data(), g2.data(), s2, n2, l1, m1, l2, m2);
+  out2[0] = v.real();
+  out2[1] = v.imag();
+  return PP_OK;
+}
+
 }  // extern "C"

@@ -638,8 +638,9 @@ int sphere_sums(pp_ctx *ctx, pp_sphere *sp, const cplx<T> *grid, int NB, int nbc
       nparts = 1;
   }
   int PM = sp->pmax <= 8 ? 8 : sp->pmax <= 18 ? 18 : sp->pmax <= 32 ? 32 : 64;
-  PP_TRY(ctx->partial.ensure(sizeof(cplx<double>) * (size_t)sp->nsites * nparts * NB * PM));
-  cplx<double> *partial = (cplx<double> *)ctx->partial.p;
+  DevBuf &pbuf = ctx->slot ? ctx->partial2 : ctx->partial;
+  PP_TRY(pbuf.ensure(sizeof(cplx<double>) * (size_t)sp->nsites * nparts * NB * PM));
+  cplx<double> *partial = (cplx<double> *)pbuf.p;
   hipEvent_t ev;
   tbegin(ctx, K_SPHERE_SUM, &ev);
   switch (PM) {

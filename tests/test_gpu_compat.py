@@ -4,8 +4,10 @@ reference's Python API (Wavefunction, Projector.single_band_projection),
 against the oracle's restatement of the same path:
 
 res = pseudoprojection (pseudoprojector.c:63-90)
-      + compensation_terms O_M (projector.c:850-910),
-with <p|psi> from setup_projections (projector.c:560-602).
+      + compensation_terms O_M, O_R, O_S, O_N (projector.c:850-963),
+with <p|psi> from setup_projections (projector.c:560-602) and the
+wave projections / off-site overlaps of overlap_setup_real
+(projector.c:604-720).
 Tolerance: 1e-9 relative (normwise) for the FP64 path.
 """
 
@@ -86,13 +88,76 @@ def test_defect_band_analysis_runs(gpu):
         assert abs((v + c) - np.sum(np.abs(res) ** 2)) < 1e-10
 
 
-def test_unmatched_sites_raise(gpu):
-    from pawpyseed_amd.core import PAWpyError, Projector, wavefunction_from_workload
+def _grid_encut(wl):
+    abc = np.linalg.norm(wl.lattice, axis=1)
+    return float(np.max((np.pi * np.asarray(wl.fftg) / abc) ** 2 / 0.262))
+
+
+def _oracle_all_terms(wl, CR, CS, band, M_R, M_S, N_R, N_S, N_RS):
+    """pseudoprojection + O_M + O_R + O_S + O_N (projector.c:850-963) from
+    the oracle, with the filtered partial waves and off-site overlaps of
+    projector.c:114-163, 604-720."""
+    nwk = len(wl.kpts)
+    Gs = wl.ensure_gvecs()
+    PR, TR = oracle_projections(wl, wl.frac_R, CR)
+    PS, TS = oracle_projections(wl, wl.frac_S, CS)
+    ge = _grid_encut(wl)
+    smooth = orc.smooth_species(wl.species, ge)
+    res = orc.pseudoprojection(CR, CS, band, nwk, 1)
+    res = orc.compensation_terms_m(res, band, nwk, 1, PR, TR, np.zeros(len(wl.frac_R), np.int32), PS[:, band], TS,
+                                   np.asarray(M_R, np.int32), np.asarray(M_S, np.int32), [wl.delta_o],
+                                   [wl.species.ls])
+    if len(N_R) or len(N_S):
+        N_R, N_S = np.asarray(N_R, np.int32), np.asarray(N_S, np.int32)
+        WT_S = orc.site_tables(wl.lattice, wl.fftg, wl.frac_R[N_R], smooth)
+        WP_S = orc.setup_projections(wl.lattice, wl.fftg, wl.kpts, Gs, CS, WT_S)
+        WT_R = orc.site_tables(wl.lattice, wl.fftg, wl.frac_S[N_S], smooth)
+        WP_R = orc.setup_projections(wl.lattice, wl.fftg, wl.kpts, Gs, CR, WT_R)
+        res = orc.compensation_terms_rs(res, nwk, 1, PR, TR, N_R, WP_S[:, band], WT_S, WP_R, WT_R, PS[:, band], TS,
+                                        N_S)
+    if len(N_RS):
+        rr, ss = [p[0] for p in N_RS], [p[1] for p in N_RS]
+        ovs, dcs = orc.offsite_blocks(wl.species, wl.species, wl.lattice, wl.frac_R, wl.frac_S, rr, ss, ge)
+        res = orc.compensation_terms_n(res, nwk, 1, PR, TR, PS[:, band], TS, rr, ss, ovs, dcs, wl.kpts)
+    return res
+
+
+@pytest.mark.parametrize("kpts", [((0, 0, 0),), ((0, 0, 0), (0.25, -0.25, 0.0))])
+def test_projector_offsite_pairs(gpu, kpts):
+    """All four augmentation terms with explicit site lists, including N_RS
+    pairs of unmatched sites (O_N, projector.c:940-959, with the off-site
+    overlaps of radial.c:116-196), against the oracle."""
+    from pawpyseed_amd.core import Projector, wavefunction_from_workload
+    wl, CR, CS = _pair(nband=5, kpts=kpts)
+    basis = wavefunction_from_workload(wl, "R", coeffs=CR)
+    wf = wavefunction_from_workload(wl, "S", coeffs=CS)
+    M = [0, 1, 2, 3]
+    N_R, N_S = [4, 5, 6, 7], [7, 4, 6, 5]
+    N_RS = [(4, 4), (5, 5), (6, 6), (7, 7), (4, 6)]
+    pr = Projector(wf, basis, site_lists=(M, M, N_R, N_S, N_RS))
+    for band in (0, 3):
+        got = pr.single_band_projection(band)
+        want = _oracle_all_terms(wl, CR, CS, band, M, M, N_R, N_S, N_RS)
+        no_on = _oracle_all_terms(wl, CR, CS, band, M, M, N_R, N_S, [])
+        assert np.abs(want - no_on).max() > 1e-6 * np.abs(want).max()  # the O_N term is not negligible
+        err = normwise(got, want)
+        print("O_M + O_R + O_S + O_N band", band, "normwise", err, flush=True)
+        assert err < 1e-9
+
+
+def test_projector_default_site_lists(gpu):
+    """make_site_lists (projector.py:115-160) on a jittered structure: no
+    site within 0.02 A, so every term but O_M is used."""
+    from pawpyseed_amd.core import Projector, wavefunction_from_workload
     wl, CR, CS = _pair(nband=4, kpts=((0, 0, 0),))
     basis = wavefunction_from_workload(wl, "R", coeffs=CR)
     wf = wavefunction_from_workload(wl, "S", coeffs=CS)
-    with pytest.raises(PAWpyError):
-        Projector(wf, basis)  # jittered sites -> O_R/O_S terms would be needed
+    pr = Projector(wf, basis)
+    M_R, M_S, N_R, N_S, N_RS = pr.make_site_lists()
+    assert len(N_R) == len(wl.frac_R) and len(N_RS) >= len(N_R)
+    got = pr.single_band_projection(1)
+    want = _oracle_all_terms(wl, CR, CS, 1, M_R, M_S, N_R, N_S, N_RS)
+    assert normwise(got, want) < 1e-9
 
 
 def test_identity_projection_diagonal(gpu):
@@ -127,10 +192,10 @@ def test_compat_fft3d(gpu):
 
 
 def test_compensation_terms_rejects_unmatched_lists(gpu):
-    """N_RS pairs (off-site O_N) are not provided; N_R lists need a
-    preceding overlap_setup_real with the same list."""
+    """N_RS and N_R lists need a preceding overlap_setup_real with the
+    same lists."""
     from pawpyseed_amd.core import wavefunction_from_workload
-    from pawpyseed_amd.core._lib import PP_ERR_ARG, PP_ERR_UNSUPPORTED, lib
+    from pawpyseed_amd.core._lib import PP_ERR_ARG, lib
     wl, CR, CS = _pair(nband=3, kpts=((0, 0, 0),))
     a = wavefunction_from_workload(wl, "R", coeffs=CR, setup_projectors=True)
     b = wavefunction_from_workload(wl, "S", coeffs=CS, setup_projectors=True)
@@ -139,17 +204,12 @@ def test_compensation_terms_rejects_unmatched_lists(gpu):
     ip = one.ctypes.data_as(ct.POINTER(ct.c_int))
     st = lib().compensation_terms(res.ctypes.data_as(ct.c_void_p), 0, b.wf_ptr, a.wf_ptr, 0, 0, 0, 1, None, None,
                                   None, None, ip, ip, None, None, None, None, None, 0)
-    assert st == PP_ERR_UNSUPPORTED
+    assert st == PP_ERR_ARG
     st = lib().compensation_terms(res.ctypes.data_as(ct.c_void_p), 0, b.wf_ptr, a.wf_ptr, 0, 1, 0, 0, None, None,
                                   ip, None, None, None, None, None, None, None, None, 0)
     assert st == PP_ERR_ARG
     assert b"overlap_setup_real" in lib().pp_last_error()
     assert not np.any(res)
-
-
-def _grid_encut(wl):
-    abc = np.linalg.norm(wl.lattice, axis=1)
-    return float(np.max((np.pi * np.asarray(wl.fftg) / abc) ** 2 / 0.262))
 
 
 @pytest.mark.parametrize("kpts", [((0, 0, 0),), ((0, 0, 0), (0.25, -0.25, 0.0))])
