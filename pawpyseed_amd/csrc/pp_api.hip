@@ -138,11 +138,44 @@ static int setup_batched(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac
   int *bnd = (int *)ctx->bounds.p;
   for (int k = 0; k < nk; k++)
     PP_TRY(sphere_bounds(ctx, fftg, gofs[k + 1] - gofs[k], G_dev + 3 * (long)gofs[k], bnd + (size_t)k * nbnd));
-  std::vector<int> alias(nk, 1);
-  for (int k = 0; k < nk; k++)
-    PP_CHECK_HIP(hipMemcpyAsync(&alias[k], bnd + (size_t)k * nbnd + nbnd - 1, sizeof(int), hipMemcpyDeviceToHost,
-                                ctx->stream));
+  ctx->bounds_host.resize((size_t)nbnd * nk);
+  PP_CHECK_HIP(hipMemcpyAsync(ctx->bounds_host.data(), bnd, sizeof(int) * ctx->bounds_host.size(),
+                              hipMemcpyDeviceToHost, ctx->stream));
   PP_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  std::vector<int> alias(nk, 1);
+  // populated lines of the z pass ((x, y) in the sphere's hull) and of the
+  // y pass ((x, z) with x in the slab), so no work-group is launched for an
+  // empty line
+  std::vector<long> zl_off(nk + 1, 0), yl_off(nk + 1, 0);
+  ctx->lines_host.clear();
+  const int n0 = fftg[0], n1 = fftg[1], n2 = fftg[2];
+  for (int k = 0; k < nk; k++) {
+    const int *hb = ctx->bounds_host.data() + (size_t)k * nbnd;
+    alias[k] = hb[nbnd - 1];
+    const int *yb = hb + 2 * n0 * n1, *xb = yb + 2 * n0;
+    zl_off[k] = (long)ctx->lines_host.size();
+    if (!alias[k])
+      for (int sx = xb[0]; sx <= xb[1]; sx++) {
+        const int x = (sx + n0) % n0;
+        for (int sy = yb[2 * x]; sy <= yb[2 * x + 1]; sy++)
+          ctx->lines_host.push_back(x * n1 + (sy + n1) % n1);
+      }
+    yl_off[k] = (long)ctx->lines_host.size();
+    if (!alias[k])
+      for (int sx = xb[0]; sx <= xb[1]; sx++)
+        for (int z = 0; z < n2; z++)
+          ctx->lines_host.push_back(((sx + n0) % n0) * n2 + z);
+    zl_off[k + 1] = (long)ctx->lines_host.size();
+  }
+  int *dlines = nullptr;
+  if (!ctx->lines_host.empty()) {
+    PP_TRY(ctx->lines.ensure(sizeof(int) * ctx->lines_host.size()));
+    dlines = (int *)ctx->lines.p;
+    // the streams are idle here (synchronised above), and lines_host lives in
+    // the context until the next call's synchronisation
+    PP_CHECK_HIP(hipMemcpyAsync(dlines, ctx->lines_host.data(), sizeof(int) * ctx->lines_host.size(),
+                                hipMemcpyHostToDevice, ctx->stream));
+  }
   // grid indices and Bloch phases of every k, so the chunks of all k can be
   // in flight on both streams
   for (int k = 0; k < nk; k++) {
@@ -166,6 +199,12 @@ static int setup_batched(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac
     const int *kb = bnd + (size_t)k * nbnd;
     SphereBounds bd = {kb, kb + 2 * fftg[0] * fftg[1], kb + 2 * fftg[0] * fftg[1] + 2 * fftg[0], fftg[0], fftg[1],
                        fftg[2]};
+    if (dlines && !alias[k] && ctx->tune_fft_prune != 3) {
+      bd.zl = dlines + zl_off[k];
+      bd.nzl = (int)(yl_off[k] - zl_off[k]);
+      bd.yl = dlines + yl_off[k];
+      bd.nyl = (int)(zl_off[k + 1] - yl_off[k]);
+    }
     const bool prune = alias[k] == 0 && ctx->tune_fft_prune != 2;
     for (int b0 = 0; b0 < nband && st == PP_OK; b0 += NB, c++) {
       const int nbc = std::min(NB, nband - b0);

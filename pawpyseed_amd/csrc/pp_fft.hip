@@ -234,6 +234,14 @@ __device__ void stage_generic(const cplx<T> *__restrict__ in, cplx<T> *__restric
 // Signed frequency of wrapped index i on an axis of length n.
 __device__ __forceinline__ int sgn_idx(int i, int n) { return i < n - n / 2 ? i : i - n; }
 
+// Line l of a pass: from the populated-line list of the z / y pass when
+// there is one, else l itself.
+template <int AXIS>
+__device__ __forceinline__ int pass_line(const SphereBounds &bd, int l) {
+  const int *ll = AXIS == 2 ? bd.zl : AXIS == 1 ? bd.yl : nullptr;
+  return ll ? ll[l] : l;
+}
+
 // One pass of 1D transforms along an axis.  Work-group: one line (fixed
 // other two coordinates) x NBW bands.  Element e of band slot b lives at
 // g[base + e*es + b].
@@ -256,7 +264,7 @@ __global__ __launch_bounds__(256) void k_fft_pass(cplx<T> *__restrict__ g, AxisP
   cplx<T> *buf1 = buf0 + n * NBW;
   const int blk = blockIdx.x;
   const int slice = blk % nslices;
-  const int line = blk / nslices;
+  const int line = pass_line<AXIS>(bd, blk / nslices);
   const int l1 = line / n2, l2 = line % n2;
   int lo = -(1 << 30), hi = 1 << 30;
   if (bd.xb) {
@@ -378,7 +386,7 @@ __global__ __launch_bounds__(256) void k_fft_pass_ip(cplx<T> *__restrict__ g, Ax
   const int n = pl.n;
   const int blk = blockIdx.x;
   const int slice = blk % nslices;
-  const int line = blk / nslices;
+  const int line = pass_line<AXIS>(bd, blk / nslices);
   const int l1 = line / n2, l2 = line % n2;
   int lo = -(1 << 30), hi = 1 << 30;
   if (bd.xb) {
@@ -472,12 +480,12 @@ __global__ __launch_bounds__(256) void k_fft_pass_pl(cplx<T> *__restrict__ g, Ax
   long line = blockIdx.x;
   int lo = 0, hi = 0;
   for (; line < nlines; line += gridDim.x) {
-    const long ln = line / nslices;
+    const long ln = pass_line<AXIS>(bd, (int)(line / nslices));
     if (line_range<AXIS>(bd, (int)(ln / n2), (int)(ln % n2), lo, hi))
       break;
   }
   auto load = [&](long L, int llo, int lhi) {
-    const long ln = L / nslices;
+    const long ln = pass_line<AXIS>(bd, (int)(L / nslices));
     const int slice = (int)(L % nslices);
     const long base = (ln / n2) * s1 + (ln % n2) * s2 + (long)slice * NBW;
 #pragma unroll
@@ -507,7 +515,7 @@ __global__ __launch_bounds__(256) void k_fft_pass_pl(cplx<T> *__restrict__ g, Ax
     long next = line + gridDim.x;
     int nlo = 0, nhi = 0;
     for (; next < nlines; next += gridDim.x) {
-      const long ln = next / nslices;
+      const long ln = pass_line<AXIS>(bd, (int)(next / nslices));
       if (line_range<AXIS>(bd, (int)(ln / n2), (int)(ln % n2), nlo, nhi))
         break;
     }
@@ -526,7 +534,7 @@ __global__ __launch_bounds__(256) void k_fft_pass_pl(cplx<T> *__restrict__ g, Ax
         stage_ip<T, NBW, 5, MAXB>(buf, tws, p, pm, Tn);
     }
     {
-      const long ln = line / nslices;
+      const long ln = pass_line<AXIS>(bd, (int)(line / nslices));
       const int slice = (int)(line % nslices);
       const long base = (ln / n2) * s1 + (ln % n2) * s2 + (long)slice * NBW;
       for (int t = threadIdx.x; t < tot; t += 256) {
@@ -549,21 +557,30 @@ __global__ __launch_bounds__(256) void k_bounds_init(int *__restrict__ bnd, int 
 
 __global__ __launch_bounds__(256) void k_bounds(int n0, int n1, int n2, int nw, const int *__restrict__ G,
                                                 int *__restrict__ bnd) {
-  int w = blockIdx.x * 256 + threadIdx.x;
-  if (w >= nw)
-    return;
+  const int w = blockIdx.x * 256 + threadIdx.x;
   int *zb = bnd, *yb = bnd + 2 * n0 * n1, *xb = yb + 2 * n0, *flag = xb + 2;
-  const int g0 = G[3 * w], g1 = G[3 * w + 1], g2 = G[3 * w + 2];
-  const int x = ((g0 % n0) + n0) % n0, y = ((g1 % n1) + n1) % n1, z = ((g2 % n2) + n2) % n2;
-  const int sx = sgn_idx(x, n0), sy = sgn_idx(y, n1), sz = sgn_idx(z, n2);
-  if (sx != g0 || sy != g1 || sz != g2)
-    atomicOr(flag, 1);  // the grid aliases this G: no pruning
-  atomicMin(&zb[2 * (x * n1 + y)], sz);
-  atomicMax(&zb[2 * (x * n1 + y) + 1], sz);
-  atomicMin(&yb[2 * x], sy);
-  atomicMax(&yb[2 * x + 1], sy);
-  atomicMin(&xb[0], sx);
-  atomicMax(&xb[1], sx);
+  int lo = 1 << 30, hi = -(1 << 30);
+  if (w < nw) {
+    const int g0 = G[3 * w], g1 = G[3 * w + 1], g2 = G[3 * w + 2];
+    const int x = ((g0 % n0) + n0) % n0, y = ((g1 % n1) + n1) % n1, z = ((g2 % n2) + n2) % n2;
+    const int sx = sgn_idx(x, n0), sy = sgn_idx(y, n1), sz = sgn_idx(z, n2);
+    if (sx != g0 || sy != g1 || sz != g2)
+      atomicOr(flag, 1);  // the grid aliases this G: no pruning
+    atomicMin(&zb[2 * (x * n1 + y)], sz);
+    atomicMax(&zb[2 * (x * n1 + y) + 1], sz);
+    atomicMin(&yb[2 * x], sy);
+    atomicMax(&yb[2 * x + 1], sy);
+    lo = hi = sx;
+  }
+  // the x extent: one atomic pair per wavefront after a shuffle reduction
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = min(lo, __shfl_xor(lo, o));
+    hi = max(hi, __shfl_xor(hi, o));
+  }
+  if ((threadIdx.x & 63) == 0 && lo <= hi) {
+    atomicMin(&xb[0], lo);
+    atomicMax(&xb[1], hi);
+  }
 }
 
 template <typename T>
@@ -609,7 +626,10 @@ template <typename T, int NBW, int AXIS>
 static int launch_pass(pp_ctx *ctx, cplx<T> *grid, const AxisPlan &ap, const cplx<T> *tw, long es, long s1,
                        long s2, int n1, int n2, int NB, int cls, const SphereBounds &bd) {
   int nslices = NB / NBW;
-  long nblk = (long)n1 * n2 * nslices;
+  const long nlin = (AXIS == 2 && bd.zl) ? bd.nzl : (AXIS == 1 && bd.yl) ? bd.nyl : (long)n1 * n2;
+  long nblk = nlin * nslices;
+  if (nblk == 0)
+    return PP_OK;
   // in-place form when every radix is 2..5 and each thread holds <= 8 butterflies
   int maxb = 0;
   bool ip = ctx->tune_fft_pingpong == 0 && NBW >= 8;

@@ -53,6 +53,10 @@ struct AxisPlan {
 struct SphereBounds {
   const int *zb, *yb, *xb;
   int n0, n1, n2;
+  // populated lines of the z and y passes (packed l1 * n2 + l2 of the
+  // pass), nullptr = every line
+  const int *zl = nullptr, *yl = nullptr;
+  int nzl = 0, nyl = 0;
 };
 
 struct FftPlan {
@@ -90,6 +94,8 @@ struct pp_ctx {
   pp::DevBuf gemm_ws;   // split-K partials
   pp::DevBuf aug_ws;    // gathered / dO-applied projections
   pp::DevBuf bounds;    // plane-wave sphere bounds per k
+  pp::DevBuf lines;     // populated z / y pass lines per k
+  std::vector<int> lines_host, bounds_host;
   std::map<long long, pp::FftPlan *> fft_plans;
   pp::Timer timer;
 };
