@@ -125,6 +125,29 @@ int overlap_setup_real(pswf_t *wf_R, pswf_t *wf_S, int *labels_R, int *labels_S,
                        int *N_RS_R, int *N_RS_S, int num_N_R, int num_N_S,
                        int num_N_RS);
 
+/* replaces projector.c:727-848 overlap_setup_recip (the aug_recip method):
+ * the plane-wave coefficients of each structure's augmentation functions
+ * at its own unmatched sites (R's bands at N_R, S's at N_S; the CAs of
+ * projector.c:420-453), computed on the GPU from the resident projections
+ * (expansion on the grid, forward FFT, G sampling) and kept in HBM (FP64;
+ * band->CAs gets float complex copies); plus the N_RS off-site overlaps as
+ * in overlap_setup_real.  Call setup_projections on both first. */
+int overlap_setup_recip(pswf_t *wf_R, pswf_t *wf_S, int *labels_R, int *labels_S,
+                        double *coords_R, double *coords_S, int *N_R, int *N_S,
+                        int *N_RS_R, int *N_RS_S, int num_N_R, int num_N_S,
+                        int num_N_RS);
+
+/* replaces projector.c:965-1077 compensation_terms_recip (adds into
+ * overlap): <CA_R|C_S> + <C_R|CA_S> over the R k-point's plane waves, plus
+ * O_M and O_N, for all band pairs at once (FP64 GEMMs), cached like
+ * compensation_terms. */
+int compensation_terms_recip(pp_cdouble *overlap, int BAND_NUM, pswf_t *wf_S,
+                             pswf_t *wf_R, int num_M, int num_N_R, int num_N_S,
+                             int num_N_RS, int *M_R, int *M_S, int *N_R, int *N_S,
+                             int *N_RS_R, int *N_RS_S, int *proj_labels,
+                             double *proj_coords, int *ref_labels,
+                             double *ref_coords, int *fft_grid, int flip_spin);
+
 /* replaces projector.c:850-963 compensation_terms (adds into overlap).
  * O_M, O_R, O_S and O_N are computed for all band pairs at once (FP64) and
  * cached per (wf_R, flip_spin, site lists); O_R/O_S/O_N need a preceding
@@ -257,6 +280,19 @@ int pp_augmentation_offsite(pp_ctx *ctx, int nwk, int nspin, int flip,
 int pp_fft3d_batched(pp_ctx *ctx, const double *lattice, const int *fftg,
                      int nw, const int *G_dev, const void *C_dev, long ldc,
                      int nband, void *grid_out_dev, int precision);
+
+/* Plane-wave coefficients of the augmentation functions of a plan's
+ * sites for every band and k (the CAs of projector.c:420-453): per band,
+ * x(r) = sum over the plan's sites s and projectors j of
+ * P[k][b][src_off[s] + j] p_j(r) e^{-i k.path}, forward-transformed with
+ * scale sqrt(V)/N^3 (linalg.c:47-79) and sampled at the G vectors of k.
+ * proj_dev c128 [nk][nband][ptot_proj]; out_dev c128 [nk][nband][ldo]
+ * (the reference keeps them as float complex).  The plan's table
+ * precision selects the FFT precision. */
+int pp_aug_freqs_batched(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac,
+                         const int *gofs, const int *G_dev, const void *proj_dev,
+                         long ptot_proj, const int *src_off, int nband,
+                         void *out_dev, long ldo, int precision);
 
 /* ---- per-species radial setup (host) ---- */
 

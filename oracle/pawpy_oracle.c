@@ -1150,3 +1150,56 @@ void or_compensation_terms_n(double *overlap, int nwk, int nspin, int nbR, const
     overlap[2 * w + 1] += ti;
   }
 }
+
+/* ------------------------------------------------------------------ */
+/* aug_recip: plane-wave coefficients of the augmentation functions     */
+/* ------------------------------------------------------------------ */
+
+/* projector.c:276-330 + 420-453 + linalg.c:47-79 for one band:
+ * x(r) = sum_s sum_p P[proj_off_P[s] + p] values_p(r) e^{-i k.path(r)} over
+ * the site tables (values = p(r) as stored, not conjugated), then the
+ * forward transform with scale sqrt(V)/N^3 sampled at the band's G
+ * vectors: CA[w] = sqrt(V)/N^3 sum_r x(r) e^{-2 pi i G.r/N}.  P holds the
+ * band's projections at the sites of the tables (site s at proj_off_P[s]).
+ * CA is returned in FP64 (the reference stores it as float complex). */
+void or_aug_freqs(double *CA, const double *lattice, const double *kfrac, const int *fftg, const int *Gs, int nw,
+                  int nsites, const int *site_pt, const int *site_nproj, const long *val_off, const int *idx,
+                  const double *path, const double *values, const double *P, const int *proj_off_P) {
+  const long n = (long)fftg[0] * fftg[1] * fftg[2];
+  double *x = (double *)calloc(2 * (size_t)n, sizeof(double));
+  double rec[9], kc[3];
+  reclat(lattice, rec);
+  frac_to_cart(kfrac, rec, kc);
+  for (int s = 0; s < nsites; s++) {
+    int p0 = site_pt[s], np = site_pt[s + 1] - site_pt[s];
+    for (int i = 0; i < np; i++) {
+      const double *pa = path + 3 * (long)(p0 + i);
+      double kr = -(kc[0] * pa[0] + kc[1] * pa[1] + kc[2] * pa[2]);
+      double er = cos(kr), ei = sin(kr);
+      double ar = 0.0, ai = 0.0;
+      for (int p = 0; p < site_nproj[s]; p++) {
+        const double *v = values + 2 * (val_off[s] + (long)p * np + i);
+        const double *pp = P + 2 * (proj_off_P[s] + p);
+        ar += pp[0] * v[0] - pp[1] * v[1];
+        ai += pp[0] * v[1] + pp[1] * v[0];
+      }
+      long g = idx[p0 + i];
+      x[2 * g] += ar * er - ai * ei;
+      x[2 * g + 1] += ar * ei + ai * er;
+    }
+  }
+  /* forward DFT = conj(backward DFT(conj x)) */
+  for (long i = 0; i < n; i++)
+    x[2 * i + 1] = -x[2 * i + 1];
+  or_ifft3d_inplace(x, fftg);
+  const double scl = sqrt(det3(lattice)) / (double)n;
+  for (int w = 0; w < nw; w++) {
+    int g1 = ((Gs[3 * w] % fftg[0]) + fftg[0]) % fftg[0];
+    int g2 = ((Gs[3 * w + 1] % fftg[1]) + fftg[1]) % fftg[1];
+    int g3 = ((Gs[3 * w + 2] % fftg[2]) + fftg[2]) % fftg[2];
+    long g = ((long)g1 * fftg[1] + g2) * fftg[2] + g3;
+    CA[2 * w] = x[2 * g] * scl;
+    CA[2 * w + 1] = -x[2 * g + 1] * scl;
+  }
+  free(x);
+}

@@ -150,6 +150,11 @@ void or_compensation_terms_n(double *overlap, int nwk, int nspin, int nbR, const
                              int num_N_RS, const int *N_RS_R, const int *N_RS_S, const double *ov,
                              const long *ov_off, const double *dcoords, const double *kpts, int flip);
 
+/* ---- aug_recip: projector.c:276-330, 420-453; linalg.c:47-79 ---- */
+void or_aug_freqs(double *CA, const double *lattice, const double *kfrac, const int *fftg, const int *Gs, int nw,
+                  int nsites, const int *site_pt, const int *site_nproj, const long *val_off, const int *idx,
+                  const double *path, const double *values, const double *P, const int *proj_off_P);
+
 #ifdef __cplusplus
 }
 #endif

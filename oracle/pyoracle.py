@@ -447,3 +447,23 @@ def offsite_blocks(species_R, species_S, lattice, frac_R, frac_S, N_RS_R, N_RS_S
         ovs.append(np.array(rows))
         dcs.append(dc)
     return ovs, np.array(dcs)
+
+
+def aug_freqs(lattice, fftg, kfrac, G, T: SiteTables, P, proj_off_P=None):
+    """projector.c:276-330, 420-453 with linalg.c:47-79: the plane-wave
+    coefficients (c128 [nw]) of sum_s sum_p P_p p_p(r) e^{-i k.path} over the
+    tables' sites; P holds the band's projections, table site s at
+    proj_off_P[s] (default: T.proj_off)."""
+    lattice = _c(lattice, np.float64)
+    fftg = _c(fftg, np.int32)
+    kf = _c(kfrac, np.float64)
+    G = _c(G, np.int32)
+    P = _c(P, np.complex128)
+    po = _c(T.proj_off if proj_off_P is None else proj_off_P, np.int32)
+    vals = _c(T.values, np.complex128)
+    CA = np.zeros(len(G), np.complex128)
+    lib().or_aug_freqs(_p(CA, _d), _p(lattice, _d), _p(kf, _d), _p(fftg, _i), _p(G, _i), len(G), T.nsites,
+                       _p(_c(T.site_pt, np.int32), _i), _p(_c(T.nproj, np.int32), _i),
+                       _c(T.val_off, np.int64).ctypes.data_as(ct.POINTER(ct.c_long)), _p(_c(T.idx, np.int32), _i),
+                       _p(_c(T.path, np.float64), _d), _p(vals, _d), _p(P, _d), _p(po, _i))
+    return CA

@@ -65,6 +65,13 @@ SIGNATURES = {
     "compensation_terms": (ct.c_int, [c_void_p, ct.c_int, c_void_p, c_void_p, ct.c_int, ct.c_int, ct.c_int,
                                       ct.c_int, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p,
                                       c_double_p, c_int_p, c_double_p, c_int_p, ct.c_int]),
+    "overlap_setup_recip": (ct.c_int, [c_void_p, c_void_p, c_int_p, c_int_p, c_double_p, c_double_p, c_int_p,
+                                       c_int_p, c_int_p, c_int_p, ct.c_int, ct.c_int, ct.c_int]),
+    "compensation_terms_recip": (ct.c_int, [c_void_p, ct.c_int, c_void_p, c_void_p, ct.c_int, ct.c_int, ct.c_int,
+                                            ct.c_int, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p,
+                                            c_double_p, c_int_p, c_double_p, c_int_p, ct.c_int]),
+    "pp_aug_freqs_batched": (ct.c_int, [c_void_p, c_void_p, ct.c_int, c_double_p, c_int_p, c_void_p, c_void_p,
+                                        ct.c_long, c_int_p, ct.c_int, c_void_p, ct.c_long, ct.c_int]),
     "fft3d": (ct.c_int, [c_void_p, c_int_p, c_double_p, c_double_p, c_int_p, c_void_p, ct.c_int, c_int_p]),
     "pp_get_projections": (ct.c_int, [c_void_p, ct.c_int, ct.c_int, c_void_p]),
     "pp_ctx_create": (ct.c_int, [ct.POINTER(c_void_p), ct.c_int]),

@@ -2,7 +2,7 @@
 
 Mirrors pawpyseed/core/projector.py (Projector, make_site_lists,
 setup_overlap, single_band_projection, proportion_conduction,
-defect_band_analysis) with the 'aug_real' and 'pseudo' methods.  The
+defect_band_analysis) with the 'aug_real', 'aug_recip' and 'pseudo' methods.  The
 pseudo-wavefunction overlap and every augmentation term run in
 libpawpy_amd.so: the on-site O_M term, the O_R/O_S terms of sites that
 differ between the structures and the off-site O_N term of unmatched sites
@@ -23,7 +23,7 @@ def _arr(a):
 
 
 class Projector:
-    METHODS = ["pseudo", "aug_real"]
+    METHODS = ["pseudo", "aug_real", "aug_recip"]
 
     def __init__(self, wf, basis, unsym_basis=False, unsym_wf=False, method="aug_real", site_lists=None):
         self.method = method
@@ -31,7 +31,9 @@ class Projector:
             self._single_band_projection = self._single_band_projection_pseudo
         elif method == "aug_real":
             self._single_band_projection = self._single_band_projection_aug_real
-        elif method in ("aug_recip", "realspace"):
+        elif method == "aug_recip":
+            self._single_band_projection = self._single_band_projection_aug_recip
+        elif method == "realspace":
             raise PAWpyError(f"method {method} is not provided by this build")
         else:
             raise PAWpyError("method not recognized for Projector")
@@ -91,13 +93,14 @@ class Projector:
         self.N_RS_R = np.asarray([p[0] for p in N_RS], np.int32)
         self.N_RS_S = np.asarray([p[1] for p in N_RS], np.int32)
         self.site_cat = [self.M_R, self.M_S, self.N_R, self.N_S, self.N_RS_R, self.N_RS_S]
-        if len(self.N_R) or len(self.N_S) or len(self.N_RS_R):
+        recip = self.method == "aug_recip"
+        if recip or len(self.N_R) or len(self.N_S) or len(self.N_RS_R):
             from ._lib import c_double_p, check, lib, ptr
-            check(lib().overlap_setup_real(self.basis.wf_ptr, self.wf.wf_ptr, ptr(self.basis.nums, _int_p),
-                                           ptr(self.wf.nums, _int_p), ptr(self.basis.coords, c_double_p),
-                                           ptr(self.wf.coords, c_double_p), _arr(self.N_R), _arr(self.N_S),
-                                           _arr(self.N_RS_R), _arr(self.N_RS_S), len(self.N_R), len(self.N_S),
-                                           len(self.N_RS_R)), "overlap_setup_real")
+            fn = lib().overlap_setup_recip if recip else lib().overlap_setup_real
+            check(fn(self.basis.wf_ptr, self.wf.wf_ptr, ptr(self.basis.nums, _int_p), ptr(self.wf.nums, _int_p),
+                     ptr(self.basis.coords, c_double_p), ptr(self.wf.coords, c_double_p), _arr(self.N_R),
+                     _arr(self.N_S), _arr(self.N_RS_R), _arr(self.N_RS_S), len(self.N_R), len(self.N_S),
+                     len(self.N_RS_R)), "overlap_setup_recip" if recip else "overlap_setup_real")
 
     def _single_band_projection_pseudo(self, band_num, flip_spin=False):
         return self.wf.pseudoprojection(band_num, self.basis, flip_spin)
@@ -107,18 +110,24 @@ class Projector:
         self._add_augmentation_terms(res, band_num, flip_spin)
         return res
 
-    def _add_augmentation_terms(self, res, band_num, flip_spin):
+    def _single_band_projection_aug_recip(self, band_num, flip_spin=False):
+        """projector.py:225-236: pseudoprojection + compensation_terms_recip."""
+        res = self.wf.pseudoprojection(band_num, self.basis, flip_spin)
+        self._add_augmentation_terms(res, band_num, flip_spin, recip=True)
+        return res
+
+    def _add_augmentation_terms(self, res, band_num, flip_spin, recip=False):
         """pawpyc.pyx:685-702 -> compensation_terms (O_M, O_R, O_S)."""
         import ctypes as ct
 
         from ._lib import c_double_p, check, lib, ptr
-        check(lib().compensation_terms(res.ctypes.data_as(ct.c_void_p), int(band_num), self.wf.wf_ptr,
-                                       self.basis.wf_ptr, len(self.M_R), len(self.N_R), len(self.N_S),
-                                       len(self.N_RS_R), _arr(self.M_R), _arr(self.M_S), _arr(self.N_R),
-                                       _arr(self.N_S), _arr(self.N_RS_R), _arr(self.N_RS_S),
-                                       ptr(self.wf.nums, _int_p), ptr(self.wf.coords, c_double_p),
-                                       ptr(self.basis.nums, _int_p), ptr(self.basis.coords, c_double_p),
-                                       ptr(self.wf.dimv, _int_p), int(bool(flip_spin))), "compensation_terms")
+        fn = lib().compensation_terms_recip if recip else lib().compensation_terms
+        check(fn(res.ctypes.data_as(ct.c_void_p), int(band_num), self.wf.wf_ptr, self.basis.wf_ptr, len(self.M_R),
+                 len(self.N_R), len(self.N_S), len(self.N_RS_R), _arr(self.M_R), _arr(self.M_S), _arr(self.N_R),
+                 _arr(self.N_S), _arr(self.N_RS_R), _arr(self.N_RS_S), ptr(self.wf.nums, _int_p),
+                 ptr(self.wf.coords, c_double_p), ptr(self.basis.nums, _int_p), ptr(self.basis.coords, c_double_p),
+                 ptr(self.wf.dimv, _int_p), int(bool(flip_spin))),
+              "compensation_terms_recip" if recip else "compensation_terms")
 
     def single_band_projection(self, band_num, **kwargs):
         """projector.py:238-272: overlaps <basis; b,k,s | wf; band_num,k,s>

@@ -97,11 +97,16 @@ static void timer_collect(pp_ctx *ctx) {
 template <typename T>
 int sphere_sums(pp_ctx *ctx, pp_sphere *sp, const cplx<T> *grid, int NB, int nbc, const cplx<T> *phase,
                 cplx<double> *out, long ptot);
-template <typename T> int make_phase(pp_ctx *ctx, pp_sphere *sp, const double *kfrac, cplx<T> *phase);
+template <typename T> int make_phase(pp_ctx *ctx, pp_sphere *sp, const double *kfrac, cplx<T> *phase, bool expand);
+template <typename T>
+int sphere_expand(pp_ctx *ctx, pp_sphere *sp, cplx<T> *grid, int NB, int nbc, const cplx<T> *phase,
+                  const cplx<double> *P, long ptotP, const int *src_off_dev);
+int sphere_nsites(pp_sphere *sp);
 long sphere_npts(pp_sphere *sp);
 int sphere_ptot(pp_sphere *sp);
 int sphere_precision(pp_sphere *sp);
 const int *sphere_fftg(pp_sphere *sp);
+const double *sphere_lattice(pp_sphere *sp);
 
 template <typename T>
 static int setup_batched(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac, const int *gofs, const int *G_dev,
@@ -180,7 +185,7 @@ static int setup_batched(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac
   // in flight on both streams
   for (int k = 0; k < nk; k++) {
     PP_TRY(compute_gidx(ctx, fftg, gofs[k + 1] - gofs[k], G_dev + 3 * (long)gofs[k], gidx + gofs[k]));
-    PP_TRY(make_phase<T>(ctx, sp, kfrac + 3 * k, phase + (size_t)k * npts_sph));
+    PP_TRY(make_phase<T>(ctx, sp, kfrac + 3 * k, phase + (size_t)k * npts_sph, false));
   }
   // Chunk pipeline: chunk c runs on stream c % 2 with buffer set c % 2, so
   // one chunk's sphere sums overlap the next chunk's FFT passes.
@@ -230,6 +235,44 @@ static int setup_batched(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac
   return st;
 }
 
+// projector.c:420-453 for every band and k of one structure: the
+// augmentation functions of the plan's sites weighted by the band's
+// projections, forward-transformed (scale sqrt(V)/N^3, linalg.c:47-79) and
+// sampled at the band's G vectors.  One band chunk at a time on one stream.
+template <typename T>
+static int aug_freqs_batched(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac, const int *gofs,
+                             const int *G_dev, const cplx<double> *P, long ptotP, const int *src_off_dev, int nband,
+                             cplx<double> *out, long ldo) {
+  const int *fftg = sphere_fftg(sp);
+  FftPlan *pl = nullptr;
+  PP_TRY(get_fft_plan(ctx, fftg, &pl));
+  const int NB = std::min(64, std::max(1, nband));
+  const long npts_grid = (long)fftg[0] * fftg[1] * fftg[2];
+  const long npts_sph = std::max<long>(1, sphere_npts(sp));
+  PP_TRY(ctx->grid.ensure(sizeof(cplx<T>) * (size_t)npts_grid * NB));
+  PP_TRY(ctx->gidx.ensure(sizeof(int) * (size_t)std::max(1, gofs[nk])));
+  PP_TRY(ctx->phase.ensure(sizeof(cplx<T>) * (size_t)npts_sph));
+  cplx<T> *grid = (cplx<T> *)ctx->grid.p;
+  cplx<T> *phase = (cplx<T> *)ctx->phase.p;
+  int *gidx = (int *)ctx->gidx.p;
+  const double scl = sqrt(det3(sphere_lattice(sp))) / (double)npts_grid;
+  ctx->slot = 0;
+  for (int k = 0; k < nk; k++) {
+    const int nw = gofs[k + 1] - gofs[k];
+    PP_TRY(compute_gidx(ctx, fftg, nw, G_dev + 3 * (long)gofs[k], gidx));
+    PP_TRY(make_phase<T>(ctx, sp, kfrac + 3 * k, phase, true));
+    for (int b0 = 0; b0 < nband; b0 += NB) {
+      const int nbc = std::min(NB, nband - b0);
+      PP_TRY(zero_async(ctx, grid, sizeof(cplx<T>) * (size_t)npts_grid * NB));
+      PP_TRY(sphere_expand<T>(ctx, sp, grid, NB, nbc, phase, P + ((long)k * nband + b0) * ptotP, ptotP,
+                              src_off_dev));
+      PP_TRY(fft_passes<T>(ctx, pl, grid, NB));
+      PP_TRY(gather_conj<T>(ctx, grid, NB, nbc, gidx, nw, scl, out + ((long)k * nband + b0) * ldo, ldo));
+    }
+  }
+  return PP_OK;
+}
+
 }  // namespace pp
 
 using namespace pp;
@@ -239,6 +282,44 @@ extern "C" {
 const char *pp_last_error(void) { return g_err.c_str(); }
 int pp_last_status(void) { return g_code; }
 const char *pp_version(void) { return "pawpyseed_amd 0.1.0 (gfx950)"; }
+
+int pp_aug_freqs_batched(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac, const int *gofs, const int *G_dev,
+                         const void *proj_dev, long ptot_proj, const int *src_off, int nband, void *out_dev, long ldo,
+                         int precision) {
+  PP_REQUIRE(ctx && sp && nk > 0 && kfrac && gofs && nband >= 0 && ptot_proj >= 0 && ldo >= 0, PP_ERR_ARG,
+             "pp_aug_freqs_batched: bad arguments");
+  PP_REQUIRE(precision == PP_FP64 || precision == PP_FP32, PP_ERR_ARG, "pp_aug_freqs_batched: precision");
+  PP_REQUIRE(sphere_precision(sp) == (precision == PP_FP64 ? PP_FP64 : PP_FP32), PP_ERR_ARG,
+             "pp_aug_freqs_batched: the plan's table precision differs from the call's");
+  if (nband == 0)
+    return PP_OK;
+  const int ns = sphere_nsites(sp);
+  PP_REQUIRE(ns == 0 || (src_off && proj_dev && out_dev), PP_ERR_ARG, "pp_aug_freqs_batched: null argument");
+  for (int k = 0; k < nk; k++)
+    PP_REQUIRE(gofs[k + 1] - gofs[k] >= 0 && gofs[k + 1] - gofs[k] <= ldo, PP_ERR_ARG,
+               "pp_aug_freqs_batched: plane-wave count exceeds ldo");
+  std::vector<int> poff(ns + 1);
+  int ptot_plan = 0;
+  PP_TRY(pp_sphere_info(sp, &ptot_plan, nullptr, poff.data()));
+  for (int s = 0; s < ns; s++)
+    PP_REQUIRE(src_off[s] >= 0 && src_off[s] + (poff[s + 1] - poff[s]) <= ptot_proj, PP_ERR_ARG,
+               "pp_aug_freqs_batched: site offset outside the projection rows");
+  PP_CHECK_HIP(hipSetDevice(ctx->device));
+  PP_TRY(ctx->aug_ws.ensure(sizeof(int) * (size_t)std::max(1, ns)));
+  int *d_src = (int *)ctx->aug_ws.p;
+  if (ns > 0)
+    PP_CHECK_HIP(hipMemcpyAsync(d_src, src_off, sizeof(int) * ns, hipMemcpyHostToDevice, ctx->stream));
+  int st;
+  if (precision == PP_FP64)
+    st = aug_freqs_batched<double>(ctx, sp, nk, kfrac, gofs, G_dev, (const cplx<double> *)proj_dev, ptot_proj, d_src,
+                                   nband, (cplx<double> *)out_dev, ldo);
+  else
+    st = aug_freqs_batched<float>(ctx, sp, nk, kfrac, gofs, G_dev, (const cplx<double> *)proj_dev, ptot_proj, d_src,
+                                  nband, (cplx<double> *)out_dev, ldo);
+  // the host copy of src_off must outlive the async upload
+  PP_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  return st;
+}
 
 int pp_ctx_create(pp_ctx **out, int device) {
   PP_REQUIRE(out, PP_ERR_ARG, "pp_ctx_create: null out");

@@ -53,6 +53,13 @@ struct WfDev {
   std::vector<pp_cdouble> on_ov;
   std::vector<long> on_off;
   std::vector<double> on_dc;
+  // aug_recip (overlap_setup_recip): plane-wave coefficients of this
+  // structure's augmentation functions at its own unmatched sites, c128
+  // [nkt][nband][ca_ld]
+  DevBuf ca;
+  long ca_ld = 0;
+  int ca_n = -1;
+  unsigned long long ca_hash = 0;
 };
 
 static std::mutex g_mu;
@@ -735,6 +742,87 @@ int setup_projections(pswf_t *wf, ppot_t *pps, int num_elems, int num_sites, int
   return fill_host_lists(wf, d->proj, ptot, num_sites, labels, proj_off, pps, false);
 }
 
+// projector.c:674-712 (and :797-840): off-site overlaps of the N_RS pairs
+// from the transforms of the partial-wave differences (radial.c:116-196),
+// kept on wf_S as the reference does.
+static int setup_offsite(pswf_t *wf_R, pswf_t *wf_S, const int *labels_R, const int *labels_S, const double *coords_R,
+                         const double *coords_S, const int *N_RS_R, const int *N_RS_S, int num_N_RS) {
+  for (int t = 0; t < num_N_RS; t++)
+    PP_REQUIRE(N_RS_R[t] >= 0 && N_RS_R[t] < wf_R->num_sites && N_RS_S[t] >= 0 && N_RS_S[t] < wf_S->num_sites,
+               PP_ERR_ARG, "overlap setup: N_RS site out of range");
+  if (wf_S->overlaps) {
+    for (int i = 0; i < wf_S->num_aug_overlap_sites; i++)
+      free(wf_S->overlaps[i]);
+    free(wf_S->overlaps);
+    wf_S->overlaps = nullptr;
+  }
+  free(wf_S->dcoords);
+  wf_S->dcoords = nullptr;
+  WfDev *dS = dev_of(wf_S);
+  dS->on_n = num_N_RS;
+  dS->on_of = wf_R;
+  dS->on_R.assign(N_RS_R, N_RS_R + num_N_RS);
+  dS->on_S.assign(N_RS_S, N_RS_S + num_N_RS);
+  dS->on_ov.clear();
+  dS->on_off.assign(1, 0);
+  dS->on_dc.assign(3 * (size_t)num_N_RS, 0.0);
+  dS->aug_cache.clear();
+  if (num_N_RS > 0) {
+    wf_S->overlaps = (pp_cdouble **)calloc(num_N_RS, sizeof(pp_cdouble *));
+    wf_S->dcoords = (double *)malloc(3 * sizeof(double) * num_N_RS);
+  }
+  for (int t = 0; t < num_N_RS; t++) {
+    const int s1 = N_RS_R[t], s2 = N_RS_S[t];
+    const ppot_t &p1 = wf_R->pps[labels_R[s1]];
+    const ppot_t &p2 = wf_S->pps[labels_S[s2]];
+    PP_REQUIRE(p1.funcs[0].kwave && p2.funcs[0].kwave, PP_ERR_ARG,
+               "overlap_setup_real: projector list without partial-wave transforms");
+    // minimum-image cartesian vector from the R site to the S site
+    double best = INFINITY, dc[3] = {0, 0, 0};
+    for (int a = -1; a <= 1; a++)
+      for (int b = -1; b <= 1; b++)
+        for (int c = -1; c <= 1; c++) {
+          const double f[3] = {coords_S[3 * s2] + a - coords_R[3 * s1], coords_S[3 * s2 + 1] + b - coords_R[3 * s1 + 1],
+                               coords_S[3 * s2 + 2] + c - coords_R[3 * s1 + 2]};
+          double v[3];
+          for (int j = 0; j < 3; j++)
+            v[j] = f[0] * wf_R->lattice[j] + f[1] * wf_R->lattice[3 + j] + f[2] * wf_R->lattice[6 + j];
+          const double r = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+          if (r < best) {
+            best = r;
+            dc[0] = v[0];
+            dc[1] = v[1];
+            dc[2] = v[2];
+          }
+        }
+    memcpy(wf_S->dcoords + 3 * t, dc, sizeof(dc));
+    memcpy(dS->on_dc.data() + 3 * t, dc, sizeof(dc));
+    std::vector<pp_cdouble> blk((size_t)p1.total_projs * p2.total_projs);
+    int tj = 0;
+    for (int j = 0; j < p1.num_projs; j++) {
+      const funcset_t &f1 = p1.funcs[j];
+      for (int m1 = -f1.l; m1 <= f1.l; m1++, tj++) {
+        int tk = 0;
+        for (int k = 0; k < p2.num_projs; k++) {
+          const funcset_t &f2 = p2.funcs[k];
+          for (int m2 = -f2.l; m2 <= f2.l; m2++, tk++)
+            blk[(size_t)tj * p2.total_projs + tk] =
+                std::conj(offsite_overlap_host(dc, p1.kwave_grid, f1.kwave, f1.kwave_spline, p1.wave_gridsize,
+                                               p2.kwave_grid, f2.kwave, f2.kwave_spline, p2.wave_gridsize, f1.l, m1,
+                                               f2.l, m2));
+        }
+      }
+    }
+    wf_S->overlaps[t] = (pp_cdouble *)malloc(sizeof(pp_cdouble) * blk.size());
+    memcpy(wf_S->overlaps[t], blk.data(), sizeof(pp_cdouble) * blk.size());
+    dS->on_ov.insert(dS->on_ov.end(), blk.begin(), blk.end());
+    dS->on_off.push_back((long)dS->on_ov.size());
+  }
+  wf_S->num_aug_overlap_sites = num_N_RS;
+  wf_R->num_aug_overlap_sites = num_N_RS;
+  return PP_OK;
+}
+
 int overlap_setup_real(pswf_t *wf_R, pswf_t *wf_S, int *labels_R, int *labels_S, double *coords_R,
                        double *coords_S, int *N_R, int *N_S, int *N_RS_R, int *N_RS_S, int num_N_R, int num_N_S,
                        int num_N_RS) {
@@ -810,81 +898,7 @@ int overlap_setup_real(pswf_t *wf_R, pswf_t *wf_S, int *labels_R, int *labels_S,
                          d->wp_off));
     PP_TRY(fill_host_lists(sd.wf, d->wp, d->wp_ptot, sd.n, lab.data(), d->wp_off, sd.owner->pps, true));
   }
-  // projector.c:674-712: off-site overlaps of the N_RS pairs from the
-  // transforms of the partial-wave differences (radial.c:116-196)
-  for (int t = 0; t < num_N_RS; t++)
-    PP_REQUIRE(N_RS_R[t] >= 0 && N_RS_R[t] < wf_R->num_sites && N_RS_S[t] >= 0 && N_RS_S[t] < wf_S->num_sites,
-               PP_ERR_ARG, "overlap_setup_real: N_RS site out of range");
-  if (wf_S->overlaps) {
-    for (int i = 0; i < wf_S->num_aug_overlap_sites; i++)
-      free(wf_S->overlaps[i]);
-    free(wf_S->overlaps);
-    wf_S->overlaps = nullptr;
-  }
-  free(wf_S->dcoords);
-  wf_S->dcoords = nullptr;
-  WfDev *dS = dev_of(wf_S);
-  dS->on_n = num_N_RS;
-  dS->on_of = wf_R;
-  dS->on_R.assign(N_RS_R, N_RS_R + num_N_RS);
-  dS->on_S.assign(N_RS_S, N_RS_S + num_N_RS);
-  dS->on_ov.clear();
-  dS->on_off.assign(1, 0);
-  dS->on_dc.assign(3 * (size_t)num_N_RS, 0.0);
-  dS->aug_cache.clear();
-  if (num_N_RS > 0) {
-    wf_S->overlaps = (pp_cdouble **)calloc(num_N_RS, sizeof(pp_cdouble *));
-    wf_S->dcoords = (double *)malloc(3 * sizeof(double) * num_N_RS);
-  }
-  for (int t = 0; t < num_N_RS; t++) {
-    const int s1 = N_RS_R[t], s2 = N_RS_S[t];
-    const ppot_t &p1 = wf_R->pps[labels_R[s1]];
-    const ppot_t &p2 = wf_S->pps[labels_S[s2]];
-    PP_REQUIRE(p1.funcs[0].kwave && p2.funcs[0].kwave, PP_ERR_ARG,
-               "overlap_setup_real: projector list without partial-wave transforms");
-    // minimum-image cartesian vector from the R site to the S site
-    double best = INFINITY, dc[3] = {0, 0, 0};
-    for (int a = -1; a <= 1; a++)
-      for (int b = -1; b <= 1; b++)
-        for (int c = -1; c <= 1; c++) {
-          const double f[3] = {coords_S[3 * s2] + a - coords_R[3 * s1], coords_S[3 * s2 + 1] + b - coords_R[3 * s1 + 1],
-                               coords_S[3 * s2 + 2] + c - coords_R[3 * s1 + 2]};
-          double v[3];
-          for (int j = 0; j < 3; j++)
-            v[j] = f[0] * wf_R->lattice[j] + f[1] * wf_R->lattice[3 + j] + f[2] * wf_R->lattice[6 + j];
-          const double r = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-          if (r < best) {
-            best = r;
-            dc[0] = v[0];
-            dc[1] = v[1];
-            dc[2] = v[2];
-          }
-        }
-    memcpy(wf_S->dcoords + 3 * t, dc, sizeof(dc));
-    memcpy(dS->on_dc.data() + 3 * t, dc, sizeof(dc));
-    std::vector<pp_cdouble> blk((size_t)p1.total_projs * p2.total_projs);
-    int tj = 0;
-    for (int j = 0; j < p1.num_projs; j++) {
-      const funcset_t &f1 = p1.funcs[j];
-      for (int m1 = -f1.l; m1 <= f1.l; m1++, tj++) {
-        int tk = 0;
-        for (int k = 0; k < p2.num_projs; k++) {
-          const funcset_t &f2 = p2.funcs[k];
-          for (int m2 = -f2.l; m2 <= f2.l; m2++, tk++)
-            blk[(size_t)tj * p2.total_projs + tk] =
-                std::conj(offsite_overlap_host(dc, p1.kwave_grid, f1.kwave, f1.kwave_spline, p1.wave_gridsize,
-                                               p2.kwave_grid, f2.kwave, f2.kwave_spline, p2.wave_gridsize, f1.l, m1,
-                                               f2.l, m2));
-        }
-      }
-    }
-    wf_S->overlaps[t] = (pp_cdouble *)malloc(sizeof(pp_cdouble) * blk.size());
-    memcpy(wf_S->overlaps[t], blk.data(), sizeof(pp_cdouble) * blk.size());
-    dS->on_ov.insert(dS->on_ov.end(), blk.begin(), blk.end());
-    dS->on_off.push_back((long)dS->on_ov.size());
-  }
-  wf_S->num_aug_overlap_sites = num_N_RS;
-  wf_R->num_aug_overlap_sites = num_N_RS;
+  PP_TRY(setup_offsite(wf_R, wf_S, labels_R, labels_S, coords_R, coords_S, N_RS_R, N_RS_S, num_N_RS));
   return PP_OK;
 }
 
@@ -1041,6 +1055,199 @@ int compensation_terms(pp_cdouble *overlap, int BAND_NUM, pswf_t *wf_S, pswf_t *
                              (int)dS->sp_nchan.size(), dS->sp_nchan.data(), dS->sp_ls.data(), I.data(), out.p));
     }
     // O_N (:940-959) = sum over N_RS pairs of conj(P_R) Ov P_S e^{2 pi i kR . dcoord}
+    if (num_N_RS > 0) {
+      std::vector<double> kp(3 * (size_t)nkt);
+      for (int k = 0; k < nkt; k++)
+        for (int j = 0; j < 3; j++)
+          kp[3 * k + j] = wf_R->kpts[k]->k[j];
+      PP_TRY(pp_augmentation_offsite(ctx, wf_R->nwk, wf_R->nspin, flip, dR->proj.p, dR->nband, dR->ptot,
+                                     dR->proj_off.data(), dS->proj.p, dS->nband, dS->ptot, dS->proj_off.data(),
+                                     num_N_RS, N_RS_R, N_RS_S, dS->on_ov.data(), dS->on_off.data(),
+                                     dS->on_dc.data(), kp.data(), out.p));
+    }
+    PairCache pc;
+    pc.nbR = dR->nband;
+    pc.nbS = dS->nband;
+    pc.m.resize(n);
+    PP_CHECK_HIP(hipMemcpyAsync(pc.m.data(), out.p, sizeof(pp_cdouble) * n, hipMemcpyDeviceToHost, ctx->stream));
+    PP_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    it = dS->aug_cache.emplace(key, std::move(pc)).first;
+  }
+  const PairCache &pc = it->second;
+  for (int b = 0; b < pc.nbR; b++)
+    for (int k = 0; k < nkt; k++)
+      overlap[(long)b * nkt + k] += pc.m[((size_t)k * pc.nbR + b) * pc.nbS + BAND_NUM];
+  return PP_OK;
+}
+
+int overlap_setup_recip(pswf_t *wf_R, pswf_t *wf_S, int *labels_R, int *labels_S, double *coords_R,
+                        double *coords_S, int *N_R, int *N_S, int *N_RS_R, int *N_RS_S, int num_N_R, int num_N_S,
+                        int num_N_RS) {
+  PP_REQUIRE(wf_R && wf_S && num_N_R >= 0 && num_N_S >= 0 && num_N_RS >= 0, PP_ERR_ARG,
+             "overlap_setup_recip: bad arguments");
+  PP_REQUIRE(wf_R->pps && wf_S->pps && wf_R->fftg && wf_S->fftg && wf_R->dev && wf_S->dev, PP_ERR_ARG,
+             "overlap_setup_recip: call setup_projections on both wavefunctions first");
+  PP_REQUIRE(num_N_R == 0 || (N_R && labels_R && coords_R), PP_ERR_ARG, "overlap_setup_recip: null N_R data");
+  PP_REQUIRE(num_N_S == 0 || (N_S && labels_S && coords_S), PP_ERR_ARG, "overlap_setup_recip: null N_S data");
+  PP_REQUIRE(num_N_RS == 0 || (N_RS_R && N_RS_S && labels_R && labels_S && coords_R && coords_S), PP_ERR_ARG,
+             "overlap_setup_recip: null N_RS data");
+  for (int s = 0; s < num_N_R; s++)
+    PP_REQUIRE(N_R[s] >= 0 && N_R[s] < wf_R->num_sites, PP_ERR_ARG, "overlap_setup_recip: N_R site out of range");
+  for (int s = 0; s < num_N_S; s++)
+    PP_REQUIRE(N_S[s] >= 0 && N_S[s] < wf_S->num_sites, PP_ERR_ARG, "overlap_setup_recip: N_S site out of range");
+  std::lock_guard<std::mutex> lk(g_mu);
+  pp_ctx *ctx;
+  PP_TRY(global_ctx(&ctx));
+  const int prec = g_precision == PP_FP64 ? PP_FP64 : PP_FP32;
+  // projector.c:727-790: each structure's bands expanded in its own
+  // filtered partial waves at its own unmatched sites (R at N_R, S at N_S)
+  struct Side {
+    pswf_t *wf;
+    int n;
+    const int *sites;
+    const int *labels;
+    const double *coords;
+  } sides[2] = {{wf_R, num_N_R, N_R, labels_R, coords_R}, {wf_S, num_N_S, N_S, labels_S, coords_S}};
+  wf_R->wp_num = 0;
+  wf_S->wp_num = 0;
+  for (auto &sd : sides) {
+    WfDev *d = dev_of(sd.wf);
+    for (int k = 0; k < sd.wf->nwk * sd.wf->nspin; k++)
+      for (int b = 0; b < sd.wf->nband; b++) {
+        free_projection_list(sd.wf->kpts[k]->bands[b]->wave_projections, sd.wf->wp_num);
+        sd.wf->kpts[k]->bands[b]->wave_projections = nullptr;
+        free(sd.wf->kpts[k]->bands[b]->CAs);
+        sd.wf->kpts[k]->bands[b]->CAs = nullptr;
+      }
+    d->wp_n = 0;
+    d->aug_cache.clear();
+    d->ca_n = sd.n;
+    d->ca_hash = hash_ints(1469598103934665603ull, sd.sites, sd.n);
+    if (sd.n == 0)
+      continue;
+    for (int e = 0; e < sd.wf->num_elems; e++)
+      PP_REQUIRE(sd.wf->pps[e].funcs[0].smooth_diffwave, PP_ERR_ARG,
+                 "overlap_setup_recip: projector list without filtered partial waves");
+    std::vector<int> lab(sd.n), src(sd.n);
+    std::vector<double> crd(3 * (size_t)sd.n);
+    for (int s = 0; s < sd.n; s++) {
+      lab[s] = sd.labels[sd.sites[s]];
+      src[s] = d->proj_off[sd.sites[s]];
+      for (int j = 0; j < 3; j++)
+        crd[3 * s + j] = sd.coords[3 * sd.sites[s] + j];
+    }
+    SpeciesArrays a;
+    species_arrays(sd.wf->pps, sd.wf->num_elems, 1, a);
+    pp_sphere *sp = nullptr;
+    PP_TRY(pp_sphere_create(ctx, &sp, sd.wf->lattice, sd.wf->fftg, sd.n, lab.data(), crd.data(), sd.wf->num_elems,
+                            a.nchan.data(), a.ls.data(), a.rmax.data(), a.ngrid.data(), a.vals.data(), prec));
+    d->ca_ld = d->ldc;
+    int st = d->ca.ensure(sizeof(pp_cdouble) * (size_t)d->nkt * d->nband * d->ca_ld);
+    std::vector<double> kf(3 * (size_t)d->nkt);
+    for (int k = 0; k < d->nkt; k++)
+      for (int j = 0; j < 3; j++)
+        kf[3 * k + j] = sd.wf->kpts[k]->k[j];
+    if (st == PP_OK)
+      st = zero_async(ctx, d->ca.p, sizeof(pp_cdouble) * (size_t)d->nkt * d->nband * d->ca_ld);
+    if (st == PP_OK)
+      st = pp_aug_freqs_batched(ctx, sp, d->nkt, kf.data(), d->gofs.data(), (const int *)d->G.p, d->proj.p, d->ptot,
+                                src.data(), d->nband, d->ca.p, d->ca_ld, prec);
+    if (st == PP_OK)
+      st = pp_ctx_synchronize(ctx);
+    pp_sphere_destroy(sp);
+    PP_TRY(st);
+    // host copies in the reference's float complex CAs of every band
+    std::vector<pp_cdouble> host((size_t)d->nkt * d->nband * d->ca_ld);
+    PP_CHECK_HIP(hipMemcpy(host.data(), d->ca.p, sizeof(pp_cdouble) * host.size(), hipMemcpyDeviceToHost));
+    for (int k = 0; k < d->nkt; k++)
+      for (int b = 0; b < d->nband; b++) {
+        const int nw = d->nw[k];
+        pp_cfloat *ca = (pp_cfloat *)malloc(sizeof(pp_cfloat) * std::max(1, nw));
+        const pp_cdouble *row = host.data() + ((size_t)k * d->nband + b) * d->ca_ld;
+        for (int w = 0; w < nw; w++)
+          ca[w] = pp_cfloat((float)row[w].real(), (float)row[w].imag());
+        sd.wf->kpts[k]->bands[b]->CAs = ca;
+      }
+  }
+  wf_R->wp_num = num_N_S;
+  wf_S->wp_num = num_N_R;
+  return setup_offsite(wf_R, wf_S, labels_R, labels_S, coords_R, coords_S, N_RS_R, N_RS_S, num_N_RS);
+}
+
+int compensation_terms_recip(pp_cdouble *overlap, int BAND_NUM, pswf_t *wf_S, pswf_t *wf_R, int num_M, int num_N_R,
+                             int num_N_S, int num_N_RS, int *M_R, int *M_S, int *N_R, int *N_S, int *N_RS_R,
+                             int *N_RS_S, int *proj_labels, double *proj_coords, int *ref_labels, double *ref_coords,
+                             int *fft_grid, int flip_spin) {
+  (void)proj_labels;
+  (void)proj_coords;
+  (void)ref_labels;
+  (void)ref_coords;
+  (void)fft_grid;
+  PP_REQUIRE(overlap && wf_S && wf_R && num_M >= 0 && num_N_R >= 0 && num_N_S >= 0 && num_N_RS >= 0, PP_ERR_ARG,
+             "compensation_terms_recip: bad arguments");
+  PP_REQUIRE(BAND_NUM >= 0 && BAND_NUM < wf_S->nband, PP_ERR_ARG,
+             "compensation_terms_recip: band index out of range");
+  PP_REQUIRE(num_M == 0 || (M_R && M_S), PP_ERR_ARG, "compensation_terms_recip: null site list");
+  PP_REQUIRE(num_N_RS == 0 || (N_RS_R && N_RS_S), PP_ERR_ARG, "compensation_terms_recip: null N_RS lists");
+  std::lock_guard<std::mutex> lk(g_mu);
+  WfDev *dS = (WfDev *)wf_S->dev, *dR = (WfDev *)wf_R->dev;
+  PP_REQUIRE(dS && dR && dS->proj_precision >= 0 && dR->proj_precision >= 0, PP_ERR_ARG,
+             "compensation_terms_recip: call setup_projections on both wavefunctions first");
+  for (int t = 0; t < num_M; t++)
+    PP_REQUIRE(M_R[t] >= 0 && M_R[t] < dR->nsites && M_S[t] >= 0 && M_S[t] < dS->nsites, PP_ERR_ARG,
+               "compensation_terms_recip: site index out of range");
+  PP_REQUIRE(num_N_R == 0 || N_R, PP_ERR_ARG, "compensation_terms_recip: null N_R list");
+  PP_REQUIRE(num_N_S == 0 || N_S, PP_ERR_ARG, "compensation_terms_recip: null N_S list");
+  PP_REQUIRE((num_N_R == 0 || (dR->ca_n == num_N_R &&
+                               dR->ca_hash == hash_ints(1469598103934665603ull, N_R, num_N_R))) &&
+                 (num_N_S == 0 || (dS->ca_n == num_N_S &&
+                                   dS->ca_hash == hash_ints(1469598103934665603ull, N_S, num_N_S))),
+             PP_ERR_ARG, "compensation_terms_recip: call overlap_setup_recip with these N_R/N_S lists first");
+  if (num_N_RS > 0) {
+    bool same = dS->on_n == num_N_RS && dS->on_of == wf_R;
+    for (int t = 0; same && t < num_N_RS; t++)
+      same = dS->on_R[t] == N_RS_R[t] && dS->on_S[t] == N_RS_S[t];
+    PP_REQUIRE(same, PP_ERR_ARG, "compensation_terms_recip: call overlap_setup_recip with these N_RS pairs first");
+  }
+  const int nkt = dR->nkt;
+  pp_ctx *ctx;
+  PP_TRY(global_ctx(&ctx));
+  int flip = (flip_spin && wf_R->nspin == 2) ? 1 : 0;
+  unsigned long long h = 0x9e3779b97f4a7c15ull;  // distinct from the real-space keys
+  h = hash_ints(h, M_R, num_M);
+  h = hash_ints(h, M_S, num_M);
+  h = hash_ints(h, N_R, num_N_R);
+  h = hash_ints(h, N_S, num_N_S);
+  h = hash_ints(h, N_RS_R, num_N_RS);
+  h = hash_ints(h, N_RS_S, num_N_RS);
+  auto key = std::make_tuple((const void *)wf_R, flip, h);
+  auto it = dS->aug_cache.find(key);
+  if (it == dS->aug_cache.end()) {
+    DevBuf out;
+    size_t n = (size_t)nkt * dR->nband * dS->nband;
+    PP_TRY(out.ensure(sizeof(pp_cdouble) * n));
+    PP_TRY(zero_async(ctx, out.p, sizeof(pp_cdouble) * n));
+    std::vector<int> amap(nkt), klen(nkt);
+    for (int k = 0; k < nkt; k++) {
+      amap[k] = flip ? (k < wf_R->nwk ? k + wf_R->nwk : k - wf_R->nwk) : k;
+      klen[k] = dR->nw[amap[k]];  // projector.c:1012, 1020: the R k-point's plane-wave count
+      PP_REQUIRE(klen[k] <= dS->ldc && klen[k] <= dR->ldc, PP_ERR_ARG,
+                 "compensation_terms_recip: plane-wave bases differ");
+    }
+    // projector.c:1008-1022: cdotc(CA_R, C_S) + cdotc(C_R, CA_S) for every pair
+    if (num_N_R > 0)
+      PP_TRY(zgemm_nt(ctx, PP_FP64, dR->nband, dS->nband, 0, 0, dR->ca.p, dR->ca_ld, (long)dR->nband * dR->ca_ld, 1,
+                      dS->C.p, dS->ldc, (long)dS->nband * dS->ldc, out.p, dS->nband, (long)dR->nband * dS->nband,
+                      nkt, amap.data(), nullptr, klen.data(), true));
+    if (num_N_S > 0)
+      PP_TRY(zgemm_nt(ctx, PP_FP64, dR->nband, dS->nband, 0, 1, dR->C.p, dR->ldc, (long)dR->nband * dR->ldc, 0,
+                      dS->ca.p, dS->ca_ld, (long)dS->nband * dS->ca_ld, out.p, dS->nband,
+                      (long)dR->nband * dS->nband, nkt, amap.data(), nullptr, klen.data(), true));
+    // O_M (:1030-1050) and O_N (:1056-1071), as in compensation_terms
+    PP_TRY(pp_augmentation(ctx, wf_R->nwk, wf_R->nspin, flip, dR->proj.p, dR->nband, dR->ptot, dR->proj_off.data(),
+                           dR->labels.data(), dS->proj.p, dS->nband, dS->ptot, dS->proj_off.data(),
+                           dS->labels.data(), num_M, M_R, M_S, (int)dR->sp_nchan.size(), dR->sp_nchan.data(),
+                           dR->sp_ls.data(), dR->sp_dO.data(), out.p));
     if (num_N_RS > 0) {
       std::vector<double> kp(3 * (size_t)nkt);
       for (int k = 0; k < nkt; k++)

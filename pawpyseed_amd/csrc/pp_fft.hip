@@ -759,6 +759,57 @@ int sphere_bounds(pp_ctx *ctx, const int *fftg, int nw, const int *G_dev, int *b
   return PP_OK;
 }
 
+// The three unpruned passes over a full chunk (no scatter, no zero fill).
+template <typename T>
+int fft_passes(pp_ctx *ctx, FftPlan *pl, cplx<T> *grid, int NB) {
+  const long n0 = pl->fftg[0], n1 = pl->fftg[1], n2 = pl->fftg[2];
+  SphereBounds bd = {nullptr, nullptr, nullptr, (int)n0, (int)n1, (int)n2};
+  PP_TRY((run_pass<T, 2>(ctx, grid, pl->ax[2], tw_of<T>(pl, 2), NB, n1 * n2 * NB, n2 * NB, (int)n0, (int)n1, NB,
+                         K_FFT_Z, bd)));
+  PP_TRY((run_pass<T, 1>(ctx, grid, pl->ax[1], tw_of<T>(pl, 1), n2 * NB, n1 * n2 * NB, NB, (int)n0, (int)n2, NB,
+                         K_FFT_Y, bd)));
+  PP_TRY((run_pass<T, 0>(ctx, grid, pl->ax[0], tw_of<T>(pl, 0), n1 * n2 * NB, n2 * NB, NB, (int)n1, (int)n2, NB,
+                         K_FFT_X, bd)));
+  return PP_OK;
+}
+
+// out[b * ldo + w] = scl conj(grid[gidx[w]][b]): the plane-wave coefficients
+// of a forward transform done as conj(backward(conj x)).
+template <typename T>
+__global__ __launch_bounds__(256) void k_gather_conj(const cplx<T> *__restrict__ g, int NB, int nbc,
+                                                     const int *__restrict__ gidx, int nw, double scl,
+                                                     cplx<double> *__restrict__ out, long ldo) {
+  long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (long)nw * nbc)
+    return;
+  const int b = (int)(t % nbc);
+  const int w = (int)(t / nbc);
+  const cplx<double> v = cvt<double>(g[(long)gidx[w] * NB + b]);
+  out[(long)b * ldo + w] = mk<double>(scl * v.x, -scl * v.y);
+}
+
+template <typename T>
+int gather_conj(pp_ctx *ctx, const cplx<T> *grid, int NB, int nbc, const int *gidx, int nw, double scl,
+                cplx<double> *out, long ldo) {
+  const long n = (long)nw * nbc;
+  if (n == 0)
+    return PP_OK;
+  hipEvent_t ev;
+  tbegin(ctx, K_AUG_PREP, &ev);
+  hipLaunchKernelGGL(k_gather_conj<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, grid, NB, nbc,
+                     gidx, nw, scl, out, ldo);
+  tend(ctx, K_AUG_PREP, ev);
+  PP_CHECK_HIP(hipGetLastError());
+  return PP_OK;
+}
+
+template int fft_passes<double>(pp_ctx *, FftPlan *, cplx<double> *, int);
+template int fft_passes<float>(pp_ctx *, FftPlan *, cplx<float> *, int);
+template int gather_conj<double>(pp_ctx *, const cplx<double> *, int, int, const int *, int, double, cplx<double> *,
+                                 long);
+template int gather_conj<float>(pp_ctx *, const cplx<float> *, int, int, const int *, int, double, cplx<double> *,
+                                long);
+
 template int fft_chunk<double>(pp_ctx *, FftPlan *, cplx<double> *, int, int, const int *, const cplx<float> *,
                                long, int, const SphereBounds *);
 template int fft_chunk<float>(pp_ctx *, FftPlan *, cplx<float> *, int, int, const int *, const cplx<float> *, long,

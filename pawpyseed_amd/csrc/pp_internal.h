@@ -115,6 +115,10 @@ int compute_gidx(pp_ctx *ctx, const int *fftg, int nw, const int *G_dev, int *gi
 template <typename T>
 int fft_chunk(pp_ctx *ctx, FftPlan *pl, cplx<T> *grid, int NB, int nw, const int *gidx_dev,
               const cplx<float> *C_dev, long ldc, int nbc, const SphereBounds *bounds);
+template <typename T> int fft_passes(pp_ctx *ctx, FftPlan *pl, cplx<T> *grid, int NB);
+template <typename T>
+int gather_conj(pp_ctx *ctx, const cplx<T> *grid, int NB, int nbc, const int *gidx, int nw, double scl,
+                cplx<double> *out, long ldo);
 // bnd_dev: 2*n0*n1 + 2*n0 + 3 ints; the last is set when the grid aliases a G.
 int sphere_bounds(pp_ctx *ctx, const int *fftg, int nw, const int *G_dev, int *bnd_dev);
 int grid_to_bandmajor(pp_ctx *ctx, int precision, const void *grid, int NB, int nbc, long npts,

@@ -441,6 +441,84 @@ __global__ __launch_bounds__(256) void k_sphere_reduce(const cplx<double> *__res
   }
 }
 
+// Coefficients of the real-harmonic table slots for the expansion
+// sum_j P_j p_j(r) (projector.c:324-327): with p_(l,+-m) built from the real
+// slots t_(l,m) = A cos m phi, t_(l,-m) = A sin m phi (m > 0),
+//   sum_m P_m Y_lm = sum_{m>0} t_(l,m) (P_m + (-1)^m P_-m)
+//                 + i t_(l,-m) (P_m - (-1)^m P_-m) + t_(l,0) P_0.
+// Q[(s * PMAX + j) * NB + b] for band b of the chunk; P rows of stride
+// ptotP, site s of the plan at src_off[s].
+template <typename T>
+__global__ __launch_bounds__(256) void k_expand_coef(int nsites, int PMAX, int NB, int nbc,
+                                                     const int *__restrict__ site_np, const int *__restrict__ proj_off,
+                                                     const int *__restrict__ proj_m, const int *__restrict__ src_off,
+                                                     const cplx<double> *__restrict__ P, long ptotP,
+                                                     cplx<T> *__restrict__ Q) {
+  long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (long)nsites * PMAX * NB)
+    return;
+  const int b = (int)(t % NB);
+  const long r = t / NB;
+  const int j = (int)(r % PMAX), s = (int)(r / PMAX);
+  cplx<double> q = mk<double>(0, 0);
+  if (j < site_np[s] && b < nbc) {
+    const int m = proj_m[proj_off[s] + j];
+    const cplx<double> *row = P + (long)b * ptotP + src_off[s];
+    if (m == 0) {
+      q = row[j];
+    } else {
+      const int am = m < 0 ? -m : m;
+      const int jp = m > 0 ? j : j + 2 * am, jn = jp - 2 * am;  // slots of +am and -am
+      const double sg = (am & 1) ? -1.0 : 1.0;
+      const cplx<double> a = row[jp], c = row[jn];
+      if (m > 0)
+        q = mk<double>(a.x + sg * c.x, a.y + sg * c.y);
+      else  // i (P_m - (-1)^m P_-m)
+        q = mk<double>(-(a.y - sg * c.y), a.x - sg * c.x);
+    }
+  }
+  Q[t] = cvt<T>(q);
+}
+
+// grid[idx][b] += conj(e^{-i k.path} sum_j Q_j t_j) for every sphere point
+// (conjugated so the backward passes give the forward transform).  Lane =
+// band; spheres of different sites may share points, so the adds are
+// atomic (vector global float/double atomics).
+template <typename T, int PMAX>
+__global__ __launch_bounds__(256) void k_sphere_expand(cplx<T> *__restrict__ g, int NB, int nbc,
+                                                       const int *__restrict__ site_pt, const int *__restrict__ site_np,
+                                                       const long *__restrict__ tab_off, const int *__restrict__ pidx,
+                                                       const cplx<T> *__restrict__ phase, const T *__restrict__ table,
+                                                       const cplx<T> *__restrict__ Q, int nparts) {
+  const int s = blockIdx.x / nparts, part = blockIdx.x % nparts;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int p0 = site_pt[s], len = site_pt[s + 1] - p0;
+  const int np = site_np[s];
+  const int chunk = (len + nparts - 1) / nparts;
+  const int a = p0 + part * chunk, e = min(p0 + len, a + chunk);
+  if (lane >= nbc)
+    return;
+  cplx<T> q[PMAX];
+#pragma unroll
+  for (int j = 0; j < PMAX; j++)
+    q[j] = j < np ? Q[((long)s * PMAX + j) * NB + lane] : mk<T>(0, 0);
+  const T *tsite = table + tab_off[s];
+  for (int i = a + wave; i < e; i += 4) {
+    const T *tp = tsite + (long)(i - p0) * np;
+    cplx<T> v = mk<T>(0, 0);
+#pragma unroll
+    for (int j = 0; j < PMAX; j++)
+      if (j < np) {
+        v.x += q[j].x * tp[j];
+        v.y += q[j].y * tp[j];
+      }
+    const cplx<T> x = v * phase[i];
+    T *dst = reinterpret_cast<T *>(g + (long)pidx[i] * NB + lane);
+    atomicAdd(dst, x.x);
+    atomicAdd(dst + 1, -x.y);
+  }
+}
+
 // ------------------------------------------------------------------ host
 static void cross(const double *a, const double *b, double *o) {
   o[0] = a[1] * b[2] - a[2] * b[1];
@@ -668,8 +746,10 @@ int sphere_sums(pp_ctx *ctx, pp_sphere *sp, const cplx<T> *grid, int NB, int nbc
   return PP_OK;
 }
 
+// phase[pt] = dv / sqrt(V) e^{i k.path} (sphere sums) or, with expand,
+// e^{-i k.path} (the augmentation functions of projector.c:276-330)
 template <typename T>
-int make_phase(pp_ctx *ctx, pp_sphere *sp, const double *kfrac, cplx<T> *phase) {
+int make_phase(pp_ctx *ctx, pp_sphere *sp, const double *kfrac, cplx<T> *phase, bool expand) {
   if (sp->npts == 0)
     return PP_OK;
   // reciprocal lattice with 2 pi, rows b_i (utils.h:131)
@@ -684,6 +764,11 @@ int make_phase(pp_ctx *ctx, pp_sphere *sp, const double *kfrac, cplx<T> *phase) 
     kc[j] = kfrac[0] * rec[j] + kfrac[1] * rec[3 + j] + kfrac[2] * rec[6 + j];
   double dv = v / sp->fftg[0] / sp->fftg[1] / sp->fftg[2];
   double scl = dv / sqrt(v);
+  if (expand) {
+    scl = 1.0;
+    for (int j = 0; j < 3; j++)
+      kc[j] = -kc[j];
+  }
   hipEvent_t ev;
   tbegin(ctx, K_PHASE, &ev);
   hipLaunchKernelGGL(k_phase<T>, dim3((unsigned)((sp->npts + 255) / 256)), dim3(256), 0, ctx->stream, sp->npts,
@@ -697,8 +782,56 @@ template int sphere_sums<double>(pp_ctx *, pp_sphere *, const cplx<double> *, in
                                  cplx<double> *, long);
 template int sphere_sums<float>(pp_ctx *, pp_sphere *, const cplx<float> *, int, int, const cplx<float> *,
                                 cplx<double> *, long);
-template int make_phase<double>(pp_ctx *, pp_sphere *, const double *, cplx<double> *);
-template int make_phase<float>(pp_ctx *, pp_sphere *, const double *, cplx<float> *);
+template int make_phase<double>(pp_ctx *, pp_sphere *, const double *, cplx<double> *, bool);
+template int make_phase<float>(pp_ctx *, pp_sphere *, const double *, cplx<float> *, bool);
+
+// Accumulate conj of the augmentation functions of the plan's sites for
+// a band chunk into grid (zeroed by the caller); P rows [nbc][ptotP] with
+// plan site s at src_off_dev[s]; phase from make_phase(expand = true).
+template <typename T>
+int sphere_expand(pp_ctx *ctx, pp_sphere *sp, cplx<T> *grid, int NB, int nbc, const cplx<T> *phase,
+                  const cplx<double> *P, long ptotP, const int *src_off_dev) {
+  if (sp->nsites == 0 || nbc == 0)
+    return PP_OK;
+  const int PM = sp->pmax <= 8 ? 8 : sp->pmax <= 18 ? 18 : sp->pmax <= 32 ? 32 : 64;
+  DevBuf &qb = ctx->slot ? ctx->partial2 : ctx->partial;
+  const long nq = (long)sp->nsites * PM * NB;
+  PP_TRY(qb.ensure(sizeof(cplx<T>) * (size_t)nq));
+  cplx<T> *Q = (cplx<T> *)qb.p;
+  hipEvent_t ev;
+  tbegin(ctx, K_AUG_PREP, &ev);
+  hipLaunchKernelGGL(k_expand_coef<T>, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, ctx->stream, sp->nsites, PM,
+                     NB, nbc, (const int *)sp->d_site_np.p, (const int *)sp->d_proj_off.p, (const int *)sp->d_proj_m.p,
+                     src_off_dev, P, ptotP, Q);
+  tend(ctx, K_AUG_PREP, ev);
+  PP_CHECK_HIP(hipGetLastError());
+  long avg = sp->npts / std::max(1, sp->nsites);
+  int nparts = (int)std::max<long>(1, std::min<long>((2048 + sp->nsites - 1) / sp->nsites, avg / 64));
+  tbegin(ctx, K_SPHERE_SUM, &ev);
+#define PP_EX(PMX)                                                                                                  \
+  hipLaunchKernelGGL((k_sphere_expand<T, PMX>), dim3((unsigned)(sp->nsites * nparts)), dim3(256), 0, ctx->stream,  \
+                     grid, NB, nbc, (const int *)sp->d_site_pt.p, (const int *)sp->d_site_np.p,                   \
+                     (const long *)sp->d_tab_off.p, (const int *)sp->d_pidx.p, phase, (const T *)sp->d_table.p, Q, \
+                     nparts)
+  if (PM == 8)
+    PP_EX(8);
+  else if (PM == 18)
+    PP_EX(18);
+  else if (PM == 32)
+    PP_EX(32);
+  else
+    PP_EX(64);
+#undef PP_EX
+  tend(ctx, K_SPHERE_SUM, ev);
+  PP_CHECK_HIP(hipGetLastError());
+  return PP_OK;
+}
+
+template int sphere_expand<double>(pp_ctx *, pp_sphere *, cplx<double> *, int, int, const cplx<double> *,
+                                   const cplx<double> *, long, const int *);
+template int sphere_expand<float>(pp_ctx *, pp_sphere *, cplx<float> *, int, int, const cplx<float> *,
+                                  const cplx<double> *, long, const int *);
+int sphere_nsites(pp_sphere *sp) { return sp->nsites; }
 
 long sphere_npts(pp_sphere *sp) { return sp->npts; }
 int sphere_ptot(pp_sphere *sp) { return sp->ptot; }

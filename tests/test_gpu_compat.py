@@ -250,3 +250,53 @@ def test_projector_unmatched_sites_or_os(gpu, kpts):
         assert err < 1e-9
 
 
+
+
+def test_projector_aug_recip(gpu):
+    """method='aug_recip' (projector.c:420-453, 727-848, 965-1077): the
+    augmentation functions of each structure's unmatched sites as plane-wave
+    coefficients (GPU expansion + forward FFT), their overlaps with the other
+    structure's coefficients, plus O_M and O_N, against the oracle
+    (or_aug_freqs is pinned against numpy's FFT)."""
+    from pawpyseed_amd.core import Projector, wavefunction_from_workload
+    wl, CR, CS = _pair(nband=5, kpts=((0, 0, 0), (0.25, -0.25, 0.0)))
+    basis = wavefunction_from_workload(wl, "R", coeffs=CR)
+    wf = wavefunction_from_workload(wl, "S", coeffs=CS)
+    M = [0, 1, 2, 3]
+    N_R, N_S = [4, 5, 6, 7], [7, 4, 6, 5]
+    N_RS = [(4, 4), (6, 6)]
+    pr = Projector(wf, basis, method="aug_recip", site_lists=(M, M, N_R, N_S, N_RS))
+    nwk = len(wl.kpts)
+    Gs = wl.ensure_gvecs()
+    PR, TR = oracle_projections(wl, wl.frac_R, CR)
+    PS, TS = oracle_projections(wl, wl.frac_S, CS)
+    smooth = orc.smooth_species(wl.species, _grid_encut(wl))
+    TsR = orc.site_tables(wl.lattice, wl.fftg, wl.frac_R[N_R], smooth)
+    TsS = orc.site_tables(wl.lattice, wl.fftg, wl.frac_S[N_S], smooth)
+    offR = np.array([TR.proj_off[s] for s in N_R] + [0], np.int32)
+    offS = np.array([TS.proj_off[s] for s in N_S] + [0], np.int32)
+    CA_R = [[orc.aug_freqs(wl.lattice, wl.fftg, wl.kpts[k], Gs[k], TsR, PR[k, b], offR) for b in range(5)]
+            for k in range(nwk)]
+    CA_S = [[orc.aug_freqs(wl.lattice, wl.fftg, wl.kpts[k], Gs[k], TsS, PS[k, b], offS) for b in range(5)]
+            for k in range(nwk)]
+    assert np.abs(CA_R[0][0]).max() > 1e-4 * np.abs(CR[0][0]).max()
+    for band in (0, 3):
+        got = pr.single_band_projection(band)
+        want = orc.pseudoprojection(CR, CS, band, nwk, 1)
+        for bR in range(5):
+            for k in range(nwk):
+                # projector.c:1008-1022: cdotc(CA_R, C_S) + cdotc(C_R, CA_S)
+                want[bR * nwk + k] += np.vdot(CA_R[k][bR], CS[k][band].astype(np.complex128))
+                want[bR * nwk + k] += np.vdot(CR[k][bR].astype(np.complex128), CA_S[k][band])
+        want = orc.compensation_terms_m(want, band, nwk, 1, PR, TR, np.zeros(len(wl.frac_R), np.int32), PS[:, band],
+                                        TS, np.array(M, np.int32), np.array(M, np.int32), [wl.delta_o],
+                                        [wl.species.ls])
+        rr, ss = [p[0] for p in N_RS], [p[1] for p in N_RS]
+        ovs, dcs = orc.offsite_blocks(wl.species, wl.species, wl.lattice, wl.frac_R, wl.frac_S, rr, ss,
+                                      _grid_encut(wl))
+        want = orc.compensation_terms_n(want, nwk, 1, PR, TR, PS[:, band], TS, rr, ss, ovs, dcs, wl.kpts)
+        err = normwise(got, want)
+        print("aug_recip band", band, "normwise", err, flush=True)
+        assert err < 1e-9
+    # the reference keeps float complex CAs on every band
+    assert basis.nband == 5
