@@ -197,7 +197,10 @@ int pp_ctx_set_tuning(pp_ctx *ctx, int fft_bands, int fft_tile, int fft_prune,
  * "fft_persistent" (0 off, 1 on with 2048 work-groups, > 1 work-group
  * count: FFT passes as a fixed grid that prefetches the next line),
  * "sphere_parts", "gemm_splitk", "gemm_f64_valu" (1 = FP64 GEMM on the
- * VALU instead of the FP64 matrix cores). */
+ * VALU instead of the FP64 matrix cores), "streams" (1 or 2 band chunks in
+ * flight, default 2), "sphere_unroll" (4 or 8 sphere-sum gathers in flight
+ * per wavefront, default 8), "fft_threads" (256 or 512: work-group size of
+ * the FFT passes when fft_tile is 32). */
 int pp_ctx_set_param(pp_ctx *ctx, const char *name, int value);
 
 /* Sphere plan of one structure (replaces the table build of

@@ -431,6 +431,10 @@ int pp_ctx_set_param(pp_ctx *ctx, const char *name, int value) {
     ctx->tune_fft_persistent = value < 0 ? 0 : value;
   else if (n == "streams")
     ctx->tune_streams = value > 1 ? 2 : 1;
+  else if (n == "sphere_unroll")
+    ctx->tune_sphere_unroll = value >= 8 ? 8 : 4;
+  else if (n == "fft_threads")
+    ctx->tune_fft_threads = value >= 512 ? 512 : 256;
   else if (n == "fft_tile")
     return pp_ctx_set_tuning(ctx, ctx->tune_fft_bands, value, ctx->tune_fft_prune, ctx->tune_sphere_parts,
                              ctx->tune_splitk);

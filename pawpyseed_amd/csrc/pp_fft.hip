@@ -329,14 +329,14 @@ __global__ __launch_bounds__(256) void k_fft_pass(cplx<T> *__restrict__ g, AxisP
 // serves as input and output (half the LDS of the ping-pong form, twice
 // the work-groups per CU).
 //
-// Thread t owns band slot b = t % NBW and butterflies j = t / NBW + q *
-// (256 / NBW); j / p comes from the multiplier pmag = ceil(2^32 / p)
+// Thread t of NT owns band slot b = t % NBW and butterflies j = t / NBW +
+// q * (NT / NBW); j / p comes from the multiplier pmag = ceil(2^32 / p)
 // (exact for j < 2^16), and the twiddles are read from the LDS copy of
 // the axis table.
-template <typename T, int NBW, int R, int MAXB>
+template <typename T, int NBW, int R, int MAXB, int NT = 256>
 __device__ __forceinline__ void stage_ip(cplx<T> *__restrict__ buf, const cplx<T> *__restrict__ tws, unsigned p,
                                          unsigned pmag, unsigned Tn) {
-  constexpr unsigned JS = 256 / NBW;
+  constexpr unsigned JS = NT / NBW;
   const unsigned b = threadIdx.x % NBW, j0 = threadIdx.x / NBW;
   cplx<T> v[MAXB][R];
   unsigned o[MAXB];
@@ -369,16 +369,16 @@ __device__ __forceinline__ void stage_ip(cplx<T> *__restrict__ buf, const cplx<T
 
 // Copy the twiddle/root table of one axis into LDS (visible after the
 // caller's next barrier).
-template <typename T>
+template <typename T, int NT = 256>
 __device__ __forceinline__ void stage_twiddles(cplx<T> *__restrict__ dst, const cplx<T> *__restrict__ tw, int len) {
-  for (int i = threadIdx.x; i < len; i += 256)
+  for (int i = threadIdx.x; i < len; i += NT)
     dst[i] = tw[i];
 }
 
-// As k_fft_pass with in-place stages; radices 2, 3, 4, 5 only, and at most
-// MAXB butterflies per thread per stage.
-template <typename T, int NBW, int AXIS, int MAXB>
-__global__ __launch_bounds__(256) void k_fft_pass_ip(cplx<T> *__restrict__ g, AxisPlan pl,
+// As k_fft_pass with in-place stages; radices 2, 3, 4, 5 only, at most MAXB
+// butterflies per thread per stage, NT threads per work-group.
+template <typename T, int NBW, int AXIS, int MAXB, int NT>
+__global__ __launch_bounds__(NT) void k_fft_pass_ip(cplx<T> *__restrict__ g, AxisPlan pl,
                                                      const cplx<T> *__restrict__ tw, long es, long s1, long s2,
                                                      int n2, int nslices, SphereBounds bd) {
   extern __shared__ unsigned char smem_raw[];
@@ -410,8 +410,8 @@ __global__ __launch_bounds__(256) void k_fft_pass_ip(cplx<T> *__restrict__ g, Ax
   const long base = (long)l1 * s1 + (long)l2 * s2 + (long)slice * NBW;
   const int tot = n * NBW;
   cplx<T> *twl = buf + tot;
-  stage_twiddles<T>(twl, tw, pl.twlen);
-  for (int t = threadIdx.x; t < tot; t += 256) {
+  stage_twiddles<T, NT>(twl, tw, pl.twlen);
+  for (int t = threadIdx.x; t < tot; t += NT) {
     int e = t / NBW, b = t % NBW;
     const int se = sgn_idx(e, n);
     buf[t] = (se >= lo && se <= hi) ? g[base + (long)e * es + b] : mk<T>(0, 0);
@@ -421,15 +421,15 @@ __global__ __launch_bounds__(256) void k_fft_pass_ip(cplx<T> *__restrict__ g, Ax
     const unsigned R = pl.radix[s], p = pl.pprev[s], pm = pl.pmag[s], Tn = n / R;
     const cplx<T> *tws = twl + pl.twoff[s];
     if (R == 4)
-      stage_ip<T, NBW, 4, MAXB>(buf, tws, p, pm, Tn);
+      stage_ip<T, NBW, 4, MAXB, NT>(buf, tws, p, pm, Tn);
     else if (R == 3)
-      stage_ip<T, NBW, 3, MAXB>(buf, tws, p, pm, Tn);
+      stage_ip<T, NBW, 3, MAXB, NT>(buf, tws, p, pm, Tn);
     else if (R == 2)
-      stage_ip<T, NBW, 2, MAXB>(buf, tws, p, pm, Tn);
+      stage_ip<T, NBW, 2, MAXB, NT>(buf, tws, p, pm, Tn);
     else
-      stage_ip<T, NBW, 5, MAXB>(buf, tws, p, pm, Tn);
+      stage_ip<T, NBW, 5, MAXB, NT>(buf, tws, p, pm, Tn);
   }
-  for (int t = threadIdx.x; t < tot; t += 256) {
+  for (int t = threadIdx.x; t < tot; t += NT) {
     int e = t / NBW, b = t % NBW;
     g[base + (long)e * es + b] = buf[t];
   }
@@ -633,10 +633,13 @@ static int launch_pass(pp_ctx *ctx, cplx<T> *grid, const AxisPlan &ap, const cpl
   // in-place form when every radix is 2..5 and each thread holds <= 8 butterflies
   int maxb = 0;
   bool ip = ctx->tune_fft_pingpong == 0 && NBW >= 8;
+  // 512-thread work-groups for 32-band tiles keep the per-thread work of a
+  // 16-band tile with 256 threads
+  const int nt = (NBW >= 32 && ctx->tune_fft_threads == 512) ? 512 : 256;
   for (int s = 0; s < ap.nst; s++) {
     if (ap.radix[s] > 5)
       ip = false;
-    maxb = std::max(maxb, ((ap.n / ap.radix[s]) * NBW + 255) / 256);
+    maxb = std::max(maxb, ((ap.n / ap.radix[s]) * NBW + nt - 1) / nt);
   }
   if (maxb > 8)
     ip = false;
@@ -665,15 +668,29 @@ static int launch_pass(pp_ctx *ctx, cplx<T> *grid, const AxisPlan &ap, const cpl
     }
     if (!launched && ip) {
       size_t lds = ((size_t)ap.n * NBW + ap.twlen) * sizeof(cplx<T>);
-#define PP_IP(MB)                                                                                              \
-  hipLaunchKernelGGL((k_fft_pass_ip<T, NBW, AXIS, MB>), dim3((unsigned)nblk), dim3(256), lds, ctx->stream, grid, ap, \
-                     tw, es, s1, s2, n2, nslices, bd)
-      if (maxb <= 2)
-        PP_IP(2);
-      else if (maxb <= 4)
-        PP_IP(4);
-      else
-        PP_IP(8);
+#define PP_IP(MB, NT)                                                                                          \
+  hipLaunchKernelGGL((k_fft_pass_ip<T, NBW, AXIS, MB, NT>), dim3((unsigned)nblk), dim3(NT), lds, ctx->stream, grid,  \
+                     ap, tw, es, s1, s2, n2, nslices, bd)
+      bool done = false;
+      if constexpr (NBW >= 32) {
+        if (nt == 512) {
+          if (maxb <= 2)
+            PP_IP(2, 512);
+          else if (maxb <= 4)
+            PP_IP(4, 512);
+          else
+            PP_IP(8, 512);
+          done = true;
+        }
+      }
+      if (!done) {
+        if (maxb <= 2)
+          PP_IP(2, 256);
+        else if (maxb <= 4)
+          PP_IP(4, 256);
+        else
+          PP_IP(8, 256);
+      }
 #undef PP_IP
       launched = true;
     }

@@ -85,6 +85,8 @@ struct pp_ctx {
   int tune_fft_persistent = 0;  // 0 off, 1 on (2048 work-groups), > 1 work-group count
   int tune_gemm_f64_valu = 0;   // 1: FP64 GEMM on the VALU instead of the matrix cores
   int tune_streams = 2;         // band chunks in flight (1 or 2 streams)
+  int tune_sphere_unroll = 8;   // sphere-sum gathers in flight per wavefront (4 or 8)
+  int tune_fft_threads = 256;   // work-group size of the in-place FFT passes with 32-band tiles (256 or 512)
   hipStream_t aux = nullptr;    // second stream of the chunk pipeline (default priority)
   int slot = 0;                 // chunk buffer set in use (0 or 1)
   pp::DevBuf grid, grid2;       // band-fastest FFT chunk, one per slot

@@ -307,7 +307,7 @@ __global__ __launch_bounds__(256) void k_phase(long npts, const double *__restri
 // fourth point of the tile, four points per step with their grid loads in
 // flight together, and reads the projector values as LDS broadcasts.  The
 // four wavefronts' sums are added in LDS at the end.
-template <typename T, int PMAX, bool FULL>
+template <typename T, int PMAX, bool FULL, int U>
 __global__ __launch_bounds__(256) void k_sphere_sum(const cplx<T> *__restrict__ g, int NB,
                                                     const int *__restrict__ site_pt, const int *__restrict__ site_np,
                                                     const long *__restrict__ tab_off, const int *__restrict__ pidx,
@@ -342,13 +342,13 @@ __global__ __launch_bounds__(256) void k_sphere_sum(const cplx<T> *__restrict__ 
     }
     __syncthreads();
     int i = wave;
-    for (; i + 12 < n; i += 16) {
-      cplx<T> x[4];
+    for (; i + 4 * (U - 1) < n; i += 4 * U) {
+      cplx<T> x[U];
 #pragma unroll
-      for (int u = 0; u < 4; u++)
+      for (int u = 0; u < U; u++)
         x[u] = active ? g[(long)s_idx[i + 4 * u] * NB + lane] : mk<T>(0, 0);
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
+      for (int u = 0; u < U; u++) {
         const cplx<T> xv = x[u] * s_ph[i + 4 * u];
         const T *tp = s_tab + (i + 4 * u) * np;
 #pragma unroll
@@ -689,14 +689,23 @@ static int sphere_build(pp_sphere *sp) {
 template <typename T, int PMAX>
 static void launch_sum(pp_ctx *ctx, pp_sphere *sp, const cplx<T> *grid, int NB, const cplx<T> *phase, int nparts,
                        cplx<double> *partial) {
-  if (sp->pmin == PMAX)
-    hipLaunchKernelGGL((k_sphere_sum<T, PMAX, true>), dim3(sp->nsites * nparts), dim3(256), 0, ctx->stream, grid,
-                       NB, (const int *)sp->d_site_pt.p, (const int *)sp->d_site_np.p, (const long *)sp->d_tab_off.p,
-                       (const int *)sp->d_pidx.p, phase, (const T *)sp->d_table.p, nparts, partial);
-  else
-    hipLaunchKernelGGL((k_sphere_sum<T, PMAX, false>), dim3(sp->nsites * nparts), dim3(256), 0, ctx->stream, grid,
-                       NB, (const int *)sp->d_site_pt.p, (const int *)sp->d_site_np.p, (const long *)sp->d_tab_off.p,
-                       (const int *)sp->d_pidx.p, phase, (const T *)sp->d_table.p, nparts, partial);
+#define PP_SS(FULL, U)                                                                                            \
+  hipLaunchKernelGGL((k_sphere_sum<T, PMAX, FULL, U>), dim3(sp->nsites * nparts), dim3(256), 0, ctx->stream, grid,  \
+                     NB, (const int *)sp->d_site_pt.p, (const int *)sp->d_site_np.p, (const long *)sp->d_tab_off.p, \
+                     (const int *)sp->d_pidx.p, phase, (const T *)sp->d_table.p, nparts, partial)
+  const bool u8 = ctx->tune_sphere_unroll == 8;
+  if (sp->pmin == PMAX) {
+    if (u8)
+      PP_SS(true, 8);
+    else
+      PP_SS(true, 4);
+  } else {
+    if (u8)
+      PP_SS(false, 8);
+    else
+      PP_SS(false, 4);
+  }
+#undef PP_SS
 }
 
 template <typename T>
