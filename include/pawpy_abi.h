@@ -200,7 +200,9 @@ int pp_ctx_set_tuning(pp_ctx *ctx, int fft_bands, int fft_tile, int fft_prune,
  * VALU instead of the FP64 matrix cores), "streams" (1 or 2 band chunks in
  * flight, default 2), "sphere_unroll" (4 or 8 sphere-sum gathers in flight
  * per wavefront, default 8), "fft_threads" (256 or 512: work-group size of
- * the FFT passes when fft_tile is 32). */
+ * the FFT passes when fft_tile is 32), "fft_radix" (0: radices 2-5,
+ * default; 1: composite radices 6/8/9/12/16 where they fit, fewer LDS
+ * round trips but more registers). */
 int pp_ctx_set_param(pp_ctx *ctx, const char *name, int value);
 
 /* Sphere plan of one structure (replaces the table build of

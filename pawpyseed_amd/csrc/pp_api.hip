@@ -435,6 +435,8 @@ int pp_ctx_set_param(pp_ctx *ctx, const char *name, int value) {
     ctx->tune_sphere_unroll = value >= 8 ? 8 : 4;
   else if (n == "fft_threads")
     ctx->tune_fft_threads = value >= 512 ? 512 : 256;
+  else if (n == "fft_radix")
+    ctx->tune_fft_radix = value != 0 ? 1 : 0;
   else if (n == "fft_tile")
     return pp_ctx_set_tuning(ctx, ctx->tune_fft_bands, value, ctx->tune_fft_prune, ctx->tune_sphere_parts,
                              ctx->tune_splitk);

@@ -14,6 +14,7 @@
 // and writes the line back in place.
 #include <cmath>
 
+#include "pp_dft.h"
 #include "pp_internal.h"
 
 namespace pp {
@@ -35,6 +36,44 @@ static int factor_axis(int n, AxisPlan &a) {
       m /= q;
     }
   return PP_OK;
+}
+
+// Composite factorization for the in-place passes: the largest radix with
+// a register-level DFT first (16, 12, 9, 8, 6, then 5, 4, 3, 2); false when
+// a prime factor above 5 remains.
+static bool factor_composite(int n, int *radix, int &nst) {
+  static const int pref[] = {16, 12, 9, 8, 6, 5, 4, 3, 2};
+  nst = 0;
+  int m = n;
+  while (m > 1) {
+    int r = 0;
+    for (int c : pref)
+      if (m % c == 0) {
+        r = c;
+        break;
+      }
+    if (r == 0 || nst >= 12)
+      return false;
+    radix[nst++] = r;
+    m /= r;
+  }
+  return true;
+}
+
+// twiddles exp(+2 pi i k r / (p R)), k < p, r = 1..R-1, then the roots
+// exp(+2 pi i m / R) of the generic small DFT, for one stage
+static void push_stage_twiddles(std::vector<double> &tw, int p, int R) {
+  for (int k = 0; k < p; k++)
+    for (int r = 1; r < R; r++) {
+      long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)(k * r) / (long double)(p * R);
+      tw.push_back((double)cosl(ang));
+      tw.push_back((double)sinl(ang));
+    }
+  for (int m = 0; m < R; m++) {
+    long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)R;
+    tw.push_back((double)cosl(ang));
+    tw.push_back((double)sinl(ang));
+  }
 }
 
 int get_fft_plan(pp_ctx *ctx, const int *fftg, FftPlan **out) {
@@ -61,21 +100,23 @@ int get_fft_plan(pp_ctx *ctx, const int *fftg, FftPlan **out) {
       ap.pprev[s] = p;
       ap.pmag[s] = p > 1 ? (unsigned)(((1ull << 32) + (unsigned long long)p - 1) / (unsigned long long)p) : 0u;
       ap.twoff[s] = (int)(tw.size() / 2) - pl->twbase[a];
-      // twiddles exp(+2 pi i k r / (p R)), k < p, r = 1..R-1
-      for (int k = 0; k < p; k++)
-        for (int r = 1; r < R; r++) {
-          long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)(k * r) /
-                            (long double)(p * R);
-          tw.push_back((double)cosl(ang));
-          tw.push_back((double)sinl(ang));
-        }
-      // roots exp(+2 pi i m / R) for the generic small DFT
-      for (int m = 0; m < R; m++) {
-        long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)R;
-        tw.push_back((double)cosl(ang));
-        tw.push_back((double)sinl(ang));
-      }
+      push_stage_twiddles(tw, p, R);
       p *= R;
+    }
+    // the composite factorization, when it has fewer stages
+    int rc[12], nc = 0;
+    ap.nstc = 0;
+    if (factor_composite(fftg[a], rc, nc) && nc < ap.nst) {
+      ap.nstc = nc;
+      p = 1;
+      for (int s = 0; s < nc; s++) {
+        ap.radixc[s] = rc[s];
+        ap.pprevc[s] = p;
+        ap.pmagc[s] = p > 1 ? (unsigned)(((1ull << 32) + (unsigned long long)p - 1) / (unsigned long long)p) : 0u;
+        ap.twoffc[s] = (int)(tw.size() / 2) - pl->twbase[a];
+        push_stage_twiddles(tw, p, rc[s]);
+        p *= rc[s];
+      }
     }
     ap.twlen = (int)(tw.size() / 2) - pl->twbase[a];
   }
@@ -142,52 +183,6 @@ __global__ __launch_bounds__(256) void k_scatter(cplx<T> *__restrict__ g, int NB
   }
 }
 
-template <typename T> struct Consts;
-template <> struct Consts<double> {
-  static constexpr double s3 = 0.86602540378443864676;
-  static constexpr double c51 = 0.30901699437494742410, c52 = -0.80901699437494742410;
-  static constexpr double s51 = 0.95105651629515357212, s52 = 0.58778525229247312917;
-};
-template <> struct Consts<float> {
-  static constexpr float s3 = 0.86602540378443864676f;
-  static constexpr float c51 = 0.30901699437494742410f, c52 = -0.80901699437494742410f;
-  static constexpr float s51 = 0.95105651629515357212f, s52 = 0.58778525229247312917f;
-};
-
-// Backward (e^{+2 pi i rq/R}) DFT of R in {2,3,4,5} values in registers.
-template <typename T>
-__device__ __forceinline__ void small_dft(int R, cplx<T> *v) {
-  if (R == 4) {
-    cplx<T> a0 = v[0] + v[2], a1 = v[0] - v[2], b0 = v[1] + v[3], b1 = mul_i(v[1] - v[3]);
-    v[0] = a0 + b0;
-    v[2] = a0 - b0;
-    v[1] = a1 + b1;
-    v[3] = a1 - b1;
-  } else if (R == 2) {
-    cplx<T> a = v[0], b = v[1];
-    v[0] = a + b;
-    v[1] = a - b;
-  } else if (R == 3) {
-    cplx<T> s = v[1] + v[2], d = v[1] - v[2];
-    cplx<T> m = v[0] - scale(s, (T)0.5);
-    cplx<T> t = mul_i(scale(d, Consts<T>::s3));
-    v[0] = v[0] + s;
-    v[1] = m + t;
-    v[2] = m - t;
-  } else if (R == 5) {
-    cplx<T> a1 = v[1] + v[4], b1 = v[1] - v[4], a2 = v[2] + v[3], b2 = v[2] - v[3];
-    cplx<T> m1 = v[0] + scale(a1, Consts<T>::c51) + scale(a2, Consts<T>::c52);
-    cplx<T> m2 = v[0] + scale(a1, Consts<T>::c52) + scale(a2, Consts<T>::c51);
-    cplx<T> t1 = mul_i(scale(b1, Consts<T>::s51) + scale(b2, Consts<T>::s52));
-    cplx<T> t2 = mul_i(scale(b1, Consts<T>::s52) - scale(b2, Consts<T>::s51));
-    v[0] = v[0] + a1 + a2;
-    v[1] = m1 + t1;
-    v[4] = m1 - t1;
-    v[2] = m2 + t2;
-    v[3] = m2 - t2;
-  }
-}
-
 // One Stockham stage of radix R over a line tile in LDS:
 // v[r] = in[j + r*Tn] * w^(k r) (k = j mod p), small DFT, then
 // out[(j/p)*p*R + k + r*p] = V[r].
@@ -202,7 +197,7 @@ __device__ __forceinline__ void stage(const cplx<T> *__restrict__ in, cplx<T> *_
 #pragma unroll
     for (int r = 1; r < R; r++)
       v[r] = in[(j + r * Tn) * NBW + b] * tws[k * (R - 1) + (r - 1)];
-    small_dft<T>(R, v);
+    Dft<T, R>::run(v);
     const int o = (j / p) * p * R + k;
 #pragma unroll
     for (int r = 0; r < R; r++)
@@ -351,7 +346,7 @@ __device__ __forceinline__ void stage_ip(cplx<T> *__restrict__ buf, const cplx<T
 #pragma unroll
       for (int r = 1; r < R; r++)
         v[q][r] = buf[(j + r * Tn) * NBW + b] * tws[k * (R - 1) + (r - 1)];
-      small_dft<T>(R, v[q]);
+      Dft<T, R>::run(v[q]);
     }
   }
   __syncthreads();
@@ -375,9 +370,57 @@ __device__ __forceinline__ void stage_twiddles(cplx<T> *__restrict__ dst, const 
     dst[i] = tw[i];
 }
 
-// As k_fft_pass with in-place stages; radices 2, 3, 4, 5 only, at most MAXB
-// butterflies per thread per stage, NT threads per work-group.
-template <typename T, int NBW, int AXIS, int MAXB, int NT>
+// All stages of a line in LDS.  Composite radices are compiled only into
+// the COMP instantiation with MAXB <= 2 (the host selects the composite
+// factorization only when every stage needs at most two butterflies per
+// thread); keeping them out of the other instantiations keeps their
+// register allocation that of radices 2-5.
+template <typename T, int NBW, int MAXB, int NT, bool COMP = false>
+__device__ __forceinline__ void stages_ip(cplx<T> *__restrict__ buf, const cplx<T> *__restrict__ twl,
+                                          const AxisPlan &pl, int n) {
+  for (int s = 0; s < pl.nst; s++) {
+    const unsigned R = pl.radix[s], p = pl.pprev[s], pm = pl.pmag[s], Tn = n / R;
+    const cplx<T> *tws = twl + pl.twoff[s];
+    switch (R) {
+    case 2:
+      stage_ip<T, NBW, 2, MAXB, NT>(buf, tws, p, pm, Tn);
+      break;
+    case 3:
+      stage_ip<T, NBW, 3, MAXB, NT>(buf, tws, p, pm, Tn);
+      break;
+    case 4:
+      stage_ip<T, NBW, 4, MAXB, NT>(buf, tws, p, pm, Tn);
+      break;
+    case 5:
+      stage_ip<T, NBW, 5, MAXB, NT>(buf, tws, p, pm, Tn);
+      break;
+    default:
+      if constexpr (COMP && MAXB <= 2) {
+        switch (R) {
+        case 6:
+          stage_ip<T, NBW, 6, MAXB, NT>(buf, tws, p, pm, Tn);
+          break;
+        case 8:
+          stage_ip<T, NBW, 8, MAXB, NT>(buf, tws, p, pm, Tn);
+          break;
+        case 9:
+          stage_ip<T, NBW, 9, MAXB, NT>(buf, tws, p, pm, Tn);
+          break;
+        case 12:
+          stage_ip<T, NBW, 12, MAXB, NT>(buf, tws, p, pm, Tn);
+          break;
+        case 16:
+          stage_ip<T, NBW, 16, MAXB, NT>(buf, tws, p, pm, Tn);
+          break;
+        }
+      }
+    }
+  }
+}
+
+// As k_fft_pass with in-place stages (radices with a register-level DFT),
+// at most MAXB butterflies per thread per stage, NT threads per work-group.
+template <typename T, int NBW, int AXIS, int MAXB, int NT, bool COMP = false>
 __global__ __launch_bounds__(NT) void k_fft_pass_ip(cplx<T> *__restrict__ g, AxisPlan pl,
                                                      const cplx<T> *__restrict__ tw, long es, long s1, long s2,
                                                      int n2, int nslices, SphereBounds bd) {
@@ -417,18 +460,7 @@ __global__ __launch_bounds__(NT) void k_fft_pass_ip(cplx<T> *__restrict__ g, Axi
     buf[t] = (se >= lo && se <= hi) ? g[base + (long)e * es + b] : mk<T>(0, 0);
   }
   __syncthreads();
-  for (int s = 0; s < pl.nst; s++) {
-    const unsigned R = pl.radix[s], p = pl.pprev[s], pm = pl.pmag[s], Tn = n / R;
-    const cplx<T> *tws = twl + pl.twoff[s];
-    if (R == 4)
-      stage_ip<T, NBW, 4, MAXB, NT>(buf, tws, p, pm, Tn);
-    else if (R == 3)
-      stage_ip<T, NBW, 3, MAXB, NT>(buf, tws, p, pm, Tn);
-    else if (R == 2)
-      stage_ip<T, NBW, 2, MAXB, NT>(buf, tws, p, pm, Tn);
-    else
-      stage_ip<T, NBW, 5, MAXB, NT>(buf, tws, p, pm, Tn);
-  }
+  stages_ip<T, NBW, MAXB, NT, COMP>(buf, twl, pl, n);
   for (int t = threadIdx.x; t < tot; t += NT) {
     int e = t / NBW, b = t % NBW;
     g[base + (long)e * es + b] = buf[t];
@@ -521,18 +553,7 @@ __global__ __launch_bounds__(256) void k_fft_pass_pl(cplx<T> *__restrict__ g, Ax
     }
     if (next < nlines)
       load(next, nlo, nhi);
-    for (int s = 0; s < pl.nst; s++) {
-      const unsigned R = pl.radix[s], p = pl.pprev[s], pm = pl.pmag[s], Tn = n / R;
-      const cplx<T> *tws = twl + pl.twoff[s];
-      if (R == 4)
-        stage_ip<T, NBW, 4, MAXB>(buf, tws, p, pm, Tn);
-      else if (R == 3)
-        stage_ip<T, NBW, 3, MAXB>(buf, tws, p, pm, Tn);
-      else if (R == 2)
-        stage_ip<T, NBW, 2, MAXB>(buf, tws, p, pm, Tn);
-      else
-        stage_ip<T, NBW, 5, MAXB>(buf, tws, p, pm, Tn);
-    }
+    stages_ip<T, NBW, MAXB, 256>(buf, twl, pl, n);
     {
       const long ln = pass_line<AXIS>(bd, (int)(line / nslices));
       const int slice = (int)(line % nslices);
@@ -643,12 +664,34 @@ static int launch_pass(pp_ctx *ctx, cplx<T> *grid, const AxisPlan &ap, const cpl
   }
   if (maxb > 8)
     ip = false;
+  // the composite factorization: fewer LDS round trips, when every stage
+  // fits two butterflies per thread of 256
+  bool comp = ip && ap.nstc > 0 && nt == 256 && ctx->tune_fft_persistent == 0 && ctx->tune_fft_radix == 1;
+  for (int s = 0; comp && s < ap.nstc; s++)
+    if (!dft_supported(ap.radixc[s]) || ((ap.n / ap.radixc[s]) * NBW + 255) / 256 > 2)
+      comp = false;
   hipEvent_t ev;
   tbegin(ctx, cls, &ev);
   bool launched = false;
   const int ept = (ap.n * NBW + 255) / 256;
   if constexpr (NBW >= 8) {
-    if (ip && ctx->tune_fft_persistent != 0 && ept <= 16) {
+    if (comp) {
+      AxisPlan apc = ap;
+      apc.nst = ap.nstc;
+      for (int s = 0; s < ap.nstc; s++) {
+        apc.radix[s] = ap.radixc[s];
+        apc.pprev[s] = ap.pprevc[s];
+        apc.pmag[s] = ap.pmagc[s];
+        apc.twoff[s] = ap.twoffc[s];
+      }
+      size_t lds = ((size_t)ap.n * NBW + ap.twlen) * sizeof(cplx<T>);
+      hipLaunchKernelGGL((k_fft_pass_ip<T, NBW, AXIS, 2, 256, true>), dim3((unsigned)nblk), dim3(256), lds,
+                         ctx->stream, grid, apc, tw, es, s1, s2, n2, nslices, bd);
+      launched = true;
+    }
+  }
+  if constexpr (NBW >= 8) {
+    if (!launched && ip && ctx->tune_fft_persistent != 0 && ept <= 16) {
       size_t lds = ((size_t)ap.n * NBW + ap.twlen) * sizeof(cplx<T>);
       long nlines = nblk;
       long grid_wg = std::min<long>(nlines, (long)ctx->tune_fft_persistent > 1 ? ctx->tune_fft_persistent : 2048);

@@ -45,7 +45,12 @@ struct AxisPlan {
   int pprev[12];
   int twoff[12];
   unsigned pmag[12];  // ceil(2^32 / pprev): j / pprev == umulhi(j, pmag) for j < 2^16 (pprev > 1)
-  int twlen = 0;      // twiddle + root entries of this axis
+  int twlen = 0;      // twiddle + root entries of this axis (both factorizations)
+  // second factorization with composite radices (6, 8, 9, 12, 16): fewer
+  // stages for the in-place passes; nstc = 0 when there is none
+  int nstc = 0;
+  int radixc[12], pprevc[12], twoffc[12];
+  unsigned pmagc[12];
 };
 
 // Plane-wave sphere bounds of one k-point in signed grid indices (device
@@ -87,6 +92,7 @@ struct pp_ctx {
   int tune_streams = 2;         // band chunks in flight (1 or 2 streams)
   int tune_sphere_unroll = 8;   // sphere-sum gathers in flight per wavefront (4 or 8)
   int tune_fft_threads = 256;   // work-group size of the in-place FFT passes with 32-band tiles (256 or 512)
+  int tune_fft_radix = 0;       // 0: radices 2-5, 1: composite radices where they fit
   hipStream_t aux = nullptr;    // second stream of the chunk pipeline (default priority)
   int slot = 0;                 // chunk buffer set in use (0 or 1)
   pp::DevBuf grid, grid2;       // band-fastest FFT chunk, one per slot

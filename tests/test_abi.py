@@ -146,6 +146,21 @@ def test_get_projector_list_overlaps_match_oracle():
     b, c, d = orc.spline_coeff(sp.projgrid, sp.realprojs[2])
     f = pp.funcs[2]
     np.testing.assert_allclose([f.proj_spline[0][i] for i in range(10)], b[:10], rtol=1e-10, atol=1e-13)
+    # projector.c:114-163: transforms and filtered partial-wave differences
+    # (grid_encut 100 eV) equal the oracle's
+    nw, npj = len(sp.grid), len(sp.projgrid)
+    kgrid = np.ctypeslib.as_array(pp.kwave_grid, shape=(nw,))
+    sg = np.ctypeslib.as_array(pp.smooth_grid, shape=(npj,))
+    np.testing.assert_allclose(sg, sp.grid[-1] / npj * np.arange(npj), rtol=0, atol=1e-15)
+    assert abs(pp.wave_rmax - sp.grid[-1]) < 1e-15
+    for ch, l in enumerate(sp.ls):
+        so, ko, wo = orc.smooth_diffwave(sp.grid, sp.aewaves[ch] - sp.pswaves[ch], l, max(sp.ls), 100.0, npj)
+        fc = pp.funcs[ch]
+        np.testing.assert_allclose(kgrid, ko, rtol=1e-14)
+        kw = np.ctypeslib.as_array(fc.kwave, shape=(nw,))
+        sd = np.ctypeslib.as_array(fc.smooth_diffwave, shape=(npj,))
+        assert np.abs(kw - wo).max() < 1e-11 * np.abs(wo).max()
+        assert np.abs(sd - so).max() < 1e-10 * np.abs(so).max()
     L.free_ppot_list(ct.c_void_p(pl), 1)
 
 

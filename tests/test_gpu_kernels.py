@@ -44,11 +44,31 @@ def _fft_case(ctx, wl, nb, precision):
 
 
 @pytest.mark.parametrize("fftg,encut", [((20, 20, 20), 150.0), ((24, 18, 30), 120.0), ((28, 14, 21), 60.0),
-                                        ((60, 60, 60), 400.0), ((26, 22, 33), 90.0)])
+                                        ((60, 60, 60), 400.0), ((26, 22, 33), 90.0), ((48, 40, 64), 300.0),
+                                        ((36, 54, 32), 250.0)])
 def test_fft3d_fp64(ctx, fftg, encut):
+    """Grids whose axes factor into radices 2-5 only, composite radices
+    (6, 8, 9, 12, 16: e.g. 108 = 12 x 9, 64 = 16 x 4, 40 = 8 x 5) and other
+    primes (7, 11, 13, generic stages)."""
     wl = synthetic.tiny_config(encut=encut, fftg=fftg)
     print("fft3d fp64", fftg, flush=True)
     assert _fft_case(ctx, wl, 5, "fp64") < 1e-12
+
+
+@pytest.mark.parametrize("fftg,encut", [((24, 18, 30), 120.0), ((60, 60, 60), 400.0), ((48, 40, 64), 300.0),
+                                        ((36, 54, 32), 250.0)])
+def test_fft3d_composite_radices(ctx, fftg, encut):
+    """The opt-in composite factorization (fft_radix = 1: 12 x 2, 9 x 2,
+    6 x 5, 12 x 5, 16 x 3, 8 x 5, 16 x 4, 12 x 3, 9 x 6, 16 x 2 here)."""
+    wl = synthetic.tiny_config(encut=encut, fftg=fftg)
+    ctx.set_param("fft_radix", 1)
+    try:
+        err64 = _fft_case(ctx, wl, 5, "fp64")
+        err32 = _fft_case(ctx, wl, 3, "fp32")
+    finally:
+        ctx.set_param("fft_radix", 0)
+    print("fft3d composite", fftg, err64, err32, flush=True)
+    assert err64 < 1e-12 and err32 < 1e-5
 
 
 def test_fft3d_fp32(ctx):
