@@ -103,6 +103,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle checks")
     ap.add_argument("--paths", default="fp32,fp64", help="comma list of fp32, fp64 setup/overlap paths")
+    ap.add_argument("--no-fused", action="store_true",
+                    help="run the x pass and the sphere sums as separate kernels (comparison)")
     ap.add_argument("--profile-step", action="store_true",
                     help="for rocprofv3 --pmc: a 1 GiB device copy (counter calibration) and one fp32 setup step")
     args = ap.parse_args()
@@ -133,6 +135,8 @@ def main():
 
     stream = torch.cuda.Stream()
     ctx = B.Context(0, stream=stream)
+    if args.no_fused:
+        ctx.set_param("fused", 0)
     labels_R = np.zeros(len(wl.frac_R), np.int32)
     labels_S = np.zeros(len(wl.frac_S), np.int32)
     paths = [p for p in args.paths.split(",") if p]
@@ -242,7 +246,10 @@ def main():
     #   fft_z: read the plane-wave coefficients in place (W_k points), write
     #     every populated (x, y) z-line in full;
     #   sphere_sum: gather NB values per sphere point, one real table row,
-    #     one phase and one grid index per point;  gemm: 8 M N K flop.
+    #     one phase and one grid index per point;  gemm: 8 M N K flop;
+    #   fft_x_sphere (the x pass fused with the sphere sums): read the
+    #     populated x-slab, one real table row and one phase per sphere
+    #     point, write one FP32 partial row (P x NB) per (work item, site).
     prec_bytes = 8 if head != "fp64" else 16
     n0, n1, n2 = (int(v) for v in wl.fftg)
     NB = 64
@@ -263,6 +270,11 @@ def main():
     sphere_bytes = planS.npts * (NB * prec_bytes + (ptotS // len(wl.frac_S)) * prec_bytes // 2 + prec_bytes + 4)
     alg = {"fft_x": float(np.mean(bx)), "fft_y": float(np.mean(by)), "fft_z": float(np.mean(bz)),
            "sphere_sum": float(sphere_bytes)}
+    usable, _, nrows, _, _ = planS.fused_info()
+    if usable:
+        P = ptotS // len(wl.frac_S)
+        xs_read = float(np.mean([n1 * n2 * NB * prec_bytes * (b / (n1 * n2 * NB * prec_bytes) - n0) for b in bx]))
+        alg["fft_x_sphere"] = xs_read + planS.npts * (P * prec_bytes // 2 + prec_bytes) + nrows * P * NB * 8
     gemm_flops = 8.0 * nb * nb * float(nw.mean())
     traffic = read_traffic(head)
     rl = {}
@@ -289,10 +301,11 @@ def main():
     roof["avg_launch_ms"] = rl[dom]["avg_launch_ms"]
     roof["timing"] = "breakdown step after the timed steps: one band chunk in flight, HIP events per launch"
     # whole setup: algorithmic bytes of all its kernels over the pipelined time
-    setup_bytes = (alg["fft_x"] + alg["fft_y"] + alg["fft_z"] + alg["sphere_sum"]) * nk * -(-nb // NB)
+    setup_bytes = sum(b for c, b in alg.items() if c in rl) * nk * -(-nb // NB)
     pipe = {"achieved": setup_bytes / R["setup_s"] / 1e9, "unit": "GB/s", "peak": PEAK_HBM / 1e9,
             "frac": setup_bytes / R["setup_s"] / PEAK_HBM,
-            "what": "algorithmic bytes of the FFT passes and sphere sums of one step / timed setup time"}
+            "what": "algorithmic bytes of the FFT passes and sphere sums (or the fused x pass) of one step / "
+                    "timed setup time"}
 
     out = {
         "metric": "band_projections_per_sec",

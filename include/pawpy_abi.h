@@ -202,7 +202,10 @@ int pp_ctx_set_tuning(pp_ctx *ctx, int fft_bands, int fft_tile, int fft_prune,
  * per wavefront, default 8), "fft_threads" (256 or 512: work-group size of
  * the FFT passes when fft_tile is 32), "fft_radix" (0: radices 2-5,
  * default; 1: composite radices 6/8/9/12/16 where they fit, fewer LDS
- * round trips but more registers). */
+ * round trips but more registers), "fused" (0, default: separate x-pass and
+ * sphere-sum kernels; 1: the FP32 path runs them as one kernel where the
+ * plan allows -- it writes no real-space grid but measured slower on the
+ * 216-atom workload). */
 int pp_ctx_set_param(pp_ctx *ctx, const char *name, int value);
 
 /* Sphere plan of one structure (replaces the table build of
@@ -224,6 +227,27 @@ int pp_sphere_info(pp_sphere *sp, int *total_projs, long *total_points,
 /* Rebuild the tables of an existing plan for new coordinates (same sites
  * and species): the per-structure setup step of setup_projections. */
 int pp_sphere_rebuild(pp_sphere *sp, const double *frac_coords);
+/* Diagnostic, no counterpart in the reference: whether the plan carries
+ * the work lists of the fused x pass with sphere sums (FP32 plans with at
+ * most 18 projectors per site whose boxes do not wrap onto themselves in
+ * y or z), their work items, the partial rows that hold a site (one per
+ * item and site), the plan's segments (runs of a site's points
+ * on one x-line), and the work-list consistency flag (0 = every segment
+ * has its slot; reading it synchronises the context's stream).  Any
+ * pointer may be NULL. */
+int pp_sphere_fused_info(pp_sphere *sp, int *usable, int *nitems, int *nrows,
+                         long *nseg, int *flag);
+/* Diagnostic, no counterpart in the reference, host only (no device
+ * needed): the fused x-pass work lists of sites frac[ns][3] with sphere
+ * grids rmax[s] / ngrid[s].  counts[0..2] = work items, line blocks, site
+ * rows; then, where the pointers are not NULL, items[4 * items] (y0, z0,
+ * sites, 0), isites[12 * items] (site per slot or -1), bitem[blocks + 1],
+ * rows_off[ns + 1], rows[counts[2]] (item * 12 + slot).  Returns
+ * PP_ERR_UNSUPPORTED when a site box wraps onto itself in y or z. */
+int pp_fused_lists_host(const double *lattice, const int *fftg, int ns,
+                        const double *frac, const double *rmax,
+                        const int *ngrid, int *counts, int *items, int *isites,
+                        int *bitem, int *rows_off, int *rows);
 
 /* <p_i|psi_nk> for all bands and k-points of one structure:
  * kfrac [nk][3] (host); gofs [nk+1] (host) offsets into G_dev (device

@@ -118,6 +118,16 @@ class SpherePlan:
         self.npts = npt.value
         self.proj_off = off
 
+    def fused_info(self):
+        """(usable, work items, partial rows, segments, flag) of the plan's
+        fused x-pass work lists; flag 0 means every segment found its slot
+        once."""
+        u, it, nr, fl = ct.c_int(), ct.c_int(), ct.c_int(), ct.c_int()
+        ns = ct.c_long()
+        check(lib().pp_sphere_fused_info(self.h, ct.byref(u), ct.byref(it), ct.byref(nr), ct.byref(ns),
+                                         ct.byref(fl)), "pp_sphere_fused_info")
+        return bool(u.value), it.value, nr.value, ns.value, fl.value
+
     def rebuild(self, frac_coords):
         fr = as_f64(frac_coords).reshape(-1, 3)
         check(lib().pp_sphere_rebuild(self.h, ptr(fr, c_double_p)), "pp_sphere_rebuild")

@@ -20,7 +20,7 @@ int last_code() { return g_code; }
 const char *kclass_name(int c) {
   static const char *names[K_NCLASS] = {"zero",          "scatter",    "fft_z",       "fft_y",    "fft_x",
                                         "phase",         "sphere_sum", "sphere_reduce", "table",  "gemm",
-                                        "gemm_reduce",   "aug_prep",   "aug_gemm", "misc"};
+                                        "gemm_reduce",   "aug_prep",   "aug_gemm", "misc", "fft_x_sphere"};
   return (c >= 0 && c < K_NCLASS) ? names[c] : "?";
 }
 
@@ -107,6 +107,10 @@ int sphere_ptot(pp_sphere *sp);
 int sphere_precision(pp_sphere *sp);
 const int *sphere_fftg(pp_sphere *sp);
 const double *sphere_lattice(pp_sphere *sp);
+// fused x pass with sphere sums (pp_fused.hip)
+bool fused_usable(pp_ctx *ctx, FftPlan *pl, pp_sphere *sp, int NB, bool fp32);
+int fused_x_sums(pp_ctx *ctx, FftPlan *pl, pp_sphere *sp, const cplx<float> *grid, int NB, int nbc,
+                 const SphereBounds *bounds, const cplx<float> *phase, cplx<double> *out, long ptot);
 
 template <typename T>
 static int setup_batched(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac, const int *gofs, const int *G_dev,
@@ -126,6 +130,8 @@ static int setup_batched(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac
     maxw = std::max(maxw, gofs[k + 1] - gofs[k]);
   const long npts_sph = std::max<long>(1, sphere_npts(sp));
   const bool two = ctx->tune_streams > 1 && (long)nk * ((nband + NB - 1) / NB) > 1;
+  // the x pass and the sphere sums as one kernel where the plan allows
+  const bool fused = fused_usable(ctx, pl, sp, NB, sizeof(T) == 4);
   PP_TRY(ctx->grid.ensure(sizeof(cplx<T>) * (size_t)npts_grid * NB));
   if (two) {
     PP_TRY(ctx->grid2.ensure(sizeof(cplx<T>) * (size_t)npts_grid * NB));
@@ -217,8 +223,12 @@ static int setup_batched(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac
       ctx->stream = streams[s];
       ctx->slot = s;
       st = fft_chunk<T>(ctx, pl, grids[s], NB, nw, gidx + gofs[k], C + ((long)k * nband + b0) * ldc, ldc, nbc,
-                        prune ? &bd : nullptr);
-      if (st == PP_OK)
+                        prune ? &bd : nullptr, !fused);
+      if (st == PP_OK && fused)
+        st = fused_x_sums(ctx, pl, sp, (const cplx<float> *)grids[s], NB, nbc, prune ? &bd : nullptr,
+                          (const cplx<float> *)(phase + (size_t)k * npts_sph), proj + ((long)k * nband + b0) * ptot,
+                          ptot);
+      else if (st == PP_OK)
         st = sphere_sums<T>(ctx, sp, grids[s], NB, nbc, phase + (size_t)k * npts_sph,
                             proj + ((long)k * nband + b0) * ptot, ptot);
     }
@@ -437,6 +447,8 @@ int pp_ctx_set_param(pp_ctx *ctx, const char *name, int value) {
     ctx->tune_fft_threads = value >= 512 ? 512 : 256;
   else if (n == "fft_radix")
     ctx->tune_fft_radix = value != 0 ? 1 : 0;
+  else if (n == "fused")
+    ctx->tune_fused = value != 0 ? 1 : 0;
   else if (n == "fft_tile")
     return pp_ctx_set_tuning(ctx, ctx->tune_fft_bands, value, ctx->tune_fft_prune, ctx->tune_sphere_parts,
                              ctx->tune_splitk);

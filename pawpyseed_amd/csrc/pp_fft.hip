@@ -14,8 +14,8 @@
 // and writes the line back in place.
 #include <cmath>
 
-#include "pp_dft.h"
 #include "pp_internal.h"
+#include "pp_stages.h"
 
 namespace pp {
 
@@ -226,9 +226,6 @@ __device__ void stage_generic(const cplx<T> *__restrict__ in, cplx<T> *__restric
   }
 }
 
-// Signed frequency of wrapped index i on an axis of length n.
-__device__ __forceinline__ int sgn_idx(int i, int n) { return i < n - n / 2 ? i : i - n; }
-
 // Line l of a pass: from the populated-line list of the z / y pass when
 // there is one, else l itself.
 template <int AXIS>
@@ -316,105 +313,6 @@ __global__ __launch_bounds__(256) void k_fft_pass(cplx<T> *__restrict__ g, AxisP
   for (int t = threadIdx.x; t < tot; t += 256) {
     int e = t / NBW, b = t % NBW;
     g[base + (long)e * es + b] = in[t];
-  }
-}
-
-// In-place Stockham stage: every thread keeps the R inputs of up to MAXB
-// butterflies in registers across one barrier, so a single LDS buffer
-// serves as input and output (half the LDS of the ping-pong form, twice
-// the work-groups per CU).
-//
-// Thread t of NT owns band slot b = t % NBW and butterflies j = t / NBW +
-// q * (NT / NBW); j / p comes from the multiplier pmag = ceil(2^32 / p)
-// (exact for j < 2^16), and the twiddles are read from the LDS copy of
-// the axis table.
-template <typename T, int NBW, int R, int MAXB, int NT = 256>
-__device__ __forceinline__ void stage_ip(cplx<T> *__restrict__ buf, const cplx<T> *__restrict__ tws, unsigned p,
-                                         unsigned pmag, unsigned Tn) {
-  constexpr unsigned JS = NT / NBW;
-  const unsigned b = threadIdx.x % NBW, j0 = threadIdx.x / NBW;
-  cplx<T> v[MAXB][R];
-  unsigned o[MAXB];
-#pragma unroll
-  for (int q = 0; q < MAXB; q++) {
-    const unsigned j = j0 + q * JS;
-    if (j < Tn) {
-      const unsigned h = p > 1 ? __umulhi(j, pmag) : j;
-      const unsigned k = j - h * p;
-      o[q] = h * p * R + k;
-      v[q][0] = buf[j * NBW + b];
-#pragma unroll
-      for (int r = 1; r < R; r++)
-        v[q][r] = buf[(j + r * Tn) * NBW + b] * tws[k * (R - 1) + (r - 1)];
-      Dft<T, R>::run(v[q]);
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < MAXB; q++) {
-    const unsigned j = j0 + q * JS;
-    if (j < Tn) {
-#pragma unroll
-      for (int r = 0; r < R; r++)
-        buf[(o[q] + r * p) * NBW + b] = v[q][r];
-    }
-  }
-  __syncthreads();
-}
-
-// Copy the twiddle/root table of one axis into LDS (visible after the
-// caller's next barrier).
-template <typename T, int NT = 256>
-__device__ __forceinline__ void stage_twiddles(cplx<T> *__restrict__ dst, const cplx<T> *__restrict__ tw, int len) {
-  for (int i = threadIdx.x; i < len; i += NT)
-    dst[i] = tw[i];
-}
-
-// All stages of a line in LDS.  Composite radices are compiled only into
-// the COMP instantiation with MAXB <= 2 (the host selects the composite
-// factorization only when every stage needs at most two butterflies per
-// thread); keeping them out of the other instantiations keeps their
-// register allocation that of radices 2-5.
-template <typename T, int NBW, int MAXB, int NT, bool COMP = false>
-__device__ __forceinline__ void stages_ip(cplx<T> *__restrict__ buf, const cplx<T> *__restrict__ twl,
-                                          const AxisPlan &pl, int n) {
-  for (int s = 0; s < pl.nst; s++) {
-    const unsigned R = pl.radix[s], p = pl.pprev[s], pm = pl.pmag[s], Tn = n / R;
-    const cplx<T> *tws = twl + pl.twoff[s];
-    switch (R) {
-    case 2:
-      stage_ip<T, NBW, 2, MAXB, NT>(buf, tws, p, pm, Tn);
-      break;
-    case 3:
-      stage_ip<T, NBW, 3, MAXB, NT>(buf, tws, p, pm, Tn);
-      break;
-    case 4:
-      stage_ip<T, NBW, 4, MAXB, NT>(buf, tws, p, pm, Tn);
-      break;
-    case 5:
-      stage_ip<T, NBW, 5, MAXB, NT>(buf, tws, p, pm, Tn);
-      break;
-    default:
-      if constexpr (COMP && MAXB <= 2) {
-        switch (R) {
-        case 6:
-          stage_ip<T, NBW, 6, MAXB, NT>(buf, tws, p, pm, Tn);
-          break;
-        case 8:
-          stage_ip<T, NBW, 8, MAXB, NT>(buf, tws, p, pm, Tn);
-          break;
-        case 9:
-          stage_ip<T, NBW, 9, MAXB, NT>(buf, tws, p, pm, Tn);
-          break;
-        case 12:
-          stage_ip<T, NBW, 12, MAXB, NT>(buf, tws, p, pm, Tn);
-          break;
-        case 16:
-          stage_ip<T, NBW, 16, MAXB, NT>(buf, tws, p, pm, Tn);
-          break;
-        }
-      }
-    }
   }
 }
 
@@ -773,17 +671,17 @@ static int run_pass(pp_ctx *ctx, cplx<T> *grid, const AxisPlan &ap, const cplx<T
   }
 }
 
-template <typename T> static const cplx<T> *tw_of(FftPlan *pl, int axis);
-template <> const cplx<double> *tw_of<double>(FftPlan *pl, int axis) {
+template <> const cplx<double> *fft_twiddles<double>(FftPlan *pl, int axis) {
   return (const cplx<double> *)pl->tw64.p + pl->twbase[axis];
 }
-template <> const cplx<float> *tw_of<float>(FftPlan *pl, int axis) {
+template <> const cplx<float> *fft_twiddles<float>(FftPlan *pl, int axis) {
   return (const cplx<float> *)pl->tw32.p + pl->twbase[axis];
 }
+template <typename T> static const cplx<T> *tw_of(FftPlan *pl, int axis) { return fft_twiddles<T>(pl, axis); }
 
 template <typename T>
 int fft_chunk(pp_ctx *ctx, FftPlan *pl, cplx<T> *grid, int NB, int nw, const int *gidx_dev,
-              const cplx<float> *C_dev, long ldc, int nbc, const SphereBounds *bounds) {
+              const cplx<float> *C_dev, long ldc, int nbc, const SphereBounds *bounds, bool last_pass) {
   const long n0 = pl->fftg[0], n1 = pl->fftg[1], n2 = pl->fftg[2];
   const long npts = n0 * n1 * n2;
   SphereBounds bd = {nullptr, nullptr, nullptr, (int)n0, (int)n1, (int)n2};
@@ -804,8 +702,9 @@ int fft_chunk(pp_ctx *ctx, FftPlan *pl, cplx<T> *grid, int NB, int nw, const int
                          K_FFT_Z, bd)));
   PP_TRY((run_pass<T, 1>(ctx, grid, pl->ax[1], tw_of<T>(pl, 1), n2 * NB, n1 * n2 * NB, NB, (int)n0, (int)n2, NB,
                          K_FFT_Y, bd)));
-  PP_TRY((run_pass<T, 0>(ctx, grid, pl->ax[0], tw_of<T>(pl, 0), n1 * n2 * NB, n2 * NB, NB, (int)n1, (int)n2, NB,
-                         K_FFT_X, bd)));
+  if (last_pass)
+    PP_TRY((run_pass<T, 0>(ctx, grid, pl->ax[0], tw_of<T>(pl, 0), n1 * n2 * NB, n2 * NB, NB, (int)n1, (int)n2, NB,
+                           K_FFT_X, bd)));
   return PP_OK;
 }
 
@@ -871,9 +770,9 @@ template int gather_conj<float>(pp_ctx *, const cplx<float> *, int, int, const i
                                 long);
 
 template int fft_chunk<double>(pp_ctx *, FftPlan *, cplx<double> *, int, int, const int *, const cplx<float> *,
-                               long, int, const SphereBounds *);
+                               long, int, const SphereBounds *, bool);
 template int fft_chunk<float>(pp_ctx *, FftPlan *, cplx<float> *, int, int, const int *, const cplx<float> *, long,
-                              int, const SphereBounds *);
+                              int, const SphereBounds *, bool);
 
 int grid_to_bandmajor(pp_ctx *ctx, int precision, const void *grid, int NB, int nbc, long npts, double scl,
                       void *out) {

@@ -25,6 +25,7 @@ enum KClass {
   K_AUG_PREP,
   K_AUG_GEMM,
   K_MISC,
+  K_FFT_X_SPHERE,
   K_NCLASS
 };
 const char *kclass_name(int c);
@@ -93,6 +94,7 @@ struct pp_ctx {
   int tune_sphere_unroll = 8;   // sphere-sum gathers in flight per wavefront (4 or 8)
   int tune_fft_threads = 256;   // work-group size of the in-place FFT passes with 32-band tiles (256 or 512)
   int tune_fft_radix = 0;       // 0: radices 2-5, 1: composite radices where they fit
+  int tune_fused = 0;           // 1: x pass fused with the sphere sums where the plan allows
   hipStream_t aux = nullptr;    // second stream of the chunk pipeline (default priority)
   int slot = 0;                 // chunk buffer set in use (0 or 1)
   pp::DevBuf grid, grid2;       // band-fastest FFT chunk, one per slot
@@ -120,9 +122,13 @@ int zero_async(pp_ctx *ctx, void *p, size_t bytes);
 
 // Band-fastest grid chunk pipeline pieces (pp_fft.hip).
 int compute_gidx(pp_ctx *ctx, const int *fftg, int nw, const int *G_dev, int *gidx_dev);
+// Scatter and backward FFT of a band chunk; with last_pass false the x
+// pass is left to the caller (the fused x pass with sphere sums).
 template <typename T>
 int fft_chunk(pp_ctx *ctx, FftPlan *pl, cplx<T> *grid, int NB, int nw, const int *gidx_dev,
-              const cplx<float> *C_dev, long ldc, int nbc, const SphereBounds *bounds);
+              const cplx<float> *C_dev, long ldc, int nbc, const SphereBounds *bounds, bool last_pass = true);
+// twiddle / root table of one axis in the precision of T
+template <typename T> const cplx<T> *fft_twiddles(FftPlan *pl, int axis);
 template <typename T> int fft_passes(pp_ctx *ctx, FftPlan *pl, cplx<T> *grid, int NB);
 template <typename T>
 int gather_conj(pp_ctx *ctx, const cplx<T> *grid, int NB, int nbc, const int *gidx, int nw, double scl,

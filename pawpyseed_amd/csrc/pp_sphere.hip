@@ -3,10 +3,13 @@
 // Table build replaces projector.c:193-209 / utils.c:590-696 (serial in the
 // reference): per site, every grid point of the bounding box
 // +-(int(|a_j x a_k| rmax / V * N_i) + 1) around round(frac * N) with
-// |r - R| < (n-1)/n * rmax, in the reference's i-j-k loop order.  Each
-// point stores its wrapped grid index, its unwrapped path r - R (for the
-// Bloch phase) and, for every projector j = (channel, l, m) of the site,
-// the real value t_j of the projector in real-harmonic form:
+// |r - R| < (n-1)/n * rmax.  The points of a site are listed line by line
+// (j, k outer, i innermost; the reference loops i-j-k, which only changes
+// the order of the floating-point sums), so the points of one site on one
+// x-line (j, k) form one run of consecutive i, a segment.  Each point
+// stores its wrapped grid index, its unwrapped path r - R (for the Bloch
+// phase) and, for every projector j = (channel, l, m) of the site, the
+// real value t_j of the projector in real-harmonic form:
 //   t_(l,m)  = A_lm cos(m phi)   (m >= 0),
 //   t_(l,-m) = A_lm sin(m phi)   (m > 0),
 // A_lm = N_lm P_l^m(cos theta) spline_l(|r|), evaluated on the
@@ -26,48 +29,9 @@
 #include <cmath>
 #include <cstring>
 
-#include "pp_internal.h"
-
-struct pp_sphere {
-  pp_ctx *ctx = nullptr;
-  int precision = PP_FP64;
-  double lattice[9];
-  int fftg[3];
-  int nsites = 0, nelems = 0;
-  std::vector<int> labels;
-  std::vector<double> frac;
-  std::vector<int> nchan, ls_off, ngrid, grid_off;
-  std::vector<int> ls;
-  std::vector<double> rmax;
-  std::vector<int> nproj_site, proj_off;  // host
-  std::vector<int> site_pt_h;
-  int ptot = 0, pmax = 0, pmin = 0;
-  long npts = 0;
-  // device
-  pp::DevBuf d_spline;  // per species: x, f, b, c, d [nchan][ngrid]
-  pp::DevBuf d_sitemeta;
-  pp::DevBuf d_site_pt, d_site_np, d_tab_off, d_proj_off, d_counts;
-  pp::DevBuf d_proj_m;  // m of every projector, [ptot]
-  pp::DevBuf d_pidx, d_path, d_table;
-  pp::DevBuf d_chan;  // per species channel l list
-};
+#include "pp_sphere.h"
 
 namespace pp {
-
-// per-site geometry handed to the table kernels
-struct SiteGeo {
-  double coord[3];
-  double rmax;
-  double R0;
-  int g[3];
-  int c[3];
-  int elem;
-  int nproj;
-  int nchan;
-  int ngrid;
-  long spl_off;  // offset (doubles) of the species spline block
-  int chan_off;  // offset into the channel-l array
-};
 
 __device__ __forceinline__ void d_frac_to_cart(const double *f, const double *lat, double *o) {
   o[0] = f[0] * lat[0] + f[1] * lat[3] + f[2] * lat[6];
@@ -79,13 +43,14 @@ struct Lat9 {
   double m[9];
 };
 
-// decode candidate c of a site's box into (i, j, k), reference loop order
+// decode candidate c of a site's box into (i, j, k): i fastest, then k,
+// then j; c % (2 g0 + 1) is the position of i on its line
 __device__ __forceinline__ void box_ijk(const SiteGeo &s, int c, int &i, int &j, int &k) {
-  int w2 = 2 * s.g[2] + 1, w1 = 2 * s.g[1] + 1;
-  k = c % w2 - s.g[2] + s.c[2];
-  int t = c / w2;
-  j = t % w1 - s.g[1] + s.c[1];
-  i = t / w1 - s.g[0] + s.c[0];
+  int w0 = 2 * s.g[0] + 1, w2 = 2 * s.g[2] + 1;
+  i = c % w0 - s.g[0] + s.c[0];
+  int t = c / w0;
+  k = t % w2 - s.g[2] + s.c[2];
+  j = t / w2 - s.g[1] + s.c[1];
 }
 
 __device__ __forceinline__ bool inside(const SiteGeo &s, const Lat9 &L, const int *fftg, int i, int j, int k,
@@ -96,44 +61,80 @@ __device__ __forceinline__ bool inside(const SiteGeo &s, const Lat9 &L, const in
   return sqrt(cv[0] * cv[0] + cv[1] * cv[1] + cv[2] * cv[2]) < s.R0;
 }
 
-__global__ __launch_bounds__(256) void k_site_count(const SiteGeo *__restrict__ geo, Lat9 L, int f0, int f1, int f2,
-                                                    int *__restrict__ counts) {
-  const SiteGeo s = geo[blockIdx.x];
-  const int fftg[3] = {f0, f1, f2};
-  int ncand = (2 * s.g[0] + 1) * (2 * s.g[1] + 1) * (2 * s.g[2] + 1);
-  int cnt = 0;
-  for (int c = threadIdx.x; c < ncand; c += 256) {
-    int i, j, k;
-    box_ijk(s, c, i, j, k);
-    double cv[3];
-    cnt += inside(s, L, fftg, i, j, k, cv) ? 1 : 0;
-  }
-  __shared__ int red[256];
-  red[threadIdx.x] = cnt;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o)
-      red[threadIdx.x] += red[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0)
-    counts[blockIdx.x] = red[0];
+// Candidate c of a site's box is inside its sphere.
+__device__ __forceinline__ bool cand_inside(const SiteGeo &s, const Lat9 &L, const int *fftg, int c, double *cv) {
+  int i, j, k;
+  box_ijk(s, c, i, j, k);
+  return inside(s, L, fftg, i, j, k, cv);
 }
 
-// Ordered compaction of the inside candidates (block scan per 256 chunk).
-__global__ __launch_bounds__(256) void k_site_fill(const SiteGeo *__restrict__ geo, Lat9 L, int f0, int f1, int f2,
-                                                   const int *__restrict__ site_pt, int *__restrict__ pidx,
-                                                   double *__restrict__ path) {
+// points (counts[s]) and segments (counts[nsites + s]) of every site.  The
+// candidates are visited in tiles of 256 consecutive ones; a segment starts
+// at an inside candidate that is the first of its line or follows an
+// outside one, judged from the same inside flags as the points.
+__global__ __launch_bounds__(256) void k_site_count(const SiteGeo *__restrict__ geo, Lat9 L, int f0, int f1, int f2,
+                                                    int nsites, int *__restrict__ counts) {
   const SiteGeo s = geo[blockIdx.x];
   const int fftg[3] = {f0, f1, f2};
-  int ncand = (2 * s.g[0] + 1) * (2 * s.g[1] + 1) * (2 * s.g[2] + 1);
-  __shared__ int scan[256];
-  __shared__ int running;
+  const int w0 = 2 * s.g[0] + 1;
+  const int ncand = w0 * (2 * s.g[1] + 1) * (2 * s.g[2] + 1);
+  __shared__ int flags[256], red[256], reds[256];
+  __shared__ int prev_last;
   if (threadIdx.x == 0)
+    prev_last = 0;
+  int cnt = 0, nseg = 0;
+  for (int c0 = 0; c0 < ncand; c0 += 256) {
+    const int c = c0 + threadIdx.x;
+    double cv[3];
+    const int in = (c < ncand && cand_inside(s, L, fftg, c, cv)) ? 1 : 0;
+    flags[threadIdx.x] = in;
+    __syncthreads();
+    const int prev = threadIdx.x > 0 ? flags[threadIdx.x - 1] : prev_last;
+    cnt += in;
+    nseg += (in && (c % w0 == 0 || !prev)) ? 1 : 0;
+    __syncthreads();
+    if (threadIdx.x == 255)
+      prev_last = flags[255];
+    __syncthreads();
+  }
+  red[threadIdx.x] = cnt;
+  reds[threadIdx.x] = nseg;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      red[threadIdx.x] += red[threadIdx.x + o];
+      reds[threadIdx.x] += reds[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    counts[blockIdx.x] = red[0];
+    counts[nsites + blockIdx.x] = reds[0];
+  }
+}
+
+// Ordered compaction of the inside candidates and of the segment starts
+// (block scans per tile of 256).  A segment is stored as (first point,
+// length, wrapped i of the first point, site) and its wrapped line
+// j * N2 + k; the lengths are filled in by k_seg_len.
+__global__ __launch_bounds__(256) void k_site_fill(const SiteGeo *__restrict__ geo, Lat9 L, int f0, int f1, int f2,
+                                                   const int *__restrict__ site_pt, const int *__restrict__ seg_off,
+                                                   int *__restrict__ pidx, double *__restrict__ path,
+                                                   int4 *__restrict__ seg, int *__restrict__ segline) {
+  const SiteGeo s = geo[blockIdx.x];
+  const int fftg[3] = {f0, f1, f2};
+  const int w0 = 2 * s.g[0] + 1;
+  const int ncand = w0 * (2 * s.g[1] + 1) * (2 * s.g[2] + 1);
+  __shared__ int flags[256], scan[256], scans[256];
+  __shared__ int running, running_s, prev_last;
+  if (threadIdx.x == 0) {
     running = site_pt[blockIdx.x];
+    running_s = seg_off[blockIdx.x];
+    prev_last = 0;
+  }
   __syncthreads();
   for (int c0 = 0; c0 < ncand; c0 += 256) {
-    int c = c0 + threadIdx.x;
+    const int c = c0 + threadIdx.x;
     int i = 0, j = 0, k = 0;
     double cv[3] = {0, 0, 0};
     int in = 0;
@@ -141,13 +142,20 @@ __global__ __launch_bounds__(256) void k_site_fill(const SiteGeo *__restrict__ g
       box_ijk(s, c, i, j, k);
       in = inside(s, L, fftg, i, j, k, cv) ? 1 : 0;
     }
-    scan[threadIdx.x] = in;
+    flags[threadIdx.x] = in;
     __syncthreads();
-    // Hillis-Steele inclusive scan
+    const int prev = threadIdx.x > 0 ? flags[threadIdx.x - 1] : prev_last;
+    const int st = (in && (c % w0 == 0 || !prev)) ? 1 : 0;
+    scan[threadIdx.x] = in;
+    scans[threadIdx.x] = st;
+    __syncthreads();
+    // Hillis-Steele inclusive scans
     for (int o = 1; o < 256; o <<= 1) {
       int v = threadIdx.x >= o ? scan[threadIdx.x - o] : 0;
+      int vs = threadIdx.x >= o ? scans[threadIdx.x - o] : 0;
       __syncthreads();
       scan[threadIdx.x] += v;
+      scans[threadIdx.x] += vs;
       __syncthreads();
     }
     int pos = running + scan[threadIdx.x] - in;
@@ -159,12 +167,32 @@ __global__ __launch_bounds__(256) void k_site_fill(const SiteGeo *__restrict__ g
       path[3 * (long)pos + 0] = cv[0];
       path[3 * (long)pos + 1] = cv[1];
       path[3 * (long)pos + 2] = cv[2];
+      if (st) {
+        const int q = running_s + scans[threadIdx.x] - 1;
+        seg[q] = make_int4(pos, 0, ii, blockIdx.x);
+        segline[q] = jj * fftg[2] + kk;
+      }
     }
     __syncthreads();
-    if (threadIdx.x == 255)
+    if (threadIdx.x == 255) {
       running += scan[255];
+      running_s += scans[255];
+      prev_last = flags[255];
+    }
     __syncthreads();
   }
+}
+
+// Segment lengths: the points of segment q run to the first point of the
+// next segment of its site, or to the end of the site's points.
+__global__ __launch_bounds__(256) void k_seg_len(int4 *__restrict__ seg, long nseg, const int *__restrict__ seg_off,
+                                                 const int *__restrict__ site_pt) {
+  const long q = (long)blockIdx.x * 256 + threadIdx.x;
+  if (q >= nseg)
+    return;
+  const int4 sg = seg[q];
+  const int end = q + 1 < seg_off[sg.w + 1] ? seg[q + 1].x : site_pt[sg.w + 1];
+  seg[q].y = end - sg.x;
 }
 
 __device__ double d_fact(int n) {
@@ -558,6 +586,29 @@ void spline_coeff_host(const double *x, const double *y, int n, double *b, doubl
   b[n - 1] = c[n - 1] = d[n - 1] = 0.0;
 }
 
+// line-block work lists of the fused x pass (pp_fused.hip)
+int fused_plan(pp_sphere *sp, const std::vector<SiteGeo> &geo);
+
+// Sphere radius and candidate box of a site with coord set:
+// R0 = (n-1)/n rmax (utils.c:651-652, the site's own species), box
+// half-widths g_i = int(|a_j x a_k| rmax / V * N_i) + 1 around
+// round(frac * N).
+void site_box(const double *lattice, const int *fftg, double rmax, int ngrid, SiteGeo &g) {
+  const double vol = det3(lattice);
+  g.rmax = rmax;
+  g.ngrid = ngrid;
+  g.R0 = (ngrid - 1) * rmax / ngrid;
+  double cr[3];
+  cross(lattice + 3, lattice + 6, cr);
+  g.g[0] = (int)(nrm(cr) * rmax / vol * fftg[0]) + 1;
+  cross(lattice + 0, lattice + 6, cr);
+  g.g[1] = (int)(nrm(cr) * rmax / vol * fftg[1]) + 1;
+  cross(lattice + 0, lattice + 3, cr);
+  g.g[2] = (int)(nrm(cr) * rmax / vol * fftg[2]) + 1;
+  for (int a = 0; a < 3; a++)
+    g.c[a] = (int)round(g.coord[a] * fftg[a]);
+}
+
 static int sphere_build(pp_sphere *sp) {
   pp_ctx *ctx = sp->ctx;
   const int ns = sp->nsites;
@@ -578,19 +629,7 @@ static int sphere_build(pp_sphere *sp) {
     int e = sp->labels[s];
     for (int a = 0; a < 3; a++)
       g.coord[a] = sp->frac[3 * s + a];
-    g.rmax = sp->rmax[e];
-    g.ngrid = sp->ngrid[e];
-    // utils.c:651-652 (radius from the site's own species)
-    g.R0 = (g.ngrid - 1) * g.rmax / g.ngrid;
-    double cr[3];
-    cross(sp->lattice + 3, sp->lattice + 6, cr);
-    g.g[0] = (int)(nrm(cr) * g.rmax / vol * sp->fftg[0]) + 1;
-    cross(sp->lattice + 0, sp->lattice + 6, cr);
-    g.g[1] = (int)(nrm(cr) * g.rmax / vol * sp->fftg[1]) + 1;
-    cross(sp->lattice + 0, sp->lattice + 3, cr);
-    g.g[2] = (int)(nrm(cr) * g.rmax / vol * sp->fftg[2]) + 1;
-    for (int a = 0; a < 3; a++)
-      g.c[a] = (int)round(g.coord[a] * sp->fftg[a]);
+    site_box(sp->lattice, sp->fftg, sp->rmax[e], sp->ngrid[e], g);
     g.elem = e;
     g.nchan = sp->nchan[e];
     int np = 0;
@@ -624,28 +663,32 @@ static int sphere_build(pp_sphere *sp) {
   memcpy(L.m, sp->lattice, sizeof(L.m));
   PP_TRY(sp->d_sitemeta.ensure(sizeof(SiteGeo) * ns));
   PP_CHECK_HIP(hipMemcpyAsync(sp->d_sitemeta.p, geo.data(), sizeof(SiteGeo) * ns, hipMemcpyHostToDevice, ctx->stream));
-  PP_TRY(sp->d_counts.ensure(sizeof(int) * ns));
+  PP_TRY(sp->d_counts.ensure(sizeof(int) * 2 * ns));
   hipEvent_t ev;
   tbegin(ctx, K_TABLE, &ev);
   hipLaunchKernelGGL(k_site_count, dim3(ns), dim3(256), 0, ctx->stream, (const SiteGeo *)sp->d_sitemeta.p, L,
-                     sp->fftg[0], sp->fftg[1], sp->fftg[2], (int *)sp->d_counts.p);
+                     sp->fftg[0], sp->fftg[1], sp->fftg[2], ns, (int *)sp->d_counts.p);
   tend(ctx, K_TABLE, ev);
   PP_CHECK_HIP(hipGetLastError());
-  std::vector<int> counts(ns);
-  PP_CHECK_HIP(hipMemcpyAsync(counts.data(), sp->d_counts.p, sizeof(int) * ns, hipMemcpyDeviceToHost, ctx->stream));
+  std::vector<int> counts(2 * ns);
+  PP_CHECK_HIP(
+      hipMemcpyAsync(counts.data(), sp->d_counts.p, sizeof(int) * 2 * ns, hipMemcpyDeviceToHost, ctx->stream));
   PP_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   sp->site_pt_h.assign(ns + 1, 0);
+  sp->seg_off_h.assign(ns + 1, 0);
   std::vector<long> tab_off(ns);
   long to = 0;
   int maxcount = 0;
   for (int s = 0; s < ns; s++) {
     sp->site_pt_h[s + 1] = sp->site_pt_h[s] + counts[s];
+    sp->seg_off_h[s + 1] = sp->seg_off_h[s] + counts[ns + s];
     tab_off[s] = to;
     to += (long)counts[s] * sp->nproj_site[s];
     if (counts[s] > maxcount)
       maxcount = counts[s];
   }
   sp->npts = sp->site_pt_h[ns];
+  sp->nseg = sp->seg_off_h[ns];
   size_t csz = sp->precision == PP_FP64 ? sizeof(double) : sizeof(float);
   PP_TRY(sp->d_site_pt.ensure(sizeof(int) * (ns + 1)));
   PP_TRY(sp->d_site_np.ensure(sizeof(int) * ns));
@@ -654,6 +697,11 @@ static int sphere_build(pp_sphere *sp) {
   PP_TRY(sp->d_pidx.ensure(sizeof(int) * std::max<long>(1, sp->npts)));
   PP_TRY(sp->d_path.ensure(sizeof(double) * 3 * std::max<long>(1, sp->npts)));
   PP_TRY(sp->d_table.ensure(csz * std::max<long>(1, to)));
+  PP_TRY(sp->d_seg_off.ensure(sizeof(int) * (ns + 1)));
+  PP_TRY(sp->d_seg.ensure(sizeof(int4) * std::max<long>(1, sp->nseg)));
+  PP_TRY(sp->d_segline.ensure(sizeof(int) * std::max<long>(1, sp->nseg)));
+  PP_CHECK_HIP(hipMemcpyAsync(sp->d_seg_off.p, sp->seg_off_h.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice,
+                              ctx->stream));
   PP_CHECK_HIP(hipMemcpyAsync(sp->d_site_pt.p, sp->site_pt_h.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice,
                               ctx->stream));
   PP_CHECK_HIP(hipMemcpyAsync(sp->d_site_np.p, sp->nproj_site.data(), sizeof(int) * ns, hipMemcpyHostToDevice,
@@ -663,8 +711,12 @@ static int sphere_build(pp_sphere *sp) {
                               ctx->stream));
   tbegin(ctx, K_TABLE, &ev);
   hipLaunchKernelGGL(k_site_fill, dim3(ns), dim3(256), 0, ctx->stream, (const SiteGeo *)sp->d_sitemeta.p, L,
-                     sp->fftg[0], sp->fftg[1], sp->fftg[2], (const int *)sp->d_site_pt.p, (int *)sp->d_pidx.p,
-                     (double *)sp->d_path.p);
+                     sp->fftg[0], sp->fftg[1], sp->fftg[2], (const int *)sp->d_site_pt.p,
+                     (const int *)sp->d_seg_off.p, (int *)sp->d_pidx.p, (double *)sp->d_path.p, (int4 *)sp->d_seg.p,
+                     (int *)sp->d_segline.p);
+  if (sp->nseg > 0)
+    hipLaunchKernelGGL(k_seg_len, dim3((unsigned)((sp->nseg + 255) / 256)), dim3(256), 0, ctx->stream,
+                       (int4 *)sp->d_seg.p, sp->nseg, (const int *)sp->d_seg_off.p, (const int *)sp->d_site_pt.p);
   tend(ctx, K_TABLE, ev);
   PP_CHECK_HIP(hipGetLastError());
   if (maxcount > 0) {
@@ -683,7 +735,7 @@ static int sphere_build(pp_sphere *sp) {
     tend(ctx, K_TABLE, ev);
     PP_CHECK_HIP(hipGetLastError());
   }
-  return PP_OK;
+  return fused_plan(sp, geo);
 }
 
 template <typename T, int PMAX>
