@@ -62,7 +62,10 @@ pswf_t *read_wavefunctions_from_str(char *start, double *kpt_weights);
  * data a WAVECAR holds).  kpts [nwk][3]; weights [nwk]; nplane [nspin*nwk];
  * Gs: concatenated int32 [sum nplane][3]; coeffs: concatenated c64
  * [sum over (spin,k) of nband*nplane] (band-major per k);
- * occ, energy: [nspin*nwk][nband].  k-index = k + spin*nwk. */
+ * occ, energy: [nspin*nwk][nband].  k-index = k + spin*nwk.  A k-point
+ * whose G list is one list of W vectors given twice (nplane = 2 W, as the
+ * reader keeps a spinor's list, reader.c:274-281) marks the wavefunction
+ * noncollinear, its coefficients being the up then the down component. */
 pswf_t *pp_pswf_from_arrays(int nspin, int nwk, int nband, double encut,
                             const double *lattice, const double *kpts,
                             const double *weights, const int *nplane,
@@ -98,9 +101,16 @@ ppot_t *get_projector_list(int num_els, int *labels, int *ls,
  * (utils.c:590-696), then scatter -> 3D FFT -> sphere sums for every
  * (band, k) on the GPU.  Takes ownership of pps as the reference does.
  * Fills band->projections of every band (host copies) and keeps the
- * projections resident in HBM for compensation_terms. */
+ * projections resident in HBM for compensation_terms.  Spinor
+ * wavefunctions (is_ncl) get band->up_projections and down_projections,
+ * each component projected on its own (onto_projector_ncl,
+ * projector.c:359-392). */
 int setup_projections(pswf_t *wf, ppot_t *pps, int num_elems, int num_sites,
                       int *fftg, int *labels, double *coords);
+
+/* overlap_setup_real / _recip and compensation_terms / _recip return
+ * PP_ERR_UNSUPPORTED for spinor wavefunctions, which the reference's
+ * Projector refuses (projector.py:74-75). */
 
 /* replaces pseudoprojector.c:63-90 pseudoprojection:
  * projections[b*NK + k] = <C_ref[k'][b] | C_proj[k][BAND_NUM]>.
@@ -164,7 +174,9 @@ int fft3d(pp_cdouble *x, int *G_bounds, double *lattice, double *kpt, int *Gs,
           pp_cfloat *Cs, int num_waves, int *fftg);
 
 /* No reference counterpart: copies projections of one (band, k-index)
- * into out (c128 [total projectors]) from the device-resident table. */
+ * into out (c128 [total projectors]) from the device-resident table; for
+ * a spinor wavefunction out holds the up component, then the down one
+ * (c128 [2][total projectors]). */
 int pp_get_projections(pswf_t *wf, int band, int kidx, pp_cdouble *out);
 
 /* ===================================================================== */
@@ -173,6 +185,7 @@ int pp_get_projections(pswf_t *wf, int band, int kidx, pp_cdouble *out);
 
 typedef struct pp_ctx pp_ctx;
 typedef struct pp_sphere pp_sphere;
+typedef struct pp_graph pp_graph;
 
 /* Context: one HIP stream (default priority) on `device`, workspaces. */
 int pp_ctx_create(pp_ctx **out, int device);
@@ -207,6 +220,38 @@ int pp_ctx_set_tuning(pp_ctx *ctx, int fft_bands, int fft_tile, int fft_prune,
  * plan allows -- it writes no real-space grid but measured slower on the
  * 216-atom workload). */
 int pp_ctx_set_param(pp_ctx *ctx, const char *name, int value);
+/* Current value of a parameter of pp_ctx_set_param (0 = the built-in
+ * default for fft_tile, fft_bands, sphere_parts, gemm_splitk). */
+int pp_ctx_get_param(pp_ctx *ctx, const char *name, int *value);
+/* Autotuner of the setup path (no counterpart in the reference; the
+ * pp_autotune of the boundary's batched entry points): coordinate descent
+ * over streams, fft_tile, fft_threads, sphere_unroll, sphere_parts,
+ * fft_radix, fft_xcd, fft_bands and (FP32) fused, each candidate timed
+ * with HIP events on the caller's own inputs, best of `repeats` runs after
+ * a warm-up.  The fastest values stay set on ctx, and proj_dev holds the
+ * projections of a final run with them.  Arguments as
+ * pp_setup_projections_batched; report (may be NULL) receives "name value
+ * ms" lines, truncated to report_len. */
+/* The setup path captured in a HIP graph (no counterpart in the
+ * reference; the pp_graph_capture/replay of the boundary's batched entry
+ * points).  Capture runs the setup once eagerly (sizing every buffer;
+ * proj_dev then holds valid projections), prepares the host-side lists,
+ * and records the chunk pipeline (scatter, FFT passes, sphere sums on the
+ * context's two streams) as one graph.  Replay launches it on the
+ * context's stream, asynchronously: it recomputes proj_dev from the
+ * current contents of C_dev for the same G vectors, k-points and plan.
+ * Replay refuses (PP_ERR_ARG) once the plan was rebuilt or the context's
+ * work buffers were reallocated by a larger call after the capture. */
+int pp_graph_capture(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac,
+                     const int *gofs, const int *G_dev, const void *C_dev,
+                     long ldc, int nband, void *proj_dev, int precision,
+                     pp_graph **out);
+int pp_graph_replay(pp_graph *graph);
+void pp_graph_destroy(pp_graph *graph);
+int pp_autotune(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac,
+                const int *gofs, const int *G_dev, const void *C_dev, long ldc,
+                int nband, void *proj_dev, int precision, int repeats,
+                char *report, int report_len);
 
 /* Sphere plan of one structure (replaces the table build of
  * projector.c:193-209 / utils.c:590-696, done on the GPU).

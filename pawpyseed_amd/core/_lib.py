@@ -84,6 +84,13 @@ SIGNATURES = {
     "pp_ctx_timing_reset": (ct.c_int, [c_void_p]),
     "pp_ctx_set_tuning": (ct.c_int, [c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_int]),
     "pp_ctx_set_param": (ct.c_int, [c_void_p, ct.c_char_p, ct.c_int]),
+    "pp_ctx_get_param": (ct.c_int, [c_void_p, ct.c_char_p, c_int_p]),
+    "pp_graph_capture": (ct.c_int, [c_void_p, c_void_p, ct.c_int, c_double_p, c_int_p, c_void_p, c_void_p,
+                                    ct.c_long, ct.c_int, c_void_p, ct.c_int, ct.POINTER(c_void_p)]),
+    "pp_graph_replay": (ct.c_int, [c_void_p]),
+    "pp_graph_destroy": (None, [c_void_p]),
+    "pp_autotune": (ct.c_int, [c_void_p, c_void_p, ct.c_int, c_double_p, c_int_p, c_void_p, c_void_p, ct.c_long,
+                               ct.c_int, c_void_p, ct.c_int, ct.c_int, ct.c_char_p, ct.c_int]),
     "pp_sphere_create": (ct.c_int, [c_void_p, ct.POINTER(c_void_p), c_double_p, c_int_p, ct.c_int, c_int_p,
                                     c_double_p, ct.c_int, c_int_p, c_int_p, c_double_p, c_int_p, c_double_p,
                                     ct.c_int]),

@@ -75,6 +75,11 @@ class Context:
     def set_param(self, name: str, value: int):
         check(lib().pp_ctx_set_param(self.h, name.encode(), int(value)), "pp_ctx_set_param")
 
+    def get_param(self, name: str) -> int:
+        v = ct.c_int()
+        check(lib().pp_ctx_get_param(self.h, name.encode(), ct.byref(v)), "pp_ctx_get_param")
+        return v.value
+
     def close(self):
         if getattr(self, "h", None):
             lib().pp_ctx_destroy(self.h)
@@ -165,6 +170,62 @@ def setup_projections(ctx: Context, plan: SpherePlan, kfrac, gofs, G_dev, C_dev,
                                              _dptr(G_dev), _dptr(C_dev), ldc, nband, _dptr(proj_out),
                                              _prec(precision)), "pp_setup_projections_batched")
     _done(ctx, sync)
+
+
+class SetupGraph:
+    """pp_graph: the setup path of fixed buffers captured once, replayed on
+    the context's stream (recomputes proj_out from the current C_dev)."""
+
+    def __init__(self, ctx: Context, plan: SpherePlan, kfrac, gofs, G_dev, C_dev, proj_out, precision="fp64"):
+        nk, nband, ldc = C_dev.shape
+        self._kf = as_f64(kfrac).reshape(-1, 3)
+        self._go = as_i32(gofs)
+        assert proj_out.shape == (nk, nband, plan.ptot)
+        self.ctx = ctx
+        self._keep = (plan, G_dev, C_dev, proj_out)
+        h = ct.c_void_p()
+        check(lib().pp_graph_capture(ctx.h, plan.h, nk, ptr(self._kf, c_double_p), ptr(self._go, c_int_p),
+                                     _dptr(G_dev), _dptr(C_dev), ldc, nband, _dptr(proj_out), _prec(precision),
+                                     ct.byref(h)), "pp_graph_capture")
+        self.h = h
+
+    def replay(self, sync=True):
+        check(lib().pp_graph_replay(self.h), "pp_graph_replay")
+        _done(self.ctx, sync)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pp_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+TUNED = ("streams", "fft_tile", "fft_threads", "sphere_unroll", "sphere_parts", "fft_radix", "fft_xcd", "fft_bands",
+         "fused")
+
+
+def autotune(ctx: Context, plan: SpherePlan, kfrac, gofs, G_dev, C_dev, proj_out, precision="fp64", repeats=2):
+    """pp_autotune on these inputs: the fastest settings stay on ctx and
+    proj_out holds the projections of a run with them.  Returns
+    (settings, [(name, value, ms), ...])."""
+    nk, nband, ldc = C_dev.shape
+    kf = as_f64(kfrac).reshape(-1, 3)
+    go = as_i32(gofs)
+    assert proj_out.shape == (nk, nband, plan.ptot)
+    rep = ct.create_string_buffer(8192)
+    check(lib().pp_autotune(ctx.h, plan.h, nk, ptr(kf, c_double_p), ptr(go, c_int_p), _dptr(G_dev), _dptr(C_dev),
+                            ldc, nband, _dptr(proj_out), _prec(precision), int(repeats), rep, len(rep)),
+          "pp_autotune")
+    trials = []
+    for line in rep.value.decode().splitlines():
+        n, v, ms = line.split()
+        trials.append((n, int(v), float(ms)))
+    return {n: ctx.get_param(n) for n in TUNED}, trials
 
 
 def overlap_gemm(ctx: Context, nwk, nspin, flip, nw, CR, CS, out, precision="fp64", sync=True):

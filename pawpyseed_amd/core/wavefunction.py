@@ -132,6 +132,15 @@ class Wavefunction:
     CWavefunction/PseudoWavefunction Cython bases)."""
 
     def __init__(self, struct, pwf: PWFPointer, cr: CoreRegion, dim, symprec=1e-4, setup_projectors=False):
+        # checked before the object takes the pointer, so a refused one is
+        # not freed here
+        if lib().is_ncl(pwf.ptr) > 0:
+            raise PAWpyError("Pseudowavefunction is noncollinear! Call NCLWavefunction(...) instead")
+        self._setup(struct, pwf, cr, dim, symprec)
+        if setup_projectors:
+            self.check_c_projectors()
+
+    def _setup(self, struct, pwf, cr, dim, symprec):
         self.band_props = pwf.band_props.copy()
         self._pwf = pwf
         self.wf_ptr = pwf.ptr
@@ -143,8 +152,6 @@ class Wavefunction:
         self.nwk = L.get_nwk(self.wf_ptr)
         self.nspin = L.get_nspin(self.wf_ptr)
         self.encut = L.get_encut(self.wf_ptr)
-        if self.ncl:
-            raise PAWpyError("Pseudowavefunction is noncollinear! Noncollinear projection is not supported")
         self.structure = struct
         self.symprec = symprec
         self.cr = cr
@@ -152,8 +159,6 @@ class Wavefunction:
         if len(self.dim) != 3:
             raise PAWpyError("Grid dimensions must be length 3")
         self.projector_owner = 0
-        if setup_projectors:
-            self.check_c_projectors()
 
     # ------------------------------------------------------------ factories
     @staticmethod
@@ -255,12 +260,14 @@ class Wavefunction:
             self._make_c_projectors()
 
     def get_projections(self, band, k, spin=0):
-        """<p_i|psi_{band,k,spin}> for all projectors of all sites (c128)."""
+        """<p_i|psi_{band,k,spin}> for all projectors of all sites (c128);
+        for a spinor wavefunction shape (2, P): up, then down component."""
         self.check_bks_spec(band, k, spin)
-        out = np.zeros(int(self._ptot()), np.complex128)
+        ncomp = 2 if self.ncl else 1
+        out = np.zeros(ncomp * int(self._ptot()), np.complex128)
         check(lib().pp_get_projections(self.wf_ptr, band, k + spin * self.nwk, out.ctypes.data_as(ct.c_void_p)),
               "pp_get_projections")
-        return out
+        return out.reshape(2, -1) if self.ncl else out
 
     def _ptot(self):
         labels = self._element_labels()

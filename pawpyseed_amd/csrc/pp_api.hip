@@ -107,142 +107,178 @@ int sphere_ptot(pp_sphere *sp);
 int sphere_precision(pp_sphere *sp);
 const int *sphere_fftg(pp_sphere *sp);
 const double *sphere_lattice(pp_sphere *sp);
+long sphere_generation(pp_sphere *sp);
 // fused x pass with sphere sums (pp_fused.hip)
 bool fused_usable(pp_ctx *ctx, FftPlan *pl, pp_sphere *sp, int NB, bool fp32);
 int fused_x_sums(pp_ctx *ctx, FftPlan *pl, pp_sphere *sp, const cplx<float> *grid, int NB, int nbc,
                  const SphereBounds *bounds, const cplx<float> *phase, cplx<double> *out, long ptot);
 
-template <typename T>
-static int setup_batched(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac, const int *gofs, const int *G_dev,
-                         const cplx<float> *C, long ldc, int nband, cplx<double> *proj) {
-  const int *fftg = sphere_fftg(sp);
+// Host-side preparation of a setup call: buffers, FFT plan, plane-wave
+// sphere bounds of every k (one host sync for the alias flags), the
+// populated-line lists, grid indices and Bloch phases.  Everything after it
+// (setup_launch) is device work on the context's streams only, so it can be
+// captured in a graph.
+struct SetupState {
   FftPlan *pl = nullptr;
-  PP_TRY(get_fft_plan(ctx, fftg, &pl));
+  int NB = 64, nbnd = 0;
+  bool two = false, fused = false;
+  long npts_sph = 1, ptot = 0;
+  void *grids[2] = {nullptr, nullptr};
+  void *phase = nullptr;
+  int *gidx = nullptr, *bnd = nullptr, *dlines = nullptr;
+  std::vector<int> alias;
+  std::vector<long> zl_off, yl_off;
+};
+
+template <typename T>
+static int setup_prepare(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac, const int *gofs, const int *G_dev,
+                         int nband, SetupState &S) {
+  const int *fftg = sphere_fftg(sp);
+  PP_TRY(get_fft_plan(ctx, fftg, &S.pl));
   int NB = ctx->tune_fft_bands > 0 ? ctx->tune_fft_bands : 64;
   if (NB > 64)
     NB = 64;
   if (NB > nband)
     NB = nband < 1 ? 1 : nband;
+  S.NB = NB;
   const long npts_grid = (long)fftg[0] * fftg[1] * fftg[2];
-  const long ptot = sphere_ptot(sp);
-  int maxw = 0;
-  for (int k = 0; k < nk; k++)
-    maxw = std::max(maxw, gofs[k + 1] - gofs[k]);
-  const long npts_sph = std::max<long>(1, sphere_npts(sp));
-  const bool two = ctx->tune_streams > 1 && (long)nk * ((nband + NB - 1) / NB) > 1;
+  S.ptot = sphere_ptot(sp);
+  S.npts_sph = std::max<long>(1, sphere_npts(sp));
+  S.two = ctx->tune_streams > 1 && (long)nk * ((nband + NB - 1) / NB) > 1;
   // the x pass and the sphere sums as one kernel where the plan allows
-  const bool fused = fused_usable(ctx, pl, sp, NB, sizeof(T) == 4);
+  S.fused = fused_usable(ctx, S.pl, sp, NB, sizeof(T) == 4);
   PP_TRY(ctx->grid.ensure(sizeof(cplx<T>) * (size_t)npts_grid * NB));
-  if (two) {
+  if (S.two) {
     PP_TRY(ctx->grid2.ensure(sizeof(cplx<T>) * (size_t)npts_grid * NB));
     if (!ctx->aux)
       PP_CHECK_HIP(hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking));
+    if (!ctx->fork)
+      PP_CHECK_HIP(hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming));
+    if (!ctx->join)
+      PP_CHECK_HIP(hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming));
   }
   PP_TRY(ctx->gidx.ensure(sizeof(int) * (size_t)std::max(1, gofs[nk])));
-  PP_TRY(ctx->phase.ensure(sizeof(cplx<T>) * (size_t)npts_sph * nk));
-  cplx<T> *grids[2] = {(cplx<T> *)ctx->grid.p, two ? (cplx<T> *)ctx->grid2.p : (cplx<T> *)ctx->grid.p};
-  cplx<T> *phase = (cplx<T> *)ctx->phase.p;
-  int *gidx = (int *)ctx->gidx.p;
+  PP_TRY(ctx->phase.ensure(sizeof(cplx<T>) * (size_t)S.npts_sph * nk));
+  S.grids[0] = ctx->grid.p;
+  S.grids[1] = S.two ? ctx->grid2.p : ctx->grid.p;
+  S.phase = ctx->phase.p;
+  S.gidx = (int *)ctx->gidx.p;
   // plane-wave sphere bounds of every k (one host sync for the alias flags)
-  const int nbnd = 2 * fftg[0] * fftg[1] + 2 * fftg[0] + 3;
-  PP_TRY(ctx->bounds.ensure(sizeof(int) * (size_t)nbnd * nk));
-  int *bnd = (int *)ctx->bounds.p;
+  S.nbnd = 2 * fftg[0] * fftg[1] + 2 * fftg[0] + 3;
+  PP_TRY(ctx->bounds.ensure(sizeof(int) * (size_t)S.nbnd * nk));
+  S.bnd = (int *)ctx->bounds.p;
   for (int k = 0; k < nk; k++)
-    PP_TRY(sphere_bounds(ctx, fftg, gofs[k + 1] - gofs[k], G_dev + 3 * (long)gofs[k], bnd + (size_t)k * nbnd));
-  ctx->bounds_host.resize((size_t)nbnd * nk);
-  PP_CHECK_HIP(hipMemcpyAsync(ctx->bounds_host.data(), bnd, sizeof(int) * ctx->bounds_host.size(),
+    PP_TRY(sphere_bounds(ctx, fftg, gofs[k + 1] - gofs[k], G_dev + 3 * (long)gofs[k], S.bnd + (size_t)k * S.nbnd));
+  ctx->bounds_host.resize((size_t)S.nbnd * nk);
+  PP_CHECK_HIP(hipMemcpyAsync(ctx->bounds_host.data(), S.bnd, sizeof(int) * ctx->bounds_host.size(),
                               hipMemcpyDeviceToHost, ctx->stream));
   PP_CHECK_HIP(hipStreamSynchronize(ctx->stream));
-  std::vector<int> alias(nk, 1);
+  S.alias.assign(nk, 1);
   // populated lines of the z pass ((x, y) in the sphere's hull) and of the
   // y pass ((x, z) with x in the slab), so no work-group is launched for an
   // empty line
-  std::vector<long> zl_off(nk + 1, 0), yl_off(nk + 1, 0);
+  S.zl_off.assign(nk + 1, 0);
+  S.yl_off.assign(nk + 1, 0);
   ctx->lines_host.clear();
   const int n0 = fftg[0], n1 = fftg[1], n2 = fftg[2];
   for (int k = 0; k < nk; k++) {
-    const int *hb = ctx->bounds_host.data() + (size_t)k * nbnd;
-    alias[k] = hb[nbnd - 1];
+    const int *hb = ctx->bounds_host.data() + (size_t)k * S.nbnd;
+    S.alias[k] = hb[S.nbnd - 1];
     const int *yb = hb + 2 * n0 * n1, *xb = yb + 2 * n0;
-    zl_off[k] = (long)ctx->lines_host.size();
-    if (!alias[k])
+    S.zl_off[k] = (long)ctx->lines_host.size();
+    if (!S.alias[k])
       for (int sx = xb[0]; sx <= xb[1]; sx++) {
         const int x = (sx + n0) % n0;
         for (int sy = yb[2 * x]; sy <= yb[2 * x + 1]; sy++)
           ctx->lines_host.push_back(x * n1 + (sy + n1) % n1);
       }
-    yl_off[k] = (long)ctx->lines_host.size();
-    if (!alias[k])
+    S.yl_off[k] = (long)ctx->lines_host.size();
+    if (!S.alias[k])
       for (int sx = xb[0]; sx <= xb[1]; sx++)
         for (int z = 0; z < n2; z++)
           ctx->lines_host.push_back(((sx + n0) % n0) * n2 + z);
-    zl_off[k + 1] = (long)ctx->lines_host.size();
+    S.zl_off[k + 1] = (long)ctx->lines_host.size();
   }
-  int *dlines = nullptr;
+  S.dlines = nullptr;
   if (!ctx->lines_host.empty()) {
     PP_TRY(ctx->lines.ensure(sizeof(int) * ctx->lines_host.size()));
-    dlines = (int *)ctx->lines.p;
+    S.dlines = (int *)ctx->lines.p;
     // the streams are idle here (synchronised above), and lines_host lives in
     // the context until the next call's synchronisation
-    PP_CHECK_HIP(hipMemcpyAsync(dlines, ctx->lines_host.data(), sizeof(int) * ctx->lines_host.size(),
+    PP_CHECK_HIP(hipMemcpyAsync(S.dlines, ctx->lines_host.data(), sizeof(int) * ctx->lines_host.size(),
                                 hipMemcpyHostToDevice, ctx->stream));
   }
   // grid indices and Bloch phases of every k, so the chunks of all k can be
   // in flight on both streams
   for (int k = 0; k < nk; k++) {
-    PP_TRY(compute_gidx(ctx, fftg, gofs[k + 1] - gofs[k], G_dev + 3 * (long)gofs[k], gidx + gofs[k]));
-    PP_TRY(make_phase<T>(ctx, sp, kfrac + 3 * k, phase + (size_t)k * npts_sph, false));
+    PP_TRY(compute_gidx(ctx, fftg, gofs[k + 1] - gofs[k], G_dev + 3 * (long)gofs[k], S.gidx + gofs[k]));
+    PP_TRY(make_phase<T>(ctx, sp, kfrac + 3 * k, (cplx<T> *)S.phase + (size_t)k * S.npts_sph, false));
   }
-  // Chunk pipeline: chunk c runs on stream c % 2 with buffer set c % 2, so
-  // one chunk's sphere sums overlap the next chunk's FFT passes.
+  return PP_OK;
+}
+
+// The chunk pipeline of a prepared setup call.  Chunk c runs on stream
+// c % 2 with buffer set c % 2, so one chunk's sphere sums overlap the next
+// chunk's FFT passes; the caller's stream waits for the second stream at
+// the end.
+template <typename T>
+static int setup_launch(pp_ctx *ctx, pp_sphere *sp, int nk, const int *gofs, const cplx<float> *C, long ldc,
+                        int nband, cplx<double> *proj, const SetupState &S) {
+  const int *fftg = sphere_fftg(sp);
+  const int NB = S.NB;
+  const long ptot = S.ptot;
+  const bool two = S.two;
   hipStream_t streams[2] = {ctx->stream, two ? ctx->aux : ctx->stream};
-  hipEvent_t fork = nullptr, join = nullptr;
   if (two) {
-    PP_CHECK_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
-    PP_CHECK_HIP(hipEventCreateWithFlags(&join, hipEventDisableTiming));
-    PP_CHECK_HIP(hipEventRecord(fork, ctx->stream));
-    PP_CHECK_HIP(hipStreamWaitEvent(ctx->aux, fork, 0));
+    PP_CHECK_HIP(hipEventRecord(ctx->fork, ctx->stream));
+    PP_CHECK_HIP(hipStreamWaitEvent(ctx->aux, ctx->fork, 0));
   }
   int st = PP_OK;
   long c = 0;
   for (int k = 0; k < nk && st == PP_OK; k++) {
     const int nw = gofs[k + 1] - gofs[k];
-    const int *kb = bnd + (size_t)k * nbnd;
+    const int *kb = S.bnd + (size_t)k * S.nbnd;
     SphereBounds bd = {kb, kb + 2 * fftg[0] * fftg[1], kb + 2 * fftg[0] * fftg[1] + 2 * fftg[0], fftg[0], fftg[1],
                        fftg[2]};
-    if (dlines && !alias[k] && ctx->tune_fft_prune != 3) {
-      bd.zl = dlines + zl_off[k];
-      bd.nzl = (int)(yl_off[k] - zl_off[k]);
-      bd.yl = dlines + yl_off[k];
-      bd.nyl = (int)(zl_off[k + 1] - yl_off[k]);
+    if (S.dlines && !S.alias[k] && ctx->tune_fft_prune != 3) {
+      bd.zl = S.dlines + S.zl_off[k];
+      bd.nzl = (int)(S.yl_off[k] - S.zl_off[k]);
+      bd.yl = S.dlines + S.yl_off[k];
+      bd.nyl = (int)(S.zl_off[k + 1] - S.yl_off[k]);
     }
-    const bool prune = alias[k] == 0 && ctx->tune_fft_prune != 2;
+    const bool prune = S.alias[k] == 0 && ctx->tune_fft_prune != 2;
+    const cplx<T> *phase_k = (const cplx<T> *)S.phase + (size_t)k * S.npts_sph;
     for (int b0 = 0; b0 < nband && st == PP_OK; b0 += NB, c++) {
       const int nbc = std::min(NB, nband - b0);
       const int s = two ? (int)(c & 1) : 0;
       ctx->stream = streams[s];
       ctx->slot = s;
-      st = fft_chunk<T>(ctx, pl, grids[s], NB, nw, gidx + gofs[k], C + ((long)k * nband + b0) * ldc, ldc, nbc,
-                        prune ? &bd : nullptr, !fused);
-      if (st == PP_OK && fused)
-        st = fused_x_sums(ctx, pl, sp, (const cplx<float> *)grids[s], NB, nbc, prune ? &bd : nullptr,
-                          (const cplx<float> *)(phase + (size_t)k * npts_sph), proj + ((long)k * nband + b0) * ptot,
-                          ptot);
+      cplx<T> *grid = (cplx<T> *)S.grids[s];
+      st = fft_chunk<T>(ctx, S.pl, grid, NB, nw, S.gidx + gofs[k], C + ((long)k * nband + b0) * ldc, ldc, nbc,
+                        prune ? &bd : nullptr, !S.fused);
+      if (st == PP_OK && S.fused)
+        st = fused_x_sums(ctx, S.pl, sp, (const cplx<float> *)grid, NB, nbc, prune ? &bd : nullptr,
+                          (const cplx<float> *)phase_k, proj + ((long)k * nband + b0) * ptot, ptot);
       else if (st == PP_OK)
-        st = sphere_sums<T>(ctx, sp, grids[s], NB, nbc, phase + (size_t)k * npts_sph,
-                            proj + ((long)k * nband + b0) * ptot, ptot);
+        st = sphere_sums<T>(ctx, sp, grid, NB, nbc, phase_k, proj + ((long)k * nband + b0) * ptot, ptot);
     }
   }
   ctx->stream = streams[0];
   ctx->slot = 0;
   if (two) {
     // the caller's stream sees everything the second stream did
-    (void)hipEventRecord(join, ctx->aux);
-    (void)hipStreamWaitEvent(ctx->stream, join, 0);
-    (void)hipEventDestroy(fork);
-    (void)hipEventDestroy(join);
+    (void)hipEventRecord(ctx->join, ctx->aux);
+    (void)hipStreamWaitEvent(ctx->stream, ctx->join, 0);
   }
   return st;
+}
+
+template <typename T>
+static int setup_batched(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac, const int *gofs, const int *G_dev,
+                         const cplx<float> *C, long ldc, int nband, cplx<double> *proj) {
+  SetupState S;
+  PP_TRY(setup_prepare<T>(ctx, sp, nk, kfrac, gofs, G_dev, nband, S));
+  return setup_launch<T>(ctx, sp, nk, gofs, C, ldc, nband, proj, S);
 }
 
 // projector.c:420-453 for every band and k of one structure: the
@@ -366,6 +402,10 @@ void pp_ctx_destroy(pp_ctx *ctx) {
     (void)hipStreamSynchronize(ctx->aux);
     (void)hipStreamDestroy(ctx->aux);
   }
+  if (ctx->fork)
+    (void)hipEventDestroy(ctx->fork);
+  if (ctx->join)
+    (void)hipEventDestroy(ctx->join);
   if (ctx->own)
     (void)hipStreamDestroy(ctx->own);
   delete ctx;
@@ -447,6 +487,8 @@ int pp_ctx_set_param(pp_ctx *ctx, const char *name, int value) {
     ctx->tune_fft_threads = value >= 512 ? 512 : 256;
   else if (n == "fft_radix")
     ctx->tune_fft_radix = value != 0 ? 1 : 0;
+  else if (n == "fft_xcd")
+    ctx->tune_fft_xcd = value != 0 ? 1 : 0;
   else if (n == "fused")
     ctx->tune_fused = value != 0 ? 1 : 0;
   else if (n == "fft_tile")
@@ -489,6 +531,203 @@ int pp_setup_projections_batched(pp_ctx *ctx, pp_sphere *sp, int nk, const doubl
                                  (cplx<double> *)proj_dev);
   return setup_batched<float>(ctx, sp, nk, kfrac, gofs, G_dev, (const cplx<float> *)C_dev, ldc, nband,
                               (cplx<double> *)proj_dev);
+}
+
+int pp_ctx_get_param(pp_ctx *ctx, const char *name, int *value) {
+  PP_REQUIRE(ctx && name && value, PP_ERR_ARG, "pp_ctx_get_param: null argument");
+  const std::string n(name);
+  const std::pair<const char *, int> table[] = {
+      {"fft_pingpong", ctx->tune_fft_pingpong}, {"gemm_f64_valu", ctx->tune_gemm_f64_valu},
+      {"fft_persistent", ctx->tune_fft_persistent}, {"streams", ctx->tune_streams},
+      {"sphere_unroll", ctx->tune_sphere_unroll}, {"fft_threads", ctx->tune_fft_threads},
+      {"fft_radix", ctx->tune_fft_radix}, {"fft_xcd", ctx->tune_fft_xcd}, {"fused", ctx->tune_fused},
+      {"fft_tile", ctx->tune_fft_tile}, {"fft_prune", ctx->tune_fft_prune}, {"fft_bands", ctx->tune_fft_bands},
+      {"sphere_parts", ctx->tune_sphere_parts}, {"gemm_splitk", ctx->tune_splitk}};
+  for (const auto &e : table)
+    if (n == e.first) {
+      *value = e.second;
+      return PP_OK;
+    }
+  PP_REQUIRE(false, PP_ERR_ARG, "pp_ctx_get_param: unknown parameter " + n);
+  return PP_OK;
+}
+
+// Project-written autotuner of the setup path: coordinate descent over the
+// context's tuning parameters, each candidate timed on the caller's own
+// inputs (HIP events around the whole setup on the context's stream, best
+// of `repeats` runs after one warm-up); the fastest value of each
+// parameter is kept before the next parameter is tried.  The outputs of
+// the last run are valid projections.  report (may be NULL) receives one
+// line per candidate: "name value ms".
+int pp_autotune(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac, const int *gofs, const int *G_dev,
+                const void *C_dev, long ldc, int nband, void *proj_dev, int precision, int repeats, char *report,
+                int report_len) {
+  PP_REQUIRE(ctx && sp && repeats >= 1, PP_ERR_ARG, "pp_autotune: bad arguments");
+  PP_CHECK_HIP(hipSetDevice(ctx->device));
+  const bool fp32 = precision != PP_FP64;
+  struct Cand {
+    const char *name;
+    std::vector<int> values;
+  };
+  std::vector<Cand> cands = {
+      {"streams", {1, 2}},
+      {"fft_tile", fp32 ? std::vector<int>{16, 32} : std::vector<int>{8, 16}},
+      {"fft_threads", {256, 512}},
+      {"sphere_unroll", {4, 8}},
+      {"sphere_parts", {0, 4, 16}},
+      {"fft_radix", {0, 1}},
+      {"fft_xcd", {0, 1}},
+      {"fft_bands", {64, 32}},
+  };
+  if (fp32)
+    cands.push_back({"fused", {0, 1}});
+  const bool timing = ctx->timer.on;
+  ctx->timer.on = false;
+  hipEvent_t e0, e1;
+  PP_CHECK_HIP(hipEventCreate(&e0));
+  PP_CHECK_HIP(hipEventCreate(&e1));
+  std::string rep;
+  auto run_once = [&](float &ms) -> int {
+    PP_CHECK_HIP(hipEventRecord(e0, ctx->stream));
+    PP_TRY(pp_setup_projections_batched(ctx, sp, nk, kfrac, gofs, G_dev, C_dev, ldc, nband, proj_dev, precision));
+    PP_CHECK_HIP(hipEventRecord(e1, ctx->stream));
+    PP_CHECK_HIP(hipEventSynchronize(e1));
+    PP_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+    return PP_OK;
+  };
+  auto trial = [&](float &best) -> int {
+    float ms = 0.f;
+    PP_TRY(run_once(ms));  // warm-up: buffers, plans
+    best = 1e30f;
+    for (int r = 0; r < repeats; r++) {
+      PP_TRY(run_once(ms));
+      best = std::min(best, ms);
+    }
+    return PP_OK;
+  };
+  int st = PP_OK;
+  for (const Cand &c : cands) {
+    int cur = 0;
+    pp_ctx_get_param(ctx, c.name, &cur);
+    int best_v = cur;
+    float best_ms = 1e30f;
+    for (int v : c.values) {
+      if ((st = pp_ctx_set_param(ctx, c.name, v)) != PP_OK)
+        break;
+      float ms = 0.f;
+      if ((st = trial(ms)) != PP_OK)
+        break;
+      rep += std::string(c.name) + " " + std::to_string(v) + " " + std::to_string(ms) + "\n";
+      if (ms < best_ms) {
+        best_ms = ms;
+        best_v = v;
+      }
+    }
+    if (st != PP_OK)
+      break;
+    pp_ctx_set_param(ctx, c.name, best_v);
+  }
+  if (st == PP_OK) {
+    float ms = 0.f;
+    st = run_once(ms);  // projections of the chosen setting
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  ctx->timer.on = timing;
+  if (report && report_len > 0) {
+    const size_t n = std::min<size_t>(rep.size(), (size_t)report_len - 1);
+    memcpy(report, rep.data(), n);
+    report[n] = 0;
+  }
+  return st;
+}
+
+}  // extern "C"
+
+// A captured setup pipeline: the graph, and what it was captured against
+// (the plan's build generation, the context buffers it reads and writes).
+struct pp_graph {
+  pp_ctx *ctx = nullptr;
+  pp_sphere *sp = nullptr;
+  long sp_gen = 0;
+  std::vector<std::pair<void *, size_t>> bufs;
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ge = nullptr;
+};
+
+static std::vector<std::pair<void *, size_t>> ctx_buffers(pp_ctx *ctx) {
+  std::vector<std::pair<void *, size_t>> v;
+  for (const DevBuf *b : {&ctx->grid, &ctx->grid2, &ctx->partial, &ctx->partial2, &ctx->gidx, &ctx->phase,
+                          &ctx->bounds, &ctx->lines})
+    v.push_back({b->p, b->bytes});
+  return v;
+}
+
+template <typename T>
+static int graph_capture(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac, const int *gofs, const int *G_dev,
+                         const cplx<float> *C, long ldc, int nband, cplx<double> *proj, pp_graph *gr) {
+  SetupState S;
+  PP_TRY(setup_prepare<T>(ctx, sp, nk, kfrac, gofs, G_dev, nband, S));
+  PP_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  PP_CHECK_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  int st = setup_launch<T>(ctx, sp, nk, gofs, C, ldc, nband, proj, S);
+  hipError_t e = hipStreamEndCapture(ctx->stream, &gr->g);
+  PP_TRY(st);
+  PP_CHECK_HIP(e);
+  PP_CHECK_HIP(hipGraphInstantiate(&gr->ge, gr->g, nullptr, nullptr, 0));
+  return PP_OK;
+}
+
+extern "C" {
+
+int pp_graph_capture(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac, const int *gofs, const int *G_dev,
+                     const void *C_dev, long ldc, int nband, void *proj_dev, int precision, pp_graph **out) {
+  PP_REQUIRE(out, PP_ERR_ARG, "pp_graph_capture: null output");
+  *out = nullptr;
+  PP_REQUIRE(nk > 0 && nband > 0, PP_ERR_ARG, "pp_graph_capture: nothing to capture");
+  // one eager run validates the arguments, sizes every buffer the graph
+  // uses and leaves valid projections
+  PP_TRY(pp_setup_projections_batched(ctx, sp, nk, kfrac, gofs, G_dev, C_dev, ldc, nband, proj_dev, precision));
+  pp_graph *gr = new pp_graph();
+  gr->ctx = ctx;
+  gr->sp = sp;
+  gr->sp_gen = sphere_generation(sp);
+  const bool timing = ctx->timer.on;
+  ctx->timer.on = false;
+  int st = precision == PP_FP64
+               ? graph_capture<double>(ctx, sp, nk, kfrac, gofs, G_dev, (const cplx<float> *)C_dev, ldc, nband,
+                                       (cplx<double> *)proj_dev, gr)
+               : graph_capture<float>(ctx, sp, nk, kfrac, gofs, G_dev, (const cplx<float> *)C_dev, ldc, nband,
+                                      (cplx<double> *)proj_dev, gr);
+  ctx->timer.on = timing;
+  if (st != PP_OK) {
+    pp_graph_destroy(gr);
+    return st;
+  }
+  gr->bufs = ctx_buffers(ctx);
+  *out = gr;
+  return PP_OK;
+}
+
+int pp_graph_replay(pp_graph *gr) {
+  PP_REQUIRE(gr && gr->ge, PP_ERR_ARG, "pp_graph_replay: null graph");
+  PP_REQUIRE(sphere_generation(gr->sp) == gr->sp_gen, PP_ERR_ARG,
+             "pp_graph_replay: the sphere plan was rebuilt since the capture");
+  PP_REQUIRE(ctx_buffers(gr->ctx) == gr->bufs, PP_ERR_ARG,
+             "pp_graph_replay: context buffers were reallocated since the capture");
+  PP_CHECK_HIP(hipSetDevice(gr->ctx->device));
+  PP_CHECK_HIP(hipGraphLaunch(gr->ge, gr->ctx->stream));
+  return PP_OK;
+}
+
+void pp_graph_destroy(pp_graph *gr) {
+  if (!gr)
+    return;
+  if (gr->ge)
+    (void)hipGraphExecDestroy(gr->ge);
+  if (gr->g)
+    (void)hipGraphDestroy(gr->g);
+  delete gr;
 }
 
 int pp_overlap_gemm(pp_ctx *ctx, int nwk, int nspin, int flip, const int *nw, const void *CR_dev, long ldR, int nbR,

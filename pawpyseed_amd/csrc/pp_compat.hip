@@ -29,7 +29,8 @@ struct WfDev {
   std::vector<int> nw, gofs;
   DevBuf C, G;  // c64 [nkt][nband][ldc]; int32 [sum nw][3]
   bool uploaded = false;
-  DevBuf proj;  // c128 [nkt][nband][ptot]
+  DevBuf proj;     // c128 [nkt][nband][ptot] (spinors: the up component)
+  DevBuf proj_dn;  // spinors: the down component, same layout
   int ptot = 0, nsites = 0, proj_precision = -1;
   std::vector<int> proj_off, labels;
   std::vector<int> sp_nchan, sp_ls;
@@ -392,6 +393,10 @@ pswf_t *pp_pswf_from_arrays(int nspin, int nwk, int nband, double encut, const d
         wf->G_bounds[2 * a + 1] = std::max(wf->G_bounds[2 * a + 1], g[a]);
       }
     }
+    // a G list that repeats itself is the two components of a spinor
+    // (a collinear list holds distinct vectors; reader.c:274-281)
+    if (np > 0 && np % 2 == 0 && !memcmp(Gs + 3 * go, Gs + 3 * (go + np / 2), sizeof(int) * 3 * (size_t)(np / 2)))
+      wf->is_ncl = 1;
     go += np;
     for (int b = 0; b < nband; b++) {
       memcpy(kp->bands[b]->Cs, coeffs + co, sizeof(pp_cfloat) * (size_t)np);
@@ -624,10 +629,13 @@ static void species_arrays(const ppot_t *pps, int num_elems, int smooth, Species
 }
 
 // <table_j | psi~_nk> for every band and k of wf at the given sites (grid
-// and lattice of wf), into out c128 [nkt][nband][ptot].
+// and lattice of wf), into out c128 [nkt][nband][ptot].  For a spinor
+// wavefunction out gets the up component and out_dn (when given) the down
+// one: each half of the 2 W coefficients is projected with the first W G
+// vectors (projector.c:359-392).
 static int project_sites(pp_ctx *ctx, pswf_t *wf, const SpeciesArrays &a, int num_elems, int nsites,
                          const int *labels, const double *coords, int prec, DevBuf &out, int &ptot,
-                         std::vector<int> &proj_off) {
+                         std::vector<int> &proj_off, DevBuf *out_dn = nullptr) {
   WfDev *d = dev_of(wf);
   pp_sphere *sp = nullptr;
   PP_TRY(pp_sphere_create(ctx, &sp, wf->lattice, wf->fftg, nsites, labels, coords, num_elems, a.nchan.data(),
@@ -640,9 +648,24 @@ static int project_sites(pp_ctx *ctx, pswf_t *wf, const SpeciesArrays &a, int nu
   for (int k = 0; k < d->nkt; k++)
     for (int j = 0; j < 3; j++)
       kf[3 * k + j] = wf->kpts[k]->k[j];
-  if (st == PP_OK)
+  if (st == PP_OK && out_dn)
+    st = out_dn->ensure(sizeof(pp_cdouble) * (size_t)d->nkt * d->nband * std::max(1, ptot));
+  if (st == PP_OK && !wf->is_ncl) {
     st = pp_setup_projections_batched(ctx, sp, d->nkt, kf.data(), d->gofs.data(), (const int *)d->G.p, d->C.p,
                                       d->ldc, d->nband, out.p, prec);
+  } else if (st == PP_OK) {
+    for (int comp = 0; comp < (out_dn ? 2 : 1) && st == PP_OK; comp++) {
+      DevBuf &dst = comp ? *out_dn : out;
+      for (int k = 0; k < d->nkt && st == PP_OK; k++) {
+        const int W = d->nw[k] / 2;
+        const int gk[2] = {0, W};
+        const pp_cfloat *Ck = (const pp_cfloat *)d->C.p + (size_t)k * d->nband * d->ldc + (size_t)comp * W;
+        st = pp_setup_projections_batched(ctx, sp, 1, kf.data() + 3 * k, gk, (const int *)d->G.p + 3 * (size_t)d->gofs[k],
+                                          Ck, d->ldc, d->nband,
+                                          (pp_cdouble *)dst.p + (size_t)k * d->nband * std::max(1, ptot), prec);
+      }
+    }
+  }
   if (st == PP_OK)
     st = pp_ctx_synchronize(ctx);
   pp_sphere_destroy(sp);
@@ -650,9 +673,11 @@ static int project_sites(pp_ctx *ctx, pswf_t *wf, const SpeciesArrays &a, int nu
 }
 
 // Host projection_t lists of the reference (per band and k) from a device
-// table [nkt][nband][ptot].
+// table [nkt][nband][ptot], stored as band->projections, wave_projections,
+// up_projections or down_projections.
+enum ListKind { L_PROJ, L_WAVE, L_UP, L_DOWN };
 static int fill_host_lists(pswf_t *wf, const DevBuf &tab, int ptot, int nsites, const int *labels,
-                           const std::vector<int> &proj_off, const ppot_t *pps, bool wave) {
+                           const std::vector<int> &proj_off, const ppot_t *pps, ListKind kind) {
   WfDev *d = dev_of(wf);
   std::vector<pp_cdouble> host((size_t)d->nkt * d->nband * std::max(1, ptot));
   PP_CHECK_HIP(hipMemcpy(host.data(), tab.p, sizeof(pp_cdouble) * host.size(), hipMemcpyDeviceToHost));
@@ -680,10 +705,9 @@ static int fill_host_lists(pswf_t *wf, const DevBuf &tab, int ptot, int nsites, 
           }
         }
       }
-      if (wave)
-        wf->kpts[k]->bands[b]->wave_projections = pl;
-      else
-        wf->kpts[k]->bands[b]->projections = pl;
+      band_t *bd = wf->kpts[k]->bands[b];
+      (kind == L_WAVE ? bd->wave_projections
+                      : kind == L_UP ? bd->up_projections : kind == L_DOWN ? bd->down_projections : bd->projections) = pl;
     }
   return PP_OK;
 }
@@ -702,7 +726,6 @@ int setup_projections(pswf_t *wf, ppot_t *pps, int num_elems, int num_sites, int
                       double *coords) {
   PP_REQUIRE(wf && pps && num_elems > 0 && num_sites >= 0 && fftg && (num_sites == 0 || (labels && coords)),
              PP_ERR_ARG, "setup_projections: bad arguments");
-  PP_REQUIRE(!wf->is_ncl, PP_ERR_UNSUPPORTED, "setup_projections: noncollinear wavefunctions are not supported");
   std::lock_guard<std::mutex> lk(g_mu);
   pp_ctx *ctx;
   PP_TRY(global_ctx(&ctx));
@@ -711,8 +734,11 @@ int setup_projections(pswf_t *wf, ppot_t *pps, int num_elems, int num_sites, int
   // release projections of an earlier setup
   for (int k = 0; k < wf->nwk * wf->nspin; k++)
     for (int b = 0; b < wf->nband; b++) {
-      free_projection_list(wf->kpts[k]->bands[b]->projections, wf->num_sites);
-      wf->kpts[k]->bands[b]->projections = nullptr;
+      band_t *bd = wf->kpts[k]->bands[b];
+      free_projection_list(bd->projections, wf->num_sites);
+      free_projection_list(bd->up_projections, wf->num_sites);
+      free_projection_list(bd->down_projections, wf->num_sites);
+      bd->projections = bd->up_projections = bd->down_projections = nullptr;
     }
   if (wf->pps && wf->pps != pps)
     free_ppot_list(wf->pps, wf->num_elems);
@@ -729,7 +755,8 @@ int setup_projections(pswf_t *wf, ppot_t *pps, int num_elems, int num_sites, int
   int prec = g_precision == PP_FP64 ? PP_FP64 : PP_FP32;
   int ptot = 0;
   std::vector<int> proj_off;
-  PP_TRY(project_sites(ctx, wf, a, num_elems, num_sites, labels, coords, prec, d->proj, ptot, proj_off));
+  PP_TRY(project_sites(ctx, wf, a, num_elems, num_sites, labels, coords, prec, d->proj, ptot, proj_off,
+                       wf->is_ncl ? &d->proj_dn : nullptr));
   d->ptot = ptot;
   d->nsites = num_sites;
   d->proj_off = proj_off;
@@ -739,7 +766,12 @@ int setup_projections(pswf_t *wf, ppot_t *pps, int num_elems, int num_sites, int
   d->sp_dO = a.dO;
   d->proj_precision = prec;
   d->aug_cache.clear();
-  return fill_host_lists(wf, d->proj, ptot, num_sites, labels, proj_off, pps, false);
+  if (wf->is_ncl) {
+    // onto_projector_ncl (projector.c:359-392): up and down lists
+    PP_TRY(fill_host_lists(wf, d->proj, ptot, num_sites, labels, proj_off, pps, L_UP));
+    return fill_host_lists(wf, d->proj_dn, ptot, num_sites, labels, proj_off, pps, L_DOWN);
+  }
+  return fill_host_lists(wf, d->proj, ptot, num_sites, labels, proj_off, pps, L_PROJ);
 }
 
 // projector.c:674-712 (and :797-840): off-site overlaps of the N_RS pairs
@@ -830,6 +862,8 @@ int overlap_setup_real(pswf_t *wf_R, pswf_t *wf_S, int *labels_R, int *labels_S,
              "overlap_setup_real: bad arguments");
   PP_REQUIRE(num_N_RS == 0 || (N_RS_R && N_RS_S && labels_R && labels_S && coords_R && coords_S), PP_ERR_ARG,
              "overlap_setup_real: null N_RS data");
+  PP_REQUIRE(!wf_R->is_ncl && !wf_S->is_ncl, PP_ERR_UNSUPPORTED,
+             "overlap_setup_real: noncollinear wavefunctions cannot be projected (projector.py:74-75)");
   PP_REQUIRE(wf_R->pps && wf_S->pps && wf_R->fftg && wf_S->fftg, PP_ERR_ARG,
              "overlap_setup_real: call setup_projections on both wavefunctions first");
   PP_REQUIRE(num_N_R == 0 || (N_R && labels_R && coords_R), PP_ERR_ARG, "overlap_setup_real: null N_R data");
@@ -896,7 +930,7 @@ int overlap_setup_real(pswf_t *wf_R, pswf_t *wf_S, int *labels_R, int *labels_S,
     // the smooth tables live on the projected structure's grid
     PP_TRY(project_sites(ctx, sd.wf, a, sd.owner->num_elems, sd.n, lab.data(), crd.data(), prec, d->wp, d->wp_ptot,
                          d->wp_off));
-    PP_TRY(fill_host_lists(sd.wf, d->wp, d->wp_ptot, sd.n, lab.data(), d->wp_off, sd.owner->pps, true));
+    PP_TRY(fill_host_lists(sd.wf, d->wp, d->wp_ptot, sd.n, lab.data(), d->wp_off, sd.owner->pps, L_WAVE));
   }
   PP_TRY(setup_offsite(wf_R, wf_S, labels_R, labels_S, coords_R, coords_S, N_RS_R, N_RS_S, num_N_RS));
   return PP_OK;
@@ -910,6 +944,9 @@ int pp_get_projections(pswf_t *wf, int band, int kidx, pp_cdouble *out) {
              "pp_get_projections: index out of range");
   PP_CHECK_HIP(hipMemcpy(out, (pp_cdouble *)d->proj.p + ((size_t)kidx * d->nband + band) * d->ptot,
                          sizeof(pp_cdouble) * d->ptot, hipMemcpyDeviceToHost));
+  if (wf->is_ncl)  // spinors: the down component follows the up one
+    PP_CHECK_HIP(hipMemcpy(out + d->ptot, (pp_cdouble *)d->proj_dn.p + ((size_t)kidx * d->nband + band) * d->ptot,
+                           sizeof(pp_cdouble) * d->ptot, hipMemcpyDeviceToHost));
   return PP_OK;
 }
 
@@ -956,6 +993,9 @@ int compensation_terms(pp_cdouble *overlap, int BAND_NUM, pswf_t *wf_S, pswf_t *
                        int num_N_S, int num_N_RS, int *M_R, int *M_S, int *N_R, int *N_S, int *N_RS_R, int *N_RS_S,
                        int *proj_labels, double *proj_coords, int *ref_labels, double *ref_coords, int *fft_grid,
                        int flip_spin) {
+  PP_REQUIRE(wf_R && wf_S, PP_ERR_ARG, "compensation_terms: null wavefunction");
+  PP_REQUIRE(!wf_R->is_ncl && !wf_S->is_ncl, PP_ERR_UNSUPPORTED,
+             "compensation_terms: noncollinear wavefunctions cannot be projected (projector.py:74-75)");
   (void)proj_labels;
   (void)proj_coords;
   (void)ref_labels;
@@ -1085,6 +1125,8 @@ int overlap_setup_recip(pswf_t *wf_R, pswf_t *wf_S, int *labels_R, int *labels_S
                         int num_N_RS) {
   PP_REQUIRE(wf_R && wf_S && num_N_R >= 0 && num_N_S >= 0 && num_N_RS >= 0, PP_ERR_ARG,
              "overlap_setup_recip: bad arguments");
+  PP_REQUIRE(!wf_R->is_ncl && !wf_S->is_ncl, PP_ERR_UNSUPPORTED,
+             "overlap_setup_recip: noncollinear wavefunctions cannot be projected (projector.py:74-75)");
   PP_REQUIRE(wf_R->pps && wf_S->pps && wf_R->fftg && wf_S->fftg && wf_R->dev && wf_S->dev, PP_ERR_ARG,
              "overlap_setup_recip: call setup_projections on both wavefunctions first");
   PP_REQUIRE(num_N_R == 0 || (N_R && labels_R && coords_R), PP_ERR_ARG, "overlap_setup_recip: null N_R data");
@@ -1178,6 +1220,9 @@ int compensation_terms_recip(pp_cdouble *overlap, int BAND_NUM, pswf_t *wf_S, ps
                              int num_N_S, int num_N_RS, int *M_R, int *M_S, int *N_R, int *N_S, int *N_RS_R,
                              int *N_RS_S, int *proj_labels, double *proj_coords, int *ref_labels, double *ref_coords,
                              int *fft_grid, int flip_spin) {
+  PP_REQUIRE(wf_R && wf_S, PP_ERR_ARG, "compensation_terms_recip: null wavefunction");
+  PP_REQUIRE(!wf_R->is_ncl && !wf_S->is_ncl, PP_ERR_UNSUPPORTED,
+             "compensation_terms_recip: noncollinear wavefunctions cannot be projected (projector.py:74-75)");
   (void)proj_labels;
   (void)proj_coords;
   (void)ref_labels;

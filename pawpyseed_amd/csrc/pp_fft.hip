@@ -318,14 +318,20 @@ __global__ __launch_bounds__(256) void k_fft_pass(cplx<T> *__restrict__ g, AxisP
 
 // As k_fft_pass with in-place stages (radices with a register-level DFT),
 // at most MAXB butterflies per thread per stage, NT threads per work-group.
+//
+// xcd_per > 0: work-group w runs tile (w % 8) * xcd_per + w / 8, so each
+// of the 8 XCDs (which receive work-groups round-robin) walks one
+// contiguous range of lines; tiles past nblk exit.
 template <typename T, int NBW, int AXIS, int MAXB, int NT, bool COMP = false>
 __global__ __launch_bounds__(NT) void k_fft_pass_ip(cplx<T> *__restrict__ g, AxisPlan pl,
                                                      const cplx<T> *__restrict__ tw, long es, long s1, long s2,
-                                                     int n2, int nslices, SphereBounds bd) {
+                                                     int n2, int nslices, SphereBounds bd, int xcd_per, int nblk) {
   extern __shared__ unsigned char smem_raw[];
   cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
   const int n = pl.n;
-  const int blk = blockIdx.x;
+  const int blk = xcd_per > 0 ? (int)(blockIdx.x % 8) * xcd_per + (int)(blockIdx.x / 8) : (int)blockIdx.x;
+  if (blk >= nblk)
+    return;
   const int slice = blk % nslices;
   const int line = pass_line<AXIS>(bd, blk / nslices);
   const int l1 = line / n2, l2 = line % n2;
@@ -477,28 +483,54 @@ __global__ __launch_bounds__(256) void k_bounds_init(int *__restrict__ bnd, int 
 __global__ __launch_bounds__(256) void k_bounds(int n0, int n1, int n2, int nw, const int *__restrict__ G,
                                                 int *__restrict__ bnd) {
   const int w = blockIdx.x * 256 + threadIdx.x;
-  int *zb = bnd, *yb = bnd + 2 * n0 * n1, *xb = yb + 2 * n0, *flag = xb + 2;
-  int lo = 1 << 30, hi = -(1 << 30);
-  if (w < nw) {
-    const int g0 = G[3 * w], g1 = G[3 * w + 1], g2 = G[3 * w + 2];
-    const int x = ((g0 % n0) + n0) % n0, y = ((g1 % n1) + n1) % n1, z = ((g2 % n2) + n2) % n2;
-    const int sx = sgn_idx(x, n0), sy = sgn_idx(y, n1), sz = sgn_idx(z, n2);
-    if (sx != g0 || sy != g1 || sz != g2)
-      atomicOr(flag, 1);  // the grid aliases this G: no pruning
-    atomicMin(&zb[2 * (x * n1 + y)], sz);
-    atomicMax(&zb[2 * (x * n1 + y) + 1], sz);
-    atomicMin(&yb[2 * x], sy);
-    atomicMax(&yb[2 * x + 1], sy);
-    lo = hi = sx;
+  if (w >= nw)
+    return;
+  int *zb = bnd, *flag = bnd + 2 * n0 * n1 + 2 * n0 + 2;
+  const int g0 = G[3 * w], g1 = G[3 * w + 1], g2 = G[3 * w + 2];
+  const int x = ((g0 % n0) + n0) % n0, y = ((g1 % n1) + n1) % n1, z = ((g2 % n2) + n2) % n2;
+  const int sx = sgn_idx(x, n0), sy = sgn_idx(y, n1), sz = sgn_idx(z, n2);
+  if (sx != g0 || sy != g1 || sz != g2)
+    atomicOr(flag, 1);  // the grid aliases this G: no pruning
+  atomicMin(&zb[2 * (x * n1 + y)], sz);
+  atomicMax(&zb[2 * (x * n1 + y) + 1], sz);
+}
+
+// y extent of every plane x and the x extent, from the z-line bounds (one
+// work-group; each thread scans the lines of its planes).
+__global__ __launch_bounds__(256) void k_bounds_rows(int n0, int n1, int *__restrict__ bnd) {
+  const int *zb = bnd;
+  int *yb = bnd + 2 * n0 * n1, *xb = yb + 2 * n0;
+  int xlo = 1 << 30, xhi = -(1 << 30);
+  for (int x = threadIdx.x; x < n0; x += 256) {
+    int lo = 1 << 30, hi = -(1 << 30);
+    for (int y = 0; y < n1; y++)
+      if (zb[2 * (x * n1 + y)] <= zb[2 * (x * n1 + y) + 1]) {
+        const int sy = sgn_idx(y, n1);
+        lo = min(lo, sy);
+        hi = max(hi, sy);
+      }
+    yb[2 * x] = lo;
+    yb[2 * x + 1] = hi;
+    if (lo <= hi) {
+      const int sx = sgn_idx(x, n0);
+      xlo = min(xlo, sx);
+      xhi = max(xhi, sx);
+    }
   }
-  // the x extent: one atomic pair per wavefront after a shuffle reduction
-  for (int o = 32; o > 0; o >>= 1) {
-    lo = min(lo, __shfl_xor(lo, o));
-    hi = max(hi, __shfl_xor(hi, o));
+  __shared__ int rlo[256], rhi[256];
+  rlo[threadIdx.x] = xlo;
+  rhi[threadIdx.x] = xhi;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      rlo[threadIdx.x] = min(rlo[threadIdx.x], rlo[threadIdx.x + o]);
+      rhi[threadIdx.x] = max(rhi[threadIdx.x], rhi[threadIdx.x + o]);
+    }
+    __syncthreads();
   }
-  if ((threadIdx.x & 63) == 0 && lo <= hi) {
-    atomicMin(&xb[0], lo);
-    atomicMax(&xb[1], hi);
+  if (threadIdx.x == 0) {
+    xb[0] = rlo[0];
+    xb[1] = rhi[0];
   }
 }
 
@@ -568,6 +600,9 @@ static int launch_pass(pp_ctx *ctx, cplx<T> *grid, const AxisPlan &ap, const cpl
   for (int s = 0; comp && s < ap.nstc; s++)
     if (!dft_supported(ap.radixc[s]) || ((ap.n / ap.radixc[s]) * NBW + 255) / 256 > 2)
       comp = false;
+  // XCD-contiguous line ranges (fft_xcd = 1)
+  const int xcd_per = ctx->tune_fft_xcd ? (int)((nblk + 7) / 8) : 0;
+  const long nlaunch = xcd_per ? 8L * xcd_per : nblk;
   hipEvent_t ev;
   tbegin(ctx, cls, &ev);
   bool launched = false;
@@ -583,8 +618,8 @@ static int launch_pass(pp_ctx *ctx, cplx<T> *grid, const AxisPlan &ap, const cpl
         apc.twoff[s] = ap.twoffc[s];
       }
       size_t lds = ((size_t)ap.n * NBW + ap.twlen) * sizeof(cplx<T>);
-      hipLaunchKernelGGL((k_fft_pass_ip<T, NBW, AXIS, 2, 256, true>), dim3((unsigned)nblk), dim3(256), lds,
-                         ctx->stream, grid, apc, tw, es, s1, s2, n2, nslices, bd);
+      hipLaunchKernelGGL((k_fft_pass_ip<T, NBW, AXIS, 2, 256, true>), dim3((unsigned)nlaunch), dim3(256), lds,
+                         ctx->stream, grid, apc, tw, es, s1, s2, n2, nslices, bd, xcd_per, (int)nblk);
       launched = true;
     }
   }
@@ -610,8 +645,8 @@ static int launch_pass(pp_ctx *ctx, cplx<T> *grid, const AxisPlan &ap, const cpl
     if (!launched && ip) {
       size_t lds = ((size_t)ap.n * NBW + ap.twlen) * sizeof(cplx<T>);
 #define PP_IP(MB, NT)                                                                                          \
-  hipLaunchKernelGGL((k_fft_pass_ip<T, NBW, AXIS, MB, NT>), dim3((unsigned)nblk), dim3(NT), lds, ctx->stream, grid,  \
-                     ap, tw, es, s1, s2, n2, nslices, bd)
+  hipLaunchKernelGGL((k_fft_pass_ip<T, NBW, AXIS, MB, NT>), dim3((unsigned)nlaunch), dim3(NT), lds, ctx->stream,    \
+                     grid, ap, tw, es, s1, s2, n2, nslices, bd, xcd_per, (int)nblk)
       bool done = false;
       if constexpr (NBW >= 32) {
         if (nt == 512) {
@@ -714,6 +749,7 @@ int sphere_bounds(pp_ctx *ctx, const int *fftg, int nw, const int *G_dev, int *b
   if (nw > 0)
     hipLaunchKernelGGL(k_bounds, dim3((nw + 255) / 256), dim3(256), 0, ctx->stream, fftg[0], fftg[1], fftg[2], nw,
                        G_dev, bnd_dev);
+  hipLaunchKernelGGL(k_bounds_rows, dim3(1), dim3(256), 0, ctx->stream, fftg[0], fftg[1], bnd_dev);
   PP_CHECK_HIP(hipGetLastError());
   return PP_OK;
 }

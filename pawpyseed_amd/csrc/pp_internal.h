@@ -94,8 +94,10 @@ struct pp_ctx {
   int tune_sphere_unroll = 8;   // sphere-sum gathers in flight per wavefront (4 or 8)
   int tune_fft_threads = 256;   // work-group size of the in-place FFT passes with 32-band tiles (256 or 512)
   int tune_fft_radix = 0;       // 0: radices 2-5, 1: composite radices where they fit
+  int tune_fft_xcd = 0;         // 1: FFT pass work-groups in XCD-contiguous line ranges
   int tune_fused = 0;           // 1: x pass fused with the sphere sums where the plan allows
   hipStream_t aux = nullptr;    // second stream of the chunk pipeline (default priority)
+  hipEvent_t fork = nullptr, join = nullptr;  // hand-over between the two streams
   int slot = 0;                 // chunk buffer set in use (0 or 1)
   pp::DevBuf grid, grid2;       // band-fastest FFT chunk, one per slot
   pp::DevBuf partial, partial2; // sphere-sum partials, one per slot

@@ -62,6 +62,7 @@ struct pp_sphere {
   std::vector<int> site_pt_h;
   int ptot = 0, pmax = 0, pmin = 0;
   long npts = 0;
+  long gen = 0;  // build count: graphs captured against older tables refuse to replay
   // device
   pp::DevBuf d_spline;  // per species: x, f, b, c, d [nchan][ngrid]
   pp::DevBuf d_sitemeta;

@@ -611,6 +611,7 @@ void site_box(const double *lattice, const int *fftg, double rmax, int ngrid, Si
 
 static int sphere_build(pp_sphere *sp) {
   pp_ctx *ctx = sp->ctx;
+  sp->gen++;
   const int ns = sp->nsites;
   // per-site geometry
   std::vector<SiteGeo> geo(ns);
@@ -893,6 +894,7 @@ template int sphere_expand<double>(pp_ctx *, pp_sphere *, cplx<double> *, int, i
 template int sphere_expand<float>(pp_ctx *, pp_sphere *, cplx<float> *, int, int, const cplx<float> *,
                                   const cplx<double> *, long, const int *);
 int sphere_nsites(pp_sphere *sp) { return sp->nsites; }
+long sphere_generation(pp_sphere *sp) { return sp->gen; }
 
 long sphere_npts(pp_sphere *sp) { return sp->npts; }
 int sphere_ptot(pp_sphere *sp) { return sp->ptot; }

@@ -109,6 +109,46 @@ def test_reader_on_reference_wavecar(wavecar):
     L.free_pswf(p)
 
 
+def test_reader_spinor_wavecar(tmp_path):
+    """A WAVECAR whose plane-wave count is twice the G vectors inside the
+    cutoff is a spinor one (reader.c:274-281): is_ncl set, the G list kept
+    twice."""
+    wl = synthetic.tiny_config(nband=2, kpts=[[0, 0, 0], [0.25, 0.25, 0.0]])
+    Gs = wl.ensure_gvecs()
+    Cu, Cd = wl.coefficients("S"), wl.coefficients("R")
+    C2 = [np.concatenate([Cu[k], Cd[k]], axis=1) for k in range(2)]
+    p = str(tmp_path / "WAVECAR")
+    write_wavecar(p, wl.lattice, wl.encut, wl.kpts, Gs, C2)
+    from pawpyseed_amd.core.wavefunction import PWFPointer
+    pw = PWFPointer.from_wavecar(p, wl.kpts, wl.kws)
+    L = lib()
+    assert L.is_ncl(pw.ptr) == 1 and L.get_nband(pw.ptr) == 2
+    L.free_pswf(pw.ptr)
+    # the same arrays given directly: a G list repeated is a spinor one
+    pw = PWFPointer.from_arrays(wl.lattice, wl.kpts, wl.kws, [np.concatenate([g, g]) for g in Gs], C2, wl.encut)
+    assert L.is_ncl(pw.ptr) == 1
+    L.free_pswf(pw.ptr)
+    pw = PWFPointer.from_arrays(wl.lattice, wl.kpts, wl.kws, Gs, Cu, wl.encut)
+    assert L.is_ncl(pw.ptr) == 0
+    L.free_pswf(pw.ptr)
+
+
+def test_reader_on_reference_ncl_wavecar():
+    """The reference's noncollinear fixture (test_files/noncollinear:
+    4 k-points, 24 spinor bands, 520 eV)."""
+    path = "/root/reference/test_files/noncollinear/WAVECAR"
+    if not os.path.exists(path):
+        pytest.skip("reference fixture not present on this machine")
+    w = np.full(4, 0.25)
+    L = lib()
+    p = L.read_wavefunctions(path.encode(), ptr(w, c_double_p))
+    assert p
+    assert L.is_ncl(p) == 1
+    assert (L.get_nband(p), L.get_nwk(p), L.get_nspin(p)) == (24, 4, 1)
+    assert L.get_encut(p) == 520.0
+    L.free_pswf(p)
+
+
 def test_reader_errors():
     L = lib()
     w = np.array([1.0])

@@ -300,3 +300,39 @@ def test_projector_aug_recip(gpu):
         assert err < 1e-9
     # the reference keeps float complex CAs on every band
     assert basis.nband == 5
+
+
+def test_noncollinear_setup_projections(gpu):
+    """Spinor bands (onto_projector_ncl, projector.c:359-392): each
+    component is projected on its own, into up_projections and
+    down_projections; the band overlap of two spinor wavefunctions is the
+    dot over both components (pseudoprojector.c:85 over num_waves = 2 W).
+    Projection between structures refuses spinors (projector.py:74-75)."""
+    from pawpyseed_amd.core import (CoreRegion, NCLWavefunction, PAWpyError, Projector, Pseudopotential, Structure,
+                                    Wavefunction)
+    wl = synthetic.tiny_config(encut=150.0, fftg=(24, 24, 24), nband=6, kpts=[[0, 0, 0], [0.25, -0.25, 0.0]],
+                               ls=(0, 1, 2), alphas=(1.5, 3.0, 1.5))
+    Gs = wl.ensure_gvecs()
+    Cu, Cd = wl.coefficients("S"), wl.coefficients("R")
+    C2 = [np.concatenate([Cu[k], Cd[k]], axis=1) for k in range(len(Gs))]
+    struct = Structure(wl.lattice, ["C"] * len(wl.frac_S), wl.frac_S)
+    cr = CoreRegion({"C": Pseudopotential.from_species(wl.species)})
+    wf = NCLWavefunction.from_arrays(struct, cr, wl.fftg, wl.kpts, wl.kws, Gs, C2, wl.encut, setup_projectors=True)
+    PU, _ = oracle_projections(wl, wl.frac_S, Cu)
+    PD, _ = oracle_projections(wl, wl.frac_S, Cd)
+    for k in range(len(Gs)):
+        for b in (0, 5):
+            got = wf.get_projections(b, k)
+            assert got.shape == (2, PU.shape[-1])
+            eu, ed = normwise(got[0], PU[k, b]), normwise(got[1], PD[k, b])
+            print("spinor projections k", k, "band", b, "up", eu, "down", ed, flush=True)
+            assert eu < 1e-9 and ed < 1e-9
+    C2b = [np.concatenate([Cd[k], Cu[k]], axis=1) for k in range(len(Gs))]
+    wf2 = NCLWavefunction.from_arrays(struct, cr, wl.fftg, wl.kpts, wl.kws, Gs, C2b, wl.encut)
+    got = wf.pseudoprojection(3, wf2)
+    want = orc.pseudoprojection(C2b, C2, 3, len(Gs), 1)
+    assert normwise(got, want) < 1e-12
+    with pytest.raises(PAWpyError):
+        Projector(wf, wf2)
+    with pytest.raises(PAWpyError):
+        Wavefunction(struct, wf._pwf, cr, wl.fftg)
