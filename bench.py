@@ -103,6 +103,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle checks")
     ap.add_argument("--paths", default="fp32,fp64", help="comma list of fp32, fp64 setup/overlap paths")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner (library defaults for the setup path)")
     ap.add_argument("--no-fused", action="store_true",
                     help="run the x pass and the sphere sums as separate kernels (comparison)")
     ap.add_argument("--profile-step", action="store_true",
@@ -181,6 +183,14 @@ def main():
 
     results = {}
     for p in paths:
+        tuned, trials = None, []
+        if not args.no_tune:
+            # the project's autotuner on this workload's own inputs (untimed)
+            pR, pS = plans[p]
+            with torch.cuda.stream(stream):
+                tuned, trials = B.autotune(ctx, pS, wl.kpts, gofs, Gdev, CS, projS[p], p, repeats=2)
+            torch.cuda.synchronize()
+            log(p, "autotune", json.dumps(tuned))
         for _ in range(args.warmup):
             evs = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
             step(p, evs)
@@ -209,13 +219,28 @@ def main():
         torch.cuda.synchronize()
         kt = ctx.timing()
         ctx.set_timing(False)
-        ctx.set_param("streams", 2)
+        ctx.set_param("streams", tuned["streams"] if tuned else 2)
         ts_serial = evs[0].elapsed_time(evs[1]) / 1e3
+        # the same setup (tables as built by the last step) replayed as one
+        # HIP graph
+        pR, pS = plans[p]
+        with torch.cuda.stream(stream):
+            graph = B.SetupGraph(ctx, pS, wl.kpts, gofs, Gdev, CS, projS[p], p)
+            g0, g1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            g0.record(stream)
+            for _ in range(args.steps):
+                graph.replay(sync=False)
+            g1.record(stream)
+        torch.cuda.synchronize()
+        ts_graph = g0.elapsed_time(g1) / 1e3 / args.steps
+        graph.close()
         nproj = nb * nk * ptotS
         ovl_flops = 8.0 * nb * nb * float(nw.sum())
         results[p] = dict(setup_s=ts, overlap_s=to, aug_s=ta, step_wall_s=wall,
                           band_projections_per_s=nproj / ts, overlap_tflops=ovl_flops / to / 1e12,
                           setup_s_one_chunk_in_flight=ts_serial,
+                          setup_s_graph_replay=ts_graph, band_projections_per_s_graph_replay=nproj / ts_graph,
+                          autotune=tuned, autotune_trials=[list(t) for t in trials],
                           kernels={k: {"ms_total": v[0], "launches": v[1]} for k, v in kt.items()})
         log(p, json.dumps({k: v for k, v in results[p].items() if k != "kernels"}))
         log(p, "kernels (breakdown step)", json.dumps(results[p]["kernels"]))
