@@ -103,6 +103,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle checks")
     ap.add_argument("--paths", default="fp32,fp64", help="comma list of fp32, fp64 setup/overlap paths")
+    ap.add_argument("--workload", default="216atom", choices=["216atom", "spinor512"],
+                    help="216atom: the headline workload; spinor512: the 512-atom noncollinear case (separate line)")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (library defaults for the setup path)")
     ap.add_argument("--no-fused", action="store_true",
@@ -121,6 +123,9 @@ def main():
     from pawpyseed_amd.core import batched as B
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.workload == "spinor512":
+        bench_spinor512(args, torch, B)
+        return
     wl = synthetic.config1(nband=args.nband)
     Gs = wl.ensure_gvecs()
     nk = len(Gs)
@@ -361,6 +366,78 @@ def main():
 
     if not args.no_cpu:
         out.update(cpu_and_errors(args, torch, wl, Gs, CR, CS, projR, projS, Om, ob, plans, results, head))
+    print(json.dumps(out), flush=True)
+
+
+def bench_spinor512(args, torch, B):
+    """The large noncollinear case: 512 sites, a = 21.72 A, 192^3 grid, 2
+    k-points, W = 275,953 per spinor component, 2048 spinor bands, 18
+    projectors per site.  One step projects both components of every band
+    (B.setup_projections_spinor); errors on sampled bands against the FP64
+    oracle.  Prints its own JSON line."""
+    from oracle import pyoracle as orc
+    from pawpyseed_amd import synthetic
+    nb = 2048 if args.nband == 512 else args.nband
+    wl = synthetic.config4(nband=nb)
+    Gs = wl.ensure_gvecs()
+    nk = len(Gs)
+    W = [len(g) for g in Gs]
+    ldc2 = -(-2 * max(W) // 64) * 64
+    C2 = make_coefficients(torch, wl, Gs, 3003, nb, ldc2)  # up components in [0, W)
+    dn = make_coefficients(torch, wl, Gs, 4004, nb, max(W))
+    for k in range(nk):
+        C2[k, :, W[k]:2 * W[k]] = dn[k, :, :W[k]]
+    del dn
+    gofs = np.concatenate([[0], np.cumsum(W)]).astype(np.int32)
+    Gdev = torch.from_numpy(np.ascontiguousarray(np.concatenate(Gs), np.int32)).cuda()
+    torch.cuda.synchronize()
+    stream = torch.cuda.Stream()
+    ctx = B.Context(0, stream=stream)
+    labels = np.zeros(len(wl.frac_S), np.int32)
+    log(f"spinor workload: {len(wl.frac_S)} sites, W={W} per component, grid={wl.fftg.tolist()}, nk={nk}, "
+        f"nband={nb}")
+    res, outs = {}, {}
+    for p in [q for q in args.paths.split(",") if q]:
+        plan = B.SpherePlan(ctx, wl.lattice, wl.fftg, wl.frac_S, labels, [wl.species], p)
+        up = torch.zeros((nk, nb, plan.ptot), dtype=torch.complex128, device="cuda")
+        down = torch.zeros_like(up)
+        with torch.cuda.stream(stream):
+            for _ in range(args.warmup):
+                B.setup_projections_spinor(ctx, plan, wl.kpts, gofs, Gdev, C2, up, down, p)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(args.steps):
+                plan.rebuild(wl.frac_S)
+                B.setup_projections_spinor(ctx, plan, wl.kpts, gofs, Gdev, C2, up, down, p, sync=False)
+            e1.record(stream)
+        torch.cuda.synchronize()
+        t = e0.elapsed_time(e1) / 1e3 / args.steps
+        nproj = 2 * nb * nk * plan.ptot
+        res[p] = {"setup_s": t, "band_projections_per_s": nproj / t}
+        sample = [0, nb - 1]
+        outs[p] = (up[:, sample].cpu().numpy(), down[:, sample].cpu().numpy())
+        log(p, json.dumps(res[p]))
+        plan.close()
+        del up, down
+    errs = {}
+    if not args.no_cpu:
+        sample = [0, nb - 1]
+        Cs = C2[:, sample].cpu().numpy()
+        T = orc.site_tables(wl.lattice, wl.fftg, wl.frac_S, [wl.species], labels)
+        want_up = orc.setup_projections(wl.lattice, wl.fftg, wl.kpts, Gs, [Cs[k, :, :W[k]] for k in range(nk)], T)
+        want_dn = orc.setup_projections(wl.lattice, wl.fftg, wl.kpts, Gs,
+                                        [Cs[k, :, W[k]:2 * W[k]] for k in range(nk)], T)
+        for p, (gu, gd) in outs.items():
+            errs[f"projections_{p}"] = max(float(np.abs(gu - want_up).max() / np.abs(want_up).max()),
+                                           float(np.abs(gd - want_dn).max() / np.abs(want_dn).max()))
+        log("errors (sampled bands", sample, "both k and components):", json.dumps(errs))
+    head = "fp32" if "fp32" in res else next(iter(res))
+    out = {"metric": "band_projections_per_sec", "value": res[head]["band_projections_per_s"],
+           "unit": "band-projections/s (each spinor component counted)", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "higher_is_better": True, "dtype": head, "data": "synthetic (seeded)",
+           "config": {"workload": "512-atom diamond cell, a=21.72 A, 520 eV (W=275953 per spinor component), 192^3 "
+                                  "grid, 2 k, 2048 spinor bands, 18 projectors/site", "nband": nb, "nk": nk},
+           "paths": res, "max_err_vs_fp64_oracle": errs}
     print(json.dumps(out), flush=True)
 
 

@@ -172,6 +172,34 @@ def setup_projections(ctx: Context, plan: SpherePlan, kfrac, gofs, G_dev, C_dev,
     _done(ctx, sync)
 
 
+def setup_projections_spinor(ctx: Context, plan: SpherePlan, kfrac, gofs, G_dev, C2_dev, proj_up, proj_dn,
+                             precision="fp64", sync=True):
+    """Spinor bands (onto_projector_ncl, projector.c:359-392): C2_dev
+    complex64 [nk, nband, ldc] holds the up component of band (k, b) in
+    [0, W_k) of its row and the down one in [W_k, 2 W_k); both go through
+    the batched setup as band sets with row stride ldc, into proj_up /
+    proj_dn complex128 [nk, nband, ptot]."""
+    nk, nband, ldc = C2_dev.shape
+    kf = as_f64(kfrac).reshape(-1, 3)
+    go = as_i32(gofs)
+    assert proj_up.shape == proj_dn.shape == (nk, nband, plan.ptot)
+    assert str(C2_dev.dtype) == "torch.complex64" and str(proj_up.dtype) == str(proj_dn.dtype) == "torch.complex128"
+    for k in range(nk):
+        W = int(go[k + 1] - go[k])
+        assert 2 * W <= ldc
+        gk = np.array([0, W], np.int32)
+        kk = np.ascontiguousarray(kf[k])
+        for comp, out in ((0, proj_up), (1, proj_dn)):
+            cptr = C2_dev.data_ptr() + ((k * nband) * ldc + comp * W) * 8
+            gptr = G_dev.data_ptr() + 3 * int(go[k]) * 4
+            pptr = out.data_ptr() + k * nband * plan.ptot * 16
+            check(lib().pp_setup_projections_batched(ctx.h, plan.h, 1, ptr(kk, c_double_p), ptr(gk, c_int_p),
+                                                     ct.c_void_p(gptr), ct.c_void_p(cptr), ldc, nband,
+                                                     ct.c_void_p(pptr), _prec(precision)),
+                  "pp_setup_projections_batched")
+    _done(ctx, sync)
+
+
 class SetupGraph:
     """pp_graph: the setup path of fixed buffers captured once, replayed on
     the context's stream (recomputes proj_out from the current C_dev)."""

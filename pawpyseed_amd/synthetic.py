@@ -266,6 +266,23 @@ def config1(nband=512):
                     kpts, np.full(len(kpts), 1.0 / len(kpts)), nband, dO, M_R, M_S)
 
 
+def config4(nband=2048):
+    """Large noncollinear case: 512 diamond sites, a=21.72 A, 520 eV
+    (W ~ 275.9k per spinor component), 2 k-points (2x1x1 MP), 192^3 grid,
+    spinor bands of two W_k-coefficient components, 18 projectors per site
+    (l=0,1,2 x alpha 1.5/3.0; rmax 1.6 A).  S: seeded 0.05 A jitter of R."""
+    st = streams()
+    lattice, frac = diamond_cell(4)
+    cart = frac @ lattice + st[0].normal(0.0, 0.05, frac.shape)
+    frac_S = np.mod(cart @ np.linalg.inv(lattice), 1.0)
+    sp = make_species([0, 0, 1, 1, 2, 2], [1.5, 3.0, 1.5, 3.0, 1.5, 3.0], 1.6)
+    dO = channel_delta_o(st[3], sp.ls)
+    idx = np.arange(len(frac), dtype=np.int32)
+    kpts = monkhorst_pack(2, 1, 1)
+    return Workload("cfg4", lattice, frac, frac_S, sp, np.array([192, 192, 192], np.int32), 520.0,
+                    kpts, np.full(len(kpts), 1.0 / len(kpts)), nband, dO, idx, idx.copy())
+
+
 def tiny_config(ncell=1, encut=150.0, fftg=(20, 20, 20), nband=8, kpts=None, ls=(0, 1), alphas=(1.5, 3.0),
                 rmax=1.1, jitter=0.03):
     """Small cell for tests that the pure-Python checks finish in seconds."""
