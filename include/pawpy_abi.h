@@ -218,7 +218,10 @@ int pp_ctx_set_tuning(pp_ctx *ctx, int fft_bands, int fft_tile, int fft_prune,
  * round trips but more registers), "fused" (0, default: separate x-pass and
  * sphere-sum kernels; 1: the FP32 path runs them as one kernel where the
  * plan allows -- it writes no real-space grid but measured slower on the
- * 216-atom workload). */
+ * 216-atom workload), "fft_xcd" (1: FFT pass work-groups walk
+ * XCD-contiguous line ranges), "fft_nt" (FFT passes: 0 cached, 1
+ * non-temporal stores, 2 non-temporal loads and stores, default 2),
+ * "sphere_nt" (1, default: sphere-sum gathers with non-temporal loads). */
 int pp_ctx_set_param(pp_ctx *ctx, const char *name, int value);
 /* Current value of a parameter of pp_ctx_set_param (0 = the built-in
  * default for fft_tile, fft_bands, sphere_parts, gemm_splitk). */
@@ -226,7 +229,8 @@ int pp_ctx_get_param(pp_ctx *ctx, const char *name, int *value);
 /* Autotuner of the setup path (no counterpart in the reference; the
  * pp_autotune of the boundary's batched entry points): coordinate descent
  * over streams, fft_tile, fft_threads, sphere_unroll, sphere_parts,
- * fft_radix, fft_xcd, fft_bands and (FP32) fused, each candidate timed
+ * fft_radix, fft_xcd, fft_nt, sphere_nt, fft_bands and (FP32) fused, each
+ * candidate timed
  * with HIP events on the caller's own inputs, best of `repeats` runs after
  * a warm-up.  The fastest values stay set on ctx, and proj_dev holds the
  * projections of a final run with them.  Arguments as

@@ -233,8 +233,8 @@ class SetupGraph:
             pass
 
 
-TUNED = ("streams", "fft_tile", "fft_threads", "sphere_unroll", "sphere_parts", "fft_radix", "fft_xcd", "fft_bands",
-         "fused")
+TUNED = ("streams", "fft_tile", "fft_threads", "sphere_unroll", "sphere_parts", "fft_radix", "fft_xcd", "fft_nt",
+         "sphere_nt", "fft_bands", "fused")
 
 
 def autotune(ctx: Context, plan: SpherePlan, kfrac, gofs, G_dev, C_dev, proj_out, precision="fp64", repeats=2):

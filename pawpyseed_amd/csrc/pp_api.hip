@@ -489,6 +489,10 @@ int pp_ctx_set_param(pp_ctx *ctx, const char *name, int value) {
     ctx->tune_fft_radix = value != 0 ? 1 : 0;
   else if (n == "fft_xcd")
     ctx->tune_fft_xcd = value != 0 ? 1 : 0;
+  else if (n == "sphere_nt")
+    ctx->tune_sphere_nt = value != 0 ? 1 : 0;
+  else if (n == "fft_nt")
+    ctx->tune_fft_nt = value < 0 ? 0 : value > 2 ? 2 : value;
   else if (n == "fused")
     ctx->tune_fused = value != 0 ? 1 : 0;
   else if (n == "fft_tile")
@@ -540,7 +544,9 @@ int pp_ctx_get_param(pp_ctx *ctx, const char *name, int *value) {
       {"fft_pingpong", ctx->tune_fft_pingpong}, {"gemm_f64_valu", ctx->tune_gemm_f64_valu},
       {"fft_persistent", ctx->tune_fft_persistent}, {"streams", ctx->tune_streams},
       {"sphere_unroll", ctx->tune_sphere_unroll}, {"fft_threads", ctx->tune_fft_threads},
-      {"fft_radix", ctx->tune_fft_radix}, {"fft_xcd", ctx->tune_fft_xcd}, {"fused", ctx->tune_fused},
+      {"fft_radix", ctx->tune_fft_radix}, {"fft_xcd", ctx->tune_fft_xcd}, {"fft_nt", ctx->tune_fft_nt},
+      {"sphere_nt", ctx->tune_sphere_nt},
+      {"fused", ctx->tune_fused},
       {"fft_tile", ctx->tune_fft_tile}, {"fft_prune", ctx->tune_fft_prune}, {"fft_bands", ctx->tune_fft_bands},
       {"sphere_parts", ctx->tune_sphere_parts}, {"gemm_splitk", ctx->tune_splitk}};
   for (const auto &e : table)
@@ -577,6 +583,8 @@ int pp_autotune(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac, const i
       {"sphere_parts", {0, 4, 16}},
       {"fft_radix", {0, 1}},
       {"fft_xcd", {0, 1}},
+      {"fft_nt", {0, 1, 2}},
+      {"sphere_nt", {0, 1}},
       {"fft_bands", {64, 32}},
   };
   if (fp32)

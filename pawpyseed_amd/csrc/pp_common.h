@@ -61,6 +61,20 @@ __host__ __device__ __forceinline__ cplx<To> cvt(cplx<Ti> a) {
   return mk<To>((To)a.x, (To)a.y);
 }
 
+// Non-temporal (streaming) access to one complex value in device memory.
+template <typename T> __device__ __forceinline__ cplx<T> load_nt(const cplx<T> *p) {
+  using V = T __attribute__((ext_vector_type(2)));
+  const V v = __builtin_nontemporal_load(reinterpret_cast<const V *>(p));
+  return mk<T>(v.x, v.y);
+}
+template <typename T> __device__ __forceinline__ void store_nt(cplx<T> *p, cplx<T> v) {
+  using V = T __attribute__((ext_vector_type(2)));
+  V w;
+  w.x = v.x;
+  w.y = v.y;
+  __builtin_nontemporal_store(w, reinterpret_cast<V *>(p));
+}
+
 // ----------------------------------------------------------------- errors
 namespace pp {
 void set_error(int code, const std::string &msg);

@@ -322,10 +322,14 @@ __global__ __launch_bounds__(256) void k_fft_pass(cplx<T> *__restrict__ g, AxisP
 // xcd_per > 0: work-group w runs tile (w % 8) * xcd_per + w / 8, so each
 // of the 8 XCDs (which receive work-groups round-robin) walks one
 // contiguous range of lines; tiles past nblk exit.
+//
+// streaming: 1 = the line is written back with non-temporal stores, 2 =
+// also read with non-temporal loads (each pass touches its data once).
 template <typename T, int NBW, int AXIS, int MAXB, int NT, bool COMP = false>
 __global__ __launch_bounds__(NT) void k_fft_pass_ip(cplx<T> *__restrict__ g, AxisPlan pl,
                                                      const cplx<T> *__restrict__ tw, long es, long s1, long s2,
-                                                     int n2, int nslices, SphereBounds bd, int xcd_per, int nblk) {
+                                                     int n2, int nslices, SphereBounds bd, int xcd_per, int nblk,
+                                                     int streaming) {
   extern __shared__ unsigned char smem_raw[];
   cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
   const int n = pl.n;
@@ -361,13 +365,19 @@ __global__ __launch_bounds__(NT) void k_fft_pass_ip(cplx<T> *__restrict__ g, Axi
   for (int t = threadIdx.x; t < tot; t += NT) {
     int e = t / NBW, b = t % NBW;
     const int se = sgn_idx(e, n);
-    buf[t] = (se >= lo && se <= hi) ? g[base + (long)e * es + b] : mk<T>(0, 0);
+    cplx<T> v = mk<T>(0, 0);
+    if (se >= lo && se <= hi)
+      v = streaming > 1 ? load_nt(g + base + (long)e * es + b) : g[base + (long)e * es + b];
+    buf[t] = v;
   }
   __syncthreads();
   stages_ip<T, NBW, MAXB, NT, COMP>(buf, twl, pl, n);
   for (int t = threadIdx.x; t < tot; t += NT) {
     int e = t / NBW, b = t % NBW;
-    g[base + (long)e * es + b] = buf[t];
+    if (streaming)
+      store_nt(g + base + (long)e * es + b, buf[t]);
+    else
+      g[base + (long)e * es + b] = buf[t];
   }
 }
 
@@ -619,7 +629,8 @@ static int launch_pass(pp_ctx *ctx, cplx<T> *grid, const AxisPlan &ap, const cpl
       }
       size_t lds = ((size_t)ap.n * NBW + ap.twlen) * sizeof(cplx<T>);
       hipLaunchKernelGGL((k_fft_pass_ip<T, NBW, AXIS, 2, 256, true>), dim3((unsigned)nlaunch), dim3(256), lds,
-                         ctx->stream, grid, apc, tw, es, s1, s2, n2, nslices, bd, xcd_per, (int)nblk);
+                         ctx->stream, grid, apc, tw, es, s1, s2, n2, nslices, bd, xcd_per, (int)nblk,
+                         ctx->tune_fft_nt);
       launched = true;
     }
   }
@@ -646,7 +657,7 @@ static int launch_pass(pp_ctx *ctx, cplx<T> *grid, const AxisPlan &ap, const cpl
       size_t lds = ((size_t)ap.n * NBW + ap.twlen) * sizeof(cplx<T>);
 #define PP_IP(MB, NT)                                                                                          \
   hipLaunchKernelGGL((k_fft_pass_ip<T, NBW, AXIS, MB, NT>), dim3((unsigned)nlaunch), dim3(NT), lds, ctx->stream,    \
-                     grid, ap, tw, es, s1, s2, n2, nslices, bd, xcd_per, (int)nblk)
+                     grid, ap, tw, es, s1, s2, n2, nslices, bd, xcd_per, (int)nblk, ctx->tune_fft_nt)
       bool done = false;
       if constexpr (NBW >= 32) {
         if (nt == 512) {

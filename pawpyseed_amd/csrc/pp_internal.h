@@ -94,6 +94,8 @@ struct pp_ctx {
   int tune_sphere_unroll = 8;   // sphere-sum gathers in flight per wavefront (4 or 8)
   int tune_fft_threads = 256;   // work-group size of the in-place FFT passes with 32-band tiles (256 or 512)
   int tune_fft_radix = 0;       // 0: radices 2-5, 1: composite radices where they fit
+  int tune_sphere_nt = 1;       // 1: sphere-sum gathers with non-temporal loads
+  int tune_fft_nt = 2;          // FFT passes: 1 non-temporal stores, 2 non-temporal loads and stores
   int tune_fft_xcd = 0;         // 1: FFT pass work-groups in XCD-contiguous line ranges
   int tune_fused = 0;           // 1: x pass fused with the sphere sums where the plan allows
   hipStream_t aux = nullptr;    // second stream of the chunk pipeline (default priority)

@@ -340,7 +340,7 @@ __global__ __launch_bounds__(256) void k_sphere_sum(const cplx<T> *__restrict__ 
                                                     const int *__restrict__ site_pt, const int *__restrict__ site_np,
                                                     const long *__restrict__ tab_off, const int *__restrict__ pidx,
                                                     const cplx<T> *__restrict__ phase, const T *__restrict__ table,
-                                                    int nparts, cplx<double> *__restrict__ partial) {
+                                                    int nparts, cplx<double> *__restrict__ partial, int streaming) {
   constexpr int TP = 64;
   __shared__ __align__(16) T s_tab[TP * PMAX];
   __shared__ cplx<T> s_ph[TP];
@@ -373,8 +373,10 @@ __global__ __launch_bounds__(256) void k_sphere_sum(const cplx<T> *__restrict__ 
     for (; i + 4 * (U - 1) < n; i += 4 * U) {
       cplx<T> x[U];
 #pragma unroll
-      for (int u = 0; u < U; u++)
-        x[u] = active ? g[(long)s_idx[i + 4 * u] * NB + lane] : mk<T>(0, 0);
+      for (int u = 0; u < U; u++) {
+        const cplx<T> *gp = g + (long)s_idx[i + 4 * u] * NB + lane;
+        x[u] = active ? (streaming ? load_nt(gp) : *gp) : mk<T>(0, 0);
+      }
 #pragma unroll
       for (int u = 0; u < U; u++) {
         const cplx<T> xv = x[u] * s_ph[i + 4 * u];
@@ -745,7 +747,8 @@ static void launch_sum(pp_ctx *ctx, pp_sphere *sp, const cplx<T> *grid, int NB, 
 #define PP_SS(FULL, U)                                                                                            \
   hipLaunchKernelGGL((k_sphere_sum<T, PMAX, FULL, U>), dim3(sp->nsites * nparts), dim3(256), 0, ctx->stream, grid,  \
                      NB, (const int *)sp->d_site_pt.p, (const int *)sp->d_site_np.p, (const long *)sp->d_tab_off.p, \
-                     (const int *)sp->d_pidx.p, phase, (const T *)sp->d_table.p, nparts, partial)
+                     (const int *)sp->d_pidx.p, phase, (const T *)sp->d_table.p, nparts, partial,             \
+                     ctx->tune_sphere_nt)
   const bool u8 = ctx->tune_sphere_unroll == 8;
   if (sp->pmin == PMAX) {
     if (u8)
