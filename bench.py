@@ -105,6 +105,7 @@ def main():
     ap.add_argument("--paths", default="fp32,fp64", help="comma list of fp32, fp64 setup/overlap paths")
     ap.add_argument("--workload", default="216atom", choices=["216atom", "spinor512"],
                     help="216atom: the headline workload; spinor512: the 512-atom noncollinear case (separate line)")
+    ap.add_argument("--no-graph", action="store_true", help="skip the graph-replay timing")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (library defaults for the setup path)")
     ap.add_argument("--no-fused", action="store_true",
@@ -229,22 +230,25 @@ def main():
         # the same setup (tables as built by the last step) replayed as one
         # HIP graph
         pR, pS = plans[p]
-        with torch.cuda.stream(stream):
-            graph = B.SetupGraph(ctx, pS, wl.kpts, gofs, Gdev, CS, projS[p], p)
-            g0, g1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            g0.record(stream)
-            for _ in range(args.steps):
-                graph.replay(sync=False)
-            g1.record(stream)
-        torch.cuda.synchronize()
-        ts_graph = g0.elapsed_time(g1) / 1e3 / args.steps
-        graph.close()
+        ts_graph = None
+        if not args.no_graph:
+            with torch.cuda.stream(stream):
+                graph = B.SetupGraph(ctx, pS, wl.kpts, gofs, Gdev, CS, projS[p], p)
+                g0, g1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                g0.record(stream)
+                for _ in range(args.steps):
+                    graph.replay(sync=False)
+                g1.record(stream)
+            torch.cuda.synchronize()
+            ts_graph = g0.elapsed_time(g1) / 1e3 / args.steps
+            graph.close()
         nproj = nb * nk * ptotS
         ovl_flops = 8.0 * nb * nb * float(nw.sum())
         results[p] = dict(setup_s=ts, overlap_s=to, aug_s=ta, step_wall_s=wall,
                           band_projections_per_s=nproj / ts, overlap_tflops=ovl_flops / to / 1e12,
                           setup_s_one_chunk_in_flight=ts_serial,
-                          setup_s_graph_replay=ts_graph, band_projections_per_s_graph_replay=nproj / ts_graph,
+                          setup_s_graph_replay=ts_graph,
+                          band_projections_per_s_graph_replay=nproj / ts_graph if ts_graph else None,
                           autotune=tuned, autotune_trials=[list(t) for t in trials],
                           kernels={k: {"ms_total": v[0], "launches": v[1]} for k, v in kt.items()})
         log(p, json.dumps({k: v for k, v in results[p].items() if k != "kernels"}))
