@@ -220,7 +220,8 @@ int pp_ctx_set_tuning(pp_ctx *ctx, int fft_bands, int fft_tile, int fft_prune,
  * plan allows -- it writes no real-space grid but measured slower on the
  * 216-atom workload), "fft_xcd" (1: FFT pass work-groups walk
  * XCD-contiguous line ranges), "fft_nt" (FFT passes: 0 cached, 1
- * non-temporal stores, 2 non-temporal loads and stores, default 2),
+ * non-temporal stores, 2 non-temporal loads and stores, default 2; 3:
+ * cached z pass, streamed stores in the y pass, streamed x pass),
  * "sphere_nt" (1, default: sphere-sum gathers with non-temporal loads). */
 int pp_ctx_set_param(pp_ctx *ctx, const char *name, int value);
 /* Current value of a parameter of pp_ctx_set_param (0 = the built-in
@@ -277,7 +278,9 @@ int pp_sphere_info(pp_sphere *sp, int *total_projs, long *total_points,
  * and species): the per-structure setup step of setup_projections. */
 int pp_sphere_rebuild(pp_sphere *sp, const double *frac_coords);
 /* Diagnostic, no counterpart in the reference: whether the plan carries
- * the work lists of the fused x pass with sphere sums (FP32 plans with at
+ * (building them now if needed; they are otherwise built on the fused
+ * path's first use after each rebuild) the work lists of the fused x pass
+ * with sphere sums (FP32 plans with at
  * most 18 projectors per site whose boxes do not wrap onto themselves in
  * y or z), their work items, the partial rows that hold a site (one per
  * item and site), the plan's segments (runs of a site's points

@@ -492,7 +492,7 @@ int pp_ctx_set_param(pp_ctx *ctx, const char *name, int value) {
   else if (n == "sphere_nt")
     ctx->tune_sphere_nt = value != 0 ? 1 : 0;
   else if (n == "fft_nt")
-    ctx->tune_fft_nt = value < 0 ? 0 : value > 2 ? 2 : value;
+    ctx->tune_fft_nt = value < 0 ? 0 : value > 3 ? 3 : value;
   else if (n == "fused")
     ctx->tune_fused = value != 0 ? 1 : 0;
   else if (n == "fft_tile")
@@ -583,7 +583,7 @@ int pp_autotune(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac, const i
       {"sphere_parts", {0, 4, 16}},
       {"fft_radix", {0, 1}},
       {"fft_xcd", {0, 1}},
-      {"fft_nt", {0, 1, 2}},
+      {"fft_nt", {0, 1, 2, 3}},
       {"sphere_nt", {0, 1}},
       {"fft_bands", {64, 32}},
   };

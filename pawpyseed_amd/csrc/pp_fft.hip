@@ -610,6 +610,10 @@ static int launch_pass(pp_ctx *ctx, cplx<T> *grid, const AxisPlan &ap, const cpl
   for (int s = 0; comp && s < ap.nstc; s++)
     if (!dft_supported(ap.radixc[s]) || ((ap.n / ap.radixc[s]) * NBW + 255) / 256 > 2)
       comp = false;
+  // streaming access: fft_nt 0..2 for every pass; 3 keeps the z pass's
+  // stores cacheable (the y pass reads them next), streams the y pass's
+  // stores and both directions of the x pass
+  const int nt_mode = ctx->tune_fft_nt < 3 ? ctx->tune_fft_nt : (AXIS == 2 ? 0 : AXIS == 1 ? 1 : 2);
   // XCD-contiguous line ranges (fft_xcd = 1)
   const int xcd_per = ctx->tune_fft_xcd ? (int)((nblk + 7) / 8) : 0;
   const long nlaunch = xcd_per ? 8L * xcd_per : nblk;
@@ -629,8 +633,7 @@ static int launch_pass(pp_ctx *ctx, cplx<T> *grid, const AxisPlan &ap, const cpl
       }
       size_t lds = ((size_t)ap.n * NBW + ap.twlen) * sizeof(cplx<T>);
       hipLaunchKernelGGL((k_fft_pass_ip<T, NBW, AXIS, 2, 256, true>), dim3((unsigned)nlaunch), dim3(256), lds,
-                         ctx->stream, grid, apc, tw, es, s1, s2, n2, nslices, bd, xcd_per, (int)nblk,
-                         ctx->tune_fft_nt);
+                         ctx->stream, grid, apc, tw, es, s1, s2, n2, nslices, bd, xcd_per, (int)nblk, nt_mode);
       launched = true;
     }
   }
@@ -657,7 +660,7 @@ static int launch_pass(pp_ctx *ctx, cplx<T> *grid, const AxisPlan &ap, const cpl
       size_t lds = ((size_t)ap.n * NBW + ap.twlen) * sizeof(cplx<T>);
 #define PP_IP(MB, NT)                                                                                          \
   hipLaunchKernelGGL((k_fft_pass_ip<T, NBW, AXIS, MB, NT>), dim3((unsigned)nlaunch), dim3(NT), lds, ctx->stream,    \
-                     grid, ap, tw, es, s1, s2, n2, nslices, bd, xcd_per, (int)nblk, ctx->tune_fft_nt)
+                     grid, ap, tw, es, s1, s2, n2, nslices, bd, xcd_per, (int)nblk, nt_mode)
       bool done = false;
       if constexpr (NBW >= 32) {
         if (nt == 512) {

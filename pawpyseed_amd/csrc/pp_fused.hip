@@ -370,6 +370,7 @@ int fused_plan(pp_sphere *sp, const std::vector<SiteGeo> &geo) {
   pp_ctx *ctx = sp->ctx;
   const int ns = sp->nsites;
   sp->fused_ok = false;
+  sp->fx_built = true;
   sp->fx_nitems = sp->fx_nrows = sp->fx_nrows_used = 0;
   if (sp->precision != PP_FP32 || sp->pmax > FX_PMAX || ns == 0 || sp->nseg == 0)
     return PP_OK;
@@ -447,8 +448,13 @@ static size_t fused_lds(const AxisPlan &ap, const pp_sphere *sp) {
          (size_t)FX_G * sp->fx_maxlen * FX_PMAX * sizeof(float);
 }
 
+// Work lists of the current tables, built on first use.
+static int fused_ensure(pp_sphere *sp) { return sp->fx_built ? PP_OK : fused_plan(sp, sp->geo_h); }
+
 bool fused_usable(pp_ctx *ctx, FftPlan *pl, pp_sphere *sp, int NB, bool fp32) {
-  if (!ctx->tune_fused || !sp->fused_ok || !fp32 || NB % 16 != 0 || ctx->tune_fft_pingpong)
+  if (!ctx->tune_fused || !fp32 || NB % 16 != 0 || ctx->tune_fft_pingpong)
+    return false;
+  if (fused_ensure(sp) != PP_OK || !sp->fused_ok)
     return false;
   const AxisPlan &ap = pl->ax[0];
   int rmax = 0;
@@ -534,6 +540,7 @@ int fused_x_sums(pp_ctx *ctx, FftPlan *pl, pp_sphere *sp, const cplx<float> *gri
 extern "C" int pp_sphere_fused_info(pp_sphere *sp, int *usable, int *nitems, int *nrows, long *nseg, int *flag) {
   PP_REQUIRE(sp, PP_ERR_ARG, "pp_sphere_fused_info: null plan");
   PP_CHECK_HIP(hipSetDevice(sp->ctx->device));
+  PP_TRY(pp::fused_ensure(sp));
   if (usable)
     *usable = sp->fused_ok ? 1 : 0;
   if (nitems)

@@ -80,6 +80,8 @@ struct pp_sphere {
   // does not fit them (FP64 tables, more than FX_PMAX projectors on a site,
   // or a site box that wraps onto itself in y or z)
   bool fused_ok = false;
+  bool fx_built = false;          // work lists built for the current tables
+  std::vector<pp::SiteGeo> geo_h;  // site geometry of the current tables
   int fx_nitems = 0, fx_nrows = 0, fx_nrows_used = 0, fx_nblocks = 0, fx_nbz = 0;
   int fx_maxlen = 1;  // longest possible segment (widest site box in x)
   std::vector<int> fx_host;  // packed host copy of the work lists

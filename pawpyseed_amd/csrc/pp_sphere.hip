@@ -738,7 +738,11 @@ static int sphere_build(pp_sphere *sp) {
     tend(ctx, K_TABLE, ev);
     PP_CHECK_HIP(hipGetLastError());
   }
-  return fused_plan(sp, geo);
+  // the fused x pass's work lists are built when that path is first used
+  sp->geo_h = geo;
+  sp->fx_built = false;
+  sp->fused_ok = false;
+  return ctx->tune_fused ? fused_plan(sp, geo) : PP_OK;
 }
 
 template <typename T, int PMAX>
