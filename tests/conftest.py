@@ -16,3 +16,9 @@ def pytest_configure(config):
 def wavecar():
     import numpy as np
     return dict(np.load(os.path.join(ROOT, "tests", "golden", "wavecar_ga4.npz")))
+
+
+@pytest.fixture(scope="session")
+def wavecar_ncl():
+    import numpy as np
+    return dict(np.load(os.path.join(ROOT, "tests", "golden", "wavecar_ncl.npz")))

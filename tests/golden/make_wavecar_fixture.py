@@ -15,6 +15,13 @@ as a fixture:
   plane-wave summation, psi(f) = V^{-1/2} sum_G C_G exp(2 pi i G.f), the
   brute-force check of the reference's fft_check (tests/tests.c:28-80).
 
+A second fixture, wavecar_ncl.npz, comes from the reference's spinor file
+test_files/noncollinear/WAVECAR (Ga4, 4 k-points, 24 spinor bands, 520 eV;
+each record holds 2 W coefficients, the up then the down component,
+reader.c:274-281): the header, and for the first NCL_K k-points the
+plane-wave count, the enumerated W G vectors (the script asserts
+2 W equals the file's count), and the first NB spinor bands.
+
 Run in the build container (needs /root/reference):
     python tests/golden/make_wavecar_fixture.py
 """
@@ -30,6 +37,8 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 from pawpyseed_amd.synthetic import gvectors  # noqa: E402
 
 SRC = "/root/reference/test_files/WAVECAR"
+SRC_NCL = "/root/reference/test_files/noncollinear/WAVECAR"
+NCL_K = 2
 NB = 6
 NGRID = (20, 20, 20)  # NGX/NGY/NGZ of test_files/vasprun.xml
 
@@ -84,5 +93,40 @@ def main():
     print("wrote", os.path.join(HERE, "wavecar_ga4.npz"), {k: np.shape(v) for k, v in out.items()})
 
 
+def main_ncl():
+    raw = np.fromfile(SRC_NCL, dtype=np.uint8)
+    head = raw[:24].view(np.float64)
+    nrecl, nspin, nprec = int(round(head[0])), int(round(head[1])), int(round(head[2]))
+    assert nprec == 45200 and nspin == 1
+    rec1 = raw[nrecl:nrecl + 12 * 8].view(np.float64)
+    nwk, nband, encut = int(round(rec1[0])), int(round(rec1[1])), float(rec1[2])
+    lattice = rec1[3:12].reshape(3, 3).copy()
+    out = dict(lattice=lattice, encut=encut, nwk=nwk, nband=nband, nb_stored=NB, nk_stored=NCL_K)
+    kpts = []
+    for iwk in range(NCL_K):
+        irec = iwk * (1 + nband)
+        off = (irec + 2) * nrecl
+        kr = raw[off:off + nrecl].view(np.float64)
+        nplane = int(round(kr[0]))
+        k = kr[1:4].copy()
+        eo = kr[4:4 + 3 * nband].reshape(nband, 3)
+        coeffs = []
+        for b in range(NB):
+            o = (irec + 1 + b + 2) * nrecl
+            coeffs.append(raw[o:o + 8 * nplane].view(np.complex64).copy())
+        G = gvectors(lattice, k, encut)
+        assert 2 * len(G) == nplane, (len(G), nplane)
+        kpts.append(k)
+        out[f"nplane_{iwk}"] = nplane
+        out[f"G_{iwk}"] = G
+        out[f"C_{iwk}"] = np.stack(coeffs)
+        out[f"energy_{iwk}"] = eo[:NB, 0].copy()
+        out[f"occ_{iwk}"] = eo[:NB, 2].copy()
+    out["kpts"] = np.array(kpts)
+    np.savez_compressed(os.path.join(HERE, "wavecar_ncl.npz"), **out)
+    print("wrote", os.path.join(HERE, "wavecar_ncl.npz"), {k: np.shape(v) for k, v in out.items()})
+
+
 if __name__ == "__main__":
     main()
+    main_ncl()

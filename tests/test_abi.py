@@ -133,6 +133,27 @@ def test_reader_spinor_wavecar(tmp_path):
     L.free_pswf(pw.ptr)
 
 
+def test_reader_spinor_fixture(tmp_path, wavecar_ncl):
+    """The spinor records of the reference's noncollinear WAVECAR (fixture
+    subset), written back in the WAVECAR layout and read: spinor flag, the
+    enumerated G count is half the stored plane-wave count, occupations."""
+    f = wavecar_ncl
+    nk = int(f["nk_stored"])
+    Cs = [f[f"C_{k}"] for k in range(nk)]
+    Gs = [f[f"G_{k}"] for k in range(nk)]
+    assert all(2 * len(Gs[k]) == int(f[f"nplane_{k}"]) == Cs[k].shape[1] for k in range(nk))
+    p = str(tmp_path / "WAVECAR")
+    write_wavecar(p, f["lattice"], float(f["encut"]), f["kpts"], Gs, Cs, occ=[f[f"occ_{k}"] for k in range(nk)],
+                  energies=[f[f"energy_{k}"] for k in range(nk)])
+    w = np.full(nk, 1.0 / nk)
+    L = lib()
+    ptr_ = L.read_wavefunctions(p.encode(), ptr(w, c_double_p))
+    assert ptr_
+    assert L.is_ncl(ptr_) == 1 and L.get_nband(ptr_) == Cs[0].shape[0] and L.get_nwk(ptr_) == nk
+    assert L.get_occ(ptr_, 2, 1, 0) == f["occ_1"][2]
+    L.free_pswf(ptr_)
+
+
 def test_reader_on_reference_ncl_wavecar():
     """The reference's noncollinear fixture (test_files/noncollinear:
     4 k-points, 24 spinor bands, 520 eV)."""

@@ -336,3 +336,39 @@ def test_noncollinear_setup_projections(gpu):
         Projector(wf, wf2)
     with pytest.raises(PAWpyError):
         Wavefunction(struct, wf._pwf, cr, wl.fftg)
+
+
+def test_noncollinear_reference_wavecar(gpu, tmp_path, wavecar_ncl):
+    """Spinor bands of the reference's noncollinear WAVECAR (Ga4, fixture
+    subset), read through the WAVECAR reader: up and down projections of
+    synthetic l = 0..2 projectors at the CONTCAR sites against the oracle
+    on each half of the coefficients."""
+    from pawpyseed_amd.core import CoreRegion, NCLWavefunction, Pseudopotential, Structure
+    from test_abi import write_wavecar
+    f = wavecar_ncl
+    nk = int(f["nk_stored"])
+    Cs = [f[f"C_{k}"] for k in range(nk)]
+    Gs = [f[f"G_{k}"] for k in range(nk)]
+    frac = np.array([[0.1560427174376784, 0.8439572825631387, 0.9166540580595424],
+                     [0.3439572825623216, 0.6560427174368613, 0.4166540580595424],
+                     [0.8439572825623216, 0.1560427174368613, 0.0833459419404576],
+                     [0.6560427174376784, 0.3439572825631387, 0.5833459419404576]])
+    p = str(tmp_path / "WAVECAR")
+    write_wavecar(p, f["lattice"], float(f["encut"]), f["kpts"], Gs, Cs)
+    sp = synthetic.make_species([0, 1, 2], [1.5, 2.0, 2.5], 1.2)
+    struct = Structure(f["lattice"], ["Ga"] * 4, frac)
+    cr = CoreRegion({"Ga": Pseudopotential.from_species(sp)})
+    kws = np.full(nk, 1.0 / nk)
+    wf = NCLWavefunction.from_wavecar(struct, p, cr, (30, 30, 30), f["kpts"], kws, setup_projectors=True)
+    wl = synthetic.Workload("ga4ncl", f["lattice"], frac, frac, sp, np.array([30, 30, 30], np.int32),
+                            float(f["encut"]), f["kpts"], kws, Cs[0].shape[0], np.eye(3), np.arange(4),
+                            np.arange(4), gvecs=Gs)
+    W = [len(g) for g in Gs]
+    PU, _ = oracle_projections(wl, frac, [Cs[k][:, :W[k]] for k in range(nk)])
+    PD, _ = oracle_projections(wl, frac, [Cs[k][:, W[k]:] for k in range(nk)])
+    for k in range(nk):
+        for b in range(Cs[0].shape[0]):
+            got = wf.get_projections(b, k)
+            eu, ed = normwise(got[0], PU[k, b]), normwise(got[1], PD[k, b])
+            assert eu < 1e-9 and ed < 1e-9, (k, b, eu, ed)
+    print("reference spinor WAVECAR: projections of", nk, "k x", Cs[0].shape[0], "bands match", flush=True)
