@@ -273,8 +273,9 @@ def main():
     # dominant kernel of the headline path and its roofline.  Algorithmic
     # bytes per launch (one 64-band chunk of one k-point):
     #   fft_x (last 1D pass): read the populated x-slab of the grid
-    #     (the plane-wave sphere spans xs_k planes), write the whole grid:
-    #     N1 N2 NB prec (xs_k + N0), averaged over k;
+    #     (the plane-wave sphere spans xs_k planes), write the grid points
+    #     the projector spheres hold (fft_xmask; else the whole grid):
+    #     NB prec (N1 N2 xs_k + covered), averaged over k;
     #   fft_y: read xs_k planes restricted to the populated y-rows ys_k(x),
     #     write the xs_k planes in full;
     #   fft_z: read the plane-wave coefficients in place (W_k points), write
@@ -288,6 +289,8 @@ def main():
     n0, n1, n2 = (int(v) for v in wl.fftg)
     NB = 64
     bx, by, bz = [], [], []
+    xmask_on = bool((R.get("autotune") or {}).get("fft_xmask", ctx.get_param("fft_xmask")))
+    covered = plans[head][1].coverage()
     for G in Gs:
         gx, gy = G[:, 0].astype(np.int64), G[:, 1].astype(np.int64)
         xs = int(gx.max() - gx.min() + 1)
@@ -296,7 +299,9 @@ def main():
         for x in np.unique(gx):
             sel = gy[gx == x]
             yrow += int(sel.max() - sel.min() + 1)
-        bx.append(n1 * n2 * NB * prec_bytes * (xs + n0))
+        # written: the whole grid, or with fft_xmask only the points the
+        # projector spheres hold (their union)
+        bx.append(n1 * n2 * NB * prec_bytes * xs + (covered if xmask_on else n0 * n1 * n2) * NB * prec_bytes)
         by.append(NB * prec_bytes * n2 * (yrow + xs * n1))
         bz.append(NB * prec_bytes * (len(G) + rows * n2))
     planS = plans[head][1]

@@ -222,7 +222,11 @@ int pp_ctx_set_tuning(pp_ctx *ctx, int fft_bands, int fft_tile, int fft_prune,
  * XCD-contiguous line ranges), "fft_nt" (FFT passes: 0 cached, 1
  * non-temporal stores, 2 non-temporal loads and stores, default 2; 3:
  * cached z pass, streamed stores in the y pass, streamed x pass),
- * "sphere_nt" (1, default: sphere-sum gathers with non-temporal loads). */
+ * "sphere_nt" (1, default: sphere-sum gathers with non-temporal loads),
+ * "fft_xmask" (1: the x pass of a setup writes only the grid points some
+ * projector sphere holds, the only ones the sphere sums read; 0, default:
+ * every point -- the masked stores measured slower on the 216-atom
+ * workload although they write 31% fewer bytes). */
 int pp_ctx_set_param(pp_ctx *ctx, const char *name, int value);
 /* Current value of a parameter of pp_ctx_set_param (0 = the built-in
  * default for fft_tile, fft_bands, sphere_parts, gemm_splitk). */
@@ -230,7 +234,8 @@ int pp_ctx_get_param(pp_ctx *ctx, const char *name, int *value);
 /* Autotuner of the setup path (no counterpart in the reference; the
  * pp_autotune of the boundary's batched entry points): coordinate descent
  * over streams, fft_tile, fft_threads, sphere_unroll, sphere_parts,
- * fft_radix, fft_xcd, fft_nt, sphere_nt, fft_bands and (FP32) fused, each
+ * fft_radix, fft_xcd, fft_xmask, fft_nt, sphere_nt, fft_bands and (FP32)
+ * fused, each
  * candidate timed
  * with HIP events on the caller's own inputs, best of `repeats` runs after
  * a warm-up.  The fastest values stay set on ctx, and proj_dev holds the
@@ -274,6 +279,10 @@ void pp_sphere_destroy(pp_sphere *sp);
  * (proj_off[num_sites+1], may be NULL) */
 int pp_sphere_info(pp_sphere *sp, int *total_projs, long *total_points,
                    int *proj_off);
+/* Grid points held by at least one of the plan's projector spheres (the
+ * union; the x pass of a setup writes only those).  Synchronises the
+ * context's stream.  No counterpart in the reference. */
+int pp_sphere_coverage(pp_sphere *sp, long *covered);
 /* Rebuild the tables of an existing plan for new coordinates (same sites
  * and species): the per-structure setup step of setup_projections. */
 int pp_sphere_rebuild(pp_sphere *sp, const double *frac_coords);

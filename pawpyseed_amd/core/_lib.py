@@ -96,6 +96,7 @@ SIGNATURES = {
                                     ct.c_int]),
     "pp_sphere_destroy": (None, [c_void_p]),
     "pp_sphere_info": (ct.c_int, [c_void_p, c_int_p, ct.POINTER(ct.c_long), c_int_p]),
+    "pp_sphere_coverage": (ct.c_int, [c_void_p, ct.POINTER(ct.c_long)]),
     "pp_sphere_fused_info": (ct.c_int, [c_void_p, c_int_p, c_int_p, c_int_p, ct.POINTER(ct.c_long), c_int_p]),
     "pp_sphere_rebuild": (ct.c_int, [c_void_p, c_double_p]),
     "pp_setup_projections_batched": (ct.c_int, [c_void_p, c_void_p, ct.c_int, c_double_p, c_int_p, c_void_p,

@@ -123,6 +123,12 @@ class SpherePlan:
         self.npts = npt.value
         self.proj_off = off
 
+    def coverage(self) -> int:
+        """Grid points held by at least one projector sphere."""
+        c = ct.c_long()
+        check(lib().pp_sphere_coverage(self.h, ct.byref(c)), "pp_sphere_coverage")
+        return c.value
+
     def fused_info(self):
         """(usable, work items, partial rows, segments, flag) of the plan's
         fused x-pass work lists; flag 0 means every segment found its slot
@@ -233,8 +239,8 @@ class SetupGraph:
             pass
 
 
-TUNED = ("streams", "fft_tile", "fft_threads", "sphere_unroll", "sphere_parts", "fft_radix", "fft_xcd", "fft_nt",
-         "sphere_nt", "fft_bands", "fused")
+TUNED = ("streams", "fft_tile", "fft_threads", "sphere_unroll", "sphere_parts", "fft_radix", "fft_xcd", "fft_xmask",
+         "fft_nt", "sphere_nt", "fft_bands", "fused")
 
 
 def autotune(ctx: Context, plan: SpherePlan, kfrac, gofs, G_dev, C_dev, proj_out, precision="fp64", repeats=2):

@@ -108,6 +108,7 @@ int sphere_precision(pp_sphere *sp);
 const int *sphere_fftg(pp_sphere *sp);
 const double *sphere_lattice(pp_sphere *sp);
 long sphere_generation(pp_sphere *sp);
+const unsigned *sphere_xmask(pp_sphere *sp, int *words);
 // fused x pass with sphere sums (pp_fused.hip)
 bool fused_usable(pp_ctx *ctx, FftPlan *pl, pp_sphere *sp, int NB, bool fp32);
 int fused_x_sums(pp_ctx *ctx, FftPlan *pl, pp_sphere *sp, const cplx<float> *grid, int NB, int nbc,
@@ -247,6 +248,8 @@ static int setup_launch(pp_ctx *ctx, pp_sphere *sp, int nk, const int *gofs, con
       bd.nyl = (int)(S.zl_off[k + 1] - S.yl_off[k]);
     }
     const bool prune = S.alias[k] == 0 && ctx->tune_fft_prune != 2;
+    if (ctx->tune_fft_xmask && !S.fused)
+      bd.xm = sphere_xmask(sp, &bd.xmw);
     const cplx<T> *phase_k = (const cplx<T> *)S.phase + (size_t)k * S.npts_sph;
     for (int b0 = 0; b0 < nband && st == PP_OK; b0 += NB, c++) {
       const int nbc = std::min(NB, nband - b0);
@@ -487,6 +490,8 @@ int pp_ctx_set_param(pp_ctx *ctx, const char *name, int value) {
     ctx->tune_fft_threads = value >= 512 ? 512 : 256;
   else if (n == "fft_radix")
     ctx->tune_fft_radix = value != 0 ? 1 : 0;
+  else if (n == "fft_xmask")
+    ctx->tune_fft_xmask = value != 0 ? 1 : 0;
   else if (n == "fft_xcd")
     ctx->tune_fft_xcd = value != 0 ? 1 : 0;
   else if (n == "sphere_nt")
@@ -544,7 +549,8 @@ int pp_ctx_get_param(pp_ctx *ctx, const char *name, int *value) {
       {"fft_pingpong", ctx->tune_fft_pingpong}, {"gemm_f64_valu", ctx->tune_gemm_f64_valu},
       {"fft_persistent", ctx->tune_fft_persistent}, {"streams", ctx->tune_streams},
       {"sphere_unroll", ctx->tune_sphere_unroll}, {"fft_threads", ctx->tune_fft_threads},
-      {"fft_radix", ctx->tune_fft_radix}, {"fft_xcd", ctx->tune_fft_xcd}, {"fft_nt", ctx->tune_fft_nt},
+      {"fft_radix", ctx->tune_fft_radix}, {"fft_xcd", ctx->tune_fft_xcd},
+      {"fft_xmask", ctx->tune_fft_xmask}, {"fft_nt", ctx->tune_fft_nt},
       {"sphere_nt", ctx->tune_sphere_nt},
       {"fused", ctx->tune_fused},
       {"fft_tile", ctx->tune_fft_tile}, {"fft_prune", ctx->tune_fft_prune}, {"fft_bands", ctx->tune_fft_bands},
@@ -583,6 +589,7 @@ int pp_autotune(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac, const i
       {"sphere_parts", {0, 4, 16}},
       {"fft_radix", {0, 1}},
       {"fft_xcd", {0, 1}},
+      {"fft_xmask", {0, 1}},
       {"fft_nt", {0, 1, 2, 3}},
       {"sphere_nt", {0, 1}},
       {"fft_bands", {64, 32}},

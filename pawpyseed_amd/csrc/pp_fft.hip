@@ -372,8 +372,11 @@ __global__ __launch_bounds__(NT) void k_fft_pass_ip(cplx<T> *__restrict__ g, Axi
   }
   __syncthreads();
   stages_ip<T, NBW, MAXB, NT, COMP>(buf, twl, pl, n);
+  const unsigned *xm = (AXIS == 0 && bd.xm) ? bd.xm + (long)line * bd.xmw : nullptr;
   for (int t = threadIdx.x; t < tot; t += NT) {
     int e = t / NBW, b = t % NBW;
+    if (xm && !((xm[e >> 5] >> (e & 31)) & 1u))
+      continue;  // no projector sphere reads this point
     if (streaming)
       store_nt(g + base + (long)e * es + b, buf[t]);
     else

@@ -63,6 +63,11 @@ struct SphereBounds {
   // pass), nullptr = every line
   const int *zl = nullptr, *yl = nullptr;
   int nzl = 0, nyl = 0;
+  // x pass: points of x-line y * n2 + z that some projector sphere holds,
+  // bit x of word xm[line * xmw + x / 32]; only those are written
+  // (nullptr = all)
+  const unsigned *xm = nullptr;
+  int xmw = 0;
 };
 
 struct FftPlan {
@@ -96,6 +101,7 @@ struct pp_ctx {
   int tune_fft_radix = 0;       // 0: radices 2-5, 1: composite radices where they fit
   int tune_sphere_nt = 1;       // 1: sphere-sum gathers with non-temporal loads
   int tune_fft_nt = 2;          // FFT passes: 1 non-temporal stores, 2 non-temporal loads and stores
+  int tune_fft_xmask = 0;        // 1: the x pass writes only points a projector sphere holds
   int tune_fft_xcd = 0;         // 1: FFT pass work-groups in XCD-contiguous line ranges
   int tune_fused = 0;           // 1: x pass fused with the sphere sums where the plan allows
   hipStream_t aux = nullptr;    // second stream of the chunk pipeline (default priority)

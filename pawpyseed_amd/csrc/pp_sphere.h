@@ -76,6 +76,10 @@ struct pp_sphere {
   long nseg = 0;
   std::vector<int> seg_off_h;
   pp::DevBuf d_seg, d_segline, d_seg_off;
+  // per x-line (y * N2 + z) bit mask of the points some sphere holds,
+  // xmask_words 32-bit words per line
+  pp::DevBuf d_xmask;
+  int xmask_words = 0;
   // fused x pass work lists (pp_fused.hip); fused_ok is false when the plan
   // does not fit them (FP64 tables, more than FX_PMAX projectors on a site,
   // or a site box that wraps onto itself in y or z)

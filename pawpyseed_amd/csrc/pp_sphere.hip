@@ -183,6 +183,22 @@ __global__ __launch_bounds__(256) void k_site_fill(const SiteGeo *__restrict__ g
   }
 }
 
+// Coverage masks of the x-lines: every segment sets the bits of its points.
+__global__ __launch_bounds__(256) void k_xmask(const int4 *__restrict__ seg, const int *__restrict__ segline,
+                                               long nseg, int n0, int mw, unsigned *__restrict__ mask) {
+  const long q = (long)blockIdx.x * 256 + threadIdx.x;
+  if (q >= nseg)
+    return;
+  const int4 sg = seg[q];
+  unsigned *m = mask + (long)segline[q] * mw;
+  int x = sg.z;
+  for (int i = 0; i < sg.y; i++) {
+    atomicOr(&m[x >> 5], 1u << (x & 31));
+    if (++x == n0)
+      x = 0;
+  }
+}
+
 // Segment lengths: the points of segment q run to the first point of the
 // next segment of its site, or to the end of the site's points.
 __global__ __launch_bounds__(256) void k_seg_len(int4 *__restrict__ seg, long nseg, const int *__restrict__ seg_off,
@@ -722,6 +738,17 @@ static int sphere_build(pp_sphere *sp) {
                        (int4 *)sp->d_seg.p, sp->nseg, (const int *)sp->d_seg_off.p, (const int *)sp->d_site_pt.p);
   tend(ctx, K_TABLE, ev);
   PP_CHECK_HIP(hipGetLastError());
+  // x-line coverage masks (the x pass writes only covered points)
+  sp->xmask_words = (sp->fftg[0] + 31) / 32;
+  const size_t mbytes = sizeof(unsigned) * (size_t)sp->fftg[1] * sp->fftg[2] * sp->xmask_words;
+  PP_TRY(sp->d_xmask.ensure(mbytes));
+  PP_TRY(zero_async(ctx, sp->d_xmask.p, mbytes));
+  if (sp->nseg > 0) {
+    hipLaunchKernelGGL(k_xmask, dim3((unsigned)((sp->nseg + 255) / 256)), dim3(256), 0, ctx->stream,
+                       (const int4 *)sp->d_seg.p, (const int *)sp->d_segline.p, sp->nseg, sp->fftg[0],
+                       sp->xmask_words, (unsigned *)sp->d_xmask.p);
+    PP_CHECK_HIP(hipGetLastError());
+  }
   if (maxcount > 0) {
     dim3 grd((maxcount + 255) / 256, ns);
     tbegin(ctx, K_TABLE, &ev);
@@ -902,6 +929,10 @@ template int sphere_expand<float>(pp_ctx *, pp_sphere *, cplx<float> *, int, int
                                   const cplx<double> *, long, const int *);
 int sphere_nsites(pp_sphere *sp) { return sp->nsites; }
 long sphere_generation(pp_sphere *sp) { return sp->gen; }
+const unsigned *sphere_xmask(pp_sphere *sp, int *words) {
+  *words = sp->xmask_words;
+  return (const unsigned *)sp->d_xmask.p;
+}
 
 long sphere_npts(pp_sphere *sp) { return sp->npts; }
 int sphere_ptot(pp_sphere *sp) { return sp->ptot; }
@@ -998,6 +1029,21 @@ extern "C" int pp_sphere_rebuild(pp_sphere *sp, const double *frac_coords) {
 }
 
 extern "C" void pp_sphere_destroy(pp_sphere *sp) { delete sp; }
+
+extern "C" int pp_sphere_coverage(pp_sphere *sp, long *covered) {
+  PP_REQUIRE(sp && covered, PP_ERR_ARG, "pp_sphere_coverage: null argument");
+  PP_CHECK_HIP(hipSetDevice(sp->ctx->device));
+  const size_t nw = (size_t)sp->fftg[1] * sp->fftg[2] * sp->xmask_words;
+  std::vector<unsigned> m(nw);
+  PP_CHECK_HIP(hipStreamSynchronize(sp->ctx->stream));
+  if (nw)
+    PP_CHECK_HIP(hipMemcpy(m.data(), sp->d_xmask.p, sizeof(unsigned) * nw, hipMemcpyDeviceToHost));
+  long c = 0;
+  for (unsigned w : m)
+    c += __builtin_popcount(w);
+  *covered = c;
+  return PP_OK;
+}
 
 extern "C" int pp_sphere_info(pp_sphere *sp, int *total_projs, long *total_points, int *proj_off) {
   PP_REQUIRE(sp, PP_ERR_ARG, "pp_sphere_info: null plan");

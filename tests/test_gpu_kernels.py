@@ -375,3 +375,32 @@ def test_empty_and_invalid_inputs(ctx):
                    torch.zeros((1, 2, 64), dtype=torch.complex64, device="cuda"), o)
     assert torch.count_nonzero(o).item() == 0
     plan.close()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_xmask_writes_only_covered_points(ctx, precision):
+    """The x pass writes only the grid points some projector sphere holds
+    (fft_xmask): the plan's coverage equals the union of the oracle's
+    sphere points, and the projections are bit-identical with the
+    unmasked pass (the sphere sums read covered points only)."""
+    from pawpyseed_amd.core import batched as B
+    wl = synthetic.config0()
+    Gs = wl.ensure_gvecs()
+    Cs = wl.coefficients("S", nband=64)
+    labels = np.zeros(len(wl.frac_S), np.int32)
+    plan = B.SpherePlan(ctx, wl.lattice, wl.fftg, wl.frac_S, labels, [wl.species], precision)
+    T = orc.site_tables(wl.lattice, wl.fftg, wl.frac_S, [wl.species], labels)
+    union = len(np.unique(T.idx))
+    assert plan.coverage() == union
+    print("coverage", union, "of", int(np.prod(wl.fftg)), flush=True)
+    Cd, Gd, gofs, nw = device_inputs(Gs, Cs)
+    outs = []
+    for xm in (0, 1):
+        ctx.set_param("fft_xmask", xm)
+        out = torch.zeros((len(Gs), 64, plan.ptot), dtype=torch.complex128, device="cuda")
+        B.setup_projections(ctx, plan, wl.kpts, gofs, Gd, Cd, out, precision)
+        torch.cuda.synchronize()
+        outs.append(out.cpu().numpy())
+    ctx.set_param("fft_xmask", 1)
+    assert np.array_equal(outs[0], outs[1])
+    plan.close()
