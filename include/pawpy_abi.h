@@ -173,6 +173,36 @@ int compensation_terms(pp_cdouble *overlap, int BAND_NUM, pswf_t *wf_S,
 int fft3d(pp_cdouble *x, int *G_bounds, double *lattice, double *kpt, int *Gs,
           pp_cfloat *Cs, int num_waves, int *fftg);
 
+/* All-electron real-space states (density.h, density.c:158-327), on the
+ * GPU over the device-resident coefficients and projections of a preceding
+ * setup_projections.  Each returns a status instead of void.
+ * realspace_state (density.c:232-310): x c128 [fftg] (x slowest) gets the
+ * state of (BAND_NUM, KPOINT_NUM) on the grid fftg with the partial-wave
+ * differences of the sites (labels, coords). */
+int realspace_state(pp_cdouble *x, int BAND_NUM, int KPOINT_NUM, pswf_t *wf,
+                    int *fftg, int *labels, double *coords);
+/* density.c:312-326: x *= e^{-2 pi i k.r} (on the GPU). */
+int remove_phase(pp_cdouble *x, int KPOINT_NUM, pswf_t *wf, int *fftg);
+/* density.c:28-39: P += |psi|^2 of one state. */
+int ae_state_density(double *P, int BAND_NUM, int KPOINT_NUM, pswf_t *wf,
+                     int *fftg, int *labels, double *coords);
+/* density.c:158-179: P += sum over k and bands with occ > 0 of
+ * |psi|^2 weight_k occ (2 / nspin). */
+int ae_chg_density(double *P, pswf_t *wf, int *fftg, int *labels,
+                   double *coords);
+/* density.c:181-203 and 328-421 (spinors): x c128 [2][fftg], the up
+ * component then the down one; the density sums both (factor 1). */
+int ncl_realspace_state(pp_cdouble *x, int BAND_NUM, int KPOINT_NUM,
+                        pswf_t *wf, int *fftg, int *labels, double *coords);
+int ncl_ae_chg_density(double *P, pswf_t *wf, int *fftg, int *labels,
+                       double *coords);
+/* density.c:205-230: projs[b nwk nspin + k] = V/N^3 sum_r
+ * conj(psi_R,b,k(r)) psi_BAND_NUM,k(r) over the bands b < wf->nband of
+ * wf_R (the `realspace` projection method). */
+int project_realspace_state(pp_cdouble *projs, int BAND_NUM, pswf_t *wf,
+                            pswf_t *wf_R, int *fftg, int *labels,
+                            double *coords, int *labels_R, double *coords_R);
+
 /* No reference counterpart: copies projections of one (band, k-index)
  * into out (c128 [total projectors]) from the device-resident table; for
  * a spinor wavefunction out holds the up component, then the down one
@@ -413,6 +443,45 @@ int pp_offsite_overlap(const double *dcoord, int n1, const double *r1,
                        const double *f1, int n2, const double *r2,
                        const double *f2, int l1, int m1, int l2, int m2,
                        double *out2);
+
+/* ---- all-electron real-space states ---- */
+
+typedef struct pp_rsplan pp_rsplan;
+/* Partial-wave spheres of one structure on a real-space grid (the site
+ * loops of realspace_state, density.c:259-309, done on the GPU).  Species
+ * e: nchan[e] channels with l = ls[...] (concatenated, l <= 3), wave_n[e]
+ * points of its log wave grid (wave_grid, concatenated) and the
+ * partial-wave differences diffwave = r (phi - phi~) on it (projector.c:
+ * 84-99; [nchan][wave_n] per species, concatenated).  A site's sphere
+ * holds the box points (density.c:266-284) within the last point of its
+ * wave grid; the table of (diffwave/r) Y_lm per point and projector is
+ * built once.  Projectors of a site: channels in order, m = -l..l. */
+int pp_rs_plan_create(pp_ctx *ctx, pp_rsplan **out, const double *lattice,
+                      const int *fftg, int num_sites, const int *labels,
+                      const double *frac_coords, int num_elems,
+                      const int *nchan, const int *ls, const int *wave_n,
+                      const double *wave_grid, const double *diffwave);
+void pp_rs_plan_destroy(pp_rsplan *plan);
+/* total projectors, sphere points, launch colours (sites whose spheres
+ * may share a grid point never share one), whether the plan adds with
+ * atomics (a site box wraps onto itself), per-site projector offsets
+ * (proj_off[num_sites + 1]); pointers may be NULL. */
+int pp_rs_plan_info(pp_rsplan *plan, int *total_projs, long *total_points,
+                    int *ncolors, int *atomic, int *proj_off);
+/* realspace_state / ae_state_density / ae_chg_density for nband bands of
+ * one k-point (density.c:28-39, 158-179, 232-310): scatter + FFT of
+ * C_dev (c64 [nband][ldc], G_dev int32 [nw][3]) on the plan's grid, the
+ * Bloch phase (omitted with remove_phase, density.c:312-326) and the
+ * partial-wave differences weighted by proj_dev (c128 [nband][ldp], the
+ * plan's projector order).  Outputs (either may be NULL): states_dev c128
+ * [nband][ldo] (x slowest) and density_dev f64 [N^3] += sum_b weights[b]
+ * |psi_b|^2 (weights: host [nband]).  precision: PP_FP64 or PP_FP32 FFT. */
+int pp_realspace_batched(pp_ctx *ctx, pp_rsplan *plan, const double *kfrac,
+                         int nw, const int *G_dev, const void *C_dev, long ldc,
+                         int nband, const void *proj_dev, long ldp,
+                         int remove_phase, void *states_dev, long ldo,
+                         const double *weights, double *density_dev,
+                         int precision);
 
 #ifdef __cplusplus
 }

@@ -1203,3 +1203,121 @@ void or_aug_freqs(double *CA, const double *lattice, const double *kfrac, const 
   }
   free(x);
 }
+
+/* ------------------------------------------------------------------ */
+/* All-electron real-space states (density.c)                          */
+/* ------------------------------------------------------------------ */
+
+/* utils.c:522-545 (wave_value2): (f(|pos|) / |pos|) Y_lm(pos) with f the
+ * log-grid spline (r below the first grid point divides by that point). */
+void or_wave_value2(int n, const double *x, const double *f, const double *b,
+                    const double *c, const double *d, int l, int m,
+                    const double *pos, double *out2) {
+  double r = norm3(pos);
+  double radial = or_wave_interpolate(r, n, x, f, b, c, d);
+  radial /= (r < x[0]) ? x[0] : r;
+  double y[2];
+  if (r == 0.0) {
+    or_ylm(l, m, 0.0, 0.0, y);
+  } else {
+    double theta = acos(pos[2] / r);
+    double phi = 0.0;
+    if (r - fabs(pos[2]) != 0.0)
+      phi = acos(pos[0] / sqrt(pos[0] * pos[0] + pos[1] * pos[1]));
+    if (pos[1] < 0)
+      phi = 2.0 * OR_PI - phi;
+    or_ylm(l, m, theta, phi, y);
+  }
+  out2[0] = radial * y[0];
+  out2[1] = radial * y[1];
+}
+
+/* density.c:232-310 (realspace_state) after its fft3d: x (c128 [N^3],
+ * x slowest) holds the transformed pseudo state on entry.  Each point is
+ * multiplied by e^{2 pi i k.frac} (:242-255); then every site s adds, at
+ * the points of its box (site_box around round(coord N)) with
+ * |r - R| < rmax_s (the last point of its wave grid; r the unwrapped box
+ * point, :277-284), sum_n wave_value2(diffwave of channel n, l_n, m_n,
+ * r - R) proj[proj_off[s] + n] e^{2 pi i k.(R + T)} with T the lattice
+ * vector that wraps the point into the cell (:285-303).  The projectors
+ * of a site run over its channels, m = -l..l within each.
+ * Species e: n_e = wn[e] wave-grid points at wg + woff[e]; channels
+ * chan_off[e] .. chan_off[e+1] with l = chan_l[ch] and diffwave spline
+ * f, b, c, d at (long)ch * wmax (wmax >= every n_e). */
+void or_realspace_state(double *x, const double *lattice, const int *fftg,
+                        const double *kfrac, int nsites, const int *labels,
+                        const double *coords, const int *wn, const long *woff,
+                        const double *wg, const int *chan_off,
+                        const int *chan_l, int wmax, const double *f,
+                        const double *b, const double *c, const double *d,
+                        const int *proj_off, const double *proj) {
+  const int n0 = fftg[0], n1 = fftg[1], n2 = fftg[2];
+  for (int i = 0; i < n0; i++)
+    for (int j = 0; j < n1; j++)
+      for (int k = 0; k < n2; k++) {
+        double kr = kfrac[0] * i / n0 + kfrac[1] * j / n1 + kfrac[2] * k / n2;
+        double ph[2] = {cos(2.0 * OR_PI * kr), sin(2.0 * OR_PI * kr)};
+        double *v = x + 2 * (((long)i * n1 + j) * n2 + k);
+        double re = v[0] * ph[0] - v[1] * ph[1];
+        double im = v[0] * ph[1] + v[1] * ph[0];
+        v[0] = re;
+        v[1] = im;
+      }
+  for (int s = 0; s < nsites; s++) {
+    const int e = labels[s];
+    const int n = wn[e];
+    const double *xg = wg + woff[e];
+    const double rmax = xg[n - 1];
+    const double *R = coords + 3 * s;
+    int g[3];
+    site_box(lattice, fftg, rmax, g);
+    int c0 = (int)round(R[0] * n0), c1 = (int)round(R[1] * n1), c2 = (int)round(R[2] * n2);
+    for (int i = c0 - g[0]; i <= c0 + g[0]; i++)
+      for (int j = c1 - g[1]; j <= c1 + g[1]; j++)
+        for (int k = c2 - g[2]; k <= c2 + g[2]; k++) {
+          double fv[3] = {(double)i / n0 - R[0], (double)j / n1 - R[1], (double)k / n2 - R[2]};
+          double cv[3];
+          frac_to_cart(fv, lattice, cv);
+          if (!(norm3(cv) < rmax))
+            continue;
+          int ii = (i % n0 + n0) % n0, jj = (j % n1 + n1) % n1, kk = (k % n2 + n2) % n2;
+          /* (ii - i) / N is exact: the wrap is a whole number of cells */
+          double pc[3] = {R[0] + (ii - i) / n0, R[1] + (jj - j) / n1, R[2] + (kk - k) / n2};
+          double phase = kfrac[0] * pc[0] + kfrac[1] * pc[1] + kfrac[2] * pc[2];
+          double e2[2] = {cos(2.0 * OR_PI * phase), sin(2.0 * OR_PI * phase)};
+          double acc[2] = {0.0, 0.0};
+          int p = proj_off[s];
+          for (int ch = chan_off[e]; ch < chan_off[e + 1]; ch++) {
+            int l = chan_l[ch];
+            long o = (long)ch * wmax;
+            for (int m = -l; m <= l; m++, p++) {
+              double w[2];
+              or_wave_value2(n, xg, f + o, b + o, c + o, d + o, l, m, cv, w);
+              double pr = proj[2 * p], pi = proj[2 * p + 1];
+              double t0 = w[0] * pr - w[1] * pi, t1 = w[0] * pi + w[1] * pr;
+              acc[0] += t0 * e2[0] - t1 * e2[1];
+              acc[1] += t0 * e2[1] + t1 * e2[0];
+            }
+          }
+          double *v = x + 2 * (((long)ii * n1 + jj) * n2 + kk);
+          v[0] += acc[0];
+          v[1] += acc[1];
+        }
+  }
+}
+
+/* density.c:312-326 (remove_phase): x *= e^{-2 pi i k.frac}. */
+void or_remove_phase(double *x, const int *fftg, const double *kfrac) {
+  const int n0 = fftg[0], n1 = fftg[1], n2 = fftg[2];
+  for (int i = 0; i < n0; i++)
+    for (int j = 0; j < n1; j++)
+      for (int k = 0; k < n2; k++) {
+        double kr = kfrac[0] * i / n0 + kfrac[1] * j / n1 + kfrac[2] * k / n2;
+        double ph[2] = {cos(2.0 * OR_PI * kr), -sin(2.0 * OR_PI * kr)};
+        double *v = x + 2 * (((long)i * n1 + j) * n2 + k);
+        double re = v[0] * ph[0] - v[1] * ph[1];
+        double im = v[0] * ph[1] + v[1] * ph[0];
+        v[0] = re;
+        v[1] = im;
+      }
+}

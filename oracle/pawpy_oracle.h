@@ -155,6 +155,19 @@ void or_aug_freqs(double *CA, const double *lattice, const double *kfrac, const 
                   int nsites, const int *site_pt, const int *site_nproj, const long *val_off, const int *idx,
                   const double *path, const double *values, const double *P, const int *proj_off_P);
 
+/* ---- all-electron real-space states: density.c:232-327, utils.c:522-545 ---- */
+void or_wave_value2(int n, const double *x, const double *f, const double *b,
+                    const double *c, const double *d, int l, int m,
+                    const double *pos, double *out2);
+void or_realspace_state(double *x, const double *lattice, const int *fftg,
+                        const double *kfrac, int nsites, const int *labels,
+                        const double *coords, const int *wn, const long *woff,
+                        const double *wg, const int *chan_off,
+                        const int *chan_l, int wmax, const double *f,
+                        const double *b, const double *c, const double *d,
+                        const int *proj_off, const double *proj);
+void or_remove_phase(double *x, const int *fftg, const double *kfrac);
+
 #ifdef __cplusplus
 }
 #endif

@@ -467,3 +467,65 @@ def aug_freqs(lattice, fftg, kfrac, G, T: SiteTables, P, proj_off_P=None):
                        _c(T.val_off, np.int64).ctypes.data_as(ct.POINTER(ct.c_long)), _p(_c(T.idx, np.int32), _i),
                        _p(_c(T.path, np.float64), _d), _p(vals, _d), _p(P, _d), _p(po, _i))
     return CA
+
+
+# ---------------------------------------------------------------- real-space states
+def wave_value2(x, f, coeffs, l, m, pos):
+    """utils.c:522-545 for one channel spline (x, f, (b, c, d))."""
+    x, f = _c(x, np.float64), _c(f, np.float64)
+    b, c, d = (_c(v, np.float64) for v in coeffs)
+    out = np.zeros(2)
+    lib().or_wave_value2(ct.c_int(len(x)), _p(x, _d), _p(f, _d), _p(b, _d), _p(c, _d), _p(d, _d), ct.c_int(l),
+                         ct.c_int(m), _p(_c(pos, np.float64), _d), _p(out, _d))
+    return complex(out[0], out[1])
+
+
+class WaveData:
+    """Per-species diffwave splines (projector.c:84-112: diffwave = ae - ps on
+    the log wave grid) packed for ``or_realspace_state``."""
+
+    def __init__(self, species, diffwaves=None):
+        self.wn = np.array([len(sp.grid) for sp in species], np.int32)
+        self.woff = np.concatenate([[0], np.cumsum(self.wn)[:-1]]).astype(np.int64)
+        self.wg = np.concatenate([np.asarray(sp.grid, np.float64) for sp in species])
+        self.chan_off = np.concatenate([[0], np.cumsum([len(sp.ls) for sp in species])]).astype(np.int32)
+        self.chan_l = np.concatenate([np.asarray(sp.ls, np.int32) for sp in species]).astype(np.int32)
+        self.wmax = int(self.wn.max())
+        nch = int(self.chan_off[-1])
+        self.f, self.b, self.c, self.d = (np.zeros((nch, self.wmax)) for _ in range(4))
+        ch = 0
+        for e, sp in enumerate(species):
+            for q in range(len(sp.ls)):
+                fq = (np.asarray(diffwaves[e][q], np.float64) if diffwaves is not None else
+                      np.asarray(sp.aewaves[q], np.float64) - np.asarray(sp.pswaves[q], np.float64))
+                n = len(sp.grid)
+                self.f[ch, :n] = fq
+                bb, cc, dd = spline_coeff(sp.grid, fq)
+                self.b[ch, :n], self.c[ch, :n], self.d[ch, :n] = bb, cc, dd
+                ch += 1
+        self.proj_counts = [sum(2 * l + 1 for l in sp.ls) for sp in species]
+
+
+def realspace_state(lattice, fftg, kfrac, G, C, W: WaveData, labels, coords, proj, remove_phase=False):
+    """density.c:232-310 for one band: fft3d of the c64 coefficients, the
+    Bloch phase, and the partial-wave differences weighted by ``proj``
+    (c128 [ptot], sites in order, projectors channel-major, m = -l..l).
+    With ``remove_phase`` the result is multiplied by e^{-2 pi i k.r}
+    afterwards (density.c:312-326)."""
+    labels = _c(labels, np.int32)
+    coords = _c(coords, np.float64).reshape(-1, 3)
+    nsites = len(labels)
+    counts = [W.proj_counts[e] for e in labels]
+    proj_off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+    proj = _c(proj, np.complex128)
+    assert len(proj) >= proj_off[-1]
+    x = np.ascontiguousarray(fft3d(lattice, G, C, fftg)).ravel()
+    fg = _c(fftg, np.int32)
+    kf = _c(kfrac, np.float64)
+    lib().or_realspace_state(_p(x, _d), _p(_c(lattice, np.float64), _d), _p(fg, _i), _p(kf, _d),
+                             ct.c_int(nsites), _p(labels, _i), _p(coords, _d), _p(W.wn, _i), _p(W.woff, _l),
+                             _p(W.wg, _d), _p(W.chan_off, _i), _p(W.chan_l, _i), ct.c_int(W.wmax), _p(W.f, _d),
+                             _p(W.b, _d), _p(W.c, _d), _p(W.d, _d), _p(proj_off, _i), _p(proj, _d))
+    if remove_phase:
+        lib().or_remove_phase(_p(x, _d), _p(fg, _i), _p(kf, _d))
+    return x.reshape(tuple(fg))

@@ -74,6 +74,21 @@ SIGNATURES = {
                                         ct.c_long, c_int_p, ct.c_int, c_void_p, ct.c_long, ct.c_int]),
     "fft3d": (ct.c_int, [c_void_p, c_int_p, c_double_p, c_double_p, c_int_p, c_void_p, ct.c_int, c_int_p]),
     "pp_get_projections": (ct.c_int, [c_void_p, ct.c_int, ct.c_int, c_void_p]),
+    "realspace_state": (ct.c_int, [c_void_p, ct.c_int, ct.c_int, c_void_p, c_int_p, c_int_p, c_double_p]),
+    "ncl_realspace_state": (ct.c_int, [c_void_p, ct.c_int, ct.c_int, c_void_p, c_int_p, c_int_p, c_double_p]),
+    "remove_phase": (ct.c_int, [c_void_p, ct.c_int, c_void_p, c_int_p]),
+    "ae_state_density": (ct.c_int, [c_double_p, ct.c_int, ct.c_int, c_void_p, c_int_p, c_int_p, c_double_p]),
+    "ae_chg_density": (ct.c_int, [c_double_p, c_void_p, c_int_p, c_int_p, c_double_p]),
+    "ncl_ae_chg_density": (ct.c_int, [c_double_p, c_void_p, c_int_p, c_int_p, c_double_p]),
+    "project_realspace_state": (ct.c_int, [c_void_p, ct.c_int, c_void_p, c_void_p, c_int_p, c_int_p, c_double_p,
+                                           c_int_p, c_double_p]),
+    "pp_rs_plan_create": (ct.c_int, [c_void_p, ct.POINTER(c_void_p), c_double_p, c_int_p, ct.c_int, c_int_p,
+                                     c_double_p, ct.c_int, c_int_p, c_int_p, c_int_p, c_double_p, c_double_p]),
+    "pp_rs_plan_destroy": (None, [c_void_p]),
+    "pp_rs_plan_info": (ct.c_int, [c_void_p, c_int_p, ct.POINTER(ct.c_long), c_int_p, c_int_p, c_int_p]),
+    "pp_realspace_batched": (ct.c_int, [c_void_p, c_void_p, c_double_p, ct.c_int, c_void_p, c_void_p, ct.c_long,
+                                        ct.c_int, c_void_p, ct.c_long, ct.c_int, c_void_p, ct.c_long, c_double_p,
+                                        c_void_p, ct.c_int]),
     "pp_ctx_create": (ct.c_int, [ct.POINTER(c_void_p), ct.c_int]),
     "pp_ctx_destroy": (None, [c_void_p]),
     "pp_ctx_set_stream": (ct.c_int, [c_void_p, c_void_p]),

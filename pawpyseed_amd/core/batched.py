@@ -300,3 +300,90 @@ def fft3d(ctx: Context, lattice, fftg, G_dev, C_dev, out, precision="fp64", sync
     check(lib().pp_fft3d_batched(ctx.h, ptr(lat, c_double_p), ptr(fg, c_int_p), G_dev.shape[0], _dptr(G_dev),
                                  _dptr(C_dev), ldc, nband, _dptr(out), _prec(precision)), "pp_fft3d_batched")
     _done(ctx, sync)
+
+
+def wave_arrays(species_list, diffwaves=None):
+    """Flatten species (objects with ls, grid, aewaves, pswaves) into the
+    pp_rs_plan_create arrays: per species its log wave grid and the
+    partial-wave differences ae - ps of every channel (projector.c:84-99),
+    or the given ``diffwaves[e][ch]``."""
+    nchan = as_i32([len(sp.ls) for sp in species_list])
+    ls = as_i32([l for sp in species_list for l in sp.ls])
+    wn = as_i32([len(sp.grid) for sp in species_list])
+    wg = as_f64(np.concatenate([np.asarray(sp.grid, np.float64) for sp in species_list]))
+    dw = []
+    for e, sp in enumerate(species_list):
+        for ch in range(len(sp.ls)):
+            dw.append(np.asarray(diffwaves[e][ch], np.float64) if diffwaves is not None else
+                      np.asarray(sp.aewaves[ch], np.float64) - np.asarray(sp.pswaves[ch], np.float64))
+    return nchan, ls, wn, wg, as_f64(np.concatenate(dw) if dw else np.zeros(1))
+
+
+class RealspacePlan:
+    """pp_rsplan: partial-wave spheres of one structure on a real-space
+    grid (the site loops of realspace_state, density.c:259-309)."""
+
+    def __init__(self, ctx: Context, lattice, fftg, frac_coords, labels, species_list, diffwaves=None):
+        self.ctx = ctx
+        lat = as_f64(lattice).ravel()
+        self.fftg = tuple(int(n) for n in fftg)
+        fg = as_i32(fftg)
+        fr = as_f64(frac_coords).reshape(-1, 3)
+        lab = as_i32(labels)
+        nchan, ls, wn, wg, dw = wave_arrays(species_list, diffwaves)
+        h = ct.c_void_p()
+        check(lib().pp_rs_plan_create(ctx.h, ct.byref(h), ptr(lat, c_double_p), ptr(fg, c_int_p), len(fr),
+                                      ptr(lab, c_int_p), ptr(fr, c_double_p), len(species_list), ptr(nchan, c_int_p),
+                                      ptr(ls, c_int_p), ptr(wn, c_int_p), ptr(wg, c_double_p), ptr(dw, c_double_p)),
+              "pp_rs_plan_create")
+        self.h = h
+        self.nsites = len(fr)
+        tp, nc, at = ct.c_int(), ct.c_int(), ct.c_int()
+        npt = ct.c_long()
+        off = np.zeros(self.nsites + 1, np.int32)
+        check(lib().pp_rs_plan_info(h, ct.byref(tp), ct.byref(npt), ct.byref(nc), ct.byref(at), ptr(off, c_int_p)),
+              "pp_rs_plan_info")
+        self.ptot, self.npts, self.ncolors, self.atomic = tp.value, npt.value, nc.value, bool(at.value)
+        self.proj_off = off
+
+    @property
+    def gridsize(self):
+        return int(np.prod(self.fftg))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pp_rs_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def realspace_states(ctx: Context, plan: RealspacePlan, kfrac, G_dev, C_dev, proj_dev, states_out=None,
+                     density_out=None, weights=None, remove_phase=False, precision="fp64", sync=True):
+    """All-electron states and/or weighted density of the bands of one
+    k-point (density.c:28-39, 158-179, 232-326).  C_dev complex64 [nband,
+    ldc], G_dev int32 [W, 3] (W <= ldc), proj_dev complex128 [nband, ldp]
+    in the plan's projector order; states_out complex128 [nband, N0, N1,
+    N2]; density_out float64 [N0, N1, N2] += sum_b weights[b] |psi_b|^2."""
+    nband, ldc = C_dev.shape
+    kf = as_f64(kfrac).ravel()
+    ldp = proj_dev.shape[1] if proj_dev is not None else 1
+    w = None
+    if density_out is not None:
+        w = as_f64(np.ones(nband) if weights is None else weights)
+        assert len(w) == nband
+    if states_out is not None:
+        assert states_out.shape[0] == nband and states_out[0].numel() == plan.gridsize
+    check(lib().pp_realspace_batched(ctx.h, plan.h, ptr(kf, c_double_p), G_dev.shape[0], _dptr(G_dev),
+                                     _dptr(C_dev), ldc, nband,
+                                     _dptr(proj_dev) if proj_dev is not None else None, ldp,
+                                     1 if remove_phase else 0,
+                                     _dptr(states_out) if states_out is not None else None, plan.gridsize,
+                                     ptr(w, c_double_p) if w is not None else None,
+                                     _dptr(density_out) if density_out is not None else None,
+                                     _prec(precision)), "pp_realspace_batched")
+    _done(ctx, sync)

@@ -2,12 +2,14 @@
 
 Mirrors pawpyseed/core/projector.py (Projector, make_site_lists,
 setup_overlap, single_band_projection, proportion_conduction,
-defect_band_analysis) with the 'aug_real', 'aug_recip' and 'pseudo' methods.  The
+defect_band_analysis) with the 'aug_real', 'aug_recip', 'realspace' and
+'pseudo' methods.  The
 pseudo-wavefunction overlap and every augmentation term run in
 libpawpy_amd.so: the on-site O_M term, the O_R/O_S terms of sites that
 differ between the structures and the off-site O_N term of unmatched sites
 with overlapping augmentation spheres (overlap_setup_real,
-compensation_terms).
+compensation_terms).  'realspace' integrates both all-electron states on a
+real-space grid (project_realspace_state).
 """
 
 from __future__ import annotations
@@ -23,7 +25,7 @@ def _arr(a):
 
 
 class Projector:
-    METHODS = ["pseudo", "aug_real", "aug_recip"]
+    METHODS = ["pseudo", "realspace", "aug_recip", "aug_real"]
 
     def __init__(self, wf, basis, unsym_basis=False, unsym_wf=False, method="aug_real", site_lists=None):
         self.method = method
@@ -34,7 +36,7 @@ class Projector:
         elif method == "aug_recip":
             self._single_band_projection = self._single_band_projection_aug_recip
         elif method == "realspace":
-            raise PAWpyError(f"method {method} is not provided by this build")
+            self._single_band_projection = self._single_band_projection_realspace
         else:
             raise PAWpyError("method not recognized for Projector")
         if wf.ncl or basis.ncl:
@@ -54,6 +56,7 @@ class Projector:
         if method != "pseudo":
             basis.check_c_projectors()
             wf.check_c_projectors()
+        if "aug" in method:
             self.setup_overlap()
 
     def make_site_lists(self):
@@ -104,6 +107,28 @@ class Projector:
 
     def _single_band_projection_pseudo(self, band_num, flip_spin=False):
         return self.wf.pseudoprojection(band_num, self.basis, flip_spin)
+
+    def _single_band_projection_realspace(self, band_num, dim=None):
+        """projector.py:199-208: both all-electron states on a real-space
+        grid (default: twice the wavefunction's grid) and their overlaps
+        V/N^3 sum_r conj(psi_R) psi_S (project_realspace_state,
+        density.c:205-230, on the GPU)."""
+        if dim is None:
+            dim = self.wf.dim * 2
+        return self._realspace_projection(band_num, dim)
+
+    def _realspace_projection(self, band_num, dim):
+        """pawpyc.pyx:723-735 (with the grid passed as int32)."""
+        import ctypes as ct
+
+        from ._lib import c_double_p, check, lib, ptr
+        res = np.zeros(self.basis.nband * self.basis.nwk * self.basis.nspin, np.complex128)
+        dimv = np.ascontiguousarray(self.wf.dimv if dim is None else dim, np.int32)
+        check(lib().project_realspace_state(res.ctypes.data_as(ct.c_void_p), int(band_num), self.wf.wf_ptr,
+                                            self.basis.wf_ptr, ptr(dimv, _int_p), ptr(self.wf.nums, _int_p),
+                                            ptr(self.wf.coords, c_double_p), ptr(self.basis.nums, _int_p),
+                                            ptr(self.basis.coords, c_double_p)), "project_realspace_state")
+        return res
 
     def _single_band_projection_aug_real(self, band_num, flip_spin=False):
         res = self.wf.pseudoprojection(band_num, self.basis, flip_spin)

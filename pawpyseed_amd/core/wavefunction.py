@@ -224,7 +224,7 @@ class Wavefunction:
             raise PAWpyError("get_projector_list failed: " + L.pp_last_error().decode(errors="replace"))
         self.nums = nums
         self.coords = coords
-        self.dimv = as_i32(self.dim)
+        self.update_dimv(self.dim)
         if getattr(self, "_paw_override", None) is not None:
             self._inject_paw_overlaps(pl, pps)
         # setup_projections takes ownership of the ppot list (projector.c:570)
@@ -273,6 +273,146 @@ class Wavefunction:
         labels = self._element_labels()
         return sum(sum(2 * l + 1 for l in self.cr.pps[s].ls) for s in self.structure.species) if labels else 0
 
+    # ------------------------------------------------------------ real space
+    def update_dim(self, dim):
+        """wavefunction.py:245-247: grid of later real-space calls."""
+        self.dim = np.array(dim, dtype=np.int32)
+        self.update_dimv(dim)
+
+    def update_dimv(self, dim):
+        """pawpyc.pyx:414-419: the state grid and the doubled density grid."""
+        self.dimv = as_i32(dim)
+        self.fdimv = as_i32(np.asarray(dim) * 2)
+        self.gridsize = int(np.prod(self.dimv))
+        self.fgridsize = int(np.prod(self.fdimv))
+
+    def get_state_realspace(self, b, k, s, dim=None, remove_phase=False):
+        """All-electron state of band b, k-point k, spin s on the grid dim
+        (default: the projection grid) as a complex array [N0, N1, N2]
+        (wavefunction.py:431-447, density.c:232-310, on the GPU)."""
+        self.check_c_projectors()
+        if dim is not None:
+            self.update_dim(np.array(dim))
+        return self._get_realspace_state(b, k, s, remove_phase)
+
+    def get_state_realspace_density(self, b, k, s, dim=None):
+        """|psi|^2 of one all-electron state on the doubled grid (dim // 2 * 2)
+        (wavefunction.py:449-465, density.c:28-39)."""
+        self.check_c_projectors()
+        if dim is not None:
+            self.update_dim(np.array(dim) // 2)
+        return self._get_realspace_state_density(b, k, s)
+
+    def get_realspace_density(self, dim=None, bands=None):
+        """All-electron charge density sum_k,b w_k occ (2 / nspin) |psi|^2 on
+        the doubled grid (wavefunction.py:467-479, density.c:158-179).  The
+        reference accepts ``bands`` here without using it; this one passes
+        it on (an int or a list: only those bands, weighted w_k / nspin, as
+        pawpyc.pyx:462-482 does for the files)."""
+        self.check_c_projectors()
+        if dim is not None:
+            self.update_dim(np.array(dim) // 2)
+        return self._get_realspace_density(bands)
+
+    def _rs_check(self, b, k, s):
+        if b < 0 or b >= self.nband:
+            raise ValueError("Invalid band choice")
+        if k < 0 or k >= self.nwk:
+            raise ValueError("Invalid k-point choice")
+        if s < 0 or s >= self.nspin:
+            raise ValueError("Invalid spin choice")
+
+    def _get_realspace_state(self, b, k, s, remove_phase=False):
+        """pawpyc.pyx:425-439."""
+        self._rs_check(b, k, s)
+        res = np.zeros(self.gridsize, np.complex128)
+        L = lib()
+        check(L.realspace_state(res.ctypes.data_as(ct.c_void_p), b, k + s * self.nwk, self.wf_ptr,
+                                ptr(self.dimv, c_int_p), ptr(self.nums, c_int_p), ptr(self.coords, c_double_p)),
+              "realspace_state")
+        if remove_phase:
+            check(L.remove_phase(res.ctypes.data_as(ct.c_void_p), k + s * self.nwk, self.wf_ptr,
+                                 ptr(self.dimv, c_int_p)), "remove_phase")
+        return res.reshape(tuple(self.dimv))
+
+    def _get_realspace_state_density(self, b, k, s):
+        """pawpyc.pyx:441-453."""
+        self._rs_check(b, k, s)
+        res = np.zeros(self.fgridsize, np.float64)
+        check(lib().ae_state_density(ptr(res, c_double_p), b, k + s * self.nwk, self.wf_ptr,
+                                     ptr(self.fdimv, c_int_p), ptr(self.nums, c_int_p),
+                                     ptr(self.coords, c_double_p)), "ae_state_density")
+        return res.reshape(tuple(self.fdimv))
+
+    def _get_realspace_density(self, bands=None):
+        """pawpyc.pyx:455-494."""
+        res = np.zeros(self.fgridsize, np.float64)
+        L = lib()
+        args = (self.wf_ptr, ptr(self.fdimv, c_int_p), ptr(self.nums, c_int_p), ptr(self.coords, c_double_p))
+        if bands is None:
+            check(L.ae_chg_density(ptr(res, c_double_p), *args), "ae_chg_density")
+        else:
+            blist = [bands] if isinstance(bands, (int, np.integer)) else list(bands)
+            for b in blist:
+                if b < 0 or b >= self.nband:
+                    raise ValueError("Invalid band choice")
+                for k in range(self.nwk * self.nspin):
+                    work = np.zeros(self.fgridsize, np.float64)
+                    check(L.ae_state_density(ptr(work, c_double_p), int(b), k, *args), "ae_state_density")
+                    res += work * self.kws[k % self.nwk] / self.nspin
+        return res.reshape(tuple(self.fdimv))
+
+    def _convert_to_vasp_volumetric(self, filename, dim):
+        """wavefunction.py:480-510: a POSCAR-style header and the grid size
+        in front of the values."""
+        latt = np.asarray(self.structure.lattice.matrix, np.float64)
+        lines = filename + "\n" + "   1.00000000000000\n"
+        for r in range(3):
+            lines += " %12.6f%12.6f%12.6f\n" % tuple(latt[r])
+        syms, counts = [], []
+        for sp in self.structure.species:
+            if syms and syms[-1] == str(sp):
+                counts[-1] += 1
+            else:
+                syms.append(str(sp))
+                counts.append(1)
+        lines += "".join("%5s" % x for x in syms) + "\n"
+        lines += "".join("%6d" % x for x in counts) + "\n"
+        lines += "Direct\n"
+        for fc in self.structure.frac_coords:
+            lines += "%10.6f%10.6f%10.6f\n" % tuple(fc)
+        lines += " \n"
+        with open(filename) as f:
+            nums = f.read()
+        with open(filename, "w") as f:
+            f.write(lines + "%d %d %d\n" % (dim[0], dim[1], dim[2]) + nums)
+
+    def write_state_realspace(self, b, k, s, fileprefix="", dim=None, scale=1, remove_phase=False):
+        """Real and imaginary parts of one state, in two VASP-style
+        volumetric files (wavefunction.py:512-549)."""
+        self.check_c_projectors()
+        if dim is not None:
+            self.update_dim(np.array(dim))
+        base = "%sB%dK%dS%d" % (fileprefix, b, k, s)
+        f1, f2 = "%s_REAL.vasp" % base, "%s_IMAG.vasp" % base
+        res = self._get_realspace_state(b, k, s, remove_phase)
+        write_volumetric(f1, np.real(res), self.dimv, scale)
+        write_volumetric(f2, np.imag(res), self.dimv, scale)
+        self._convert_to_vasp_volumetric(f1, self.dim)
+        self._convert_to_vasp_volumetric(f2, self.dim)
+        return res
+
+    def write_density_realspace(self, filename="PYAECCAR.vasp", dim=None, scale=1, bands=None):
+        """The all-electron charge density in a VASP-style volumetric file
+        (wavefunction.py:551-580)."""
+        self.check_c_projectors()
+        if dim is not None:
+            self.update_dim(np.array(dim) // 2)
+        res = self._get_realspace_density(bands)
+        write_volumetric(filename, res, self.fdimv, scale)
+        self._convert_to_vasp_volumetric(filename, self.dim * 2)
+        return res
+
     # ------------------------------------------------------------ overlaps
     def pseudoprojection(self, band_num, basis, flip_spin=False):
         """<psibt_n1k|psit_n2k> for all n1, k of basis and n2 = band_num of
@@ -313,6 +453,18 @@ class Wavefunction:
             self.free()
         except Exception:
             pass
+
+
+def write_volumetric(filename, x, dim, scale):
+    """density.c:461-477: values x[i][j][k] * scale, z slowest, "%E   " each,
+    five to a line."""
+    v = np.asarray(x, np.float64).reshape(tuple(int(n) for n in dim)).transpose(2, 1, 0).ravel() * scale
+    with open(filename, "w") as f:
+        for r in range(0, len(v), 5):
+            row = v[r:r + 5]
+            f.write("".join("%E   " % t for t in row))
+            if len(row) == 5:
+                f.write("\n")
 
 
 def wavefunction_from_workload(wl, which="R", nband=None, coeffs=None, setup_projectors=False, element="C"):

@@ -61,6 +61,14 @@ struct WfDev {
   long ca_ld = 0;
   int ca_n = -1;
   unsigned long long ca_hash = 0;
+  // real-space states: the partial-wave sphere plan of the last
+  // (grid, labels, coords) asked for
+  pp_rsplan *rs = nullptr;
+  unsigned long long rs_hash = 0;
+  ~WfDev() {
+    if (rs)
+      pp_rs_plan_destroy(rs);
+  }
 };
 
 static std::mutex g_mu;
@@ -1340,6 +1348,233 @@ int fft3d(pp_cdouble *x, int *G_bounds, double *lattice, double *kpt, int *Gs, p
                           g_precision == PP_FP64 ? PP_FP64 : PP_FP32));
   PP_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   PP_CHECK_HIP(hipMemcpy(x, dX.p, sizeof(pp_cdouble) * npts, hipMemcpyDeviceToHost));
+  return PP_OK;
+}
+
+
+// ---- all-electron real-space states (density.c) ----
+
+}  // extern "C"
+
+namespace pp {
+
+// The partial-wave sphere plan of wf on grid fftg for (labels, coords),
+// cached on the wavefunction; its projector layout must be the one of the
+// projections of setup_projections.
+static int rs_plan_for(pp_ctx *ctx, pswf_t *wf, const int *fftg, const int *labels, const double *coords,
+                       pp_rsplan **out) {
+  WfDev *d = dev_of(wf);
+  PP_REQUIRE(wf->pps && d->proj_precision >= 0, PP_ERR_ARG, "realspace: no projections (call setup_projections)");
+  const int ns = wf->num_sites;
+  for (int s = 0; s < ns; s++)
+    PP_REQUIRE(labels[s] >= 0 && labels[s] < wf->num_elems, PP_ERR_ARG, "realspace: label out of range");
+  unsigned long long h = 1469598103934665603ull;
+  h = hash_ints(h, fftg, 3);
+  h = hash_ints(h, labels, ns);
+  h = hash_ints(h, (const int *)coords, 6 * ns);
+  const void *pps = wf->pps;
+  h = hash_ints(h, (const int *)&pps, (int)(sizeof(pps) / sizeof(int)));
+  if (d->rs && d->rs_hash == h) {
+    *out = d->rs;
+    return PP_OK;
+  }
+  std::vector<int> nchan, ls, wn;
+  std::vector<double> wg, dw;
+  for (int e = 0; e < wf->num_elems; e++) {
+    const ppot_t &pp = wf->pps[e];
+    nchan.push_back(pp.num_projs);
+    wn.push_back(pp.wave_gridsize);
+    wg.insert(wg.end(), pp.wave_grid, pp.wave_grid + pp.wave_gridsize);
+    for (int ch = 0; ch < pp.num_projs; ch++) {
+      ls.push_back(pp.funcs[ch].l);
+      dw.insert(dw.end(), pp.funcs[ch].diffwave, pp.funcs[ch].diffwave + pp.wave_gridsize);
+    }
+  }
+  pp_rsplan *rp = nullptr;
+  PP_TRY(pp_rs_plan_create(ctx, &rp, wf->lattice, fftg, ns, labels, coords, wf->num_elems, nchan.data(), ls.data(),
+                           wn.data(), wg.data(), dw.data()));
+  std::vector<int> po(ns + 1, 0);
+  pp_rs_plan_info(rp, nullptr, nullptr, nullptr, nullptr, po.data());
+  if (po != d->proj_off) {
+    pp_rs_plan_destroy(rp);
+    PP_REQUIRE(false, PP_ERR_ARG, "realspace: the sites' projectors differ from those of setup_projections");
+  }
+  if (d->rs)
+    pp_rs_plan_destroy(d->rs);
+  d->rs = rp;
+  d->rs_hash = h;
+  *out = rp;
+  return PP_OK;
+}
+
+// States (states_dev c128 [nb][N^3]) and/or density (dens_dev +=
+// sum_b w_b |psi_b|^2) of bands b0 .. b0+nb-1 at k-index k, spinor
+// component comp, on the device.
+static int rs_run(pp_ctx *ctx, pswf_t *wf, pp_rsplan *rp, int k, int b0, int nb, int comp, int remove,
+                  void *states_dev, long ldo, const double *w, double *dens_dev) {
+  WfDev *d = dev_of(wf);
+  const int W = wf->is_ncl ? d->nw[k] / 2 : d->nw[k];
+  const pp_cfloat *C = (const pp_cfloat *)d->C.p + ((size_t)k * d->nband + b0) * d->ldc + (size_t)comp * W;
+  const DevBuf &pj = comp ? d->proj_dn : d->proj;
+  const pp_cdouble *P = (const pp_cdouble *)pj.p + ((size_t)k * d->nband + b0) * std::max(1, d->ptot);
+  return pp_realspace_batched(ctx, rp, wf->kpts[k]->k, W, (const int *)d->G.p + 3 * (size_t)d->gofs[k], C, d->ldc,
+                              nb, P, std::max(1, d->ptot), remove, states_dev, ldo, w, dens_dev,
+                              g_precision == PP_FP64 ? PP_FP64 : PP_FP32);
+}
+
+static int rs_check(pswf_t *wf, const int *fftg, const int *labels, const double *coords, const char *who) {
+  PP_REQUIRE(wf && fftg && (wf->num_sites == 0 || (labels && coords)), PP_ERR_ARG,
+             std::string(who) + ": bad arguments");
+  PP_REQUIRE(fftg[0] > 0 && fftg[1] > 0 && fftg[2] > 0, PP_ERR_ARG, std::string(who) + ": bad grid");
+  return PP_OK;
+}
+
+// one state (spinors: both components) into host x
+static int rs_state_host(pp_cdouble *x, int band, int k, pswf_t *wf, int *fftg, int *labels, double *coords,
+                         int ncomp, const char *who) {
+  PP_TRY(rs_check(wf, fftg, labels, coords, who));
+  PP_REQUIRE(x, PP_ERR_ARG, std::string(who) + ": null output");
+  PP_REQUIRE(band >= 0 && band < wf->nband && k >= 0 && k < wf->nwk * wf->nspin, PP_ERR_ARG,
+             std::string(who) + ": band or k-point index out of range");
+  std::lock_guard<std::mutex> lk(g_mu);
+  pp_ctx *ctx;
+  PP_TRY(global_ctx(&ctx));
+  pp_rsplan *rp;
+  PP_TRY(rs_plan_for(ctx, wf, fftg, labels, coords, &rp));
+  const long npts = (long)fftg[0] * fftg[1] * fftg[2];
+  DevBuf X;
+  PP_TRY(X.ensure(sizeof(pp_cdouble) * (size_t)npts * ncomp));
+  for (int c = 0; c < ncomp; c++)
+    PP_TRY(rs_run(ctx, wf, rp, k, band, 1, c, 0, (pp_cdouble *)X.p + (size_t)c * npts, npts, nullptr, nullptr));
+  PP_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  PP_CHECK_HIP(hipMemcpy(x, X.p, sizeof(pp_cdouble) * (size_t)npts * ncomp, hipMemcpyDeviceToHost));
+  return PP_OK;
+}
+
+// P += weighted densities of every k and the bands with occ > 0
+static int rs_density_host(double *P, pswf_t *wf, int *fftg, int *labels, double *coords, int only_band, int only_k,
+                           const char *who) {
+  PP_TRY(rs_check(wf, fftg, labels, coords, who));
+  PP_REQUIRE(P, PP_ERR_ARG, std::string(who) + ": null output");
+  std::lock_guard<std::mutex> lk(g_mu);
+  pp_ctx *ctx;
+  PP_TRY(global_ctx(&ctx));
+  pp_rsplan *rp;
+  PP_TRY(rs_plan_for(ctx, wf, fftg, labels, coords, &rp));
+  const long npts = (long)fftg[0] * fftg[1] * fftg[2];
+  DevBuf D;
+  PP_TRY(D.ensure(sizeof(double) * (size_t)npts));
+  PP_CHECK_HIP(hipMemcpy(D.p, P, sizeof(double) * npts, hipMemcpyHostToDevice));
+  const int ncomp = wf->is_ncl ? 2 : 1;
+  if (only_band >= 0) {
+    const double one = 1.0;
+    PP_TRY(rs_run(ctx, wf, rp, only_k, only_band, 1, 0, 0, nullptr, 0, &one, (double *)D.p));
+  } else {
+    // density.c:158-203: weight_k occ (2 / nspin; spinors 1) for occ > 0
+    const double spin_mult = wf->is_ncl ? 1.0 : 2.0 / wf->nspin;
+    std::vector<double> w(wf->nband);
+    for (int k = 0; k < wf->nwk * wf->nspin; k++) {
+      int nb = 0;
+      for (int b = 0; b < wf->nband; b++) {
+        const double occ = wf->kpts[k]->bands[b]->occ;
+        w[b] = occ > 0 ? wf->kpts[k]->weight * occ * spin_mult : 0.0;
+        if (occ > 0)
+          nb = b + 1;
+      }
+      for (int c = 0; c < ncomp; c++)
+        PP_TRY(rs_run(ctx, wf, rp, k, 0, nb, c, 0, nullptr, 0, w.data(), (double *)D.p));
+    }
+  }
+  PP_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  PP_CHECK_HIP(hipMemcpy(P, D.p, sizeof(double) * npts, hipMemcpyDeviceToHost));
+  return PP_OK;
+}
+
+}  // namespace pp
+
+extern "C" {
+
+int realspace_state(pp_cdouble *x, int BAND_NUM, int KPOINT_NUM, pswf_t *wf, int *fftg, int *labels,
+                    double *coords) {
+  PP_REQUIRE(wf && !wf->is_ncl, PP_ERR_ARG, "realspace_state: spinor wavefunction (use ncl_realspace_state)");
+  return rs_state_host(x, BAND_NUM, KPOINT_NUM, wf, fftg, labels, coords, 1, "realspace_state");
+}
+
+int ncl_realspace_state(pp_cdouble *x, int BAND_NUM, int KPOINT_NUM, pswf_t *wf, int *fftg, int *labels,
+                        double *coords) {
+  PP_REQUIRE(wf && wf->is_ncl, PP_ERR_ARG, "ncl_realspace_state: not a spinor wavefunction");
+  return rs_state_host(x, BAND_NUM, KPOINT_NUM, wf, fftg, labels, coords, 2, "ncl_realspace_state");
+}
+
+int remove_phase(pp_cdouble *x, int KPOINT_NUM, pswf_t *wf, int *fftg) {
+  PP_REQUIRE(x && wf && fftg && KPOINT_NUM >= 0 && KPOINT_NUM < wf->nwk * wf->nspin, PP_ERR_ARG,
+             "remove_phase: bad arguments");
+  std::lock_guard<std::mutex> lk(g_mu);
+  pp_ctx *ctx;
+  PP_TRY(global_ctx(&ctx));
+  const long npts = (long)fftg[0] * fftg[1] * fftg[2];
+  DevBuf X;
+  PP_TRY(X.ensure(sizeof(pp_cdouble) * (size_t)npts));
+  PP_CHECK_HIP(hipMemcpy(X.p, x, sizeof(pp_cdouble) * npts, hipMemcpyHostToDevice));
+  PP_TRY(rs_remove_phase(ctx, (cplx<double> *)X.p, fftg, wf->kpts[KPOINT_NUM]->k));
+  PP_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  PP_CHECK_HIP(hipMemcpy(x, X.p, sizeof(pp_cdouble) * npts, hipMemcpyDeviceToHost));
+  return PP_OK;
+}
+
+int ae_state_density(double *P, int BAND_NUM, int KPOINT_NUM, pswf_t *wf, int *fftg, int *labels, double *coords) {
+  PP_REQUIRE(wf && !wf->is_ncl, PP_ERR_ARG, "ae_state_density: spinor wavefunction");
+  PP_REQUIRE(BAND_NUM >= 0 && BAND_NUM < wf->nband && KPOINT_NUM >= 0 && KPOINT_NUM < wf->nwk * wf->nspin,
+             PP_ERR_ARG, "ae_state_density: band or k-point index out of range");
+  return rs_density_host(P, wf, fftg, labels, coords, BAND_NUM, KPOINT_NUM, "ae_state_density");
+}
+
+int ae_chg_density(double *P, pswf_t *wf, int *fftg, int *labels, double *coords) {
+  PP_REQUIRE(wf && !wf->is_ncl, PP_ERR_ARG, "ae_chg_density: spinor wavefunction (use ncl_ae_chg_density)");
+  return rs_density_host(P, wf, fftg, labels, coords, -1, -1, "ae_chg_density");
+}
+
+int ncl_ae_chg_density(double *P, pswf_t *wf, int *fftg, int *labels, double *coords) {
+  PP_REQUIRE(wf && wf->is_ncl, PP_ERR_ARG, "ncl_ae_chg_density: not a spinor wavefunction");
+  return rs_density_host(P, wf, fftg, labels, coords, -1, -1, "ncl_ae_chg_density");
+}
+
+int project_realspace_state(pp_cdouble *projs, int BAND_NUM, pswf_t *wf, pswf_t *wf_R, int *fftg, int *labels,
+                            double *coords, int *labels_R, double *coords_R) {
+  PP_REQUIRE(projs && wf && wf_R && !wf->is_ncl && !wf_R->is_ncl, PP_ERR_ARG,
+             "project_realspace_state: bad arguments");
+  PP_TRY(rs_check(wf, fftg, labels, coords, "project_realspace_state"));
+  PP_TRY(rs_check(wf_R, fftg, labels_R, coords_R, "project_realspace_state"));
+  PP_REQUIRE(BAND_NUM >= 0 && BAND_NUM < wf->nband, PP_ERR_ARG, "project_realspace_state: band index out of range");
+  PP_REQUIRE(wf_R->nband >= wf->nband && wf_R->nwk * wf_R->nspin >= wf->nwk * wf->nspin, PP_ERR_ARG,
+             "project_realspace_state: the basis has fewer bands or k-points than the projected wavefunction");
+  std::lock_guard<std::mutex> lk(g_mu);
+  pp_ctx *ctx;
+  PP_TRY(global_ctx(&ctx));
+  pp_rsplan *rpS, *rpR;
+  PP_TRY(rs_plan_for(ctx, wf, fftg, labels, coords, &rpS));
+  PP_TRY(rs_plan_for(ctx, wf_R, fftg, labels_R, coords_R, &rpR));
+  const long npts = (long)fftg[0] * fftg[1] * fftg[2];
+  const int nkt = wf->nwk * wf->nspin, nband = wf->nband;
+  const int cb = std::min(64, nband);
+  DevBuf S, R, O;
+  PP_TRY(S.ensure(sizeof(pp_cdouble) * (size_t)npts));
+  PP_TRY(R.ensure(sizeof(pp_cdouble) * (size_t)npts * cb));
+  PP_TRY(O.ensure(sizeof(pp_cdouble) * (size_t)nband));
+  std::vector<pp_cdouble> o(nband);
+  const double scale = det3(wf->lattice) / npts;
+  for (int k = 0; k < nkt; k++) {
+    PP_TRY(rs_run(ctx, wf, rpS, k, BAND_NUM, 1, 0, 0, S.p, npts, nullptr, nullptr));
+    for (int b0 = 0; b0 < nband; b0 += cb) {
+      const int nb = std::min(cb, nband - b0);
+      PP_TRY(rs_run(ctx, wf_R, rpR, k, b0, nb, 0, 0, R.p, npts, nullptr, nullptr));
+      PP_TRY(rs_dots(ctx, (const cplx<double> *)R.p, npts, nb, (const cplx<double> *)S.p, npts, scale,
+                     (cplx<double> *)O.p + b0));
+    }
+    PP_CHECK_HIP(hipMemcpy(o.data(), O.p, sizeof(pp_cdouble) * nband, hipMemcpyDeviceToHost));
+    for (int b = 0; b < nband; b++)
+      projs[(long)b * nkt + k] = o[b];
+  }
   return PP_OK;
 }
 

@@ -26,6 +26,8 @@ enum KClass {
   K_AUG_GEMM,
   K_MISC,
   K_FFT_X_SPHERE,
+  K_RS_AUG,
+  K_RS_EMIT,
   K_NCLASS
 };
 const char *kclass_name(int c);
@@ -148,6 +150,11 @@ int sphere_bounds(pp_ctx *ctx, const int *fftg, int nw, const int *G_dev, int *b
 int grid_to_bandmajor(pp_ctx *ctx, int precision, const void *grid, int NB, int nbc, long npts,
                       double scl, void *out);
 double det3(const double *m);
+// Real-space states (pp_realspace.hip): x *= e^{-2 pi i k.r} on the grid
+// fftg; out[b] = scale sum_r conj(R[b * ldr + r]) S[r] (synchronises).
+int rs_remove_phase(pp_ctx *ctx, cplx<double> *x, const int *fftg, const double *kf);
+int rs_dots(pp_ctx *ctx, const cplx<double> *R, long ldr, int nb, const cplx<double> *S, long n, double scale,
+            cplx<double> *out);
 void spline_coeff_host(const double *x, const double *y, int n, double *b, double *c, double *d);
 int smooth_diffwave_host(int n, const double *wave_grid, const double *diffwave, int l, int lmax, double grid_encut,
                          int np, double *sdw, double *kgrid, double *kwave);

@@ -1,0 +1,735 @@
+// All-electron real-space states on the GPU (density.c:158-327).
+//
+// realspace_state (density.c:232-310) forms, for one band,
+//   psi(r) = fft3d(C)(r) e^{2 pi i k.r}
+//          + sum_s sum_n (phi_n - phi~_n)(r - R_s) <p~_n|psi~> e^{2 pi i k.(R_s + T)}
+// with the partial-wave differences (utils.c:522-545: diffwave(|d|)/|d|
+// times the complex Y_lm of d, spline on the log wave grid) taken at
+// the unwrapped box vector d = r - R_s of every grid point with
+// |d| < rmax_s (the last point of the species' wave grid), T the lattice
+// vector that wraps the point into the cell.
+//
+// Plan (pp_rs_plan_create): per site the box points inside its sphere
+// (count, host scan, ordered fill; the order of the reference's i-j-k
+// loops) and a complex table of the partial-wave values per point and
+// projector, independent of k and band.  Sites whose spheres can share a
+// grid point get different colours, so one launch per colour adds the
+// augmentation with plain read-modify-writes (deterministic); when a
+// site's box wraps onto itself the plan uses atomic adds instead.
+//
+// Per band chunk (pp_realspace_batched): scatter + backward FFT into the
+// band-fastest chunk (the projection path's kernels), augmentation added
+// in the chunk with the phase e^{-2 pi i k.(r_unwrapped - R)} / scale so
+// that the emit kernel's common factor scale e^{2 pi i k.r} restores the
+// reference's terms, then either the band-major states (LDS transpose,
+// with or without the Bloch phase, i.e. remove_phase) or the weighted
+// density sum_b w_b |psi_b|^2 (ae_state_density / ae_chg_density,
+// density.c:158-179).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "pp_geom.h"
+#include "pp_internal.h"
+
+namespace pp {
+
+struct RsSite {
+  double R[3];
+  double rmax;
+  int c[3], g[3];
+  int nproj, nchan, n;  // projectors, channels, wave-grid points
+  int chan_off;         // first channel of the species in chan_l
+  int proj_off;         // first projection of the site
+  long spl_off;         // species block: x[n], then f, b, c, d (n each) per channel
+};
+
+struct RsLat {
+  double m[9];
+};
+
+struct RsWeights {
+  double w[64];
+};
+
+// candidate c of a site's box -> unwrapped (i, j, k), k fastest (the
+// reference's loop order, density.c:272-276)
+__device__ __forceinline__ void rs_cand(const RsSite &s, int c, int &i, int &j, int &k) {
+  const int w1 = 2 * s.g[1] + 1, w2 = 2 * s.g[2] + 1;
+  k = c % w2 - s.g[2] + s.c[2];
+  j = (c / w2) % w1 - s.g[1] + s.c[1];
+  i = c / (w1 * w2) - s.g[0] + s.c[0];
+}
+
+// unwrapped box vector r - R in cartesian coordinates (density.c:277-281)
+__device__ __forceinline__ double rs_vec(const RsSite &s, const RsLat &L, int f0, int f1, int f2, int i, int j, int k,
+                                         double *v) {
+  double f[3] = {(double)i / f0 - s.R[0], (double)j / f1 - s.R[1], (double)k / f2 - s.R[2]};
+  d_frac_to_cart(f, L.m, v);
+  return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+}
+
+__global__ __launch_bounds__(256) void k_rs_count(const RsSite *__restrict__ geo, RsLat L, int f0, int f1, int f2,
+                                                  int nblk, int *__restrict__ blkcnt) {
+  const int s = blockIdx.y;
+  const RsSite g = geo[s];
+  const int ncand = (2 * g.g[0] + 1) * (2 * g.g[1] + 1) * (2 * g.g[2] + 1);
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  bool in = false;
+  if (c < ncand) {
+    int i, j, k;
+    double v[3];
+    rs_cand(g, c, i, j, k);
+    in = rs_vec(g, L, f0, f1, f2, i, j, k, v) < g.rmax;
+  }
+  const int cnt = __syncthreads_count(in);
+  if (threadIdx.x == 0)
+    blkcnt[(long)s * nblk + blockIdx.x] = cnt;
+}
+
+// points in candidate order: (wrapped grid index, unwrapped i, j, k)
+__global__ __launch_bounds__(256) void k_rs_fill(const RsSite *__restrict__ geo, RsLat L, int f0, int f1, int f2,
+                                                 int nblk, const long *__restrict__ blkoff, int4 *__restrict__ pts) {
+  __shared__ int wsum[4];
+  const int s = blockIdx.y;
+  const RsSite g = geo[s];
+  const int ncand = (2 * g.g[0] + 1) * (2 * g.g[1] + 1) * (2 * g.g[2] + 1);
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  bool in = false;
+  int i = 0, j = 0, k = 0;
+  if (c < ncand) {
+    double v[3];
+    rs_cand(g, c, i, j, k);
+    in = rs_vec(g, L, f0, f1, f2, i, j, k, v) < g.rmax;
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const unsigned long long m = __ballot(in);
+  const int pre = __popcll(m & ((1ull << lane) - 1ull));
+  if (lane == 0)
+    wsum[w] = __popcll(m);
+  __syncthreads();
+  int base = 0;
+  for (int q = 0; q < w; q++)
+    base += wsum[q];
+  if (in) {
+    const int ii = (i % f0 + f0) % f0, jj = (j % f1 + f1) % f1, kk = (k % f2 + f2) % f2;
+    pts[blkoff[(long)s * nblk + blockIdx.x] + base + pre] = make_int4((ii * f1 + jj) * f2 + kk, i, j, k);
+  }
+}
+
+// tab[tab_off[s] + q * P + p] = wave_value2 of projector p at point q
+// (utils.c:477-489, 522-545; Y_lm of utils.c:441-449)
+__global__ __launch_bounds__(256) void k_rs_values(const RsSite *__restrict__ geo, RsLat L, int f0, int f1, int f2,
+                                                   const int *__restrict__ site_pt, const int4 *__restrict__ pts,
+                                                   const long *__restrict__ tab_off, const double *__restrict__ spl,
+                                                   const int *__restrict__ chan_l, cplx<double> *__restrict__ tab) {
+  const int s = blockIdx.y;
+  const RsSite g = geo[s];
+  const int p0 = site_pt[s], np = site_pt[s + 1] - p0;
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= np)
+    return;
+  const int4 pt = pts[p0 + q];
+  double v[3];
+  const double r = rs_vec(g, L, f0, f1, f2, pt.y, pt.z, pt.w, v);
+  double theta = 0.0, phi = 0.0;
+  if (r != 0.0) {
+    theta = acos(v[2] / r);
+    if (r - fabs(v[2]) != 0.0)
+      phi = acos(v[0] / sqrt(v[0] * v[0] + v[1] * v[1]));
+    if (v[1] < 0)
+      phi = 2.0 * M_PI - phi;
+  }
+  const double ct = cos(theta);
+  const int n = g.n;
+  const double *x = spl + g.spl_off;
+  int id = 0;
+  double t = 0.0;
+  const bool below = r < x[0], beyond = r > x[n - 1];
+  if (!below && !beyond) {
+    id = (int)(log(r / x[0]) / log(x[1] / x[0]));
+    if (id > n - 2)
+      id = n - 2;
+    t = r - x[id];
+  }
+  const double rdiv = below ? x[0] : r;
+  cplx<double> *out = tab + tab_off[s] + (long)q * g.nproj;
+  int p = 0;
+  for (int ch = 0; ch < g.nchan; ch++) {
+    const int l = chan_l[g.chan_off + ch];
+    const double *fc = x + (long)n * (1 + 4 * ch);
+    const double *bc = fc + n, *cc = bc + n, *dc = cc + n;
+    double radial = beyond ? 0.0 : below ? fc[0] : fc[id] + t * (bc[id] + t * (cc[id] + t * dc[id]));
+    radial /= rdiv;
+    for (int m = -l; m <= l; m++, p++) {
+      const double norm = sqrt((2 * l + 1) / (4.0 * M_PI) * d_fact(l - m) / d_fact(l + m));
+      const double a = norm * d_legendre(l, m, ct) * radial;
+      double sn, cs;
+      sincos(m * phi, &sn, &cs);
+      out[p] = mk<double>(a * cs, a * sn);
+    }
+  }
+}
+
+// Adds the augmentation of the sites sites[blockIdx.y] to the band-fastest
+// chunk: lane = band; each wavefront takes points of the block's range.
+// proj rows: proj[b * ldp + proj_off + p] for the chunk's bands.
+template <typename T, bool ATOMIC>
+__global__ __launch_bounds__(256) void k_rs_augment(cplx<T> *__restrict__ grid, int NB, int nbc,
+                                                    const RsSite *__restrict__ geo, const int *__restrict__ sites,
+                                                    const int *__restrict__ site_pt, const int4 *__restrict__ pts,
+                                                    const long *__restrict__ tab_off,
+                                                    const cplx<double> *__restrict__ tab,
+                                                    const cplx<double> *__restrict__ proj, long ldp, double k0,
+                                                    double k1, double k2, double iscl, int f0, int f1, int f2,
+                                                    int ppb) {
+  extern __shared__ cplx<double> coef[];  // [P][64]
+  const int s = sites[blockIdx.y];
+  const RsSite g = geo[s];
+  const int P = g.nproj;
+  const int p0 = site_pt[s], np = site_pt[s + 1] - p0;
+  const int begin = blockIdx.x * ppb;
+  if (begin >= np)
+    return;  // uniform over the block
+  const int end = min(np, begin + ppb);
+  for (int t = threadIdx.x; t < P * 64; t += 256) {
+    const int p = t % P, b = t / P;
+    coef[p * 64 + b] = b < nbc ? proj[(long)b * ldp + g.proj_off + p] : mk<double>(0.0, 0.0);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int q = begin + wv; q < end; q += 4) {
+    const int4 pt = pts[p0 + q];
+    const cplx<double> *row = tab + tab_off[s] + (long)q * P;
+    double ar = 0.0, ai = 0.0;
+    for (int p = 0; p < P; p++) {
+      const cplx<double> w = row[p], c = coef[p * 64 + lane];
+      ar += w.x * c.x - w.y * c.y;
+      ai += w.x * c.y + w.y * c.x;
+    }
+    // e^{2 pi i k.(R + T)} e^{-2 pi i k.r} = e^{-2 pi i k.(r_unwrapped - R)}
+    const double kt = k0 * ((double)pt.y / f0 - g.R[0]) + k1 * ((double)pt.z / f1 - g.R[1]) +
+                      k2 * ((double)pt.w / f2 - g.R[2]);
+    double sn, cs;
+    sincospi(-2.0 * kt, &sn, &cs);
+    const double vr = (ar * cs - ai * sn) * iscl, vi = (ar * sn + ai * cs) * iscl;
+    if (lane < nbc) {
+      cplx<T> *dst = grid + (long)pt.x * NB + lane;
+      if (ATOMIC) {
+        atomicAdd(&dst->x, (T)vr);
+        atomicAdd(&dst->y, (T)vi);
+      } else {
+        cplx<T> o = *dst;
+        o.x += (T)vr;
+        o.y += (T)vi;
+        *dst = o;
+      }
+    }
+  }
+}
+
+// out[b * ldo + pt] = scl grid[pt][b] (e^{2 pi i k.r} unless remove_phase):
+// a tile of RS_TP points x NB bands through LDS.
+constexpr int RS_TP = 32;
+template <typename T>
+__global__ __launch_bounds__(256) void k_rs_state(const cplx<T> *__restrict__ grid, int NB, int nbc, long npts,
+                                                  double scl, int f0, int f1, int f2, double k0, double k1,
+                                                  double k2, int phase_on, cplx<double> *__restrict__ out,
+                                                  long ldo) {
+  __shared__ cplx<double> tile[RS_TP * 65];
+  __shared__ cplx<double> ph[RS_TP];
+  const long pt0 = (long)blockIdx.x * RS_TP;
+  const int npt = (int)min((long)RS_TP, npts - pt0);
+  if (threadIdx.x < RS_TP) {
+    cplx<double> e = mk<double>(scl, 0.0);
+    const long pt = pt0 + threadIdx.x;
+    if (phase_on && threadIdx.x < npt) {
+      const int kk = (int)(pt % f2), jj = (int)((pt / f2) % f1), ii = (int)(pt / ((long)f1 * f2));
+      const double a = k0 * ii / f0 + k1 * jj / f1 + k2 * kk / f2;
+      double sn, cs;
+      sincospi(2.0 * a, &sn, &cs);
+      e = mk<double>(scl * cs, scl * sn);
+    }
+    ph[threadIdx.x] = e;
+  }
+  const cplx<T> *src = grid + pt0 * NB;
+  for (int t = threadIdx.x; t < npt * NB; t += 256) {
+    const int p = t / NB, b = t - p * NB;
+    tile[p * 65 + b] = cvt<double>(src[t]);
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < RS_TP * nbc; t += 256) {
+    const int b = t / RS_TP, p = t - b * RS_TP;
+    if (p < npt)
+      out[(long)b * ldo + pt0 + p] = tile[p * 65 + b] * ph[p];
+  }
+}
+
+// dens[pt] += scl^2 sum_b w_b |grid[pt][b]|^2: lane = band, one point per
+// wavefront step.
+template <typename T>
+__global__ __launch_bounds__(256) void k_rs_density(const cplx<T> *__restrict__ grid, int NB, int nbc, long npts,
+                                                    double scl2, RsWeights W, int ppw,
+                                                    double *__restrict__ dens) {
+  const int lane = threadIdx.x & 63;
+  const long w0 = ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * ppw;
+  const double wl = lane < nbc ? W.w[lane] : 0.0;
+  for (int q = 0; q < ppw; q++) {
+    const long pt = w0 + q;
+    if (pt >= npts)
+      break;
+    double v = 0.0;
+    if (lane < nbc) {
+      const cplx<double> a = cvt<double>(grid[pt * NB + lane]);
+      v = wl * (a.x * a.x + a.y * a.y);
+    }
+    for (int o = 32; o > 0; o >>= 1)
+      v += __shfl_xor(v, o, 64);
+    if (lane == 0)
+      dens[pt] += scl2 * v;
+  }
+}
+
+}  // namespace pp
+
+using namespace pp;
+
+struct pp_rsplan {
+  pp_ctx *ctx = nullptr;
+  double lattice[9];
+  int fftg[3];
+  int nsites = 0, ptot = 0, pmax = 0;
+  long npts = 0;
+  bool atomic = false;
+  std::vector<int> proj_off, site_pt;
+  std::vector<int> color_off, color_sites, color_maxpts;
+  DevBuf d_geo, d_pts, d_tab, d_tab_off, d_site_pt, d_colors, d_spl, d_chan_l;
+};
+
+namespace {
+
+double cross_norm(const double *a, const double *b) {
+  const double c0 = a[1] * b[2] - a[2] * b[1], c1 = a[2] * b[0] - a[0] * b[2], c2 = a[0] * b[1] - a[1] * b[0];
+  return sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+}
+
+// Greedy colouring: sites s, t conflict when some lattice image brings
+// them within rmax_s + rmax_t (their spheres may share a grid point).
+void colour_sites(const double *lat, const std::vector<RsSite> &geo, std::vector<int> &colour) {
+  const int ns = (int)geo.size();
+  colour.assign(ns, 0);
+  std::vector<char> used;
+  for (int s = 0; s < ns; s++) {
+    used.assign(ns + 1, 0);
+    for (int t = 0; t < s; t++) {
+      const double lim = geo[s].rmax + geo[t].rmax;
+      bool hit = false;
+      for (int a = -2; a <= 2 && !hit; a++)
+        for (int b = -2; b <= 2 && !hit; b++)
+          for (int c = -2; c <= 2 && !hit; c++) {
+            const double f[3] = {geo[s].R[0] - geo[t].R[0] + a, geo[s].R[1] - geo[t].R[1] + b,
+                                 geo[s].R[2] - geo[t].R[2] + c};
+            double v[3];
+            for (int q = 0; q < 3; q++)
+              v[q] = f[0] * lat[q] + f[1] * lat[3 + q] + f[2] * lat[6 + q];
+            hit = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) < lim;
+          }
+      if (hit)
+        used[colour[t]] = 1;
+    }
+    int c = 0;
+    while (used[c])
+      c++;
+    colour[s] = c;
+  }
+}
+
+}  // namespace
+
+extern "C" int pp_rs_plan_create(pp_ctx *ctx, pp_rsplan **out, const double *lattice, const int *fftg, int num_sites,
+                                 const int *labels, const double *frac_coords, int num_elems, const int *nchan,
+                                 const int *ls, const int *wave_n, const double *wave_grid,
+                                 const double *diffwave) {
+  PP_REQUIRE(ctx && out && lattice && fftg && num_sites >= 0 && num_elems > 0 && nchan && wave_n, PP_ERR_ARG,
+             "pp_rs_plan_create: bad arguments");
+  PP_REQUIRE(num_sites == 0 || (labels && frac_coords), PP_ERR_ARG, "pp_rs_plan_create: null site arrays");
+  PP_REQUIRE(fftg[0] > 0 && fftg[1] > 0 && fftg[2] > 0 && (long)fftg[0] * fftg[1] * fftg[2] < (1L << 31),
+             PP_ERR_ARG, "pp_rs_plan_create: bad grid");
+  const double vol = det3(lattice);
+  PP_REQUIRE(vol > 0, PP_ERR_ARG, "pp_rs_plan_create: the lattice must be right-handed");
+  PP_CHECK_HIP(hipSetDevice(ctx->device));
+  // species blocks: x[n], then f, b, c, d per channel (host splines)
+  std::vector<long> spl_off(num_elems);
+  std::vector<int> chan_off(num_elems + 1, 0), nproj(num_elems, 0);
+  std::vector<double> spl;
+  long goff = 0, doff = 0;
+  for (int e = 0; e < num_elems; e++) {
+    const int n = wave_n[e];
+    PP_REQUIRE(n >= 2 && nchan[e] >= 0, PP_ERR_ARG, "pp_rs_plan_create: wave grid needs 2 or more points");
+    PP_REQUIRE(wave_grid[goff] > 0 && wave_grid[goff + 1] > wave_grid[goff], PP_ERR_ARG,
+               "pp_rs_plan_create: the wave grid must be positive and increasing");
+    chan_off[e + 1] = chan_off[e] + nchan[e];
+    spl_off[e] = (long)spl.size();
+    spl.insert(spl.end(), wave_grid + goff, wave_grid + goff + n);
+    for (int ch = 0; ch < nchan[e]; ch++) {
+      const int l = ls[chan_off[e] + ch];
+      PP_REQUIRE(l >= 0 && l <= 3, PP_ERR_ARG, "pp_rs_plan_create: l must be 0..3");
+      nproj[e] += 2 * l + 1;
+      const double *f = diffwave + doff + (long)ch * n;
+      std::vector<double> b(n), c(n), d(n);
+      spline_coeff_host(wave_grid + goff, f, n, b.data(), c.data(), d.data());
+      spl.insert(spl.end(), f, f + n);
+      spl.insert(spl.end(), b.begin(), b.end());
+      spl.insert(spl.end(), c.begin(), c.end());
+      spl.insert(spl.end(), d.begin(), d.end());
+    }
+    goff += n;
+    doff += (long)nchan[e] * n;
+  }
+  std::unique_ptr<pp_rsplan> hold(new pp_rsplan());
+  pp_rsplan *rp = hold.get();
+  rp->ctx = ctx;
+  memcpy(rp->lattice, lattice, sizeof(rp->lattice));
+  memcpy(rp->fftg, fftg, sizeof(rp->fftg));
+  rp->nsites = num_sites;
+  std::vector<RsSite> geo(num_sites);
+  rp->proj_off.assign(num_sites + 1, 0);
+  int nblk = 1;
+  for (int s = 0; s < num_sites; s++) {
+    const int e = labels[s];
+    PP_REQUIRE(e >= 0 && e < num_elems, PP_ERR_ARG, "pp_rs_plan_create: label out of range");
+    RsSite &g = geo[s];
+    const double *x = wave_grid;
+    long o = 0;
+    for (int q = 0; q < e; q++)
+      o += wave_n[q];
+    g.rmax = x[o + wave_n[e] - 1];
+    for (int q = 0; q < 3; q++) {
+      g.R[q] = frac_coords[3 * s + q];
+      g.c[q] = (int)std::round(g.R[q] * fftg[q]);
+    }
+    // density.c:266-274
+    g.g[0] = (int)(cross_norm(lattice + 3, lattice + 6) * g.rmax / vol * fftg[0]) + 1;
+    g.g[1] = (int)(cross_norm(lattice + 0, lattice + 6) * g.rmax / vol * fftg[1]) + 1;
+    g.g[2] = (int)(cross_norm(lattice + 0, lattice + 3) * g.rmax / vol * fftg[2]) + 1;
+    for (int q = 0; q < 3; q++)
+      if (2 * g.g[q] + 1 > fftg[q])
+        rp->atomic = true;  // the box wraps onto itself
+    g.nproj = nproj[e];
+    g.nchan = nchan[e];
+    g.n = wave_n[e];
+    g.chan_off = chan_off[e];
+    g.spl_off = spl_off[e];
+    g.proj_off = rp->proj_off[s];
+    rp->proj_off[s + 1] = rp->proj_off[s] + nproj[e];
+    rp->pmax = std::max(rp->pmax, nproj[e]);
+    const long ncand = (long)(2 * g.g[0] + 1) * (2 * g.g[1] + 1) * (2 * g.g[2] + 1);
+    PP_REQUIRE(ncand < (1L << 31), PP_ERR_ARG, "pp_rs_plan_create: site box too large");
+    nblk = std::max(nblk, (int)((ncand + 255) / 256));
+  }
+  rp->ptot = rp->proj_off[num_sites];
+  PP_REQUIRE(rp->pmax <= 64, PP_ERR_ARG, "pp_rs_plan_create: more than 64 projectors per site");
+  int st = PP_OK;
+  const int ns = std::max(1, num_sites);
+  if ((st = rp->d_geo.ensure(sizeof(RsSite) * ns)) || (st = rp->d_spl.ensure(sizeof(double) * std::max<size_t>(1, spl.size()))) ||
+      (st = rp->d_chan_l.ensure(sizeof(int) * std::max(1, chan_off[num_elems]))))
+    return st;
+  hipStream_t sm = ctx->stream;
+  if (num_sites)
+    PP_CHECK_HIP(hipMemcpyAsync(rp->d_geo.p, geo.data(), sizeof(RsSite) * num_sites, hipMemcpyHostToDevice, sm));
+  if (!spl.empty())
+    PP_CHECK_HIP(hipMemcpyAsync(rp->d_spl.p, spl.data(), sizeof(double) * spl.size(), hipMemcpyHostToDevice, sm));
+  if (chan_off[num_elems])
+    PP_CHECK_HIP(hipMemcpyAsync(rp->d_chan_l.p, ls, sizeof(int) * chan_off[num_elems], hipMemcpyHostToDevice, sm));
+  RsLat L;
+  memcpy(L.m, lattice, sizeof(L.m));
+  rp->site_pt.assign(num_sites + 1, 0);
+  if (num_sites) {
+    DevBuf cnt, boff;
+    if ((st = cnt.ensure(sizeof(int) * (size_t)num_sites * nblk)) ||
+        (st = boff.ensure(sizeof(long) * (size_t)num_sites * nblk)))
+      return st;
+    hipLaunchKernelGGL(k_rs_count, dim3(nblk, num_sites), dim3(256), 0, sm, (const RsSite *)rp->d_geo.p, L, fftg[0],
+                       fftg[1], fftg[2], nblk, (int *)cnt.p);
+    PP_CHECK_HIP(hipGetLastError());
+    std::vector<int> hc((size_t)num_sites * nblk);
+    PP_CHECK_HIP(hipMemcpyAsync(hc.data(), cnt.p, sizeof(int) * hc.size(), hipMemcpyDeviceToHost, sm));
+    PP_CHECK_HIP(hipStreamSynchronize(sm));
+    std::vector<long> ho(hc.size());
+    long acc = 0;
+    for (int s = 0; s < num_sites; s++) {
+      for (int b = 0; b < nblk; b++) {
+        ho[(size_t)s * nblk + b] = acc;
+        acc += hc[(size_t)s * nblk + b];
+      }
+      PP_REQUIRE(acc < (1L << 31), PP_ERR_ARG, "pp_rs_plan_create: more than 2^31 sphere points");
+      rp->site_pt[s + 1] = (int)acc;
+    }
+    rp->npts = acc;
+    if ((st = rp->d_pts.ensure(sizeof(int4) * std::max(1L, acc))))
+      return st;
+    PP_CHECK_HIP(hipMemcpyAsync(boff.p, ho.data(), sizeof(long) * ho.size(), hipMemcpyHostToDevice, sm));
+    hipLaunchKernelGGL(k_rs_fill, dim3(nblk, num_sites), dim3(256), 0, sm, (const RsSite *)rp->d_geo.p, L, fftg[0],
+                       fftg[1], fftg[2], nblk, (const long *)boff.p, (int4 *)rp->d_pts.p);
+    PP_CHECK_HIP(hipGetLastError());
+    // value table
+    std::vector<long> toff(num_sites + 1, 0);
+    int maxp = 1;
+    for (int s = 0; s < num_sites; s++) {
+      const int np = rp->site_pt[s + 1] - rp->site_pt[s];
+      toff[s + 1] = toff[s] + (long)np * geo[s].nproj;
+      maxp = std::max(maxp, np);
+    }
+    if ((st = rp->d_tab.ensure(sizeof(cplx<double>) * std::max(1L, toff[num_sites]))) ||
+        (st = rp->d_tab_off.ensure(sizeof(long) * (num_sites + 1))) ||
+        (st = rp->d_site_pt.ensure(sizeof(int) * (num_sites + 1))))
+      return st;
+    PP_CHECK_HIP(
+        hipMemcpyAsync(rp->d_tab_off.p, toff.data(), sizeof(long) * toff.size(), hipMemcpyHostToDevice, sm));
+    PP_CHECK_HIP(hipMemcpyAsync(rp->d_site_pt.p, rp->site_pt.data(), sizeof(int) * rp->site_pt.size(),
+                                hipMemcpyHostToDevice, sm));
+    hipLaunchKernelGGL(k_rs_values, dim3((maxp + 255) / 256, num_sites), dim3(256), 0, sm,
+                       (const RsSite *)rp->d_geo.p, L, fftg[0], fftg[1], fftg[2], (const int *)rp->d_site_pt.p,
+                       (const int4 *)rp->d_pts.p, (const long *)rp->d_tab_off.p, (const double *)rp->d_spl.p,
+                       (const int *)rp->d_chan_l.p, (cplx<double> *)rp->d_tab.p);
+    PP_CHECK_HIP(hipGetLastError());
+    // colours (one launch per colour; a single "colour" with atomics when
+    // a box wraps onto itself)
+    std::vector<int> colour;
+    if (rp->atomic)
+      colour.assign(num_sites, 0);
+    else
+      colour_sites(lattice, geo, colour);
+    const int nc = *std::max_element(colour.begin(), colour.end()) + 1;
+    rp->color_off.assign(nc + 1, 0);
+    rp->color_maxpts.assign(nc, 0);
+    for (int s = 0; s < num_sites; s++)
+      rp->color_off[colour[s] + 1]++;
+    for (int c = 0; c < nc; c++)
+      rp->color_off[c + 1] += rp->color_off[c];
+    rp->color_sites.assign(num_sites, 0);
+    std::vector<int> fillc(rp->color_off.begin(), rp->color_off.end() - 1);
+    for (int s = 0; s < num_sites; s++) {
+      rp->color_sites[fillc[colour[s]]++] = s;
+      rp->color_maxpts[colour[s]] =
+          std::max(rp->color_maxpts[colour[s]], rp->site_pt[s + 1] - rp->site_pt[s]);
+    }
+    if ((st = rp->d_colors.ensure(sizeof(int) * num_sites)))
+      return st;
+    PP_CHECK_HIP(hipMemcpyAsync(rp->d_colors.p, rp->color_sites.data(), sizeof(int) * num_sites,
+                                hipMemcpyHostToDevice, sm));
+    PP_CHECK_HIP(hipStreamSynchronize(sm));  // host staging buffers end here
+  }
+  *out = hold.release();
+  return PP_OK;
+}
+
+extern "C" void pp_rs_plan_destroy(pp_rsplan *rp) { delete rp; }
+
+extern "C" int pp_rs_plan_info(pp_rsplan *rp, int *total_projs, long *total_points, int *ncolors, int *atomic,
+                               int *proj_off) {
+  PP_REQUIRE(rp, PP_ERR_ARG, "pp_rs_plan_info: null plan");
+  if (total_projs)
+    *total_projs = rp->ptot;
+  if (total_points)
+    *total_points = rp->npts;
+  if (ncolors)
+    *ncolors = rp->color_off.empty() ? 0 : (int)rp->color_off.size() - 1;
+  if (atomic)
+    *atomic = rp->atomic ? 1 : 0;
+  if (proj_off)
+    memcpy(proj_off, rp->proj_off.data(), sizeof(int) * rp->proj_off.size());
+  return PP_OK;
+}
+
+namespace {
+
+template <typename T>
+int rs_chunk(pp_ctx *ctx, pp_rsplan *rp, FftPlan *pl, const double *kf, int nw, const int *gidx,
+             const cplx<float> *C, long ldc, int nbc, const cplx<double> *proj, long ldp, int remove_phase,
+             cplx<double> *states, long ldo, const double *wts, double *dens) {
+  const int NB = nbc;
+  const long npts = (long)rp->fftg[0] * rp->fftg[1] * rp->fftg[2];
+  cplx<T> *grid = (cplx<T> *)ctx->grid.p;
+  PP_TRY(fft_chunk<T>(ctx, pl, grid, NB, nw, gidx, C, ldc, nbc, nullptr));
+  const double scl = 1.0 / sqrt(det3(rp->lattice));
+  hipEvent_t ev;
+  if (rp->nsites && rp->ptot) {
+    tbegin(ctx, K_RS_AUG, &ev);
+    const int ppb = 64;
+    const size_t lds = sizeof(cplx<double>) * 64 * (size_t)std::max(1, rp->pmax);
+    const int nc = (int)rp->color_off.size() - 1;
+    for (int c = 0; c < nc; c++) {
+      const int nsc = rp->color_off[c + 1] - rp->color_off[c];
+      const int gx = std::max(1, (rp->color_maxpts[c] + ppb - 1) / ppb);
+      const int *sites = (const int *)rp->d_colors.p + rp->color_off[c];
+      if (rp->atomic)
+        hipLaunchKernelGGL((k_rs_augment<T, true>), dim3(gx, nsc), dim3(256), lds, ctx->stream, grid, NB, nbc,
+                           (const RsSite *)rp->d_geo.p, sites, (const int *)rp->d_site_pt.p,
+                           (const int4 *)rp->d_pts.p, (const long *)rp->d_tab_off.p,
+                           (const cplx<double> *)rp->d_tab.p, proj, ldp, kf[0], kf[1], kf[2], 1.0 / scl,
+                           rp->fftg[0], rp->fftg[1], rp->fftg[2], ppb);
+      else
+        hipLaunchKernelGGL((k_rs_augment<T, false>), dim3(gx, nsc), dim3(256), lds, ctx->stream, grid, NB, nbc,
+                           (const RsSite *)rp->d_geo.p, sites, (const int *)rp->d_site_pt.p,
+                           (const int4 *)rp->d_pts.p, (const long *)rp->d_tab_off.p,
+                           (const cplx<double> *)rp->d_tab.p, proj, ldp, kf[0], kf[1], kf[2], 1.0 / scl,
+                           rp->fftg[0], rp->fftg[1], rp->fftg[2], ppb);
+      PP_CHECK_HIP(hipGetLastError());
+    }
+    tend(ctx, K_RS_AUG, ev);
+  }
+  tbegin(ctx, K_RS_EMIT, &ev);
+  if (states) {
+    hipLaunchKernelGGL(k_rs_state<T>, dim3((unsigned)((npts + RS_TP - 1) / RS_TP)), dim3(256), 0, ctx->stream, grid,
+                       NB, nbc, npts, scl, rp->fftg[0], rp->fftg[1], rp->fftg[2], kf[0], kf[1], kf[2],
+                       remove_phase ? 0 : 1, states, ldo);
+    PP_CHECK_HIP(hipGetLastError());
+  }
+  if (dens) {
+    RsWeights W;
+    for (int b = 0; b < 64; b++)
+      W.w[b] = b < nbc ? wts[b] : 0.0;
+    const int ppw = 16;
+    const long blocks = (npts + 4L * ppw - 1) / (4L * ppw);
+    hipLaunchKernelGGL(k_rs_density<T>, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, grid, NB, nbc, npts,
+                       scl * scl, W, ppw, dens);
+    PP_CHECK_HIP(hipGetLastError());
+  }
+  tend(ctx, K_RS_EMIT, ev);
+  return PP_OK;
+}
+
+}  // namespace
+
+extern "C" int pp_realspace_batched(pp_ctx *ctx, pp_rsplan *rp, const double *kfrac, int nw, const int *G_dev,
+                                    const void *C_dev, long ldc, int nband, const void *proj_dev, long ldp,
+                                    int remove_phase, void *states_dev, long ldo, const double *weights,
+                                    double *density_dev, int precision) {
+  PP_REQUIRE(ctx && rp && kfrac && nw >= 0 && nband >= 0 && ldc >= nw, PP_ERR_ARG,
+             "pp_realspace_batched: bad arguments");
+  PP_REQUIRE(precision == PP_FP64 || precision == PP_FP32, PP_ERR_ARG, "pp_realspace_batched: precision");
+  PP_REQUIRE(states_dev || density_dev, PP_ERR_ARG, "pp_realspace_batched: no output");
+  PP_REQUIRE(!density_dev || weights, PP_ERR_ARG, "pp_realspace_batched: density needs weights");
+  const long npts = (long)rp->fftg[0] * rp->fftg[1] * rp->fftg[2];
+  PP_REQUIRE(!states_dev || ldo >= npts, PP_ERR_ARG, "pp_realspace_batched: ldo below the grid size");
+  PP_REQUIRE(rp->ptot == 0 || (proj_dev && ldp >= rp->ptot), PP_ERR_ARG,
+             "pp_realspace_batched: projections missing or ldp below the plan's projector count");
+  PP_REQUIRE(nw == 0 || (G_dev && C_dev), PP_ERR_ARG, "pp_realspace_batched: null coefficients");
+  if (nband == 0)
+    return PP_OK;
+  PP_CHECK_HIP(hipSetDevice(ctx->device));
+  FftPlan *pl = nullptr;
+  PP_TRY(get_fft_plan(ctx, rp->fftg, &pl));
+  const bool f64 = precision == PP_FP64;
+  const int nbmax = std::min(64, nband);
+  const size_t csz = f64 ? sizeof(cplx<double>) : sizeof(cplx<float>);
+  PP_TRY(ctx->grid.ensure(csz * (size_t)npts * nbmax));
+  PP_TRY(ctx->gidx.ensure(sizeof(int) * (size_t)std::max(1, nw)));
+  PP_TRY(compute_gidx(ctx, rp->fftg, nw, G_dev, (int *)ctx->gidx.p));
+  for (int b0 = 0; b0 < nband; b0 += 64) {
+    const int nbc = std::min(64, nband - b0);
+    if (density_dev && !states_dev) {
+      bool any = false;
+      for (int b = 0; b < nbc; b++)
+        any = any || weights[b0 + b] != 0.0;
+      if (!any)
+        continue;  // no state of this chunk contributes
+    }
+    const cplx<float> *C = (const cplx<float> *)C_dev + (size_t)b0 * ldc;
+    const cplx<double> *P = proj_dev ? (const cplx<double> *)proj_dev + (size_t)b0 * ldp : nullptr;
+    cplx<double> *S = states_dev ? (cplx<double> *)states_dev + (size_t)b0 * ldo : nullptr;
+    const double *W = density_dev ? weights + b0 : nullptr;
+    if (f64)
+      PP_TRY(rs_chunk<double>(ctx, rp, pl, kfrac, nw, (const int *)ctx->gidx.p, C, ldc, nbc, P, ldp, remove_phase,
+                              S, ldo, W, density_dev));
+    else
+      PP_TRY(rs_chunk<float>(ctx, rp, pl, kfrac, nw, (const int *)ctx->gidx.p, C, ldc, nbc, P, ldp, remove_phase, S,
+                             ldo, W, density_dev));
+  }
+  return PP_OK;
+}
+
+namespace pp {
+
+// x[pt] *= e^{-2 pi i k.r} (density.c:312-326)
+__global__ __launch_bounds__(256) void k_rs_unphase(cplx<double> *__restrict__ x, long npts, int f0, int f1, int f2,
+                                                    double k0, double k1, double k2) {
+  const long pt = (long)blockIdx.x * 256 + threadIdx.x;
+  if (pt >= npts)
+    return;
+  const int kk = (int)(pt % f2), jj = (int)((pt / f2) % f1), ii = (int)(pt / ((long)f1 * f2));
+  double sn, cs;
+  sincospi(-2.0 * (k0 * ii / f0 + k1 * jj / f1 + k2 * kk / f2), &sn, &cs);
+  x[pt] = x[pt] * mk<double>(cs, sn);
+}
+
+// part[b * nparts + q] = sum over slice q of conj(R[b][r]) S[r]
+__global__ __launch_bounds__(256) void k_rs_dot(const cplx<double> *__restrict__ R, long ldr,
+                                                const cplx<double> *__restrict__ S, long n, int nparts,
+                                                cplx<double> *__restrict__ part) {
+  __shared__ double sr[256], si[256];
+  const int b = blockIdx.y, q = blockIdx.x;
+  const long chunk = (n + nparts - 1) / nparts;
+  const long r0 = (long)q * chunk, r1 = min(n, r0 + chunk);
+  double ar = 0.0, ai = 0.0;
+  for (long r = r0 + threadIdx.x; r < r1; r += 256) {
+    const cplx<double> a = R[(long)b * ldr + r], s = S[r];
+    ar += a.x * s.x + a.y * s.y;
+    ai += a.x * s.y - a.y * s.x;
+  }
+  sr[threadIdx.x] = ar;
+  si[threadIdx.x] = ai;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      sr[threadIdx.x] += sr[threadIdx.x + o];
+      si[threadIdx.x] += si[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    part[(long)b * nparts + q] = mk<double>(sr[0], si[0]);
+}
+
+__global__ __launch_bounds__(64) void k_rs_dot_sum(const cplx<double> *__restrict__ part, int nparts, int nb,
+                                                   double scale, cplx<double> *__restrict__ out) {
+  const int b = blockIdx.x * 64 + threadIdx.x;
+  if (b >= nb)
+    return;
+  double ar = 0.0, ai = 0.0;
+  for (int q = 0; q < nparts; q++) {
+    ar += part[(long)b * nparts + q].x;
+    ai += part[(long)b * nparts + q].y;
+  }
+  out[b] = mk<double>(scale * ar, scale * ai);
+}
+
+int rs_remove_phase(pp_ctx *ctx, cplx<double> *x, const int *fftg, const double *kf) {
+  const long npts = (long)fftg[0] * fftg[1] * fftg[2];
+  hipLaunchKernelGGL(k_rs_unphase, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, ctx->stream, x, npts, fftg[0],
+                     fftg[1], fftg[2], kf[0], kf[1], kf[2]);
+  PP_CHECK_HIP(hipGetLastError());
+  return PP_OK;
+}
+
+int rs_dots(pp_ctx *ctx, const cplx<double> *R, long ldr, int nb, const cplx<double> *S, long n, double scale,
+            cplx<double> *out) {
+  if (nb <= 0)
+    return PP_OK;
+  const int nparts = (int)std::min<long>(256, std::max<long>(1, n / 4096));
+  DevBuf part;
+  PP_TRY(part.ensure(sizeof(cplx<double>) * (size_t)nb * nparts));
+  hipLaunchKernelGGL(k_rs_dot, dim3(nparts, nb), dim3(256), 0, ctx->stream, R, ldr, S, n, nparts,
+                     (cplx<double> *)part.p);
+  PP_CHECK_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_rs_dot_sum, dim3((nb + 63) / 64), dim3(64), 0, ctx->stream, (const cplx<double> *)part.p,
+                     nparts, nb, scale, out);
+  PP_CHECK_HIP(hipGetLastError());
+  PP_CHECK_HIP(hipStreamSynchronize(ctx->stream));  // part is released on return
+  return PP_OK;
+}
+
+}  // namespace pp

@@ -29,15 +29,10 @@
 #include <cmath>
 #include <cstring>
 
+#include "pp_geom.h"
 #include "pp_sphere.h"
 
 namespace pp {
-
-__device__ __forceinline__ void d_frac_to_cart(const double *f, const double *lat, double *o) {
-  o[0] = f[0] * lat[0] + f[1] * lat[3] + f[2] * lat[6];
-  o[1] = f[0] * lat[1] + f[1] * lat[4] + f[2] * lat[7];
-  o[2] = f[0] * lat[2] + f[1] * lat[5] + f[2] * lat[8];
-}
 
 struct Lat9 {
   double m[9];
@@ -211,47 +206,6 @@ __global__ __launch_bounds__(256) void k_seg_len(int4 *__restrict__ seg, long ns
   seg[q].y = end - sg.x;
 }
 
-__device__ double d_fact(int n) {
-  double r = 1.0;
-  for (int i = 2; i <= n; i++)
-    r *= i;
-  return r;
-}
-
-// associated Legendre P_l^m(x), Condon-Shortley phase (utils.c:377-386)
-__device__ double d_legendre(int l, int m, double x) {
-  double sgn = 1.0, fac = 1.0;
-  if (m < 0) {
-    m = -m;
-    sgn = (m & 1) ? -1.0 : 1.0;
-    fac = d_fact(l - m) / d_fact(l + m);
-  }
-  if (m > l)
-    return 0.0;
-  double pmm = 1.0, s = sqrt((1.0 - x) * (1.0 + x)), odd = 1.0;
-  for (int i = 1; i <= m; i++) {
-    pmm *= -odd * s;
-    odd += 2.0;
-  }
-  double res;
-  if (l == m) {
-    res = pmm;
-  } else {
-    double pm1 = x * (2 * m + 1) * pmm;
-    if (l == m + 1) {
-      res = pm1;
-    } else {
-      double pl = 0.0;
-      for (int ll = m + 2; ll <= l; ll++) {
-        pl = (x * (2 * ll - 1) * pm1 - (ll + m - 1) * pmm) / (ll - m);
-        pmm = pm1;
-        pm1 = pl;
-      }
-      res = pl;
-    }
-  }
-  return sgn * fac * res;
-}
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_site_values(const SiteGeo *__restrict__ geo, Lat9 L, int f0, int f1, int f2,
