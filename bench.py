@@ -78,6 +78,18 @@ def cpu_cores():
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_216atom.md")
+RS_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "realspace_aedensity_216atom.md")
+
+
+def read_traffic_file(path):
+    """{path: {class: HBM bytes per launch}} from the json block of a
+    rocprofv3 --pmc summary kept with the profiles ({} when absent)."""
+    try:
+        txt = open(path).read()
+        blk = txt.split("```json", 1)[1].split("```", 1)[0]
+        return {p: {k: float(v) for k, v in d.items()} for p, d in json.loads(blk)["traffic_bytes_per_launch"].items()}
+    except Exception:
+        return {}
 
 
 def read_traffic(path_name):
@@ -103,8 +115,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle checks")
     ap.add_argument("--paths", default="fp32,fp64", help="comma list of fp32, fp64 setup/overlap paths")
-    ap.add_argument("--workload", default="216atom", choices=["216atom", "spinor512"],
-                    help="216atom: the headline workload; spinor512: the 512-atom noncollinear case (separate line)")
+    ap.add_argument("--workload", default="216atom", choices=["216atom", "spinor512", "aedensity216"],
+                    help="216atom: the headline workload; spinor512: the 512-atom noncollinear case; "
+                         "aedensity216: all-electron density of the 216-atom cell on the doubled grid "
+                         "(separate lines)")
     ap.add_argument("--no-graph", action="store_true", help="skip the graph-replay timing")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (library defaults for the setup path)")
@@ -126,6 +140,9 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     if args.workload == "spinor512":
         bench_spinor512(args, torch, B)
+        return
+    if args.workload == "aedensity216":
+        bench_aedensity(args, torch, B)
         return
     wl = synthetic.config1(nband=args.nband)
     Gs = wl.ensure_gvecs()
@@ -448,6 +465,159 @@ def bench_spinor512(args, torch, B):
                                   "grid, 2 k, 2048 spinor bands, 18 projectors/site", "nband": nb, "nk": nk},
            "paths": res, "max_err_vs_fp64_oracle": errs}
     print(json.dumps(out), flush=True)
+
+
+def bench_aedensity(args, torch, B):
+    """All-electron charge density of the 216-atom S structure on the doubled
+    grid 216^3 (get_realspace_density's default, density.c:158-179): per
+    k-point and 64-band chunk, scatter + FP64/FP32 FFT, the partial-wave
+    differences of the 215 sites (spheres of radius 1.6 A) weighted by the
+    projections, and sum_b w_b |psi_b|^2; 512 bands x 4 k per step (all
+    weighted 2 w_k).  Prints its own JSON line."""
+    import time
+
+    from oracle import pyoracle as orc
+    from pawpyseed_amd import synthetic
+    wl = synthetic.config1(nband=args.nband)
+    Gs = wl.ensure_gvecs()
+    nk, nb = len(Gs), args.nband
+    nw = np.array([len(g) for g in Gs], np.int32)
+    ldc = int(-(-nw.max() // 64) * 64)
+    gofs = np.concatenate([[0], np.cumsum(nw)]).astype(np.int32)
+    Gdev = torch.from_numpy(np.ascontiguousarray(np.concatenate(Gs), np.int32)).cuda()
+    CS = make_coefficients(torch, wl, Gs, 2002, nb, ldc)
+    torch.cuda.synchronize()
+    stream = torch.cuda.Stream()
+    ctx = B.Context(0, stream=stream)
+    labels = np.zeros(len(wl.frac_S), np.int32)
+    plan = B.SpherePlan(ctx, wl.lattice, wl.fftg, wl.frac_S, labels, [wl.species], "fp64")
+    proj = torch.zeros((nk, nb, plan.ptot), dtype=torch.complex128, device="cuda")
+    with torch.cuda.stream(stream):
+        B.setup_projections(ctx, plan, wl.kpts, gofs, Gdev, CS, proj, "fp64")
+    fine = tuple(int(2 * n) for n in wl.fftg)
+    t0 = time.perf_counter()
+    rs = B.RealspacePlan(ctx, wl.lattice, fine, wl.frac_S, labels, [wl.species])
+    plan_s = time.perf_counter() - t0
+    log(f"aedensity: grid {fine}, {rs.npts} sphere points, {rs.ncolors} colours, atomic {rs.atomic}, "
+        f"plan {plan_s:.3f} s")
+    dens = torch.zeros(fine, dtype=torch.float64, device="cuda")
+    wk = [np.full(nb, 2.0 * wl.kws[k]) for k in range(nk)]
+
+    def step(p, ks=range(nk)):
+        for k in ks:
+            B.realspace_states(ctx, rs, wl.kpts[k], Gdev[gofs[k]:gofs[k + 1]], CS[k], proj[k], density_out=dens,
+                               weights=wk[k], precision=p, sync=False)
+
+    if args.profile_step:
+        # for rocprofv3 --pmc: the 1 GiB copy of the counter calibration and
+        # one k-point (8 band chunks) of the first path
+        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        with torch.cuda.stream(stream):
+            step([q for q in args.paths.split(",") if q][0], ks=[1])
+        torch.cuda.synchronize()
+        log("profile step done")
+        return
+
+    res = {}
+    for p in [q for q in args.paths.split(",") if q]:
+        with torch.cuda.stream(stream):
+            for _ in range(args.warmup):
+                step(p)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(args.steps):
+                step(p)
+            e1.record(stream)
+        torch.cuda.synchronize()
+        t = e0.elapsed_time(e1) / 1e3 / args.steps
+        ctx.set_timing(True)
+        ctx.timing_reset()
+        with torch.cuda.stream(stream):
+            step(p)
+        torch.cuda.synchronize()
+        kern = ctx.timing()
+        ctx.set_timing(False)
+        res[p] = {"step_s": t, "band_states_per_s": nb * nk / t, "kernels": kern}
+        log(p, json.dumps({k: v for k, v in res[p].items() if k != "kernels"}))
+        log(p, "kernels", json.dumps(kern))
+    # algorithmic bytes per 64-band chunk (averaged over k): the FFT passes
+    # restricted to the plane-wave sphere as in the projection path (z:
+    # coefficients in, populated z-lines out; y: populated rows of the
+    # x-slab in, the slab out; x: the slab in, the whole grid out),
+    # augmentation: the table (16 P per sphere point) and a read-modify-write
+    # of the point's NB values, density emit: the chunk in, the density
+    # read and written
+    npts = int(np.prod(fine))
+    n0, n1, n2 = fine
+    NBc = 64
+    nchunks = nk * -(-nb // NBc)
+    P = 18
+    prune = bool(ctx.get_param("rs_prune"))
+    rs_traffic = read_traffic_file(RS_TRAFFIC_FILE)
+    roof = {}
+    for p, r in res.items():
+        pb = 16 if p == "fp64" else 8
+        bx, by, bz = [], [], []
+        for G in Gs:
+            gx, gy = G[:, 0].astype(np.int64), G[:, 1].astype(np.int64)
+            xs = int(gx.max() - gx.min() + 1) if prune else n0
+            rows = np.unique(gx * 4096 + gy).size if prune else n0 * n1
+            yrow = sum(int(gy[gx == x].max() - gy[gx == x].min() + 1) for x in np.unique(gx)) if prune else n0 * n1
+            bx.append(NBc * pb * (n1 * n2 * xs + npts))
+            by.append(NBc * pb * n2 * (yrow + xs * n1))
+            bz.append(NBc * pb * ((len(G) if prune else npts) + rows * n2))
+        alg = {"fft_z": float(np.mean(bz)), "fft_y": float(np.mean(by)), "fft_x": float(np.mean(bx)),
+               "rs_augment": rs.npts * (16 * P + 2 * NBc * pb), "rs_emit": npts * (NBc * pb + 16)}
+        kr = {}
+        for c, b in alg.items():
+            kc = r["kernels"].get(c)
+            if not kc:
+                continue
+            ms = kc[0] / nchunks  # per chunk (one launch per augmentation colour)
+            kr[c] = {"bound": "hbm", "achieved": b / (ms * 1e-3) / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                     "alg_bytes_per_chunk": b, "ms_per_chunk": ms, "ms_per_step": kc[0], "launches": kc[1]}
+            kr[c]["frac"] = kr[c]["achieved"] / kr[c]["peak"]
+            kr[c]["traffic"] = rs_traffic.get(p, {}).get(c)
+        roof[p] = kr
+    # oracle check and CPU baseline on one state (band 0, k 1)
+    errs, cpu = {}, None
+    if not args.no_cpu:
+        k, b = 1, 0
+        W = orc.WaveData([wl.species])
+        pr = proj[k, b].cpu().numpy()
+        C0 = CS[k, b, :nw[k]].cpu().numpy()
+        t0 = time.perf_counter()
+        want = orc.realspace_state(wl.lattice, fine, wl.kpts[k], Gs[k], C0, W, labels, wl.frac_S, pr)
+        tc = time.perf_counter() - t0
+        cpu = {"value": 1.0 / tc, "unit": "band-states/s", "cores": 1, "kind": "port",
+               "sample": f"oracle realspace_state of one state (band {b}, k {k}) on {fine}: {tc:.1f} s"}
+        for p in res:
+            one = torch.zeros((1,) + fine, dtype=torch.complex128, device="cuda")
+            with torch.cuda.stream(stream):
+                B.realspace_states(ctx, rs, wl.kpts[k], Gdev[gofs[k]:gofs[k + 1]], CS[k, b:b + 1], proj[k, b:b + 1],
+                                   states_out=one, precision=p)
+            got = one[0].cpu().numpy()
+            errs[f"state_{p}"] = float(np.abs(got - want).max() / np.abs(want).max())
+            del one
+        log("errors vs the FP64 oracle:", json.dumps(errs), "cpu", json.dumps(cpu))
+    head = "fp64" if "fp64" in res else next(iter(res))
+    out = {"metric": "ae_density_band_states_per_sec", "value": res[head]["band_states_per_s"],
+           "unit": "band-states/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": res[head]["step_s"] * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": head, "data": "synthetic (seeded)",
+           "config": {"workload": "all-electron density of the 216-atom S cell (215 sites, partial-wave spheres "
+                                  "1.6 A) on the doubled grid 216^3, 4 k x 512 bands", "grid": list(fine),
+                      "sphere_points": rs.npts, "colours": rs.ncolors, "plan_s": plan_s},
+           "paths": {p: {k: v for k, v in r.items() if k != "kernels"} for p, r in res.items()},
+           "kernels": roof, "max_err_vs_fp64_oracle": errs, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+    rs.close()
+    plan.close()
 
 
 def cpu_and_errors(args, torch, wl, Gs, CR, CS, projR, projS, Om, ob, plans, results, head):

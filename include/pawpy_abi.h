@@ -256,7 +256,14 @@ int pp_ctx_set_tuning(pp_ctx *ctx, int fft_bands, int fft_tile, int fft_prune,
  * "fft_xmask" (1: the x pass of a setup writes only the grid points some
  * projector sphere holds, the only ones the sphere sums read; 0, default:
  * every point -- the masked stores measured slower on the 216-atom
- * workload although they write 31% fewer bytes). */
+ * workload although they write 31% fewer bytes), "rs_kernel" (real-space
+ * augmentation: 0 a site's coefficients in LDS, 1 in registers (up to 32
+ * projectors per site), 2, default: tiles of points with their table rows
+ * in LDS and the coefficients in registers (up to 18 projectors per site;
+ * else 0)), "rs_unroll" (1, 2 or 4,
+ * default 4: sphere points of a wavefront in flight in the real-space
+ * augmentation), "rs_prune" (1, default: the real-space FFT passes skip
+ * lines outside the plane-wave sphere). */
 int pp_ctx_set_param(pp_ctx *ctx, const char *name, int value);
 /* Current value of a parameter of pp_ctx_set_param (0 = the built-in
  * default for fft_tile, fft_bands, sphere_parts, gemm_splitk). */

@@ -185,7 +185,9 @@ static int setup_prepare(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac
   const int n0 = fftg[0], n1 = fftg[1], n2 = fftg[2];
   for (int k = 0; k < nk; k++) {
     const int *hb = ctx->bounds_host.data() + (size_t)k * S.nbnd;
-    S.alias[k] = hb[S.nbnd - 1];
+    // no pruning when the grid aliases a G or the G set leaves holes in a
+    // z-line's interval (not a cutoff sphere)
+    S.alias[k] = hb[S.nbnd - 1] || !bounds_filled(hb, fftg, gofs[k + 1] - gofs[k]);
     const int *yb = hb + 2 * n0 * n1, *xb = yb + 2 * n0;
     S.zl_off[k] = (long)ctx->lines_host.size();
     if (!S.alias[k])
@@ -501,6 +503,12 @@ int pp_ctx_set_param(pp_ctx *ctx, const char *name, int value) {
     ctx->tune_fft_nt = value < 0 ? 0 : value > 3 ? 3 : value;
   else if (n == "fused")
     ctx->tune_fused = value != 0 ? 1 : 0;
+  else if (n == "rs_kernel")
+    ctx->tune_rs_kernel = value < 0 ? 0 : value > 2 ? 2 : value;
+  else if (n == "rs_prune")
+    ctx->tune_rs_prune = value != 0 ? 1 : 0;
+  else if (n == "rs_unroll")
+    ctx->tune_rs_unroll = value >= 4 ? 4 : value >= 2 ? 2 : 1;
   else if (n == "fft_tile")
     return pp_ctx_set_tuning(ctx, ctx->tune_fft_bands, value, ctx->tune_fft_prune, ctx->tune_sphere_parts,
                              ctx->tune_splitk);
@@ -553,7 +561,8 @@ int pp_ctx_get_param(pp_ctx *ctx, const char *name, int *value) {
       {"fft_radix", ctx->tune_fft_radix}, {"fft_xcd", ctx->tune_fft_xcd},
       {"fft_xmask", ctx->tune_fft_xmask}, {"fft_nt", ctx->tune_fft_nt},
       {"sphere_nt", ctx->tune_sphere_nt},
-      {"fused", ctx->tune_fused},
+      {"fused", ctx->tune_fused}, {"rs_kernel", ctx->tune_rs_kernel}, {"rs_prune", ctx->tune_rs_prune},
+      {"rs_unroll", ctx->tune_rs_unroll},
       {"fft_tile", ctx->tune_fft_tile}, {"fft_prune", ctx->tune_fft_prune}, {"fft_bands", ctx->tune_fft_bands},
       {"sphere_parts", ctx->tune_sphere_parts}, {"gemm_splitk", ctx->tune_splitk}};
   for (const auto &e : table)

@@ -771,6 +771,15 @@ int sphere_bounds(pp_ctx *ctx, const int *fftg, int nw, const int *G_dev, int *b
   return PP_OK;
 }
 
+bool bounds_filled(const int *hb, const int *fftg, long nw) {
+  const long nl = (long)fftg[0] * fftg[1];
+  long tot = 0;
+  for (long l = 0; l < nl; l++)
+    if (hb[2 * l] <= hb[2 * l + 1])
+      tot += hb[2 * l + 1] - hb[2 * l] + 1;
+  return tot == nw;
+}
+
 // The three unpruned passes over a full chunk (no scatter, no zero fill).
 template <typename T>
 int fft_passes(pp_ctx *ctx, FftPlan *pl, cplx<T> *grid, int NB) {

@@ -106,6 +106,9 @@ struct pp_ctx {
   int tune_fft_xmask = 0;        // 1: the x pass writes only points a projector sphere holds
   int tune_fft_xcd = 0;         // 1: FFT pass work-groups in XCD-contiguous line ranges
   int tune_fused = 0;           // 1: x pass fused with the sphere sums where the plan allows
+  int tune_rs_kernel = 2;       // real-space augmentation: 0 LDS coefficients, 1 registers, 2 tiled
+  int tune_rs_unroll = 4;       // real-space augmentation: points of a wavefront in flight (1, 2, 4)
+  int tune_rs_prune = 1;        // real-space FFT passes restricted to the plane-wave sphere's lines
   hipStream_t aux = nullptr;    // second stream of the chunk pipeline (default priority)
   hipEvent_t fork = nullptr, join = nullptr;  // hand-over between the two streams
   int slot = 0;                 // chunk buffer set in use (0 or 1)
@@ -147,6 +150,10 @@ int gather_conj(pp_ctx *ctx, const cplx<T> *grid, int NB, int nbc, const int *gi
                 cplx<double> *out, long ldo);
 // bnd_dev: 2*n0*n1 + 2*n0 + 3 ints; the last is set when the grid aliases a G.
 int sphere_bounds(pp_ctx *ctx, const int *fftg, int nw, const int *G_dev, int *bnd_dev);
+// True when the G vectors fill every z-line interval of the host copy hb
+// of those bounds (sum of interval lengths == nw), as a cutoff sphere does;
+// the pruned passes rely on it, since they load no zero fill.
+bool bounds_filled(const int *hb, const int *fftg, long nw);
 int grid_to_bandmajor(pp_ctx *ctx, int precision, const void *grid, int NB, int nbc, long npts,
                       double scl, void *out);
 double det3(const double *m);

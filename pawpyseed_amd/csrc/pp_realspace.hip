@@ -174,9 +174,11 @@ __global__ __launch_bounds__(256) void k_rs_values(const RsSite *__restrict__ ge
 }
 
 // Adds the augmentation of the sites sites[blockIdx.y] to the band-fastest
-// chunk: lane = band; each wavefront takes points of the block's range.
+// chunk: lane = band, the site's coefficients in LDS; each wavefront takes
+// U points of the block's range at a time, their grid values loaded
+// together before the sums so that the read-modify-writes overlap.
 // proj rows: proj[b * ldp + proj_off + p] for the chunk's bands.
-template <typename T, bool ATOMIC>
+template <typename T, bool ATOMIC, int U>
 __global__ __launch_bounds__(256) void k_rs_augment(cplx<T> *__restrict__ grid, int NB, int nbc,
                                                     const RsSite *__restrict__ geo, const int *__restrict__ sites,
                                                     const int *__restrict__ site_pt, const int4 *__restrict__ pts,
@@ -200,31 +202,199 @@ __global__ __launch_bounds__(256) void k_rs_augment(cplx<T> *__restrict__ grid, 
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int q = begin + wv; q < end; q += 4) {
-    const int4 pt = pts[p0 + q];
-    const cplx<double> *row = tab + tab_off[s] + (long)q * P;
-    double ar = 0.0, ai = 0.0;
-    for (int p = 0; p < P; p++) {
-      const cplx<double> w = row[p], c = coef[p * 64 + lane];
-      ar += w.x * c.x - w.y * c.y;
-      ai += w.x * c.y + w.y * c.x;
+  const bool act = lane < nbc;
+  const cplx<double> *tsite = tab + tab_off[s];
+  for (int q0 = begin + wv * U; q0 < end; q0 += 4 * U) {
+    int4 pt[U];
+    cplx<T> v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      pt[u] = q0 + u < end ? pts[p0 + q0 + u] : make_int4(-1, 0, 0, 0);
+      v[u] = mk<T>((T)0, (T)0);
     }
+    if (!ATOMIC) {
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (pt[u].x >= 0 && act)
+          v[u] = grid[(long)pt[u].x * NB + lane];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (pt[u].x < 0)
+        continue;
+      const cplx<double> *row = tsite + (long)(q0 + u) * P;
+      double ar = 0.0, ai = 0.0;
+      for (int p = 0; p < P; p++) {
+        const cplx<double> w = row[p], c = coef[p * 64 + lane];
+        ar += w.x * c.x - w.y * c.y;
+        ai += w.x * c.y + w.y * c.x;
+      }
+      // e^{2 pi i k.(R + T)} e^{-2 pi i k.r} = e^{-2 pi i k.(r_unwrapped - R)}
+      const double kt = k0 * ((double)pt[u].y / f0 - g.R[0]) + k1 * ((double)pt[u].z / f1 - g.R[1]) +
+                        k2 * ((double)pt[u].w / f2 - g.R[2]);
+      double sn, cs;
+      sincospi(-2.0 * kt, &sn, &cs);
+      const double vr = (ar * cs - ai * sn) * iscl, vi = (ar * sn + ai * cs) * iscl;
+      if (act) {
+        cplx<T> *dst = grid + (long)pt[u].x * NB + lane;
+        if (ATOMIC) {
+          atomicAdd(&dst->x, (T)vr);
+          atomicAdd(&dst->y, (T)vi);
+        } else {
+          v[u].x += (T)vr;
+          v[u].y += (T)vi;
+          *dst = v[u];
+        }
+      }
+    }
+  }
+}
+
+// The same with the site's coefficients of the lane's band in registers
+// (P <= RP) and U points of a wavefront in flight: their grid values are
+// loaded together before the sums, so the read-modify-writes of one
+// wavefront overlap.
+template <typename T, bool ATOMIC, int RP, int U>
+__global__ __launch_bounds__(256) void k_rs_augment_r(cplx<T> *__restrict__ grid, int NB, int nbc,
+                                                      const RsSite *__restrict__ geo, const int *__restrict__ sites,
+                                                      const int *__restrict__ site_pt, const int4 *__restrict__ pts,
+                                                      const long *__restrict__ tab_off,
+                                                      const cplx<double> *__restrict__ tab,
+                                                      const cplx<double> *__restrict__ proj, long ldp, double k0,
+                                                      double k1, double k2, double iscl, int f0, int f1, int f2,
+                                                      int ppb) {
+  const int s = sites[blockIdx.y];
+  const RsSite g = geo[s];
+  const int P = g.nproj;
+  const int p0 = site_pt[s], np = site_pt[s + 1] - p0;
+  const int begin = blockIdx.x * ppb;
+  if (begin >= np)
+    return;
+  const int end = min(np, begin + ppb);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const bool act = lane < nbc;
+  cplx<double> cf[RP];
+#pragma unroll
+  for (int p = 0; p < RP; p++)
+    cf[p] = (p < P && act) ? proj[(long)lane * ldp + g.proj_off + p] : mk<double>(0.0, 0.0);
+  const cplx<double> *tsite = tab + tab_off[s];
+  for (int q0 = begin + wv * U; q0 < end; q0 += 4 * U) {
+    int4 pt[U];
+    cplx<T> v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      pt[u] = q0 + u < end ? pts[p0 + q0 + u] : make_int4(-1, 0, 0, 0);
+      v[u] = mk<T>((T)0, (T)0);
+    }
+    if (!ATOMIC) {
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (pt[u].x >= 0 && act)
+          v[u] = grid[(long)pt[u].x * NB + lane];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (pt[u].x < 0)
+        continue;
+      const cplx<double> *row = tsite + (long)(q0 + u) * P;
+      double ar = 0.0, ai = 0.0;
+#pragma unroll
+      for (int p = 0; p < RP; p++) {
+        if (p < P) {
+          const cplx<double> w = row[p];
+          ar += w.x * cf[p].x - w.y * cf[p].y;
+          ai += w.x * cf[p].y + w.y * cf[p].x;
+        }
+      }
+      const double kt = k0 * ((double)pt[u].y / f0 - g.R[0]) + k1 * ((double)pt[u].z / f1 - g.R[1]) +
+                        k2 * ((double)pt[u].w / f2 - g.R[2]);
+      double sn, cs;
+      sincospi(-2.0 * kt, &sn, &cs);
+      const double vr = (ar * cs - ai * sn) * iscl, vi = (ar * sn + ai * cs) * iscl;
+      if (act) {
+        cplx<T> *dst = grid + (long)pt[u].x * NB + lane;
+        if (ATOMIC) {
+          atomicAdd(&dst->x, (T)vr);
+          atomicAdd(&dst->y, (T)vi);
+        } else {
+          v[u].x += (T)vr;
+          v[u].y += (T)vi;
+          *dst = v[u];
+        }
+      }
+    }
+  }
+}
+
+// Tiled form: a work-group stages the table rows, grid indices and phase
+// factors of its ppb points in LDS (one contiguous read of the rows), the
+// lane's band coefficients sit in registers (P <= RP), and each wavefront
+// sums U points at a time from broadcast LDS reads of the rows.
+template <typename T, int RP, int U>
+__global__ __launch_bounds__(256) void k_rs_augment_t(cplx<T> *__restrict__ grid, int NB, int nbc,
+                                                      const RsSite *__restrict__ geo, const int *__restrict__ sites,
+                                                      const int *__restrict__ site_pt, const int4 *__restrict__ pts,
+                                                      const long *__restrict__ tab_off,
+                                                      const cplx<double> *__restrict__ tab,
+                                                      const cplx<double> *__restrict__ proj, long ldp, double k0,
+                                                      double k1, double k2, double iscl, int f0, int f1, int f2,
+                                                      int ppb) {
+  extern __shared__ cplx<double> smem[];
+  const int s = sites[blockIdx.y];
+  const RsSite g = geo[s];
+  const int P = g.nproj;
+  const int p0 = site_pt[s], np = site_pt[s + 1] - p0;
+  const int begin = blockIdx.x * ppb;
+  if (begin >= np)
+    return;  // uniform over the block
+  const int cnt = min(ppb, np - begin);
+  cplx<double> *Ts = smem;       // [cnt][P]
+  cplx<double> *Ph = Ts + (long)ppb * P;  // [cnt]
+  int *Ix = (int *)(Ph + ppb);   // [cnt]
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const bool act = lane < nbc;
+  const cplx<double> *rows = tab + tab_off[s] + (long)begin * P;
+  for (int t = threadIdx.x; t < cnt * P; t += 256)
+    Ts[t] = rows[t];
+  for (int t = threadIdx.x; t < cnt; t += 256) {
+    const int4 pt = pts[p0 + begin + t];
+    Ix[t] = pt.x;
     // e^{2 pi i k.(R + T)} e^{-2 pi i k.r} = e^{-2 pi i k.(r_unwrapped - R)}
     const double kt = k0 * ((double)pt.y / f0 - g.R[0]) + k1 * ((double)pt.z / f1 - g.R[1]) +
                       k2 * ((double)pt.w / f2 - g.R[2]);
     double sn, cs;
     sincospi(-2.0 * kt, &sn, &cs);
-    const double vr = (ar * cs - ai * sn) * iscl, vi = (ar * sn + ai * cs) * iscl;
-    if (lane < nbc) {
-      cplx<T> *dst = grid + (long)pt.x * NB + lane;
-      if (ATOMIC) {
-        atomicAdd(&dst->x, (T)vr);
-        atomicAdd(&dst->y, (T)vi);
-      } else {
-        cplx<T> o = *dst;
-        o.x += (T)vr;
-        o.y += (T)vi;
-        *dst = o;
+    Ph[t] = mk<double>(cs * iscl, sn * iscl);
+  }
+  cplx<double> cf[RP];
+#pragma unroll
+  for (int p = 0; p < RP; p++)
+    cf[p] = (p < P && act) ? proj[(long)lane * ldp + g.proj_off + p] : mk<double>(0.0, 0.0);
+  __syncthreads();
+  for (int q0 = wv * U; q0 < cnt; q0 += 4 * U) {
+    cplx<T> v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      v[u] = (q0 + u < cnt && act) ? grid[(long)Ix[q0 + u] * NB + lane] : mk<T>((T)0, (T)0);
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (q0 + u >= cnt)
+        break;
+      const cplx<double> *row = Ts + (q0 + u) * P;
+      double ar = 0.0, ai = 0.0;
+#pragma unroll
+      for (int p = 0; p < RP; p++) {
+        if (p < P) {
+          const cplx<double> w = row[p];
+          ar += w.x * cf[p].x - w.y * cf[p].y;
+          ai += w.x * cf[p].y + w.y * cf[p].x;
+        }
+      }
+      const cplx<double> e = Ph[q0 + u];
+      if (act) {
+        v[u].x += (T)(ar * e.x - ai * e.y);
+        v[u].y += (T)(ar * e.y + ai * e.x);
+        grid[(long)Ix[q0 + u] * NB + lane] = v[u];
       }
     }
   }
@@ -546,14 +716,61 @@ extern "C" int pp_rs_plan_info(pp_rsplan *rp, int *total_projs, long *total_poin
 
 namespace {
 
+// Plane-wave sphere bounds of one k-point on the plan's grid and the
+// populated lines of the z and y passes (one host synchronisation), so the
+// passes skip empty lines and the chunk needs no zero fill.  prune is
+// false when the grid aliases a G vector or the G set does not fill its
+// z-line intervals (the passes then run in full).
+int rs_bounds(pp_ctx *ctx, const int *fftg, int nw, const int *G_dev, SphereBounds &bd, bool &prune) {
+  const int n0 = fftg[0], n1 = fftg[1], n2 = fftg[2];
+  const int nbnd = 2 * n0 * n1 + 2 * n0 + 3;
+  prune = false;
+  if (nw == 0 || !ctx->tune_rs_prune)
+    return PP_OK;
+  PP_TRY(ctx->bounds.ensure(sizeof(int) * (size_t)nbnd));
+  int *bnd = (int *)ctx->bounds.p;
+  PP_TRY(sphere_bounds(ctx, fftg, nw, G_dev, bnd));
+  ctx->bounds_host.resize(nbnd);
+  PP_CHECK_HIP(hipMemcpyAsync(ctx->bounds_host.data(), bnd, sizeof(int) * nbnd, hipMemcpyDeviceToHost, ctx->stream));
+  PP_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  const int *hb = ctx->bounds_host.data();
+  if (hb[nbnd - 1] || !bounds_filled(hb, fftg, nw))
+    return PP_OK;  // aliasing, or a G set with holes: no pruning
+  bd = SphereBounds{bnd, bnd + 2 * n0 * n1, bnd + 2 * n0 * n1 + 2 * n0, n0, n1, n2};
+  const int *yb = hb + 2 * n0 * n1, *xb = yb + 2 * n0;
+  ctx->lines_host.clear();
+  for (int sx = xb[0]; sx <= xb[1]; sx++) {
+    const int x = (sx + n0) % n0;
+    for (int sy = yb[2 * x]; sy <= yb[2 * x + 1]; sy++)
+      ctx->lines_host.push_back(x * n1 + (sy + n1) % n1);
+  }
+  const int nzl = (int)ctx->lines_host.size();
+  for (int sx = xb[0]; sx <= xb[1]; sx++)
+    for (int z = 0; z < n2; z++)
+      ctx->lines_host.push_back(((sx + n0) % n0) * n2 + z);
+  if (!ctx->lines_host.empty()) {
+    PP_TRY(ctx->lines.ensure(sizeof(int) * ctx->lines_host.size()));
+    // the stream is idle (synchronised above); lines_host stays in the
+    // context until the next call's synchronisation
+    PP_CHECK_HIP(hipMemcpyAsync(ctx->lines.p, ctx->lines_host.data(), sizeof(int) * ctx->lines_host.size(),
+                                hipMemcpyHostToDevice, ctx->stream));
+    bd.zl = (const int *)ctx->lines.p;
+    bd.nzl = nzl;
+    bd.yl = (const int *)ctx->lines.p + nzl;
+    bd.nyl = (int)ctx->lines_host.size() - nzl;
+  }
+  prune = true;
+  return PP_OK;
+}
+
 template <typename T>
 int rs_chunk(pp_ctx *ctx, pp_rsplan *rp, FftPlan *pl, const double *kf, int nw, const int *gidx,
              const cplx<float> *C, long ldc, int nbc, const cplx<double> *proj, long ldp, int remove_phase,
-             cplx<double> *states, long ldo, const double *wts, double *dens) {
+             cplx<double> *states, long ldo, const double *wts, double *dens, const SphereBounds *bd) {
   const int NB = nbc;
   const long npts = (long)rp->fftg[0] * rp->fftg[1] * rp->fftg[2];
   cplx<T> *grid = (cplx<T> *)ctx->grid.p;
-  PP_TRY(fft_chunk<T>(ctx, pl, grid, NB, nw, gidx, C, ldc, nbc, nullptr));
+  PP_TRY(fft_chunk<T>(ctx, pl, grid, NB, nw, gidx, C, ldc, nbc, bd));
   const double scl = 1.0 / sqrt(det3(rp->lattice));
   hipEvent_t ev;
   if (rp->nsites && rp->ptot) {
@@ -565,18 +782,60 @@ int rs_chunk(pp_ctx *ctx, pp_rsplan *rp, FftPlan *pl, const double *kf, int nw, 
       const int nsc = rp->color_off[c + 1] - rp->color_off[c];
       const int gx = std::max(1, (rp->color_maxpts[c] + ppb - 1) / ppb);
       const int *sites = (const int *)rp->d_colors.p + rp->color_off[c];
-      if (rp->atomic)
-        hipLaunchKernelGGL((k_rs_augment<T, true>), dim3(gx, nsc), dim3(256), lds, ctx->stream, grid, NB, nbc,
-                           (const RsSite *)rp->d_geo.p, sites, (const int *)rp->d_site_pt.p,
-                           (const int4 *)rp->d_pts.p, (const long *)rp->d_tab_off.p,
-                           (const cplx<double> *)rp->d_tab.p, proj, ldp, kf[0], kf[1], kf[2], 1.0 / scl,
-                           rp->fftg[0], rp->fftg[1], rp->fftg[2], ppb);
-      else
-        hipLaunchKernelGGL((k_rs_augment<T, false>), dim3(gx, nsc), dim3(256), lds, ctx->stream, grid, NB, nbc,
-                           (const RsSite *)rp->d_geo.p, sites, (const int *)rp->d_site_pt.p,
-                           (const int4 *)rp->d_pts.p, (const long *)rp->d_tab_off.p,
-                           (const cplx<double> *)rp->d_tab.p, proj, ldp, kf[0], kf[1], kf[2], 1.0 / scl,
-                           rp->fftg[0], rp->fftg[1], rp->fftg[2], ppb);
+#define RS_AUG_R(AT, RP)                                                                                         \
+  hipLaunchKernelGGL((k_rs_augment_r<T, AT, RP, 4>), dim3(gx, nsc), dim3(256), 0, ctx->stream, grid, NB, nbc,        \
+                     (const RsSite *)rp->d_geo.p, sites, (const int *)rp->d_site_pt.p, (const int4 *)rp->d_pts.p,   \
+                     (const long *)rp->d_tab_off.p, (const cplx<double> *)rp->d_tab.p, proj, ldp, kf[0], kf[1],      \
+                     kf[2], 1.0 / scl, rp->fftg[0], rp->fftg[1], rp->fftg[2], ppb)
+      if (rp->pmax <= 18 && ctx->tune_rs_kernel == 2 && !rp->atomic) {
+        const size_t tl = (sizeof(cplx<double>) * ((size_t)ppb * rp->pmax + ppb) + sizeof(int) * ppb);
+        const int u = ctx->tune_rs_unroll;
+        if (u >= 4)
+          hipLaunchKernelGGL((k_rs_augment_t<T, 18, 4>), dim3(gx, nsc), dim3(256), tl, ctx->stream, grid, NB, nbc,
+                             (const RsSite *)rp->d_geo.p, sites, (const int *)rp->d_site_pt.p,
+                             (const int4 *)rp->d_pts.p, (const long *)rp->d_tab_off.p,
+                             (const cplx<double> *)rp->d_tab.p, proj, ldp, kf[0], kf[1], kf[2], 1.0 / scl,
+                             rp->fftg[0], rp->fftg[1], rp->fftg[2], ppb);
+        else if (u == 2)
+          hipLaunchKernelGGL((k_rs_augment_t<T, 18, 2>), dim3(gx, nsc), dim3(256), tl, ctx->stream, grid, NB, nbc,
+                             (const RsSite *)rp->d_geo.p, sites, (const int *)rp->d_site_pt.p,
+                             (const int4 *)rp->d_pts.p, (const long *)rp->d_tab_off.p,
+                             (const cplx<double> *)rp->d_tab.p, proj, ldp, kf[0], kf[1], kf[2], 1.0 / scl,
+                             rp->fftg[0], rp->fftg[1], rp->fftg[2], ppb);
+        else
+          hipLaunchKernelGGL((k_rs_augment_t<T, 18, 1>), dim3(gx, nsc), dim3(256), tl, ctx->stream, grid, NB, nbc,
+                             (const RsSite *)rp->d_geo.p, sites, (const int *)rp->d_site_pt.p,
+                             (const int4 *)rp->d_pts.p, (const long *)rp->d_tab_off.p,
+                             (const cplx<double> *)rp->d_tab.p, proj, ldp, kf[0], kf[1], kf[2], 1.0 / scl,
+                             rp->fftg[0], rp->fftg[1], rp->fftg[2], ppb);
+      } else if (rp->pmax <= 18 && ctx->tune_rs_kernel == 1) {
+        if (rp->atomic)
+          RS_AUG_R(true, 18);
+        else
+          RS_AUG_R(false, 18);
+      } else if (rp->pmax <= 32 && ctx->tune_rs_kernel == 1) {
+        if (rp->atomic)
+          RS_AUG_R(true, 32);
+        else
+          RS_AUG_R(false, 32);
+      } else {
+#define RS_AUG_L(AT, U)                                                                                          \
+  hipLaunchKernelGGL((k_rs_augment<T, AT, U>), dim3(gx, nsc), dim3(256), lds, ctx->stream, grid, NB, nbc,           \
+                     (const RsSite *)rp->d_geo.p, sites, (const int *)rp->d_site_pt.p, (const int4 *)rp->d_pts.p,   \
+                     (const long *)rp->d_tab_off.p, (const cplx<double> *)rp->d_tab.p, proj, ldp, kf[0], kf[1],      \
+                     kf[2], 1.0 / scl, rp->fftg[0], rp->fftg[1], rp->fftg[2], ppb)
+        const int u = ctx->tune_rs_unroll;
+        if (rp->atomic)
+          RS_AUG_L(true, 1);
+        else if (u >= 4)
+          RS_AUG_L(false, 4);
+        else if (u == 2)
+          RS_AUG_L(false, 2);
+        else
+          RS_AUG_L(false, 1);
+#undef RS_AUG_L
+      }
+#undef RS_AUG_R
       PP_CHECK_HIP(hipGetLastError());
     }
     tend(ctx, K_RS_AUG, ev);
@@ -629,6 +888,9 @@ extern "C" int pp_realspace_batched(pp_ctx *ctx, pp_rsplan *rp, const double *kf
   PP_TRY(ctx->grid.ensure(csz * (size_t)npts * nbmax));
   PP_TRY(ctx->gidx.ensure(sizeof(int) * (size_t)std::max(1, nw)));
   PP_TRY(compute_gidx(ctx, rp->fftg, nw, G_dev, (int *)ctx->gidx.p));
+  SphereBounds bd{};
+  bool prune = false;
+  PP_TRY(rs_bounds(ctx, rp->fftg, nw, G_dev, bd, prune));
   for (int b0 = 0; b0 < nband; b0 += 64) {
     const int nbc = std::min(64, nband - b0);
     if (density_dev && !states_dev) {
@@ -644,10 +906,10 @@ extern "C" int pp_realspace_batched(pp_ctx *ctx, pp_rsplan *rp, const double *kf
     const double *W = density_dev ? weights + b0 : nullptr;
     if (f64)
       PP_TRY(rs_chunk<double>(ctx, rp, pl, kfrac, nw, (const int *)ctx->gidx.p, C, ldc, nbc, P, ldp, remove_phase,
-                              S, ldo, W, density_dev));
+                              S, ldo, W, density_dev, prune ? &bd : nullptr));
     else
       PP_TRY(rs_chunk<float>(ctx, rp, pl, kfrac, nw, (const int *)ctx->gidx.p, C, ldc, nbc, P, ldp, remove_phase, S,
-                             ldo, W, density_dev));
+                             ldo, W, density_dev, prune ? &bd : nullptr));
   }
   return PP_OK;
 }
