@@ -535,6 +535,9 @@ def bench_aedensity(args, torch, B):
             e1.record(stream)
         torch.cuda.synchronize()
         t = e0.elapsed_time(e1) / 1e3 / args.steps
+        # breakdown step: one chunk in flight, HIP events around every launch
+        nstreams = ctx.get_param("streams")
+        ctx.set_param("streams", 1)
         ctx.set_timing(True)
         ctx.timing_reset()
         with torch.cuda.stream(stream):
@@ -542,6 +545,7 @@ def bench_aedensity(args, torch, B):
         torch.cuda.synchronize()
         kern = ctx.timing()
         ctx.set_timing(False)
+        ctx.set_param("streams", nstreams)
         res[p] = {"step_s": t, "band_states_per_s": nb * nk / t, "kernels": kern}
         log(p, json.dumps({k: v for k, v in res[p].items() if k != "kernels"}))
         log(p, "kernels", json.dumps(kern))

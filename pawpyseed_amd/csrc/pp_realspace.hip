@@ -17,8 +17,9 @@
 // augmentation with plain read-modify-writes (deterministic); when a
 // site's box wraps onto itself the plan uses atomic adds instead.
 //
-// Per band chunk (pp_realspace_batched): scatter + backward FFT into the
-// band-fastest chunk (the projection path's kernels), augmentation added
+// Per band chunk (pp_realspace_batched; two chunks in flight on two
+// streams): scatter + backward FFT into the band-fastest chunk (the
+// projection path's kernels), augmentation added
 // in the chunk with the phase e^{-2 pi i k.(r_unwrapped - R)} / scale so
 // that the emit kernel's common factor scale e^{2 pi i k.r} restores the
 // reference's terms, then either the band-major states (LDS transpose,
@@ -462,6 +463,12 @@ __global__ __launch_bounds__(256) void k_rs_density(const cplx<T> *__restrict__ 
   }
 }
 
+__global__ __launch_bounds__(256) void k_rs_add(double *__restrict__ d, const double *__restrict__ e, long n) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n)
+    d[i] += e[i];
+}
+
 }  // namespace pp
 
 using namespace pp;
@@ -476,6 +483,7 @@ struct pp_rsplan {
   std::vector<int> proj_off, site_pt;
   std::vector<int> color_off, color_sites, color_maxpts;
   DevBuf d_geo, d_pts, d_tab, d_tab_off, d_site_pt, d_colors, d_spl, d_chan_l;
+  DevBuf dens2;  // densities of the second stream's chunks
 };
 
 namespace {
@@ -766,10 +774,10 @@ int rs_bounds(pp_ctx *ctx, const int *fftg, int nw, const int *G_dev, SphereBoun
 template <typename T>
 int rs_chunk(pp_ctx *ctx, pp_rsplan *rp, FftPlan *pl, const double *kf, int nw, const int *gidx,
              const cplx<float> *C, long ldc, int nbc, const cplx<double> *proj, long ldp, int remove_phase,
-             cplx<double> *states, long ldo, const double *wts, double *dens, const SphereBounds *bd) {
+             cplx<double> *states, long ldo, const double *wts, double *dens, const SphereBounds *bd,
+             cplx<T> *grid) {
   const int NB = nbc;
   const long npts = (long)rp->fftg[0] * rp->fftg[1] * rp->fftg[2];
-  cplx<T> *grid = (cplx<T> *)ctx->grid.p;
   PP_TRY(fft_chunk<T>(ctx, pl, grid, NB, nw, gidx, C, ldc, nbc, bd));
   const double scl = 1.0 / sqrt(det3(rp->lattice));
   hipEvent_t ev;
@@ -891,27 +899,73 @@ extern "C" int pp_realspace_batched(pp_ctx *ctx, pp_rsplan *rp, const double *kf
   SphereBounds bd{};
   bool prune = false;
   PP_TRY(rs_bounds(ctx, rp->fftg, nw, G_dev, bd, prune));
+  // chunks to run (a density-only call skips chunks whose weights are all 0)
+  std::vector<int> starts;
   for (int b0 = 0; b0 < nband; b0 += 64) {
-    const int nbc = std::min(64, nband - b0);
-    if (density_dev && !states_dev) {
-      bool any = false;
-      for (int b = 0; b < nbc; b++)
-        any = any || weights[b0 + b] != 0.0;
-      if (!any)
-        continue;  // no state of this chunk contributes
+    bool any = !density_dev || states_dev;
+    for (int b = 0; b < std::min(64, nband - b0) && !any; b++)
+      any = weights[b0 + b] != 0.0;
+    if (any)
+      starts.push_back(b0);
+  }
+  // two chunks in flight on two streams (default priority), each with its
+  // own chunk buffer; the second stream's densities go to a buffer of the
+  // plan that is added to density_dev at the end
+  const bool two = ctx->tune_streams > 1 && starts.size() > 1;
+  hipStream_t streams[2] = {ctx->stream, ctx->stream};
+  if (two) {
+    PP_TRY(ctx->grid2.ensure(csz * (size_t)npts * nbmax));
+    if (!ctx->aux)
+      PP_CHECK_HIP(hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking));
+    if (!ctx->fork)
+      PP_CHECK_HIP(hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming));
+    if (!ctx->join)
+      PP_CHECK_HIP(hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming));
+    if (density_dev)
+      PP_TRY(rp->dens2.ensure(sizeof(double) * (size_t)npts));
+    streams[1] = ctx->aux;
+    PP_CHECK_HIP(hipEventRecord(ctx->fork, ctx->stream));
+    PP_CHECK_HIP(hipStreamWaitEvent(ctx->aux, ctx->fork, 0));
+    if (density_dev) {
+      ctx->stream = ctx->aux;
+      int zs = zero_async(ctx, rp->dens2.p, sizeof(double) * (size_t)npts);
+      ctx->stream = streams[0];
+      PP_TRY(zs);
     }
+  }
+  int st = PP_OK;
+  for (size_t c = 0; c < starts.size() && st == PP_OK; c++) {
+    const int b0 = starts[c];
+    const int nbc = std::min(64, nband - b0);
+    const int sl = two ? (int)(c & 1) : 0;
+    ctx->stream = streams[sl];
+    ctx->slot = sl;
+    void *grid = sl ? ctx->grid2.p : ctx->grid.p;
+    double *dens = density_dev ? (sl ? (double *)rp->dens2.p : density_dev) : nullptr;
     const cplx<float> *C = (const cplx<float> *)C_dev + (size_t)b0 * ldc;
     const cplx<double> *P = proj_dev ? (const cplx<double> *)proj_dev + (size_t)b0 * ldp : nullptr;
     cplx<double> *S = states_dev ? (cplx<double> *)states_dev + (size_t)b0 * ldo : nullptr;
     const double *W = density_dev ? weights + b0 : nullptr;
     if (f64)
-      PP_TRY(rs_chunk<double>(ctx, rp, pl, kfrac, nw, (const int *)ctx->gidx.p, C, ldc, nbc, P, ldp, remove_phase,
-                              S, ldo, W, density_dev, prune ? &bd : nullptr));
+      st = rs_chunk<double>(ctx, rp, pl, kfrac, nw, (const int *)ctx->gidx.p, C, ldc, nbc, P, ldp, remove_phase, S,
+                            ldo, W, dens, prune ? &bd : nullptr, (cplx<double> *)grid);
     else
-      PP_TRY(rs_chunk<float>(ctx, rp, pl, kfrac, nw, (const int *)ctx->gidx.p, C, ldc, nbc, P, ldp, remove_phase, S,
-                             ldo, W, density_dev, prune ? &bd : nullptr));
+      st = rs_chunk<float>(ctx, rp, pl, kfrac, nw, (const int *)ctx->gidx.p, C, ldc, nbc, P, ldp, remove_phase, S,
+                           ldo, W, dens, prune ? &bd : nullptr, (cplx<float> *)grid);
   }
-  return PP_OK;
+  ctx->stream = streams[0];
+  ctx->slot = 0;
+  if (two) {
+    // the caller's stream sees everything the second stream did
+    (void)hipEventRecord(ctx->join, ctx->aux);
+    (void)hipStreamWaitEvent(ctx->stream, ctx->join, 0);
+    if (st == PP_OK && density_dev) {
+      hipLaunchKernelGGL(k_rs_add, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, ctx->stream, density_dev,
+                         (const double *)rp->dens2.p, npts);
+      PP_CHECK_HIP(hipGetLastError());
+    }
+  }
+  return st;
 }
 
 namespace pp {

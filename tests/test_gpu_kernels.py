@@ -404,3 +404,35 @@ def test_xmask_writes_only_covered_points(ctx, precision):
     ctx.set_param("fft_xmask", 1)
     assert np.array_equal(outs[0], outs[1])
     plan.close()
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-9), ("fp32", 1e-5)])
+def test_projections_g_set_with_holes(ctx, precision, tol):
+    """A plane-wave list that is not a cutoff sphere (a random subset of
+    it, so z-line intervals have holes): the pruned passes, which load no
+    zero fill, must not be used (the interval-fill check on the host
+    bounds), and the projections still equal the oracle's."""
+    from pawpyseed_amd.core import batched as B
+    wl = synthetic.tiny_config(encut=150.0, fftg=(24, 24, 24), nband=6, kpts=[[0.25, -0.25, 0.1]],
+                               ls=(0, 1, 2), alphas=(1.5, 3.0, 1.5))
+    G = wl.ensure_gvecs()[0]
+    keep = np.sort(np.random.default_rng(3).choice(len(G), size=len(G) * 2 // 3, replace=False))
+    Gs = [np.ascontiguousarray(G[keep])]
+    Cs = [wl.coefficients("S", nband=6)[0][:, keep]]
+    labels = np.zeros(len(wl.frac_S), np.int32)
+    T = orc.site_tables(wl.lattice, wl.fftg, wl.frac_S, [wl.species], labels)
+    want = orc.setup_projections(wl.lattice, wl.fftg, wl.kpts, Gs, Cs, T)
+    Cd, Gd, gofs, nw = device_inputs(Gs, Cs)
+    plan = B.SpherePlan(ctx, wl.lattice, wl.fftg, wl.frac_S, labels, [wl.species], precision)
+    out = torch.zeros((1, 6, plan.ptot), dtype=torch.complex128, device="cuda")
+    # garbage in the work buffers first: a pruned pass would read it
+    big = synthetic.tiny_config(encut=150.0, fftg=(24, 24, 24), nband=64, kpts=[[0, 0, 0]])
+    Cb, Gb, gob, _ = device_inputs(big.ensure_gvecs(), big.coefficients("R", nband=64))
+    junk = torch.zeros((1, 64, plan.ptot), dtype=torch.complex128, device="cuda")
+    B.setup_projections(ctx, plan, big.kpts, gob, Gb, Cb, junk, precision)
+    B.setup_projections(ctx, plan, wl.kpts, gofs, Gd, Cd, out, precision)
+    torch.cuda.synchronize()
+    err = normwise(out.cpu().numpy(), want)
+    print("projections with a holed G set", precision, err, flush=True)
+    assert err < tol
+    plan.close()
