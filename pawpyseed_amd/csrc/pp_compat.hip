@@ -1548,6 +1548,11 @@ int project_realspace_state(pp_cdouble *projs, int BAND_NUM, pswf_t *wf, pswf_t 
   PP_REQUIRE(BAND_NUM >= 0 && BAND_NUM < wf->nband, PP_ERR_ARG, "project_realspace_state: band index out of range");
   PP_REQUIRE(wf_R->nband >= wf->nband && wf_R->nwk * wf_R->nspin >= wf->nwk * wf->nspin, PP_ERR_ARG,
              "project_realspace_state: the basis has fewer bands or k-points than the projected wavefunction");
+  if (wf == wf_R)  // one wavefunction holds one cached plan
+    for (int i = 0; i < wf->num_sites; i++)
+      PP_REQUIRE(labels[i] == labels_R[i] && coords[3 * i] == coords_R[3 * i] &&
+                     coords[3 * i + 1] == coords_R[3 * i + 1] && coords[3 * i + 2] == coords_R[3 * i + 2],
+                 PP_ERR_ARG, "project_realspace_state: one wavefunction with two different site lists");
   std::lock_guard<std::mutex> lk(g_mu);
   pp_ctx *ctx;
   PP_TRY(global_ctx(&ctx));

@@ -533,6 +533,7 @@ extern "C" int pp_rs_plan_create(pp_ctx *ctx, pp_rsplan **out, const double *lat
   PP_REQUIRE(ctx && out && lattice && fftg && num_sites >= 0 && num_elems > 0 && nchan && wave_n, PP_ERR_ARG,
              "pp_rs_plan_create: bad arguments");
   PP_REQUIRE(num_sites == 0 || (labels && frac_coords), PP_ERR_ARG, "pp_rs_plan_create: null site arrays");
+  PP_REQUIRE(num_sites <= 65535, PP_ERR_ARG, "pp_rs_plan_create: more than 65535 sites (one grid row per site)");
   PP_REQUIRE(fftg[0] > 0 && fftg[1] > 0 && fftg[2] > 0 && (long)fftg[0] * fftg[1] * fftg[2] < (1L << 31),
              PP_ERR_ARG, "pp_rs_plan_create: bad grid");
   const double vol = det3(lattice);

@@ -236,3 +236,21 @@ def test_pswf_from_arrays_accessors():
     assert L.get_nspin(pw.ptr) == 2 and L.get_nwk(pw.ptr) == 1
     assert L.get_occ(pw.ptr, 2, 0, 0) == 0.2 and L.get_occ(pw.ptr, 0, 0, 1) == 1.0
     L.free_pswf(pw.ptr)
+
+
+def test_write_volumetric_format(tmp_path):
+    """density.c:461-477: values x[i][j][k] * scale with z slowest and x
+    fastest, printf "%E   " each, a newline after every fifth value; the
+    VASP-style header of wavefunction.py:480-510 in front."""
+    from pawpyseed_amd.core.wavefunction import write_volumetric
+    dim = (3, 4, 2)
+    x = np.arange(24, dtype=np.float64).reshape(dim) * 0.5 - 3.0
+    p = tmp_path / "v.vasp"
+    write_volumetric(str(p), x, dim, 2.0)
+    txt = p.read_text()
+    lines = txt.split("\n")
+    assert all(len(l.split()) == 5 for l in lines[:4]) and len(lines[4].split()) == 4
+    vals = [float(v) for v in txt.split()]
+    want = [x[i, j, k] * 2.0 for k in range(2) for j in range(4) for i in range(3)]
+    assert vals == want
+    assert txt.split()[0] == "%E" % want[0] and "%E   " % want[1] in txt
