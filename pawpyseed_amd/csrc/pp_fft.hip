@@ -362,25 +362,49 @@ __global__ __launch_bounds__(NT) void k_fft_pass_ip(cplx<T> *__restrict__ g, Axi
   const int tot = n * NBW;
   cplx<T> *twl = buf + tot;
   stage_twiddles<T, NT>(twl, tw, pl.twlen);
-  for (int t = threadIdx.x; t < tot; t += NT) {
-    int e = t / NBW, b = t % NBW;
-    const int se = sgn_idx(e, n);
-    cplx<T> v = mk<T>(0, 0);
-    if (se >= lo && se <= hi)
-      v = streaming > 1 ? load_nt(g + base + (long)e * es + b) : g[base + (long)e * es + b];
-    buf[t] = v;
+  // the line's loads are issued in batches of FFT_LB per thread before any
+  // is written to LDS, so that many are in flight at once (a load-then-store
+  // loop waits for each load in turn)
+  for (int t0 = threadIdx.x; t0 < tot; t0 += NT * FFT_LB) {
+    cplx<T> v[FFT_LB];
+#pragma unroll
+    for (int u = 0; u < FFT_LB; u++) {
+      const int t = t0 + u * NT;
+      v[u] = mk<T>(0, 0);
+      if (t < tot) {
+        const int e = t / NBW, b = t % NBW;
+        const int se = sgn_idx(e, n);
+        if (se >= lo && se <= hi)
+          v[u] = streaming > 1 ? load_nt(g + base + (long)e * es + b) : g[base + (long)e * es + b];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < FFT_LB; u++)
+      if (t0 + u * NT < tot)
+        buf[t0 + u * NT] = v[u];
   }
   __syncthreads();
   stages_ip<T, NBW, MAXB, NT, COMP>(buf, twl, pl, n);
   const unsigned *xm = (AXIS == 0 && bd.xm) ? bd.xm + (long)line * bd.xmw : nullptr;
-  for (int t = threadIdx.x; t < tot; t += NT) {
-    int e = t / NBW, b = t % NBW;
-    if (xm && !((xm[e >> 5] >> (e & 31)) & 1u))
-      continue;  // no projector sphere reads this point
-    if (streaming)
-      store_nt(g + base + (long)e * es + b, buf[t]);
-    else
-      g[base + (long)e * es + b] = buf[t];
+  for (int t0 = threadIdx.x; t0 < tot; t0 += NT * FFT_LB) {
+    cplx<T> v[FFT_LB];
+#pragma unroll
+    for (int u = 0; u < FFT_LB; u++)
+      if (t0 + u * NT < tot)
+        v[u] = buf[t0 + u * NT];
+#pragma unroll
+    for (int u = 0; u < FFT_LB; u++) {
+      const int t = t0 + u * NT;
+      if (t >= tot)
+        break;
+      const int e = t / NBW, b = t % NBW;
+      if (xm && !((xm[e >> 5] >> (e & 31)) & 1u))
+        continue;  // no projector sphere reads this point
+      if (streaming)
+        store_nt(g + base + (long)e * es + b, v[u]);
+      else
+        g[base + (long)e * es + b] = v[u];
+    }
   }
 }
 

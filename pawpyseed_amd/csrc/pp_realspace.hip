@@ -426,9 +426,20 @@ __global__ __launch_bounds__(256) void k_rs_state(const cplx<T> *__restrict__ gr
     ph[threadIdx.x] = e;
   }
   const cplx<T> *src = grid + pt0 * NB;
-  for (int t = threadIdx.x; t < npt * NB; t += 256) {
-    const int p = t / NB, b = t - p * NB;
-    tile[p * 65 + b] = cvt<double>(src[t]);
+  // all of a thread's loads in flight before the LDS writes
+  cplx<T> v[RS_TP * 64 / 256];
+#pragma unroll
+  for (int u = 0; u < RS_TP * 64 / 256; u++) {
+    const int t = threadIdx.x + u * 256;
+    v[u] = t < npt * NB ? src[t] : mk<T>((T)0, (T)0);
+  }
+#pragma unroll
+  for (int u = 0; u < RS_TP * 64 / 256; u++) {
+    const int t = threadIdx.x + u * 256;
+    if (t < npt * NB) {
+      const int p = t / NB, b = t - p * NB;
+      tile[p * 65 + b] = cvt<double>(v[u]);
+    }
   }
   __syncthreads();
   for (int t = threadIdx.x; t < RS_TP * nbc; t += 256) {
@@ -447,19 +458,26 @@ __global__ __launch_bounds__(256) void k_rs_density(const cplx<T> *__restrict__ 
   const int lane = threadIdx.x & 63;
   const long w0 = ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * ppw;
   const double wl = lane < nbc ? W.w[lane] : 0.0;
-  for (int q = 0; q < ppw; q++) {
-    const long pt = w0 + q;
-    if (pt >= npts)
-      break;
-    double v = 0.0;
-    if (lane < nbc) {
-      const cplx<double> a = cvt<double>(grid[pt * NB + lane]);
-      v = wl * (a.x * a.x + a.y * a.y);
+  constexpr int U = 4;  // points whose loads are in flight together
+  for (int q0 = 0; q0 < ppw; q0 += U) {
+    cplx<T> a[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const long pt = w0 + q0 + u;
+      a[u] = (q0 + u < ppw && pt < npts && lane < nbc) ? grid[pt * NB + lane] : mk<T>((T)0, (T)0);
     }
-    for (int o = 32; o > 0; o >>= 1)
-      v += __shfl_xor(v, o, 64);
-    if (lane == 0)
-      dens[pt] += scl2 * v;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const long pt = w0 + q0 + u;
+      if (q0 + u >= ppw || pt >= npts)
+        break;  // uniform over the wavefront
+      const cplx<double> c = cvt<double>(a[u]);
+      double v = wl * (c.x * c.x + c.y * c.y);
+      for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o, 64);
+      if (lane == 0)
+        dens[pt] += scl2 * v;
+    }
   }
 }
 

@@ -57,9 +57,21 @@ __device__ __forceinline__ void stage_ip(cplx<T> *__restrict__ buf, const cplx<T
 // caller's next barrier).
 template <typename T, int NT = 256>
 __device__ __forceinline__ void stage_twiddles(cplx<T> *__restrict__ dst, const cplx<T> *__restrict__ tw, int len) {
-  for (int i = threadIdx.x; i < len; i += NT)
-    dst[i] = tw[i];
+  for (int i0 = threadIdx.x; i0 < len; i0 += 4 * NT) {
+    cplx<T> v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (i0 + u * NT < len)
+        v[u] = tw[i0 + u * NT];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (i0 + u * NT < len)
+        dst[i0 + u * NT] = v[u];
+  }
 }
+
+// loads per thread issued together when a pass stages its line in LDS
+constexpr int FFT_LB = 8;
 
 // All stages of a line in LDS (radices up to RMAX; the caller guarantees
 // the plan has no larger one).  Composite radices are compiled only into
