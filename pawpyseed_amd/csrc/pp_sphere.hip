@@ -329,14 +329,37 @@ __global__ __launch_bounds__(256) void k_sphere_sum(const cplx<T> *__restrict__ 
   for (int j = 0; j < PMAX; j++)
     acc[j] = mk<T>(0, 0);
   const T *tsite = table + tab_off[s];
+  // a tile's table values, indices and phases: every load of a thread is
+  // issued before any is written to LDS, so they are in flight together
+  constexpr int SL = (TP * PMAX + 255) / 256;
+  T tv[SL];
+  int iv = 0;
+  cplx<T> pv = mk<T>(0, 0);
+  auto fetch = [&](int f0) {
+    const int fn = min(TP, e - f0);
+    const T *tsrc = tsite + (long)(f0 - p0) * np;
+#pragma unroll
+    for (int u = 0; u < SL; u++) {
+      const int q = tid + u * 256;
+      tv[u] = q < fn * np ? tsrc[q] : (T)0;
+    }
+    if (tid < fn) {
+      iv = pidx[f0 + tid];
+      pv = phase[f0 + tid];
+    }
+  };
   for (int t0 = a; t0 < e; t0 += TP) {
     const int n = min(TP, e - t0);
-    const T *tsrc = tsite + (long)(t0 - p0) * np;
-    for (int q = tid; q < n * np; q += 256)
-      s_tab[q] = tsrc[q];
+    fetch(t0);
+#pragma unroll
+    for (int u = 0; u < SL; u++) {
+      const int q = tid + u * 256;
+      if (q < n * np)
+        s_tab[q] = tv[u];
+    }
     if (tid < n) {
-      s_idx[tid] = pidx[t0 + tid];
-      s_ph[tid] = phase[t0 + tid];
+      s_idx[tid] = iv;
+      s_ph[tid] = pv;
     }
     __syncthreads();
     int i = wave;
