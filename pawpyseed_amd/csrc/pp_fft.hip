@@ -326,7 +326,7 @@ __global__ __launch_bounds__(256) void k_fft_pass(cplx<T> *__restrict__ g, AxisP
 // streaming: 1 = the line is written back with non-temporal stores, 2 =
 // also read with non-temporal loads (each pass touches its data once).
 template <typename T, int NBW, int AXIS, int MAXB, int NT, bool COMP = false>
-__global__ __launch_bounds__(NT) void k_fft_pass_ip(cplx<T> *__restrict__ g, AxisPlan pl,
+__global__ __launch_bounds__(NT, (FftWaves<T, MAXB, COMP>::value)) void k_fft_pass_ip(cplx<T> *__restrict__ g, AxisPlan pl,
                                                      const cplx<T> *__restrict__ tw, long es, long s1, long s2,
                                                      int n2, int nslices, SphereBounds bd, int xcd_per, int nblk,
                                                      int streaming) {

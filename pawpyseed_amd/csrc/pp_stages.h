@@ -72,6 +72,14 @@ __device__ __forceinline__ void stage_twiddles(cplx<T> *__restrict__ dst, const 
 
 // loads per thread issued together when a pass stages its line in LDS
 constexpr int FFT_LB = 8;
+// Minimum wavefronts per SIMD the in-place pass kernels are compiled for:
+// 8 (at most 64 registers) for the FP32 radix-2..5 kernels with up to four
+// butterflies per thread, where the extra resident work-groups keep more
+// line loads in flight (x pass 8.4 -> 7.8 ms per step at 108^3, a few
+// registers spilled); no bound for the others, which would spill heavily.
+template <typename T, int MAXB, bool COMP> struct FftWaves {
+  static constexpr int value = (!COMP && ((sizeof(T) == 4 && MAXB <= 4) || (sizeof(T) == 8 && MAXB <= 2))) ? 8 : 1;
+};
 
 // All stages of a line in LDS (radices up to RMAX; the caller guarantees
 // the plan has no larger one).  Composite radices are compiled only into
