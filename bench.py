@@ -77,6 +77,17 @@ def cpu_cores():
     return max(1, n)
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), for the CPU baseline."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_216atom.md")
 RS_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "realspace_aedensity_216atom.md")
 
@@ -598,7 +609,7 @@ def bench_aedensity(args, torch, B):
         t0 = time.perf_counter()
         want = orc.realspace_state(wl.lattice, fine, wl.kpts[k], Gs[k], C0, W, labels, wl.frac_S, pr)
         tc = time.perf_counter() - t0
-        cpu = {"value": 1.0 / tc, "unit": "band-states/s", "cores": 1, "kind": "port",
+        cpu = {"value": 1.0 / tc, "unit": "band-states/s", "cores": 1, "kind": "port", "cpu_model": cpu_model(),
                "sample": f"oracle realspace_state of one state (band {b}, k {k}) on {fine}: {tc:.1f} s"}
         for p in res:
             one = torch.zeros((1,) + fine, dtype=torch.complex128, device="cuda")
@@ -651,17 +662,22 @@ def cpu_and_errors(args, torch, wl, Gs, CR, CS, projR, projS, Om, ob, plans, res
     done = set(pairs)
     more = [p for p in ((int(rng.integers(nk)), int(rng.integers(nb))) for _ in range(4 * extra + 8))
             if p not in done][:extra]
-    t_more = 0.0
-    if more:
+    # the sample in three timed batches; the median per-pair time of the
+    # batches (the first, calibration batch also warms up the thread pool)
+    t_more, per_pair_batches = 0.0, []
+    for batch in (np.array_split(np.array(more, np.int32), 3) if len(more) >= 3 else []):
         t0 = time.perf_counter()
-        orc.setup_projections(wl.lattice, wl.fftg, wl.kpts, Gs, CSh, TS, pairs=np.array(more, np.int32),
-                              nthreads=cores)
-        t_more = time.perf_counter() - t0
+        orc.setup_projections(wl.lattice, wl.fftg, wl.kpts, Gs, CSh, TS, pairs=batch, nthreads=cores)
+        dt = time.perf_counter() - t0
+        t_more += dt
+        per_pair_batches.append(dt / len(batch))
     npairs = len(pairs) + len(more)
     t_pairs = t_first + t_more
-    t_cpu_all = t_tab + t_pairs / npairs * nk * nb
+    per_pair_med = float(np.median(per_pair_batches)) if per_pair_batches else t_first / len(pairs)
+    t_cpu_all = t_tab + per_pair_med * nk * nb
     cpu_value = nb * nk * TS.ptot / t_cpu_all
-    log(f"cpu: tables {t_tab:.2f}s, {npairs} pairs in {t_pairs:.2f}s on {cores} threads -> "
+    log(f"cpu: tables {t_tab:.2f}s, {npairs} pairs in {t_pairs:.2f}s on {cores} threads "
+        f"(median of {len(per_pair_batches)} batches: {per_pair_med * 1e3:.2f} ms per pair) -> "
         f"{cpu_value:.4g} band-projections/s")
 
     # CPU overlap: the reference's FP32-accumulating cdotc, one projected band
@@ -714,8 +730,10 @@ def cpu_and_errors(args, torch, wl, Gs, CR, CS, projR, projS, Om, ob, plans, res
     return {
         "cpu_baseline": {"value": cpu_value, "unit": "band-projections/s", "cores": cores, "kind": "port",
                          "sample": f"oracle on the host: S sphere tables (215 sites, {t_tab:.1f}s) + scatter/FFT/"
-                                   f"sphere sums of {npairs} of {nk * nb} (band,k) pairs in {t_pairs:.1f}s, "
+                                   f"sphere sums of {npairs} of {nk * nb} (band,k) pairs in {t_pairs:.1f}s "
+                                   f"(median per-pair time of {len(per_pair_batches)} timed batches), "
                                    f"extrapolated to all pairs",
+                         "cpu_model": cpu_model(),
                          "overlap_tflops": cpu_ov_tflops,
                          "overlap_sample": f"FP32-accumulated dots of 1 projected band vs {nb}x{nk} basis bands"},
         "max_err_vs_fp64_oracle": worst,
