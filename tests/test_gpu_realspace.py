@@ -247,3 +247,60 @@ def test_volumetric_writers(gpu, tmp_path):
         # z slowest, x fastest (density.c:461-477)
         np.testing.assert_allclose(vals, np.asarray(arr).transpose(2, 1, 0).ravel(), rtol=1e-5, atol=1e-12)
     wf.free()
+
+
+@pytest.mark.parametrize("kernel,unroll", [(0, 1), (0, 4), (1, 4), (2, 1), (2, 2)])
+def test_augmentation_variants_agree(ctx, kernel, unroll):
+    """Every selectable augmentation kernel (rs_kernel 0: coefficients in
+    LDS, 1: in registers, 2: tiled; rs_unroll points in flight) gives the
+    oracle's states, FP64 and FP32 chunks, on the overlapping cubic case."""
+    from pawpyseed_amd.core import batched as B
+    rng = np.random.default_rng(17)
+    nband = 6
+    lat, fftg, k, G, C, species, labels, coords = _case("cubic", nband, rng)
+    W = orc.WaveData(species)
+    plan = B.RealspacePlan(ctx, lat, fftg, coords, labels, species)
+    proj = rng.normal(size=(nband, plan.ptot)) + 1j * rng.normal(size=(nband, plan.ptot))
+    ldc = -(-len(G) // 64) * 64
+    Ch = np.zeros((nband, ldc), np.complex64)
+    Ch[:, :len(G)] = C
+    Cd, Gd = torch.from_numpy(Ch).cuda(), torch.from_numpy(np.ascontiguousarray(G, np.int32)).cuda()
+    Pd = torch.from_numpy(proj).cuda()
+    before = (ctx.get_param("rs_kernel"), ctx.get_param("rs_unroll"))
+    try:
+        ctx.set_param("rs_kernel", kernel)
+        ctx.set_param("rs_unroll", unroll)
+        for precision, tol in (("fp64", 1e-10), ("fp32", 1e-5)):
+            out = torch.zeros((nband,) + tuple(fftg), dtype=torch.complex128, device="cuda")
+            B.realspace_states(ctx, plan, k, Gd, Cd, Pd, states_out=out, precision=precision)
+            got = out.cpu().numpy()
+            for b in range(nband):
+                want = orc.realspace_state(lat, fftg, k, G, C[b], W, labels, coords, proj[b])
+                assert normwise(got[b], want) < tol, (precision, b)
+    finally:
+        ctx.set_param("rs_kernel", before[0])
+        ctx.set_param("rs_unroll", before[1])
+    plan.close()
+
+
+def test_wrapping_boxes_fp32_atomics(ctx):
+    """Atomic adds of the FP32 chunk where site boxes wrap onto themselves."""
+    from pawpyseed_amd.core import batched as B
+    rng = np.random.default_rng(19)
+    nband = 3
+    lat, fftg, k, G, C, species, labels, coords = _case("wrapping", nband, rng)
+    W = orc.WaveData(species)
+    plan = B.RealspacePlan(ctx, lat, fftg, coords, labels, species)
+    assert plan.atomic
+    proj = rng.normal(size=(nband, plan.ptot)) + 1j * rng.normal(size=(nband, plan.ptot))
+    ldc = -(-len(G) // 64) * 64
+    Ch = np.zeros((nband, ldc), np.complex64)
+    Ch[:, :len(G)] = C
+    out = torch.zeros((nband,) + tuple(fftg), dtype=torch.complex128, device="cuda")
+    B.realspace_states(ctx, plan, k, torch.from_numpy(np.ascontiguousarray(G, np.int32)).cuda(),
+                       torch.from_numpy(Ch).cuda(), torch.from_numpy(proj).cuda(), states_out=out, precision="fp32")
+    got = out.cpu().numpy()
+    for b in range(nband):
+        want = orc.realspace_state(lat, fftg, k, G, C[b], W, labels, coords, proj[b])
+        assert normwise(got[b], want) < 1e-5
+    plan.close()
