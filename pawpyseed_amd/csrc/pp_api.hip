@@ -412,6 +412,12 @@ void pp_ctx_destroy(pp_ctx *ctx) {
     (void)hipEventDestroy(ctx->fork);
   if (ctx->join)
     (void)hipEventDestroy(ctx->join);
+  for (int i = 0; i < 2; i++) {
+    if (ctx->gemm_done[i])
+      (void)hipEventDestroy(ctx->gemm_done[i]);
+    if (ctx->red_done[i])
+      (void)hipEventDestroy(ctx->red_done[i]);
+  }
   if (ctx->own)
     (void)hipStreamDestroy(ctx->own);
   delete ctx;

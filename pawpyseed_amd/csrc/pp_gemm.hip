@@ -176,14 +176,14 @@ __global__ __launch_bounds__(256) void k_splitk_reduce(int M, int N, int nsplit,
 }
 
 int gemm_fp32_mfma(pp_ctx *ctx, int precision, int M, int N, int K, const void *A, long lda, const void *B, long ldb,
-                   cplx<double> *C, long ldc, bool accumulate);
+                   cplx<double> *C, long ldc, bool accumulate, int slot);
 
 static int zgemm_one(pp_ctx *ctx, int precision, int M, int N, int K, int atype, const void *A, long lda, int btype,
-                     const void *B, long ldb, cplx<double> *C, long ldc, bool accumulate, int cls) {
+                     const void *B, long ldb, cplx<double> *C, long ldc, bool accumulate, int cls, int slot) {
   if (M <= 0 || N <= 0)
     return PP_OK;
   if (precision != PP_FP64 && atype == 1 && btype == 1)
-    return gemm_fp32_mfma(ctx, precision, M, N, K, A, lda, B, ldb, C, ldc, accumulate);
+    return gemm_fp32_mfma(ctx, precision, M, N, K, A, lda, B, ldb, C, ldc, accumulate, slot);
   int tiles = ((M + 63) / 64) * ((N + 63) / 64);
   int nsplit = ctx->tune_splitk;
   if (nsplit <= 0) {
@@ -251,16 +251,39 @@ int zgemm_nt(pp_ctx *ctx, int precision, int M, int N, int K, int atype, const v
              const void *B, long ldb, long sB, void *C, long ldc, long sC, int nbatch, const int *amap,
              const int *bmap, const int *klen, bool accumulate) {
   size_t ea = atype == 0 ? 16 : 8, eb = btype == 0 ? 16 : 8;
-  for (int b = 0; b < nbatch; b++) {
+  // FP32 / 3xBF16 batches: the split-K reduction of one batch runs on the
+  // second stream while the next batch's GEMM runs (two partials buffers)
+  const bool pipe = precision != PP_FP64 && atype == 1 && btype == 1 && nbatch > 1 && ctx->tune_streams > 1 &&
+                    M > 0 && N > 0;
+  if (pipe) {
+    if (!ctx->aux)
+      PP_CHECK_HIP(hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking));
+    for (int i = 0; i < 2; i++) {
+      if (!ctx->gemm_done[i])
+        PP_CHECK_HIP(hipEventCreateWithFlags(&ctx->gemm_done[i], hipEventDisableTiming));
+      if (!ctx->red_done[i])
+        PP_CHECK_HIP(hipEventCreateWithFlags(&ctx->red_done[i], hipEventDisableTiming));
+      PP_CHECK_HIP(hipEventRecord(ctx->red_done[i], ctx->stream));  // no reduction pending yet
+    }
+    if (!ctx->join)
+      PP_CHECK_HIP(hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming));
+  }
+  int st = PP_OK;
+  for (int b = 0; b < nbatch && st == PP_OK; b++) {
     int ia = amap ? amap[b] : b, ib = bmap ? bmap[b] : b;
     int kk = klen ? klen[b] : K;
     const char *pa = (const char *)A + (size_t)ia * sA * ea;
     const char *pb = (const char *)B + (size_t)ib * sB * eb;
     cplx<double> *pc = (cplx<double> *)C + (size_t)b * sC;
-    PP_TRY(zgemm_one(ctx, precision, M, N, kk, atype, pa, lda, btype, pb, ldb, pc, ldc, accumulate,
-                     atype == 0 ? K_AUG_GEMM : K_GEMM));
+    st = zgemm_one(ctx, precision, M, N, kk, atype, pa, lda, btype, pb, ldb, pc, ldc, accumulate,
+                   atype == 0 ? K_AUG_GEMM : K_GEMM, pipe ? (b & 1) : -1);
   }
-  return PP_OK;
+  if (pipe) {
+    // the caller's stream sees every reduction
+    (void)hipEventRecord(ctx->join, ctx->aux);
+    (void)hipStreamWaitEvent(ctx->stream, ctx->join, 0);
+  }
+  return st;
 }
 
 }  // namespace pp

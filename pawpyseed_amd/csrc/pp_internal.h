@@ -117,6 +117,9 @@ struct pp_ctx {
   pp::DevBuf gidx;      // wrapped grid index per G
   pp::DevBuf phase;     // per-point Bloch phase of every k
   pp::DevBuf gemm_ws;   // split-K partials
+  pp::DevBuf gemm_ws2;  // second partials buffer (reduction of one batch overlaps the next GEMM)
+  hipEvent_t gemm_done[2] = {nullptr, nullptr};  // GEMM of a partials slot finished
+  hipEvent_t red_done[2] = {nullptr, nullptr};   // reduction of a partials slot finished
   pp::DevBuf aug_ws;    // gathered / dO-applied projections
   pp::DevBuf bounds;    // plane-wave sphere bounds per k
   pp::DevBuf lines;     // populated z / y pass lines per k

@@ -268,8 +268,13 @@ __global__ __launch_bounds__(256) void k_splitk_reduce2(int M, int N, int nsplit
   C[(long)m * ldc + n] = s;
 }
 
+// slot < 0: partials in gemm_ws, reduction on the context's stream.
+// slot 0 / 1: partials in gemm_ws / gemm_ws2 and the reduction on the
+// second stream (after the GEMM's event), so that it overlaps the next
+// batch's GEMM; the GEMM first waits until the slot's previous reduction
+// has read its partials.  The caller joins the second stream at the end.
 int gemm_fp32_mfma(pp_ctx *ctx, int precision, int M, int N, int K, const void *A, long lda, const void *B, long ldb,
-                   cplx<double> *C, long ldc, bool accumulate) {
+                   cplx<double> *C, long ldc, bool accumulate, int slot) {
   const int kstep = precision == PP_BF16X3 ? 32 : 16;
   int tiles = ((M + 63) / 64) * ((N + 63) / 64);
   // FP32 accumulation runs over one split-K chunk of at most 512 complex
@@ -288,8 +293,11 @@ int gemm_fp32_mfma(pp_ctx *ctx, int precision, int M, int N, int K, const void *
   if (kchunk == 0)
     kchunk = kstep;
   nsplit = std::max(1, (K + kchunk - 1) / kchunk);
-  PP_TRY(ctx->gemm_ws.ensure(sizeof(cplx<double>) * (size_t)nsplit * M * N));
-  cplx<double> *P = (cplx<double> *)ctx->gemm_ws.p;
+  DevBuf &ws = slot == 1 ? ctx->gemm_ws2 : ctx->gemm_ws;
+  PP_TRY(ws.ensure(sizeof(cplx<double>) * (size_t)nsplit * M * N));
+  cplx<double> *P = (cplx<double> *)ws.p;
+  if (slot >= 0)
+    PP_CHECK_HIP(hipStreamWaitEvent(ctx->stream, ctx->red_done[slot], 0));
   bool vec = (lda % 2 == 0) && (ldb % 2 == 0) && ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0);
   dim3 grd(tiles, nsplit);
   hipEvent_t ev;
@@ -319,12 +327,21 @@ int gemm_fp32_mfma(pp_ctx *ctx, int precision, int M, int N, int K, const void *
   }
   tend(ctx, K_GEMM, ev);
   PP_CHECK_HIP(hipGetLastError());
+  hipStream_t main = ctx->stream;
+  if (slot >= 0) {
+    PP_CHECK_HIP(hipEventRecord(ctx->gemm_done[slot], main));
+    PP_CHECK_HIP(hipStreamWaitEvent(ctx->aux, ctx->gemm_done[slot], 0));
+    ctx->stream = ctx->aux;
+  }
   long tot = (long)M * N;
   tbegin(ctx, K_GEMM_REDUCE, &ev);
   hipLaunchKernelGGL(k_splitk_reduce2, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream, M, N, nsplit,
                      (const cplx<double> *)P, C, ldc, accumulate ? 1 : 0);
   tend(ctx, K_GEMM_REDUCE, ev);
+  ctx->stream = main;
   PP_CHECK_HIP(hipGetLastError());
+  if (slot >= 0)
+    PP_CHECK_HIP(hipEventRecord(ctx->red_done[slot], ctx->aux));
   return PP_OK;
 }
 
