@@ -105,16 +105,24 @@ __global__ __launch_bounds__(256) void k_cgemm_f32(int M, int N, int kchunk, int
         dr[a][b][v] = di[a][b][v] = 0.0;
   const int lrow = threadIdx.x >> 2, part = threadIdx.x & 3;
   int since_flush = 0;
+  // the next k-step's A and B values are loaded into registers while the
+  // current step's MFMAs run
+  float va[8], vb[8];
+  if (k0 < k1) {
+    load8<VEC>(A, lda, tm * 64 + lrow, M, k0 + part * 4, k1, va);
+    load8<VEC>(B, ldb, tn * 64 + lrow, N, k0 + part * 4, k1, vb);
+  }
   for (int kb = k0; kb < k1; kb += 16) {
-    float va[8], vb[8];
-    load8<VEC>(A, lda, tm * 64 + lrow, M, kb + part * 4, k1, va);
-    load8<VEC>(B, ldb, tn * 64 + lrow, N, kb + part * 4, k1, vb);
 #pragma unroll
     for (int c = 0; c < 8; c++) {
       As[lrow][part * 8 + c] = va[c];
       Bs[lrow][part * 8 + c] = vb[c];
     }
     __syncthreads();
+    if (kb + 16 < k1) {
+      load8<VEC>(A, lda, tm * 64 + lrow, M, kb + 16 + part * 4, k1, va);
+      load8<VEC>(B, ldb, tn * 64 + lrow, N, kb + 16 + part * 4, k1, vb);
+    }
     const int sub = lane & 15, kq = lane >> 4;
 #pragma unroll
     for (int kk = 0; kk < 32; kk += 4) {
