@@ -112,23 +112,32 @@ __global__ __launch_bounds__(256) void k_zgemm_mfma64(int M, int N, int kchunk, 
     }
   const int lrow = threadIdx.x >> 2, part = threadIdx.x & 3;
   const int gm = tm * 64 + lrow, gn = tn * 64 + lrow;
-  for (int kb = k0; kb < k1; kb += 16) {
+  // a k-step's 4 + 4 values of a thread are loaded together, the next
+  // step's while the current step's MFMAs run
+  cplx<TA> pa[4];
+  cplx<TB> pb[4];
+  auto fetch = [&](int kb) {
 #pragma unroll
     for (int c = 0; c < 4; c++) {
       const int gk = kb + part * 4 + c;
-      cplx<double> va = mk<double>(0, 0), vb = mk<double>(0, 0);
-      if (gk < k1) {
-        if (gm < M)
-          va = cvt<double>(A[(long)gm * lda + gk]);
-        if (gn < N)
-          vb = cvt<double>(B[(long)gn * ldb + gk]);
-      }
+      pa[c] = (gk < k1 && gm < M) ? A[(long)gm * lda + gk] : mk<TA>((TA)0, (TA)0);
+      pb[c] = (gk < k1 && gn < N) ? B[(long)gn * ldb + gk] : mk<TB>((TB)0, (TB)0);
+    }
+  };
+  if (k0 < k1)
+    fetch(k0);
+  for (int kb = k0; kb < k1; kb += 16) {
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const cplx<double> va = cvt<double>(pa[c]), vb = cvt<double>(pb[c]);
       As[lrow][part * 8 + 2 * c] = va.x;
       As[lrow][part * 8 + 2 * c + 1] = va.y;
       Bs[lrow][part * 8 + 2 * c] = vb.x;
       Bs[lrow][part * 8 + 2 * c + 1] = vb.y;
     }
     __syncthreads();
+    if (kb + 16 < k1)
+      fetch(kb + 16);
     const int sub = lane & 15, kq = lane >> 4;
 #pragma unroll
     for (int kk = 0; kk < 32; kk += 4) {
