@@ -204,13 +204,23 @@ __global__ __launch_bounds__(256) void k_cgemm_bf16x3(int M, int N, int kchunk, 
       ci[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
     }
   const int lrow = threadIdx.x >> 2, part = threadIdx.x & 3;
+  // a k-step's values of a thread are loaded together, the next step's
+  // while the current step's MFMAs run
+  float pa[2][8], pb[2][8];
+  auto fetch = [&](int kb) {
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int gk = kb + part * 8 + h * 4;
+      load8<VEC>(A, lda, tm * 64 + lrow, M, gk, k1, pa[h]);
+      load8<VEC>(B, ldb, tn * 64 + lrow, N, gk, k1, pb[h]);
+    }
+  };
+  if (k0 < k1)
+    fetch(k0);
   for (int kb = k0; kb < k1; kb += 32) {
 #pragma unroll
     for (int h = 0; h < 2; h++) {
-      float va[8], vb[8];
-      const int gk = kb + part * 8 + h * 4;
-      load8<VEC>(A, lda, tm * 64 + lrow, M, gk, k1, va);
-      load8<VEC>(B, ldb, tn * 64 + lrow, N, gk, k1, vb);
+      const float *va = pa[h], *vb = pb[h];
       const int o = part * 16 + h * 8;
 #pragma unroll
       for (int c = 0; c < 8; c++) {
@@ -228,6 +238,8 @@ __global__ __launch_bounds__(256) void k_cgemm_bf16x3(int M, int N, int kchunk, 
       }
     }
     __syncthreads();
+    if (kb + 32 < k1)
+      fetch(kb + 32);
     const int sub = lane & 15, kq = lane >> 4;
 #pragma unroll
     for (int kk = 0; kk < 64; kk += 32) {
