@@ -662,10 +662,10 @@ def cpu_and_errors(args, torch, wl, Gs, CR, CS, projR, projS, Om, ob, plans, res
     done = set(pairs)
     more = [p for p in ((int(rng.integers(nk)), int(rng.integers(nb))) for _ in range(4 * extra + 8))
             if p not in done][:extra]
-    # the sample in three timed batches; the median per-pair time of the
+    # the sample in five timed batches; the median per-pair time of the
     # batches (the first, calibration batch also warms up the thread pool)
     t_more, per_pair_batches = 0.0, []
-    for batch in (np.array_split(np.array(more, np.int32), 3) if len(more) >= 3 else []):
+    for batch in (np.array_split(np.array(more, np.int32), 5) if len(more) >= 5 else []):
         t0 = time.perf_counter()
         orc.setup_projections(wl.lattice, wl.fftg, wl.kpts, Gs, CSh, TS, pairs=batch, nthreads=cores)
         dt = time.perf_counter() - t0
