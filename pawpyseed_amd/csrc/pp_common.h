@@ -75,6 +75,26 @@ template <typename T> __device__ __forceinline__ void store_nt(cplx<T> *p, cplx<
   __builtin_nontemporal_store(w, reinterpret_cast<V *>(p));
 }
 
+// Two adjacent complex floats as one 16-byte access (p 16-byte aligned).
+__device__ __forceinline__ void load2(const cplx<float> *p, bool nt, cplx<float> &a, cplx<float> &b) {
+  using V = float __attribute__((ext_vector_type(4)));
+  const V v = nt ? __builtin_nontemporal_load(reinterpret_cast<const V *>(p)) : *reinterpret_cast<const V *>(p);
+  a = mk<float>(v.x, v.y);
+  b = mk<float>(v.z, v.w);
+}
+__device__ __forceinline__ void store2(cplx<float> *p, bool nt, cplx<float> a, cplx<float> b) {
+  using V = float __attribute__((ext_vector_type(4)));
+  V w;
+  w.x = a.x;
+  w.y = a.y;
+  w.z = b.x;
+  w.w = b.y;
+  if (nt)
+    __builtin_nontemporal_store(w, reinterpret_cast<V *>(p));
+  else
+    *reinterpret_cast<V *>(p) = w;
+}
+
 // ----------------------------------------------------------------- errors
 namespace pp {
 void set_error(int code, const std::string &msg);

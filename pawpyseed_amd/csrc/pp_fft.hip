@@ -364,7 +364,31 @@ __global__ __launch_bounds__(NT, (FftWaves<T, MAXB, COMP>::value)) void k_fft_pa
   stage_twiddles<T, NT>(twl, tw, pl.twlen);
   // the line's loads are issued in batches of FFT_LB per thread before any
   // is written to LDS, so that many are in flight at once (a load-then-store
-  // loop waits for each load in turn)
+  // loop waits for each load in turn); FP32 lines move two bands (16 bytes)
+  // per access
+  if constexpr (sizeof(T) == 4 && NBW % 2 == 0) {
+    const int tot2 = tot / 2;
+    for (int t0 = threadIdx.x; t0 < tot2; t0 += NT * FFT_LB) {
+      cplx<T> va[FFT_LB], vb[FFT_LB];
+#pragma unroll
+      for (int u = 0; u < FFT_LB; u++) {
+        const int t = t0 + u * NT;
+        va[u] = vb[u] = mk<T>(0, 0);
+        if (t < tot2) {
+          const int e = 2 * t / NBW, b = 2 * t % NBW;
+          const int se = sgn_idx(e, n);
+          if (se >= lo && se <= hi)
+            load2(reinterpret_cast<const cplx<float> *>(g + base + (long)e * es + b), streaming > 1, va[u], vb[u]);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < FFT_LB; u++)
+        if (t0 + u * NT < tot2) {
+          buf[2 * (t0 + u * NT)] = va[u];
+          buf[2 * (t0 + u * NT) + 1] = vb[u];
+        }
+    }
+  } else
   for (int t0 = threadIdx.x; t0 < tot; t0 += NT * FFT_LB) {
     cplx<T> v[FFT_LB];
 #pragma unroll
@@ -386,6 +410,29 @@ __global__ __launch_bounds__(NT, (FftWaves<T, MAXB, COMP>::value)) void k_fft_pa
   __syncthreads();
   stages_ip<T, NBW, MAXB, NT, COMP>(buf, twl, pl, n);
   const unsigned *xm = (AXIS == 0 && bd.xm) ? bd.xm + (long)line * bd.xmw : nullptr;
+  if constexpr (sizeof(T) == 4 && NBW % 2 == 0) {
+    const int tot2 = tot / 2;
+    for (int t0 = threadIdx.x; t0 < tot2; t0 += NT * FFT_LB) {
+      cplx<T> va[FFT_LB], vb[FFT_LB];
+#pragma unroll
+      for (int u = 0; u < FFT_LB; u++)
+        if (t0 + u * NT < tot2) {
+          va[u] = buf[2 * (t0 + u * NT)];
+          vb[u] = buf[2 * (t0 + u * NT) + 1];
+        }
+#pragma unroll
+      for (int u = 0; u < FFT_LB; u++) {
+        const int t = t0 + u * NT;
+        if (t >= tot2)
+          break;
+        const int e = 2 * t / NBW, b = 2 * t % NBW;
+        if (xm && !((xm[e >> 5] >> (e & 31)) & 1u))
+          continue;  // no projector sphere reads this point
+        store2(reinterpret_cast<cplx<float> *>(g + base + (long)e * es + b), streaming != 0, va[u], vb[u]);
+      }
+    }
+    return;
+  }
   for (int t0 = threadIdx.x; t0 < tot; t0 += NT * FFT_LB) {
     cplx<T> v[FFT_LB];
 #pragma unroll
