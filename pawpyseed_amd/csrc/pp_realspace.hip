@@ -416,7 +416,7 @@ __global__ __launch_bounds__(256) void k_rs_state(const cplx<T> *__restrict__ gr
   if (threadIdx.x < RS_TP) {
     cplx<double> e = mk<double>(scl, 0.0);
     const long pt = pt0 + threadIdx.x;
-    if (phase_on && threadIdx.x < npt) {
+    if (phase_on && (int)threadIdx.x < npt) {
       const int kk = (int)(pt % f2), jj = (int)((pt / f2) % f1), ii = (int)(pt / ((long)f1 * f2));
       const double a = k0 * ii / f0 + k1 * jj / f1 + k2 * kk / f2;
       double sn, cs;
@@ -1019,7 +1019,7 @@ __global__ __launch_bounds__(256) void k_rs_dot(const cplx<double> *__restrict__
   si[threadIdx.x] = ai;
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o) {
+    if ((int)threadIdx.x < o) {
       sr[threadIdx.x] += sr[threadIdx.x + o];
       si[threadIdx.x] += si[threadIdx.x + o];
     }

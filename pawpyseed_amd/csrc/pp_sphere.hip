@@ -610,7 +610,6 @@ static int sphere_build(pp_sphere *sp) {
   const int ns = sp->nsites;
   // per-site geometry
   std::vector<SiteGeo> geo(ns);
-  double vol = det3(sp->lattice);
   std::vector<long> spl_off(sp->nelems);
   long so = 0;
   for (int e = 0; e < sp->nelems; e++) {
