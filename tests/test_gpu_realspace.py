@@ -304,3 +304,43 @@ def test_wrapping_boxes_fp32_atomics(ctx):
         want = orc.realspace_state(lat, fftg, k, G, C[b], W, labels, coords, proj[b])
         assert normwise(got[b], want) < 1e-5
     plan.close()
+
+
+def test_full_size_216_atoms_doubled_grid(ctx):
+    """The all-electron density workload's size: 216-atom cell (215 S
+    sites, 1.6 A spheres) on the 216^3 grid.  Parseval for the pseudo part
+    (projections 0): sum_r |psi(r)|^2 V/N^3 = sum_G |C_G|^2 to 1e-12, and
+    one full state (band 0, k 1) against the oracle to 1e-10."""
+    from pawpyseed_amd.core import batched as B
+    wl = synthetic.config1(nband=2)
+    Gs = wl.ensure_gvecs()
+    k = 1
+    G = np.ascontiguousarray(Gs[k], np.int32)
+    C = wl.coefficients("S", nband=2)[k]
+    fine = tuple(int(2 * n) for n in wl.fftg)
+    labels = np.zeros(len(wl.frac_S), np.int32)
+    plan = B.RealspacePlan(ctx, wl.lattice, fine, wl.frac_S, labels, [wl.species])
+    print("full-size realspace: grid", fine, "points", plan.npts, "colours", plan.ncolors, flush=True)
+    ldc = -(-len(G) // 64) * 64
+    Ch = np.zeros((2, ldc), np.complex64)
+    Ch[:, :len(G)] = C
+    Cd, Gd = torch.from_numpy(Ch).cuda(), torch.from_numpy(G).cuda()
+    rng = np.random.default_rng(23)
+    proj = 0.05 * (rng.normal(size=(2, plan.ptot)) + 1j * rng.normal(size=(2, plan.ptot)))
+    zero = torch.zeros((2, plan.ptot), dtype=torch.complex128, device="cuda")
+    out = torch.zeros((2,) + fine, dtype=torch.complex128, device="cuda")
+    B.realspace_states(ctx, plan, wl.kpts[k], Gd, Cd, zero, states_out=out, precision="fp64")
+    vol = abs(np.linalg.det(wl.lattice))
+    for b in range(2):
+        norm = float((out[b].abs() ** 2).sum().item()) * vol / np.prod(fine)
+        want = float(np.sum(np.abs(C[b].astype(np.complex128)) ** 2))
+        assert abs(norm - want) < 1e-12 * want, (b, norm, want)
+    B.realspace_states(ctx, plan, wl.kpts[k], Gd, Cd, torch.from_numpy(proj).cuda(), states_out=out,
+                       precision="fp64")
+    got = out[0].cpu().numpy()
+    W = orc.WaveData([wl.species])
+    want = orc.realspace_state(wl.lattice, fine, wl.kpts[k], G, C[0], W, labels, wl.frac_S, proj[0])
+    err = normwise(got, want)
+    print("full-size realspace state error", err, flush=True)
+    assert err < 1e-10
+    plan.close()
