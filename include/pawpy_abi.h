@@ -482,7 +482,11 @@ int pp_rs_plan_info(pp_rsplan *plan, int *total_projs, long *total_points,
  * partial-wave differences weighted by proj_dev (c128 [nband][ldp], the
  * plan's projector order).  Outputs (either may be NULL): states_dev c128
  * [nband][ldo] (x slowest) and density_dev f64 [N^3] += sum_b weights[b]
- * |psi_b|^2 (weights: host [nband]).  precision: PP_FP64 or PP_FP32 FFT. */
+ * |psi_b|^2 (weights: host [nband]).  precision: PP_FP64 or PP_FP32 FFT.
+ * Asynchronous on the context's stream after one host synchronisation
+ * (the plane-wave bounds); with the "streams" parameter at 2 two 64-band
+ * chunks are in flight on two streams, joined into the context's stream
+ * before the call returns. */
 int pp_realspace_batched(pp_ctx *ctx, pp_rsplan *plan, const double *kfrac,
                          int nw, const int *G_dev, const void *C_dev, long ldc,
                          int nband, const void *proj_dev, long ldp,
