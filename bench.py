@@ -123,7 +123,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nband", type=int, default=512)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline work")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="target CPU-baseline work (seconds)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle checks")
     ap.add_argument("--paths", default="fp32,fp64", help="comma list of fp32, fp64 setup/overlap paths")
     ap.add_argument("--workload", default="216atom", choices=["216atom", "spinor512", "aedensity216"],
