@@ -78,7 +78,7 @@ constexpr int FFT_LB = 8;
 // line loads in flight (x pass 8.4 -> 7.8 ms per step at 108^3, a few
 // registers spilled); no bound for the others, which would spill heavily.
 template <typename T, int MAXB, bool COMP> struct FftWaves {
-  static constexpr int value = (!COMP && ((sizeof(T) == 4 && MAXB <= 4) || (sizeof(T) == 8 && MAXB <= 2))) ? 8 : 1;
+  static constexpr int value = (!COMP && sizeof(T) == 4 && MAXB <= 4) ? 8 : 1;
 };
 
 // All stages of a line in LDS (radices up to RMAX; the caller guarantees
