@@ -75,8 +75,9 @@ constexpr int FFT_LB = 8;
 // Minimum wavefronts per SIMD the in-place pass kernels are compiled for:
 // 8 (at most 64 registers) for the FP32 radix-2..5 kernels with up to four
 // butterflies per thread, where the extra resident work-groups keep more
-// line loads in flight (x pass 8.4 -> 7.8 ms per step at 108^3, a few
-// registers spilled); no bound for the others, which would spill heavily.
+// line loads in flight (x pass 8.4 -> 7.8 ms per step at 108^3; the
+// 8-band-tile kernels spill a few registers); no bound for the others,
+// which would spill.
 template <typename T, int MAXB, bool COMP> struct FftWaves {
   static constexpr int value = (!COMP && sizeof(T) == 4 && MAXB <= 4) ? 8 : 1;
 };
