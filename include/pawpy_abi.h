@@ -20,6 +20,8 @@
 #ifndef PAWPY_ABI_H
 #define PAWPY_ABI_H
 
+#include <stddef.h>
+
 #include "pawpy_types.h"
 
 #ifdef __cplusplus
@@ -89,9 +91,10 @@ void set_num_sites(pswf_t *wf, int nsites);
 void free_ptr(void *ptr);
 
 /* replaces projector.c:20-171 get_projector_list: projector splines
- * (utils.c:699-749) and PAW partial-wave overlap matrices
- * (projector.c:507-558).  The SBT-filtered smooth_diffwave used only by the
- * O_R/O_S terms is not built (left NULL). */
+ * (utils.c:699-749), PAW partial-wave overlap matrices
+ * (projector.c:507-558), and the spherical Bessel transforms and
+ * plane-wave-filtered partial-wave differences (kwave, smooth_diffwave,
+ * projector.c:114-163) that the O_R/O_S and O_N terms use. */
 ppot_t *get_projector_list(int num_els, int *labels, int *ls,
                            double *wave_grids, double *projectors,
                            double *aewaves, double *pswaves, double *rmaxs,
@@ -364,6 +367,23 @@ int pp_setup_projections_batched(pp_ctx *ctx, pp_sphere *sp, int nk,
 int pp_overlap_gemm(pp_ctx *ctx, int nwk, int nspin, int flip, const int *nw,
                     const void *CR_dev, long ldR, int nbR, const void *CS_dev,
                     long ldS, int nbS, void *out_dev, int precision);
+
+/* No counterpart in the reference: pp_overlap_gemm without any device
+ * allocation, for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and result; PP_FP64 or PP_FP32 only.  The split-K partial sums
+ * live in a fixed 64 MiB device array of the library (one per GPU, held
+ * until the process ends), so nbR * nbS may not exceed
+ * pp_noalloc_scratch_bytes() / 32 (2,097,152) output entries per k-point
+ * (PP_ERR_ARG otherwise); the result equals pp_overlap_gemm's whenever
+ * its split count fits the array (e.g. 512 x 512 bands).  Runs on the
+ * context's stream only and touches none of the context's buffers.  Calls
+ * that overlap in time on one GPU share the array and race: serialise them. */
+int pp_overlap_gemm_noalloc(pp_ctx *ctx, int nwk, int nspin, int flip, const int *nw,
+                            const void *CR_dev, long ldR, int nbR, const void *CS_dev,
+                            long ldS, int nbS, void *out_dev, int precision);
+/* Size in bytes of the scratch array of pp_overlap_gemm_noalloc. */
+size_t pp_noalloc_scratch_bytes(void);
 
 /* O_M augmentation of every band pair (projector.c:890-910), FP64:
  * out[k][r][s] += sum_{t in M} sum_{i,j: (l,m) equal}

@@ -529,3 +529,55 @@ def realspace_state(lattice, fftg, kfrac, G, C, W: WaveData, labels, coords, pro
     if remove_phase:
         lib().or_remove_phase(_p(x, _d), _p(fg, _i), _p(kf, _d))
     return x.reshape(tuple(fg))
+
+
+def proportion_conduction(res, occs, kws, nband, nwk, nspin, spinpol=False, pseudo=False):
+    """projector.py:384-435 restated loop by loop: the valence (v) and
+    conduction (c) weight of one projected band from its projections
+    res[b * nwk * nspin + s * nwk + k] onto the basis states, whose
+    occupations occs share that ordering (pawpyc.pyx:532-538)."""
+    res = np.asarray(res)
+    occs = np.asarray(occs, np.float64)
+    if spinpol:
+        v, c = [0.0] * nspin, [0.0] * nspin
+        for b in range(nband):
+            for s in range(nspin):
+                for k in range(nwk):
+                    i = (b * nspin + s) * nwk + k
+                    p = abs(res[i]) ** 2
+                    v[s] += p * occs[i] * kws[k]
+                    c[s] += p * (1.0 - occs[i]) * kws[k]
+    else:
+        v = c = 0.0
+        for i in range(nband * nwk * nspin):
+            p = abs(res[i]) ** 2 * kws[i % nwk] / nspin
+            if occs[i] > 0.5:
+                v += p
+            else:
+                c += p
+    if pseudo:
+        if spinpol:
+            t = [a + b for a, b in zip(v, c)]
+            v, c = [a / x for a, x in zip(v, t)], [b / x for b, x in zip(c, t)]
+        else:
+            t = v + c
+            v, c = v / t, c / t
+    return v, c
+
+
+def analysis_bands(occs_wf, nband, num_below_ef, num_above_ef, vbmband=None, band_list=None, analyze_all=False,
+                   nband_basis=None):
+    """projector.py:477-494: the bands defect_band_analysis looks at; occs_wf
+    in get_occs order (band-major over the wavefunction's k-points)."""
+    if analyze_all:
+        return list(range(nband_basis if nband_basis is not None else nband))
+    if band_list:
+        return list(band_list)
+    nk = len(occs_wf) // nband
+    vbm = 0
+    for i in range(nband):
+        if occs_wf[i * nk] > 0.5:
+            vbm = i
+    if vbmband is not None:
+        vbm = vbmband
+    return list(range(max(vbm - num_below_ef, 0), min(vbm + num_above_ef, nband - 1) + 1))

@@ -118,6 +118,9 @@ SIGNATURES = {
                                                 c_void_p, ct.c_long, ct.c_int, c_void_p, ct.c_int]),
     "pp_overlap_gemm": (ct.c_int, [c_void_p, ct.c_int, ct.c_int, ct.c_int, c_int_p, c_void_p, ct.c_long, ct.c_int,
                                    c_void_p, ct.c_long, ct.c_int, c_void_p, ct.c_int]),
+    "pp_overlap_gemm_noalloc": (ct.c_int, [c_void_p, ct.c_int, ct.c_int, ct.c_int, c_int_p, c_void_p, ct.c_long,
+                                           ct.c_int, c_void_p, ct.c_long, ct.c_int, c_void_p, ct.c_int]),
+    "pp_noalloc_scratch_bytes": (ct.c_size_t, []),
     "pp_augmentation": (ct.c_int, [c_void_p, ct.c_int, ct.c_int, ct.c_int, c_void_p, ct.c_int, ct.c_int, c_int_p,
                                    c_int_p, c_void_p, ct.c_int, ct.c_int, c_int_p, c_int_p, ct.c_int, c_int_p,
                                    c_int_p, ct.c_int, c_int_p, c_int_p, c_double_p, c_void_p]),

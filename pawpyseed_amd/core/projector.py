@@ -162,55 +162,57 @@ class Projector:
         return self._single_band_projection(band_num, **kwargs)
 
     def proportion_conduction(self, band_num, spinpol=False):
-        """projector.py:384-435."""
+        """Valence and conduction weight of band ``band_num`` of the
+        projected structure in the basis (projector.py:384-435).
+
+        The projections are viewed as a (band, spin, k) array and weighted
+        by |<basis|wf>|^2 w_k.  Spin-resolved, every basis state counts
+        toward valence with its occupation f and toward conduction with
+        1 - f; otherwise a state is valence when f > 0.5 and the sums are
+        divided by the spin count.  The 'pseudo' method normalises v + c to
+        one.  Returns floats, or per-spin lists when ``spinpol``."""
         basis = self.basis
-        nband, nwk, nspin = basis.nband, basis.nwk, basis.nspin
-        occs = basis._get_occs()
-        res = self.single_band_projection(band_num)
+        shape = (basis.nband, basis.nspin, basis.nwk)
+        amp2 = np.abs(np.asarray(self.single_band_projection(band_num))).reshape(shape) ** 2
+        weighted = amp2 * np.asarray(self.wf.kws, np.float64)[None, None, :]
+        occ = np.asarray(basis._get_occs(), np.float64).reshape(shape)
         if spinpol:
-            c, v = np.zeros(nspin), np.zeros(nspin)
-            for b in range(nband):
-                for s in range(nspin):
-                    ind = b * nspin + s
-                    prop = np.absolute(res[ind * nwk:(ind + 1) * nwk]) ** 2
-                    c[s] += np.dot(prop, (1 - occs[ind * nwk:(ind + 1) * nwk]) * self.wf.kws)
-                    v[s] += np.dot(prop, occs[ind * nwk:(ind + 1) * nwk] * self.wf.kws)
+            v = np.einsum("bsk,bsk->s", weighted, occ)
+            c = np.einsum("bsk,bsk->s", weighted, 1.0 - occ)
         else:
-            c, v = 0, 0
-            for i in range(nband * nwk * nspin):
-                if occs[i] > 0.5:
-                    v += np.absolute(res[i]) ** 2 * self.wf.kws[i % nwk] / nspin
-                else:
-                    c += np.absolute(res[i]) ** 2 * self.wf.kws[i % nwk] / nspin
+            filled = occ > 0.5
+            v = float(weighted[filled].sum()) / basis.nspin
+            c = float(weighted[~filled].sum()) / basis.nspin
         if self.method == "pseudo":
-            t = v + c
-            v /= t
-            c /= t
+            total = v + c
+            v, c = v / total, c / total
         if spinpol:
-            v, c = v.tolist(), c.tolist()
+            return np.asarray(v, np.float64).tolist(), np.asarray(c, np.float64).tolist()
         return v, c
+
+    def _analysis_bands(self, num_below_ef, num_above_ef, vbmband, band_list, analyze_all):
+        """Bands defect_band_analysis looks at (projector.py:477-494): all of
+        them, an explicit list, or a window around the valence band maximum,
+        i.e. the highest band whose first (k, spin) state has f > 0.5."""
+        if analyze_all:
+            return list(range(self.basis.nband))
+        if band_list:
+            return list(band_list)
+        nb = self.wf.nband
+        first_state_occ = np.asarray(self.wf._get_occs(), np.float64).reshape(nb, -1)[:, 0]
+        occupied = np.flatnonzero(first_state_occ > 0.5)
+        vbm = int(occupied[-1]) if occupied.size else 0
+        if vbmband is not None:
+            vbm = vbmband
+        return list(range(max(vbm - num_below_ef, 0), min(vbm + num_above_ef, nb - 1) + 1))
 
     def defect_band_analysis(self, num_below_ef=20, num_above_ef=20, spinpol=False, return_energies=False,
                              vbmband=None, band_list=None, analyze_all=False):
-        """projector.py:437-503."""
-        if num_below_ef < 0 or num_above_ef < 0:
+        """proportion_conduction of each band picked by _analysis_bands, as
+        {band: (v, c)} (projector.py:437-503); with ``return_energies`` also
+        the bands' energies and occupations."""
+        if min(num_below_ef, num_above_ef) < 0:
             raise ValueError("num_above_ef and num_below_ef must both be nonnegative.")
-        nband = self.basis.nband
-        occs = self.wf._get_occs()
-        if analyze_all:
-            totest = list(range(nband))
-        elif band_list:
-            totest = band_list[:]
-        else:
-            vbm = 0
-            for i in range(self.wf.nband):
-                if occs[i * self.wf.nwk * self.wf.nspin] > 0.5:
-                    vbm = i
-            if vbmband is not None:
-                vbm = vbmband
-            lo, hi = max(vbm - num_below_ef, 0), min(vbm + num_above_ef, self.wf.nband - 1)
-            totest = list(range(lo, hi + 1))
-        results = {b: self.proportion_conduction(b, spinpol=spinpol) for b in totest}
-        if return_energies:
-            return results, self.wf._get_energy_list(totest)
-        return results
+        bands = self._analysis_bands(num_below_ef, num_above_ef, vbmband, band_list, analyze_all)
+        results = {b: self.proportion_conduction(b, spinpol=spinpol) for b in bands}
+        return (results, self.wf._get_energy_list(bands)) if return_energies else results

@@ -467,15 +467,21 @@ def write_volumetric(filename, x, dim, scale):
                 f.write("\n")
 
 
-def wavefunction_from_workload(wl, which="R", nband=None, coeffs=None, setup_projectors=False, element="C"):
-    """Wavefunction of one structure of a synthetic workload (tests, smoke, bench)."""
+def wavefunction_from_workload(wl, which="R", nband=None, coeffs=None, setup_projectors=False, element="C",
+                               paw_overlap=True, occs=None):
+    """Wavefunction of one structure of a synthetic workload (tests, smoke,
+    bench).  With ``paw_overlap`` the workload's channel matrix replaces the
+    partial-wave overlap difference; without it the matrices are those
+    get_projector_list integrates from the species' partial waves.  ``occs``:
+    optional occupations [nk][nband]."""
     frac = wl.frac_R if which == "R" else wl.frac_S
     struct = Structure(wl.lattice, [element] * len(frac), frac)
     cr = CoreRegion({element: Pseudopotential.from_species(wl.species)})
     Gs = wl.ensure_gvecs()
     C = coeffs if coeffs is not None else wl.coefficients(which, nband=nband)
-    wf = Wavefunction.from_arrays(struct, cr, wl.fftg, wl.kpts, wl.kws, Gs, C, wl.encut)
-    wf.set_paw_overlap(element, wl.delta_o)
+    wf = Wavefunction.from_arrays(struct, cr, wl.fftg, wl.kpts, wl.kws, Gs, C, wl.encut, occs=occs)
+    if paw_overlap:
+        wf.set_paw_overlap(element, wl.delta_o)
     if setup_projectors:
         wf.check_c_projectors()
     return wf

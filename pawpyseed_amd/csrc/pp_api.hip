@@ -132,9 +132,21 @@ struct SetupState {
   std::vector<long> zl_off, yl_off;
 };
 
+// Where a prepared setup keeps what setup_prepare computes from the call's
+// G vectors and k-points: the context's buffers for eager calls, a graph's
+// own buffers for a captured pipeline (so that later eager calls on the
+// same context cannot change what a replay reads).
+struct PrepBufs {
+  DevBuf *gidx, *phase, *bounds, *lines;
+  std::vector<int> *lines_host, *bounds_host;
+};
+static PrepBufs ctx_prep_bufs(pp_ctx *ctx) {
+  return {&ctx->gidx, &ctx->phase, &ctx->bounds, &ctx->lines, &ctx->lines_host, &ctx->bounds_host};
+}
+
 template <typename T>
 static int setup_prepare(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac, const int *gofs, const int *G_dev,
-                         int nband, SetupState &S) {
+                         int nband, SetupState &S, const PrepBufs &pb) {
   const int *fftg = sphere_fftg(sp);
   PP_TRY(get_fft_plan(ctx, fftg, &S.pl));
   int NB = ctx->tune_fft_bands > 0 ? ctx->tune_fft_bands : 64;
@@ -159,20 +171,20 @@ static int setup_prepare(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac
     if (!ctx->join)
       PP_CHECK_HIP(hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming));
   }
-  PP_TRY(ctx->gidx.ensure(sizeof(int) * (size_t)std::max(1, gofs[nk])));
-  PP_TRY(ctx->phase.ensure(sizeof(cplx<T>) * (size_t)S.npts_sph * nk));
+  PP_TRY(pb.gidx->ensure(sizeof(int) * (size_t)std::max(1, gofs[nk])));
+  PP_TRY(pb.phase->ensure(sizeof(cplx<T>) * (size_t)S.npts_sph * nk));
   S.grids[0] = ctx->grid.p;
   S.grids[1] = S.two ? ctx->grid2.p : ctx->grid.p;
-  S.phase = ctx->phase.p;
-  S.gidx = (int *)ctx->gidx.p;
+  S.phase = pb.phase->p;
+  S.gidx = (int *)pb.gidx->p;
   // plane-wave sphere bounds of every k (one host sync for the alias flags)
   S.nbnd = 2 * fftg[0] * fftg[1] + 2 * fftg[0] + 3;
-  PP_TRY(ctx->bounds.ensure(sizeof(int) * (size_t)S.nbnd * nk));
-  S.bnd = (int *)ctx->bounds.p;
+  PP_TRY(pb.bounds->ensure(sizeof(int) * (size_t)S.nbnd * nk));
+  S.bnd = (int *)pb.bounds->p;
   for (int k = 0; k < nk; k++)
     PP_TRY(sphere_bounds(ctx, fftg, gofs[k + 1] - gofs[k], G_dev + 3 * (long)gofs[k], S.bnd + (size_t)k * S.nbnd));
-  ctx->bounds_host.resize((size_t)S.nbnd * nk);
-  PP_CHECK_HIP(hipMemcpyAsync(ctx->bounds_host.data(), S.bnd, sizeof(int) * ctx->bounds_host.size(),
+  pb.bounds_host[0].resize((size_t)S.nbnd * nk);
+  PP_CHECK_HIP(hipMemcpyAsync(pb.bounds_host[0].data(), S.bnd, sizeof(int) * pb.bounds_host[0].size(),
                               hipMemcpyDeviceToHost, ctx->stream));
   PP_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   S.alias.assign(nk, 1);
@@ -181,35 +193,35 @@ static int setup_prepare(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac
   // empty line
   S.zl_off.assign(nk + 1, 0);
   S.yl_off.assign(nk + 1, 0);
-  ctx->lines_host.clear();
+  pb.lines_host[0].clear();
   const int n0 = fftg[0], n1 = fftg[1], n2 = fftg[2];
   for (int k = 0; k < nk; k++) {
-    const int *hb = ctx->bounds_host.data() + (size_t)k * S.nbnd;
+    const int *hb = pb.bounds_host[0].data() + (size_t)k * S.nbnd;
     // no pruning when the grid aliases a G or the G set leaves holes in a
     // z-line's interval (not a cutoff sphere)
     S.alias[k] = hb[S.nbnd - 1] || !bounds_filled(hb, fftg, gofs[k + 1] - gofs[k]);
     const int *yb = hb + 2 * n0 * n1, *xb = yb + 2 * n0;
-    S.zl_off[k] = (long)ctx->lines_host.size();
+    S.zl_off[k] = (long)pb.lines_host[0].size();
     if (!S.alias[k])
       for (int sx = xb[0]; sx <= xb[1]; sx++) {
         const int x = (sx + n0) % n0;
         for (int sy = yb[2 * x]; sy <= yb[2 * x + 1]; sy++)
-          ctx->lines_host.push_back(x * n1 + (sy + n1) % n1);
+          pb.lines_host[0].push_back(x * n1 + (sy + n1) % n1);
       }
-    S.yl_off[k] = (long)ctx->lines_host.size();
+    S.yl_off[k] = (long)pb.lines_host[0].size();
     if (!S.alias[k])
       for (int sx = xb[0]; sx <= xb[1]; sx++)
         for (int z = 0; z < n2; z++)
-          ctx->lines_host.push_back(((sx + n0) % n0) * n2 + z);
-    S.zl_off[k + 1] = (long)ctx->lines_host.size();
+          pb.lines_host[0].push_back(((sx + n0) % n0) * n2 + z);
+    S.zl_off[k + 1] = (long)pb.lines_host[0].size();
   }
   S.dlines = nullptr;
-  if (!ctx->lines_host.empty()) {
-    PP_TRY(ctx->lines.ensure(sizeof(int) * ctx->lines_host.size()));
-    S.dlines = (int *)ctx->lines.p;
+  if (!pb.lines_host[0].empty()) {
+    PP_TRY(pb.lines->ensure(sizeof(int) * pb.lines_host[0].size()));
+    S.dlines = (int *)pb.lines->p;
     // the streams are idle here (synchronised above), and lines_host lives in
     // the context until the next call's synchronisation
-    PP_CHECK_HIP(hipMemcpyAsync(S.dlines, ctx->lines_host.data(), sizeof(int) * ctx->lines_host.size(),
+    PP_CHECK_HIP(hipMemcpyAsync(S.dlines, pb.lines_host[0].data(), sizeof(int) * pb.lines_host[0].size(),
                                 hipMemcpyHostToDevice, ctx->stream));
   }
   // grid indices and Bloch phases of every k, so the chunks of all k can be
@@ -283,7 +295,7 @@ template <typename T>
 static int setup_batched(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac, const int *gofs, const int *G_dev,
                          const cplx<float> *C, long ldc, int nband, cplx<double> *proj) {
   SetupState S;
-  PP_TRY(setup_prepare<T>(ctx, sp, nk, kfrac, gofs, G_dev, nband, S));
+  PP_TRY(setup_prepare<T>(ctx, sp, nk, kfrac, gofs, G_dev, nband, S, ctx_prep_bufs(ctx)));
   return setup_launch<T>(ctx, sp, nk, gofs, C, ldc, nband, proj, S);
 }
 
@@ -649,13 +661,22 @@ int pp_autotune(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac, const i
       if ((st = trial(ms)) != PP_OK)
         break;
       rep += std::string(c.name) + " " + std::to_string(v) + " " + std::to_string(ms) + "\n";
-      if (ms < best_ms) {
+      // a candidate replaces the value in place only when it is faster by
+      // more than the run-to-run spread (2%), so that equal settings keep
+      // the library default and timed runs stay comparable
+      const bool incumbent = v == cur;
+      if (best_ms == 1e30f || (incumbent ? ms < best_ms * 1.02f : ms * 1.02f < best_ms)) {
         best_ms = ms;
         best_v = v;
       }
     }
-    if (st != PP_OK)
+    if (st != PP_OK) {
+      // leave the context as it was before this parameter's trials
+      const std::string msg = pp_last_error();
+      pp_ctx_set_param(ctx, c.name, cur);
+      set_error(st, msg);
       break;
+    }
     pp_ctx_set_param(ctx, c.name, best_v);
   }
   if (st == PP_OK) {
@@ -677,19 +698,25 @@ int pp_autotune(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac, const i
 
 // A captured setup pipeline: the graph, and what it was captured against
 // (the plan's build generation, the context buffers it reads and writes).
+// The graph owns the buffers setup_prepare fills from the call's G vectors
+// and k-points (grid indices, Bloch phases, plane-wave bounds, line lists):
+// eager calls with other inputs on the same context leave them alone.  The
+// context's chunk grids and partial sums are scratch whose contents no
+// launch carries over; only their addresses must stay those of the capture.
 struct pp_graph {
   pp_ctx *ctx = nullptr;
   pp_sphere *sp = nullptr;
   long sp_gen = 0;
   std::vector<std::pair<void *, size_t>> bufs;
+  DevBuf gidx, phase, bounds, lines;
+  std::vector<int> lines_host, bounds_host;
   hipGraph_t g = nullptr;
   hipGraphExec_t ge = nullptr;
 };
 
 static std::vector<std::pair<void *, size_t>> ctx_buffers(pp_ctx *ctx) {
   std::vector<std::pair<void *, size_t>> v;
-  for (const DevBuf *b : {&ctx->grid, &ctx->grid2, &ctx->partial, &ctx->partial2, &ctx->gidx, &ctx->phase,
-                          &ctx->bounds, &ctx->lines})
+  for (const DevBuf *b : {&ctx->grid, &ctx->grid2, &ctx->partial, &ctx->partial2})
     v.push_back({b->p, b->bytes});
   return v;
 }
@@ -698,7 +725,8 @@ template <typename T>
 static int graph_capture(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac, const int *gofs, const int *G_dev,
                          const cplx<float> *C, long ldc, int nband, cplx<double> *proj, pp_graph *gr) {
   SetupState S;
-  PP_TRY(setup_prepare<T>(ctx, sp, nk, kfrac, gofs, G_dev, nband, S));
+  const PrepBufs pb = {&gr->gidx, &gr->phase, &gr->bounds, &gr->lines, &gr->lines_host, &gr->bounds_host};
+  PP_TRY(setup_prepare<T>(ctx, sp, nk, kfrac, gofs, G_dev, nband, S, pb));
   PP_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   PP_CHECK_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
   int st = setup_launch<T>(ctx, sp, nk, gofs, C, ldc, nband, proj, S);

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -23,7 +24,17 @@ struct PairCache {
   int nbR = 0, nbS = 0;
 };
 
+// Identities of wavefunctions for the caches other wavefunctions keep: a
+// process-wide counter, never reused, so a freed pswf_t whose address comes
+// back from calloc for another structure can never hit an old entry.
+static std::atomic<unsigned long long> g_uid{0};
+static unsigned long long next_uid() { return ++g_uid; }
+
 struct WfDev {
+  // identity of this wavefunction's coefficients (fixed once uploaded) and
+  // of its current projections (renewed by every setup_projections)
+  unsigned long long uid = next_uid();
+  unsigned long long proj_uid = 0;
   int nkt = 0, nband = 0;
   long ldc = 0;
   std::vector<int> nw, gofs;
@@ -35,21 +46,23 @@ struct WfDev {
   std::vector<int> proj_off, labels;
   std::vector<int> sp_nchan, sp_ls;
   std::vector<double> sp_dO;  // aepw - pspw per species
-  std::map<std::tuple<const void *, int, int>, PairCache> overlap_cache;
-  std::map<std::tuple<const void *, int, unsigned long long>, PairCache> aug_cache;
+  // keyed by the other wavefunction's uid (overlaps) or proj_uid (terms
+  // built from its projections)
+  std::map<std::tuple<unsigned long long, int, int>, PairCache> overlap_cache;
+  std::map<std::tuple<unsigned long long, int, unsigned long long>, PairCache> aug_cache;
   // <(phi - phi~)_smooth | psi~> at the other structure's unmatched sites
   // (overlap_setup_real): c128 [nkt][nband][wp_ptot]; wp_labels index the
   // other structure's species
   DevBuf wp;
   int wp_n = 0, wp_ptot = 0;
-  const void *wp_of = nullptr;  // the pswf_t whose sites they are
+  unsigned long long wp_of = 0;  // uid of the wavefunction whose sites they are
   std::vector<int> wp_off, wp_labels, wp_nchan, wp_ls;
   unsigned long long wp_hash = 0;
   // off-site overlap blocks of the N_RS pairs (overlap_setup_real), kept on
   // the projected structure: conj of the partial-wave overlaps, [P_R x P_S]
   // per pair at on_off[t], and the minimum-image pair vectors
   int on_n = 0;
-  const void *on_of = nullptr;
+  unsigned long long on_of = 0;  // uid of the reference wavefunction
   std::vector<int> on_R, on_S;
   std::vector<pp_cdouble> on_ov;
   std::vector<long> on_off;
@@ -249,19 +262,35 @@ static pswf_t *read_wavecar_src(Src &src, const double *kpt_weights) {
     set_error(PP_ERR_IO, "WAVECAR: cannot read header");
     return nullptr;
   }
-  long nrecl = lround(head[0]);
-  int nspin = (int)lround(head[1]);
-  int nprec = (int)lround(head[2]);
-  if (nrecl <= 0 || (nspin != 1 && nspin != 2) || nprec != 45200) {
+  if (!(head[1] == 1.0 || head[1] == 2.0) || head[2] != 45200.0) {
     set_error(PP_ERR_IO, "WAVECAR: unsupported header (only single-precision WAVECARs)");
     return nullptr;
   }
-  std::vector<double> rec(nrecl / 8);
+  const int nspin = (int)head[1];
+  // a record holds at least the 12 doubles of record 1 (counts, cutoff,
+  // lattice) and whole doubles; the size bound keeps offsets in range
+  const long nrecl = (head[0] >= 96.0 && head[0] <= (double)(1L << 40)) ? lround(head[0]) : -1;
+  if (nrecl < 96 || nrecl % 8 != 0) {
+    set_error(PP_ERR_IO, "WAVECAR: bad record length " + std::to_string(nrecl));
+    return nullptr;
+  }
+  const long nrd = nrecl / 8;  // doubles per record
+  std::vector<double> rec(nrd);
   if (!src.read(nrecl, rec.data(), nrecl)) {
     set_error(PP_ERR_IO, "WAVECAR: cannot read record 1");
     return nullptr;
   }
-  int nwk = (int)lround(rec[0]), nband = (int)lround(rec[1]);
+  if (!std::isfinite(rec[0]) || !std::isfinite(rec[1]) || !(rec[2] > 0.0) || !std::isfinite(rec[2])) {
+    set_error(PP_ERR_IO, "WAVECAR: bad record 1");
+    return nullptr;
+  }
+  const double nwk_d = rec[0], nband_d = rec[1];
+  // each k-point record holds 4 doubles and 3 per band (energy, occupation)
+  if (nwk_d < 1 || nband_d < 1 || nwk_d > 1e7 || 7 + 3 * (nband_d - 1) > (double)nrd) {
+    set_error(PP_ERR_IO, "WAVECAR: k-point or band count does not fit the record length");
+    return nullptr;
+  }
+  int nwk = (int)lround(nwk_d), nband = (int)lround(nband_d);
   double encut = rec[2];
   pswf_t *wf = new_pswf(nspin, nwk, nband, encut, &rec[3]);
   double nb[3];
@@ -272,6 +301,12 @@ static pswf_t *read_wavecar_src(Src &src, const double *kpt_weights) {
     long irec = (long)iwk * (1 + nband);
     if (!src.read((irec + 2) * nrecl, rec.data(), nrecl)) {
       set_error(PP_ERR_IO, "WAVECAR: truncated k-point record");
+      free_pswf(wf);
+      return nullptr;
+    }
+    // a band record of nplane complex64 values must fit one record
+    if (!std::isfinite(rec[0]) || rec[0] < 0 || rec[0] > (double)nrd) {
+      set_error(PP_ERR_IO, "WAVECAR: plane-wave count does not fit the record length");
       free_pswf(wf);
       return nullptr;
     }
@@ -773,6 +808,7 @@ int setup_projections(pswf_t *wf, ppot_t *pps, int num_elems, int num_sites, int
   d->sp_ls = a.ls;
   d->sp_dO = a.dO;
   d->proj_precision = prec;
+  d->proj_uid = next_uid();
   d->aug_cache.clear();
   if (wf->is_ncl) {
     // onto_projector_ncl (projector.c:359-392): up and down lists
@@ -800,7 +836,7 @@ static int setup_offsite(pswf_t *wf_R, pswf_t *wf_S, const int *labels_R, const 
   wf_S->dcoords = nullptr;
   WfDev *dS = dev_of(wf_S);
   dS->on_n = num_N_RS;
-  dS->on_of = wf_R;
+  dS->on_of = dev_of(wf_R)->uid;
   dS->on_R.assign(N_RS_R, N_RS_R + num_N_RS);
   dS->on_S.assign(N_RS_S, N_RS_S + num_N_RS);
   dS->on_ov.clear();
@@ -915,7 +951,7 @@ int overlap_setup_real(pswf_t *wf_R, pswf_t *wf_S, int *labels_R, int *labels_S,
       }
     sd.wf->wp_num = sd.n;
     d->wp_n = sd.n;
-    d->wp_of = sd.owner;
+    d->wp_of = dev_of(sd.owner)->uid;
     d->aug_cache.clear();
     d->wp_hash = hash_ints(1469598103934665603ull, sd.sites, sd.n);
     std::vector<int> lab(sd.n);
@@ -971,7 +1007,7 @@ int pseudoprojection(pp_cdouble *projections, pswf_t *wf_ref, pswf_t *wf_proj, i
   WfDev *dr = dev_of(wf_ref), *dp = dev_of(wf_proj);
   const int nkt = dr->nkt;
   int flip = (flip_spin && wf_ref->nspin == 2) ? 1 : 0;
-  auto key = std::make_tuple((const void *)wf_ref, flip, g_precision);
+  auto key = std::make_tuple(dr->uid, flip, g_precision);
   auto it = dp->overlap_cache.find(key);
   if (it == dp->overlap_cache.end()) {
     // pseudoprojector.c:85: length of the reference band at the unflipped k
@@ -1032,12 +1068,12 @@ int compensation_terms(pp_cdouble *overlap, int BAND_NUM, pswf_t *wf_S, pswf_t *
   // the wave projections of overlap_setup_real must be those of these lists
   const unsigned long long hR = hash_ints(1469598103934665603ull, N_R, num_N_R);
   const unsigned long long hS = hash_ints(1469598103934665603ull, N_S, num_N_S);
-  PP_REQUIRE(num_N_R == 0 || (dS->wp_n == num_N_R && dS->wp_of == wf_R && dS->wp_hash == hR), PP_ERR_ARG,
+  PP_REQUIRE(num_N_R == 0 || (dS->wp_n == num_N_R && dS->wp_of == dR->uid && dS->wp_hash == hR), PP_ERR_ARG,
              "compensation_terms: call overlap_setup_real with this N_R list first");
-  PP_REQUIRE(num_N_S == 0 || (dR->wp_n == num_N_S && dR->wp_of == wf_S && dR->wp_hash == hS), PP_ERR_ARG,
+  PP_REQUIRE(num_N_S == 0 || (dR->wp_n == num_N_S && dR->wp_of == dS->uid && dR->wp_hash == hS), PP_ERR_ARG,
              "compensation_terms: call overlap_setup_real with this N_S list first");
   if (num_N_RS > 0) {
-    bool same = dS->on_n == num_N_RS && dS->on_of == wf_R;
+    bool same = dS->on_n == num_N_RS && dS->on_of == dR->uid;
     for (int t = 0; same && t < num_N_RS; t++)
       same = dS->on_R[t] == N_RS_R[t] && dS->on_S[t] == N_RS_S[t];
     PP_REQUIRE(same, PP_ERR_ARG, "compensation_terms: call overlap_setup_real with these N_RS pairs first");
@@ -1059,7 +1095,7 @@ int compensation_terms(pp_cdouble *overlap, int BAND_NUM, pswf_t *wf_S, pswf_t *
     h = hash_ints(h, N_RS_R, num_N_RS);
     h = hash_ints(h, N_RS_S, num_N_RS);
   }
-  auto key = std::make_tuple((const void *)wf_R, flip, h);
+  auto key = std::make_tuple(dR->proj_uid, flip, h);
   auto it = dS->aug_cache.find(key);
   if (it == dS->aug_cache.end()) {
     DevBuf out;
@@ -1257,7 +1293,7 @@ int compensation_terms_recip(pp_cdouble *overlap, int BAND_NUM, pswf_t *wf_S, ps
                                    dS->ca_hash == hash_ints(1469598103934665603ull, N_S, num_N_S))),
              PP_ERR_ARG, "compensation_terms_recip: call overlap_setup_recip with these N_R/N_S lists first");
   if (num_N_RS > 0) {
-    bool same = dS->on_n == num_N_RS && dS->on_of == wf_R;
+    bool same = dS->on_n == num_N_RS && dS->on_of == dR->uid;
     for (int t = 0; same && t < num_N_RS; t++)
       same = dS->on_R[t] == N_RS_R[t] && dS->on_S[t] == N_RS_S[t];
     PP_REQUIRE(same, PP_ERR_ARG, "compensation_terms_recip: call overlap_setup_recip with these N_RS pairs first");
@@ -1273,7 +1309,7 @@ int compensation_terms_recip(pp_cdouble *overlap, int BAND_NUM, pswf_t *wf_S, ps
   h = hash_ints(h, N_S, num_N_S);
   h = hash_ints(h, N_RS_R, num_N_RS);
   h = hash_ints(h, N_RS_S, num_N_RS);
-  auto key = std::make_tuple((const void *)wf_R, flip, h);
+  auto key = std::make_tuple(dR->proj_uid, flip, h);
   auto it = dS->aug_cache.find(key);
   if (it == dS->aug_cache.end()) {
     DevBuf out;
@@ -1372,8 +1408,9 @@ static int rs_plan_for(pp_ctx *ctx, pswf_t *wf, const int *fftg, const int *labe
   h = hash_ints(h, fftg, 3);
   h = hash_ints(h, labels, ns);
   h = hash_ints(h, (const int *)coords, 6 * ns);
-  const void *pps = wf->pps;
-  h = hash_ints(h, (const int *)&pps, (int)(sizeof(pps) / sizeof(int)));
+  // the partial waves come from the projector list of the last
+  // setup_projections, whose identity proj_uid carries
+  h = hash_ints(h, (const int *)&d->proj_uid, (int)(sizeof(d->proj_uid) / sizeof(int)));
   if (d->rs && d->rs_hash == h) {
     *out = d->rs;
     return PP_OK;

@@ -187,22 +187,76 @@ __global__ __launch_bounds__(256) void k_splitk_reduce(int M, int N, int nsplit,
 int gemm_fp32_mfma(pp_ctx *ctx, int precision, int M, int N, int K, const void *A, long lda, const void *B, long ldb,
                    cplx<double> *C, long ldc, bool accumulate, int slot);
 
+// Split-K policy of the overlap GEMMs: enough K chunks that the 64x64
+// output tiles times the chunks give about 1024 work-groups (four per CU),
+// and chunks of at least 256 complex terms.  The FP32 kernels fold their
+// FP32 sums into FP64 every 512 terms inside a chunk, so the chunk length
+// does not bound their accuracy; 3xBF16 keeps chunks of at most 512 terms
+// (its FP32 sums are not folded).  A positive `tune` forces the count.
+int splitk_count(int precision, int M, int N, int K, int tune) {
+  if (tune > 0)
+    return tune;
+  const int tiles = std::max(1, ((M + 63) / 64) * ((N + 63) / 64));
+  int n = (1024 + tiles - 1) / tiles;
+  if (precision == PP_BF16X3)
+    n = std::max((2048 + tiles - 1) / tiles, (K + 511) / 512);
+  n = std::min(n, std::max(1, K / 256));
+  return std::max(1, n);
+}
+
+// FP64 GEMM of one batch into C (nsplit == 1) or into per-chunk partials P
+// ([nsplit][M][N]); kchunk a multiple of 16.  No allocation.
+int launch_zgemm64(hipStream_t st, bool valu, int M, int N, int K, int atype, const void *A, long lda, int btype,
+                   const void *B, long ldb, cplx<double> *dst, long dld, long sSplit, int nsplit, int kchunk) {
+  const int tiles = ((M + 63) / 64) * ((N + 63) / 64);
+  dim3 grd(tiles, nsplit);
+  if (!valu) {
+#define PP_MF(TA, TB)                                                                                          \
+  hipLaunchKernelGGL((k_zgemm_mfma64<TA, TB>), grd, dim3(256), 0, st, M, N, kchunk, K, (const cplx<TA> *)A, lda, \
+                     (const cplx<TB> *)B, ldb, dst, dld, sSplit)
+    if (atype == 1 && btype == 1)
+      PP_MF(float, float);
+    else if (atype == 0 && btype == 0)
+      PP_MF(double, double);
+    else if (atype == 0 && btype == 1)
+      PP_MF(double, float);
+    else
+      PP_MF(float, double);
+#undef PP_MF
+  } else {
+#define PP_MV(TA, TB)                                                                                          \
+  hipLaunchKernelGGL((k_zgemm_f64<TA, TB>), grd, dim3(256), 0, st, M, N, kchunk, K, (const cplx<TA> *)A, lda,    \
+                     (const cplx<TB> *)B, ldb, dst, dld, sSplit)
+    if (atype == 1 && btype == 1)
+      PP_MV(float, float);
+    else if (atype == 0 && btype == 0)
+      PP_MV(double, double);
+    else if (atype == 0 && btype == 1)
+      PP_MV(double, float);
+    else
+      PP_MV(float, double);
+#undef PP_MV
+  }
+  PP_CHECK_HIP(hipGetLastError());
+  return PP_OK;
+}
+
+int launch_splitk_reduce(hipStream_t st, int M, int N, int nsplit, const cplx<double> *P, cplx<double> *C, long ldc,
+                         bool accumulate) {
+  const long tot = (long)M * N;
+  hipLaunchKernelGGL(k_splitk_reduce, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, M, N, nsplit, P, C, ldc,
+                     accumulate ? 1 : 0);
+  PP_CHECK_HIP(hipGetLastError());
+  return PP_OK;
+}
+
 static int zgemm_one(pp_ctx *ctx, int precision, int M, int N, int K, int atype, const void *A, long lda, int btype,
                      const void *B, long ldb, cplx<double> *C, long ldc, bool accumulate, int cls, int slot) {
   if (M <= 0 || N <= 0)
     return PP_OK;
   if (precision != PP_FP64 && atype == 1 && btype == 1)
     return gemm_fp32_mfma(ctx, precision, M, N, K, A, lda, B, ldb, C, ldc, accumulate, slot);
-  int tiles = ((M + 63) / 64) * ((N + 63) / 64);
-  int nsplit = ctx->tune_splitk;
-  if (nsplit <= 0) {
-    nsplit = (2048 + tiles - 1) / tiles;
-    int maxs = std::max(1, K / 512);
-    if (nsplit > maxs)
-      nsplit = maxs;
-  }
-  if (nsplit < 1)
-    nsplit = 1;
+  int nsplit = splitk_count(PP_FP64, M, N, K, ctx->tune_splitk);
   int kchunk = (K + nsplit - 1) / nsplit;
   kchunk = (kchunk + 15) / 16 * 16;
   nsplit = std::max(1, (K + kchunk - 1) / std::max(1, kchunk));
@@ -215,45 +269,18 @@ static int zgemm_one(pp_ctx *ctx, int precision, int M, int N, int K, int atype,
     dld = N;
     sSplit = (long)M * N;
   }
-  dim3 grd(tiles, nsplit);
   hipEvent_t ev;
   tbegin(ctx, cls, &ev);
-  if (ctx->tune_gemm_f64_valu == 0) {
-#define PP_MF(TA, TB)                                                                                          \
-  hipLaunchKernelGGL((k_zgemm_mfma64<TA, TB>), grd, dim3(256), 0, ctx->stream, M, N, kchunk, K, (const cplx<TA> *)A, \
-                     lda, (const cplx<TB> *)B, ldb, dst, dld, sSplit)
-    if (atype == 1 && btype == 1)
-      PP_MF(float, float);
-    else if (atype == 0 && btype == 0)
-      PP_MF(double, double);
-    else if (atype == 0 && btype == 1)
-      PP_MF(double, float);
-    else
-      PP_MF(float, double);
-#undef PP_MF
-  } else if (atype == 1 && btype == 1)
-    hipLaunchKernelGGL((k_zgemm_f64<float, float>), grd, dim3(256), 0, ctx->stream, M, N, kchunk, K,
-                       (const cplx<float> *)A, lda, (const cplx<float> *)B, ldb, dst, dld, sSplit);
-  else if (atype == 0 && btype == 0)
-    hipLaunchKernelGGL((k_zgemm_f64<double, double>), grd, dim3(256), 0, ctx->stream, M, N, kchunk, K,
-                       (const cplx<double> *)A, lda, (const cplx<double> *)B, ldb, dst, dld, sSplit);
-  else if (atype == 0 && btype == 1)
-    hipLaunchKernelGGL((k_zgemm_f64<double, float>), grd, dim3(256), 0, ctx->stream, M, N, kchunk, K,
-                       (const cplx<double> *)A, lda, (const cplx<float> *)B, ldb, dst, dld, sSplit);
-  else
-    hipLaunchKernelGGL((k_zgemm_f64<float, double>), grd, dim3(256), 0, ctx->stream, M, N, kchunk, K,
-                       (const cplx<float> *)A, lda, (const cplx<double> *)B, ldb, dst, dld, sSplit);
+  int st = launch_zgemm64(ctx->stream, ctx->tune_gemm_f64_valu != 0, M, N, K, atype, A, lda, btype, B, ldb, dst, dld,
+                          sSplit, nsplit, std::max(16, kchunk));
   tend(ctx, cls, ev);
-  PP_CHECK_HIP(hipGetLastError());
+  PP_TRY(st);
   if (!direct) {
-    long tot = (long)M * N;
     tbegin(ctx, K_GEMM_REDUCE, &ev);
-    hipLaunchKernelGGL(k_splitk_reduce, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream, M, N, nsplit,
-                       (const cplx<double> *)ctx->gemm_ws.p, C, ldc, accumulate ? 1 : 0);
+    st = launch_splitk_reduce(ctx->stream, M, N, nsplit, (const cplx<double> *)ctx->gemm_ws.p, C, ldc, accumulate);
     tend(ctx, K_GEMM_REDUCE, ev);
-    PP_CHECK_HIP(hipGetLastError());
   }
-  return PP_OK;
+  return st;
 }
 
 int zgemm_nt(pp_ctx *ctx, int precision, int M, int N, int K, int atype, const void *A, long lda, long sA, int btype,

@@ -7,8 +7,9 @@
 // so no operand repacking is needed for the real part and B'' is the
 // stored pair swapped with one sign flip.
 //
-// PP_FP32: v_mfma_f32_16x16x4_f32, FP32 accumulation over split-K chunks
-// of at most 512 complex terms, chunks summed in FP64.  PP_BF16X3: each FP32 operand is split into
+// PP_FP32: v_mfma_f32_16x16x4_f32; FP32 sums are folded into FP64 every
+// 512 complex terms inside a split-K chunk, chunks summed in FP64.
+// PP_BF16X3: each FP32 operand is split into
 // hi = bf16(x) and lo = bf16(x - hi); hi*hi + hi*lo + lo*hi on
 // v_mfma_f32_16x16x32_bf16 with FP32 accumulation.
 //
@@ -276,16 +277,55 @@ __global__ __launch_bounds__(256) void k_cgemm_bf16x3(int M, int N, int kchunk, 
                  lane);
 }
 
-__global__ __launch_bounds__(256) void k_splitk_reduce2(int M, int N, int nsplit, const cplx<double> *__restrict__ P,
-                                                        cplx<double> *__restrict__ C, long ldc, int accumulate) {
-  long t = (long)blockIdx.x * 256 + threadIdx.x;
-  if (t >= (long)M * N)
-    return;
-  int m = (int)(t / N), n = (int)(t % N);
-  cplx<double> s = accumulate ? C[(long)m * ldc + n] : mk<double>(0, 0);
-  for (int q = 0; q < nsplit; q++)
-    s = s + P[(long)q * M * N + t];
-  C[(long)m * ldc + n] = s;
+int splitk_count(int precision, int M, int N, int K, int tune);
+int launch_splitk_reduce(hipStream_t st, int M, int N, int nsplit, const cplx<double> *P, cplx<double> *C, long ldc,
+                         bool accumulate);
+
+// K chunk (complex terms) of an FP32 / 3xBF16 GEMM split nsplit ways: a
+// multiple of the kernel's k-step; nsplit is reduced to the chunks needed.
+int cgemm_chunk(int precision, int K, int &nsplit) {
+  const int kstep = precision == PP_BF16X3 ? 32 : 16;
+  int kchunk = (K + nsplit - 1) / std::max(1, nsplit);
+  kchunk = (kchunk + kstep - 1) / kstep * kstep;
+  if (kchunk == 0)
+    kchunk = kstep;
+  nsplit = std::max(1, (K + kchunk - 1) / kchunk);
+  return kchunk;
+}
+
+// The FP32 / 3xBF16 GEMM of one batch into FP64 partials P
+// ([nsplit][M][ldp], split stride sSplit; with nsplit == 1 P may be the
+// output itself).  FP32 chunks longer than 512 complex terms fold their
+// FP32 sums into FP64 every 512 terms.  No allocation.
+int launch_cgemm32(hipStream_t st, int precision, int M, int N, int K, const void *A, long lda, const void *B,
+                   long ldb, cplx<double> *P, long ldp, long sSplit, int nsplit, int kchunk) {
+  const int tiles = ((M + 63) / 64) * ((N + 63) / 64);
+  const bool vec = (lda % 2 == 0) && (ldb % 2 == 0) && ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0);
+  dim3 grd(tiles, nsplit);
+  if (precision == PP_BF16X3) {
+    if (vec)
+      hipLaunchKernelGGL(k_cgemm_bf16x3<true>, grd, dim3(256), 0, st, M, N, kchunk, K, (const float *)A, lda,
+                         (const float *)B, ldb, P, ldp, sSplit);
+    else
+      hipLaunchKernelGGL(k_cgemm_bf16x3<false>, grd, dim3(256), 0, st, M, N, kchunk, K, (const float *)A, lda,
+                         (const float *)B, ldb, P, ldp, sSplit);
+  } else {
+    const bool fold = kchunk > 512;
+    if (vec && fold)
+      hipLaunchKernelGGL((k_cgemm_f32<true, true>), grd, dim3(256), 0, st, M, N, kchunk, K, (const float *)A, lda,
+                         (const float *)B, ldb, P, ldp, sSplit);
+    else if (vec)
+      hipLaunchKernelGGL((k_cgemm_f32<true, false>), grd, dim3(256), 0, st, M, N, kchunk, K, (const float *)A, lda,
+                         (const float *)B, ldb, P, ldp, sSplit);
+    else if (fold)
+      hipLaunchKernelGGL((k_cgemm_f32<false, true>), grd, dim3(256), 0, st, M, N, kchunk, K, (const float *)A, lda,
+                         (const float *)B, ldb, P, ldp, sSplit);
+    else
+      hipLaunchKernelGGL((k_cgemm_f32<false, false>), grd, dim3(256), 0, st, M, N, kchunk, K, (const float *)A, lda,
+                         (const float *)B, ldb, P, ldp, sSplit);
+  }
+  PP_CHECK_HIP(hipGetLastError());
+  return PP_OK;
 }
 
 // slot < 0: partials in gemm_ws, reduction on the context's stream.
@@ -295,71 +335,29 @@ __global__ __launch_bounds__(256) void k_splitk_reduce2(int M, int N, int nsplit
 // has read its partials.  The caller joins the second stream at the end.
 int gemm_fp32_mfma(pp_ctx *ctx, int precision, int M, int N, int K, const void *A, long lda, const void *B, long ldb,
                    cplx<double> *C, long ldc, bool accumulate, int slot) {
-  const int kstep = precision == PP_BF16X3 ? 32 : 16;
-  int tiles = ((M + 63) / 64) * ((N + 63) / 64);
-  // FP32 accumulation runs over one split-K chunk of at most 512 complex
-  // terms; chunks are summed in FP64 by the reduction kernel.
-  int nsplit = ctx->tune_splitk;
-  if (nsplit <= 0) {
-    nsplit = std::max((2048 + tiles - 1) / tiles, (K + 511) / 512);
-    int maxs = std::max(1, K / 256);
-    if (nsplit > maxs)
-      nsplit = maxs;
-  }
-  if (nsplit < 1)
-    nsplit = 1;
-  int kchunk = (K + nsplit - 1) / std::max(1, nsplit);
-  kchunk = (kchunk + kstep - 1) / kstep * kstep;
-  if (kchunk == 0)
-    kchunk = kstep;
-  nsplit = std::max(1, (K + kchunk - 1) / kchunk);
+  int nsplit = splitk_count(precision, M, N, K, ctx->tune_splitk);
+  const int kchunk = cgemm_chunk(precision, K, nsplit);
   DevBuf &ws = slot == 1 ? ctx->gemm_ws2 : ctx->gemm_ws;
   PP_TRY(ws.ensure(sizeof(cplx<double>) * (size_t)nsplit * M * N));
   cplx<double> *P = (cplx<double> *)ws.p;
   if (slot >= 0)
     PP_CHECK_HIP(hipStreamWaitEvent(ctx->stream, ctx->red_done[slot], 0));
-  bool vec = (lda % 2 == 0) && (ldb % 2 == 0) && ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0);
-  dim3 grd(tiles, nsplit);
   hipEvent_t ev;
   tbegin(ctx, K_GEMM, &ev);
-  if (precision == PP_BF16X3) {
-    if (vec)
-      hipLaunchKernelGGL(k_cgemm_bf16x3<true>, grd, dim3(256), 0, ctx->stream, M, N, kchunk, K, (const float *)A,
-                         lda, (const float *)B, ldb, P, (long)N, (long)M * N);
-    else
-      hipLaunchKernelGGL(k_cgemm_bf16x3<false>, grd, dim3(256), 0, ctx->stream, M, N, kchunk, K, (const float *)A,
-                         lda, (const float *)B, ldb, P, (long)N, (long)M * N);
-  } else {
-    // FP64 folds inside the tile only when a caller forces long chunks
-    const bool fold = ctx->tune_splitk > 0 && (K + ctx->tune_splitk - 1) / ctx->tune_splitk > 1024;
-    if (vec && fold)
-      hipLaunchKernelGGL((k_cgemm_f32<true, true>), grd, dim3(256), 0, ctx->stream, M, N, kchunk, K,
-                         (const float *)A, lda, (const float *)B, ldb, P, (long)N, (long)M * N);
-    else if (vec)
-      hipLaunchKernelGGL((k_cgemm_f32<true, false>), grd, dim3(256), 0, ctx->stream, M, N, kchunk, K,
-                         (const float *)A, lda, (const float *)B, ldb, P, (long)N, (long)M * N);
-    else if (fold)
-      hipLaunchKernelGGL((k_cgemm_f32<false, true>), grd, dim3(256), 0, ctx->stream, M, N, kchunk, K,
-                         (const float *)A, lda, (const float *)B, ldb, P, (long)N, (long)M * N);
-    else
-      hipLaunchKernelGGL((k_cgemm_f32<false, false>), grd, dim3(256), 0, ctx->stream, M, N, kchunk, K,
-                         (const float *)A, lda, (const float *)B, ldb, P, (long)N, (long)M * N);
-  }
+  int st = launch_cgemm32(ctx->stream, precision, M, N, K, A, lda, B, ldb, P, (long)N, (long)M * N, nsplit, kchunk);
   tend(ctx, K_GEMM, ev);
-  PP_CHECK_HIP(hipGetLastError());
+  PP_TRY(st);
   hipStream_t main = ctx->stream;
   if (slot >= 0) {
     PP_CHECK_HIP(hipEventRecord(ctx->gemm_done[slot], main));
     PP_CHECK_HIP(hipStreamWaitEvent(ctx->aux, ctx->gemm_done[slot], 0));
     ctx->stream = ctx->aux;
   }
-  long tot = (long)M * N;
   tbegin(ctx, K_GEMM_REDUCE, &ev);
-  hipLaunchKernelGGL(k_splitk_reduce2, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream, M, N, nsplit,
-                     (const cplx<double> *)P, C, ldc, accumulate ? 1 : 0);
+  st = launch_splitk_reduce(ctx->stream, M, N, nsplit, P, C, ldc, accumulate);
   tend(ctx, K_GEMM_REDUCE, ev);
   ctx->stream = main;
-  PP_CHECK_HIP(hipGetLastError());
+  PP_TRY(st);
   if (slot >= 0)
     PP_CHECK_HIP(hipEventRecord(ctx->red_done[slot], ctx->aux));
   return PP_OK;

@@ -296,3 +296,55 @@ def tiny_config(ncell=1, encut=150.0, fftg=(20, 20, 20), nband=8, kpts=None, ls=
     idx = np.arange(len(frac), dtype=np.int32)
     return Workload("tiny", lattice, frac, frac_S, sp, np.array(fftg, np.int32), encut,
                     kp, np.full(len(kp), 1.0 / len(kp)), nband, dO, idx, idx.copy())
+
+
+def device_coefficients(torch, wl, Gs, seed, nband, ldc, nbands=None):
+    """Gaussian-damped random coefficients (as gaussian_coefficients) drawn
+    on the GPU with a seeded torch generator: complex64 [nk, nband, ldc],
+    zero beyond each k-point's W_k.  ``nbands`` (optional, per k) zeroes the
+    bands of k beyond nbands[k]."""
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(int(seed))
+    rec = reciprocal_lattice(wl.lattice)
+    gcut = math.sqrt(C_EV * wl.encut)
+    C = torch.zeros((len(Gs), nband, ldc), dtype=torch.complex64, device="cuda")
+    for k, (kp, G) in enumerate(zip(wl.kpts, Gs)):
+        kg = (G.astype(np.float64) + np.asarray(kp)[None, :]) @ rec
+        env = np.exp(-np.einsum("ij,ij->i", kg, kg) / (2.0 * (0.5 * gcut) ** 2)).astype(np.float32)
+        envt = torch.from_numpy(env).cuda()
+        nb = nband if nbands is None else int(nbands[k])
+        re = torch.randn((nb, len(G)), generator=gen, device="cuda")
+        im = torch.randn((nb, len(G)), generator=gen, device="cuda")
+        c = torch.complex(re * envt, im * envt)
+        c /= torch.linalg.vector_norm(c, dim=1, keepdim=True)
+        C[k, :nb, : len(G)] = c
+        del re, im, c
+    return C
+
+
+#: overlap magnitudes of the planted near-orthogonal band pairs
+NEAR_ORTHOGONAL_EPS = (1e-8, 1e-7, 1e-6, 1e-5, 1e-4, 1e-3, 1e-2, 1e-1, 1.0)
+
+
+def plant_near_orthogonal(torch, CR, CS, nw, pairs, eps=NEAR_ORTHOGONAL_EPS):
+    """Near-orthogonal band pairs of the precision workload: for each
+    (k, r, s) of ``pairs`` and magnitude e of ``eps`` (cycled), S band s at k
+    becomes (d + e R_r) / |d + e R_r| with d its old coefficients made
+    orthogonal to R band r (Gram-Schmidt in FP64) and normalised, so that
+    <R_r | S_s> = e / sqrt(1 + e^2) before the rounding to complex64.
+    CS is changed in place; returns the list (k, r, s, e)."""
+    out = []
+    for i, (k, r, s) in enumerate(pairs):
+        e = float(eps[i % len(eps)])
+        W = int(nw[k])
+        rv = CR[k, r, :W].to(torch.complex128)
+        d = CS[k, s, :W].to(torch.complex128)
+        rv = rv / torch.linalg.vector_norm(rv)
+        for _ in range(2):  # twice, so the residual overlap is at FP64 rounding
+            d = d - torch.vdot(rv, d) * rv
+        d = d / torch.linalg.vector_norm(d)
+        v = d + e * rv
+        v = v / torch.linalg.vector_norm(v)
+        CS[k, s, :W] = v.to(torch.complex64)
+        out.append((int(k), int(r), int(s), e))
+    return out

@@ -372,3 +372,70 @@ def test_noncollinear_reference_wavecar(gpu, tmp_path, wavecar_ncl):
             eu, ed = normwise(got[0], PU[k, b]), normwise(got[1], PD[k, b])
             assert eu < 1e-9 and ed < 1e-9, (k, b, eu, ed)
     print("reference spinor WAVECAR: projections of", nk, "k x", Cs[0].shape[0], "bands match", flush=True)
+
+
+def test_projector_own_pwave_overlaps(gpu):
+    """O_M with the partial-wave overlap matrices get_projector_list
+    integrates itself (make_pwave_overlap_matrices, projector.c:507-558), no
+    channel matrix put in their place: against the oracle's O_M with
+    dO = aeov - psov from or_pwave_overlaps (pinned against scipy)."""
+    from pawpyseed_amd.core import Projector, wavefunction_from_workload
+    wl, CR, CS = _pair(nband=6)
+    basis = wavefunction_from_workload(wl, "R", coeffs=CR, paw_overlap=False)
+    wf = wavefunction_from_workload(wl, "S", coeffs=CS, paw_overlap=False)
+    sp = wl.species
+    psov, aeov, _ = orc.pwave_overlaps(sp.grid, sp.ls, sp.aewaves, sp.pswaves)
+    dO = aeov - psov
+    assert np.abs(dO).max() > 1e-3  # the synthetic partial waves differ
+    nwk = len(wl.kpts)
+    PR, TR = oracle_projections(wl, wl.frac_R, CR)
+    PS, TS = oracle_projections(wl, wl.frac_S, CS)
+    pr = Projector(wf, basis, site_lists=(wl.M_R, wl.M_S))
+    for band in (1, 4):
+        got = pr.single_band_projection(band)
+        want = orc.pseudoprojection(CR, CS, band, nwk, 1)
+        want = orc.compensation_terms_m(want, band, nwk, 1, PR, TR, np.zeros(len(wl.frac_R), np.int32), PS[:, band],
+                                        TS, wl.M_R, wl.M_S, [dO], [sp.ls])
+        with_dO = orc.compensation_terms_m(orc.pseudoprojection(CR, CS, band, nwk, 1), band, nwk, 1, PR, TR,
+                                           np.zeros(len(wl.frac_R), np.int32), PS[:, band], TS, wl.M_R, wl.M_S,
+                                           [wl.delta_o], [sp.ls])
+        assert normwise(with_dO, want) > 1e-4  # the workload's channel matrix would not pass
+        err = normwise(got, want)
+        print("own partial-wave overlaps, band", band, "normwise", err, flush=True)
+        assert err < 1e-9
+
+
+@pytest.mark.parametrize("method", ["aug_real", "pseudo"])
+def test_proportion_conduction_with_occupations(gpu, method):
+    """The valence/conduction split (projector.py:384-435) of GPU
+    projections with fractional and 0/1 basis occupations, spin-resolved and
+    not, against the oracle's restatement of the loops on the same
+    projections, which are themselves checked against the oracle."""
+    from pawpyseed_amd.core import Projector, wavefunction_from_workload
+    wl, CR, CS = _pair(nband=8)
+    nwk = len(wl.kpts)
+    rng = np.random.default_rng(17)
+    occ = np.clip(rng.uniform(-0.3, 1.3, (nwk, 8)), 0.0, 1.0)
+    occ[:, :3] = 1.0
+    basis = wavefunction_from_workload(wl, "R", coeffs=CR, occs=occ)
+    wf = wavefunction_from_workload(wl, "S", coeffs=CS, occs=occ[:, ::-1].copy())
+    pr = Projector(wf, basis, method=method, **({} if method == "pseudo" else dict(site_lists=(wl.M_R, wl.M_S))))
+    occs_basis = basis._get_occs()
+    assert np.allclose(occs_basis, occ.T.ravel())
+    for band in (0, 6):
+        res = pr.single_band_projection(band)
+        want_res = orc.pseudoprojection(CR, CS, band, nwk, 1)
+        if method != "pseudo":
+            want_res, _, _ = _oracle_single_band(wl, CR, CS, band)
+        assert normwise(res, want_res) < 1e-9
+        for spinpol in (False, True):
+            v, c = pr.proportion_conduction(band, spinpol=spinpol)
+            wv, wc = orc.proportion_conduction(res, occs_basis, wf.kws, 8, nwk, 1, spinpol=spinpol,
+                                               pseudo=method == "pseudo")
+            assert np.allclose(v, wv, rtol=1e-12, atol=1e-15) and np.allclose(c, wc, rtol=1e-12, atol=1e-15)
+            assert np.all(np.asarray(v) > 0) and np.all(np.asarray(c) > 0)
+    out = pr.defect_band_analysis(num_below_ef=1, num_above_ef=2)
+    # the projected structure's last band filled at its first k: occ reversed, so band 7 - 0 ... wf bands 5..7 full
+    wf_occ = wf._get_occs().reshape(8, -1)[:, 0]
+    vbm = int(np.flatnonzero(wf_occ > 0.5)[-1])
+    assert sorted(out) == list(range(max(vbm - 1, 0), min(vbm + 2, 7) + 1))

@@ -101,3 +101,42 @@ def test_autotune_keeps_results(ctx):
         for n, v in before.items():
             ctx.set_param(n, v)
         plan.close()
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-9), ("fp32", 1e-5)])
+def test_graph_replay_after_other_inputs(ctx, precision, tol):
+    """A replay after eager calls on the same context with other k-points,
+    another plane-wave set and another plan still gives the captured
+    inputs' projections: the graph keeps its own grid indices, phases,
+    bounds and line lists."""
+    from pawpyseed_amd.core import batched as B
+    wl, Gs, Cs, want = _case(nband=40)
+    Cd, Gd, gofs, nw = device_inputs(Gs, Cs)
+    labels = np.zeros(len(wl.frac_S), np.int32)
+    plan = B.SpherePlan(ctx, wl.lattice, wl.fftg, wl.frac_S, labels, [wl.species], precision)
+    out = torch.zeros((len(Gs), 40, plan.ptot), dtype=torch.complex128, device="cuda")
+    g = B.SetupGraph(ctx, plan, wl.kpts, gofs, Gd, Cd, out, precision)
+    torch.cuda.synchronize()
+    first = out.clone()
+    assert normwise(first.cpu().numpy(), want) < tol
+    # other inputs on the same context: a lower cutoff (fewer plane waves),
+    # other k-points, the R sites, fewer bands
+    wl2 = synthetic.tiny_config(encut=110.0, fftg=(24, 24, 24), nband=24, kpts=[[0.1, 0.2, -0.3], [0.5, 0, 0]],
+                                ls=(0, 1, 2), alphas=(1.5, 3.0, 1.5))
+    Gs2 = wl2.ensure_gvecs()
+    Cs2 = wl2.coefficients("R", nband=24)
+    Cd2, Gd2, gofs2, _ = device_inputs(Gs2, Cs2)
+    plan2 = B.SpherePlan(ctx, wl2.lattice, wl2.fftg, wl2.frac_R, labels, [wl2.species], precision)
+    other = torch.zeros((len(Gs2), 24, plan2.ptot), dtype=torch.complex128, device="cuda")
+    B.setup_projections(ctx, plan2, wl2.kpts, gofs2, Gd2, Cd2, other, precision)
+    T2 = orc.site_tables(wl2.lattice, wl2.fftg, wl2.frac_R, [wl2.species], labels)
+    want2 = orc.setup_projections(wl2.lattice, wl2.fftg, wl2.kpts, Gs2, Cs2, T2)
+    assert normwise(other.cpu().numpy(), want2) < tol
+    out.zero_()
+    torch.cuda.synchronize()
+    g.replay()
+    print("replay after other inputs", precision, flush=True)
+    assert torch.equal(out, first)
+    g.close()
+    plan.close()
+    plan2.close()
