@@ -266,7 +266,8 @@ int pp_ctx_set_tuning(pp_ctx *ctx, int fft_bands, int fft_tile, int fft_prune,
  * else 0)), "rs_unroll" (1, 2 or 4,
  * default 4: sphere points of a wavefront in flight in the real-space
  * augmentation), "rs_prune" (1, default: the real-space FFT passes skip
- * lines outside the plane-wave sphere). */
+ * lines outside the plane-wave sphere), "multi_items" (1..64, default 8:
+ * work items per launch of pp_setup_projections_multi). */
 int pp_ctx_set_param(pp_ctx *ctx, const char *name, int value);
 /* Current value of a parameter of pp_ctx_set_param (0 = the built-in
  * default for fft_tile, fft_bands, sphere_parts, gemm_splitk). */
@@ -358,6 +359,39 @@ int pp_setup_projections_batched(pp_ctx *ctx, pp_sphere *sp, int nk,
                                  const double *kfrac, const int *gofs,
                                  const int *G_dev, const void *C_dev, long ldc,
                                  int nband, void *proj_dev, int precision);
+
+/* Several structures sharing one plane-wave basis, set up as one work list
+ * (the setup_projections of every Wavefunction that the reference's
+ * setup_multiple_projections generator yields, projector.py:311-382, each
+ * projector.c:560-602): nstruct plans (same FFT grid and precision); the
+ * basis of nk k-points as in pp_setup_projections_batched (kfrac, gofs,
+ * G_dev); per (structure s, k-point k), index i = s * nk + k, the band
+ * count nband[i], coefficients C_dev[i] (device c64 [nband[i]][ldc[i]])
+ * and projections proj_dev[i] (device c128 [nband[i]][total projectors of
+ * plan s]).  Every (structure, k, 64-band chunk) is a work item; a batch of
+ * "multi_items" items (tuning parameter, default 8) runs as one launch per
+ * kernel, batches alternating over two streams.  One host synchronisation
+ * (the plane-wave bounds of every k); asynchronous afterwards.  Inputs the
+ * work-item kernels do not cover (a grid that aliases a G vector, a G set
+ * with holes) go through the single-structure path instead. */
+int pp_setup_projections_multi(pp_ctx *ctx, int nstruct, pp_sphere *const *plans, int nk,
+                               const double *kfrac, const int *gofs, const int *G_dev,
+                               const int *nband, const void *const *C_dev, const long *ldc,
+                               void *const *proj_dev, int precision);
+/* The same as one HIP graph (owning its prepared data and chunk buffers);
+ * replay with pp_graph_replay, which refuses once a plan was rebuilt. */
+int pp_graph_capture_multi(pp_ctx *ctx, int nstruct, pp_sphere *const *plans, int nk,
+                           const double *kfrac, const int *gofs, const int *G_dev,
+                           const int *nband, const void *const *C_dev, const long *ldc,
+                           void *const *proj_dev, int precision, pp_graph **out);
+/* Autotuner of pp_setup_projections_multi (as pp_autotune) over
+ * multi_items, streams, fft_tile, sphere_unroll, sphere_parts, fft_nt,
+ * sphere_nt and fft_xmask. */
+int pp_autotune_multi(pp_ctx *ctx, int nstruct, pp_sphere *const *plans, int nk,
+                      const double *kfrac, const int *gofs, const int *G_dev,
+                      const int *nband, const void *const *C_dev, const long *ldc,
+                      void *const *proj_dev, int precision, int repeats, char *report,
+                      int report_len);
 
 /* Band overlaps of every k as one complex GEMM each:
  * out[k][r][s] = sum_G conj(CR[kR(k)][r][G]) CS[k][s][G], G < nw[k],

@@ -118,6 +118,15 @@ SIGNATURES = {
                                                 c_void_p, ct.c_long, ct.c_int, c_void_p, ct.c_int]),
     "pp_overlap_gemm": (ct.c_int, [c_void_p, ct.c_int, ct.c_int, ct.c_int, c_int_p, c_void_p, ct.c_long, ct.c_int,
                                    c_void_p, ct.c_long, ct.c_int, c_void_p, ct.c_int]),
+    "pp_setup_projections_multi": (ct.c_int, [c_void_p, ct.c_int, c_void_p, ct.c_int, c_double_p, c_int_p,
+                                              c_void_p, c_int_p, c_void_p, ct.POINTER(ct.c_long), c_void_p,
+                                              ct.c_int]),
+    "pp_graph_capture_multi": (ct.c_int, [c_void_p, ct.c_int, c_void_p, ct.c_int, c_double_p, c_int_p, c_void_p,
+                                          c_int_p, c_void_p, ct.POINTER(ct.c_long), c_void_p, ct.c_int,
+                                          ct.POINTER(c_void_p)]),
+    "pp_autotune_multi": (ct.c_int, [c_void_p, ct.c_int, c_void_p, ct.c_int, c_double_p, c_int_p, c_void_p,
+                                     c_int_p, c_void_p, ct.POINTER(ct.c_long), c_void_p, ct.c_int, ct.c_int,
+                                     ct.c_char_p, ct.c_int]),
     "pp_overlap_gemm_noalloc": (ct.c_int, [c_void_p, ct.c_int, ct.c_int, ct.c_int, c_int_p, c_void_p, ct.c_long,
                                            ct.c_int, c_void_p, ct.c_long, ct.c_int, c_void_p, ct.c_int]),
     "pp_noalloc_scratch_bytes": (ct.c_size_t, []),

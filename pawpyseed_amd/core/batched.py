@@ -387,3 +387,98 @@ def realspace_states(ctx: Context, plan: RealspacePlan, kfrac, G_dev, C_dev, pro
                                      _dptr(density_out) if density_out is not None else None,
                                      _prec(precision)), "pp_realspace_batched")
     _done(ctx, sync)
+
+
+def _multi_arrays(plans, C, proj):
+    """Pointer, stride and band-count arrays of the multi-structure entry
+    points: C[s][k] complex64 [nband_sk, ldc_sk] and proj[s][k] complex128
+    [nband_sk, ptot_s] (cuda tensors, rows contiguous; views into larger
+    tensors allowed)."""
+    ns, nk = len(plans), len(C[0])
+    assert len(C) == ns and len(proj) == ns and all(len(c) == nk and len(p) == nk for c, p in zip(C, proj))
+    nb = np.zeros(ns * nk, np.int32)
+    ld = np.zeros(ns * nk, np.int64)
+    cp = (ct.c_void_p * (ns * nk))()
+    pp = (ct.c_void_p * (ns * nk))()
+    for s in range(ns):
+        for k in range(nk):
+            c, p = C[s][k], proj[s][k]
+            assert str(c.dtype) == "torch.complex64" and str(p.dtype) == "torch.complex128"
+            assert c.shape[0] == p.shape[0] and p.shape[1] == plans[s].ptot
+            assert c.stride(1) == 1 and p.stride(1) == 1 and p.stride(0) == plans[s].ptot
+            i = s * nk + k
+            nb[i] = c.shape[0]
+            ld[i] = c.stride(0) if c.shape[0] > 1 else max(c.shape[1], 1)
+            cp[i] = c.data_ptr()
+            pp[i] = p.data_ptr()
+    hp = (ct.c_void_p * ns)(*[p.h.value for p in plans])
+    return hp, nb, ld, cp, pp
+
+
+def setup_projections_multi(ctx: Context, plans, kfrac, gofs, G_dev, C, proj, precision="fp64", sync=True):
+    """<p|psi> of several structures sharing one plane-wave basis, every
+    (structure, k, 64-band chunk) as one work list (pp_setup_projections_multi):
+    plans[s] the sphere plans; kfrac [nk, 3], gofs [nk + 1] and G_dev int32
+    [sum W, 3] the basis; C[s][k] complex64 [nband_sk, ldc] and proj[s][k]
+    complex128 [nband_sk, ptot_s] per structure and k-point."""
+    hp, nb, ld, cp, pp = _multi_arrays(plans, C, proj)
+    kf = as_f64(kfrac).reshape(-1, 3)
+    go = as_i32(gofs)
+    check(lib().pp_setup_projections_multi(ctx.h, len(plans), hp, len(kf), ptr(kf, c_double_p), ptr(go, c_int_p),
+                                           _dptr(G_dev), ptr(nb, c_int_p), cp,
+                                           ld.ctypes.data_as(ct.POINTER(ct.c_long)), pp, _prec(precision)),
+          "pp_setup_projections_multi")
+    _done(ctx, sync)
+
+
+class MultiGraph:
+    """pp_graph_capture_multi: the multi-structure setup of fixed buffers
+    captured once (after one eager run), replayed on the context's stream."""
+
+    def __init__(self, ctx: Context, plans, kfrac, gofs, G_dev, C, proj, precision="fp64"):
+        hp, nb, ld, cp, pp = _multi_arrays(plans, C, proj)
+        self._kf = as_f64(kfrac).reshape(-1, 3)
+        self._go = as_i32(gofs)
+        self._keep = (plans, G_dev, C, proj, hp, nb, ld, cp, pp)
+        self.ctx = ctx
+        h = ct.c_void_p()
+        check(lib().pp_graph_capture_multi(ctx.h, len(plans), hp, len(self._kf), ptr(self._kf, c_double_p),
+                                           ptr(self._go, c_int_p), _dptr(G_dev), ptr(nb, c_int_p), cp,
+                                           ld.ctypes.data_as(ct.POINTER(ct.c_long)), pp, _prec(precision),
+                                           ct.byref(h)), "pp_graph_capture_multi")
+        self.h = h
+
+    def replay(self, sync=True):
+        check(lib().pp_graph_replay(self.h), "pp_graph_replay")
+        _done(self.ctx, sync)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pp_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+TUNED_MULTI = ("multi_items", "streams", "fft_tile", "sphere_unroll", "sphere_parts", "fft_nt", "sphere_nt",
+               "fft_xmask")
+
+
+def autotune_multi(ctx: Context, plans, kfrac, gofs, G_dev, C, proj, precision="fp64", repeats=2):
+    """pp_autotune_multi on these inputs; returns (settings, trials) as autotune."""
+    hp, nb, ld, cp, pp = _multi_arrays(plans, C, proj)
+    kf = as_f64(kfrac).reshape(-1, 3)
+    go = as_i32(gofs)
+    rep = ct.create_string_buffer(8192)
+    check(lib().pp_autotune_multi(ctx.h, len(plans), hp, len(kf), ptr(kf, c_double_p), ptr(go, c_int_p),
+                                  _dptr(G_dev), ptr(nb, c_int_p), cp, ld.ctypes.data_as(ct.POINTER(ct.c_long)), pp,
+                                  _prec(precision), int(repeats), rep, len(rep)), "pp_autotune_multi")
+    trials = []
+    for line in rep.value.decode().splitlines():
+        n, v, ms = line.split()
+        trials.append((n, int(v), float(ms)))
+    return {n: ctx.get_param(n) for n in TUNED_MULTI}, trials

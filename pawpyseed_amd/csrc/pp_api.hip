@@ -1,5 +1,6 @@
 // Context, errors, timing and the batched device-resident entry points.
 #include <cstring>
+#include <functional>
 #include <mutex>
 
 #include "pp_internal.h"
@@ -527,6 +528,8 @@ int pp_ctx_set_param(pp_ctx *ctx, const char *name, int value) {
     ctx->tune_rs_prune = value != 0 ? 1 : 0;
   else if (n == "rs_unroll")
     ctx->tune_rs_unroll = value >= 4 ? 4 : value >= 2 ? 2 : 1;
+  else if (n == "multi_items")
+    ctx->tune_multi_items = value < 1 ? 1 : value > 64 ? 64 : value;
   else if (n == "fft_tile")
     return pp_ctx_set_tuning(ctx, ctx->tune_fft_bands, value, ctx->tune_fft_prune, ctx->tune_sphere_parts,
                              ctx->tune_splitk);
@@ -580,7 +583,7 @@ int pp_ctx_get_param(pp_ctx *ctx, const char *name, int *value) {
       {"fft_xmask", ctx->tune_fft_xmask}, {"fft_nt", ctx->tune_fft_nt},
       {"sphere_nt", ctx->tune_sphere_nt},
       {"fused", ctx->tune_fused}, {"rs_kernel", ctx->tune_rs_kernel}, {"rs_prune", ctx->tune_rs_prune},
-      {"rs_unroll", ctx->tune_rs_unroll},
+      {"rs_unroll", ctx->tune_rs_unroll}, {"multi_items", ctx->tune_multi_items},
       {"fft_tile", ctx->tune_fft_tile}, {"fft_prune", ctx->tune_fft_prune}, {"fft_bands", ctx->tune_fft_bands},
       {"sphere_parts", ctx->tune_sphere_parts}, {"gemm_splitk", ctx->tune_splitk}};
   for (const auto &e : table)
@@ -592,38 +595,21 @@ int pp_ctx_get_param(pp_ctx *ctx, const char *name, int *value) {
   return PP_OK;
 }
 
-// Project-written autotuner of the setup path: coordinate descent over the
-// context's tuning parameters, each candidate timed on the caller's own
-// inputs (HIP events around the whole setup on the context's stream, best
-// of `repeats` runs after one warm-up); the fastest value of each
-// parameter is kept before the next parameter is tried.  The outputs of
-// the last run are valid projections.  report (may be NULL) receives one
-// line per candidate: "name value ms".
-int pp_autotune(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac, const int *gofs, const int *G_dev,
-                const void *C_dev, long ldc, int nband, void *proj_dev, int precision, int repeats, char *report,
-                int report_len) {
-  PP_REQUIRE(ctx && sp && repeats >= 1, PP_ERR_ARG, "pp_autotune: bad arguments");
-  PP_CHECK_HIP(hipSetDevice(ctx->device));
-  const bool fp32 = precision != PP_FP64;
-  struct Cand {
-    const char *name;
-    std::vector<int> values;
-  };
-  std::vector<Cand> cands = {
-      {"streams", {1, 2}},
-      {"fft_tile", fp32 ? std::vector<int>{16, 32} : std::vector<int>{8, 16}},
-      {"fft_threads", {256, 512}},
-      {"sphere_unroll", {4, 8}},
-      {"sphere_parts", {0, 4, 16}},
-      {"fft_radix", {0, 1}},
-      {"fft_xcd", {0, 1}},
-      {"fft_xmask", {0, 1}},
-      {"fft_nt", {0, 1, 2, 3}},
-      {"sphere_nt", {0, 1}},
-      {"fft_bands", {64, 32}},
-  };
-  if (fp32)
-    cands.push_back({"fused", {0, 1}});
+}  // extern "C"
+
+namespace pp {
+// Coordinate descent over tuning parameters: each candidate value is timed
+// with HIP events around run (the caller's own inputs, on the context's
+// stream), best of `repeats` after one warm-up run, and the fastest value
+// of a parameter is kept before the next parameter is tried.  A candidate
+// replaces the value in place only when it is faster by more than the
+// run-to-run spread (2%), so that equal settings keep the library default.
+// On failure the parameter being tried gets its previous value back.  A
+// final run leaves the outputs of the chosen setting.  report (may be
+// NULL) receives one line per candidate: "name value ms".
+int autotune_core(pp_ctx *ctx, const std::vector<TuneCand> &cands, const std::function<int()> &run, int repeats,
+                  char *report, int report_len) {
+  PP_REQUIRE(repeats >= 1, PP_ERR_ARG, "autotune: repeats must be >= 1");
   const bool timing = ctx->timer.on;
   ctx->timer.on = false;
   hipEvent_t e0, e1;
@@ -632,7 +618,7 @@ int pp_autotune(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac, const i
   std::string rep;
   auto run_once = [&](float &ms) -> int {
     PP_CHECK_HIP(hipEventRecord(e0, ctx->stream));
-    PP_TRY(pp_setup_projections_batched(ctx, sp, nk, kfrac, gofs, G_dev, C_dev, ldc, nband, proj_dev, precision));
+    PP_TRY(run());
     PP_CHECK_HIP(hipEventRecord(e1, ctx->stream));
     PP_CHECK_HIP(hipEventSynchronize(e1));
     PP_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
@@ -649,7 +635,7 @@ int pp_autotune(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac, const i
     return PP_OK;
   };
   int st = PP_OK;
-  for (const Cand &c : cands) {
+  for (const TuneCand &c : cands) {
     int cur = 0;
     pp_ctx_get_param(ctx, c.name, &cur);
     int best_v = cur;
@@ -661,9 +647,6 @@ int pp_autotune(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac, const i
       if ((st = trial(ms)) != PP_OK)
         break;
       rep += std::string(c.name) + " " + std::to_string(v) + " " + std::to_string(ms) + "\n";
-      // a candidate replaces the value in place only when it is faster by
-      // more than the run-to-run spread (2%), so that equal settings keep
-      // the library default and timed runs stay comparable
       const bool incumbent = v == cur;
       if (best_ms == 1e30f || (incumbent ? ms < best_ms * 1.02f : ms * 1.02f < best_ms)) {
         best_ms = ms;
@@ -671,7 +654,6 @@ int pp_autotune(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac, const i
       }
     }
     if (st != PP_OK) {
-      // leave the context as it was before this parameter's trials
       const std::string msg = pp_last_error();
       pp_ctx_set_param(ctx, c.name, cur);
       set_error(st, msg);
@@ -681,7 +663,7 @@ int pp_autotune(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac, const i
   }
   if (st == PP_OK) {
     float ms = 0.f;
-    st = run_once(ms);  // projections of the chosen setting
+    st = run_once(ms);  // outputs of the chosen setting
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -693,26 +675,43 @@ int pp_autotune(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac, const i
   }
   return st;
 }
+}  // namespace pp
+
+extern "C" {
+
+// Project-written autotuner of the setup path over streams, fft_tile,
+// fft_threads, sphere_unroll, sphere_parts, fft_radix, fft_xcd, fft_xmask,
+// fft_nt, sphere_nt, fft_bands and (FP32) fused, on the caller's inputs.
+int pp_autotune(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac, const int *gofs, const int *G_dev,
+                const void *C_dev, long ldc, int nband, void *proj_dev, int precision, int repeats, char *report,
+                int report_len) {
+  PP_REQUIRE(ctx && sp && repeats >= 1, PP_ERR_ARG, "pp_autotune: bad arguments");
+  PP_CHECK_HIP(hipSetDevice(ctx->device));
+  const bool fp32 = precision != PP_FP64;
+  std::vector<TuneCand> cands = {
+      {"streams", {1, 2}},
+      {"fft_tile", fp32 ? std::vector<int>{16, 32} : std::vector<int>{8, 16}},
+      {"fft_threads", {256, 512}},
+      {"sphere_unroll", {4, 8}},
+      {"sphere_parts", {0, 4, 16}},
+      {"fft_radix", {0, 1}},
+      {"fft_xcd", {0, 1}},
+      {"fft_xmask", {0, 1}},
+      {"fft_nt", {0, 1, 2, 3}},
+      {"sphere_nt", {0, 1}},
+      {"fft_bands", {64, 32}},
+  };
+  if (fp32)
+    cands.push_back({"fused", {0, 1}});
+  return autotune_core(
+      ctx, cands,
+      [&]() {
+        return pp_setup_projections_batched(ctx, sp, nk, kfrac, gofs, G_dev, C_dev, ldc, nband, proj_dev, precision);
+      },
+      repeats, report, report_len);
+}
 
 }  // extern "C"
-
-// A captured setup pipeline: the graph, and what it was captured against
-// (the plan's build generation, the context buffers it reads and writes).
-// The graph owns the buffers setup_prepare fills from the call's G vectors
-// and k-points (grid indices, Bloch phases, plane-wave bounds, line lists):
-// eager calls with other inputs on the same context leave them alone.  The
-// context's chunk grids and partial sums are scratch whose contents no
-// launch carries over; only their addresses must stay those of the capture.
-struct pp_graph {
-  pp_ctx *ctx = nullptr;
-  pp_sphere *sp = nullptr;
-  long sp_gen = 0;
-  std::vector<std::pair<void *, size_t>> bufs;
-  DevBuf gidx, phase, bounds, lines;
-  std::vector<int> lines_host, bounds_host;
-  hipGraph_t g = nullptr;
-  hipGraphExec_t ge = nullptr;
-};
 
 static std::vector<std::pair<void *, size_t>> ctx_buffers(pp_ctx *ctx) {
   std::vector<std::pair<void *, size_t>> v;
@@ -749,8 +748,7 @@ int pp_graph_capture(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac, co
   PP_TRY(pp_setup_projections_batched(ctx, sp, nk, kfrac, gofs, G_dev, C_dev, ldc, nband, proj_dev, precision));
   pp_graph *gr = new pp_graph();
   gr->ctx = ctx;
-  gr->sp = sp;
-  gr->sp_gen = sphere_generation(sp);
+  gr->plans.push_back({sp, sphere_generation(sp)});
   const bool timing = ctx->timer.on;
   ctx->timer.on = false;
   int st = precision == PP_FP64
@@ -770,9 +768,10 @@ int pp_graph_capture(pp_ctx *ctx, pp_sphere *sp, int nk, const double *kfrac, co
 
 int pp_graph_replay(pp_graph *gr) {
   PP_REQUIRE(gr && gr->ge, PP_ERR_ARG, "pp_graph_replay: null graph");
-  PP_REQUIRE(sphere_generation(gr->sp) == gr->sp_gen, PP_ERR_ARG,
-             "pp_graph_replay: the sphere plan was rebuilt since the capture");
-  PP_REQUIRE(ctx_buffers(gr->ctx) == gr->bufs, PP_ERR_ARG,
+  for (const auto &pg : gr->plans)
+    PP_REQUIRE(sphere_generation(pg.first) == pg.second, PP_ERR_ARG,
+               "pp_graph_replay: a sphere plan was rebuilt since the capture");
+  PP_REQUIRE(gr->multi || ctx_buffers(gr->ctx) == gr->bufs, PP_ERR_ARG,
              "pp_graph_replay: context buffers were reallocated since the capture");
   PP_CHECK_HIP(hipSetDevice(gr->ctx->device));
   PP_CHECK_HIP(hipGraphLaunch(gr->ge, gr->ctx->stream));

@@ -164,10 +164,9 @@ __global__ __launch_bounds__(256) void k_gidx(int n0, int n1, int n2, int nw, co
 // Scatter a 32-coefficient x NB-band tile through LDS: coalesced reads
 // along G of each band row, contiguous NB-band writes per grid point.
 template <typename T>
-__global__ __launch_bounds__(256) void k_scatter(cplx<T> *__restrict__ g, int NB, const int *__restrict__ gidx,
-                                                 int nw, const cplx<float> *__restrict__ C, long ldc, int nbc) {
+__device__ __forceinline__ void scatter_tile(cplx<T> *__restrict__ g, int NB, const int *__restrict__ gidx, int nw,
+                                             const cplx<float> *__restrict__ C, long ldc, int nbc, int g0) {
   __shared__ cplx<T> tile[64][33];
-  int g0 = blockIdx.x * 32;
   for (int e = threadIdx.x; e < 64 * 32; e += 256) {
     int b = e >> 5, gg = e & 31;
     cplx<T> v = mk<T>(0, 0);
@@ -181,6 +180,20 @@ __global__ __launch_bounds__(256) void k_scatter(cplx<T> *__restrict__ g, int NB
     if (b < nbc && b < NB && g0 + gg < nw)
       g[(long)gidx[g0 + gg] * NB + b] = tile[b][gg];
   }
+}
+template <typename T>
+__global__ __launch_bounds__(256) void k_scatter(cplx<T> *__restrict__ g, int NB, const int *__restrict__ gidx,
+                                                 int nw, const cplx<float> *__restrict__ C, long ldc, int nbc) {
+  scatter_tile<T>(g, NB, gidx, nw, C, ldc, nbc, blockIdx.x * 32);
+}
+// The scatter of a batch of work items (one per blockIdx.y).
+template <typename T>
+__global__ __launch_bounds__(256) void k_scatter_items(const ChunkItem *__restrict__ items, int NB) {
+  const ChunkItem &it = items[blockIdx.y];
+  const int g0 = blockIdx.x * 32;
+  if (g0 >= it.nw)
+    return;
+  scatter_tile<T>((cplx<T> *)it.grid, NB, it.gidx, it.nw, it.C, it.ldc, it.nbc, g0);
 }
 
 // One Stockham stage of radix R over a line tile in LDS:
@@ -325,17 +338,15 @@ __global__ __launch_bounds__(256) void k_fft_pass(cplx<T> *__restrict__ g, AxisP
 //
 // streaming: 1 = the line is written back with non-temporal stores, 2 =
 // also read with non-temporal loads (each pass touches its data once).
-template <typename T, int NBW, int AXIS, int MAXB, int NT, bool COMP = false>
-__global__ __launch_bounds__(NT, (FftWaves<T, MAXB, COMP>::value)) void k_fft_pass_ip(cplx<T> *__restrict__ g, AxisPlan pl,
-                                                     const cplx<T> *__restrict__ tw, long es, long s1, long s2,
-                                                     int n2, int nslices, SphereBounds bd, int xcd_per, int nblk,
-                                                     int streaming) {
+// The work of one work-group of the in-place pass: tile blk (line
+// blk / nslices, band slice blk % nslices) of the chunk g.
+template <typename T, int NBW, int AXIS, int MAXB, int NT, bool COMP>
+__device__ __forceinline__ void fft_pass_ip_tile(cplx<T> *__restrict__ g, const AxisPlan &pl,
+                                                 const cplx<T> *__restrict__ tw, long es, long s1, long s2, int n2,
+                                                 int nslices, const SphereBounds &bd, int blk, int streaming) {
   extern __shared__ unsigned char smem_raw[];
   cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
   const int n = pl.n;
-  const int blk = xcd_per > 0 ? (int)(blockIdx.x % 8) * xcd_per + (int)(blockIdx.x / 8) : (int)blockIdx.x;
-  if (blk >= nblk)
-    return;
   const int slice = blk % nslices;
   const int line = pass_line<AXIS>(bd, blk / nslices);
   const int l1 = line / n2, l2 = line % n2;
@@ -453,6 +464,35 @@ __global__ __launch_bounds__(NT, (FftWaves<T, MAXB, COMP>::value)) void k_fft_pa
         g[base + (long)e * es + b] = v[u];
     }
   }
+}
+
+template <typename T, int NBW, int AXIS, int MAXB, int NT, bool COMP = false>
+__global__ __launch_bounds__(NT, (FftWaves<T, MAXB, COMP>::value)) void k_fft_pass_ip(cplx<T> *__restrict__ g, AxisPlan pl,
+                                                     const cplx<T> *__restrict__ tw, long es, long s1, long s2,
+                                                     int n2, int nslices, SphereBounds bd, int xcd_per, int nblk,
+                                                     int streaming) {
+  const int blk = xcd_per > 0 ? (int)(blockIdx.x % 8) * xcd_per + (int)(blockIdx.x / 8) : (int)blockIdx.x;
+  if (blk >= nblk)
+    return;
+  fft_pass_ip_tile<T, NBW, AXIS, MAXB, NT, COMP>(g, pl, tw, es, s1, s2, n2, nslices, bd, blk, streaming);
+}
+
+// The same pass over the chunks of a batch of work items (one per
+// blockIdx.y): each item brings its own chunk buffer and the plane-wave
+// bounds and line lists of its k-point; tiles past the item's lines exit.
+template <typename T, int NBW, int AXIS, int MAXB, int NT>
+__global__ __launch_bounds__(NT, (FftWaves<T, MAXB, false>::value)) void k_fft_pass_ip_items(
+    const ChunkItem *__restrict__ items, AxisPlan pl, const cplx<T> *__restrict__ tw, long es, long s1, long s2,
+    int n2, int nslices, int streaming) {
+  const ChunkItem &it = items[blockIdx.y];
+  const SphereBounds &bd = it.bd;
+  const long nlin = AXIS == 2 ? (bd.zl ? bd.nzl : (long)bd.n0 * bd.n1)
+                              : AXIS == 1 ? (bd.yl ? bd.nyl : (long)bd.n0 * bd.n2) : (long)bd.n1 * bd.n2;
+  const int blk = (int)blockIdx.x;
+  if (blk >= nlin * nslices)
+    return;
+  fft_pass_ip_tile<T, NBW, AXIS, MAXB, NT, false>((cplx<T> *)it.grid, pl, tw, es, s1, s2, n2, nslices, bd, blk,
+                                                  streaming);
 }
 
 // Line bounds of a pass (see k_fft_pass); false when the line is skipped.
@@ -801,6 +841,133 @@ template <> const cplx<float> *fft_twiddles<float>(FftPlan *pl, int axis) {
   return (const cplx<float> *)pl->tw32.p + pl->twbase[axis];
 }
 template <typename T> static const cplx<T> *tw_of(FftPlan *pl, int axis) { return fft_twiddles<T>(pl, axis); }
+
+// One pass over a batch of work items (k_fft_pass_ip_items): the in-place
+// form only; PP_ERR_UNSUPPORTED when the axis needs another form (a radix
+// above 5, more than 8 butterflies per thread, the ping-pong stages).
+template <typename T, int NBW, int AXIS>
+static int launch_pass_items(pp_ctx *ctx, const ChunkItem *items, int nitems, long maxlines, const AxisPlan &ap,
+                             const cplx<T> *tw, long es, long s1, long s2, int n2, int NB, int cls) {
+  if constexpr (NBW < 8) {
+    return PP_ERR_UNSUPPORTED;
+  } else {
+    const int nslices = NB / NBW;
+    const long nblk = maxlines * nslices;
+    if (nblk == 0 || nitems == 0)
+      return PP_OK;
+    const int nt = (NBW >= 32 && ctx->tune_fft_threads == 512) ? 512 : 256;
+    int maxb = 0;
+    bool ip = ctx->tune_fft_pingpong == 0;
+    for (int s = 0; s < ap.nst; s++) {
+      if (ap.radix[s] > 5)
+        ip = false;
+      maxb = std::max(maxb, ((ap.n / ap.radix[s]) * NBW + nt - 1) / nt);
+    }
+    if (!ip || maxb > 8)
+      return PP_ERR_UNSUPPORTED;
+    const int nt_mode = ctx->tune_fft_nt < 3 ? ctx->tune_fft_nt : (AXIS == 2 ? 0 : AXIS == 1 ? 1 : 2);
+    const size_t lds = ((size_t)ap.n * NBW + ap.twlen) * sizeof(cplx<T>);
+    dim3 grd((unsigned)nblk, (unsigned)nitems);
+    hipEvent_t ev;
+    tbegin(ctx, cls, &ev);
+#define PP_IPI(MB, NT)                                                                                           \
+  hipLaunchKernelGGL((k_fft_pass_ip_items<T, NBW, AXIS, MB, NT>), grd, dim3(NT), lds, ctx->stream, items, ap, tw, es, \
+                     s1, s2, n2, nslices, nt_mode)
+    bool done = false;
+    if constexpr (NBW >= 32) {
+      if (nt == 512) {
+        if (maxb <= 2)
+          PP_IPI(2, 512);
+        else if (maxb <= 4)
+          PP_IPI(4, 512);
+        else
+          PP_IPI(8, 512);
+        done = true;
+      }
+    }
+    if (!done) {
+      if (maxb <= 2)
+        PP_IPI(2, 256);
+      else if (maxb <= 4)
+        PP_IPI(4, 256);
+      else
+        PP_IPI(8, 256);
+    }
+#undef PP_IPI
+    tend(ctx, cls, ev);
+    PP_CHECK_HIP(hipGetLastError());
+    return PP_OK;
+  }
+}
+
+template <typename T, int AXIS>
+static int run_pass_items(pp_ctx *ctx, const ChunkItem *items, int nitems, long maxlines, const AxisPlan &ap,
+                          const cplx<T> *tw, long es, long s1, long s2, int n2, int NB, int cls) {
+  const size_t ldsmax = 64 * 1024;
+  int nbw = ctx->tune_fft_tile > 0 ? ctx->tune_fft_tile : (sizeof(T) == 4 ? 16 : 8);
+  while (nbw > 1 && (NB % nbw != 0 || (size_t)2 * ap.n * nbw * sizeof(cplx<T>) > ldsmax))
+    nbw >>= 1;
+  switch (nbw) {
+  case 32:
+    return launch_pass_items<T, 32, AXIS>(ctx, items, nitems, maxlines, ap, tw, es, s1, s2, n2, NB, cls);
+  case 16:
+    return launch_pass_items<T, 16, AXIS>(ctx, items, nitems, maxlines, ap, tw, es, s1, s2, n2, NB, cls);
+  case 8:
+    return launch_pass_items<T, 8, AXIS>(ctx, items, nitems, maxlines, ap, tw, es, s1, s2, n2, NB, cls);
+  default:
+    return PP_ERR_UNSUPPORTED;
+  }
+}
+
+// True when the item form of the passes serves this grid and chunk width.
+template <typename T> bool fft_items_supported(pp_ctx *ctx, FftPlan *pl, int NB) {
+  if (ctx->tune_fft_pingpong)
+    return false;
+  int nbw = ctx->tune_fft_tile > 0 ? ctx->tune_fft_tile : (sizeof(T) == 4 ? 16 : 8);
+  for (int a = 0; a < 3; a++) {
+    const AxisPlan &ap = pl->ax[a];
+    int w = nbw;
+    while (w > 1 && (NB % w != 0 || (size_t)2 * ap.n * w * sizeof(cplx<T>) > 64 * 1024))
+      w >>= 1;
+    if (w < 8)
+      return false;
+    const int nt = (w >= 32 && ctx->tune_fft_threads == 512) ? 512 : 256;
+    for (int s = 0; s < ap.nst; s++)
+      if (ap.radix[s] > 5 || ((ap.n / ap.radix[s]) * w + nt - 1) / nt > 8)
+        return false;
+  }
+  return true;
+}
+
+// Scatter and the three pruned passes of a batch of work items (device
+// array `items` of nitems), every item's chunk pruned to its k-point's
+// plane-wave sphere: one launch per kernel for the whole batch.  maxw,
+// maxzl and maxyl bound the plane waves, z lines and y lines of the items.
+template <typename T>
+int fft_items(pp_ctx *ctx, FftPlan *pl, const ChunkItem *items, int nitems, int NB, int maxw, long maxzl,
+              long maxyl) {
+  const long n0 = pl->fftg[0], n1 = pl->fftg[1], n2 = pl->fftg[2];
+  if (maxw > 0) {
+    hipEvent_t ev;
+    tbegin(ctx, K_SCATTER, &ev);
+    hipLaunchKernelGGL(k_scatter_items<T>, dim3((unsigned)((maxw + 31) / 32), (unsigned)nitems), dim3(256), 0,
+                       ctx->stream, items, NB);
+    tend(ctx, K_SCATTER, ev);
+    PP_CHECK_HIP(hipGetLastError());
+  }
+  PP_TRY((run_pass_items<T, 2>(ctx, items, nitems, maxzl, pl->ax[2], fft_twiddles<T>(pl, 2), NB, n1 * n2 * NB,
+                               n2 * NB, (int)n1, NB, K_FFT_Z)));
+  PP_TRY((run_pass_items<T, 1>(ctx, items, nitems, maxyl, pl->ax[1], fft_twiddles<T>(pl, 1), n2 * NB, n1 * n2 * NB,
+                               NB, (int)n2, NB, K_FFT_Y)));
+  PP_TRY((run_pass_items<T, 0>(ctx, items, nitems, n1 * n2, pl->ax[0], fft_twiddles<T>(pl, 0), n1 * n2 * NB,
+                               n2 * NB, NB, (int)n2, NB, K_FFT_X)));
+  return PP_OK;
+}
+template int fft_items<double>(pp_ctx *, FftPlan *, const ChunkItem *, int, int, int, long, long);
+template int fft_items<float>(pp_ctx *, FftPlan *, const ChunkItem *, int, int, int, long, long);
+template bool fft_items_supported<double>(pp_ctx *, FftPlan *, int);
+template bool fft_items_supported<float>(pp_ctx *, FftPlan *, int);
+
 
 template <typename T>
 int fft_chunk(pp_ctx *ctx, FftPlan *pl, cplx<T> *grid, int NB, int nw, const int *gidx_dev,

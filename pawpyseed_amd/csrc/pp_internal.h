@@ -1,5 +1,6 @@
 // Internal host-side interfaces shared by the library's translation units.
 #pragma once
+#include <functional>
 #include <map>
 #include <string>
 #include <complex>
@@ -72,6 +73,28 @@ struct SphereBounds {
   int xmw = 0;
 };
 
+// One work item of a batched setup over several structures and k-points
+// (pp_setup_projections_multi): a chunk of up to NB bands of one k-point
+// of one structure, with everything its kernels read.  The item kernels
+// take an array of these, one per blockIdx.y.
+struct ChunkItem {
+  void *grid;               // band-fastest chunk buffer of the item's slot
+  const cplx<float> *C;     // coefficients of the chunk's first band, row stride ldc
+  long ldc;
+  int nbc, nw;              // bands in the chunk, plane waves of the k-point
+  const int *gidx;          // wrapped grid index of every G of the k-point
+  SphereBounds bd;          // plane-wave sphere bounds and line lists of the k-point
+  const void *phase;        // Bloch phase of every sphere point of (structure, k-point)
+  // sphere plan of the structure
+  const int *site_pt, *site_np, *proj_off, *proj_m, *pidx;
+  const long *tab_off;
+  const void *table;
+  int nsites, nparts;
+  void *partial;            // sphere-sum partial sums of the slot
+  cplx<double> *out;        // projections of the chunk's first band, row stride ptot
+  long ptot;
+};
+
 struct FftPlan {
   int fftg[3] = {0, 0, 0};
   AxisPlan ax[3];
@@ -109,6 +132,7 @@ struct pp_ctx {
   int tune_rs_kernel = 2;       // real-space augmentation: 0 LDS coefficients, 1 registers, 2 tiled
   int tune_rs_unroll = 4;       // real-space augmentation: points of a wavefront in flight (1, 2, 4)
   int tune_rs_prune = 1;        // real-space FFT passes restricted to the plane-wave sphere's lines
+  int tune_multi_items = 8;     // work items per launch of the multi-structure setup
   hipStream_t aux = nullptr;    // second stream of the chunk pipeline (default priority)
   hipEvent_t fork = nullptr, join = nullptr;  // hand-over between the two streams
   int slot = 0;                 // chunk buffer set in use (0 or 1)
@@ -123,9 +147,31 @@ struct pp_ctx {
   pp::DevBuf aug_ws;    // gathered / dO-applied projections
   pp::DevBuf bounds;    // plane-wave sphere bounds per k
   pp::DevBuf lines;     // populated z / y pass lines per k
+  pp::DevBuf m_items, m_grids, m_partials;  // work items, chunk grids, partial sums of the multi-structure setup
   std::vector<int> lines_host, bounds_host;
   std::map<long long, pp::FftPlan *> fft_plans;
   pp::Timer timer;
+};
+
+// A captured setup pipeline: the graph, and what it was captured against
+// (the build generation of every sphere plan it reads).  The graph owns the
+// buffers the host-side preparation fills from the call's G vectors and
+// k-points (grid indices, Bloch phases, plane-wave bounds, line lists, work
+// items), so eager calls with other inputs on the same context leave them
+// alone.  A single-structure graph uses the context's chunk grids and
+// partial sums, scratch whose contents no launch carries over, so only
+// their addresses must stay those of the capture (bufs); a multi-structure
+// graph owns those too.
+struct pp_graph {
+  pp_ctx *ctx = nullptr;
+  std::vector<std::pair<pp_sphere *, long>> plans;
+  std::vector<std::pair<void *, size_t>> bufs;
+  bool multi = false;
+  pp::DevBuf gidx, phase, bounds, lines;
+  std::vector<int> lines_host, bounds_host;
+  pp::DevBuf items, grids, partials;
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ge = nullptr;
 };
 
 namespace pp {
@@ -172,6 +218,15 @@ double wigner3j_host(int j1, int j2, int j3, int m1, int m2, int m3);
 std::complex<double> offsite_overlap_host(const double *dcoord, const double *k1, const double *f1,
                                           const double *const *s1, int n1, const double *k2, const double *f2,
                                           const double *const *s2, int n2, int l1, int m1, int l2, int m2);
+
+// Autotuner (pp_api.hip): coordinate descent over the named parameters,
+// timing run() on the context's stream.
+struct TuneCand {
+  const char *name;
+  std::vector<int> values;
+};
+int autotune_core(pp_ctx *ctx, const std::vector<TuneCand> &cands, const std::function<int()> &run, int repeats,
+                  char *report, int report_len);
 
 // Complex GEMM out[b][m][n] (+)= sum_k conj(A[b][m][k]) B[b][n][k] (pp_gemm.hip).
 // type codes: 0 = c128, 1 = c64.  Batch strides in elements; amap/bmap

@@ -306,16 +306,17 @@ __global__ __launch_bounds__(256) void k_phase(long npts, const double *__restri
 // flight together, and reads the projector values as LDS broadcasts.  The
 // four wavefronts' sums are added in LDS at the end.
 template <typename T, int PMAX, bool FULL, int U>
-__global__ __launch_bounds__(256) void k_sphere_sum(const cplx<T> *__restrict__ g, int NB,
-                                                    const int *__restrict__ site_pt, const int *__restrict__ site_np,
-                                                    const long *__restrict__ tab_off, const int *__restrict__ pidx,
-                                                    const cplx<T> *__restrict__ phase, const T *__restrict__ table,
-                                                    int nparts, cplx<double> *__restrict__ partial, int streaming) {
+__device__ __forceinline__ void sphere_sum_block(const cplx<T> *__restrict__ g, int NB,
+                                                 const int *__restrict__ site_pt, const int *__restrict__ site_np,
+                                                 const long *__restrict__ tab_off, const int *__restrict__ pidx,
+                                                 const cplx<T> *__restrict__ phase, const T *__restrict__ table,
+                                                 int nparts, cplx<double> *__restrict__ partial, int streaming,
+                                                 int blk) {
   constexpr int TP = 64;
   __shared__ __align__(16) T s_tab[TP * PMAX];
   __shared__ cplx<T> s_ph[TP];
   __shared__ int s_idx[TP];
-  const int s = blockIdx.x / nparts, part = blockIdx.x % nparts;
+  const int s = blk / nparts, part = blk % nparts;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int p0 = site_pt[s], p1 = site_pt[s + 1];
   const int np = FULL ? PMAX : site_np[s];  // FULL: every site has PMAX projectors
@@ -417,24 +418,42 @@ __global__ __launch_bounds__(256) void k_sphere_sum(const cplx<T> *__restrict__ 
     }
   }
   if (wave == 0 && active) {
-    cplx<double> *o = partial + ((long)blockIdx.x * NB + lane) * PMAX;
+    cplx<double> *o = partial + ((long)blk * NB + lane) * PMAX;
 #pragma unroll
     for (int j = 0; j < PMAX; j++)
       if (FULL || j < np)
         o[j] = cvt<double>(acc[j]);
   }
 }
+template <typename T, int PMAX, bool FULL, int U>
+__global__ __launch_bounds__(256) void k_sphere_sum(const cplx<T> *__restrict__ g, int NB,
+                                                    const int *__restrict__ site_pt, const int *__restrict__ site_np,
+                                                    const long *__restrict__ tab_off, const int *__restrict__ pidx,
+                                                    const cplx<T> *__restrict__ phase, const T *__restrict__ table,
+                                                    int nparts, cplx<double> *__restrict__ partial, int streaming) {
+  sphere_sum_block<T, PMAX, FULL, U>(g, NB, site_pt, site_np, tab_off, pidx, phase, table, nparts, partial, streaming,
+                                     blockIdx.x);
+}
+// The sphere sums of a batch of work items (one per blockIdx.y), each with
+// its own chunk, plan and phases.
+template <typename T, int PMAX, bool FULL, int U>
+__global__ __launch_bounds__(256) void k_sphere_sum_items(const ChunkItem *__restrict__ items, int NB, int streaming) {
+  const ChunkItem &it = items[blockIdx.y];
+  const int blk = blockIdx.x;
+  if (blk >= it.nsites * it.nparts)
+    return;
+  sphere_sum_block<T, PMAX, FULL, U>((const cplx<T> *)it.grid, NB, it.site_pt, it.site_np, it.tab_off, it.pidx,
+                                     (const cplx<T> *)it.phase, (const T *)it.table, it.nparts,
+                                     (cplx<double> *)it.partial, streaming, blk);
+}
 
 // Sum the parts of every (site, band, projector) in FP64 and form the
 // complex projections from the real-harmonic sums: thread (s, b, j) with
 // m_j >= 0 writes slot j and, for m_j > 0, its partner slot j - 2 m_j.
-__global__ __launch_bounds__(256) void k_sphere_reduce(const cplx<double> *__restrict__ partial, int NB, int nbc,
-                                                       int PMAX, int nparts, int nsites,
-                                                       const int *__restrict__ site_np,
-                                                       const int *__restrict__ proj_off,
-                                                       const int *__restrict__ proj_m, long ptot,
-                                                       cplx<double> *__restrict__ out) {
-  long t = (long)blockIdx.x * 256 + threadIdx.x;
+__device__ __forceinline__ void sphere_reduce_one(const cplx<double> *__restrict__ partial, int NB, int nbc,
+                                                  int PMAX, int nparts, int nsites, const int *__restrict__ site_np,
+                                                  const int *__restrict__ proj_off, const int *__restrict__ proj_m,
+                                                  long ptot, cplx<double> *__restrict__ out, long t) {
   long total = (long)nsites * nbc * PMAX;
   if (t >= total)
     return;
@@ -462,6 +481,20 @@ __global__ __launch_bounds__(256) void k_sphere_reduce(const cplx<double> *__res
     o[j] = mk<double>(c.x + sn.y, c.y - sn.x);
     o[j - 2 * m] = mk<double>(sg * (c.x - sn.y), sg * (c.y + sn.x));
   }
+}
+__global__ __launch_bounds__(256) void k_sphere_reduce(const cplx<double> *__restrict__ partial, int NB, int nbc,
+                                                       int PMAX, int nparts, int nsites,
+                                                       const int *__restrict__ site_np,
+                                                       const int *__restrict__ proj_off,
+                                                       const int *__restrict__ proj_m, long ptot,
+                                                       cplx<double> *__restrict__ out) {
+  sphere_reduce_one(partial, NB, nbc, PMAX, nparts, nsites, site_np, proj_off, proj_m, ptot, out,
+                    (long)blockIdx.x * 256 + threadIdx.x);
+}
+__global__ __launch_bounds__(256) void k_sphere_reduce_items(const ChunkItem *__restrict__ items, int NB, int PMAX) {
+  const ChunkItem &it = items[blockIdx.y];
+  sphere_reduce_one((const cplx<double> *)it.partial, NB, it.nbc, PMAX, it.nparts, it.nsites, it.site_np,
+                    it.proj_off, it.proj_m, it.ptot, it.out, (long)blockIdx.x * 256 + threadIdx.x);
 }
 
 // Coefficients of the real-harmonic table slots for the expansion
@@ -771,23 +804,16 @@ static void launch_sum(pp_ctx *ctx, pp_sphere *sp, const cplx<T> *grid, int NB, 
 #undef PP_SS
 }
 
+int sphere_parts_of(pp_ctx *ctx, pp_sphere *sp);
+int sphere_pm_of(pp_sphere *sp, bool *full);
+
 template <typename T>
 int sphere_sums(pp_ctx *ctx, pp_sphere *sp, const cplx<T> *grid, int NB, int nbc, const cplx<T> *phase,
                 cplx<double> *out, long ptot) {
   if (sp->nsites == 0 || nbc == 0)
     return PP_OK;
-  int nparts = ctx->tune_sphere_parts;
-  if (nparts <= 0) {
-    // aim for ~8 four-wave work-groups per CU (256 CUs)
-    nparts = (2048 + sp->nsites - 1) / sp->nsites;
-    long avg = sp->npts / std::max(1, sp->nsites);
-    int maxp = (int)std::max<long>(1, avg / 256);
-    if (nparts > maxp)
-      nparts = maxp;
-    if (nparts < 1)
-      nparts = 1;
-  }
-  int PM = sp->pmax <= 8 ? 8 : sp->pmax <= 18 ? 18 : sp->pmax <= 32 ? 32 : 64;
+  const int nparts = sphere_parts_of(ctx, sp);
+  const int PM = sphere_pm_of(sp, nullptr);
   DevBuf &pbuf = ctx->slot ? ctx->partial2 : ctx->partial;
   PP_TRY(pbuf.ensure(sizeof(cplx<double>) * (size_t)sp->nsites * nparts * NB * PM));
   cplx<double> *partial = (cplx<double> *)pbuf.p;
@@ -817,6 +843,95 @@ int sphere_sums(pp_ctx *ctx, pp_sphere *sp, const cplx<T> *grid, int NB, int nbc
   PP_CHECK_HIP(hipGetLastError());
   return PP_OK;
 }
+
+// Sphere-sum parts per site of a plan (the tuning value, else about eight
+// four-wave work-groups per CU).
+int sphere_parts_of(pp_ctx *ctx, pp_sphere *sp) {
+  int nparts = ctx->tune_sphere_parts;
+  if (nparts <= 0) {
+    nparts = (2048 + sp->nsites - 1) / std::max(1, sp->nsites);
+    long avg = sp->npts / std::max(1, sp->nsites);
+    int maxp = (int)std::max<long>(1, avg / 256);
+    if (nparts > maxp)
+      nparts = maxp;
+    if (nparts < 1)
+      nparts = 1;
+  }
+  return nparts;
+}
+// Projector slots per site the sum kernels hold for a plan (8, 18, 32, 64)
+// and whether every site has exactly that many.
+int sphere_pm_of(pp_sphere *sp, bool *full) {
+  const int PM = sp->pmax <= 8 ? 8 : sp->pmax <= 18 ? 18 : sp->pmax <= 32 ? 32 : 64;
+  if (full)
+    *full = sp->pmin == PM;
+  return PM;
+}
+// The plan's part of a work item.
+void sphere_item_fields(pp_ctx *ctx, pp_sphere *sp, ChunkItem &it) {
+  it.site_pt = (const int *)sp->d_site_pt.p;
+  it.site_np = (const int *)sp->d_site_np.p;
+  it.proj_off = (const int *)sp->d_proj_off.p;
+  it.proj_m = (const int *)sp->d_proj_m.p;
+  it.pidx = (const int *)sp->d_pidx.p;
+  it.tab_off = (const long *)sp->d_tab_off.p;
+  it.table = sp->d_table.p;
+  it.nsites = sp->nsites;
+  it.nparts = sphere_parts_of(ctx, sp);
+}
+
+// Sphere sums and their reduction for a batch of work items: one launch
+// each.  PM and full as sphere_pm_of for every item's plan; maxblk bounds
+// nsites * nparts and maxred nsites * nbc * PM over the items.
+template <typename T>
+int sphere_sums_items(pp_ctx *ctx, const ChunkItem *items, int nitems, int NB, int PM, bool full, long maxblk,
+                      long maxred) {
+  if (nitems == 0 || maxblk == 0)
+    return PP_OK;
+  hipEvent_t ev;
+  tbegin(ctx, K_SPHERE_SUM, &ev);
+  dim3 grd((unsigned)maxblk, (unsigned)nitems);
+  const bool u8 = ctx->tune_sphere_unroll == 8;
+#define PP_SI(P, F, U)                                                                                           \
+  hipLaunchKernelGGL((k_sphere_sum_items<T, P, F, U>), grd, dim3(256), 0, ctx->stream, items, NB, ctx->tune_sphere_nt)
+#define PP_SIP(P)                                                                                                \
+  do {                                                                                                           \
+    if (full) {                                                                                                  \
+      if (u8)                                                                                                    \
+        PP_SI(P, true, 8);                                                                                       \
+      else                                                                                                       \
+        PP_SI(P, true, 4);                                                                                       \
+    } else if (u8)                                                                                               \
+      PP_SI(P, false, 8);                                                                                        \
+    else                                                                                                         \
+      PP_SI(P, false, 4);                                                                                        \
+  } while (0)
+  switch (PM) {
+  case 8:
+    PP_SIP(8);
+    break;
+  case 18:
+    PP_SIP(18);
+    break;
+  case 32:
+    PP_SIP(32);
+    break;
+  default:
+    PP_SIP(64);
+  }
+#undef PP_SIP
+#undef PP_SI
+  tend(ctx, K_SPHERE_SUM, ev);
+  PP_CHECK_HIP(hipGetLastError());
+  tbegin(ctx, K_SPHERE_REDUCE, &ev);
+  hipLaunchKernelGGL(k_sphere_reduce_items, dim3((unsigned)((maxred + 255) / 256), (unsigned)nitems), dim3(256), 0,
+                     ctx->stream, items, NB, PM);
+  tend(ctx, K_SPHERE_REDUCE, ev);
+  PP_CHECK_HIP(hipGetLastError());
+  return PP_OK;
+}
+template int sphere_sums_items<double>(pp_ctx *, const ChunkItem *, int, int, int, bool, long, long);
+template int sphere_sums_items<float>(pp_ctx *, const ChunkItem *, int, int, int, bool, long, long);
 
 // phase[pt] = dv / sqrt(V) e^{i k.path} (sphere sums) or, with expand,
 // e^{-i k.path} (the augmentation functions of projector.c:276-330)

@@ -348,3 +348,72 @@ def plant_near_orthogonal(torch, CR, CS, nw, pairs, eps=NEAR_ORTHOGONAL_EPS):
         CS[k, s, :W] = v.to(torch.complex64)
         out.append((int(k), int(r), int(s), e))
     return out
+
+
+@dataclasses.dataclass
+class MultiWorkload:
+    """Several structures projected onto one reference over many k-points
+    with uneven plane-wave and band counts.  ``encuts[k]`` is the cutoff of
+    k-point k's sphere (its G list: ``gvecs[k]``), ``nbands[i][k]`` the band
+    count of structure i (0 = the reference) at k."""
+
+    name: str
+    lattice: np.ndarray
+    frac: list                   # [reference, structure 1, ...] fractional coordinates
+    species: Species
+    fftg: np.ndarray
+    kpts: np.ndarray
+    kws: np.ndarray
+    encuts: np.ndarray
+    nbands: np.ndarray           # [nstruct + 1, nk]
+    delta_o: np.ndarray
+    M: list                      # per structure: (M_R, M_S) matched-site lists
+    gvecs: list = None
+
+    @property
+    def encut(self):
+        return float(self.encuts.max())
+
+    def ensure_gvecs(self):
+        if self.gvecs is None:
+            self.gvecs = [gvectors(self.lattice, k, e) for k, e in zip(self.kpts, self.encuts)]
+        return self.gvecs
+
+
+def wedge_kpoints(rng, nk):
+    """Random k-points in the irreducible wedge of the simple-cubic
+    Brillouin zone, 0 <= kz <= ky <= kx <= 1/2 (fractional)."""
+    k = np.sort(rng.uniform(0.0, 0.5, (nk, 3)), axis=1)[:, ::-1]
+    return np.ascontiguousarray(k)
+
+
+def config3(nk=64, nband_range=(384, 640), spread=0.10):
+    """Batched uneven workload: the 216-site cell (a = 16.29 A, 108^3 grid,
+    18 projectors per site, rmax 1.6 A) and three perturbed structures
+    projected onto the bulk reference: the vacancy of config1 (215 sites),
+    a seeded 0.05 A jitter of every site, and 8 sites moved 0.1 A.  nk
+    random k-points in the irreducible wedge (stream 4).  With one cutoff,
+    shifting k changes the sphere's plane-wave count by well under 1% for
+    this cell, so each k-point's cutoff is drawn around 520 eV as
+    520 (1 + u)^(2/3), u ~ U[-spread, spread], which makes W_k vary by
+    +-spread; bands per (structure, k) ~ U[nband_range] (stream 4)."""
+    st = streams()
+    rng = st[4]
+    base = config1()
+    lattice, frac_R = base.lattice, base.frac_R
+    cart = frac_R @ lattice
+    inv = np.linalg.inv(lattice)
+    jit = np.mod((cart + st[0].normal(0.0, 0.05, cart.shape)) @ inv, 1.0)
+    moved = cart.copy()
+    sel = rng.choice(len(cart), 8, replace=False)
+    dirs = rng.normal(size=(8, 3))
+    moved[sel] += 0.1 * dirs / np.linalg.norm(dirs, axis=1, keepdims=True)
+    moved = np.mod(moved @ inv, 1.0)
+    kpts = wedge_kpoints(rng, nk)
+    u = rng.uniform(-spread, spread, nk)
+    encuts = 520.0 * (1.0 + u) ** (2.0 / 3.0)
+    nbands = rng.integers(nband_range[0], nband_range[1] + 1, (4, nk))
+    idx = np.arange(len(frac_R), dtype=np.int32)
+    M = [(base.M_R, base.M_S), (idx, idx.copy()), (idx, idx.copy())]
+    return MultiWorkload("cfg3", lattice, [frac_R, base.frac_S, jit, moved], base.species, base.fftg.copy(), kpts,
+                         np.full(nk, 1.0 / nk), encuts, nbands, base.delta_o, M)
