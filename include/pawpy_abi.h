@@ -39,7 +39,8 @@ extern "C" {
 /* arithmetic of the hot path */
 #define PP_FP64 0   /* FP64 everywhere (the plain path, the parity path) */
 #define PP_FP32 1   /* FFT, sphere sums and overlap GEMM in FP32 (MFMA), FP64 reductions */
-#define PP_BF16X3 2 /* as PP_FP32, overlap GEMM as 3-term split bf16 MFMA products */
+#define PP_BF16X3 2 /* as PP_FP32, overlap GEMM as 3 products of (hi, lo) bf16 planes on MFMA */
+#define PP_BF16X6 3 /* as PP_FP32, overlap GEMM as 6 products of (hi, mid, lo) bf16 planes on MFMA */
 
 /* Message of the last failing call on this thread ("" if none). */
 const char *pp_last_error(void);

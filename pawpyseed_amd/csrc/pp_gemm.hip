@@ -189,17 +189,15 @@ int gemm_fp32_mfma(pp_ctx *ctx, int precision, int M, int N, int K, const void *
 
 // Split-K policy of the overlap GEMMs: enough K chunks that the 64x64
 // output tiles times the chunks give about 1024 work-groups (four per CU),
-// and chunks of at least 256 complex terms.  The FP32 kernels fold their
-// FP32 sums into FP64 every 512 terms inside a chunk, so the chunk length
-// does not bound their accuracy; 3xBF16 keeps chunks of at most 512 terms
-// (its FP32 sums are not folded).  A positive `tune` forces the count.
+// and chunks of at least 256 complex terms.  The FP32 and split-bf16
+// kernels fold their FP32 sums into FP64 every 512 terms inside a chunk, so
+// the chunk length does not bound their accuracy.  A positive `tune`
+// forces the count.
 int splitk_count(int precision, int M, int N, int K, int tune) {
   if (tune > 0)
     return tune;
   const int tiles = std::max(1, ((M + 63) / 64) * ((N + 63) / 64));
   int n = (1024 + tiles - 1) / tiles;
-  if (precision == PP_BF16X3)
-    n = std::max((2048 + tiles - 1) / tiles, (K + 511) / 512);
   n = std::min(n, std::max(1, K / 256));
   return std::max(1, n);
 }

@@ -142,6 +142,7 @@ struct pp_ctx {
   pp::DevBuf phase;     // per-point Bloch phase of every k
   pp::DevBuf gemm_ws;   // split-K partials
   pp::DevBuf gemm_ws2;  // second partials buffer (reduction of one batch overlaps the next GEMM)
+  pp::DevBuf bf16_ws;   // bf16 planes of the split-bf16 GEMM's operands
   hipEvent_t gemm_done[2] = {nullptr, nullptr};  // GEMM of a partials slot finished
   hipEvent_t red_done[2] = {nullptr, nullptr};   // reduction of a partials slot finished
   pp::DevBuf aug_ws;    // gathered / dO-applied projections

@@ -9,9 +9,10 @@
 //
 // PP_FP32: v_mfma_f32_16x16x4_f32; FP32 sums are folded into FP64 every
 // 512 complex terms inside a split-K chunk, chunks summed in FP64.
-// PP_BF16X3: each FP32 operand is split into
-// hi = bf16(x) and lo = bf16(x - hi); hi*hi + hi*lo + lo*hi on
-// v_mfma_f32_16x16x32_bf16 with FP32 accumulation.
+// PP_BF16X3 / PP_BF16X6: each FP32 operand is split once per call into two
+// (hi, lo) or three (hi, mid, lo) bf16 planes; the 3 or 6 plane products
+// that carry 16 or 24 significant bits run on v_mfma_f32_16x16x32_bf16
+// with FP32 accumulation, folded into FP64 like the FP32 path.
 //
 // Work-group: 4 wavefronts, 64x64 complex outputs, each wavefront a 32x32
 // quadrant as 2x2 16x16 MFMA tiles (real and imaginary accumulators).
@@ -180,14 +181,74 @@ __device__ __forceinline__ bf16x8 ld8(const unsigned short *p) {
   return __builtin_bit_cast(bf16x8, u);
 }
 
-template <bool VEC>
-__global__ __launch_bounds__(256) void k_cgemm_bf16x3(int M, int N, int kchunk, int K, const float *__restrict__ A,
-                                                      long lda, const float *__restrict__ B, long ldb,
-                                                      cplx<double> *__restrict__ C, long ldc, long sSplit) {
-  // 32 complex = 64 real k' per stage; rows padded to 72 (144 B)
-  __shared__ __attribute__((aligned(16))) unsigned short Ah[64][72], Al[64][72];
-  __shared__ __attribute__((aligned(16))) unsigned short Bh[64][72], Bl[64][72];
-  __shared__ __attribute__((aligned(16))) unsigned short Qh[64][72], Ql[64][72];
+// Split of FP32 data into NT bf16 planes, x = hi + mid (+ lo) up to the
+// rounding of the last plane: hi = bf16(x), mid = bf16(x - hi),
+// lo = bf16(x - hi - mid), each residual exact in FP32.  X holds rows of
+// K complex values (row stride ldx complex); plane t of row r occupies
+// P[t * sp + r * ldp + j] for the 2K real values j, zeros up to ldp (a
+// multiple of 64).  One thread converts 8 consecutive reals.
+template <int NT>
+__global__ __launch_bounds__(256) void k_split_bf16(const float *__restrict__ X, long ldx, int rows, int K, long ldp,
+                                                    long sp, unsigned short *__restrict__ P) {
+  const long per_row = ldp / 8;
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  const long r = t / per_row, j0 = (t % per_row) * 8;
+  if (r >= rows)
+    return;
+  const float *x = X + r * 2 * ldx;
+  const long nreal = 2L * K;
+  float v[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++)
+    v[i] = j0 + i < nreal ? x[j0 + i] : 0.f;
+  unsigned short h[NT][8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    float rem = v[i];
+#pragma unroll
+    for (int q = 0; q < NT; q++) {
+      h[q][i] = bf16_bits(rem);
+      rem -= bf16_val(h[q][i]);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NT; q++) {
+    uint4 w;
+    w.x = (unsigned)h[q][0] | ((unsigned)h[q][1] << 16);
+    w.y = (unsigned)h[q][2] | ((unsigned)h[q][3] << 16);
+    w.z = (unsigned)h[q][4] | ((unsigned)h[q][5] << 16);
+    w.w = (unsigned)h[q][6] | ((unsigned)h[q][7] << 16);
+    *reinterpret_cast<uint4 *>(P + q * sp + r * ldp + j0) = w;
+  }
+}
+
+// (re, im) bf16 pairs of a B row -> the pairs (im, -re) of B'': swap the
+// halves of each 32-bit word and flip the sign of the new upper half.
+__device__ __forceinline__ bf16x8 qswap(uint4 b) {
+  uint4 q;
+  q.x = ((b.x >> 16) | (b.x << 16)) ^ 0x80000000u;
+  q.y = ((b.y >> 16) | (b.y << 16)) ^ 0x80000000u;
+  q.z = ((b.z >> 16) | (b.z << 16)) ^ 0x80000000u;
+  q.w = ((b.w >> 16) | (b.w << 16)) ^ 0x80000000u;
+  return __builtin_bit_cast(bf16x8, q);
+}
+
+// Complex conj(A) B^T from NT bf16 planes of each operand (k_split_bf16),
+// on v_mfma_f32_16x16x32_bf16: every product of planes (i, j) with
+// i + j < NT, the smallest terms first (NT = 2: 3 products, 16 significant
+// bits per operand; NT = 3: 6 products, 24 bits, FP32-like).  A tile's
+// planes go global -> registers -> LDS as 16-byte accesses, the next
+// k-step's loads in flight during the current step's MFMAs; B'' for the
+// imaginary part is formed from B's registers (qswap).  Planes hold
+// zeros past the data, so chunks need no k bounds.  FOLD: FP32 sums folded
+// into FP64 every 512 complex terms.
+template <int NT, bool FOLD>
+__global__ __launch_bounds__(256) void k_cgemm_bf16p(int M, int N, int kchunk, int K, const unsigned short *__restrict__ A,
+                                                     long lda, long spa, const unsigned short *__restrict__ B,
+                                                     long ldb, long spb, cplx<double> *__restrict__ C, long ldc,
+                                                     long sSplit) {
+  __shared__ __attribute__((aligned(16))) unsigned short As[NT][64][72];
+  __shared__ __attribute__((aligned(16))) unsigned short Bs[NT][64][72];
   const int split = blockIdx.y;
   const int tilesN = (N + 63) / 64;
   const int tm = blockIdx.x / tilesN, tn = blockIdx.x % tilesN;
@@ -197,84 +258,102 @@ __global__ __launch_bounds__(256) void k_cgemm_bf16x3(int M, int N, int kchunk, 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   floatx4 cr[2][2], ci[2][2];
+  double dr[2][2][4], di[2][2][4];
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
     for (int b = 0; b < 2; b++) {
       cr[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
       ci[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int v = 0; v < 4; v++)
+        dr[a][b][v] = di[a][b][v] = 0.0;
     }
   const int lrow = threadIdx.x >> 2, part = threadIdx.x & 3;
-  // a k-step's values of a thread are loaded together, the next step's
-  // while the current step's MFMAs run
-  float pa[2][8], pb[2][8];
+  const int ra = tm * 64 + lrow, rb = tn * 64 + lrow;
+  const uint4 zero = {0u, 0u, 0u, 0u};
+  uint4 pa[NT][2], pb[NT][2];
   auto fetch = [&](int kb) {
+    const long off = 2L * kb + part * 16;
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
-      const int gk = kb + part * 8 + h * 4;
-      load8<VEC>(A, lda, tm * 64 + lrow, M, gk, k1, pa[h]);
-      load8<VEC>(B, ldb, tn * 64 + lrow, N, gk, k1, pb[h]);
-    }
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        pa[t][h] = ra < M ? *reinterpret_cast<const uint4 *>(A + t * spa + (long)ra * lda + off + h * 8) : zero;
+        pb[t][h] = rb < N ? *reinterpret_cast<const uint4 *>(B + t * spb + (long)rb * ldb + off + h * 8) : zero;
+      }
   };
   if (k0 < k1)
     fetch(k0);
+  int since = 0;
+  const int sub = lane & 15, kq = lane >> 4;
   for (int kb = k0; kb < k1; kb += 32) {
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
-      const float *va = pa[h], *vb = pb[h];
-      const int o = part * 16 + h * 8;
+    for (int t = 0; t < NT; t++)
 #pragma unroll
-      for (int c = 0; c < 8; c++) {
-        unsigned short ah = bf16_bits(va[c]);
-        Ah[lrow][o + c] = ah;
-        Al[lrow][o + c] = bf16_bits(va[c] - bf16_val(ah));
-        unsigned short bh = bf16_bits(vb[c]);
-        Bh[lrow][o + c] = bh;
-        Bl[lrow][o + c] = bf16_bits(vb[c] - bf16_val(bh));
-        // B'' = (b_im, -b_re)
-        const float q = (c & 1) ? -vb[c - 1] : vb[c + 1];
-        unsigned short qh = bf16_bits(q);
-        Qh[lrow][o + c] = qh;
-        Ql[lrow][o + c] = bf16_bits(q - bf16_val(qh));
+      for (int h = 0; h < 2; h++) {
+        *reinterpret_cast<uint4 *>(&As[t][lrow][part * 16 + h * 8]) = pa[t][h];
+        *reinterpret_cast<uint4 *>(&Bs[t][lrow][part * 16 + h * 8]) = pb[t][h];
       }
-    }
     __syncthreads();
     if (kb + 32 < k1)
       fetch(kb + 32);
-    const int sub = lane & 15, kq = lane >> 4;
 #pragma unroll
     for (int kk = 0; kk < 64; kk += 32) {
       const int kc = kk + kq * 8;
-      bf16x8 ah[2], al[2];
+      bf16x8 a[NT][2];
 #pragma unroll
-      for (int mi = 0; mi < 2; mi++) {
-        ah[mi] = ld8(&Ah[wm * 32 + mi * 16 + sub][kc]);
-        al[mi] = ld8(&Al[wm * 32 + mi * 16 + sub][kc]);
-      }
+      for (int t = 0; t < NT; t++)
+#pragma unroll
+        for (int mi = 0; mi < 2; mi++)
+          a[t][mi] = ld8(&As[t][wm * 32 + mi * 16 + sub][kc]);
 #pragma unroll
       for (int nj = 0; nj < 2; nj++) {
         const int col = wn * 32 + nj * 16 + sub;
-        const bf16x8 bh = ld8(&Bh[col][kc]), bl = ld8(&Bl[col][kc]);
-        const bf16x8 qh = ld8(&Qh[col][kc]), ql = ld8(&Ql[col][kc]);
+        bf16x8 b[NT], q[NT];
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+          const uint4 raw = *reinterpret_cast<const uint4 *>(&Bs[t][col][kc]);
+          b[t] = __builtin_bit_cast(bf16x8, raw);
+          q[t] = qswap(raw);
+        }
 #pragma unroll
         for (int mi = 0; mi < 2; mi++) {
-          cr[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[mi], bh, cr[mi][nj], 0, 0, 0);
-          cr[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mi], bl, cr[mi][nj], 0, 0, 0);
-          cr[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mi], bh, cr[mi][nj], 0, 0, 0);
-          ci[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[mi], qh, ci[mi][nj], 0, 0, 0);
-          ci[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mi], ql, ci[mi][nj], 0, 0, 0);
-          ci[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mi], qh, ci[mi][nj], 0, 0, 0);
+          // plane pairs (i, j), i + j < NT, smallest terms first
+#pragma unroll
+          for (int d = NT - 1; d >= 0; d--)
+#pragma unroll
+            for (int i = d; i >= 0; i--) {
+              const int j = d - i;
+              cr[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][mi], b[j], cr[mi][nj], 0, 0, 0);
+              ci[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][mi], q[j], ci[mi][nj], 0, 0, 0);
+            }
         }
       }
     }
     __syncthreads();
+    if (FOLD && ++since == 16) {
+      since = 0;
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            dr[a][b][v] += (double)cr[a][b][v];
+            di[a][b][v] += (double)ci[a][b][v];
+          }
+          cr[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+          ci[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+        }
+    }
   }
 #pragma unroll
   for (int mi = 0; mi < 2; mi++)
 #pragma unroll
     for (int nj = 0; nj < 2; nj++)
-      store_tile(C, ldc, M, N, tm * 64 + wm * 32 + mi * 16, tn * 64 + wn * 32 + nj * 16, cr[mi][nj], ci[mi][nj],
-                 lane);
+      store_tile64(C, ldc, M, N, tm * 64 + wm * 32 + mi * 16, tn * 64 + wn * 32 + nj * 16, dr[mi][nj], di[mi][nj],
+                   cr[mi][nj], ci[mi][nj], lane);
 }
 
 int splitk_count(int precision, int M, int N, int K, int tune);
@@ -284,7 +363,7 @@ int launch_splitk_reduce(hipStream_t st, int M, int N, int nsplit, const cplx<do
 // K chunk (complex terms) of an FP32 / 3xBF16 GEMM split nsplit ways: a
 // multiple of the kernel's k-step; nsplit is reduced to the chunks needed.
 int cgemm_chunk(int precision, int K, int &nsplit) {
-  const int kstep = precision == PP_BF16X3 ? 32 : 16;
+  const int kstep = (precision == PP_BF16X3 || precision == PP_BF16X6) ? 32 : 16;
   int kchunk = (K + nsplit - 1) / std::max(1, nsplit);
   kchunk = (kchunk + kstep - 1) / kstep * kstep;
   if (kchunk == 0)
@@ -293,37 +372,74 @@ int cgemm_chunk(int precision, int K, int &nsplit) {
   return kchunk;
 }
 
-// The FP32 / 3xBF16 GEMM of one batch into FP64 partials P
-// ([nsplit][M][ldp], split stride sSplit; with nsplit == 1 P may be the
-// output itself).  FP32 chunks longer than 512 complex terms fold their
-// FP32 sums into FP64 every 512 terms.  No allocation.
+// The FP32 GEMM of one batch into FP64 partials P ([nsplit][M][ldp],
+// split stride sSplit; with nsplit == 1 P may be the output itself).
+// Chunks longer than 512 complex terms fold their FP32 sums into FP64
+// every 512 terms.  No allocation.
 int launch_cgemm32(hipStream_t st, int precision, int M, int N, int K, const void *A, long lda, const void *B,
                    long ldb, cplx<double> *P, long ldp, long sSplit, int nsplit, int kchunk) {
+  (void)precision;
   const int tiles = ((M + 63) / 64) * ((N + 63) / 64);
   const bool vec = (lda % 2 == 0) && (ldb % 2 == 0) && ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0);
   dim3 grd(tiles, nsplit);
-  if (precision == PP_BF16X3) {
-    if (vec)
-      hipLaunchKernelGGL(k_cgemm_bf16x3<true>, grd, dim3(256), 0, st, M, N, kchunk, K, (const float *)A, lda,
-                         (const float *)B, ldb, P, ldp, sSplit);
-    else
-      hipLaunchKernelGGL(k_cgemm_bf16x3<false>, grd, dim3(256), 0, st, M, N, kchunk, K, (const float *)A, lda,
-                         (const float *)B, ldb, P, ldp, sSplit);
+  const bool fold = kchunk > 512;
+  if (vec && fold)
+    hipLaunchKernelGGL((k_cgemm_f32<true, true>), grd, dim3(256), 0, st, M, N, kchunk, K, (const float *)A, lda,
+                       (const float *)B, ldb, P, ldp, sSplit);
+  else if (vec)
+    hipLaunchKernelGGL((k_cgemm_f32<true, false>), grd, dim3(256), 0, st, M, N, kchunk, K, (const float *)A, lda,
+                       (const float *)B, ldb, P, ldp, sSplit);
+  else if (fold)
+    hipLaunchKernelGGL((k_cgemm_f32<false, true>), grd, dim3(256), 0, st, M, N, kchunk, K, (const float *)A, lda,
+                       (const float *)B, ldb, P, ldp, sSplit);
+  else
+    hipLaunchKernelGGL((k_cgemm_f32<false, false>), grd, dim3(256), 0, st, M, N, kchunk, K, (const float *)A, lda,
+                       (const float *)B, ldb, P, ldp, sSplit);
+  PP_CHECK_HIP(hipGetLastError());
+  return PP_OK;
+}
+
+// Split-bf16 GEMM of one batch (PP_BF16X3: two planes per operand, 3
+// products; PP_BF16X6: three planes, 6 products): both operands split once
+// into the plane buffer `planes` ((M + N) * NT * ldp shorts, ldp = 2K
+// rounded up to 64), then the plane GEMM into partials P as launch_cgemm32.
+int launch_cgemm_bf16(hipStream_t st, int precision, int M, int N, int K, const void *A, long lda, const void *B,
+                      long ldb, unsigned short *planes, cplx<double> *P, long ldp_out, long sSplit, int nsplit,
+                      int kchunk) {
+  const int NT = precision == PP_BF16X6 ? 3 : 2;
+  const long ldp = (2L * K + 63) / 64 * 64;
+  unsigned short *PA = planes, *PB = planes + (size_t)NT * M * ldp;
+  const long nA = (long)M * (ldp / 8), nB = (long)N * (ldp / 8);
+  if (NT == 3) {
+    hipLaunchKernelGGL(k_split_bf16<3>, dim3((unsigned)((nA + 255) / 256)), dim3(256), 0, st, (const float *)A, lda, M,
+                       K, ldp, (long)M * ldp, PA);
+    hipLaunchKernelGGL(k_split_bf16<3>, dim3((unsigned)((nB + 255) / 256)), dim3(256), 0, st, (const float *)B, ldb, N,
+                       K, ldp, (long)N * ldp, PB);
   } else {
-    const bool fold = kchunk > 512;
-    if (vec && fold)
-      hipLaunchKernelGGL((k_cgemm_f32<true, true>), grd, dim3(256), 0, st, M, N, kchunk, K, (const float *)A, lda,
-                         (const float *)B, ldb, P, ldp, sSplit);
-    else if (vec)
-      hipLaunchKernelGGL((k_cgemm_f32<true, false>), grd, dim3(256), 0, st, M, N, kchunk, K, (const float *)A, lda,
-                         (const float *)B, ldb, P, ldp, sSplit);
-    else if (fold)
-      hipLaunchKernelGGL((k_cgemm_f32<false, true>), grd, dim3(256), 0, st, M, N, kchunk, K, (const float *)A, lda,
-                         (const float *)B, ldb, P, ldp, sSplit);
-    else
-      hipLaunchKernelGGL((k_cgemm_f32<false, false>), grd, dim3(256), 0, st, M, N, kchunk, K, (const float *)A, lda,
-                         (const float *)B, ldb, P, ldp, sSplit);
+    hipLaunchKernelGGL(k_split_bf16<2>, dim3((unsigned)((nA + 255) / 256)), dim3(256), 0, st, (const float *)A, lda, M,
+                       K, ldp, (long)M * ldp, PA);
+    hipLaunchKernelGGL(k_split_bf16<2>, dim3((unsigned)((nB + 255) / 256)), dim3(256), 0, st, (const float *)B, ldb, N,
+                       K, ldp, (long)N * ldp, PB);
   }
+  PP_CHECK_HIP(hipGetLastError());
+  const int tiles = ((M + 63) / 64) * ((N + 63) / 64);
+  dim3 grd(tiles, nsplit);
+  const bool fold = kchunk > 512;
+#define PP_BP(NTV, F)                                                                                            \
+  hipLaunchKernelGGL((k_cgemm_bf16p<NTV, F>), grd, dim3(256), 0, st, M, N, kchunk, K, PA, ldp, (long)M * ldp, PB, ldp, \
+                     (long)N * ldp, P, ldp_out, sSplit)
+  if (NT == 3) {
+    if (fold)
+      PP_BP(3, true);
+    else
+      PP_BP(3, false);
+  } else {
+    if (fold)
+      PP_BP(2, true);
+    else
+      PP_BP(2, false);
+  }
+#undef PP_BP
   PP_CHECK_HIP(hipGetLastError());
   return PP_OK;
 }
@@ -343,8 +459,18 @@ int gemm_fp32_mfma(pp_ctx *ctx, int precision, int M, int N, int K, const void *
   if (slot >= 0)
     PP_CHECK_HIP(hipStreamWaitEvent(ctx->stream, ctx->red_done[slot], 0));
   hipEvent_t ev;
-  tbegin(ctx, K_GEMM, &ev);
-  int st = launch_cgemm32(ctx->stream, precision, M, N, K, A, lda, B, ldb, P, (long)N, (long)M * N, nsplit, kchunk);
+  int st;
+  if (precision == PP_BF16X3 || precision == PP_BF16X6) {
+    const int NT = precision == PP_BF16X6 ? 3 : 2;
+    const long ldp = (2L * K + 63) / 64 * 64;
+    PP_TRY(ctx->bf16_ws.ensure(sizeof(unsigned short) * (size_t)NT * (M + N) * ldp));
+    tbegin(ctx, K_GEMM, &ev);
+    st = launch_cgemm_bf16(ctx->stream, precision, M, N, K, A, lda, B, ldb, (unsigned short *)ctx->bf16_ws.p, P,
+                           (long)N, (long)M * N, nsplit, kchunk);
+  } else {
+    tbegin(ctx, K_GEMM, &ev);
+    st = launch_cgemm32(ctx->stream, precision, M, N, K, A, lda, B, ldb, P, (long)N, (long)M * N, nsplit, kchunk);
+  }
   tend(ctx, K_GEMM, ev);
   PP_TRY(st);
   hipStream_t main = ctx->stream;

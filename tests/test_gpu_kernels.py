@@ -225,7 +225,7 @@ def test_overlap_fp64_spin_flip(ctx):
     assert np.abs(got - want).max() < 1e-12
 
 
-@pytest.mark.parametrize("precision,tol", [("fp32", 1e-6), ("bf16x3", 2e-5)])
+@pytest.mark.parametrize("precision,tol", [("fp32", 1e-6), ("bf16x3", 2e-5), ("bf16x6", 1e-6)])
 def test_overlap_mixed(ctx, precision, tol):
     got, want = _overlap_case(ctx, precision, nbR=130, nbS=67)
     err = normwise(got, want)
@@ -256,7 +256,7 @@ def test_overlap_near_orthogonal_pairs(ctx):
     want = np.stack([orc.pseudoprojection(CR, CS, b, 1, 1) for b in range(9)], axis=1)  # [bR, bS]
     diag = np.abs(np.diag(want))
     assert diag.min() < 1e-7 and diag.max() > 0.5  # the stored pairs really span the range
-    for precision, tol in (("fp64", None), ("fp32", 1e-6), ("bf16x3", 2e-5)):
+    for precision, tol in (("fp64", None), ("fp32", 1e-6), ("bf16x3", 2e-5), ("bf16x6", 1e-6)):
         out = torch.zeros((1, 9, 9), dtype=torch.complex128, device="cuda")
         B.overlap_gemm(ctx, 1, 1, False, nw, CRd, CSd, out, precision)
         got = out.cpu().numpy()[0]
