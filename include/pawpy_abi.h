@@ -410,10 +410,12 @@ int pp_overlap_gemm(pp_ctx *ctx, int nwk, int nspin, int flip, const int *nw,
  * live in a fixed 64 MiB device array of the library (one per GPU, held
  * until the process ends), so nbR * nbS may not exceed
  * pp_noalloc_scratch_bytes() / 32 (2,097,152) output entries per k-point
- * (PP_ERR_ARG otherwise); the result equals pp_overlap_gemm's whenever
- * its split count fits the array (e.g. 512 x 512 bands).  Runs on the
- * context's stream only and touches none of the context's buffers.  Calls
- * that overlap in time on one GPU share the array and race: serialise them. */
+ * (PP_ERR_ARG otherwise).  The result equals pp_overlap_gemm's whenever
+ * its split-K partial sums fit the array (FP64 at 512 x 512 bands), else
+ * it differs by the rounding of another split (FP32: fewer, longer chunks
+ * with in-chunk FP64 folds).  Runs on the context's stream only and touches
+ * none of the context's buffers.  Calls that overlap in time on one GPU
+ * share the array and race: serialise them. */
 int pp_overlap_gemm_noalloc(pp_ctx *ctx, int nwk, int nspin, int flip, const int *nw,
                             const void *CR_dev, long ldR, int nbR, const void *CS_dev,
                             long ldS, int nbS, void *out_dev, int precision);

@@ -480,18 +480,35 @@ __global__ __launch_bounds__(NT, (FftWaves<T, MAXB, COMP>::value)) void k_fft_pa
 // The same pass over the chunks of a batch of work items (one per
 // blockIdx.y): each item brings its own chunk buffer and the plane-wave
 // bounds and line lists of its k-point; tiles past the item's lines exit.
+// (7 rather than 8 waves per SIMD where the single-chunk pass takes 8: the
+// descriptor's pointers and counts cost a few registers, and 64 would spill)
 template <typename T, int NBW, int AXIS, int MAXB, int NT>
-__global__ __launch_bounds__(NT, (FftWaves<T, MAXB, false>::value)) void k_fft_pass_ip_items(
+__global__ __launch_bounds__(NT, (FftWaves<T, MAXB, false>::value == 8 ? 7 : FftWaves<T, MAXB, false>::value))
+void k_fft_pass_ip_items(
     const ChunkItem *__restrict__ items, AxisPlan pl, const cplx<T> *__restrict__ tw, long es, long s1, long s2,
     int n2, int nslices, int streaming) {
-  const ChunkItem &it = items[blockIdx.y];
-  const SphereBounds &bd = it.bd;
+  const ChunkItem *it = items + blockIdx.y;
+  // the item's bounds field by field into a local (uniform, held in scalar
+  // registers), so that no private copy of the descriptor is made
+  SphereBounds bd;
+  bd.zb = it->bd.zb;
+  bd.yb = it->bd.yb;
+  bd.xb = it->bd.xb;
+  bd.n0 = it->bd.n0;
+  bd.n1 = it->bd.n1;
+  bd.n2 = it->bd.n2;
+  bd.zl = AXIS == 2 ? it->bd.zl : nullptr;
+  bd.nzl = AXIS == 2 ? it->bd.nzl : 0;
+  bd.yl = AXIS == 1 ? it->bd.yl : nullptr;
+  bd.nyl = AXIS == 1 ? it->bd.nyl : 0;
+  bd.xm = AXIS == 0 ? it->bd.xm : nullptr;
+  bd.xmw = AXIS == 0 ? it->bd.xmw : 0;
   const long nlin = AXIS == 2 ? (bd.zl ? bd.nzl : (long)bd.n0 * bd.n1)
                               : AXIS == 1 ? (bd.yl ? bd.nyl : (long)bd.n0 * bd.n2) : (long)bd.n1 * bd.n2;
   const int blk = (int)blockIdx.x;
   if (blk >= nlin * nslices)
     return;
-  fft_pass_ip_tile<T, NBW, AXIS, MAXB, NT, false>((cplx<T> *)it.grid, pl, tw, es, s1, s2, n2, nslices, bd, blk,
+  fft_pass_ip_tile<T, NBW, AXIS, MAXB, NT, false>((cplx<T> *)it->grid, pl, tw, es, s1, s2, n2, nslices, bd, blk,
                                                   streaming);
 }
 

@@ -12,14 +12,17 @@
 namespace pp {
 
 template <typename TA, typename TB>
-__global__ __launch_bounds__(256) void k_zgemm_f64(int M, int N, int kbeg_stride, int K, const cplx<TA> *__restrict__ A,
-                                                   long lda, const cplx<TB> *__restrict__ B, long ldb,
+__global__ __launch_bounds__(256) void k_zgemm_f64(int M, int N, int kbeg_stride, int K, int nsplit,
+                                                   const cplx<TA> *__restrict__ A, long lda,
+                                                   const cplx<TB> *__restrict__ B, long ldb,
                                                    cplx<double> *__restrict__ C, long ldc, long sSplit) {
   __shared__ cplx<double> As[16][64 + 1];
   __shared__ cplx<double> Bs[16][64 + 1];
-  const int split = blockIdx.y;
   const int tilesN = (N + 63) / 64;
-  const int tm = blockIdx.x / tilesN, tn = blockIdx.x % tilesN;
+  int split, tile;
+  if (!gemm_block(((M + 63) / 64) * tilesN, nsplit, split, tile))
+    return;
+  const int tm = tile / tilesN, tn = tile % tilesN;
   const int k0 = split * kbeg_stride;
   const int k1 = min(K, k0 + kbeg_stride);
   C += (long)split * sSplit;
@@ -88,15 +91,17 @@ typedef double doublex4 __attribute__((ext_vector_type(4)));
 // layout of a 16x16 tile: lane l, register v holds row l/16 + 4v, column
 // l%16 (unlike the FP32 16x16x4 form, whose register v is row 4(l/16)+v).
 template <typename TA, typename TB>
-__global__ __launch_bounds__(256) void k_zgemm_mfma64(int M, int N, int kchunk, int K,
+__global__ __launch_bounds__(256) void k_zgemm_mfma64(int M, int N, int kchunk, int K, int nsplit,
                                                       const cplx<TA> *__restrict__ A, long lda,
                                                       const cplx<TB> *__restrict__ B, long ldb,
                                                       cplx<double> *__restrict__ C, long ldc, long sSplit) {
   __shared__ double As[64][36];
   __shared__ double Bs[64][36];
-  const int split = blockIdx.y;
   const int tilesN = (N + 63) / 64;
-  const int tm = blockIdx.x / tilesN, tn = blockIdx.x % tilesN;
+  int split, tile;
+  if (!gemm_block(((M + 63) / 64) * tilesN, nsplit, split, tile))
+    return;
+  const int tm = tile / tilesN, tn = tile % tilesN;
   const int k0 = split * kchunk;
   const int k1 = min(K, k0 + kchunk);
   C += (long)split * sSplit;
@@ -171,8 +176,10 @@ __global__ __launch_bounds__(256) void k_zgemm_mfma64(int M, int N, int kchunk, 
     }
 }
 
-// C[m][n] (+)= sum over splits of partial[s][m][n]
-__global__ __launch_bounds__(256) void k_splitk_reduce(int M, int N, int nsplit, const cplx<double> *__restrict__ P,
+// C[m][n] (+)= sum over splits of partial[s][m][n], in FP64 (partials FP64
+// or FP32)
+template <typename TP>
+__global__ __launch_bounds__(256) void k_splitk_reduce(int M, int N, int nsplit, const cplx<TP> *__restrict__ P,
                                                        cplx<double> *__restrict__ C, long ldc, int accumulate) {
   long t = (long)blockIdx.x * 256 + threadIdx.x;
   if (t >= (long)M * N)
@@ -180,24 +187,29 @@ __global__ __launch_bounds__(256) void k_splitk_reduce(int M, int N, int nsplit,
   int m = (int)(t / N), n = (int)(t % N);
   cplx<double> s = accumulate ? C[(long)m * ldc + n] : mk<double>(0, 0);
   for (int q = 0; q < nsplit; q++)
-    s = s + P[(long)q * M * N + t];
+    s = s + cvt<double>(P[(long)q * M * N + t]);
   C[(long)m * ldc + n] = s;
 }
 
 int gemm_fp32_mfma(pp_ctx *ctx, int precision, int M, int N, int K, const void *A, long lda, const void *B, long ldb,
                    cplx<double> *C, long ldc, bool accumulate, int slot);
 
-// Split-K policy of the overlap GEMMs: enough K chunks that the 64x64
-// output tiles times the chunks give about 1024 work-groups (four per CU),
-// and chunks of at least 256 complex terms.  The FP32 and split-bf16
-// kernels fold their FP32 sums into FP64 every 512 terms inside a chunk, so
-// the chunk length does not bound their accuracy.  A positive `tune`
-// forces the count.
+// Split-K policy of the overlap GEMMs.  FP64: enough K chunks that the
+// 64x64 output tiles times the chunks give about 1024 work-groups (four per
+// CU), chunks of at least 256 complex terms.  FP32: chunks of at most 512
+// complex terms, whose FP32 sums are exact to the FP32 accumulation of 512
+// terms, summed in FP64 by the reduction (a kernel that instead folds its
+// FP32 sums into FP64 inside longer chunks holds 64 more registers per lane
+// and runs at a third of the occupancy: 94 against 119 TFLOP/s at 216
+// atoms).  Split bf16: the FP64 policy with in-chunk folds.  A positive
+// `tune` forces the count.
 int splitk_count(int precision, int M, int N, int K, int tune) {
   if (tune > 0)
     return tune;
   const int tiles = std::max(1, ((M + 63) / 64) * ((N + 63) / 64));
   int n = (1024 + tiles - 1) / tiles;
+  if (precision == PP_FP32)
+    n = std::max(n, (K + 511) / 512);
   n = std::min(n, std::max(1, K / 256));
   return std::max(1, n);
 }
@@ -207,10 +219,10 @@ int splitk_count(int precision, int M, int N, int K, int tune) {
 int launch_zgemm64(hipStream_t st, bool valu, int M, int N, int K, int atype, const void *A, long lda, int btype,
                    const void *B, long ldb, cplx<double> *dst, long dld, long sSplit, int nsplit, int kchunk) {
   const int tiles = ((M + 63) / 64) * ((N + 63) / 64);
-  dim3 grd(tiles, nsplit);
+  const dim3 grd = gemm_grid(tiles, nsplit);
   if (!valu) {
 #define PP_MF(TA, TB)                                                                                          \
-  hipLaunchKernelGGL((k_zgemm_mfma64<TA, TB>), grd, dim3(256), 0, st, M, N, kchunk, K, (const cplx<TA> *)A, lda, \
+  hipLaunchKernelGGL((k_zgemm_mfma64<TA, TB>), grd, dim3(256), 0, st, M, N, kchunk, K, nsplit, (const cplx<TA> *)A, lda, \
                      (const cplx<TB> *)B, ldb, dst, dld, sSplit)
     if (atype == 1 && btype == 1)
       PP_MF(float, float);
@@ -223,7 +235,7 @@ int launch_zgemm64(hipStream_t st, bool valu, int M, int N, int K, int atype, co
 #undef PP_MF
   } else {
 #define PP_MV(TA, TB)                                                                                          \
-  hipLaunchKernelGGL((k_zgemm_f64<TA, TB>), grd, dim3(256), 0, st, M, N, kchunk, K, (const cplx<TA> *)A, lda,    \
+  hipLaunchKernelGGL((k_zgemm_f64<TA, TB>), grd, dim3(256), 0, st, M, N, kchunk, K, nsplit, (const cplx<TA> *)A, lda, \
                      (const cplx<TB> *)B, ldb, dst, dld, sSplit)
     if (atype == 1 && btype == 1)
       PP_MV(float, float);
@@ -239,11 +251,15 @@ int launch_zgemm64(hipStream_t st, bool valu, int M, int N, int K, int atype, co
   return PP_OK;
 }
 
-int launch_splitk_reduce(hipStream_t st, int M, int N, int nsplit, const cplx<double> *P, cplx<double> *C, long ldc,
-                         bool accumulate) {
+int launch_splitk_reduce(hipStream_t st, int M, int N, int nsplit, const void *P, bool fp32_partials,
+                         cplx<double> *C, long ldc, bool accumulate) {
   const long tot = (long)M * N;
-  hipLaunchKernelGGL(k_splitk_reduce, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, M, N, nsplit, P, C, ldc,
-                     accumulate ? 1 : 0);
+  if (fp32_partials)
+    hipLaunchKernelGGL(k_splitk_reduce<float>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, M, N, nsplit,
+                       (const cplx<float> *)P, C, ldc, accumulate ? 1 : 0);
+  else
+    hipLaunchKernelGGL(k_splitk_reduce<double>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, M, N, nsplit,
+                       (const cplx<double> *)P, C, ldc, accumulate ? 1 : 0);
   PP_CHECK_HIP(hipGetLastError());
   return PP_OK;
 }
@@ -275,7 +291,7 @@ static int zgemm_one(pp_ctx *ctx, int precision, int M, int N, int K, int atype,
   PP_TRY(st);
   if (!direct) {
     tbegin(ctx, K_GEMM_REDUCE, &ev);
-    st = launch_splitk_reduce(ctx->stream, M, N, nsplit, (const cplx<double> *)ctx->gemm_ws.p, C, ldc, accumulate);
+    st = launch_splitk_reduce(ctx->stream, M, N, nsplit, ctx->gemm_ws.p, false, C, ldc, accumulate);
     tend(ctx, K_GEMM_REDUCE, ev);
   }
   return st;

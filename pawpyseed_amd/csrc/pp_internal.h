@@ -220,6 +220,20 @@ std::complex<double> offsite_overlap_host(const double *dcoord, const double *k1
                                           const double *const *s1, int n1, const double *k2, const double *f2,
                                           const double *const *s2, int n2, int l1, int m1, int l2, int m2);
 
+// Split-K GEMM work-group -> (K chunk, output tile): work-groups are dealt
+// round-robin to the 8 XCDs in launch order, so block b runs on XCD b % 8;
+// giving every tile of one chunk the same XCD lets that XCD's L2 hold the
+// chunk's A and B rows for all of its tiles, instead of every XCD reading
+// every chunk.  Grid: gemm_grid(tiles, nsplit) work-groups, of which those
+// past nsplit chunks do nothing.
+__device__ __forceinline__ bool gemm_block(int ntiles, int nsplit, int &split, int &tile) {
+  const int b = (int)blockIdx.x, r = b >> 3;
+  split = (r / ntiles) * 8 + (b & 7);
+  tile = r % ntiles;
+  return split < nsplit;
+}
+inline dim3 gemm_grid(int ntiles, int nsplit) { return dim3((unsigned)(((nsplit + 7) / 8) * 8 * ntiles)); }
+
 // Autotuner (pp_api.hip): coordinate descent over the named parameters,
 // timing run() on the context's stream.
 struct TuneCand {

@@ -25,14 +25,15 @@ namespace pp {
 
 // MFMA 16x16 output layout: lane l, register v holds row 4*(l/16)+v,
 // column l%16 of the 16x16 tile.
-__device__ __forceinline__ void store_tile(cplx<double> *C, long ldc, int M, int N, int r0, int c0, floatx4 re,
+template <typename TO>
+__device__ __forceinline__ void store_tile(cplx<TO> *C, long ldc, int M, int N, int r0, int c0, floatx4 re,
                                            floatx4 im, int lane) {
   const int j = c0 + (lane & 15);
 #pragma unroll
   for (int v = 0; v < 4; v++) {
     const int i = r0 + 4 * (lane >> 4) + v;
     if (i < M && j < N)
-      C[(long)i * ldc + j] = mk<double>((double)re[v], (double)im[v]);
+      C[(long)i * ldc + j] = mk<TO>((TO)re[v], (TO)im[v]);
   }
 }
 
@@ -75,18 +76,22 @@ __device__ __forceinline__ void load8(const float *__restrict__ P, long ld, int 
   }
 }
 
+// FOLD: FP32 sums folded into FP64 every 512 complex terms, FP64 partial
+// sums out; otherwise a chunk's FP32 sums are its (FP32) partial sums.
 template <bool VEC, bool FOLD>
-__global__ __launch_bounds__(256) void k_cgemm_f32(int M, int N, int kchunk, int K, const float *__restrict__ A,
-                                                   long lda, const float *__restrict__ B, long ldb,
-                                                   cplx<double> *__restrict__ C, long ldc, long sSplit) {
+__global__ __launch_bounds__(256) void k_cgemm_f32(int M, int N, int kchunk, int K, int nsplit,
+                                                   const float *__restrict__ A, long lda,
+                                                   const float *__restrict__ B, long ldb, void *__restrict__ Cv,
+                                                   long ldc, long sSplit) {
   __shared__ float As[64][36];
   __shared__ float Bs[64][36];
-  const int split = blockIdx.y;
   const int tilesN = (N + 63) / 64;
-  const int tm = blockIdx.x / tilesN, tn = blockIdx.x % tilesN;
+  int split, tile;
+  if (!gemm_block(((M + 63) / 64) * tilesN, nsplit, split, tile))
+    return;
+  const int tm = tile / tilesN, tn = tile % tilesN;
   const int k0 = split * kchunk;
   const int k1 = min(K, k0 + kchunk);
-  C += (long)split * sSplit;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   floatx4 cr[2][2], ci[2][2];
@@ -163,9 +168,14 @@ __global__ __launch_bounds__(256) void k_cgemm_f32(int M, int N, int kchunk, int
 #pragma unroll
   for (int mi = 0; mi < 2; mi++)
 #pragma unroll
-    for (int nj = 0; nj < 2; nj++)
-      store_tile64(C, ldc, M, N, tm * 64 + wm * 32 + mi * 16, tn * 64 + wn * 32 + nj * 16, dr[mi][nj], di[mi][nj],
-                   cr[mi][nj], ci[mi][nj], lane);
+    for (int nj = 0; nj < 2; nj++) {
+      const int r0 = tm * 64 + wm * 32 + mi * 16, c0 = tn * 64 + wn * 32 + nj * 16;
+      if constexpr (FOLD)
+        store_tile64((cplx<double> *)Cv + (long)split * sSplit, ldc, M, N, r0, c0, dr[mi][nj], di[mi][nj],
+                     cr[mi][nj], ci[mi][nj], lane);
+      else
+        store_tile<float>((cplx<float> *)Cv + (long)split * sSplit, ldc, M, N, r0, c0, cr[mi][nj], ci[mi][nj], lane);
+    }
 }
 
 // round-to-nearest-even float -> bf16 bits
@@ -243,15 +253,17 @@ __device__ __forceinline__ bf16x8 qswap(uint4 b) {
 // zeros past the data, so chunks need no k bounds.  FOLD: FP32 sums folded
 // into FP64 every 512 complex terms.
 template <int NT, bool FOLD>
-__global__ __launch_bounds__(256) void k_cgemm_bf16p(int M, int N, int kchunk, int K, const unsigned short *__restrict__ A,
-                                                     long lda, long spa, const unsigned short *__restrict__ B,
-                                                     long ldb, long spb, cplx<double> *__restrict__ C, long ldc,
-                                                     long sSplit) {
+__global__ __launch_bounds__(256) void k_cgemm_bf16p(int M, int N, int kchunk, int K, int nsplit,
+                                                     const unsigned short *__restrict__ A, long lda, long spa,
+                                                     const unsigned short *__restrict__ B, long ldb, long spb,
+                                                     cplx<double> *__restrict__ C, long ldc, long sSplit) {
   __shared__ __attribute__((aligned(16))) unsigned short As[NT][64][72];
   __shared__ __attribute__((aligned(16))) unsigned short Bs[NT][64][72];
-  const int split = blockIdx.y;
   const int tilesN = (N + 63) / 64;
-  const int tm = blockIdx.x / tilesN, tn = blockIdx.x % tilesN;
+  int split, tile;
+  if (!gemm_block(((M + 63) / 64) * tilesN, nsplit, split, tile))
+    return;
+  const int tm = tile / tilesN, tn = tile % tilesN;
   const int k0 = split * kchunk;
   const int k1 = min(K, k0 + kchunk);
   C += (long)split * sSplit;
@@ -357,8 +369,8 @@ __global__ __launch_bounds__(256) void k_cgemm_bf16p(int M, int N, int kchunk, i
 }
 
 int splitk_count(int precision, int M, int N, int K, int tune);
-int launch_splitk_reduce(hipStream_t st, int M, int N, int nsplit, const cplx<double> *P, cplx<double> *C, long ldc,
-                         bool accumulate);
+int launch_splitk_reduce(hipStream_t st, int M, int N, int nsplit, const void *P, bool fp32_partials,
+                         cplx<double> *C, long ldc, bool accumulate);
 
 // K chunk (complex terms) of an FP32 / 3xBF16 GEMM split nsplit ways: a
 // multiple of the kernel's k-step; nsplit is reduced to the chunks needed.
@@ -372,29 +384,29 @@ int cgemm_chunk(int precision, int K, int &nsplit) {
   return kchunk;
 }
 
-// The FP32 GEMM of one batch into FP64 partials P ([nsplit][M][ldp],
-// split stride sSplit; with nsplit == 1 P may be the output itself).
-// Chunks longer than 512 complex terms fold their FP32 sums into FP64
-// every 512 terms.  No allocation.
+// The FP32 GEMM of one batch into partial sums P ([nsplit][M][ldp], split
+// stride sSplit): FP32 partials for chunks of at most 512 complex terms,
+// else FP64 partials with the FP32 sums folded into FP64 every 512 terms
+// (fp32_partials_for says which).  No allocation.
+bool fp32_partials_for(int precision, int kchunk) { return precision == PP_FP32 && kchunk <= 512; }
 int launch_cgemm32(hipStream_t st, int precision, int M, int N, int K, const void *A, long lda, const void *B,
-                   long ldb, cplx<double> *P, long ldp, long sSplit, int nsplit, int kchunk) {
-  (void)precision;
+                   long ldb, void *P, long ldp, long sSplit, int nsplit, int kchunk) {
   const int tiles = ((M + 63) / 64) * ((N + 63) / 64);
   const bool vec = (lda % 2 == 0) && (ldb % 2 == 0) && ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0);
-  dim3 grd(tiles, nsplit);
-  const bool fold = kchunk > 512;
+  const dim3 grd = gemm_grid(tiles, nsplit);
+  const bool fold = !fp32_partials_for(precision, kchunk);
+#define PP_CF(V, F)                                                                                              \
+  hipLaunchKernelGGL((k_cgemm_f32<V, F>), grd, dim3(256), 0, st, M, N, kchunk, K, nsplit, (const float *)A, lda,    \
+                     (const float *)B, ldb, P, ldp, sSplit)
   if (vec && fold)
-    hipLaunchKernelGGL((k_cgemm_f32<true, true>), grd, dim3(256), 0, st, M, N, kchunk, K, (const float *)A, lda,
-                       (const float *)B, ldb, P, ldp, sSplit);
+    PP_CF(true, true);
   else if (vec)
-    hipLaunchKernelGGL((k_cgemm_f32<true, false>), grd, dim3(256), 0, st, M, N, kchunk, K, (const float *)A, lda,
-                       (const float *)B, ldb, P, ldp, sSplit);
+    PP_CF(true, false);
   else if (fold)
-    hipLaunchKernelGGL((k_cgemm_f32<false, true>), grd, dim3(256), 0, st, M, N, kchunk, K, (const float *)A, lda,
-                       (const float *)B, ldb, P, ldp, sSplit);
+    PP_CF(false, true);
   else
-    hipLaunchKernelGGL((k_cgemm_f32<false, false>), grd, dim3(256), 0, st, M, N, kchunk, K, (const float *)A, lda,
-                       (const float *)B, ldb, P, ldp, sSplit);
+    PP_CF(false, false);
+#undef PP_CF
   PP_CHECK_HIP(hipGetLastError());
   return PP_OK;
 }
@@ -423,11 +435,11 @@ int launch_cgemm_bf16(hipStream_t st, int precision, int M, int N, int K, const 
   }
   PP_CHECK_HIP(hipGetLastError());
   const int tiles = ((M + 63) / 64) * ((N + 63) / 64);
-  dim3 grd(tiles, nsplit);
+  const dim3 grd = gemm_grid(tiles, nsplit);
   const bool fold = kchunk > 512;
 #define PP_BP(NTV, F)                                                                                            \
-  hipLaunchKernelGGL((k_cgemm_bf16p<NTV, F>), grd, dim3(256), 0, st, M, N, kchunk, K, PA, ldp, (long)M * ldp, PB, ldp, \
-                     (long)N * ldp, P, ldp_out, sSplit)
+  hipLaunchKernelGGL((k_cgemm_bf16p<NTV, F>), grd, dim3(256), 0, st, M, N, kchunk, K, nsplit, PA, ldp, (long)M * ldp, \
+                     PB, ldp, (long)N * ldp, P, ldp_out, sSplit)
   if (NT == 3) {
     if (fold)
       PP_BP(3, true);
@@ -480,7 +492,7 @@ int gemm_fp32_mfma(pp_ctx *ctx, int precision, int M, int N, int K, const void *
     ctx->stream = ctx->aux;
   }
   tbegin(ctx, K_GEMM_REDUCE, &ev);
-  st = launch_splitk_reduce(ctx->stream, M, N, nsplit, P, C, ldc, accumulate);
+  st = launch_splitk_reduce(ctx->stream, M, N, nsplit, P, fp32_partials_for(precision, kchunk), C, ldc, accumulate);
   tend(ctx, K_GEMM_REDUCE, ev);
   ctx->stream = main;
   PP_TRY(st);

@@ -397,7 +397,7 @@ int pp_autotune_multi(pp_ctx *ctx, int nstruct, pp_sphere *const *plans, int nk,
   PP_CHECK_HIP(hipSetDevice(ctx->device));
   const bool fp32 = precision != PP_FP64;
   std::vector<TuneCand> cands = {
-      {"multi_items", {4, 8, 16}},
+      {"multi_items", {1, 2, 4, 8, 16}},
       {"streams", {1, 2}},
       {"fft_tile", fp32 ? std::vector<int>{16, 32} : std::vector<int>{8, 16}},
       {"sphere_unroll", {4, 8}},

@@ -23,11 +23,12 @@ namespace pp {
 int splitk_count(int precision, int M, int N, int K, int tune);
 int cgemm_chunk(int precision, int K, int &nsplit);
 int launch_cgemm32(hipStream_t st, int precision, int M, int N, int K, const void *A, long lda, const void *B,
-                   long ldb, cplx<double> *P, long ldp, long sSplit, int nsplit, int kchunk);
+                   long ldb, void *P, long ldp, long sSplit, int nsplit, int kchunk);
+bool fp32_partials_for(int precision, int kchunk);
 int launch_zgemm64(hipStream_t st, bool valu, int M, int N, int K, int atype, const void *A, long lda, int btype,
                    const void *B, long ldb, cplx<double> *dst, long dld, long sSplit, int nsplit, int kchunk);
-int launch_splitk_reduce(hipStream_t st, int M, int N, int nsplit, const cplx<double> *P, cplx<double> *C, long ldc,
-                         bool accumulate);
+int launch_splitk_reduce(hipStream_t st, int M, int N, int nsplit, const void *P, bool fp32_partials,
+                         cplx<double> *C, long ldc, bool accumulate);
 
 static constexpr size_t kScratchBytes = size_t(64) << 20;
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
@@ -80,7 +81,7 @@ int pp_overlap_gemm_noalloc(pp_ctx *ctx, int nwk, int nspin, int flip, const int
   PP_CHECK_HIP(hipSetDevice(ctx->device));
   cplx<double> *P = nullptr;
   PP_TRY(scratch_ptr(ctx->device, &P));
-  const int max_split = (int)(kScratchBytes / plane);
+  const int max_split64 = (int)(kScratchBytes / plane);  // FP64 partial planes that fit
   hipStream_t st = ctx->stream;
   for (int k = 0; k < nkt; k++) {
     const int kR = (flip && nspin == 2) ? (k < nwk ? k + nwk : k - nwk) : k;
@@ -88,14 +89,23 @@ int pp_overlap_gemm_noalloc(pp_ctx *ctx, int nwk, int nspin, int flip, const int
     const void *B = (const char *)CS_dev + (size_t)k * nbS * ldS * sizeof(cplx<float>);
     cplx<double> *C = (cplx<double> *)out_dev + (size_t)k * nbR * nbS;
     const int K = nw[k];
-    // the split policy of pp_overlap_gemm, within the scratch array
-    int nsplit = std::min(splitk_count(precision, nbR, nbS, K, ctx->tune_splitk), max_split);
+    // the split policy of pp_overlap_gemm where its partials fit the array
+    int nsplit = splitk_count(precision, nbR, nbS, K, ctx->tune_splitk);
     int kchunk;
+    bool fp32p = false;
     if (precision == PP_FP32) {
       kchunk = cgemm_chunk(precision, K, nsplit);
+      fp32p = fp32_partials_for(precision, kchunk);
+      if ((fp32p ? (size_t)nsplit * plane / 2 : (size_t)nsplit * plane) > kScratchBytes) {
+        // fewer, longer chunks with in-chunk FP64 folds and FP64 partials
+        nsplit = max_split64;
+        kchunk = cgemm_chunk(precision, K, nsplit);
+        fp32p = fp32_partials_for(precision, kchunk);
+      }
       PP_TRY(launch_cgemm32(st, precision, nbR, nbS, K, A, ldR, B, ldS, P, (long)nbS, (long)nbR * nbS, nsplit,
                             kchunk));
     } else {
+      nsplit = std::min(nsplit, max_split64);
       kchunk = (K + nsplit - 1) / nsplit;
       kchunk = std::max(16, (kchunk + 15) / 16 * 16);
       nsplit = std::max(1, (K + kchunk - 1) / kchunk);
@@ -104,7 +114,7 @@ int pp_overlap_gemm_noalloc(pp_ctx *ctx, int nwk, int nspin, int flip, const int
     }
     // the next k-point's GEMM overwrites the partials only after this
     // reduction has read them (one stream, in order)
-    PP_TRY(launch_splitk_reduce(st, nbR, nbS, nsplit, P, C, nbS, false));
+    PP_TRY(launch_splitk_reduce(st, nbR, nbS, nsplit, P, fp32p, C, nbS, false));
   }
   return PP_OK;
 }

@@ -1,9 +1,10 @@
 """pp_overlap_gemm_noalloc (no device allocation; split-K partials in the
 library's fixed scratch array) against pp_overlap_gemm through the C-ABI,
 on two calls in a row, the second with a smaller input, in FP64 and FP32.
-Where the split count of pp_overlap_gemm fits the scratch array the two
-run the same kernels on the same chunks, so the results are identical;
-the FP64 result also matches an FP64 numpy sum to 1e-12 normwise.  A call
+In FP64 both split K the same way within the scratch array, so the results
+are identical, and match an FP64 numpy sum to 1e-12 normwise; in FP32 the
+allocation-free call folds fewer, longer chunks (its FP32 partial sums
+would not fit), so the two agree to 1e-6 normwise, the FP32 bound.  A call
 beyond the scratch limit is refused with PP_ERR_ARG."""
 
 import ctypes as ct
@@ -56,9 +57,12 @@ def test_noalloc_matches_overlap_gemm(ctx, prec):
         assert _call("pp_overlap_gemm", ctx, nwk, nspin, flip, nw, CR, CS, ref, prec) == 0
         assert _call("pp_overlap_gemm_noalloc", ctx, nwk, nspin, flip, nw, CR, CS, got, prec) == 0
         ctx.synchronize()
-        print(f"call {call} prec {prec}: max |noalloc - overlap_gemm| = {(got - ref).abs().max().item():.3e}",
-              flush=True)
-        assert torch.equal(got, ref)
+        diff = ((got - ref).abs().max() / ref.abs().max()).item()
+        print(f"call {call} prec {prec}: max |noalloc - overlap_gemm| / max |overlap_gemm| = {diff:.3e}", flush=True)
+        if prec == 0:
+            assert torch.equal(got, ref)  # the same FP64 split
+        else:
+            assert diff < 1e-6  # FP32: another split of the same sums
         if prec == 0:
             kR = [k + nwk if k < nwk else k - nwk for k in range(nwk * nspin)]
             for k in range(nwk * nspin):
