@@ -231,6 +231,11 @@ def main():
         torch.cuda.synchronize()
         evs = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
         step(paths[0], evs)
+        # the overlap GEMM of every other arithmetic once (their MFMA counters)
+        ob = torch.zeros((nk, nb, nb), dtype=torch.complex128, device="cuda")
+        for prec in ("fp64", "bf16x3", "bf16x6"):
+            with torch.cuda.stream(stream):
+                B.overlap_gemm(ctx, nk, 1, False, nw, CR, CS, ob, prec)
         torch.cuda.synchronize()
         log("profile step done")
         return
@@ -374,6 +379,26 @@ def main():
         rl[c] = {"bound": "hbm", "achieved": b / (ms * 1e-3) / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                  "alg_bytes": b, "avg_launch_ms": ms, "ms_per_step": kc["ms_total"],
                  "traffic": traffic.get(c)}
+    # augmentation (O_M of all pairs): the gather of R's matched projector
+    # rows and dO applied to S's (read once, written once each), then an
+    # FP64 GEMM over the KM matched projectors per k
+    KM = int(sum(plans[head][1].proj_off[s + 1] - plans[head][1].proj_off[s] for s in wl.M_S))
+    aug_bytes = 32.0 * KM * (nb + nb)
+    kc = R["kernels"].get("aug_prep")
+    if kc and kc["launches"]:
+        ms = kc["ms_total"] / kc["launches"]  # one timed pair of launches covers all k
+        rl["aug_prep"] = {"bound": "hbm", "achieved": aug_bytes * nk / (ms * 1e-3) / 1e9, "peak": PEAK_HBM / 1e9,
+                          "unit": "GB/s", "alg_bytes": aug_bytes * nk, "avg_launch_ms": ms,
+                          "ms_per_step": kc["ms_total"], "traffic": traffic.get("aug_prep"),
+                          "what": "k_aug_gather + k_aug_apply of one call (all k): 16 B per gathered / applied "
+                                  "value, read and written"}
+    kc = R["kernels"].get("aug_gemm")
+    if kc and kc["launches"]:
+        ms = kc["ms_total"] / kc["launches"]
+        rl["aug_gemm"] = {"bound": "mfma", "achieved": 8.0 * nb * nb * KM / (ms * 1e-3) / 1e12,
+                          "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s", "alg_flops": 8.0 * nb * nb * KM,
+                          "avg_launch_ms": ms, "ms_per_step": kc["ms_total"], "traffic": traffic.get("aug_gemm"),
+                          "what": "FP64 MFMA conj(X_R) Y^T of one k-point over the matched projectors"}
     kc = R["kernels"].get("gemm")
     if kc and kc["launches"]:
         ms = kc["ms_total"] / kc["launches"]
@@ -383,6 +408,18 @@ def main():
                       "traffic": traffic.get("gemm")}
     for v in rl.values():
         v["frac"] = v["achieved"] / v["peak"]
+    # the (a) stage against its unit of 8 W + 16 N^3 bytes per (band, k):
+    # scatter and the three FFT passes of a step over their summed time
+    stage_a = [c for c in ("scatter", "fft_z", "fft_y", "fft_x") if c in R["kernels"]]
+    if stage_a:
+        t_a = sum(R["kernels"][c]["ms_total"] for c in stage_a) * 1e-3
+        unit_a = 8.0 * float(nw.mean()) + 16.0 * n0 * n1 * n2
+        rl_stage_a = {"bound": "hbm", "unit": "GB/s", "peak": PEAK_HBM / 1e9,
+                      "achieved": unit_a * nb * nk / t_a / 1e9, "alg_bytes_per_band_k": unit_a,
+                      "ms_per_step": t_a * 1e3,
+                      "what": "scatter + z/y/x passes of a step (breakdown timing) against 8 W + 16 N^3 bytes per "
+                              "(band, k)"}
+        rl_stage_a["frac"] = rl_stage_a["achieved"] / rl_stage_a["peak"]
     dom = max(rl, key=lambda c: rl[c]["ms_per_step"])
     roof = {k: rl[dom][k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")}
     roof["kernel"] = dom
@@ -417,6 +454,7 @@ def main():
         "overlap_tflops": R["overlap_tflops"],
         "paths": {p: {k: v for k, v in r.items() if k != "kernels"} for p, r in results.items()},
         "kernels": rl,
+        "stage_a": rl_stage_a if stage_a else None,
         "roofline": roof,
         "setup_pipeline_bandwidth": pipe,
     }
@@ -479,7 +517,41 @@ def bench_spinor512(args, torch, B):
         log(p, json.dumps(res[p]))
         plan.close()
         del up, down
+    # the band overlap of the spinor bands with those of a second structure
+    # (2 W-long vectors, pseudoprojector.c:85 with num_waves = 2 W)
+    CR2 = make_coefficients(torch, wl, Gs, 5005, nb, ldc2)
+    dn = make_coefficients(torch, wl, Gs, 6006, nb, max(W))
+    for k in range(nk):
+        CR2[k, :, W[k]:2 * W[k]] = dn[k, :, :W[k]]
+    del dn
+    nw2 = np.array([2 * w for w in W], np.int32)
+    ovl, ovl_out = {}, {}
+    ob = torch.zeros((nk, nb, nb), dtype=torch.complex128, device="cuda")
+    for p in [q for q in args.paths.split(",") if q] + ["bf16x6"]:
+        with torch.cuda.stream(stream):
+            B.overlap_gemm(ctx, nk, 1, False, nw2, CR2, C2, ob, p)  # warm
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(args.steps):
+                B.overlap_gemm(ctx, nk, 1, False, nw2, CR2, C2, ob, p, sync=False)
+            e1.record(stream)
+        torch.cuda.synchronize()
+        t = e0.elapsed_time(e1) / 1e3 / args.steps
+        ovl[p] = {"overlap_s": t, "overlap_tflops": 8.0 * nb * nb * float(nw2.sum()) / t / 1e12}
+        ovl_out[p] = ob[:, :4, :4].cpu().numpy()
+        log(p, "spinor overlap", json.dumps(ovl[p]))
+    del ob
     errs = {}
+    if not args.no_cpu:
+        # FP64 sums of the stored coefficients for the sampled 4 x 4 blocks
+        for p, o in ovl_out.items():
+            e = 0.0
+            for k in range(nk):
+                a = CR2[k, :4, : 2 * W[k]].cpu().numpy().astype(np.complex128)
+                b = C2[k, :4, : 2 * W[k]].cpu().numpy().astype(np.complex128)
+                ref = a.conj() @ b.T
+                e = max(e, float(np.abs(o[k] - ref).max() / np.abs(ref).max()))
+            errs[f"overlap_{p}"] = e
     if not args.no_cpu:
         sample = [0, nb - 1]
         Cs = C2[:, sample].cpu().numpy()
@@ -497,7 +569,7 @@ def bench_spinor512(args, torch, B):
            "warmup": args.warmup, "higher_is_better": True, "dtype": head, "data": "synthetic (seeded)",
            "config": {"workload": "512-atom diamond cell, a=21.72 A, 520 eV (W=275953 per spinor component), 192^3 "
                                   "grid, 2 k, 2048 spinor bands, 18 projectors/site", "nband": nb, "nk": nk},
-           "paths": res, "max_err_vs_fp64_oracle": errs}
+           "paths": res, "spinor_overlap": ovl, "max_err_vs_fp64_oracle": errs}
     print(json.dumps(out), flush=True)
 
 

@@ -161,6 +161,33 @@ class Projector:
             raise ValueError("Band index out of range (0-indexed)")
         return self._single_band_projection(band_num, **kwargs)
 
+    @staticmethod
+    def setup_multiple_projections(basis, wfs, method="aug_real", ignore_errors=False, site_lists=None):
+        """The batch driver of projector.py:311-382 over wavefunctions already
+        in memory (the reference reads each from a VASP directory; here the
+        caller passes the Wavefunction objects, as a list or a {name: wf}
+        mapping).  Yields [name, Projector] for each, all projected onto
+        ``basis``; with ``ignore_errors`` a wavefunction whose projector
+        cannot be set up is skipped and counted, else PAWpyError names it.
+        ``site_lists`` (optional, per name) is passed to Projector."""
+        items = wfs.items() if isinstance(wfs, dict) else enumerate(wfs)
+        basis.check_c_projectors()
+        errors, made = 0, 0
+        for name, wf in items:
+            try:
+                sl = site_lists.get(name) if isinstance(site_lists, dict) else site_lists
+                pr = Projector(wf, basis, method=method, site_lists=sl)
+            except Exception as e:
+                if not ignore_errors:
+                    raise PAWpyError(f"Unable to set up wavefunction {name}\nGot the following error:\n{e}")
+                errors += 1
+                continue
+            made += 1
+            yield [name, pr]
+        if not made:
+            raise PAWpyError("Could not generate any projector setups")
+        print("Number of errors:", errors)
+
     def proportion_conduction(self, band_num, spinpol=False):
         """Valence and conduction weight of band ``band_num`` of the
         projected structure in the basis (projector.py:384-435).

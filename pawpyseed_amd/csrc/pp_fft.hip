@@ -193,7 +193,8 @@ __global__ __launch_bounds__(256) void k_scatter_items(const ChunkItem *__restri
   const int g0 = blockIdx.x * 32;
   if (g0 >= it.nw)
     return;
-  scatter_tile<T>((cplx<T> *)it.grid, NB, it.gidx, it.nw, it.C, it.ldc, it.nbc, g0);
+  scatter_tile<T>((cplx<T> *)it.grid, NB, (const int *)it.gidx, it.nw, (const cplx<float> *)it.C, it.ldc, it.nbc,
+                  g0);
 }
 
 // One Stockham stage of radix R over a line tile in LDS:
@@ -488,21 +489,19 @@ void k_fft_pass_ip_items(
     const ChunkItem *__restrict__ items, AxisPlan pl, const cplx<T> *__restrict__ tw, long es, long s1, long s2,
     int n2, int nslices, int streaming) {
   const ChunkItem *it = items + blockIdx.y;
-  // the item's bounds field by field into a local (uniform, held in scalar
-  // registers), so that no private copy of the descriptor is made
   SphereBounds bd;
-  bd.zb = it->bd.zb;
-  bd.yb = it->bd.yb;
-  bd.xb = it->bd.xb;
-  bd.n0 = it->bd.n0;
-  bd.n1 = it->bd.n1;
-  bd.n2 = it->bd.n2;
-  bd.zl = AXIS == 2 ? it->bd.zl : nullptr;
-  bd.nzl = AXIS == 2 ? it->bd.nzl : 0;
-  bd.yl = AXIS == 1 ? it->bd.yl : nullptr;
-  bd.nyl = AXIS == 1 ? it->bd.nyl : 0;
-  bd.xm = AXIS == 0 ? it->bd.xm : nullptr;
-  bd.xmw = AXIS == 0 ? it->bd.xmw : 0;
+  bd.zb = (const int *)it->zb;
+  bd.yb = (const int *)it->yb;
+  bd.xb = (const int *)it->xb;
+  bd.n0 = it->n0;
+  bd.n1 = it->n1;
+  bd.n2 = it->n2;
+  bd.zl = AXIS == 2 ? (const int *)it->zl : nullptr;
+  bd.nzl = AXIS == 2 ? it->nzl : 0;
+  bd.yl = AXIS == 1 ? (const int *)it->yl : nullptr;
+  bd.nyl = AXIS == 1 ? it->nyl : 0;
+  bd.xm = AXIS == 0 ? (const unsigned *)it->xm : nullptr;
+  bd.xmw = AXIS == 0 ? it->xmw : 0;
   const long nlin = AXIS == 2 ? (bd.zl ? bd.nzl : (long)bd.n0 * bd.n1)
                               : AXIS == 1 ? (bd.yl ? bd.nyl : (long)bd.n0 * bd.n2) : (long)bd.n1 * bd.n2;
   const int blk = (int)blockIdx.x;

@@ -73,25 +73,33 @@ struct SphereBounds {
   int xmw = 0;
 };
 
+// Pointers of the work-item descriptors below are typed as global memory:
+// a generic pointer read from device memory would make the compiler emit
+// flat accesses through it (which also wait on the LDS counter).
+#define PP_G __attribute__((address_space(1)))
+
 // One work item of a batched setup over several structures and k-points
 // (pp_setup_projections_multi): a chunk of up to NB bands of one k-point
 // of one structure, with everything its kernels read.  The item kernels
 // take an array of these, one per blockIdx.y.
 struct ChunkItem {
-  void *grid;               // band-fastest chunk buffer of the item's slot
-  const cplx<float> *C;     // coefficients of the chunk's first band, row stride ldc
+  PP_G void *grid;             // band-fastest chunk buffer of the item's slot
+  const PP_G cplx<float> *C;   // coefficients of the chunk's first band, row stride ldc
   long ldc;
-  int nbc, nw;              // bands in the chunk, plane waves of the k-point
-  const int *gidx;          // wrapped grid index of every G of the k-point
-  SphereBounds bd;          // plane-wave sphere bounds and line lists of the k-point
-  const void *phase;        // Bloch phase of every sphere point of (structure, k-point)
+  int nbc, nw;                 // bands in the chunk, plane waves of the k-point
+  const PP_G int *gidx;        // wrapped grid index of every G of the k-point
+  // plane-wave sphere bounds and line lists of the k-point (SphereBounds)
+  const PP_G int *zb, *yb, *xb, *zl, *yl;
+  const PP_G unsigned *xm;
+  int n0, n1, n2, nzl, nyl, xmw;
+  const PP_G void *phase;      // Bloch phase of every sphere point of (structure, k-point)
   // sphere plan of the structure
-  const int *site_pt, *site_np, *proj_off, *proj_m, *pidx;
-  const long *tab_off;
-  const void *table;
+  const PP_G int *site_pt, *site_np, *proj_off, *proj_m, *pidx;
+  const PP_G long *tab_off;
+  const PP_G void *table;
   int nsites, nparts;
-  void *partial;            // sphere-sum partial sums of the slot
-  cplx<double> *out;        // projections of the chunk's first band, row stride ptot
+  PP_G void *partial;          // sphere-sum partial sums of the slot
+  PP_G cplx<double> *out;      // projections of the chunk's first band, row stride ptot
   long ptot;
 };
 

@@ -173,15 +173,26 @@ static int multi_prepare(pp_ctx *ctx, const MultiArgs &a, const MultiBufs &mb, M
       for (int b0 = 0; b0 < a.nband[i]; b0 += NB) {
         ChunkItem it;
         memset(&it, 0, sizeof(it));
-        it.C = (const cplx<float> *)a.C[i] + (long)b0 * a.ldc[i];
+        it.C = (const PP_G cplx<float> *)((const cplx<float> *)a.C[i] + (long)b0 * a.ldc[i]);
         it.ldc = a.ldc[i];
         it.nbc = std::min(NB, a.nband[i] - b0);
         it.nw = a.gofs[k + 1] - a.gofs[k];
-        it.gidx = gidx + a.gofs[k];
-        it.bd = bds;
-        it.phase = phase + ph_off[i];
+        it.gidx = (const PP_G int *)(gidx + a.gofs[k]);
+        it.zb = (const PP_G int *)bds.zb;
+        it.yb = (const PP_G int *)bds.yb;
+        it.xb = (const PP_G int *)bds.xb;
+        it.zl = (const PP_G int *)bds.zl;
+        it.yl = (const PP_G int *)bds.yl;
+        it.xm = (const PP_G unsigned *)bds.xm;
+        it.n0 = bds.n0;
+        it.n1 = bds.n1;
+        it.n2 = bds.n2;
+        it.nzl = bds.nzl;
+        it.nyl = bds.nyl;
+        it.xmw = bds.xmw;
+        it.phase = (const PP_G void *)(phase + ph_off[i]);
         sphere_item_fields(ctx, a.plans[s], it);
-        it.out = (cplx<double> *)a.proj[i] + (long)b0 * ptot;
+        it.out = (PP_G cplx<double> *)((cplx<double> *)a.proj[i] + (long)b0 * ptot);
         it.ptot = ptot;
         items.push_back(it);
       }
@@ -209,11 +220,11 @@ static int multi_prepare(pp_ctx *ctx, const MultiArgs &a, const MultiBufs &mb, M
     for (int j = 0; j < bt.count; j++) {
       ChunkItem &it = items[bt.first + j];
       const size_t slot = (size_t)bt.set * slots + j;
-      it.grid = (char *)mb.grids->p + slot * grid_bytes;
-      it.partial = (char *)mb.partials->p + slot * part_bytes;
+      it.grid = (PP_G void *)((char *)mb.grids->p + slot * grid_bytes);
+      it.partial = (PP_G void *)((char *)mb.partials->p + slot * part_bytes);
       bt.maxw = std::max(bt.maxw, it.nw);
-      bt.maxzl = std::max<long>(bt.maxzl, it.bd.nzl);
-      bt.maxyl = std::max<long>(bt.maxyl, it.bd.nyl);
+      bt.maxzl = std::max<long>(bt.maxzl, it.nzl);
+      bt.maxyl = std::max<long>(bt.maxyl, it.nyl);
       bt.maxblk = std::max<long>(bt.maxblk, (long)it.nsites * it.nparts);
       bt.maxred = std::max<long>(bt.maxred, (long)it.nsites * it.nbc * S.PM);
     }

@@ -442,7 +442,8 @@ __global__ __launch_bounds__(256) void k_sphere_sum_items(const ChunkItem *__res
   const int blk = blockIdx.x;
   if (blk >= it.nsites * it.nparts)
     return;
-  sphere_sum_block<T, PMAX, FULL, U>((const cplx<T> *)it.grid, NB, it.site_pt, it.site_np, it.tab_off, it.pidx,
+  sphere_sum_block<T, PMAX, FULL, U>((const cplx<T> *)it.grid, NB, (const int *)it.site_pt,
+                                     (const int *)it.site_np, (const long *)it.tab_off, (const int *)it.pidx,
                                      (const cplx<T> *)it.phase, (const T *)it.table, it.nparts,
                                      (cplx<double> *)it.partial, streaming, blk);
 }
@@ -493,8 +494,9 @@ __global__ __launch_bounds__(256) void k_sphere_reduce(const cplx<double> *__res
 }
 __global__ __launch_bounds__(256) void k_sphere_reduce_items(const ChunkItem *__restrict__ items, int NB, int PMAX) {
   const ChunkItem &it = items[blockIdx.y];
-  sphere_reduce_one((const cplx<double> *)it.partial, NB, it.nbc, PMAX, it.nparts, it.nsites, it.site_np,
-                    it.proj_off, it.proj_m, it.ptot, it.out, (long)blockIdx.x * 256 + threadIdx.x);
+  sphere_reduce_one((const cplx<double> *)it.partial, NB, it.nbc, PMAX, it.nparts, it.nsites,
+                    (const int *)it.site_np, (const int *)it.proj_off, (const int *)it.proj_m, it.ptot,
+                    (cplx<double> *)it.out, (long)blockIdx.x * 256 + threadIdx.x);
 }
 
 // Coefficients of the real-harmonic table slots for the expansion
@@ -869,13 +871,13 @@ int sphere_pm_of(pp_sphere *sp, bool *full) {
 }
 // The plan's part of a work item.
 void sphere_item_fields(pp_ctx *ctx, pp_sphere *sp, ChunkItem &it) {
-  it.site_pt = (const int *)sp->d_site_pt.p;
-  it.site_np = (const int *)sp->d_site_np.p;
-  it.proj_off = (const int *)sp->d_proj_off.p;
-  it.proj_m = (const int *)sp->d_proj_m.p;
-  it.pidx = (const int *)sp->d_pidx.p;
-  it.tab_off = (const long *)sp->d_tab_off.p;
-  it.table = sp->d_table.p;
+  it.site_pt = (const PP_G int *)sp->d_site_pt.p;
+  it.site_np = (const PP_G int *)sp->d_site_np.p;
+  it.proj_off = (const PP_G int *)sp->d_proj_off.p;
+  it.proj_m = (const PP_G int *)sp->d_proj_m.p;
+  it.pidx = (const PP_G int *)sp->d_pidx.p;
+  it.tab_off = (const PP_G long *)sp->d_tab_off.p;
+  it.table = (const PP_G void *)sp->d_table.p;
   it.nsites = sp->nsites;
   it.nparts = sphere_parts_of(ctx, sp);
 }

@@ -439,3 +439,29 @@ def test_proportion_conduction_with_occupations(gpu, method):
     wf_occ = wf._get_occs().reshape(8, -1)[:, 0]
     vbm = int(np.flatnonzero(wf_occ > 0.5)[-1])
     assert sorted(out) == list(range(max(vbm - 1, 0), min(vbm + 2, 7) + 1))
+
+
+def test_setup_multiple_projections(gpu):
+    """The batch driver (projector.py:311-382) over in-memory wavefunctions:
+    every yielded Projector's single_band_projection equals a Projector
+    built on its own; a wavefunction on another k-point grid is refused,
+    or skipped and counted with ignore_errors."""
+    from pawpyseed_amd.core import PAWpyError, Projector, wavefunction_from_workload
+    wl, CR, CS = _pair(nband=5)
+    basis = wavefunction_from_workload(wl, "R", coeffs=CR)
+    wfs = {"S": wavefunction_from_workload(wl, "S", coeffs=CS),
+           "S2": wavefunction_from_workload(wl, "S", coeffs=[c * np.exp(0.7j) for c in CS])}
+    got = dict(Projector.setup_multiple_projections(basis, wfs, site_lists=(wl.M_R, wl.M_S)))
+    assert sorted(got) == ["S", "S2"]
+    for name, pr in got.items():
+        alone = Projector(wfs[name], basis, site_lists=(wl.M_R, wl.M_S))
+        for band in (0, 4):
+            assert normwise(pr.single_band_projection(band), alone.single_band_projection(band)) < 1e-12
+    want, _, _ = _oracle_single_band(wl, CR, CS, 2)
+    assert normwise(got["S"].single_band_projection(2), want) < 1e-9
+    other, _, _ = _pair(nband=5, kpts=((0.1, 0, 0),))
+    bad = {"S": wfs["S"], "odd": wavefunction_from_workload(other, "S", coeffs=other.coefficients("S"))}
+    with pytest.raises(PAWpyError):
+        list(Projector.setup_multiple_projections(basis, bad, site_lists=(wl.M_R, wl.M_S)))
+    kept = list(Projector.setup_multiple_projections(basis, bad, ignore_errors=True, site_lists=(wl.M_R, wl.M_S)))
+    assert [n for n, _ in kept] == ["S"]

@@ -71,3 +71,45 @@ def test_spinor_projections_large_cell(ctx, case, precision, tol):
     print("spinor", precision, "up", eu, "down", ed, flush=True)
     assert eu < tol and ed < tol
     plan.close()
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", 1e-6), ("bf16x6", 1e-6)])
+def test_spinor_overlap_large_cell(ctx, case, precision, tol):
+    """The band overlap of spinor bands (pseudoprojector.c:63-90 over the
+    reference band's num_waves = 2 W, reader.c:274-281: the up then the down
+    component) at the large cell's full plane-wave count, 2 k-points, 16 x 16
+    bands of two structures, against the oracle's FP64 sums."""
+    from pawpyseed_amd.core import batched as B
+    wl, Gs, Cu, Cd, labels, _, _ = case
+    nk = len(Gs)
+    W = [len(g) for g in Gs]
+    ldc = -(-2 * max(W) // 64) * 64
+    rng = np.random.default_rng(23)
+    spin = []
+    for which in ("R", "S"):
+        C2 = np.zeros((nk, NB, ldc), np.complex64)
+        for k in range(nk):
+            up = wl.coefficients(which, rng=rng, nband=NB)[k]
+            dn = wl.coefficients(which, rng=rng, nband=NB)[k]
+            both = np.concatenate([up, dn], axis=1)
+            C2[k, :, : 2 * W[k]] = both / np.linalg.norm(both, axis=1, keepdims=True)
+        spin.append(C2)
+    CRd, CSd = (torch.from_numpy(c).cuda() for c in spin)
+    nw2 = np.array([2 * w for w in W], np.int32)
+    out = torch.zeros((nk, NB, NB), dtype=torch.complex128, device="cuda")
+    B.overlap_gemm(ctx, nk, 1, False, nw2, CRd, CSd, out, precision)
+    got = out.cpu().numpy()
+    CR = [spin[0][k, :, : 2 * W[k]] for k in range(nk)]
+    CS = [spin[1][k, :, : 2 * W[k]] for k in range(nk)]
+    want = np.zeros_like(got)
+    for b in range(NB):
+        want[:, :, b] = orc.pseudoprojection(CR, CS, b, nk, 1).reshape(NB, nk).T
+    # only the down halves: the up-only overlap misses them by O(1)
+    up_only = np.zeros_like(got)
+    for b in range(NB):
+        up_only[:, :, b] = orc.pseudoprojection([c[:, : W[k]] for k, c in enumerate(CR)],
+                                                [c[:, : W[k]] for k, c in enumerate(CS)], b, nk, 1).reshape(NB, nk).T
+    assert normwise(up_only, want) > 0.1
+    err = normwise(got, want)
+    print(f"spinor overlap {precision} (2W = {nw2.tolist()}): {err:.2e}", flush=True)
+    assert err < tol
