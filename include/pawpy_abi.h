@@ -41,6 +41,8 @@ extern "C" {
 #define PP_FP32 1   /* FFT, sphere sums and overlap GEMM in FP32 (MFMA), FP64 reductions */
 #define PP_BF16X3 2 /* as PP_FP32, overlap GEMM as 3 products of (hi, lo) bf16 planes on MFMA */
 #define PP_BF16X6 3 /* as PP_FP32, overlap GEMM as 6 products of (hi, mid, lo) bf16 planes on MFMA */
+#define PP_FP16X3 4 /* as PP_FP32, overlap GEMM as 3 products of (hi, lo) fp16 planes, each 1024-term chunk of a row
+                       scaled by a power of two (22 significant bits per operand) */
 
 /* Message of the last failing call on this thread ("" if none). */
 const char *pp_last_error(void);

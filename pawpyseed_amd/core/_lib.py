@@ -15,8 +15,8 @@ _HERE = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(_HERE, "libpawpy_amd.so")
 
 PP_OK, PP_ERR_ARG, PP_ERR_HIP, PP_ERR_NOMEM, PP_ERR_UNSUPPORTED, PP_ERR_IO = range(6)
-PP_FP64, PP_FP32, PP_BF16X3, PP_BF16X6 = 0, 1, 2, 3
-PRECISIONS = {"fp64": PP_FP64, "fp32": PP_FP32, "bf16x3": PP_BF16X3, "bf16x6": PP_BF16X6}
+PP_FP64, PP_FP32, PP_BF16X3, PP_BF16X6, PP_FP16X3 = 0, 1, 2, 3, 4
+PRECISIONS = {"fp64": PP_FP64, "fp32": PP_FP32, "bf16x3": PP_BF16X3, "bf16x6": PP_BF16X6, "fp16x3": PP_FP16X3}
 
 
 class PAWpyError(Exception):

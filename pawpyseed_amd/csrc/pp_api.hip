@@ -793,7 +793,7 @@ int pp_overlap_gemm(pp_ctx *ctx, int nwk, int nspin, int flip, const int *nw, co
   PP_REQUIRE(ctx && nw && nwk > 0 && (nspin == 1 || nspin == 2) && nbR >= 0 && nbS >= 0, PP_ERR_ARG,
              "pp_overlap_gemm: bad arguments");
   PP_REQUIRE(precision == PP_FP64 || precision == PP_FP32 || precision == PP_BF16X3 ||
-                 precision == PP_BF16X6,
+                 precision == PP_BF16X6 || precision == PP_FP16X3,
              PP_ERR_ARG,
              "pp_overlap_gemm: unknown precision");
   const int nkt = nwk * nspin;

@@ -382,7 +382,7 @@ extern "C" {
 
 int pp_set_precision(int precision) {
   PP_REQUIRE(precision == PP_FP64 || precision == PP_FP32 || precision == PP_BF16X3 ||
-                 precision == PP_BF16X6,
+                 precision == PP_BF16X6 || precision == PP_FP16X3,
              PP_ERR_ARG,
              "pp_set_precision: unknown precision");
   std::lock_guard<std::mutex> lk(g_mu);

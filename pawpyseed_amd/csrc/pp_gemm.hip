@@ -201,8 +201,8 @@ int gemm_fp32_mfma(pp_ctx *ctx, int precision, int M, int N, int K, const void *
 // terms, summed in FP64 by the reduction (a kernel that instead folds its
 // FP32 sums into FP64 inside longer chunks holds 64 more registers per lane
 // and runs at a third of the occupancy: 94 against 119 TFLOP/s at 216
-// atoms).  Split bf16: the FP64 policy with in-chunk folds.  A positive
-// `tune` forces the count.
+// atoms).  (The split-plane GEMMs chunk K by their plane layout.)  A
+// positive `tune` forces the count.
 int splitk_count(int precision, int M, int N, int K, int tune) {
   if (tune > 0)
     return tune;

@@ -7,15 +7,18 @@
 // so no operand repacking is needed for the real part and B'' is the
 // stored pair swapped with one sign flip.
 //
-// PP_FP32: v_mfma_f32_16x16x4_f32; FP32 sums are folded into FP64 every
-// 512 complex terms inside a split-K chunk, chunks summed in FP64.
-// PP_BF16X3 / PP_BF16X6: each FP32 operand is split once per call into two
-// (hi, lo) or three (hi, mid, lo) bf16 planes; the 3 or 6 plane products
-// that carry 16 or 24 significant bits run on v_mfma_f32_16x16x32_bf16
-// with FP32 accumulation, folded into FP64 like the FP32 path.
+// PP_FP32: v_mfma_f32_16x16x4_f32 over chunks of at most 512 complex
+// terms with FP32 partial sums (or FP32 sums folded into FP64 every 512
+// terms inside longer chunks), chunks summed in FP64.
+// PP_BF16X3 / PP_BF16X6 / PP_FP16X3: each FP32 operand is split once per
+// call into two (hi, lo) or three (hi, mid, lo) 16-bit planes; the 3 or 6
+// plane products run on v_mfma_f32_16x16x32_{bf16,f16} with FP32
+// accumulation over chunks of 512 complex terms, chunks summed in FP64.  The fp16 split scales each chunk of a row by a power of two first,
+// so that its two planes carry 22 significant bits where bf16's carry 16.
 //
-// Work-group: 4 wavefronts, 64x64 complex outputs, each wavefront a 32x32
-// quadrant as 2x2 16x16 MFMA tiles (real and imaginary accumulators).
+// FP32 work-group: 4 wavefronts, 64x64 complex outputs, each wavefront a
+// 32x32 quadrant as 2x2 16x16 MFMA tiles (real and imaginary
+// accumulators); split-plane work-group: 128x128, 64x64 per wavefront.
 #include "pp_internal.h"
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
@@ -186,196 +189,278 @@ __device__ __forceinline__ unsigned short bf16_bits(float x) {
 }
 __device__ __forceinline__ float bf16_val(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
 
-__device__ __forceinline__ bf16x8 ld8(const unsigned short *p) {
-  uint4 u = *reinterpret_cast<const uint4 *>(p);
-  return __builtin_bit_cast(bf16x8, u);
-}
+// Planes are split and scaled in chunks of kPlaneChunk reals (1024 complex
+// terms) of a row; rows are padded with zeros to a multiple of it.
+static constexpr int kPlaneChunk = 2048;
 
-// Split of FP32 data into NT bf16 planes, x = hi + mid (+ lo) up to the
-// rounding of the last plane: hi = bf16(x), mid = bf16(x - hi),
-// lo = bf16(x - hi - mid), each residual exact in FP32.  X holds rows of
-// K complex values (row stride ldx complex); plane t of row r occupies
-// P[t * sp + r * ldp + j] for the 2K real values j, zeros up to ldp (a
-// multiple of 64).  One thread converts 8 consecutive reals.
-template <int NT>
-__global__ __launch_bounds__(256) void k_split_bf16(const float *__restrict__ X, long ldx, int rows, int K, long ldp,
-                                                    long sp, unsigned short *__restrict__ P) {
-  const long per_row = ldp / 8;
-  const long t = (long)blockIdx.x * 256 + threadIdx.x;
-  const long r = t / per_row, j0 = (t % per_row) * 8;
-  if (r >= rows)
-    return;
-  const float *x = X + r * 2 * ldx;
+// Split of FP32 rows into NT 16-bit planes, one work-group per (row, chunk).
+// bf16 (F16 false): x = hi + mid (+ lo) up to the rounding of the last
+// plane, hi = bf16(x), mid = bf16(x - hi), lo = bf16(x - hi - mid), each
+// residual exact in FP32.  fp16 (F16 true, NT = 2): the chunk is first
+// scaled by 2^e, e chosen so that its largest magnitude lies in
+// [2^14, 2^15), then hi = fp16(y), lo = fp16(y - hi): 11 + 11 significant
+// bits, all of them normal for entries within 2^-17 of the chunk's largest,
+// and the rest off by at most 2^-40 of it; e goes to E[chunk][row].  The
+// rows of X (M rows, stride ldx complex) and Y (N rows) are planes' rows 0..M-1
+// and M..M+N-1; plane t of row r occupies P[t * sp + r * ldp + j] for the
+// ldp reals j (2K data, zeros after).  One thread converts 8 consecutive
+// reals.
+template <int NT, bool F16>
+__global__ __launch_bounds__(256) void k_split_planes(const float *__restrict__ X, long ldx, int M,
+                                                      const float *__restrict__ Y, long ldy, int N, int K, long ldp,
+                                                      long sp, unsigned short *__restrict__ P, int *__restrict__ E) {
+  const int r = blockIdx.x, chunk = blockIdx.y;
+  const float *x = r < M ? X + (long)r * 2 * ldx : Y + (long)(r - M) * 2 * ldy;
+  const long j0 = (long)chunk * kPlaneChunk + threadIdx.x * 8;
   const long nreal = 2L * K;
   float v[8];
+  if (j0 + 8 <= nreal && (((uintptr_t)(x + j0)) & 15) == 0) {
+    const float4 p = *reinterpret_cast<const float4 *>(x + j0), q = *reinterpret_cast<const float4 *>(x + j0 + 4);
+    v[0] = p.x;
+    v[1] = p.y;
+    v[2] = p.z;
+    v[3] = p.w;
+    v[4] = q.x;
+    v[5] = q.y;
+    v[6] = q.z;
+    v[7] = q.w;
+  } else {
 #pragma unroll
-  for (int i = 0; i < 8; i++)
-    v[i] = j0 + i < nreal ? x[j0 + i] : 0.f;
-  unsigned short h[NT][8];
+    for (int i = 0; i < 8; i++)
+      v[i] = j0 + i < nreal ? x[j0 + i] : 0.f;
+  }
+  uint4 w[NT];
+  if constexpr (F16) {
+    static_assert(NT == 2, "fp16 planes come in pairs");
+    __shared__ float wmax[4];
+    float m = 0.f;
 #pragma unroll
-  for (int i = 0; i < 8; i++) {
-    float rem = v[i];
+    for (int i = 0; i < 8; i++)
+      m = fmaxf(m, fabsf(v[i]));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+      m = fmaxf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0)
+      wmax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    m = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+    int ex = 0;
+    if (m > 0.f && m <= 3.4e38f)
+      (void)frexpf(m, &ex);  // m < 2^ex
+    const int e = m > 0.f && m <= 3.4e38f ? 15 - ex : 0;
+    if (threadIdx.x == 0)
+      E[(long)chunk * (M + N) + r] = e;
+    unsigned short h[2][8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const float y = ldexpf(v[i], e);
+      const _Float16 hi = (_Float16)y;
+      const _Float16 lo = (_Float16)(y - (float)hi);
+      h[0][i] = __builtin_bit_cast(unsigned short, hi);
+      h[1][i] = __builtin_bit_cast(unsigned short, lo);
+    }
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      w[q].x = (unsigned)h[q][0] | ((unsigned)h[q][1] << 16);
+      w[q].y = (unsigned)h[q][2] | ((unsigned)h[q][3] << 16);
+      w[q].z = (unsigned)h[q][4] | ((unsigned)h[q][5] << 16);
+      w[q].w = (unsigned)h[q][6] | ((unsigned)h[q][7] << 16);
+    }
+  } else {
+    unsigned short h[NT][8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      float rem = v[i];
+#pragma unroll
+      for (int q = 0; q < NT; q++) {
+        h[q][i] = bf16_bits(rem);
+        rem -= bf16_val(h[q][i]);
+      }
+    }
 #pragma unroll
     for (int q = 0; q < NT; q++) {
-      h[q][i] = bf16_bits(rem);
-      rem -= bf16_val(h[q][i]);
+      w[q].x = (unsigned)h[q][0] | ((unsigned)h[q][1] << 16);
+      w[q].y = (unsigned)h[q][2] | ((unsigned)h[q][3] << 16);
+      w[q].z = (unsigned)h[q][4] | ((unsigned)h[q][5] << 16);
+      w[q].w = (unsigned)h[q][6] | ((unsigned)h[q][7] << 16);
     }
   }
 #pragma unroll
-  for (int q = 0; q < NT; q++) {
-    uint4 w;
-    w.x = (unsigned)h[q][0] | ((unsigned)h[q][1] << 16);
-    w.y = (unsigned)h[q][2] | ((unsigned)h[q][3] << 16);
-    w.z = (unsigned)h[q][4] | ((unsigned)h[q][5] << 16);
-    w.w = (unsigned)h[q][6] | ((unsigned)h[q][7] << 16);
-    *reinterpret_cast<uint4 *>(P + q * sp + r * ldp + j0) = w;
+  for (int q = 0; q < NT; q++)
+    *reinterpret_cast<uint4 *>(P + q * sp + (long)r * ldp + j0) = w[q];
+}
+
+// (re, im) 16-bit pairs of a B row -> the pairs (im, -re) of B'': swap the
+// halves of each 32-bit word and flip the sign of the new upper half (the
+// sign is bit 15 in bf16 and fp16 alike).
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ u32x4 qswap(u32x4 b) { return ((b >> 16) | (b << 16)) ^ 0x80000000u; }
+
+template <bool F16>
+__device__ __forceinline__ floatx4 mfma16(u32x4 a, u32x4 b, floatx4 c) {
+  if constexpr (F16) {
+    typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(halfx8, a), __builtin_bit_cast(halfx8, b), c, 0,
+                                                  0, 0);
+  } else {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0,
+                                                   0, 0);
   }
 }
 
-// (re, im) bf16 pairs of a B row -> the pairs (im, -re) of B'': swap the
-// halves of each 32-bit word and flip the sign of the new upper half.
-__device__ __forceinline__ bf16x8 qswap(uint4 b) {
-  uint4 q;
-  q.x = ((b.x >> 16) | (b.x << 16)) ^ 0x80000000u;
-  q.y = ((b.y >> 16) | (b.y << 16)) ^ 0x80000000u;
-  q.z = ((b.z >> 16) | (b.z << 16)) ^ 0x80000000u;
-  q.w = ((b.w >> 16) | (b.w << 16)) ^ 0x80000000u;
-  return __builtin_bit_cast(bf16x8, q);
-}
-
-// Complex conj(A) B^T from NT bf16 planes of each operand (k_split_bf16),
-// on v_mfma_f32_16x16x32_bf16: every product of planes (i, j) with
-// i + j < NT, the smallest terms first (NT = 2: 3 products, 16 significant
-// bits per operand; NT = 3: 6 products, 24 bits, FP32-like).  A tile's
-// planes go global -> registers -> LDS as 16-byte accesses, the next
-// k-step's loads in flight during the current step's MFMAs; B'' for the
-// imaginary part is formed from B's registers (qswap).  Planes hold
-// zeros past the data, so chunks need no k bounds.  FOLD: FP32 sums folded
-// into FP64 every 512 complex terms.
-template <int NT, bool FOLD>
-__global__ __launch_bounds__(256) void k_cgemm_bf16p(int M, int N, int kchunk, int K, int nsplit,
-                                                     const unsigned short *__restrict__ A, long lda, long spa,
-                                                     const unsigned short *__restrict__ B, long ldb, long spb,
-                                                     cplx<double> *__restrict__ C, long ldc, long sSplit) {
-  __shared__ __attribute__((aligned(16))) unsigned short As[NT][64][72];
-  __shared__ __attribute__((aligned(16))) unsigned short Bs[NT][64][72];
-  const int tilesN = (N + 63) / 64;
+// Complex conj(A) B^T from NT 16-bit planes of each operand (k_split_planes)
+// on v_mfma_f32_16x16x32_{bf16,f16}: every product of planes (i, j) with
+// i + j < NT, the smallest terms first (NT = 2: 3 products; NT = 3: 6).
+// Work-group: 4 wavefronts, 128x128 complex outputs, each wavefront a 64x64
+// quadrant as 4x4 16x16 MFMA tiles (real and imaginary accumulators, 128
+// registers), so that every 16-byte operand read from LDS feeds 16 or 32
+// MFMAs and the global operand traffic per output is half that of a 64x64
+// tile.  A k-step of KS reals per row: its planes go global -> registers
+// -> LDS as 16-byte accesses into one of two LDS buffers while the other
+// buffer's step is computed, one barrier per step.  Planes hold zeros past
+// the data, so a chunk of CH reals needs no k bounds.  The chunk's FP32
+// sums are its partial sums (row stride ldc, split stride sSplit); fp16
+// planes carry per-(chunk, row) scale exponents EA / EB, undone exactly at
+// the store.
+template <int NT, int KS, bool F16, int CH>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_cgemm_planes(int M, int N, int nsplit, const unsigned short *__restrict__ A,
+                                                      const unsigned short *__restrict__ B, long ldp, long sp,
+                                                      const int *__restrict__ EA, const int *__restrict__ EB,
+                                                      long sE, cplx<float> *__restrict__ C, long ldc, long sSplit) {
+  constexpr int LS = KS + 8;           // LDS row stride (16-bit words): 16 bytes of padding
+  constexpr int HP = KS / 32;          // 16-byte loads per (row, plane) per thread
+  __shared__ __attribute__((aligned(16))) unsigned short As[2][NT][128][LS];
+  __shared__ __attribute__((aligned(16))) unsigned short Bs[2][NT][128][LS];
+  const int tilesN = (N + 127) / 128;
   int split, tile;
-  if (!gemm_block(((M + 63) / 64) * tilesN, nsplit, split, tile))
+  if (!gemm_block(((M + 127) / 128) * tilesN, nsplit, split, tile))
     return;
   const int tm = tile / tilesN, tn = tile % tilesN;
-  const int k0 = split * kchunk;
-  const int k1 = min(K, k0 + kchunk);
-  C += (long)split * sSplit;
+  const long k0 = (long)split * CH;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  floatx4 cr[2][2], ci[2][2];
-  double dr[2][2][4], di[2][2][4];
+  const int lrow = threadIdx.x >> 2, part = threadIdx.x & 3;
+  // This thread's rows of the tile: lrow and lrow + 64 of A and of B.  A row
+  // past the matrix reads the last row instead: it only feeds outputs of
+  // rows (columns) past the matrix, which are not stored.
+  const unsigned short *pa0 = A + (long)min(tm * 128 + lrow, M - 1) * ldp + k0 + part * (KS / 4);
+  const unsigned short *pa1 = A + (long)min(tm * 128 + 64 + lrow, M - 1) * ldp + k0 + part * (KS / 4);
+  const unsigned short *pb0 = B + (long)min(tn * 128 + lrow, N - 1) * ldp + k0 + part * (KS / 4);
+  const unsigned short *pb1 = B + (long)min(tn * 128 + 64 + lrow, N - 1) * ldp + k0 + part * (KS / 4);
+  // global -> register staging of one k-step: [row half][plane][16-byte piece]
+  u32x4 ga0[NT][HP], ga1[NT][HP], gb0[NT][HP], gb1[NT][HP];
+#define PP_FETCH(step)                                                                                           \
+  {                                                                                                              \
+    const long off_ = (long)(step) * KS;                                                                         \
+    _Pragma("unroll") for (int t = 0; t < NT; t++) _Pragma("unroll") for (int q = 0; q < HP; q++) {             \
+      ga0[t][q] = *reinterpret_cast<const u32x4 *>(pa0 + t * sp + off_ + q * 8);                                 \
+      ga1[t][q] = *reinterpret_cast<const u32x4 *>(pa1 + t * sp + off_ + q * 8);                                 \
+      gb0[t][q] = *reinterpret_cast<const u32x4 *>(pb0 + t * sp + off_ + q * 8);                                 \
+      gb1[t][q] = *reinterpret_cast<const u32x4 *>(pb1 + t * sp + off_ + q * 8);                                 \
+    }                                                                                                            \
+  }
+#define PP_STASH(buf)                                                                                            \
+  {                                                                                                              \
+    _Pragma("unroll") for (int t = 0; t < NT; t++) _Pragma("unroll") for (int q = 0; q < HP; q++) {             \
+      *reinterpret_cast<u32x4 *>(&As[buf][t][lrow][part * (KS / 4) + q * 8]) = ga0[t][q];                        \
+      *reinterpret_cast<u32x4 *>(&As[buf][t][64 + lrow][part * (KS / 4) + q * 8]) = ga1[t][q];                   \
+      *reinterpret_cast<u32x4 *>(&Bs[buf][t][lrow][part * (KS / 4) + q * 8]) = gb0[t][q];                        \
+      *reinterpret_cast<u32x4 *>(&Bs[buf][t][64 + lrow][part * (KS / 4) + q * 8]) = gb1[t][q];                   \
+    }                                                                                                            \
+  }
+  floatx4 cr[4][4], ci[4][4];
 #pragma unroll
-  for (int a = 0; a < 2; a++)
+  for (int a = 0; a < 4; a++)
 #pragma unroll
-    for (int b = 0; b < 2; b++) {
+    for (int b = 0; b < 4; b++) {
       cr[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
       ci[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int v = 0; v < 4; v++)
-        dr[a][b][v] = di[a][b][v] = 0.0;
     }
-  const int lrow = threadIdx.x >> 2, part = threadIdx.x & 3;
-  const int ra = tm * 64 + lrow, rb = tn * 64 + lrow;
-  const uint4 zero = {0u, 0u, 0u, 0u};
-  uint4 pa[NT][2], pb[NT][2];
-  auto fetch = [&](int kb) {
-    const long off = 2L * kb + part * 16;
-#pragma unroll
-    for (int t = 0; t < NT; t++)
-#pragma unroll
-      for (int h = 0; h < 2; h++) {
-        pa[t][h] = ra < M ? *reinterpret_cast<const uint4 *>(A + t * spa + (long)ra * lda + off + h * 8) : zero;
-        pb[t][h] = rb < N ? *reinterpret_cast<const uint4 *>(B + t * spb + (long)rb * ldb + off + h * 8) : zero;
-      }
-  };
-  if (k0 < k1)
-    fetch(k0);
-  int since = 0;
+  constexpr int nsteps = CH / KS;
+  static_assert(nsteps >= 2, "at least two k-steps per chunk");
   const int sub = lane & 15, kq = lane >> 4;
-  for (int kb = k0; kb < k1; kb += 32) {
+  PP_FETCH(0);
+  PP_STASH(0);
+  __syncthreads();
+  PP_FETCH(1);
+  // step s: compute from LDS buffer s & 1 while step s + 1 goes from
+  // registers into the other buffer and step s + 2 is loaded (the last
+  // steps re-load the chunk's last step and fill a buffer no one reads)
+  for (int s = 0; s < nsteps; s++) {
+    const int buf = s & 1;
 #pragma unroll
-    for (int t = 0; t < NT; t++)
-#pragma unroll
-      for (int h = 0; h < 2; h++) {
-        *reinterpret_cast<uint4 *>(&As[t][lrow][part * 16 + h * 8]) = pa[t][h];
-        *reinterpret_cast<uint4 *>(&Bs[t][lrow][part * 16 + h * 8]) = pb[t][h];
-      }
-    __syncthreads();
-    if (kb + 32 < k1)
-      fetch(kb + 32);
-#pragma unroll
-    for (int kk = 0; kk < 64; kk += 32) {
+    for (int kk = 0; kk < KS; kk += 32) {
       const int kc = kk + kq * 8;
-      bf16x8 a[NT][2];
+      u32x4 a[NT][4];
 #pragma unroll
       for (int t = 0; t < NT; t++)
 #pragma unroll
-        for (int mi = 0; mi < 2; mi++)
-          a[t][mi] = ld8(&As[t][wm * 32 + mi * 16 + sub][kc]);
+        for (int mi = 0; mi < 4; mi++)
+          a[t][mi] = *reinterpret_cast<const u32x4 *>(&As[buf][t][wm * 64 + mi * 16 + sub][kc]);
 #pragma unroll
-      for (int nj = 0; nj < 2; nj++) {
-        const int col = wn * 32 + nj * 16 + sub;
-        bf16x8 b[NT], q[NT];
+      for (int nj = 0; nj < 4; nj++) {
+        const int col = wn * 64 + nj * 16 + sub;
+        u32x4 b[NT], q[NT];
 #pragma unroll
         for (int t = 0; t < NT; t++) {
-          const uint4 raw = *reinterpret_cast<const uint4 *>(&Bs[t][col][kc]);
-          b[t] = __builtin_bit_cast(bf16x8, raw);
-          q[t] = qswap(raw);
+          b[t] = *reinterpret_cast<const u32x4 *>(&Bs[buf][t][col][kc]);
+          q[t] = qswap(b[t]);
         }
 #pragma unroll
-        for (int mi = 0; mi < 2; mi++) {
-          // plane pairs (i, j), i + j < NT, smallest terms first
+        for (int mi = 0; mi < 4; mi++) {
 #pragma unroll
           for (int d = NT - 1; d >= 0; d--)
 #pragma unroll
             for (int i = d; i >= 0; i--) {
-              const int j = d - i;
-              cr[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][mi], b[j], cr[mi][nj], 0, 0, 0);
-              ci[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][mi], q[j], ci[mi][nj], 0, 0, 0);
+              cr[mi][nj] = mfma16<F16>(a[i][mi], b[d - i], cr[mi][nj]);
+              ci[mi][nj] = mfma16<F16>(a[i][mi], q[d - i], ci[mi][nj]);
             }
         }
       }
     }
+    PP_STASH(buf ^ 1);
+    PP_FETCH(min(s + 2, nsteps - 1));
     __syncthreads();
-    if (FOLD && ++since == 16) {
-      since = 0;
+  }
+#undef PP_FETCH
+#undef PP_STASH
+  C += (long)split * sSplit;
 #pragma unroll
-      for (int a = 0; a < 2; a++)
+  for (int mi = 0; mi < 4; mi++) {
+    const int r0 = tm * 128 + wm * 64 + mi * 16 + 4 * (lane >> 4);
 #pragma unroll
-        for (int b = 0; b < 2; b++) {
+    for (int nj = 0; nj < 4; nj++) {
+      const int j = tn * 128 + wn * 64 + nj * 16 + (lane & 15);
+      if (j >= N)
+        continue;
+      // the scale exponents of the plane chunk this K chunk lies in
+      const long ec = (long)split * CH / kPlaneChunk * sE;
+      const int eb = F16 ? EB[ec + j] : 0;
 #pragma unroll
-          for (int v = 0; v < 4; v++) {
-            dr[a][b][v] += (double)cr[a][b][v];
-            di[a][b][v] += (double)ci[a][b][v];
+      for (int v = 0; v < 4; v++) {
+        const int i = r0 + v;
+        if (i < M) {
+          float re = cr[mi][nj][v], im = ci[mi][nj][v];
+          if constexpr (F16) {
+            const int e = EA[ec + i] + eb;
+            re = ldexpf(re, -e);
+            im = ldexpf(im, -e);
           }
-          cr[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
-          ci[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+          C[(long)i * ldc + j] = mk<float>(re, im);
         }
+      }
     }
   }
-#pragma unroll
-  for (int mi = 0; mi < 2; mi++)
-#pragma unroll
-    for (int nj = 0; nj < 2; nj++)
-      store_tile64(C, ldc, M, N, tm * 64 + wm * 32 + mi * 16, tn * 64 + wn * 32 + nj * 16, dr[mi][nj], di[mi][nj],
-                   cr[mi][nj], ci[mi][nj], lane);
 }
 
 int splitk_count(int precision, int M, int N, int K, int tune);
 int launch_splitk_reduce(hipStream_t st, int M, int N, int nsplit, const void *P, bool fp32_partials,
                          cplx<double> *C, long ldc, bool accumulate);
 
-// K chunk (complex terms) of an FP32 / 3xBF16 GEMM split nsplit ways: a
-// multiple of the kernel's k-step; nsplit is reduced to the chunks needed.
+// K chunk (complex terms) of an FP32 GEMM split nsplit ways: a multiple
+// of the kernel's k-step; nsplit is reduced to the chunks needed.
 int cgemm_chunk(int precision, int K, int &nsplit) {
-  const int kstep = (precision == PP_BF16X3 || precision == PP_BF16X6) ? 32 : 16;
+  const int kstep = 16;
   int kchunk = (K + nsplit - 1) / std::max(1, nsplit);
   kchunk = (kchunk + kstep - 1) / kstep * kstep;
   if (kchunk == 0)
@@ -411,47 +496,64 @@ int launch_cgemm32(hipStream_t st, int precision, int M, int N, int K, const voi
   return PP_OK;
 }
 
-// Split-bf16 GEMM of one batch (PP_BF16X3: two planes per operand, 3
-// products; PP_BF16X6: three planes, 6 products): both operands split once
-// into the plane buffer `planes` ((M + N) * NT * ldp shorts, ldp = 2K
-// rounded up to 64), then the plane GEMM into partials P as launch_cgemm32.
-int launch_cgemm_bf16(hipStream_t st, int precision, int M, int N, int K, const void *A, long lda, const void *B,
-                      long ldb, unsigned short *planes, cplx<double> *P, long ldp_out, long sSplit, int nsplit,
-                      int kchunk) {
-  const int NT = precision == PP_BF16X6 ? 3 : 2;
-  const long ldp = (2L * K + 63) / 64 * 64;
-  unsigned short *PA = planes, *PB = planes + (size_t)NT * M * ldp;
-  const long nA = (long)M * (ldp / 8), nB = (long)N * (ldp / 8);
-  if (NT == 3) {
-    hipLaunchKernelGGL(k_split_bf16<3>, dim3((unsigned)((nA + 255) / 256)), dim3(256), 0, st, (const float *)A, lda, M,
-                       K, ldp, (long)M * ldp, PA);
-    hipLaunchKernelGGL(k_split_bf16<3>, dim3((unsigned)((nB + 255) / 256)), dim3(256), 0, st, (const float *)B, ldb, N,
-                       K, ldp, (long)N * ldp, PB);
-  } else {
-    hipLaunchKernelGGL(k_split_bf16<2>, dim3((unsigned)((nA + 255) / 256)), dim3(256), 0, st, (const float *)A, lda, M,
-                       K, ldp, (long)M * ldp, PA);
-    hipLaunchKernelGGL(k_split_bf16<2>, dim3((unsigned)((nB + 255) / 256)), dim3(256), 0, st, (const float *)B, ldb, N,
-                       K, ldp, (long)N * ldp, PB);
-  }
+bool plane_precision(int precision) {
+  return precision == PP_BF16X3 || precision == PP_BF16X6 || precision == PP_FP16X3;
+}
+
+// Layout of a plane GEMM's workspace and partial sums: ldp reals per plane
+// row (2K rounded up to whole plane chunks of 1024 complex terms), NT planes
+// of M + N rows, then the fp16 scale exponents [plane chunk][M + N].  The
+// GEMM accumulates K chunks of 512 complex terms in FP32 (two per plane
+// chunk), one partial-sum plane each, summed in FP64: FP32 sums over
+// 1024 terms of 3 products per k-step would round more than the FP32
+// GEMM's 512-term chunks do.
+void plane_layout(int precision, int M, int N, int K, long &ldp, int &nt, int &nsplit, size_t &ws_bytes) {
+  nt = precision == PP_BF16X6 ? 3 : 2;
+  ldp = (2L * K + kPlaneChunk - 1) / kPlaneChunk * kPlaneChunk;
+  nsplit = (int)std::max<long>(1, ldp / (kPlaneChunk / 2));
+  const long nchunks = ldp / kPlaneChunk;
+  ws_bytes = sizeof(unsigned short) * (size_t)nt * (M + N) * ldp;
+  if (precision == PP_FP16X3)
+    ws_bytes += sizeof(int) * (size_t)nchunks * (M + N);
+}
+
+// Split of both operands into planes (ws as plane_layout) and the plane
+// GEMM into FP32 partials P ([nsplit][M][ldp_out], split stride sSplit).
+// No allocation.
+int launch_cgemm_planes(hipStream_t st, int precision, int M, int N, int K, const void *A, long lda, const void *B,
+                        long ldb, void *ws, cplx<float> *P, long ldp_out, long sSplit) {
+  long ldp;
+  int NT, nsplit;
+  size_t wsb;
+  plane_layout(precision, M, N, K, ldp, NT, nsplit, wsb);
+  const long nchunks = ldp / kPlaneChunk;
+  PP_REQUIRE(nchunks <= 65535, PP_ERR_ARG, "plane GEMM: K too large");
+  unsigned short *PA = (unsigned short *)ws, *PB = PA + (size_t)M * ldp;
+  const long sp = (long)(M + N) * ldp;
+  int *E = (int *)(PA + (size_t)NT * sp);
+  const dim3 sg((unsigned)(M + N), (unsigned)nchunks);
+  if (precision == PP_FP16X3)
+    hipLaunchKernelGGL((k_split_planes<2, true>), sg, dim3(256), 0, st, (const float *)A, lda, M, (const float *)B,
+                       ldb, N, K, ldp, sp, PA, E);
+  else if (NT == 3)
+    hipLaunchKernelGGL((k_split_planes<3, false>), sg, dim3(256), 0, st, (const float *)A, lda, M, (const float *)B,
+                       ldb, N, K, ldp, sp, PA, nullptr);
+  else
+    hipLaunchKernelGGL((k_split_planes<2, false>), sg, dim3(256), 0, st, (const float *)A, lda, M, (const float *)B,
+                       ldb, N, K, ldp, sp, PA, nullptr);
   PP_CHECK_HIP(hipGetLastError());
-  const int tiles = ((M + 63) / 64) * ((N + 63) / 64);
+  const int tiles = ((M + 127) / 128) * ((N + 127) / 128);
   const dim3 grd = gemm_grid(tiles, nsplit);
-  const bool fold = kchunk > 512;
-#define PP_BP(NTV, F)                                                                                            \
-  hipLaunchKernelGGL((k_cgemm_bf16p<NTV, F>), grd, dim3(256), 0, st, M, N, kchunk, K, nsplit, PA, ldp, (long)M * ldp, \
-                     PB, ldp, (long)N * ldp, P, ldp_out, sSplit)
-  if (NT == 3) {
-    if (fold)
-      PP_BP(3, true);
-    else
-      PP_BP(3, false);
-  } else {
-    if (fold)
-      PP_BP(2, true);
-    else
-      PP_BP(2, false);
-  }
-#undef PP_BP
+  const long sE = M + N;
+  if (precision == PP_FP16X3)
+    hipLaunchKernelGGL((k_cgemm_planes<2, 64, true, kPlaneChunk / 2>), grd, dim3(256), 0, st, M, N, nsplit, PA, PB,
+                       ldp, sp, E, E + M, sE, P, ldp_out, sSplit);
+  else if (NT == 3)
+    hipLaunchKernelGGL((k_cgemm_planes<3, 32, false, kPlaneChunk / 2>), grd, dim3(256), 0, st, M, N, nsplit, PA, PB,
+                       ldp, sp, nullptr, nullptr, sE, P, ldp_out, sSplit);
+  else
+    hipLaunchKernelGGL((k_cgemm_planes<2, 64, false, kPlaneChunk / 2>), grd, dim3(256), 0, st, M, N, nsplit, PA, PB,
+                       ldp, sp, nullptr, nullptr, sE, P, ldp_out, sSplit);
   PP_CHECK_HIP(hipGetLastError());
   return PP_OK;
 }
@@ -463,22 +565,30 @@ int launch_cgemm_bf16(hipStream_t st, int precision, int M, int N, int K, const 
 // has read its partials.  The caller joins the second stream at the end.
 int gemm_fp32_mfma(pp_ctx *ctx, int precision, int M, int N, int K, const void *A, long lda, const void *B, long ldb,
                    cplx<double> *C, long ldc, bool accumulate, int slot) {
-  int nsplit = splitk_count(precision, M, N, K, ctx->tune_splitk);
-  const int kchunk = cgemm_chunk(precision, K, nsplit);
+  const bool planes = plane_precision(precision);
+  int nsplit, kchunk = 0;
+  long ldp;
+  int nt;
+  size_t wsb = 0;
+  if (planes) {
+    plane_layout(precision, M, N, K, ldp, nt, nsplit, wsb);
+  } else {
+    nsplit = splitk_count(precision, M, N, K, ctx->tune_splitk);
+    kchunk = cgemm_chunk(precision, K, nsplit);
+  }
+  const bool fp32p = planes || fp32_partials_for(precision, kchunk);
   DevBuf &ws = slot == 1 ? ctx->gemm_ws2 : ctx->gemm_ws;
-  PP_TRY(ws.ensure(sizeof(cplx<double>) * (size_t)nsplit * M * N));
-  cplx<double> *P = (cplx<double> *)ws.p;
+  PP_TRY(ws.ensure((fp32p ? sizeof(cplx<float>) : sizeof(cplx<double>)) * (size_t)nsplit * M * N));
+  void *P = ws.p;
   if (slot >= 0)
     PP_CHECK_HIP(hipStreamWaitEvent(ctx->stream, ctx->red_done[slot], 0));
   hipEvent_t ev;
   int st;
-  if (precision == PP_BF16X3 || precision == PP_BF16X6) {
-    const int NT = precision == PP_BF16X6 ? 3 : 2;
-    const long ldp = (2L * K + 63) / 64 * 64;
-    PP_TRY(ctx->bf16_ws.ensure(sizeof(unsigned short) * (size_t)NT * (M + N) * ldp));
+  if (planes) {
+    PP_TRY(ctx->bf16_ws.ensure(wsb));
     tbegin(ctx, K_GEMM, &ev);
-    st = launch_cgemm_bf16(ctx->stream, precision, M, N, K, A, lda, B, ldb, (unsigned short *)ctx->bf16_ws.p, P,
-                           (long)N, (long)M * N, nsplit, kchunk);
+    st = launch_cgemm_planes(ctx->stream, precision, M, N, K, A, lda, B, ldb, ctx->bf16_ws.p, (cplx<float> *)P,
+                             (long)N, (long)M * N);
   } else {
     tbegin(ctx, K_GEMM, &ev);
     st = launch_cgemm32(ctx->stream, precision, M, N, K, A, lda, B, ldb, P, (long)N, (long)M * N, nsplit, kchunk);
@@ -492,7 +602,7 @@ int gemm_fp32_mfma(pp_ctx *ctx, int precision, int M, int N, int K, const void *
     ctx->stream = ctx->aux;
   }
   tbegin(ctx, K_GEMM_REDUCE, &ev);
-  st = launch_splitk_reduce(ctx->stream, M, N, nsplit, P, fp32_partials_for(precision, kchunk), C, ldc, accumulate);
+  st = launch_splitk_reduce(ctx->stream, M, N, nsplit, P, fp32p, C, ldc, accumulate);
   tend(ctx, K_GEMM_REDUCE, ev);
   ctx->stream = main;
   PP_TRY(st);

@@ -278,7 +278,7 @@ static int multi_check(pp_ctx *ctx, const MultiArgs &a, int precision) {
   PP_REQUIRE(ctx && a.nstruct > 0 && a.plans && a.nk > 0 && a.kfrac && a.gofs && a.nband && a.C && a.ldc && a.proj,
              PP_ERR_ARG, "pp_setup_projections_multi: bad arguments");
   PP_REQUIRE(precision == PP_FP64 || precision == PP_FP32 || precision == PP_BF16X3 ||
-                 precision == PP_BF16X6,
+                 precision == PP_BF16X6 || precision == PP_FP16X3,
              PP_ERR_ARG,
              "pp_setup_projections_multi: unknown precision");
   const int want = precision == PP_FP64 ? PP_FP64 : PP_FP32;

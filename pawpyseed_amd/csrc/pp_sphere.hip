@@ -1044,7 +1044,7 @@ extern "C" int pp_sphere_create(pp_ctx *ctx, pp_sphere **out, const double *latt
   PP_REQUIRE(ctx && out && lattice && fftg && num_sites >= 0 && num_elems > 0, PP_ERR_ARG,
              "pp_sphere_create: bad arguments");
   PP_REQUIRE(precision == PP_FP64 || precision == PP_FP32 || precision == PP_BF16X3 ||
-                 precision == PP_BF16X6,
+                 precision == PP_BF16X6 || precision == PP_FP16X3,
              PP_ERR_ARG,
              "pp_sphere_create: unknown precision");
   PP_CHECK_HIP(hipSetDevice(ctx->device));

@@ -183,7 +183,8 @@ def test_near_orthogonal_pairs_216_atoms(big):
     rows_b = sorted({r for _, r, _, _ in planted[:9]} | {0})
     rows, _ = _exact_rows_cols(d, CS_h, rows_b, [0])
     report = {}
-    for precision, tol_norm in (("fp64", 1e-10), ("fp32", 1e-6), ("bf16x3", 2e-5), ("bf16x6", 1e-6)):
+    for precision, tol_norm in (("fp64", 1e-10), ("fp32", 1e-6), ("bf16x3", 2e-5), ("bf16x6", 1e-6),
+                                ("fp16x3", 1e-6)):
         out = torch.zeros((d["nk"], d["nb"], d["nb"]), dtype=torch.complex128, device="cuda")
         B.overlap_gemm(d["ctx"], d["nk"], 1, False, d["nw"], d["CR"], CS, out, precision)
         o = out.cpu().numpy()

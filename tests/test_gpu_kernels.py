@@ -3,7 +3,7 @@
 Tolerances (written per test): FP64 path <= 1e-9 relative (normwise,
 max|d| / max|ref|) against the FP64 oracle -- in practice ~1e-14; FP32
 path <= 1e-5 for FFT / projections and <= 1e-6 for the overlap GEMM;
-3xBF16 overlap <= 2e-5.
+split-plane overlaps: bf16x3 <= 2e-5, bf16x6 and fp16x3 <= 1e-6.
 """
 
 import numpy as np
@@ -225,12 +225,50 @@ def test_overlap_fp64_spin_flip(ctx):
     assert np.abs(got - want).max() < 1e-12
 
 
-@pytest.mark.parametrize("precision,tol", [("fp32", 1e-6), ("bf16x3", 2e-5), ("bf16x6", 1e-6)])
+@pytest.mark.parametrize("precision,tol", [("fp32", 1e-6), ("bf16x3", 2e-5), ("bf16x6", 1e-6), ("fp16x3", 1e-6)])
 def test_overlap_mixed(ctx, precision, tol):
     got, want = _overlap_case(ctx, precision, nbR=130, nbS=67)
     err = normwise(got, want)
     print("overlap", precision, "normwise error", err)
     assert err < tol
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16x3", "bf16x6"])
+def test_overlap_split_planes_row_scales(ctx, precision):
+    """Split-plane overlaps of rows whose magnitudes span eleven decades (and
+    one zero row), 150 S bands (two 128-column tiles), two plane chunks of
+    K, against FP64: the fp16 split scales each 1024-term chunk of a row by
+    its own power of two, so no row loses accuracy to the others'
+    magnitudes.  With the overlaps divided by the row scales of both
+    operands, normwise error <= 1e-6 and every R row's own normwise error
+    <= 3e-6 (the FP32 GEMM, held to the same bounds, is the control); the
+    zero row's overlaps are exactly zero."""
+    from pawpyseed_amd.core import batched as B
+    wl = synthetic.tiny_config(encut=250.0, nband=20, kpts=[[0.1, 0.2, -0.3]])
+    rng = np.random.default_rng(23)
+    G = wl.ensure_gvecs()[0]
+    assert len(G) > 1024
+    c = wl.coefficients("R", rng=rng, nband=20)[0]
+    d = wl.coefficients("S", rng=rng, nband=150)[0]
+    scale_r = 10.0 ** rng.uniform(-8, 3, 20)
+    scale_r[5] = 0.0
+    scale_s = 10.0 ** rng.uniform(-8, 3, 150)
+    CR = [(c * scale_r[:, None]).astype(np.complex64)]
+    CS = [(d * scale_s[:, None]).astype(np.complex64)]
+    CRd, _, _, nw = device_inputs([G], CR)
+    CSd, _, _, _ = device_inputs([G], CS)
+    out = torch.zeros((1, 20, 150), dtype=torch.complex128, device="cuda")
+    B.overlap_gemm(ctx, 1, 1, False, nw, CRd, CSd, out, precision)
+    got = out.cpu().numpy()[0]
+    want = np.conj(CR[0].astype(np.complex128)) @ CS[0].astype(np.complex128).T
+    assert np.all(got[5] == 0)
+    rows = [m for m in range(20) if m != 5]
+    norm = np.outer(scale_r[rows], scale_s)
+    g, w = got[rows] / norm, want[rows] / norm
+    err = normwise(g, w)
+    row_err = max(np.linalg.norm(g[m] - w[m]) / np.linalg.norm(w[m]) for m in range(len(rows)))
+    print(f"{precision} row-scaled overlap: normwise {err:.2e}, worst row {row_err:.2e}", flush=True)
+    assert err < 1e-6 and row_err < 3e-6
 
 
 def test_overlap_near_orthogonal_pairs(ctx):
@@ -256,7 +294,7 @@ def test_overlap_near_orthogonal_pairs(ctx):
     want = np.stack([orc.pseudoprojection(CR, CS, b, 1, 1) for b in range(9)], axis=1)  # [bR, bS]
     diag = np.abs(np.diag(want))
     assert diag.min() < 1e-7 and diag.max() > 0.5  # the stored pairs really span the range
-    for precision, tol in (("fp64", None), ("fp32", 1e-6), ("bf16x3", 2e-5), ("bf16x6", 1e-6)):
+    for precision, tol in (("fp64", None), ("fp32", 1e-6), ("bf16x3", 2e-5), ("bf16x6", 1e-6), ("fp16x3", 1e-6)):
         out = torch.zeros((1, 9, 9), dtype=torch.complex128, device="cuda")
         B.overlap_gemm(ctx, 1, 1, False, nw, CRd, CSd, out, precision)
         got = out.cpu().numpy()[0]

@@ -73,7 +73,7 @@ def test_spinor_projections_large_cell(ctx, case, precision, tol):
     plan.close()
 
 
-@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", 1e-6), ("bf16x6", 1e-6)])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", 1e-6), ("bf16x6", 1e-6), ("fp16x3", 1e-6)])
 def test_spinor_overlap_large_cell(ctx, case, precision, tol):
     """The band overlap of spinor bands (pseudoprojector.c:63-90 over the
     reference band's num_waves = 2 W, reader.c:274-281: the up then the down
