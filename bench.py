@@ -17,7 +17,9 @@ One step = the full projection of S onto R:
 The basis R is set up once before timing (one basis serves many projected
 structures).  Paths: "fp32" (FFT, sphere sums and overlap GEMM in FP32 on
 MFMA, FP64 reductions and augmentation) is the headline `value`; "fp64" is
-the plain FP64 path; "bf16x3" re-runs the overlap GEMM as split-bf16 MFMA.
+the plain FP64 path.  The overlap GEMM is re-run as split-plane products on
+MFMA: "fp16x3" (two power-of-two-scaled fp16 planes per operand, 3
+products: the headline overlap TFLOP/s), "bf16x3" and "bf16x6".
 
 Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
 """
@@ -37,7 +39,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM = 8.0e12
-PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16": 2.5e15}
+PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16": 2.5e15, "fp16": 2.5e15}
+# split-plane overlap modes: MFMA arithmetic and products per complex term
+SPLIT_MODES = {"fp16x3": ("fp16", 3), "bf16x3": ("bf16", 3), "bf16x6": ("bf16", 6)}
 
 
 def log(*a):
@@ -115,16 +119,18 @@ def read_traffic(path_name, settings):
     summary kept with the profiles (FETCH_SIZE and WRITE_SIZE per dispatch,
     scaled by the factors a plain 1 GiB device copy reads on the same
     counters), for the path and the settings that change what a launch
-    moves (fft_xmask: the x pass writes only covered points).  The summary
-    keys its entries "<path>/fft_xmask=<0|1>".  Empty when there is no
-    entry for those settings."""
+    moves (fft_xmask: the x pass writes only covered points; sphere_parts:
+    the sphere sums' partial rows).  The summary keys its entries
+    "<path>/fft_xmask=<x>,sphere_parts=<p>".  Empty when there is no entry
+    for those settings."""
     try:
         txt = open(TRAFFIC_FILE).read()
         blk = txt.split("```json", 1)[1].split("```", 1)[0]
         d = json.loads(blk)
     except Exception:
         return {}
-    key = f"{path_name}/fft_xmask={int(settings.get('fft_xmask', 0))}"
+    key = (f"{path_name}/fft_xmask={int(settings.get('fft_xmask', 0))},"
+           f"sphere_parts={int(settings.get('sphere_parts', 0))}")
     return {k: float(v) for k, v in d.get("traffic_bytes_per_launch", {}).get(key, {}).items()}
 
 
@@ -229,11 +235,12 @@ def main():
         torch.cuda.synchronize()
         b.copy_(a)  # calibration: reads and writes exactly 1 GiB
         torch.cuda.synchronize()
+        log("profile step settings", json.dumps({n: ctx.get_param(n) for n in B.TUNED}))
         evs = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
         step(paths[0], evs)
         # the overlap GEMM of every other arithmetic once (their MFMA counters)
         ob = torch.zeros((nk, nb, nb), dtype=torch.complex128, device="cuda")
-        for prec in ("fp64", "bf16x3", "bf16x6"):
+        for prec in ("fp64", "fp16x3", "bf16x3", "bf16x6"):
             with torch.cuda.stream(stream):
                 B.overlap_gemm(ctx, nk, 1, False, nw, CR, CS, ob, prec)
         torch.cuda.synchronize()
@@ -307,19 +314,33 @@ def main():
         log(p, json.dumps({k: v for k, v in results[p].items() if k != "kernels"}))
         log(p, "kernels (breakdown step)", json.dumps(results[p]["kernels"]))
 
-    # overlap as 3-term split bf16 on MFMA (same inputs)
-    ob = torch.zeros((nk, nb, nb), dtype=torch.complex128, device="cuda")
-    with torch.cuda.stream(stream):
-        B.overlap_gemm(ctx, nk, 1, False, nw, CR, CS, ob, "bf16x3")  # warm
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        for _ in range(args.steps):
-            B.overlap_gemm(ctx, nk, 1, False, nw, CR, CS, ob, "bf16x3", sync=False)
-        e1.record(stream)
-    torch.cuda.synchronize()
-    t_bf = e0.elapsed_time(e1) / 1e3 / args.steps
-    results["bf16x3"] = dict(overlap_s=t_bf, overlap_tflops=8.0 * nb * nb * float(nw.sum()) / t_bf / 1e12)
-    log("bf16x3 overlap", results["bf16x3"])
+    # the overlap as split-plane products on MFMA (same inputs): fp16x3 (two
+    # scaled fp16 planes, 3 products), bf16x3 and bf16x6; timed over
+    # args.steps calls of all k, then one call with per-launch events for
+    # the split / GEMM / reduction breakdown
+    ob = {}
+    ovl_flops = 8.0 * nb * nb * float(nw.sum())
+    for mode in SPLIT_MODES:
+        ob[mode] = torch.zeros((nk, nb, nb), dtype=torch.complex128, device="cuda")
+        with torch.cuda.stream(stream):
+            B.overlap_gemm(ctx, nk, 1, False, nw, CR, CS, ob[mode], mode)  # warm
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(args.steps):
+                B.overlap_gemm(ctx, nk, 1, False, nw, CR, CS, ob[mode], mode, sync=False)
+            e1.record(stream)
+        torch.cuda.synchronize()
+        t_m = e0.elapsed_time(e1) / 1e3 / args.steps
+        ctx.set_timing(True)
+        ctx.timing_reset()
+        with torch.cuda.stream(stream):
+            B.overlap_gemm(ctx, nk, 1, False, nw, CR, CS, ob[mode], mode)
+        torch.cuda.synchronize()
+        kt = ctx.timing()
+        ctx.set_timing(False)
+        results[mode] = dict(overlap_s=t_m, overlap_tflops=ovl_flops / t_m / 1e12,
+                             kernels={k: {"ms_total": v[0], "launches": v[1]} for k, v in kt.items()})
+        log(mode, "overlap", json.dumps(results[mode]))
 
     head = "fp32" if "fp32" in results else paths[0]
     R = results[head]
@@ -406,6 +427,27 @@ def main():
                       "peak": PEAK["fp32" if head != "fp64" else "fp64"] / 1e12, "unit": "TFLOP/s",
                       "alg_flops": gemm_flops, "avg_launch_ms": ms, "ms_per_step": kc["ms_total"],
                       "traffic": traffic.get("gemm")}
+    # split-plane overlap GEMMs: the plane GEMM on the MFMA peak of its
+    # 16-bit arithmetic (products x 8 M N K flop per k-point), and the split
+    # of both operands (read 8 B, write 2 B per plane per real of each c64)
+    for mode, (arith, nprod) in SPLIT_MODES.items():
+        kt = results.get(mode, {}).get("kernels", {})
+        kc, ks = kt.get("gemm"), kt.get("gemm_split")
+        if kc and kc["launches"]:
+            ms = kc["ms_total"] / kc["launches"]
+            rl[f"gemm_{mode}"] = {"bound": "mfma", "achieved": nprod * gemm_flops / (ms * 1e-3) / 1e12,
+                                  "peak": PEAK[arith] / 1e12, "unit": "TFLOP/s", "alg_flops": nprod * gemm_flops,
+                                  "tflops_equivalent": gemm_flops / (ms * 1e-3) / 1e12, "avg_launch_ms": ms,
+                                  "ms_per_step": kc["ms_total"], "traffic": traffic.get(f"gemm_{mode}"),
+                                  "what": f"plane GEMM of one k-point ({nprod} {arith} products per complex term)"}
+        if ks and ks["launches"]:
+            ms = ks["ms_total"] / ks["launches"]
+            nplanes = 3 if mode == "bf16x6" else 2
+            sb = (2.0 * nb) * float(nw.mean()) * (8.0 + 4.0 * nplanes)
+            rl[f"gemm_split_{mode}"] = {"bound": "hbm", "achieved": sb / (ms * 1e-3) / 1e9, "peak": PEAK_HBM / 1e9,
+                                        "unit": "GB/s", "alg_bytes": sb, "avg_launch_ms": ms,
+                                        "ms_per_step": ks["ms_total"], "traffic": traffic.get(f"gemm_split_{mode}"),
+                                        "what": "plane split of both operands of one k-point"}
     for v in rl.values():
         v["frac"] = v["achieved"] / v["peak"]
     # the (a) stage against its unit of 8 W + 16 N^3 bytes per (band, k):
@@ -420,7 +462,7 @@ def main():
                       "what": "scatter + z/y/x passes of a step (breakdown timing) against 8 W + 16 N^3 bytes per "
                               "(band, k)"}
         rl_stage_a["frac"] = rl_stage_a["achieved"] / rl_stage_a["peak"]
-    dom = max(rl, key=lambda c: rl[c]["ms_per_step"])
+    dom = max((c for c in rl if not c.startswith("gemm_")), key=lambda c: rl[c]["ms_per_step"])
     roof = {k: rl[dom][k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")}
     roof["kernel"] = dom
     roof["avg_launch_ms"] = rl[dom]["avg_launch_ms"]
@@ -451,7 +493,11 @@ def main():
                                "2x2x1 MP (4 k), 512 bands/k, 18 projectors/site, S (215 sites) onto R (216)",
                    "nband": nb, "nk": nk, "fftg": wl.fftg.tolist(), "plane_waves": nw.tolist(),
                    "projectors_S": ptotS, "projectors_R": ptotR},
-        "overlap_tflops": R["overlap_tflops"],
+        "overlap_tflops": results["fp16x3"]["overlap_tflops"],
+        "overlap_precision": "fp16x3 (two power-of-two-scaled fp16 planes per operand, 3 products, FP32 "
+                             "accumulation over 512 terms, FP64 across chunks)",
+        "overlap_tflops_by_precision": {m: results[m]["overlap_tflops"] for m in ("fp32", "fp64", *SPLIT_MODES)
+                                        if m in results},
         "paths": {p: {k: v for k, v in r.items() if k != "kernels"} for p, r in results.items()},
         "kernels": rl,
         "stage_a": rl_stage_a if stage_a else None,
@@ -527,7 +573,7 @@ def bench_spinor512(args, torch, B):
     nw2 = np.array([2 * w for w in W], np.int32)
     ovl, ovl_out = {}, {}
     ob = torch.zeros((nk, nb, nb), dtype=torch.complex128, device="cuda")
-    for p in [q for q in args.paths.split(",") if q] + ["bf16x6"]:
+    for p in [q for q in args.paths.split(",") if q] + ["fp16x3", "bf16x6"]:
         with torch.cuda.stream(stream):
             B.overlap_gemm(ctx, nk, 1, False, nw2, CR2, C2, ob, p)  # warm
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -1066,9 +1112,10 @@ def cpu_and_errors(args, torch, wl, Gs, CR, CS, projR, projS, Om, ob, plans, res
                                      [wl.delta_o], [wl.species.ls])
         aug[:, :, j] = a.reshape(len(RB), nk).T
     total_ref = ovref + aug
-    obh = ob.cpu().numpy()[:, RB][:, :, SB]
-    errs["overlap_bf16x3_pseudo"] = {k: float(np.abs(obh[k] - ovref[k]).max() / np.abs(ovref[k]).max())
-                                     for k in range(nk)}
+    for mode, o in ob.items():
+        obh = o.cpu().numpy()[:, RB][:, :, SB]
+        errs[f"overlap_{mode}_pseudo"] = {k: float(np.abs(obh[k] - ovref[k]).max() / np.abs(ovref[k]).max())
+                                          for k in range(nk)}
     for p in Om:
         o = Om[p].cpu().numpy()[:, RB][:, :, SB]
         errs[f"overlap_total_{p}"] = {

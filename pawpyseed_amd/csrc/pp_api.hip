@@ -22,7 +22,7 @@ const char *kclass_name(int c) {
   static const char *names[K_NCLASS] = {"zero",          "scatter",    "fft_z",       "fft_y",    "fft_x",
                                         "phase",         "sphere_sum", "sphere_reduce", "table",  "gemm",
                                         "gemm_reduce",   "aug_prep",   "aug_gemm", "misc", "fft_x_sphere",
-                                        "rs_augment",    "rs_emit"};
+                                        "rs_augment",    "rs_emit",    "gemm_split"};
   return (c >= 0 && c < K_NCLASS) ? names[c] : "?";
 }
 

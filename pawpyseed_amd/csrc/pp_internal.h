@@ -29,6 +29,7 @@ enum KClass {
   K_FFT_X_SPHERE,
   K_RS_AUG,
   K_RS_EMIT,
+  K_GEMM_SPLIT,
   K_NCLASS
 };
 const char *kclass_name(int c);

@@ -517,18 +517,16 @@ void plane_layout(int precision, int M, int N, int K, long &ldp, int &nt, int &n
     ws_bytes += sizeof(int) * (size_t)nchunks * (M + N);
 }
 
-// Split of both operands into planes (ws as plane_layout) and the plane
-// GEMM into FP32 partials P ([nsplit][M][ldp_out], split stride sSplit).
-// No allocation.
-int launch_cgemm_planes(hipStream_t st, int precision, int M, int N, int K, const void *A, long lda, const void *B,
-                        long ldb, void *ws, cplx<float> *P, long ldp_out, long sSplit) {
+// Split of both operands into planes (ws as plane_layout).  No allocation.
+int launch_split_planes(hipStream_t st, int precision, int M, int N, int K, const void *A, long lda, const void *B,
+                        long ldb, void *ws) {
   long ldp;
   int NT, nsplit;
   size_t wsb;
   plane_layout(precision, M, N, K, ldp, NT, nsplit, wsb);
   const long nchunks = ldp / kPlaneChunk;
   PP_REQUIRE(nchunks <= 65535, PP_ERR_ARG, "plane GEMM: K too large");
-  unsigned short *PA = (unsigned short *)ws, *PB = PA + (size_t)M * ldp;
+  unsigned short *PA = (unsigned short *)ws;
   const long sp = (long)(M + N) * ldp;
   int *E = (int *)(PA + (size_t)NT * sp);
   const dim3 sg((unsigned)(M + N), (unsigned)nchunks);
@@ -542,6 +540,20 @@ int launch_cgemm_planes(hipStream_t st, int precision, int M, int N, int K, cons
     hipLaunchKernelGGL((k_split_planes<2, false>), sg, dim3(256), 0, st, (const float *)A, lda, M, (const float *)B,
                        ldb, N, K, ldp, sp, PA, nullptr);
   PP_CHECK_HIP(hipGetLastError());
+  return PP_OK;
+}
+
+// The plane GEMM over the planes in ws into FP32 partials P ([nsplit][M]
+// [ldp_out], split stride sSplit).  No allocation.
+int launch_plane_gemm(hipStream_t st, int precision, int M, int N, int K, const void *ws, cplx<float> *P,
+                      long ldp_out, long sSplit) {
+  long ldp;
+  int NT, nsplit;
+  size_t wsb;
+  plane_layout(precision, M, N, K, ldp, NT, nsplit, wsb);
+  const unsigned short *PA = (const unsigned short *)ws, *PB = PA + (size_t)M * ldp;
+  const long sp = (long)(M + N) * ldp;
+  const int *E = (const int *)(PA + (size_t)NT * sp);
   const int tiles = ((M + 127) / 128) * ((N + 127) / 128);
   const dim3 grd = gemm_grid(tiles, nsplit);
   const long sE = M + N;
@@ -586,9 +598,12 @@ int gemm_fp32_mfma(pp_ctx *ctx, int precision, int M, int N, int K, const void *
   int st;
   if (planes) {
     PP_TRY(ctx->bf16_ws.ensure(wsb));
+    tbegin(ctx, K_GEMM_SPLIT, &ev);
+    st = launch_split_planes(ctx->stream, precision, M, N, K, A, lda, B, ldb, ctx->bf16_ws.p);
+    tend(ctx, K_GEMM_SPLIT, ev);
+    PP_TRY(st);
     tbegin(ctx, K_GEMM, &ev);
-    st = launch_cgemm_planes(ctx->stream, precision, M, N, K, A, lda, B, ldb, ctx->bf16_ws.p, (cplx<float> *)P,
-                             (long)N, (long)M * N);
+    st = launch_plane_gemm(ctx->stream, precision, M, N, K, ctx->bf16_ws.p, (cplx<float> *)P, (long)N, (long)M * N);
   } else {
     tbegin(ctx, K_GEMM, &ev);
     st = launch_cgemm32(ctx->stream, precision, M, N, K, A, lda, B, ldb, P, (long)N, (long)M * N, nsplit, kchunk);
