@@ -431,6 +431,16 @@ void pp_ctx_destroy(pp_ctx *ctx) {
     if (ctx->red_done[i])
       (void)hipEventDestroy(ctx->red_done[i]);
   }
+  if (ctx->aug_upload) {
+    (void)hipEventSynchronize(ctx->aug_upload);
+    (void)hipEventDestroy(ctx->aug_upload);
+  }
+  if (ctx->aug_done) {
+    (void)hipEventSynchronize(ctx->aug_done);
+    (void)hipEventDestroy(ctx->aug_done);
+  }
+  if (ctx->aug_pinned)
+    (void)hipHostFree(ctx->aug_pinned);
   if (ctx->own)
     (void)hipStreamDestroy(ctx->own);
   delete ctx;

@@ -155,6 +155,17 @@ struct pp_ctx {
   hipEvent_t gemm_done[2] = {nullptr, nullptr};  // GEMM of a partials slot finished
   hipEvent_t red_done[2] = {nullptr, nullptr};   // reduction of a partials slot finished
   pp::DevBuf aug_ws;    // gathered / dO-applied projections
+  // Index tables and dO coefficients of the last augmentation call, on the
+  // device and (as the content key) on the host; uploads of changed tables
+  // go through a pinned staging buffer.
+  pp::DevBuf aug_tab;
+  std::vector<int> aug_ints;
+  std::vector<char> aug_coefs;
+  void *aug_pinned = nullptr;
+  size_t aug_pinned_bytes = 0;
+  hipEvent_t aug_upload = nullptr;  // the last upload out of aug_pinned finished
+  hipEvent_t aug_done = nullptr;    // the last call's kernels (readers of aug_tab / aug_ws) finished
+  bool aug_pending = false;
   pp::DevBuf bounds;    // plane-wave sphere bounds per k
   pp::DevBuf lines;     // populated z / y pass lines per k
   pp::DevBuf m_items, m_grids, m_partials;  // work items, chunk grids, partial sums of the multi-structure setup
