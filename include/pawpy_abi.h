@@ -270,7 +270,12 @@ int pp_ctx_set_tuning(pp_ctx *ctx, int fft_bands, int fft_tile, int fft_prune,
  * default 4: sphere points of a wavefront in flight in the real-space
  * augmentation), "rs_prune" (1, default: the real-space FFT passes skip
  * lines outside the plane-wave sphere), "multi_items" (1..64, default 8:
- * work items per launch of pp_setup_projections_multi). */
+ * work items per launch of pp_setup_projections_multi), "gemm_planes"
+ * (two-plane split GEMMs, PP_FP16X3 and PP_BF16X3: 1 one work-group per
+ * CU with 64-real k-steps over planes split beforehand, 2 two work-groups
+ * per CU with 32-real k-steps, 3, default: as 2 with the FP32 operands
+ * split inside the GEMM, where their rows are 16-byte aligned: no plane
+ * buffer; all three give bit-identical results). */
 int pp_ctx_set_param(pp_ctx *ctx, const char *name, int value);
 /* Current value of a parameter of pp_ctx_set_param (0 = the built-in
  * default for fft_tile, fft_bands, sphere_parts, gemm_splitk). */

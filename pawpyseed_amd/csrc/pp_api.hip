@@ -430,6 +430,8 @@ void pp_ctx_destroy(pp_ctx *ctx) {
       (void)hipEventDestroy(ctx->gemm_done[i]);
     if (ctx->red_done[i])
       (void)hipEventDestroy(ctx->red_done[i]);
+    if (ctx->split_done[i])
+      (void)hipEventDestroy(ctx->split_done[i]);
   }
   if (ctx->aug_upload) {
     (void)hipEventSynchronize(ctx->aug_upload);
@@ -540,6 +542,8 @@ int pp_ctx_set_param(pp_ctx *ctx, const char *name, int value) {
     ctx->tune_rs_unroll = value >= 4 ? 4 : value >= 2 ? 2 : 1;
   else if (n == "multi_items")
     ctx->tune_multi_items = value < 1 ? 1 : value > 64 ? 64 : value;
+  else if (n == "gemm_planes")
+    ctx->tune_gemm_planes = value >= 3 ? 3 : value == 2 ? 2 : 1;
   else if (n == "fft_tile")
     return pp_ctx_set_tuning(ctx, ctx->tune_fft_bands, value, ctx->tune_fft_prune, ctx->tune_sphere_parts,
                              ctx->tune_splitk);
@@ -594,6 +598,7 @@ int pp_ctx_get_param(pp_ctx *ctx, const char *name, int *value) {
       {"sphere_nt", ctx->tune_sphere_nt},
       {"fused", ctx->tune_fused}, {"rs_kernel", ctx->tune_rs_kernel}, {"rs_prune", ctx->tune_rs_prune},
       {"rs_unroll", ctx->tune_rs_unroll}, {"multi_items", ctx->tune_multi_items},
+      {"gemm_planes", ctx->tune_gemm_planes},
       {"fft_tile", ctx->tune_fft_tile}, {"fft_prune", ctx->tune_fft_prune}, {"fft_bands", ctx->tune_fft_bands},
       {"sphere_parts", ctx->tune_sphere_parts}, {"gemm_splitk", ctx->tune_splitk}};
   for (const auto &e : table)

@@ -193,6 +193,10 @@ __global__ __launch_bounds__(256) void k_splitk_reduce(int M, int N, int nsplit,
 
 int gemm_fp32_mfma(pp_ctx *ctx, int precision, int M, int N, int K, const void *A, long lda, const void *B, long ldb,
                    cplx<double> *C, long ldc, bool accumulate, int slot);
+bool plane_precision(int precision);
+int plane_gemm_pipeline(pp_ctx *ctx, int precision, int M, int N, int K, const void *A, long lda, long sA,
+                        const void *B, long ldb, long sB, cplx<double> *C, long ldc, long sC, int nbatch,
+                        const int *amap, const int *bmap, const int *klen, bool accumulate);
 
 // Split-K policy of the overlap GEMMs.  FP64: enough K chunks that the
 // 64x64 output tiles times the chunks give about 1024 work-groups (four per
@@ -319,6 +323,13 @@ int zgemm_nt(pp_ctx *ctx, int precision, int M, int N, int K, int atype, const v
       PP_CHECK_HIP(hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming));
   }
   int st = PP_OK;
+  if (pipe && plane_precision(precision)) {
+    st = plane_gemm_pipeline(ctx, precision, M, N, K, A, lda, sA, B, ldb, sB, (cplx<double> *)C, ldc, sC, nbatch,
+                             amap, bmap, klen, accumulate);
+    (void)hipEventRecord(ctx->join, ctx->aux);
+    (void)hipStreamWaitEvent(ctx->stream, ctx->join, 0);
+    return st;
+  }
   for (int b = 0; b < nbatch && st == PP_OK; b++) {
     int ia = amap ? amap[b] : b, ib = bmap ? bmap[b] : b;
     int kk = klen ? klen[b] : K;

@@ -142,6 +142,7 @@ struct pp_ctx {
   int tune_rs_unroll = 4;       // real-space augmentation: points of a wavefront in flight (1, 2, 4)
   int tune_rs_prune = 1;        // real-space FFT passes restricted to the plane-wave sphere's lines
   int tune_multi_items = 8;     // work items per launch of the multi-structure setup
+  int tune_gemm_planes = 3;     // two-plane split GEMMs: 1, 2 work-groups per CU; 3: as 2, operands split in the GEMM
   hipStream_t aux = nullptr;    // second stream of the chunk pipeline (default priority)
   hipEvent_t fork = nullptr, join = nullptr;  // hand-over between the two streams
   int slot = 0;                 // chunk buffer set in use (0 or 1)
@@ -151,9 +152,11 @@ struct pp_ctx {
   pp::DevBuf phase;     // per-point Bloch phase of every k
   pp::DevBuf gemm_ws;   // split-K partials
   pp::DevBuf gemm_ws2;  // second partials buffer (reduction of one batch overlaps the next GEMM)
-  pp::DevBuf bf16_ws;   // bf16 planes of the split-bf16 GEMM's operands
+  pp::DevBuf bf16_ws;   // 16-bit planes of the split-plane GEMMs' operands
+  pp::DevBuf bf16_ws2;  // second plane buffer (the split of one batch overlaps the previous batch's GEMM)
   hipEvent_t gemm_done[2] = {nullptr, nullptr};  // GEMM of a partials slot finished
   hipEvent_t red_done[2] = {nullptr, nullptr};   // reduction of a partials slot finished
+  hipEvent_t split_done[2] = {nullptr, nullptr}; // operand split into a plane buffer finished
   pp::DevBuf aug_ws;    // gathered / dO-applied projections
   // Index tables and dO coefficients of the last augmentation call, on the
   // device and (as the content key) on the host; uploads of changed tables

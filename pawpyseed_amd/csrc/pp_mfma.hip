@@ -193,7 +193,7 @@ __device__ __forceinline__ float bf16_val(unsigned short h) { return __uint_as_f
 // terms) of a row; rows are padded with zeros to a multiple of it.
 static constexpr int kPlaneChunk = 2048;
 
-// Split of FP32 rows into NT 16-bit planes, one work-group per (row, chunk).
+// Split of FP32 rows into NT 16-bit planes, one wavefront per (row, chunk).
 // bf16 (F16 false): x = hi + mid (+ lo) up to the rounding of the last
 // plane, hi = bf16(x), mid = bf16(x - hi), lo = bf16(x - hi - mid), each
 // residual exact in FP32.  fp16 (F16 true, NT = 2): the chunk is first
@@ -202,98 +202,101 @@ static constexpr int kPlaneChunk = 2048;
 // bits, all of them normal for entries within 2^-17 of the chunk's largest,
 // and the rest off by at most 2^-40 of it; e goes to E[chunk][row].  The
 // rows of X (M rows, stride ldx complex) and Y (N rows) are planes' rows 0..M-1
-// and M..M+N-1; plane t of row r occupies P[t * sp + r * ldp + j] for the
-// ldp reals j (2K data, zeros after).  One thread converts 8 consecutive
-// reals.
-template <int NT, bool F16>
-__global__ __launch_bounds__(256) void k_split_planes(const float *__restrict__ X, long ldx, int M,
-                                                      const float *__restrict__ Y, long ldy, int N, int K, long ldp,
-                                                      long sp, unsigned short *__restrict__ P, int *__restrict__ E) {
+// and M..M+N-1.  Planes are chunk-major: real j of row r in plane t is
+// P[t * sp + ((j / 2048) * (M + N) + r) * 2048 + j % 2048] (2K data, zeros
+// up to ldp), so that the rows a GEMM work-group reads for one K chunk form
+// one contiguous block (few translation entries) instead of rows a whole
+// row length apart.  Each lane converts 4 groups of 8
+// consecutive reals; the chunk maximum is a wavefront reduction, so the
+// kernel holds no LDS and runs beside an LDS-bound kernel on the same CUs.
+template <int NT, bool F16, bool PLANES>
+__global__ __launch_bounds__(64) void k_split_planes(const float *__restrict__ X, long ldx, int M,
+                                                     const float *__restrict__ Y, long ldy, int N, int K, long ldp,
+                                                     long sp, unsigned short *__restrict__ P, int *__restrict__ E) {
+  constexpr int G = kPlaneChunk / 512;  // groups of 8 reals per lane
   const int r = blockIdx.x, chunk = blockIdx.y;
   const float *x = r < M ? X + (long)r * 2 * ldx : Y + (long)(r - M) * 2 * ldy;
-  const long j0 = (long)chunk * kPlaneChunk + threadIdx.x * 8;
   const long nreal = 2L * K;
-  float v[8];
-  if (j0 + 8 <= nreal && (((uintptr_t)(x + j0)) & 15) == 0) {
-    const float4 p = *reinterpret_cast<const float4 *>(x + j0), q = *reinterpret_cast<const float4 *>(x + j0 + 4);
-    v[0] = p.x;
-    v[1] = p.y;
-    v[2] = p.z;
-    v[3] = p.w;
-    v[4] = q.x;
-    v[5] = q.y;
-    v[6] = q.z;
-    v[7] = q.w;
-  } else {
+  float v[G][8];
 #pragma unroll
-    for (int i = 0; i < 8; i++)
-      v[i] = j0 + i < nreal ? x[j0 + i] : 0.f;
+  for (int g = 0; g < G; g++) {
+    const long j0 = (long)chunk * kPlaneChunk + g * 512 + threadIdx.x * 8;
+    if (j0 + 8 <= nreal && (((uintptr_t)(x + j0)) & 15) == 0) {
+      const float4 p = *reinterpret_cast<const float4 *>(x + j0), q = *reinterpret_cast<const float4 *>(x + j0 + 4);
+      v[g][0] = p.x;
+      v[g][1] = p.y;
+      v[g][2] = p.z;
+      v[g][3] = p.w;
+      v[g][4] = q.x;
+      v[g][5] = q.y;
+      v[g][6] = q.z;
+      v[g][7] = q.w;
+    } else {
+#pragma unroll
+      for (int i = 0; i < 8; i++)
+        v[g][i] = j0 + i < nreal ? x[j0 + i] : 0.f;
+    }
   }
-  uint4 w[NT];
+  int e = 0;
   if constexpr (F16) {
     static_assert(NT == 2, "fp16 planes come in pairs");
-    __shared__ float wmax[4];
     float m = 0.f;
 #pragma unroll
-    for (int i = 0; i < 8; i++)
-      m = fmaxf(m, fabsf(v[i]));
+    for (int g = 0; g < G; g++)
+#pragma unroll
+      for (int i = 0; i < 8; i++)
+        m = fmaxf(m, fabsf(v[g][i]));
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1)
       m = fmaxf(m, __shfl_xor(m, o));
-    if ((threadIdx.x & 63) == 0)
-      wmax[threadIdx.x >> 6] = m;
-    __syncthreads();
-    m = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
     int ex = 0;
     if (m > 0.f && m <= 3.4e38f)
       (void)frexpf(m, &ex);  // m < 2^ex
-    const int e = m > 0.f && m <= 3.4e38f ? 15 - ex : 0;
+    e = m > 0.f && m <= 3.4e38f ? 15 - ex : 0;
     if (threadIdx.x == 0)
       E[(long)chunk * (M + N) + r] = e;
-    unsigned short h[2][8];
+  }
+  if constexpr (!PLANES)
+    return;
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-      const float y = ldexpf(v[i], e);
-      const _Float16 hi = (_Float16)y;
-      const _Float16 lo = (_Float16)(y - (float)hi);
-      h[0][i] = __builtin_bit_cast(unsigned short, hi);
-      h[1][i] = __builtin_bit_cast(unsigned short, lo);
-    }
-#pragma unroll
-    for (int q = 0; q < 2; q++) {
-      w[q].x = (unsigned)h[q][0] | ((unsigned)h[q][1] << 16);
-      w[q].y = (unsigned)h[q][2] | ((unsigned)h[q][3] << 16);
-      w[q].z = (unsigned)h[q][4] | ((unsigned)h[q][5] << 16);
-      w[q].w = (unsigned)h[q][6] | ((unsigned)h[q][7] << 16);
-    }
-  } else {
+  for (int g = 0; g < G; g++) {
     unsigned short h[NT][8];
 #pragma unroll
     for (int i = 0; i < 8; i++) {
-      float rem = v[i];
+      if constexpr (F16) {
+        const float y = ldexpf(v[g][i], e);
+        const _Float16 hi = (_Float16)y;
+        const _Float16 lo = (_Float16)(y - (float)hi);
+        h[0][i] = __builtin_bit_cast(unsigned short, hi);
+        h[1][i] = __builtin_bit_cast(unsigned short, lo);
+      } else {
+        float rem = v[g][i];
 #pragma unroll
-      for (int q = 0; q < NT; q++) {
-        h[q][i] = bf16_bits(rem);
-        rem -= bf16_val(h[q][i]);
+        for (int q = 0; q < NT; q++) {
+          h[q][i] = bf16_bits(rem);
+          rem -= bf16_val(h[q][i]);
+        }
       }
     }
+    const long j0 = (long)chunk * kPlaneChunk + g * 512 + threadIdx.x * 8;
 #pragma unroll
     for (int q = 0; q < NT; q++) {
-      w[q].x = (unsigned)h[q][0] | ((unsigned)h[q][1] << 16);
-      w[q].y = (unsigned)h[q][2] | ((unsigned)h[q][3] << 16);
-      w[q].z = (unsigned)h[q][4] | ((unsigned)h[q][5] << 16);
-      w[q].w = (unsigned)h[q][6] | ((unsigned)h[q][7] << 16);
+      uint4 w;
+      w.x = (unsigned)h[q][0] | ((unsigned)h[q][1] << 16);
+      w.y = (unsigned)h[q][2] | ((unsigned)h[q][3] << 16);
+      w.z = (unsigned)h[q][4] | ((unsigned)h[q][5] << 16);
+      w.w = (unsigned)h[q][6] | ((unsigned)h[q][7] << 16);
+      *reinterpret_cast<uint4 *>(P + q * sp + ((long)chunk * (M + N) + r) * kPlaneChunk + g * 512 +
+                                 threadIdx.x * 8) = w;
     }
   }
-#pragma unroll
-  for (int q = 0; q < NT; q++)
-    *reinterpret_cast<uint4 *>(P + q * sp + (long)r * ldp + j0) = w[q];
 }
 
 // (re, im) 16-bit pairs of a B row -> the pairs (im, -re) of B'': swap the
 // halves of each 32-bit word and flip the sign of the new upper half (the
 // sign is bit 15 in bf16 and fp16 alike).
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ u32x4 qswap(u32x4 b) { return ((b >> 16) | (b << 16)) ^ 0x80000000u; }
 
 template <bool F16>
@@ -317,16 +320,54 @@ __device__ __forceinline__ floatx4 mfma16(u32x4 a, u32x4 b, floatx4 c) {
 // MFMAs and the global operand traffic per output is half that of a 64x64
 // tile.  A k-step of KS reals per row: its planes go global -> registers
 // -> LDS as 16-byte accesses into one of two LDS buffers while the other
-// buffer's step is computed, one barrier per step.  Planes hold zeros past
+// buffer's step is computed, one barrier per step.  WPE wavefronts per
+// SIMD: 1 (up to 512 registers, KS = 64: 144 KiB of LDS) or 2 (256
+// registers with the accumulators in architectural VGPRs, KS = 32: 80 KiB,
+// two work-groups per CU whose waves cover each other's LDS and memory
+// waits).  Planes hold zeros past
 // the data, so a chunk of CH reals needs no k bounds.  The chunk's FP32
 // sums are its partial sums (row stride ldc, split stride sSplit); fp16
 // planes carry per-(chunk, row) scale exponents EA / EB, undone exactly at
 // the store.
-template <int NT, int KS, bool F16, int CH>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_cgemm_planes(int M, int N, int nsplit, const unsigned short *__restrict__ A,
-                                                      const unsigned short *__restrict__ B, long ldp, long sp,
-                                                      const int *__restrict__ EA, const int *__restrict__ EB,
-                                                      long sE, cplx<float> *__restrict__ C, long ldc, long sSplit) {
+// hi and lo 16-bit planes of 8 consecutive reals, exactly as k_split_planes
+// forms them (scale 2^e for fp16)
+template <bool F16>
+__device__ __forceinline__ void split8(f32x4 x0, f32x4 x1, int e, u32x4 &hi, u32x4 &lo) {
+  float v[8] = {x0[0], x0[1], x0[2], x0[3], x1[0], x1[1], x1[2], x1[3]};
+  unsigned short h[8], l[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    if constexpr (F16) {
+      const float y = ldexpf(v[i], e);
+      const _Float16 a = (_Float16)y;
+      const _Float16 b = (_Float16)(y - (float)a);
+      h[i] = __builtin_bit_cast(unsigned short, a);
+      l[i] = __builtin_bit_cast(unsigned short, b);
+    } else {
+      h[i] = bf16_bits(v[i]);
+      l[i] = bf16_bits(v[i] - bf16_val(h[i]));
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    hi[i] = (unsigned)h[2 * i] | ((unsigned)h[2 * i + 1] << 16);
+    lo[i] = (unsigned)l[2 * i] | ((unsigned)l[2 * i + 1] << 16);
+  }
+}
+
+// Operands: CVT false -- NT chunk-major planes of 16-bit values
+// (k_split_planes): rows of lda = ldb = 2048 reals per chunk block, block
+// stride cs, plane stride sp; CVT true (two-plane
+// modes) -- the FP32 rows themselves (row stride lda, ldb complex, K
+// complex terms each), split into hi / lo planes on their way into LDS with
+// the same arithmetic, so that no plane buffer goes through HBM (fp16: the
+// exponents EA / EB of an exponent-only k_split_planes pass).
+template <int NT, int KS, bool F16, int CH, int WPE, bool CVT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_cgemm_planes(
+    int M, int N, int nsplit, int K, const void *__restrict__ Av, long lda, const void *__restrict__ Bv, long ldb,
+    long sp, long cs, const int *__restrict__ EA, const int *__restrict__ EB, long sE, cplx<float> *__restrict__ C,
+    long ldc, long sSplit) {
+  static_assert(!CVT || NT == 2, "in-kernel splits form two planes");
   constexpr int LS = KS + 8;           // LDS row stride (16-bit words): 16 bytes of padding
   constexpr int HP = KS / 32;          // 16-byte loads per (row, plane) per thread
   __shared__ __attribute__((aligned(16))) unsigned short As[2][NT][128][LS];
@@ -343,24 +384,81 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   // This thread's rows of the tile: lrow and lrow + 64 of A and of B.  A row
   // past the matrix reads the last row instead: it only feeds outputs of
   // rows (columns) past the matrix, which are not stored.
-  const unsigned short *pa0 = A + (long)min(tm * 128 + lrow, M - 1) * ldp + k0 + part * (KS / 4);
-  const unsigned short *pa1 = A + (long)min(tm * 128 + 64 + lrow, M - 1) * ldp + k0 + part * (KS / 4);
-  const unsigned short *pb0 = B + (long)min(tn * 128 + lrow, N - 1) * ldp + k0 + part * (KS / 4);
-  const unsigned short *pb1 = B + (long)min(tn * 128 + 64 + lrow, N - 1) * ldp + k0 + part * (KS / 4);
+  const int ra0 = min(tm * 128 + lrow, M - 1), ra1 = min(tm * 128 + 64 + lrow, M - 1);
+  const int rb0 = min(tn * 128 + lrow, N - 1), rb1 = min(tn * 128 + 64 + lrow, N - 1);
+  const long kp = k0 + part * (KS / 4);
+  // the scale exponents of the plane chunk this K chunk lies in
+  const long ec = (long)split * CH / kPlaneChunk * sE;
+  const unsigned short *pa0 = nullptr, *pa1 = nullptr, *pb0 = nullptr, *pb1 = nullptr;
+  const float *xa0 = nullptr, *xa1 = nullptr, *xb0 = nullptr, *xb1 = nullptr;
+  int ea0 = 0, ea1 = 0, eb0 = 0, eb1 = 0;
+  bool whole = true;  // the chunk lies inside the 2K reals of every row
+  if constexpr (CVT) {
+    const float *A = (const float *)Av, *B = (const float *)Bv;
+    xa0 = A + (long)ra0 * 2 * lda + kp;
+    xa1 = A + (long)ra1 * 2 * lda + kp;
+    xb0 = B + (long)rb0 * 2 * ldb + kp;
+    xb1 = B + (long)rb1 * 2 * ldb + kp;
+    whole = k0 + CH <= 2L * K;
+    if constexpr (F16) {
+      ea0 = EA[ec + ra0];
+      ea1 = EA[ec + ra1];
+      eb0 = EB[ec + rb0];
+      eb1 = EB[ec + rb1];
+    }
+  } else {
+    // chunk-major planes: rows of lda reals within the chunk's block of
+    // stride cs
+    const long cb = k0 / lda * cs, kc0 = k0 % lda + part * (KS / 4);
+    const unsigned short *A = (const unsigned short *)Av + cb, *B = (const unsigned short *)Bv + cb;
+    pa0 = A + (long)ra0 * lda + kc0;
+    pa1 = A + (long)ra1 * lda + kc0;
+    pb0 = B + (long)rb0 * ldb + kc0;
+    pb1 = B + (long)rb1 * ldb + kc0;
+  }
+  const long nreal_left = 2L * K - kp;  // reals of a row from this thread's first
   // global -> register staging of one k-step: [row half][plane][16-byte piece]
+  // (planes), or [row half][16-byte piece] of FP32 values (CVT)
   u32x4 ga0[NT][HP], ga1[NT][HP], gb0[NT][HP], gb1[NT][HP];
+  f32x4 fa0[2 * HP], fa1[2 * HP], fb0[2 * HP], fb1[2 * HP];
+  auto ld4 = [&](const float *x, long off) -> f32x4 {
+    if (whole)
+      return *reinterpret_cast<const f32x4 *>(x + off);
+    f32x4 r;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+      r[i] = off + i < nreal_left ? x[off + i] : 0.f;
+    return r;
+  };
 #define PP_FETCH(step)                                                                                           \
   {                                                                                                              \
     const long off_ = (long)(step) * KS;                                                                         \
-    _Pragma("unroll") for (int t = 0; t < NT; t++) _Pragma("unroll") for (int q = 0; q < HP; q++) {             \
-      ga0[t][q] = *reinterpret_cast<const u32x4 *>(pa0 + t * sp + off_ + q * 8);                                 \
-      ga1[t][q] = *reinterpret_cast<const u32x4 *>(pa1 + t * sp + off_ + q * 8);                                 \
-      gb0[t][q] = *reinterpret_cast<const u32x4 *>(pb0 + t * sp + off_ + q * 8);                                 \
-      gb1[t][q] = *reinterpret_cast<const u32x4 *>(pb1 + t * sp + off_ + q * 8);                                 \
+    if constexpr (CVT) {                                                                                         \
+      _Pragma("unroll") for (int q = 0; q < 2 * HP; q++) {                                                       \
+        fa0[q] = ld4(xa0, off_ + q * 4);                                                                         \
+        fa1[q] = ld4(xa1, off_ + q * 4);                                                                         \
+        fb0[q] = ld4(xb0, off_ + q * 4);                                                                         \
+        fb1[q] = ld4(xb1, off_ + q * 4);                                                                         \
+      }                                                                                                          \
+    } else {                                                                                                     \
+      _Pragma("unroll") for (int t = 0; t < NT; t++) _Pragma("unroll") for (int q = 0; q < HP; q++) {           \
+        ga0[t][q] = *reinterpret_cast<const u32x4 *>(pa0 + t * sp + off_ + q * 8);                               \
+        ga1[t][q] = *reinterpret_cast<const u32x4 *>(pa1 + t * sp + off_ + q * 8);                               \
+        gb0[t][q] = *reinterpret_cast<const u32x4 *>(pb0 + t * sp + off_ + q * 8);                               \
+        gb1[t][q] = *reinterpret_cast<const u32x4 *>(pb1 + t * sp + off_ + q * 8);                               \
+      }                                                                                                          \
     }                                                                                                            \
   }
 #define PP_STASH(buf)                                                                                            \
   {                                                                                                              \
+    if constexpr (CVT) {                                                                                         \
+      _Pragma("unroll") for (int q = 0; q < HP; q++) {                                                           \
+        split8<F16>(fa0[2 * q], fa0[2 * q + 1], ea0, ga0[0][q], ga0[1][q]);                                      \
+        split8<F16>(fa1[2 * q], fa1[2 * q + 1], ea1, ga1[0][q], ga1[1][q]);                                      \
+        split8<F16>(fb0[2 * q], fb0[2 * q + 1], eb0, gb0[0][q], gb0[1][q]);                                      \
+        split8<F16>(fb1[2 * q], fb1[2 * q + 1], eb1, gb1[0][q], gb1[1][q]);                                      \
+      }                                                                                                          \
+    }                                                                                                            \
     _Pragma("unroll") for (int t = 0; t < NT; t++) _Pragma("unroll") for (int q = 0; q < HP; q++) {             \
       *reinterpret_cast<u32x4 *>(&As[buf][t][lrow][part * (KS / 4) + q * 8]) = ga0[t][q];                        \
       *reinterpret_cast<u32x4 *>(&As[buf][t][64 + lrow][part * (KS / 4) + q * 8]) = ga1[t][q];                   \
@@ -433,8 +531,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
       const int j = tn * 128 + wn * 64 + nj * 16 + (lane & 15);
       if (j >= N)
         continue;
-      // the scale exponents of the plane chunk this K chunk lies in
-      const long ec = (long)split * CH / kPlaneChunk * sE;
       const int eb = F16 ? EB[ec + j] : 0;
 #pragma unroll
       for (int v = 0; v < 4; v++) {
@@ -502,70 +598,98 @@ bool plane_precision(int precision) {
 
 // Layout of a plane GEMM's workspace and partial sums: ldp reals per plane
 // row (2K rounded up to whole plane chunks of 1024 complex terms), NT planes
-// of M + N rows, then the fp16 scale exponents [plane chunk][M + N].  The
-// GEMM accumulates K chunks of 512 complex terms in FP32 (two per plane
-// chunk), one partial-sum plane each, summed in FP64: FP32 sums over
-// 1024 terms of 3 products per k-step would round more than the FP32
-// GEMM's 512-term chunks do.
-void plane_layout(int precision, int M, int N, int K, long &ldp, int &nt, int &nsplit, size_t &ws_bytes) {
+// of M + N rows, then the fp16 scale exponents [plane chunk][M + N] (with
+// cvt, the GEMM splits its FP32 operands itself and the workspace holds the
+// exponents alone).  The GEMM accumulates K chunks of 512 complex terms in
+// FP32 (two per plane chunk), one partial-sum plane each, summed in FP64:
+// FP32 sums over 1024 terms of 3 products per k-step would round more than
+// the FP32 GEMM's 512-term chunks do.
+void plane_layout(int precision, int M, int N, int K, bool cvt, long &ldp, int &nt, int &nsplit, size_t &ws_bytes) {
   nt = precision == PP_BF16X6 ? 3 : 2;
   ldp = (2L * K + kPlaneChunk - 1) / kPlaneChunk * kPlaneChunk;
   nsplit = (int)std::max<long>(1, ldp / (kPlaneChunk / 2));
   const long nchunks = ldp / kPlaneChunk;
-  ws_bytes = sizeof(unsigned short) * (size_t)nt * (M + N) * ldp;
+  ws_bytes = cvt ? 256 : sizeof(unsigned short) * (size_t)nt * (M + N) * ldp;
   if (precision == PP_FP16X3)
     ws_bytes += sizeof(int) * (size_t)nchunks * (M + N);
 }
 
-// Split of both operands into planes (ws as plane_layout).  No allocation.
+// Whether the GEMM splits its operands itself under the context's settings
+// (gemm_planes 3, two-plane modes, 16-byte aligned FP32 rows).
+bool plane_cvt(pp_ctx *ctx, int precision, const void *A, long lda, const void *B, long ldb) {
+  return ctx->tune_gemm_planes == 3 && (precision == PP_FP16X3 || precision == PP_BF16X3) && lda % 2 == 0 &&
+         ldb % 2 == 0 && (uintptr_t)A % 16 == 0 && (uintptr_t)B % 16 == 0;
+}
+
+// Split of both operands into planes (ws as plane_layout); with cvt the
+// fp16 exponents only (nothing at all for bf16).  No allocation.
 int launch_split_planes(hipStream_t st, int precision, int M, int N, int K, const void *A, long lda, const void *B,
-                        long ldb, void *ws) {
+                        long ldb, void *ws, bool cvt) {
   long ldp;
   int NT, nsplit;
   size_t wsb;
-  plane_layout(precision, M, N, K, ldp, NT, nsplit, wsb);
+  plane_layout(precision, M, N, K, cvt, ldp, NT, nsplit, wsb);
   const long nchunks = ldp / kPlaneChunk;
   PP_REQUIRE(nchunks <= 65535, PP_ERR_ARG, "plane GEMM: K too large");
+  if (cvt && precision != PP_FP16X3)
+    return PP_OK;
   unsigned short *PA = (unsigned short *)ws;
   const long sp = (long)(M + N) * ldp;
-  int *E = (int *)(PA + (size_t)NT * sp);
+  int *E = cvt ? (int *)ws : (int *)(PA + (size_t)NT * sp);
   const dim3 sg((unsigned)(M + N), (unsigned)nchunks);
-  if (precision == PP_FP16X3)
-    hipLaunchKernelGGL((k_split_planes<2, true>), sg, dim3(256), 0, st, (const float *)A, lda, M, (const float *)B,
-                       ldb, N, K, ldp, sp, PA, E);
+  if (cvt)
+    hipLaunchKernelGGL((k_split_planes<2, true, false>), sg, dim3(64), 0, st, (const float *)A, lda, M,
+                       (const float *)B, ldb, N, K, ldp, sp, PA, E);
+  else if (precision == PP_FP16X3)
+    hipLaunchKernelGGL((k_split_planes<2, true, true>), sg, dim3(64), 0, st, (const float *)A, lda, M,
+                       (const float *)B, ldb, N, K, ldp, sp, PA, E);
   else if (NT == 3)
-    hipLaunchKernelGGL((k_split_planes<3, false>), sg, dim3(256), 0, st, (const float *)A, lda, M, (const float *)B,
-                       ldb, N, K, ldp, sp, PA, nullptr);
+    hipLaunchKernelGGL((k_split_planes<3, false, true>), sg, dim3(64), 0, st, (const float *)A, lda, M,
+                       (const float *)B, ldb, N, K, ldp, sp, PA, nullptr);
   else
-    hipLaunchKernelGGL((k_split_planes<2, false>), sg, dim3(256), 0, st, (const float *)A, lda, M, (const float *)B,
-                       ldb, N, K, ldp, sp, PA, nullptr);
+    hipLaunchKernelGGL((k_split_planes<2, false, true>), sg, dim3(64), 0, st, (const float *)A, lda, M,
+                       (const float *)B, ldb, N, K, ldp, sp, PA, nullptr);
   PP_CHECK_HIP(hipGetLastError());
   return PP_OK;
 }
 
-// The plane GEMM over the planes in ws into FP32 partials P ([nsplit][M]
-// [ldp_out], split stride sSplit).  No allocation.
-int launch_plane_gemm(hipStream_t st, int precision, int M, int N, int K, const void *ws, cplx<float> *P,
-                      long ldp_out, long sSplit) {
+// The plane GEMM over the planes in ws (or, with cvt, over A and B
+// themselves) into FP32 partials P ([nsplit][M][ldp_out], split stride
+// sSplit).  No allocation.
+int launch_plane_gemm(hipStream_t st, int precision, int M, int N, int K, const void *A, long lda, const void *B,
+                      long ldb, const void *ws, cplx<float> *P, long ldp_out, long sSplit, int wgs_per_cu,
+                      bool cvt) {
   long ldp;
   int NT, nsplit;
   size_t wsb;
-  plane_layout(precision, M, N, K, ldp, NT, nsplit, wsb);
-  const unsigned short *PA = (const unsigned short *)ws, *PB = PA + (size_t)M * ldp;
-  const long sp = (long)(M + N) * ldp;
-  const int *E = (const int *)(PA + (size_t)NT * sp);
+  plane_layout(precision, M, N, K, cvt, ldp, NT, nsplit, wsb);
+  const unsigned short *PA = (const unsigned short *)ws, *PB = PA + (size_t)M * kPlaneChunk;
+  const long sp = (long)(M + N) * ldp, cs = (long)(M + N) * kPlaneChunk;
+  constexpr long LP = kPlaneChunk;
+  const int *E = cvt ? (const int *)ws : (const int *)(PA + (size_t)NT * sp);
   const int tiles = ((M + 127) / 128) * ((N + 127) / 128);
   const dim3 grd = gemm_grid(tiles, nsplit);
   const long sE = M + N;
-  if (precision == PP_FP16X3)
-    hipLaunchKernelGGL((k_cgemm_planes<2, 64, true, kPlaneChunk / 2>), grd, dim3(256), 0, st, M, N, nsplit, PA, PB,
-                       ldp, sp, E, E + M, sE, P, ldp_out, sSplit);
+  constexpr int CH = kPlaneChunk / 2;
+#define PP_PG(NTV, KSV, F, WPEV, CVTV, AA, LA, BB, LB, EE)                                                      \
+  hipLaunchKernelGGL((k_cgemm_planes<NTV, KSV, F, CH, WPEV, CVTV>), grd, dim3(256), 0, st, M, N, nsplit, K, AA, LA, BB, \
+                     LB, sp, cs, EE, EE ? EE + M : nullptr, sE, P, ldp_out, sSplit)
+  const int *noE = nullptr;
+  if (cvt && precision == PP_FP16X3)
+    PP_PG(2, 32, true, 2, true, A, lda, B, ldb, E);
+  else if (cvt)
+    PP_PG(2, 32, false, 2, true, A, lda, B, ldb, noE);
+  else if (precision == PP_FP16X3 && wgs_per_cu >= 2)
+    PP_PG(2, 32, true, 2, false, PA, LP, PB, LP, E);
+  else if (precision == PP_FP16X3)
+    PP_PG(2, 64, true, 1, false, PA, LP, PB, LP, E);
+  else if (NT == 2 && wgs_per_cu >= 2)
+    PP_PG(2, 32, false, 2, false, PA, LP, PB, LP, noE);
   else if (NT == 3)
-    hipLaunchKernelGGL((k_cgemm_planes<3, 32, false, kPlaneChunk / 2>), grd, dim3(256), 0, st, M, N, nsplit, PA, PB,
-                       ldp, sp, nullptr, nullptr, sE, P, ldp_out, sSplit);
+    PP_PG(3, 32, false, 1, false, PA, LP, PB, LP, noE);
   else
-    hipLaunchKernelGGL((k_cgemm_planes<2, 64, false, kPlaneChunk / 2>), grd, dim3(256), 0, st, M, N, nsplit, PA, PB,
-                       ldp, sp, nullptr, nullptr, sE, P, ldp_out, sSplit);
+    PP_PG(2, 64, false, 1, false, PA, LP, PB, LP, noE);
+#undef PP_PG
   PP_CHECK_HIP(hipGetLastError());
   return PP_OK;
 }
@@ -578,12 +702,13 @@ int launch_plane_gemm(hipStream_t st, int precision, int M, int N, int K, const 
 int gemm_fp32_mfma(pp_ctx *ctx, int precision, int M, int N, int K, const void *A, long lda, const void *B, long ldb,
                    cplx<double> *C, long ldc, bool accumulate, int slot) {
   const bool planes = plane_precision(precision);
+  const bool cvt = planes && plane_cvt(ctx, precision, A, lda, B, ldb);
   int nsplit, kchunk = 0;
   long ldp;
   int nt;
   size_t wsb = 0;
   if (planes) {
-    plane_layout(precision, M, N, K, ldp, nt, nsplit, wsb);
+    plane_layout(precision, M, N, K, cvt, ldp, nt, nsplit, wsb);
   } else {
     nsplit = splitk_count(precision, M, N, K, ctx->tune_splitk);
     kchunk = cgemm_chunk(precision, K, nsplit);
@@ -598,12 +723,15 @@ int gemm_fp32_mfma(pp_ctx *ctx, int precision, int M, int N, int K, const void *
   int st;
   if (planes) {
     PP_TRY(ctx->bf16_ws.ensure(wsb));
-    tbegin(ctx, K_GEMM_SPLIT, &ev);
-    st = launch_split_planes(ctx->stream, precision, M, N, K, A, lda, B, ldb, ctx->bf16_ws.p);
-    tend(ctx, K_GEMM_SPLIT, ev);
-    PP_TRY(st);
+    if (!cvt || precision == PP_FP16X3) {
+      tbegin(ctx, K_GEMM_SPLIT, &ev);
+      st = launch_split_planes(ctx->stream, precision, M, N, K, A, lda, B, ldb, ctx->bf16_ws.p, cvt);
+      tend(ctx, K_GEMM_SPLIT, ev);
+      PP_TRY(st);
+    }
     tbegin(ctx, K_GEMM, &ev);
-    st = launch_plane_gemm(ctx->stream, precision, M, N, K, ctx->bf16_ws.p, (cplx<float> *)P, (long)N, (long)M * N);
+    st = launch_plane_gemm(ctx->stream, precision, M, N, K, A, lda, B, ldb, ctx->bf16_ws.p, (cplx<float> *)P,
+                           (long)N, (long)M * N, ctx->tune_gemm_planes, cvt);
   } else {
     tbegin(ctx, K_GEMM, &ev);
     st = launch_cgemm32(ctx->stream, precision, M, N, K, A, lda, B, ldb, P, (long)N, (long)M * N, nsplit, kchunk);
@@ -623,6 +751,98 @@ int gemm_fp32_mfma(pp_ctx *ctx, int precision, int M, int N, int K, const void *
   PP_TRY(st);
   if (slot >= 0)
     PP_CHECK_HIP(hipEventRecord(ctx->red_done[slot], ctx->aux));
+  return PP_OK;
+}
+
+// A batch of split-plane GEMMs (one per k-point) as a two-stream pipeline:
+// the operand split of batch b + 1 runs on the second stream while batch
+// b's GEMM runs on the context's stream (two plane buffers), and the split-K
+// reduction of batch b follows it on the second stream (two partials
+// buffers).  The split holds no LDS, so its wavefronts run on the CUs the
+// LDS-bound GEMM fills.  The second stream's work is ordered
+//   split(0), split(1), reduce(0), split(2), reduce(1), ...
+// and every buffer hand-over is an event: a plane buffer is split into
+// after the GEMM two batches back read it (gemm_done), a GEMM starts after
+// its split (split_done) and after the reduction two batches back read its
+// partials (red_done).  The caller's stream joins the second stream at the
+// end (zgemm_nt).
+int plane_gemm_pipeline(pp_ctx *ctx, int precision, int M, int N, int K, const void *A, long lda, long sA,
+                        const void *B, long ldb, long sB, cplx<double> *C, long ldc, long sC, int nbatch,
+                        const int *amap, const int *bmap, const int *klen, bool accumulate) {
+  size_t wsmax = 0, pmax = 0;
+  const bool cvt = plane_cvt(ctx, precision, A, lda, B, ldb) && sA % 2 == 0 && sB % 2 == 0;
+  for (int b = 0; b < nbatch; b++) {
+    long ldp;
+    int nt, nsplit;
+    size_t wsb;
+    plane_layout(precision, M, N, klen ? klen[b] : K, cvt, ldp, nt, nsplit, wsb);
+    wsmax = std::max(wsmax, wsb);
+    pmax = std::max(pmax, sizeof(cplx<float>) * (size_t)nsplit * M * N);
+  }
+  PP_TRY(ctx->bf16_ws.ensure(wsmax));
+  PP_TRY(ctx->bf16_ws2.ensure(wsmax));
+  PP_TRY(ctx->gemm_ws.ensure(pmax));
+  PP_TRY(ctx->gemm_ws2.ensure(pmax));
+  hipStream_t main = ctx->stream, aux = ctx->aux;
+  for (int i = 0; i < 2; i++) {
+    if (!ctx->split_done[i])
+      PP_CHECK_HIP(hipEventCreateWithFlags(&ctx->split_done[i], hipEventDisableTiming));
+    PP_CHECK_HIP(hipEventRecord(ctx->gemm_done[i], main));  // no GEMM reads a plane buffer yet
+  }
+  void *planes[2] = {ctx->bf16_ws.p, ctx->bf16_ws2.p};
+  void *parts[2] = {ctx->gemm_ws.p, ctx->gemm_ws2.p};
+  auto operand = [&](int b, const void *&pa, const void *&pb, int &kk) {
+    const int ia = amap ? amap[b] : b, ib = bmap ? bmap[b] : b;
+    kk = klen ? klen[b] : K;
+    pa = (const char *)A + (size_t)ia * sA * sizeof(cplx<float>);
+    pb = (const char *)B + (size_t)ib * sB * sizeof(cplx<float>);
+  };
+  auto split = [&](int b) -> int {
+    const int slot = b & 1;
+    const void *pa, *pb;
+    int kk;
+    operand(b, pa, pb, kk);
+    PP_CHECK_HIP(hipStreamWaitEvent(aux, ctx->gemm_done[slot], 0));
+    hipEvent_t ev;
+    ctx->stream = aux;
+    tbegin(ctx, K_GEMM_SPLIT, &ev);
+    const int st = launch_split_planes(aux, precision, M, N, kk, pa, lda, pb, ldb, planes[slot], cvt);
+    tend(ctx, K_GEMM_SPLIT, ev);
+    ctx->stream = main;
+    PP_TRY(st);
+    PP_CHECK_HIP(hipEventRecord(ctx->split_done[slot], aux));
+    return PP_OK;
+  };
+  PP_TRY(split(0));
+  for (int b = 0; b < nbatch; b++) {
+    const int slot = b & 1;
+    if (b + 1 < nbatch)
+      PP_TRY(split(b + 1));
+    const void *pa, *pb;
+    int kk;
+    operand(b, pa, pb, kk);
+    long ldp;
+    int nt, nsplit;
+    size_t wsb;
+    plane_layout(precision, M, N, kk, cvt, ldp, nt, nsplit, wsb);
+    PP_CHECK_HIP(hipStreamWaitEvent(main, ctx->split_done[slot], 0));
+    PP_CHECK_HIP(hipStreamWaitEvent(main, ctx->red_done[slot], 0));
+    hipEvent_t ev;
+    tbegin(ctx, K_GEMM, &ev);
+    int st = launch_plane_gemm(main, precision, M, N, kk, pa, lda, pb, ldb, planes[slot],
+                               (cplx<float> *)parts[slot], (long)N, (long)M * N, ctx->tune_gemm_planes, cvt);
+    tend(ctx, K_GEMM, ev);
+    PP_TRY(st);
+    PP_CHECK_HIP(hipEventRecord(ctx->gemm_done[slot], main));
+    PP_CHECK_HIP(hipStreamWaitEvent(aux, ctx->gemm_done[slot], 0));
+    ctx->stream = aux;
+    tbegin(ctx, K_GEMM_REDUCE, &ev);
+    st = launch_splitk_reduce(aux, M, N, nsplit, parts[slot], true, C + (size_t)b * sC, ldc, accumulate);
+    tend(ctx, K_GEMM_REDUCE, ev);
+    ctx->stream = main;
+    PP_TRY(st);
+    PP_CHECK_HIP(hipEventRecord(ctx->red_done[slot], aux));
+  }
   return PP_OK;
 }
 

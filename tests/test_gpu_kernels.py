@@ -271,6 +271,29 @@ def test_overlap_split_planes_row_scales(ctx, precision):
     assert err < 1e-6 and row_err < 3e-6
 
 
+@pytest.mark.parametrize("precision", ["fp16x3", "bf16x3"])
+def test_overlap_split_planes_work_group_variants(ctx, precision):
+    """The two-plane GEMM with one work-group per CU (64-real k-steps), with
+    two (32-real k-steps), and with two whose GEMM splits the FP32 operands
+    itself (no plane buffer): every output accumulates the same products of
+    the same planes in the same order, so the results are bit-identical;
+    130 x 67 bands, two k-points with odd plane-wave counts (a partial last
+    chunk), against FP64 <= 1e-6 (fp16x3) / 2e-5 (bf16x3)."""
+    before = ctx.get_param("gemm_planes")
+    try:
+        out = {}
+        for gp in (1, 2, 3):
+            ctx.set_param("gemm_planes", gp)
+            out[gp], want = _overlap_case(ctx, precision, nbR=130, nbS=67)
+    finally:
+        ctx.set_param("gemm_planes", before)
+    err = normwise(out[2], want)
+    print(precision, "gemm_planes 1 / 3 vs 2 identical:", np.array_equal(out[1], out[2]),
+          np.array_equal(out[3], out[2]), "error", err, flush=True)
+    assert np.array_equal(out[1], out[2]) and np.array_equal(out[3], out[2])
+    assert err < (1e-6 if precision == "fp16x3" else 2e-5)
+
+
 def test_overlap_near_orthogonal_pairs(ctx):
     """Pairs of nearly orthogonal bands: S band i = d_i + eps_i c_i with d_i
     orthogonal to R band c_i, eps_i from 1e-8 to 1, so overlap entries span
