@@ -120,9 +120,10 @@ def read_traffic(path_name, settings):
     scaled by the factors a plain 1 GiB device copy reads on the same
     counters), for the path and the settings that change what a launch
     moves (fft_xmask: the x pass writes only covered points; sphere_parts:
-    the sphere sums' partial rows).  The summary keys its entries
-    "<path>/fft_xmask=<x>,sphere_parts=<p>".  Empty when there is no entry
-    for those settings."""
+    the sphere sums' partial rows; gemm_planes: whether the split-plane
+    GEMMs read materialised planes).  The summary keys its entries
+    "<path>/fft_xmask=<x>,sphere_parts=<p>,gemm_planes=<g>".  Empty when
+    there is no entry for those settings."""
     try:
         txt = open(TRAFFIC_FILE).read()
         blk = txt.split("```json", 1)[1].split("```", 1)[0]
@@ -130,7 +131,7 @@ def read_traffic(path_name, settings):
     except Exception:
         return {}
     key = (f"{path_name}/fft_xmask={int(settings.get('fft_xmask', 0))},"
-           f"sphere_parts={int(settings.get('sphere_parts', 0))}")
+           f"sphere_parts={int(settings.get('sphere_parts', 0))},gemm_planes={int(settings.get('gemm_planes', 3))}")
     return {k: float(v) for k, v in d.get("traffic_bytes_per_launch", {}).get(key, {}).items()}
 
 
@@ -389,7 +390,8 @@ def main():
         xs_read = float(np.mean([n1 * n2 * NB * prec_bytes * (b / (n1 * n2 * NB * prec_bytes) - n0) for b in bx]))
         alg["fft_x_sphere"] = xs_read + planS.npts * (P * prec_bytes // 2 + prec_bytes) + nrows * P * NB * 8
     gemm_flops = 8.0 * nb * nb * float(nw.mean())
-    timed_settings = R.get("autotune") or {n: ctx.get_param(n) for n in B.TUNED}
+    timed_settings = dict(R.get("autotune") or {n: ctx.get_param(n) for n in B.TUNED})
+    timed_settings["gemm_planes"] = ctx.get_param("gemm_planes")
     traffic = read_traffic(head, timed_settings)
     rl = {}
     for c, b in alg.items():
@@ -443,11 +445,16 @@ def main():
         if ks and ks["launches"]:
             ms = ks["ms_total"] / ks["launches"]
             nplanes = 3 if mode == "bf16x6" else 2
-            sb = (2.0 * nb) * float(nw.mean()) * (8.0 + 4.0 * nplanes)
+            # gemm_planes 3: the two-plane GEMMs split their operands
+            # themselves; fp16x3 first reads both operands for the chunk
+            # exponents (and bf16x3 has no pass at all)
+            cvt = timed_settings["gemm_planes"] == 3 and nplanes == 2
+            sb = (2.0 * nb) * float(nw.mean()) * (8.0 if cvt else 8.0 + 4.0 * nplanes)
             rl[f"gemm_split_{mode}"] = {"bound": "hbm", "achieved": sb / (ms * 1e-3) / 1e9, "peak": PEAK_HBM / 1e9,
                                         "unit": "GB/s", "alg_bytes": sb, "avg_launch_ms": ms,
                                         "ms_per_step": ks["ms_total"], "traffic": traffic.get(f"gemm_split_{mode}"),
-                                        "what": "plane split of both operands of one k-point"}
+                                        "what": "exponent pass over both operands of one k-point (reads only)"
+                                        if cvt else "plane split of both operands of one k-point"}
     for v in rl.values():
         v["frac"] = v["achieved"] / v["peak"]
     # the (a) stage against its unit of 8 W + 16 N^3 bytes per (band, k):
@@ -467,7 +474,7 @@ def main():
     roof["kernel"] = dom
     roof["avg_launch_ms"] = rl[dom]["avg_launch_ms"]
     roof["timing"] = "breakdown step after the timed steps: one band chunk in flight, HIP events per launch"
-    roof["settings"] = {k: timed_settings.get(k) for k in ("fft_xmask", "fft_nt", "fft_tile", "sphere_parts")}
+    roof["settings"] = {k: timed_settings.get(k) for k in ("fft_xmask", "fft_nt", "fft_tile", "sphere_parts", "gemm_planes")}
     # whole setup: algorithmic bytes of all its kernels over the pipelined time
     setup_bytes = sum(b for c, b in alg.items() if c in rl) * nk * -(-nb // NB)
     pipe = {"achieved": setup_bytes / R["setup_s"] / 1e9, "unit": "GB/s", "peak": PEAK_HBM / 1e9,

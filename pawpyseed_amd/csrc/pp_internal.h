@@ -243,19 +243,27 @@ std::complex<double> offsite_overlap_host(const double *dcoord, const double *k1
                                           const double *const *s1, int n1, const double *k2, const double *f2,
                                           const double *const *s2, int n2, int l1, int m1, int l2, int m2);
 
-// Split-K GEMM work-group -> (K chunk, output tile): work-groups are dealt
-// round-robin to the 8 XCDs in launch order, so block b runs on XCD b % 8;
-// giving every tile of one chunk the same XCD lets that XCD's L2 hold the
-// chunk's A and B rows for all of its tiles, instead of every XCD reading
-// every chunk.  Grid: gemm_grid(tiles, nsplit) work-groups, of which those
-// past nsplit chunks do nothing.
+// Work-group -> (split-K chunk, output tile) of the overlap GEMMs.  The
+// hardware hands work-groups round-robin to the 8 XCDs in launch order, so
+// block b runs on XCD b % 8.  The (chunk, tile) items, chunk-major, are cut
+// into 8 contiguous ranges and XCD x takes range x: with many chunks an XCD
+// holds whole chunks, so its L2 keeps a chunk's A and B rows for all of the
+// chunk's tiles; with fewer chunks than XCDs the tiles of a chunk spread
+// over all of them (consecutive tiles share their A rows).  Grid:
+// gemm_grid(tiles, nsplit) work-groups, of which those past the items do
+// nothing.
 __device__ __forceinline__ bool gemm_block(int ntiles, int nsplit, int &split, int &tile) {
-  const int b = (int)blockIdx.x, r = b >> 3;
-  split = (r / ntiles) * 8 + (b & 7);
-  tile = r % ntiles;
-  return split < nsplit;
+  const long total = (long)ntiles * nsplit, per = (total + 7) / 8;
+  const long r = (long)(blockIdx.x >> 3), i = (long)(blockIdx.x & 7) * per + r;
+  if (r >= per || i >= total)
+    return false;
+  split = (int)(i / ntiles);
+  tile = (int)(i % ntiles);
+  return true;
 }
-inline dim3 gemm_grid(int ntiles, int nsplit) { return dim3((unsigned)(((nsplit + 7) / 8) * 8 * ntiles)); }
+inline dim3 gemm_grid(int ntiles, int nsplit) {
+  return dim3((unsigned)(((long)ntiles * nsplit + 7) / 8 * 8));
+}
 
 // Autotuner (pp_api.hip): coordinate descent over the named parameters,
 // timing run() on the context's stream.

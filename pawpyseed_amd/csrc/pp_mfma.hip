@@ -803,6 +803,10 @@ int plane_gemm_pipeline(pp_ctx *ctx, int precision, int M, int N, int K, const v
     int kk;
     operand(b, pa, pb, kk);
     PP_CHECK_HIP(hipStreamWaitEvent(aux, ctx->gemm_done[slot], 0));
+    if (cvt && precision != PP_FP16X3) {  // the GEMM splits bf16 rows itself: nothing to prepare
+      PP_CHECK_HIP(hipEventRecord(ctx->split_done[slot], aux));
+      return PP_OK;
+    }
     hipEvent_t ev;
     ctx->stream = aux;
     tbegin(ctx, K_GEMM_SPLIT, &ev);
