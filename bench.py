@@ -88,6 +88,15 @@ def cpu_cores():
     return max(1, n)
 
 
+def cpu_fft(orc):
+    """The 3D FFT the CPU baseline runs: MKL's (the reference's library),
+    loaded from libmkl_rt through its FFTW3 entry points, where the host has
+    it; else the oracle's own mixed-radix transform."""
+    if orc.fft_use_mkl(True):
+        return "MKL (libmkl_rt, FFTW3 interface over DFTI, sequential inside each OpenMP thread)"
+    return "the oracle's own mixed-radix FFT (libmkl_rt not loadable)"
+
+
 def cpu_model():
     """The host CPU's model name (/proc/cpuinfo), for the CPU baseline."""
     try:
@@ -965,6 +974,7 @@ def bench_cfg3(args, torch, B):
     errs, cpu = {}, None
     if not args.no_cpu:
         cores = cpu_cores()
+        fft_lib = cpu_fft(orc)
         tsum, npairs = 0.0, 0
         want_P = {}
         for s in range(1, 4):
@@ -982,7 +992,7 @@ def bench_cfg3(args, torch, B):
         nproj_all = sum(int(wl.nbands[s][k]) * (3888 if len(wl.frac[s]) == 216 else 3870) for s in range(1, 4)
                         for k in range(nk))
         cpu = {"value": nproj_all / (per_pair * total_pairs), "unit": "band-projections/s", "cores": cores,
-               "kind": "port", "cpu_model": cpu_model(),
+               "kind": "port", "cpu_model": cpu_model(), "fft": fft_lib,
                "sample": f"oracle scatter/FFT/sphere sums of {npairs} (structure, k, band) samples in {tsum:.1f} s "
                          f"on {cores} threads ({per_pair * 1e3:.1f} ms each), extrapolated to all {total_pairs}"}
         for p, (gp, go) in keep_out.items():
@@ -1046,6 +1056,7 @@ def cpu_and_errors(args, torch, wl, Gs, CR, CS, projR, projS, Om, ob, plans, res
     nk = len(Gs)
     nb = CR.shape[1]
     cores = cpu_cores()
+    fft_lib = cpu_fft(orc)
     # S sphere tables (the per-structure setup of the CPU path)
     t0 = time.perf_counter()
     TS = orc.site_tables(wl.lattice, wl.fftg, wl.frac_S, wl.species)
@@ -1134,6 +1145,7 @@ def cpu_and_errors(args, torch, wl, Gs, CR, CS, projR, projS, Om, ob, plans, res
     log("errors (max over k of max|d|/max|ref|):", json.dumps(worst))
     return {
         "cpu_baseline": {"value": cpu_value, "unit": "band-projections/s", "cores": cores, "kind": "port",
+                         "fft": fft_lib,
                          "sample": f"oracle on the host: S sphere tables (215 sites, {t_tab:.1f}s) + scatter/FFT/"
                                    f"sphere sums of {npairs} of {nk * nb} (band,k) pairs in {t_pairs:.1f}s "
                                    f"(median per-pair time of {len(per_pair_batches)} timed batches), "

@@ -8,10 +8,15 @@
  * routines (scipy splines and spherical harmonics, numpy FFTs), brute-force
  * enumerations and closed forms, and against the reference's own fixture
  * test_files/WAVECAR.  The FFT below is a project-written mixed-radix
- * transform (the reference uses MKL DFTI).
+ * transform (the reference uses MKL DFTI); or_fft_use_mkl switches the 3D
+ * backward transforms to MKL's, through the FFTW3 interface that libmkl_rt
+ * exports, where that library can be loaded at run time (the CPU baseline
+ * of the benchmark times the reference's FFT library that way).
  */
+#define _GNU_SOURCE /* setenv, dlopen */
 #include "pawpy_oracle.h"
 
+#include <dlfcn.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -400,8 +405,79 @@ static void fft_batch(const fft_plan *pl, double *a, double *b, int nb) {
     memcpy(a, in, sizeof(double) * 2 * (size_t)n * nb);
 }
 
+/* MKL's 3D transforms, through the FFTW3 entry points of libmkl_rt
+ * (FFTW3 API: fftw_plan_dft_3d(n0, n1, n2, in, out, sign, flags), row-major
+ * complex doubles, sign +1 = the unnormalised backward transform, flag
+ * FFTW_ESTIMATE = 1 << 6; fftw_execute_dft reuses a plan on another array
+ * of the same layout and is thread safe).  MKL runs sequentially inside:
+ * the callers' OpenMP loops over (band, k) provide the parallelism.  One
+ * plan for the last grid, made under a lock. */
+typedef void *(*plan3d_fn)(int, int, int, double *, double *, int, unsigned);
+typedef void (*exec_fn)(void *, double *, double *);
+typedef void (*destroy_fn)(void *);
+static plan3d_fn mkl_plan3d;
+static exec_fn mkl_exec;
+static destroy_fn mkl_destroy;
+static int mkl_use;           /* 1: 3D transforms through MKL */
+static void *mkl_plan;
+static int mkl_dims[3];
+
+int or_fft_use_mkl(int enable) {
+  if (!enable) {
+    mkl_use = 0;
+    return 0;
+  }
+  if (!mkl_exec) {
+    const char *env = getenv("PP_ORACLE_MKL");
+    const char *cands[] = {env, "libmkl_rt.so.1", "libmkl_rt.so", "/opt/conda/lib/libmkl_rt.so.1",
+                           "/opt/intel/oneapi/mkl/latest/lib/libmkl_rt.so.2"};
+    void *h = NULL;
+    /* MKL reads its threading layer at its first call */
+    setenv("MKL_THREADING_LAYER", "SEQUENTIAL", 0);
+    for (unsigned i = 0; i < sizeof(cands) / sizeof(cands[0]) && !h; i++)
+      if (cands[i])
+        h = dlopen(cands[i], RTLD_NOW | RTLD_LOCAL);
+    if (!h)
+      return 0;
+    mkl_plan3d = (plan3d_fn)dlsym(h, "fftw_plan_dft_3d");
+    mkl_exec = (exec_fn)dlsym(h, "fftw_execute_dft");
+    mkl_destroy = (destroy_fn)dlsym(h, "fftw_destroy_plan");
+    if (!mkl_plan3d || !mkl_exec || !mkl_destroy) {
+      mkl_exec = NULL;
+      return 0;
+    }
+  }
+  mkl_use = 1;
+  return 1;
+}
+
+/* x in place through MKL; 0 when MKL is not in use */
+static int mkl_ifft3d(double *x, const int *fftg) {
+  if (!mkl_use)
+    return 0;
+  void *plan;
+#pragma omp critical(or_mkl_plan)
+  {
+    if (!mkl_plan || mkl_dims[0] != fftg[0] || mkl_dims[1] != fftg[1] || mkl_dims[2] != fftg[2]) {
+      if (mkl_plan)
+        mkl_destroy(mkl_plan);
+      mkl_plan = mkl_plan3d(fftg[0], fftg[1], fftg[2], x, x, 1, 1U << 6);
+      mkl_dims[0] = fftg[0];
+      mkl_dims[1] = fftg[1];
+      mkl_dims[2] = fftg[2];
+    }
+    plan = mkl_plan;
+  }
+  if (!plan)
+    return 0;
+  mkl_exec(plan, x, x);
+  return 1;
+}
+
 /* work: 4 * nmax * nmax doubles (two [n][nb] complex buffers) */
 static void ifft3d_with(double *x, const int *fftg, fft_plan *pl, double *work) {
+  if (mkl_ifft3d(x, fftg))
+    return;
   int n0 = fftg[0], n1 = fftg[1], n2 = fftg[2];
   int nmax = n0 > n1 ? n0 : n1;
   nmax = nmax > n2 ? nmax : n2;

@@ -53,6 +53,12 @@ int or_site_table(const double *lattice, const int *fftg, const double *coord,
 int or_site_capacity(const double *lattice, const int *fftg, double rmax);
 
 /* ---- FFT: linalg.c:14-45 (backward, scale 1/sqrt(V)) ---- */
+/* Switch the 3D backward transforms (or_fft3d, or_ifft3d_inplace,
+ * or_setup_projections) to MKL's, loaded at run time from libmkl_rt
+ * (PP_ORACLE_MKL names another path); returns 1 when MKL is in use, 0 when
+ * it cannot be loaded (the oracle's own transform stays).  enable = 0
+ * switches back. */
+int or_fft_use_mkl(int enable);
 void or_fft3d(double *x, const double *lattice, const int *Gs,
               const float *Cs_c64, int nw, const int *fftg);
 /* plain backward 3D DFT with no scale and no scatter, for tests */

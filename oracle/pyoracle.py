@@ -49,7 +49,15 @@ def lib():
         L.or_sbtfac.restype = ct.c_double
         L.or_sbtfac.argtypes = [ct.c_int] * 5
         L.or_wave_interpolate.restype = ct.c_double
+        L.or_fft_use_mkl.restype = ct.c_int
+        L.or_fft_use_mkl.argtypes = [ct.c_int]
     return _LIB
+
+
+def fft_use_mkl(enable=True):
+    """Run the oracle's 3D backward transforms through MKL (libmkl_rt, loaded
+    at run time) instead of its own; True when MKL is in use."""
+    return bool(lib().or_fft_use_mkl(1 if enable else 0))
 
 
 def _p(a, t):
