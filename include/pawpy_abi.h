@@ -275,7 +275,8 @@ int pp_ctx_set_tuning(pp_ctx *ctx, int fft_bands, int fft_tile, int fft_prune,
  * CU with 64-real k-steps over planes split beforehand, 2 two work-groups
  * per CU with 32-real k-steps, 3, default: as 2 with the FP32 operands
  * split inside the GEMM, where their rows are 16-byte aligned: no plane
- * buffer; all three give bit-identical results). */
+ * buffer; 4 as 3 with 256 x 128 output tiles of 8 wavefronts; all give
+ * bit-identical results). */
 int pp_ctx_set_param(pp_ctx *ctx, const char *name, int value);
 /* Current value of a parameter of pp_ctx_set_param (0 = the built-in
  * default for fft_tile, fft_bands, sphere_parts, gemm_splitk). */

@@ -543,7 +543,7 @@ int pp_ctx_set_param(pp_ctx *ctx, const char *name, int value) {
   else if (n == "multi_items")
     ctx->tune_multi_items = value < 1 ? 1 : value > 64 ? 64 : value;
   else if (n == "gemm_planes")
-    ctx->tune_gemm_planes = value >= 3 ? 3 : value == 2 ? 2 : 1;
+    ctx->tune_gemm_planes = value >= 4 ? 4 : value == 3 ? 3 : value == 2 ? 2 : 1;
   else if (n == "fft_tile")
     return pp_ctx_set_tuning(ctx, ctx->tune_fft_bands, value, ctx->tune_fft_prune, ctx->tune_sphere_parts,
                              ctx->tune_splitk);

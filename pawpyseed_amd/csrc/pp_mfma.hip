@@ -362,65 +362,76 @@ __device__ __forceinline__ void split8(f32x4 x0, f32x4 x1, int e, u32x4 &hi, u32
 // complex terms each), split into hi / lo planes on their way into LDS with
 // the same arithmetic, so that no plane buffer goes through HBM (fp16: the
 // exponents EA / EB of an exponent-only k_split_planes pass).
-template <int NT, int KS, bool F16, int CH, int WPE, bool CVT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_cgemm_planes(
+template <int NT, int KS, bool F16, int CH, int WPE, bool CVT, int TMW>
+__global__ __launch_bounds__(128 * TMW) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_cgemm_planes(
     int M, int N, int nsplit, int K, const void *__restrict__ Av, long lda, const void *__restrict__ Bv, long ldb,
     long sp, long cs, const int *__restrict__ EA, const int *__restrict__ EB, long sE, cplx<float> *__restrict__ C,
     long ldc, long sSplit) {
   static_assert(!CVT || NT == 2, "in-kernel splits form two planes");
+  static_assert(TMW == 2 || TMW == 4, "2 or 4 wavefronts along M");
   constexpr int LS = KS + 8;           // LDS row stride (16-bit words): 16 bytes of padding
   constexpr int HP = KS / 32;          // 16-byte loads per (row, plane) per thread
-  __shared__ __attribute__((aligned(16))) unsigned short As[2][NT][128][LS];
+  constexpr int TM = 64 * TMW;         // output rows (A rows) of the tile; 128 columns
+  constexpr int RS = 32 * TMW;         // rows one pass of the work-group's threads stages
+  constexpr int RA = TM / RS, RB = 128 / RS;  // passes over A (2) and B (2 or 1)
+  __shared__ __attribute__((aligned(16))) unsigned short As[2][NT][TM][LS];
   __shared__ __attribute__((aligned(16))) unsigned short Bs[2][NT][128][LS];
   const int tilesN = (N + 127) / 128;
   int split, tile;
-  if (!gemm_block(((M + 127) / 128) * tilesN, nsplit, split, tile))
+  if (!gemm_block(((M + TM - 1) / TM) * tilesN, nsplit, split, tile))
     return;
   const int tm = tile / tilesN, tn = tile % tilesN;
   const long k0 = (long)split * CH;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const int lrow = threadIdx.x >> 2, part = threadIdx.x & 3;
-  // This thread's rows of the tile: lrow and lrow + 64 of A and of B.  A row
-  // past the matrix reads the last row instead: it only feeds outputs of
-  // rows (columns) past the matrix, which are not stored.
-  const int ra0 = min(tm * 128 + lrow, M - 1), ra1 = min(tm * 128 + 64 + lrow, M - 1);
-  const int rb0 = min(tn * 128 + lrow, N - 1), rb1 = min(tn * 128 + 64 + lrow, N - 1);
+  // This thread's rows of the tile: lrow + h * RS of A and of B.  A row past
+  // the matrix reads the last row instead: it only feeds outputs of rows
+  // (columns) past the matrix, which are not stored.
+  int ra[RA], rb[RB];
+#pragma unroll
+  for (int h = 0; h < RA; h++)
+    ra[h] = min(tm * TM + h * RS + lrow, M - 1);
+#pragma unroll
+  for (int h = 0; h < RB; h++)
+    rb[h] = min(tn * 128 + h * RS + lrow, N - 1);
   const long kp = k0 + part * (KS / 4);
   // the scale exponents of the plane chunk this K chunk lies in
   const long ec = (long)split * CH / kPlaneChunk * sE;
-  const unsigned short *pa0 = nullptr, *pa1 = nullptr, *pb0 = nullptr, *pb1 = nullptr;
-  const float *xa0 = nullptr, *xa1 = nullptr, *xb0 = nullptr, *xb1 = nullptr;
-  int ea0 = 0, ea1 = 0, eb0 = 0, eb1 = 0;
+  const unsigned short *pa[RA], *pb[RB];
+  const float *xa[RA], *xb[RB];
+  int ea[RA], eb[RB];
   bool whole = true;  // the chunk lies inside the 2K reals of every row
   if constexpr (CVT) {
     const float *A = (const float *)Av, *B = (const float *)Bv;
-    xa0 = A + (long)ra0 * 2 * lda + kp;
-    xa1 = A + (long)ra1 * 2 * lda + kp;
-    xb0 = B + (long)rb0 * 2 * ldb + kp;
-    xb1 = B + (long)rb1 * 2 * ldb + kp;
-    whole = k0 + CH <= 2L * K;
-    if constexpr (F16) {
-      ea0 = EA[ec + ra0];
-      ea1 = EA[ec + ra1];
-      eb0 = EB[ec + rb0];
-      eb1 = EB[ec + rb1];
+#pragma unroll
+    for (int h = 0; h < RA; h++) {
+      xa[h] = A + (long)ra[h] * 2 * lda + kp;
+      ea[h] = F16 ? EA[ec + ra[h]] : 0;
     }
+#pragma unroll
+    for (int h = 0; h < RB; h++) {
+      xb[h] = B + (long)rb[h] * 2 * ldb + kp;
+      eb[h] = F16 ? EB[ec + rb[h]] : 0;
+    }
+    whole = k0 + CH <= 2L * K;
   } else {
     // chunk-major planes: rows of lda reals within the chunk's block of
     // stride cs
     const long cb = k0 / lda * cs, kc0 = k0 % lda + part * (KS / 4);
     const unsigned short *A = (const unsigned short *)Av + cb, *B = (const unsigned short *)Bv + cb;
-    pa0 = A + (long)ra0 * lda + kc0;
-    pa1 = A + (long)ra1 * lda + kc0;
-    pb0 = B + (long)rb0 * ldb + kc0;
-    pb1 = B + (long)rb1 * ldb + kc0;
+#pragma unroll
+    for (int h = 0; h < RA; h++)
+      pa[h] = A + (long)ra[h] * lda + kc0;
+#pragma unroll
+    for (int h = 0; h < RB; h++)
+      pb[h] = B + (long)rb[h] * ldb + kc0;
   }
   const long nreal_left = 2L * K - kp;  // reals of a row from this thread's first
-  // global -> register staging of one k-step: [row half][plane][16-byte piece]
-  // (planes), or [row half][16-byte piece] of FP32 values (CVT)
-  u32x4 ga0[NT][HP], ga1[NT][HP], gb0[NT][HP], gb1[NT][HP];
-  f32x4 fa0[2 * HP], fa1[2 * HP], fb0[2 * HP], fb1[2 * HP];
+  // global -> register staging of one k-step: [row pass][plane][16-byte piece]
+  // (planes), or [row pass][16-byte piece] of FP32 values (CVT)
+  u32x4 ga[RA][NT][HP], gb[RB][NT][HP];
+  f32x4 fa[RA][2 * HP], fb[RB][2 * HP];
   auto ld4 = [&](const float *x, long off) -> f32x4 {
     if (whole)
       return *reinterpret_cast<const f32x4 *>(x + off);
@@ -435,17 +446,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
     const long off_ = (long)(step) * KS;                                                                         \
     if constexpr (CVT) {                                                                                         \
       _Pragma("unroll") for (int q = 0; q < 2 * HP; q++) {                                                       \
-        fa0[q] = ld4(xa0, off_ + q * 4);                                                                         \
-        fa1[q] = ld4(xa1, off_ + q * 4);                                                                         \
-        fb0[q] = ld4(xb0, off_ + q * 4);                                                                         \
-        fb1[q] = ld4(xb1, off_ + q * 4);                                                                         \
+        _Pragma("unroll") for (int h = 0; h < RA; h++) fa[h][q] = ld4(xa[h], off_ + q * 4);                      \
+        _Pragma("unroll") for (int h = 0; h < RB; h++) fb[h][q] = ld4(xb[h], off_ + q * 4);                      \
       }                                                                                                          \
     } else {                                                                                                     \
       _Pragma("unroll") for (int t = 0; t < NT; t++) _Pragma("unroll") for (int q = 0; q < HP; q++) {           \
-        ga0[t][q] = *reinterpret_cast<const u32x4 *>(pa0 + t * sp + off_ + q * 8);                               \
-        ga1[t][q] = *reinterpret_cast<const u32x4 *>(pa1 + t * sp + off_ + q * 8);                               \
-        gb0[t][q] = *reinterpret_cast<const u32x4 *>(pb0 + t * sp + off_ + q * 8);                               \
-        gb1[t][q] = *reinterpret_cast<const u32x4 *>(pb1 + t * sp + off_ + q * 8);                               \
+        _Pragma("unroll") for (int h = 0; h < RA; h++) ga[h][t][q] =                                             \
+            *reinterpret_cast<const u32x4 *>(pa[h] + t * sp + off_ + q * 8);                                     \
+        _Pragma("unroll") for (int h = 0; h < RB; h++) gb[h][t][q] =                                             \
+            *reinterpret_cast<const u32x4 *>(pb[h] + t * sp + off_ + q * 8);                                     \
       }                                                                                                          \
     }                                                                                                            \
   }
@@ -453,17 +462,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
   {                                                                                                              \
     if constexpr (CVT) {                                                                                         \
       _Pragma("unroll") for (int q = 0; q < HP; q++) {                                                           \
-        split8<F16>(fa0[2 * q], fa0[2 * q + 1], ea0, ga0[0][q], ga0[1][q]);                                      \
-        split8<F16>(fa1[2 * q], fa1[2 * q + 1], ea1, ga1[0][q], ga1[1][q]);                                      \
-        split8<F16>(fb0[2 * q], fb0[2 * q + 1], eb0, gb0[0][q], gb0[1][q]);                                      \
-        split8<F16>(fb1[2 * q], fb1[2 * q + 1], eb1, gb1[0][q], gb1[1][q]);                                      \
+        _Pragma("unroll") for (int h = 0; h < RA; h++)                                                           \
+            split8<F16>(fa[h][2 * q], fa[h][2 * q + 1], ea[h], ga[h][0][q], ga[h][1][q]);                        \
+        _Pragma("unroll") for (int h = 0; h < RB; h++)                                                           \
+            split8<F16>(fb[h][2 * q], fb[h][2 * q + 1], eb[h], gb[h][0][q], gb[h][1][q]);                        \
       }                                                                                                          \
     }                                                                                                            \
     _Pragma("unroll") for (int t = 0; t < NT; t++) _Pragma("unroll") for (int q = 0; q < HP; q++) {             \
-      *reinterpret_cast<u32x4 *>(&As[buf][t][lrow][part * (KS / 4) + q * 8]) = ga0[t][q];                        \
-      *reinterpret_cast<u32x4 *>(&As[buf][t][64 + lrow][part * (KS / 4) + q * 8]) = ga1[t][q];                   \
-      *reinterpret_cast<u32x4 *>(&Bs[buf][t][lrow][part * (KS / 4) + q * 8]) = gb0[t][q];                        \
-      *reinterpret_cast<u32x4 *>(&Bs[buf][t][64 + lrow][part * (KS / 4) + q * 8]) = gb1[t][q];                   \
+      _Pragma("unroll") for (int h = 0; h < RA; h++)                                                             \
+          *reinterpret_cast<u32x4 *>(&As[buf][t][h * RS + lrow][part * (KS / 4) + q * 8]) = ga[h][t][q];         \
+      _Pragma("unroll") for (int h = 0; h < RB; h++)                                                             \
+          *reinterpret_cast<u32x4 *>(&Bs[buf][t][h * RS + lrow][part * (KS / 4) + q * 8]) = gb[h][t][q];         \
     }                                                                                                            \
   }
   floatx4 cr[4][4], ci[4][4];
@@ -525,7 +534,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
   C += (long)split * sSplit;
 #pragma unroll
   for (int mi = 0; mi < 4; mi++) {
-    const int r0 = tm * 128 + wm * 64 + mi * 16 + 4 * (lane >> 4);
+    const int r0 = tm * TM + wm * 64 + mi * 16 + 4 * (lane >> 4);
 #pragma unroll
     for (int nj = 0; nj < 4; nj++) {
       const int j = tn * 128 + wn * 64 + nj * 16 + (lane & 15);
@@ -617,7 +626,7 @@ void plane_layout(int precision, int M, int N, int K, bool cvt, long &ldp, int &
 // Whether the GEMM splits its operands itself under the context's settings
 // (gemm_planes 3, two-plane modes, 16-byte aligned FP32 rows).
 bool plane_cvt(pp_ctx *ctx, int precision, const void *A, long lda, const void *B, long ldb) {
-  return ctx->tune_gemm_planes == 3 && (precision == PP_FP16X3 || precision == PP_BF16X3) && lda % 2 == 0 &&
+  return ctx->tune_gemm_planes >= 3 && (precision == PP_FP16X3 || precision == PP_BF16X3) && lda % 2 == 0 &&
          ldb % 2 == 0 && (uintptr_t)A % 16 == 0 && (uintptr_t)B % 16 == 0;
 }
 
@@ -655,9 +664,11 @@ int launch_split_planes(hipStream_t st, int precision, int M, int N, int K, cons
 
 // The plane GEMM over the planes in ws (or, with cvt, over A and B
 // themselves) into FP32 partials P ([nsplit][M][ldp_out], split stride
-// sSplit).  No allocation.
+// sSplit); variant: the context's gemm_planes (1: 128x128 tiles, one
+// work-group per CU; 2, 3: two per CU; 4: 256x128 tiles of 8 wavefronts,
+// with cvt).  No allocation.
 int launch_plane_gemm(hipStream_t st, int precision, int M, int N, int K, const void *A, long lda, const void *B,
-                      long ldb, const void *ws, cplx<float> *P, long ldp_out, long sSplit, int wgs_per_cu,
+                      long ldb, const void *ws, cplx<float> *P, long ldp_out, long sSplit, int variant,
                       bool cvt) {
   long ldp;
   int NT, nsplit;
@@ -672,24 +683,33 @@ int launch_plane_gemm(hipStream_t st, int precision, int M, int N, int K, const 
   const long sE = M + N;
   constexpr int CH = kPlaneChunk / 2;
 #define PP_PG(NTV, KSV, F, WPEV, CVTV, AA, LA, BB, LB, EE)                                                      \
-  hipLaunchKernelGGL((k_cgemm_planes<NTV, KSV, F, CH, WPEV, CVTV>), grd, dim3(256), 0, st, M, N, nsplit, K, AA, LA, BB, \
+  hipLaunchKernelGGL((k_cgemm_planes<NTV, KSV, F, CH, WPEV, CVTV, 2>), grd, dim3(256), 0, st, M, N, nsplit, K, AA, LA,  \
+                     BB, LB, sp, cs, EE, EE ? EE + M : nullptr, sE, P, ldp_out, sSplit)
+#define PP_PG4(F, AA, LA, BB, LB, EE)                                                                          \
+  hipLaunchKernelGGL((k_cgemm_planes<2, 32, F, CH, 2, true, 4>), grd4, dim3(512), 0, st, M, N, nsplit, K, AA, LA, BB, \
                      LB, sp, cs, EE, EE ? EE + M : nullptr, sE, P, ldp_out, sSplit)
+  const dim3 grd4 = gemm_grid(((M + 255) / 256) * ((N + 127) / 128), nsplit);
   const int *noE = nullptr;
-  if (cvt && precision == PP_FP16X3)
+  if (cvt && variant >= 4 && precision == PP_FP16X3)
+    PP_PG4(true, A, lda, B, ldb, E);
+  else if (cvt && variant >= 4)
+    PP_PG4(false, A, lda, B, ldb, noE);
+  else if (cvt && precision == PP_FP16X3)
     PP_PG(2, 32, true, 2, true, A, lda, B, ldb, E);
   else if (cvt)
     PP_PG(2, 32, false, 2, true, A, lda, B, ldb, noE);
-  else if (precision == PP_FP16X3 && wgs_per_cu >= 2)
+  else if (precision == PP_FP16X3 && variant >= 2)
     PP_PG(2, 32, true, 2, false, PA, LP, PB, LP, E);
   else if (precision == PP_FP16X3)
     PP_PG(2, 64, true, 1, false, PA, LP, PB, LP, E);
-  else if (NT == 2 && wgs_per_cu >= 2)
+  else if (NT == 2 && variant >= 2)
     PP_PG(2, 32, false, 2, false, PA, LP, PB, LP, noE);
   else if (NT == 3)
     PP_PG(3, 32, false, 1, false, PA, LP, PB, LP, noE);
   else
     PP_PG(2, 64, false, 1, false, PA, LP, PB, LP, noE);
 #undef PP_PG
+#undef PP_PG4
   PP_CHECK_HIP(hipGetLastError());
   return PP_OK;
 }

@@ -274,23 +274,24 @@ def test_overlap_split_planes_row_scales(ctx, precision):
 @pytest.mark.parametrize("precision", ["fp16x3", "bf16x3"])
 def test_overlap_split_planes_work_group_variants(ctx, precision):
     """The two-plane GEMM with one work-group per CU (64-real k-steps), with
-    two (32-real k-steps), and with two whose GEMM splits the FP32 operands
-    itself (no plane buffer): every output accumulates the same products of
-    the same planes in the same order, so the results are bit-identical;
-    130 x 67 bands, two k-points with odd plane-wave counts (a partial last
-    chunk), against FP64 <= 1e-6 (fp16x3) / 2e-5 (bf16x3)."""
+    two (32-real k-steps), with two whose GEMM splits the FP32 operands
+    itself (no plane buffer), and with 256 x 128 tiles of 8 wavefronts that
+    split them too: every output accumulates the same products of the same
+    planes in the same order, so the results are bit-identical; 300 x 67
+    bands (a partial 256-row tile), two k-points with odd plane-wave counts
+    (a partial last chunk), against FP64 <= 1e-6 (fp16x3) / 2e-5 (bf16x3)."""
     before = ctx.get_param("gemm_planes")
     try:
         out = {}
-        for gp in (1, 2, 3):
+        for gp in (1, 2, 3, 4):
             ctx.set_param("gemm_planes", gp)
-            out[gp], want = _overlap_case(ctx, precision, nbR=130, nbS=67)
+            out[gp], want = _overlap_case(ctx, precision, nbR=300, nbS=67)
     finally:
         ctx.set_param("gemm_planes", before)
     err = normwise(out[2], want)
-    print(precision, "gemm_planes 1 / 3 vs 2 identical:", np.array_equal(out[1], out[2]),
-          np.array_equal(out[3], out[2]), "error", err, flush=True)
-    assert np.array_equal(out[1], out[2]) and np.array_equal(out[3], out[2])
+    same = {gp: np.array_equal(out[gp], out[2]) for gp in (1, 3, 4)}
+    print(precision, "gemm_planes 1 / 3 / 4 vs 2 identical:", same, "error", err, flush=True)
+    assert all(same.values())
     assert err < (1e-6 if precision == "fp16x3" else 2e-5)
 
 
@@ -413,6 +414,59 @@ def test_augmentation_matches_oracle(ctx):
                                             np.zeros(nsR, np.int32), pS[:, band], TS, M_R, M_S, [dO], [sp.ls],
                                             flip=flip)
             np.testing.assert_allclose(got[:, :, band], want.reshape(nbR, nkt).T, atol=1e-11)
+
+
+def test_augmentation_dense_delta_o_cfg0(ctx):
+    """The 64-site cell of the 400 eV / 60^3 workload with a dense ΔO (all
+    4 x 4 channel entries random, cross-l included) over all 64 bands: O_M
+    uses only the entries between projectors of equal (l, m)
+    (projector.c:897-906), so the GPU result equals the oracle's with the
+    same dense matrix (1e-11) and, to rounding, the GPU result with the
+    cross-l entries zeroed (1e-13)."""
+    from pawpyseed_amd.core import batched as B
+    wl = synthetic.config0()
+    rng = np.random.default_rng(17)
+    dense = synthetic.channel_delta_o(rng, wl.species.ls, dense=True)
+    ls = np.asarray(wl.species.ls)
+    assert np.abs(dense[ls[:, None] != ls[None, :]]).min() > 0  # really dense
+    same_l = dense.copy()
+    same_l[ls[:, None] != ls[None, :]] = 0.0
+    Gs = wl.ensure_gvecs()
+    CR = wl.coefficients("R", nband=64)
+    CS = wl.coefficients("S", nband=64)
+    labels = np.zeros(len(wl.frac_R), np.int32)
+    proj = {}
+    for name, frac, C in (("R", wl.frac_R, CR), ("S", wl.frac_S, CS)):
+        plan = B.SpherePlan(ctx, wl.lattice, wl.fftg, frac, labels, [wl.species], "fp64")
+        Cd, Gd, gofs, nw = device_inputs(Gs, C)
+        out = torch.zeros((1, 64, plan.ptot), dtype=torch.complex128, device="cuda")
+        B.setup_projections(ctx, plan, wl.kpts, gofs, Gd, Cd, out, "fp64")
+        proj[name] = (out, plan.proj_off.copy())
+        plan.close()
+    (pR, offR), (pS, offS) = proj["R"], proj["S"]
+    got = {}
+    for key, dO in (("dense", dense), ("same_l", same_l)):
+        o = torch.zeros((1, 64, 64), dtype=torch.complex128, device="cuda")
+        B.augmentation(ctx, 1, 1, False, pR, offR, labels, pS, offS, labels, wl.M_R, wl.M_S, [wl.species], [dO], o)
+        got[key] = o.cpu().numpy()[0]
+    pRh, pSh = pR.cpu().numpy(), pS.cpu().numpy()
+
+    class T:
+        pass
+
+    TR, TS = T(), T()
+    TR.ptot, TR.proj_off = pRh.shape[2], offR
+    TS.ptot, TS.proj_off = pSh.shape[2], offS
+    want = np.zeros((64, 64), complex)
+    for band in range(64):
+        w = orc.compensation_terms_m(np.zeros(64, complex), band, 1, 1, pRh, TR, labels, pSh[:, band], TS,
+                                     wl.M_R, wl.M_S, [dense], [wl.species.ls])
+        want[:, band] = w
+    scale = np.abs(want).max()
+    e_or = np.abs(got["dense"] - want).max() / scale
+    e_l = np.abs(got["dense"] - got["same_l"]).max() / scale
+    print(f"cfg0 dense dO: vs oracle {e_or:.2e}, vs equal-l part {e_l:.2e}", flush=True)
+    assert e_or < 1e-11 and e_l < 1e-13
 
 
 def test_empty_and_invalid_inputs(ctx):
