@@ -181,13 +181,32 @@ __global__ __launch_bounds__(256) void k_cgemm_f32(int M, int N, int kchunk, int
     }
 }
 
-// round-to-nearest-even float -> bf16 bits
-__device__ __forceinline__ unsigned short bf16_bits(float x) {
-  unsigned u = __float_as_uint(x);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (unsigned short)(u >> 16);
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// The NT 16-bit planes of two consecutive reals y (already scaled for
+// fp16): each plane is the round-to-nearest-even conversion of what the
+// planes before it leave (one packed conversion per pair on gfx950), the
+// residuals exact in FP32; w[q] holds plane q of the pair, the first real
+// in its low half.
+template <int NT, bool F16>
+__device__ __forceinline__ void split_pair(f32x2 y, unsigned (&w)[NT]) {
+#pragma unroll
+  for (int q = 0; q < NT; q++) {
+    if constexpr (F16) {
+      const f16x2 h = __builtin_convertvector(y, f16x2);
+      w[q] = __builtin_bit_cast(unsigned, h);
+      if (q + 1 < NT)
+        y = y - __builtin_convertvector(h, f32x2);
+    } else {
+      const bf16x2 h = __builtin_convertvector(y, bf16x2);
+      w[q] = __builtin_bit_cast(unsigned, h);
+      if (q + 1 < NT)
+        y = y - __builtin_convertvector(h, f32x2);
+    }
+  }
 }
-__device__ __forceinline__ float bf16_val(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
 
 // Planes are split and scaled in chunks of kPlaneChunk reals (1024 complex
 // terms) of a row; rows are padded with zeros to a multiple of it.
@@ -260,34 +279,23 @@ __global__ __launch_bounds__(64) void k_split_planes(const float *__restrict__ X
     return;
 #pragma unroll
   for (int g = 0; g < G; g++) {
-    unsigned short h[NT][8];
+    unsigned w[4][NT];
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-      if constexpr (F16) {
-        const float y = ldexpf(v[g][i], e);
-        const _Float16 hi = (_Float16)y;
-        const _Float16 lo = (_Float16)(y - (float)hi);
-        h[0][i] = __builtin_bit_cast(unsigned short, hi);
-        h[1][i] = __builtin_bit_cast(unsigned short, lo);
-      } else {
-        float rem = v[g][i];
-#pragma unroll
-        for (int q = 0; q < NT; q++) {
-          h[q][i] = bf16_bits(rem);
-          rem -= bf16_val(h[q][i]);
-        }
-      }
+    for (int i = 0; i < 4; i++) {
+      f32x2 y = {v[g][2 * i], v[g][2 * i + 1]};
+      if constexpr (F16)
+        y = f32x2{ldexpf(y[0], e), ldexpf(y[1], e)};
+      split_pair<NT, F16>(y, w[i]);
     }
-    const long j0 = (long)chunk * kPlaneChunk + g * 512 + threadIdx.x * 8;
 #pragma unroll
     for (int q = 0; q < NT; q++) {
-      uint4 w;
-      w.x = (unsigned)h[q][0] | ((unsigned)h[q][1] << 16);
-      w.y = (unsigned)h[q][2] | ((unsigned)h[q][3] << 16);
-      w.z = (unsigned)h[q][4] | ((unsigned)h[q][5] << 16);
-      w.w = (unsigned)h[q][6] | ((unsigned)h[q][7] << 16);
+      uint4 o;
+      o.x = w[0][q];
+      o.y = w[1][q];
+      o.z = w[2][q];
+      o.w = w[3][q];
       *reinterpret_cast<uint4 *>(P + q * sp + ((long)chunk * (M + N) + r) * kPlaneChunk + g * 512 +
-                                 threadIdx.x * 8) = w;
+                                 threadIdx.x * 8) = o;
     }
   }
 }
@@ -327,31 +335,20 @@ __device__ __forceinline__ floatx4 mfma16(u32x4 a, u32x4 b, floatx4 c) {
 // waits).  Planes hold zeros past
 // the data, so a chunk of CH reals needs no k bounds.  The chunk's FP32
 // sums are its partial sums (row stride ldc, split stride sSplit); fp16
-// planes carry per-(chunk, row) scale exponents EA / EB, undone exactly at
-// the store.
 // hi and lo 16-bit planes of 8 consecutive reals, exactly as k_split_planes
 // forms them (scale 2^e for fp16)
 template <bool F16>
 __device__ __forceinline__ void split8(f32x4 x0, f32x4 x1, int e, u32x4 &hi, u32x4 &lo) {
-  float v[8] = {x0[0], x0[1], x0[2], x0[3], x1[0], x1[1], x1[2], x1[3]};
-  unsigned short h[8], l[8];
-#pragma unroll
-  for (int i = 0; i < 8; i++) {
-    if constexpr (F16) {
-      const float y = ldexpf(v[i], e);
-      const _Float16 a = (_Float16)y;
-      const _Float16 b = (_Float16)(y - (float)a);
-      h[i] = __builtin_bit_cast(unsigned short, a);
-      l[i] = __builtin_bit_cast(unsigned short, b);
-    } else {
-      h[i] = bf16_bits(v[i]);
-      l[i] = bf16_bits(v[i] - bf16_val(h[i]));
-    }
-  }
+  const float v[8] = {x0[0], x0[1], x0[2], x0[3], x1[0], x1[1], x1[2], x1[3]};
 #pragma unroll
   for (int i = 0; i < 4; i++) {
-    hi[i] = (unsigned)h[2 * i] | ((unsigned)h[2 * i + 1] << 16);
-    lo[i] = (unsigned)l[2 * i] | ((unsigned)l[2 * i + 1] << 16);
+    f32x2 y = {v[2 * i], v[2 * i + 1]};
+    if constexpr (F16)
+      y = f32x2{ldexpf(y[0], e), ldexpf(y[1], e)};
+    unsigned w[2];
+    split_pair<2, F16>(y, w);
+    hi[i] = w[0];
+    lo[i] = w[1];
   }
 }
 
