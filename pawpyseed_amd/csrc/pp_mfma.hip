@@ -185,25 +185,33 @@ typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 
-// The NT 16-bit planes of two consecutive reals y (already scaled for
-// fp16): each plane is the round-to-nearest-even conversion of what the
-// planes before it leave (one packed conversion per pair on gfx950), the
-// residuals exact in FP32; w[q] holds plane q of the pair, the first real
-// in its low half.
+// 2^e as a float, for -126 <= e <= 127
+__device__ __forceinline__ float pow2f(int e) { return __uint_as_float((unsigned)(e + 127) << 23); }
+
+// The NT 16-bit planes of two consecutive reals x, each plane the
+// round-to-nearest-even conversion of what the planes before it leave (one
+// packed conversion per pair on gfx950); w[q] holds plane q of the pair,
+// the first real in its low half.  bf16: the residuals are exact in FP32.
+// fp16 (NT = 2): the reals are scaled by sc = 2^e first, hi = fp16(x sc)
+// and lo = fp16(x sc - hi) rounded once from the exact value (v_fma_mix),
+// which equals the rounding of the FP32 residual whenever x sc is exact in
+// FP32 (the residual of a normal FP32 number and its fp16 rounding is).
 template <int NT, bool F16>
-__device__ __forceinline__ void split_pair(f32x2 y, unsigned (&w)[NT]) {
+__device__ __forceinline__ void split_pair(f32x2 x, float sc, unsigned (&w)[NT]) {
+  if constexpr (F16) {
+    static_assert(NT == 2, "fp16 planes come in pairs");
+    const f16x2 h = __builtin_convertvector(x * sc, f16x2);
+    const f16x2 l = {(_Float16)__builtin_fmaf(x[0], sc, -(float)h[0]),
+                     (_Float16)__builtin_fmaf(x[1], sc, -(float)h[1])};
+    w[0] = __builtin_bit_cast(unsigned, h);
+    w[1] = __builtin_bit_cast(unsigned, l);
+  } else {
 #pragma unroll
-  for (int q = 0; q < NT; q++) {
-    if constexpr (F16) {
-      const f16x2 h = __builtin_convertvector(y, f16x2);
+    for (int q = 0; q < NT; q++) {
+      const bf16x2 h = __builtin_convertvector(x, bf16x2);
       w[q] = __builtin_bit_cast(unsigned, h);
       if (q + 1 < NT)
-        y = y - __builtin_convertvector(h, f32x2);
-    } else {
-      const bf16x2 h = __builtin_convertvector(y, bf16x2);
-      w[q] = __builtin_bit_cast(unsigned, h);
-      if (q + 1 < NT)
-        y = y - __builtin_convertvector(h, f32x2);
+        x = x - __builtin_convertvector(h, f32x2);
     }
   }
 }
@@ -217,9 +225,10 @@ static constexpr int kPlaneChunk = 2048;
 // plane, hi = bf16(x), mid = bf16(x - hi), lo = bf16(x - hi - mid), each
 // residual exact in FP32.  fp16 (F16 true, NT = 2): the chunk is first
 // scaled by 2^e, e chosen so that its largest magnitude lies in
-// [2^14, 2^15), then hi = fp16(y), lo = fp16(y - hi): 11 + 11 significant
-// bits, all of them normal for entries within 2^-17 of the chunk's largest,
-// and the rest off by at most 2^-40 of it; e goes to E[chunk][row].  The
+// [2^14, 2^15) (e at most 127, which only chunks below 2^-112 reach), then
+// hi = fp16(y), lo = fp16(y - hi): 11 + 11 significant bits, all of them
+// normal for entries within 2^-17 of the chunk's largest, and the rest off
+// by at most 2^-40 of it; e goes to E[chunk][row].  The
 // rows of X (M rows, stride ldx complex) and Y (N rows) are planes' rows 0..M-1
 // and M..M+N-1.  Planes are chunk-major: real j of row r in plane t is
 // P[t * sp + ((j / 2048) * (M + N) + r) * 2048 + j % 2048] (2K data, zeros
@@ -270,8 +279,8 @@ __global__ __launch_bounds__(64) void k_split_planes(const float *__restrict__ X
       m = fmaxf(m, __shfl_xor(m, o));
     int ex = 0;
     if (m > 0.f && m <= 3.4e38f)
-      (void)frexpf(m, &ex);  // m < 2^ex
-    e = m > 0.f && m <= 3.4e38f ? 15 - ex : 0;
+      (void)frexpf(m, &ex);  // m < 2^ex <= 2^128
+    e = m > 0.f && m <= 3.4e38f ? min(15 - ex, 127) : 0;
     if (threadIdx.x == 0)
       E[(long)chunk * (M + N) + r] = e;
   }
@@ -281,12 +290,8 @@ __global__ __launch_bounds__(64) void k_split_planes(const float *__restrict__ X
   for (int g = 0; g < G; g++) {
     unsigned w[4][NT];
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      f32x2 y = {v[g][2 * i], v[g][2 * i + 1]};
-      if constexpr (F16)
-        y = f32x2{ldexpf(y[0], e), ldexpf(y[1], e)};
-      split_pair<NT, F16>(y, w[i]);
-    }
+    for (int i = 0; i < 4; i++)
+      split_pair<NT, F16>(f32x2{v[g][2 * i], v[g][2 * i + 1]}, pow2f(e), w[i]);
 #pragma unroll
     for (int q = 0; q < NT; q++) {
       uint4 o;
@@ -336,19 +341,31 @@ __device__ __forceinline__ floatx4 mfma16(u32x4 a, u32x4 b, floatx4 c) {
 // the data, so a chunk of CH reals needs no k bounds.  The chunk's FP32
 // sums are its partial sums (row stride ldc, split stride sSplit); fp16
 // hi and lo 16-bit planes of 8 consecutive reals, exactly as k_split_planes
-// forms them (scale 2^e for fp16)
+// forms them (scale sc = 2^e for fp16)
 template <bool F16>
-__device__ __forceinline__ void split8(f32x4 x0, f32x4 x1, int e, u32x4 &hi, u32x4 &lo) {
+__device__ __forceinline__ void split8(f32x4 x0, f32x4 x1, float sc, u32x4 &hi, u32x4 &lo) {
   const float v[8] = {x0[0], x0[1], x0[2], x0[3], x1[0], x1[1], x1[2], x1[3]};
 #pragma unroll
   for (int i = 0; i < 4; i++) {
-    f32x2 y = {v[2 * i], v[2 * i + 1]};
-    if constexpr (F16)
-      y = f32x2{ldexpf(y[0], e), ldexpf(y[1], e)};
     unsigned w[2];
-    split_pair<2, F16>(y, w);
+    split_pair<2, F16>(f32x2{v[2 * i], v[2 * i + 1]}, sc, w);
     hi[i] = w[0];
     lo[i] = w[1];
+  }
+}
+
+// four FP32 values of a row from x + off; WH false: those at or past
+// `left` read as zeros
+template <bool WH>
+__device__ __forceinline__ f32x4 ld4(const float *x, long off, long left) {
+  if constexpr (WH) {
+    return *reinterpret_cast<const f32x4 *>(x + off);
+  } else {
+    f32x4 r;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+      r[i] = off + i < left ? x[off + i] : 0.f;
+    return r;
   }
 }
 
@@ -397,19 +414,19 @@ __global__ __launch_bounds__(128 * TMW) __attribute__((amdgpu_waves_per_eu(WPE, 
   const long ec = (long)split * CH / kPlaneChunk * sE;
   const unsigned short *pa[RA], *pb[RB];
   const float *xa[RA], *xb[RB];
-  int ea[RA], eb[RB];
+  float sa[RA], sb[RB];  // fp16: the rows' chunk scales 2^e
   bool whole = true;  // the chunk lies inside the 2K reals of every row
   if constexpr (CVT) {
     const float *A = (const float *)Av, *B = (const float *)Bv;
 #pragma unroll
     for (int h = 0; h < RA; h++) {
       xa[h] = A + (long)ra[h] * 2 * lda + kp;
-      ea[h] = F16 ? EA[ec + ra[h]] : 0;
+      sa[h] = F16 ? pow2f(EA[ec + ra[h]]) : 1.f;
     }
 #pragma unroll
     for (int h = 0; h < RB; h++) {
       xb[h] = B + (long)rb[h] * 2 * ldb + kp;
-      eb[h] = F16 ? EB[ec + rb[h]] : 0;
+      sb[h] = F16 ? pow2f(EB[ec + rb[h]]) : 1.f;
     }
     whole = k0 + CH <= 2L * K;
   } else {
@@ -429,22 +446,16 @@ __global__ __launch_bounds__(128 * TMW) __attribute__((amdgpu_waves_per_eu(WPE, 
   // (planes), or [row pass][16-byte piece] of FP32 values (CVT)
   u32x4 ga[RA][NT][HP], gb[RB][NT][HP];
   f32x4 fa[RA][2 * HP], fb[RB][2 * HP];
-  auto ld4 = [&](const float *x, long off) -> f32x4 {
-    if (whole)
-      return *reinterpret_cast<const f32x4 *>(x + off);
-    f32x4 r;
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-      r[i] = off + i < nreal_left ? x[off + i] : 0.f;
-    return r;
-  };
-#define PP_FETCH(step)                                                                                           \
+  // WH: the chunk is whole (loads need no bounds); the main loop is
+  // instantiated for both cases under one uniform branch, so that neither
+  // carries a branch per load
+#define PP_FETCH(step, WH)                                                                                       \
   {                                                                                                              \
     const long off_ = (long)(step) * KS;                                                                         \
     if constexpr (CVT) {                                                                                         \
       _Pragma("unroll") for (int q = 0; q < 2 * HP; q++) {                                                       \
-        _Pragma("unroll") for (int h = 0; h < RA; h++) fa[h][q] = ld4(xa[h], off_ + q * 4);                      \
-        _Pragma("unroll") for (int h = 0; h < RB; h++) fb[h][q] = ld4(xb[h], off_ + q * 4);                      \
+        _Pragma("unroll") for (int h = 0; h < RA; h++) fa[h][q] = ld4<WH>(xa[h], off_ + q * 4, nreal_left);      \
+        _Pragma("unroll") for (int h = 0; h < RB; h++) fb[h][q] = ld4<WH>(xb[h], off_ + q * 4, nreal_left);      \
       }                                                                                                          \
     } else {                                                                                                     \
       _Pragma("unroll") for (int t = 0; t < NT; t++) _Pragma("unroll") for (int q = 0; q < HP; q++) {           \
@@ -460,9 +471,9 @@ __global__ __launch_bounds__(128 * TMW) __attribute__((amdgpu_waves_per_eu(WPE, 
     if constexpr (CVT) {                                                                                         \
       _Pragma("unroll") for (int q = 0; q < HP; q++) {                                                           \
         _Pragma("unroll") for (int h = 0; h < RA; h++)                                                           \
-            split8<F16>(fa[h][2 * q], fa[h][2 * q + 1], ea[h], ga[h][0][q], ga[h][1][q]);                        \
+            split8<F16>(fa[h][2 * q], fa[h][2 * q + 1], sa[h], ga[h][0][q], ga[h][1][q]);                        \
         _Pragma("unroll") for (int h = 0; h < RB; h++)                                                           \
-            split8<F16>(fb[h][2 * q], fb[h][2 * q + 1], eb[h], gb[h][0][q], gb[h][1][q]);                        \
+            split8<F16>(fb[h][2 * q], fb[h][2 * q + 1], sb[h], gb[h][0][q], gb[h][1][q]);                        \
       }                                                                                                          \
     }                                                                                                            \
     _Pragma("unroll") for (int t = 0; t < NT; t++) _Pragma("unroll") for (int q = 0; q < HP; q++) {             \
@@ -483,49 +494,47 @@ __global__ __launch_bounds__(128 * TMW) __attribute__((amdgpu_waves_per_eu(WPE, 
   constexpr int nsteps = CH / KS;
   static_assert(nsteps >= 2, "at least two k-steps per chunk");
   const int sub = lane & 15, kq = lane >> 4;
-  PP_FETCH(0);
-  PP_STASH(0);
-  __syncthreads();
-  PP_FETCH(1);
   // step s: compute from LDS buffer s & 1 while step s + 1 goes from
   // registers into the other buffer and step s + 2 is loaded (the last
   // steps re-load the chunk's last step and fill a buffer no one reads)
-  for (int s = 0; s < nsteps; s++) {
-    const int buf = s & 1;
-#pragma unroll
-    for (int kk = 0; kk < KS; kk += 32) {
-      const int kc = kk + kq * 8;
-      u32x4 a[NT][4];
-#pragma unroll
-      for (int t = 0; t < NT; t++)
-#pragma unroll
-        for (int mi = 0; mi < 4; mi++)
-          a[t][mi] = *reinterpret_cast<const u32x4 *>(&As[buf][t][wm * 64 + mi * 16 + sub][kc]);
-#pragma unroll
-      for (int nj = 0; nj < 4; nj++) {
-        const int col = wn * 64 + nj * 16 + sub;
-        u32x4 b[NT], q[NT];
-#pragma unroll
-        for (int t = 0; t < NT; t++) {
-          b[t] = *reinterpret_cast<const u32x4 *>(&Bs[buf][t][col][kc]);
-          q[t] = qswap(b[t]);
-        }
-#pragma unroll
-        for (int mi = 0; mi < 4; mi++) {
-#pragma unroll
-          for (int d = NT - 1; d >= 0; d--)
-#pragma unroll
-            for (int i = d; i >= 0; i--) {
-              cr[mi][nj] = mfma16<F16>(a[i][mi], b[d - i], cr[mi][nj]);
-              ci[mi][nj] = mfma16<F16>(a[i][mi], q[d - i], ci[mi][nj]);
-            }
-        }
-      }
-    }
-    PP_STASH(buf ^ 1);
-    PP_FETCH(min(s + 2, nsteps - 1));
-    __syncthreads();
+#define PP_KLOOP(WH)                                                                                             \
+  {                                                                                                              \
+    PP_FETCH(0, WH);                                                                                             \
+    PP_STASH(0);                                                                                                 \
+    __syncthreads();                                                                                             \
+    PP_FETCH(1, WH);                                                                                             \
+    for (int s = 0; s < nsteps; s++) {                                                                           \
+      const int buf = s & 1;                                                                                     \
+      _Pragma("unroll") for (int kk = 0; kk < KS; kk += 32) {                                                    \
+        const int kc = kk + kq * 8;                                                                              \
+        u32x4 a[NT][4];                                                                                          \
+        _Pragma("unroll") for (int t = 0; t < NT; t++) _Pragma("unroll") for (int mi = 0; mi < 4; mi++) a[t][mi] = \
+            *reinterpret_cast<const u32x4 *>(&As[buf][t][wm * 64 + mi * 16 + sub][kc]);                        \
+        _Pragma("unroll") for (int nj = 0; nj < 4; nj++) {                                                       \
+          const int col = wn * 64 + nj * 16 + sub;                                                               \
+          u32x4 b[NT], q[NT];                                                                                    \
+          _Pragma("unroll") for (int t = 0; t < NT; t++) {                                                       \
+            b[t] = *reinterpret_cast<const u32x4 *>(&Bs[buf][t][col][kc]);                                       \
+            q[t] = qswap(b[t]);                                                                                  \
+          }                                                                                                      \
+          _Pragma("unroll") for (int mi = 0; mi < 4; mi++) {                                                     \
+            _Pragma("unroll") for (int d = NT - 1; d >= 0; d--) _Pragma("unroll") for (int i = d; i >= 0; i--) { \
+              cr[mi][nj] = mfma16<F16>(a[i][mi], b[d - i], cr[mi][nj]);                                          \
+              ci[mi][nj] = mfma16<F16>(a[i][mi], q[d - i], ci[mi][nj]);                                          \
+            }                                                                                                    \
+          }                                                                                                      \
+        }                                                                                                        \
+      }                                                                                                          \
+      PP_STASH(buf ^ 1);                                                                                         \
+      PP_FETCH(min(s + 2, nsteps - 1), WH);                                                                      \
+      __syncthreads();                                                                                           \
+    }                                                                                                            \
   }
+  if (whole)
+    PP_KLOOP(true)
+  else
+    PP_KLOOP(false)
+#undef PP_KLOOP
 #undef PP_FETCH
 #undef PP_STASH
   C += (long)split * sSplit;
