@@ -528,6 +528,8 @@ int pp_ctx_set_param(pp_ctx *ctx, const char *name, int value) {
     ctx->tune_fft_xmask = value != 0 ? 1 : 0;
   else if (n == "fft_xcd")
     ctx->tune_fft_xcd = value != 0 ? 1 : 0;
+  else if (n == "fft_fixed")
+    ctx->tune_fft_fixed = value != 0 ? 1 : 0;
   else if (n == "sphere_nt")
     ctx->tune_sphere_nt = value != 0 ? 1 : 0;
   else if (n == "fft_nt")
@@ -593,7 +595,7 @@ int pp_ctx_get_param(pp_ctx *ctx, const char *name, int *value) {
       {"fft_pingpong", ctx->tune_fft_pingpong}, {"gemm_f64_valu", ctx->tune_gemm_f64_valu},
       {"fft_persistent", ctx->tune_fft_persistent}, {"streams", ctx->tune_streams},
       {"sphere_unroll", ctx->tune_sphere_unroll}, {"fft_threads", ctx->tune_fft_threads},
-      {"fft_radix", ctx->tune_fft_radix}, {"fft_xcd", ctx->tune_fft_xcd},
+      {"fft_radix", ctx->tune_fft_radix}, {"fft_xcd", ctx->tune_fft_xcd}, {"fft_fixed", ctx->tune_fft_fixed},
       {"fft_xmask", ctx->tune_fft_xmask}, {"fft_nt", ctx->tune_fft_nt},
       {"sphere_nt", ctx->tune_sphere_nt},
       {"fused", ctx->tune_fused}, {"rs_kernel", ctx->tune_rs_kernel}, {"rs_prune", ctx->tune_rs_prune},

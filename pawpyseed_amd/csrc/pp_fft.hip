@@ -341,13 +341,16 @@ __global__ __launch_bounds__(256) void k_fft_pass(cplx<T> *__restrict__ g, AxisP
 // also read with non-temporal loads (each pass touches its data once).
 // The work of one work-group of the in-place pass: tile blk (line
 // blk / nslices, band slice blk % nslices) of the chunk g.
-template <typename T, int NBW, int AXIS, int MAXB, int NT, bool COMP>
+// N > 0: the axis length and its stages are compile-time constants
+// (stages_fixed); the host launches that form only when its plan has the
+// same stages.
+template <typename T, int NBW, int AXIS, int MAXB, int NT, bool COMP, int N = 0>
 __device__ __forceinline__ void fft_pass_ip_tile(cplx<T> *__restrict__ g, const AxisPlan &pl,
                                                  const cplx<T> *__restrict__ tw, long es, long s1, long s2, int n2,
                                                  int nslices, const SphereBounds &bd, int blk, int streaming) {
   extern __shared__ unsigned char smem_raw[];
   cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
-  const int n = pl.n;
+  const int n = N > 0 ? N : pl.n;
   const int slice = blk % nslices;
   const int line = pass_line<AXIS>(bd, blk / nslices);
   const int l1 = line / n2, l2 = line % n2;
@@ -420,7 +423,10 @@ __device__ __forceinline__ void fft_pass_ip_tile(cplx<T> *__restrict__ g, const 
         buf[t0 + u * NT] = v[u];
   }
   __syncthreads();
-  stages_ip<T, NBW, MAXB, NT, COMP>(buf, twl, pl, n);
+  if constexpr (N > 0)
+    stages_fixed<T, NBW, N, NT>(buf, twl);
+  else
+    stages_ip<T, NBW, MAXB, NT, COMP>(buf, twl, pl, n);
   const unsigned *xm = (AXIS == 0 && bd.xm) ? bd.xm + (long)line * bd.xmw : nullptr;
   if constexpr (sizeof(T) == 4 && NBW % 2 == 0) {
     const int tot2 = tot / 2;
@@ -478,13 +484,26 @@ __global__ __launch_bounds__(NT, (FftWaves<T, MAXB, COMP>::value)) void k_fft_pa
   fft_pass_ip_tile<T, NBW, AXIS, MAXB, NT, COMP>(g, pl, tw, es, s1, s2, n2, nslices, bd, blk, streaming);
 }
 
+// The in-place pass for an axis of compile-time length N (see
+// fft_pass_ip_tile); FP32 kernels at 8 wavefronts per SIMD as
+// k_fft_pass_ip's.
+template <typename T, int NBW, int AXIS, int N, int NT>
+__global__ __launch_bounds__(NT, (PassWaves<T, 4, N>::value)) void k_fft_pass_fx(
+    cplx<T> *__restrict__ g, AxisPlan pl, const cplx<T> *__restrict__ tw, long es, long s1, long s2, int n2,
+    int nslices, SphereBounds bd, int xcd_per, int nblk, int streaming) {
+  const int blk = xcd_per > 0 ? (int)(blockIdx.x % 8) * xcd_per + (int)(blockIdx.x / 8) : (int)blockIdx.x;
+  if (blk >= nblk)
+    return;
+  fft_pass_ip_tile<T, NBW, AXIS, 4, NT, false, N>(g, pl, tw, es, s1, s2, n2, nslices, bd, blk, streaming);
+}
+
 // The same pass over the chunks of a batch of work items (one per
 // blockIdx.y): each item brings its own chunk buffer and the plane-wave
 // bounds and line lists of its k-point; tiles past the item's lines exit.
 // (7 rather than 8 waves per SIMD where the single-chunk pass takes 8: the
 // descriptor's pointers and counts cost a few registers, and 64 would spill)
-template <typename T, int NBW, int AXIS, int MAXB, int NT>
-__global__ __launch_bounds__(NT, (FftWaves<T, MAXB, false>::value == 8 ? 7 : FftWaves<T, MAXB, false>::value))
+template <typename T, int NBW, int AXIS, int MAXB, int NT, int N = 0>
+__global__ __launch_bounds__(NT, (PassWaves<T, MAXB, N>::value == 8 ? 7 : PassWaves<T, MAXB, N>::value))
 void k_fft_pass_ip_items(
     const ChunkItem *__restrict__ items, AxisPlan pl, const cplx<T> *__restrict__ tw, long es, long s1, long s2,
     int n2, int nslices, int streaming) {
@@ -507,8 +526,8 @@ void k_fft_pass_ip_items(
   const int blk = (int)blockIdx.x;
   if (blk >= nlin * nslices)
     return;
-  fft_pass_ip_tile<T, NBW, AXIS, MAXB, NT, false>((cplx<T> *)it->grid, pl, tw, es, s1, s2, n2, nslices, bd, blk,
-                                                  streaming);
+  fft_pass_ip_tile<T, NBW, AXIS, MAXB, NT, false, N>((cplx<T> *)it->grid, pl, tw, es, s1, s2, n2, nslices, bd, blk,
+                                                     streaming);
 }
 
 // Line bounds of a pass (see k_fft_pass); false when the line is skipped.
@@ -713,6 +732,29 @@ int compute_gidx(pp_ctx *ctx, const int *fftg, int nw, const int *G_dev, int *gi
   return PP_OK;
 }
 
+// Axis lengths with a compile-time pass (k_fft_pass_fx): the grids of the
+// 64-atom (60), 216-atom (108), 512-atom (192) cells and the doubled
+// 216-atom grid (216).
+#define PP_FIXED_LENGTHS(X) X(60) X(108) X(192) X(216)
+static bool fixed_length(int n) {
+#define PP_FX_IS(NN) n == NN ||
+  return PP_FIXED_LENGTHS(PP_FX_IS) false;
+#undef PP_FX_IS
+}
+// The compile-time pass serves an axis when its length has one and the
+// host plan's stages and twiddle offsets are the ones it was compiled with.
+static bool fixed_plan_matches(const AxisPlan &ap) {
+  if (!fixed_length(ap.n))
+    return false;
+  const FftFactors f = fft_factors(ap.n);
+  if (f.nst != ap.nst)
+    return false;
+  for (int s = 0; s < f.nst; s++)
+    if (f.r[s] != ap.radix[s] || f.p[s] != ap.pprev[s] || f.twoff[s] != ap.twoff[s])
+      return false;
+  return true;
+}
+
 template <typename T, int NBW, int AXIS>
 static int launch_pass(pp_ctx *ctx, cplx<T> *grid, const AxisPlan &ap, const cplx<T> *tw, long es, long s1,
                        long s2, int n1, int n2, int NB, int cls, const SphereBounds &bd) {
@@ -785,6 +827,20 @@ static int launch_pass(pp_ctx *ctx, cplx<T> *grid, const AxisPlan &ap, const cpl
         PP_PL(8, 16);
 #undef PP_PL
       launched = true;
+    }
+    // the compile-time pass for the default tile width (16 FP32, 8 FP64 bands)
+    if constexpr (NBW == (sizeof(T) == 4 ? 16 : 8)) {
+      if (!launched && ip && nt == 256 && ctx->tune_fft_fixed && fixed_plan_matches(ap)) {
+        size_t lds = ((size_t)ap.n * NBW + ap.twlen) * sizeof(cplx<T>);
+#define PP_FX(NN)                                                                                               \
+  case NN:                                                                                                      \
+    hipLaunchKernelGGL((k_fft_pass_fx<T, NBW, AXIS, NN, 256>), dim3((unsigned)nlaunch), dim3(256), lds,         \
+                       ctx->stream, grid, ap, tw, es, s1, s2, n2, nslices, bd, xcd_per, (int)nblk, nt_mode);    \
+    launched = true;                                                                                            \
+    break;
+        switch (ap.n) { PP_FIXED_LENGTHS(PP_FX) default: break; }
+#undef PP_FX
+      }
     }
     if (!launched && ip) {
       size_t lds = ((size_t)ap.n * NBW + ap.twlen) * sizeof(cplx<T>);
@@ -886,12 +942,24 @@ static int launch_pass_items(pp_ctx *ctx, const ChunkItem *items, int nitems, lo
     dim3 grd((unsigned)nblk, (unsigned)nitems);
     hipEvent_t ev;
     tbegin(ctx, cls, &ev);
+    bool done = false;
+    if constexpr (NBW == (sizeof(T) == 4 ? 16 : 8)) {
+      if (nt == 256 && ctx->tune_fft_fixed && fixed_plan_matches(ap)) {
+#define PP_FXI(NN)                                                                                              \
+  case NN:                                                                                                      \
+    hipLaunchKernelGGL((k_fft_pass_ip_items<T, NBW, AXIS, 4, 256, NN>), grd, dim3(256), lds, ctx->stream, items, ap, \
+                       tw, es, s1, s2, n2, nslices, nt_mode);                                                   \
+    done = true;                                                                                                \
+    break;
+        switch (ap.n) { PP_FIXED_LENGTHS(PP_FXI) default: break; }
+#undef PP_FXI
+      }
+    }
 #define PP_IPI(MB, NT)                                                                                           \
   hipLaunchKernelGGL((k_fft_pass_ip_items<T, NBW, AXIS, MB, NT>), grd, dim3(NT), lds, ctx->stream, items, ap, tw, es, \
                      s1, s2, n2, nslices, nt_mode)
-    bool done = false;
     if constexpr (NBW >= 32) {
-      if (nt == 512) {
+      if (!done && nt == 512) {
         if (maxb <= 2)
           PP_IPI(2, 512);
         else if (maxb <= 4)

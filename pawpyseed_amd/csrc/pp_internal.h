@@ -137,6 +137,7 @@ struct pp_ctx {
   int tune_fft_nt = 2;          // FFT passes: 1 non-temporal stores, 2 non-temporal loads and stores
   int tune_fft_xmask = 0;        // 1: the x pass writes only points a projector sphere holds
   int tune_fft_xcd = 0;         // 1: FFT pass work-groups in XCD-contiguous line ranges
+  int tune_fft_fixed = 1;       // 1: compile-time passes for the axis lengths that have one
   int tune_fused = 0;           // 1: x pass fused with the sphere sums where the plan allows
   int tune_rs_kernel = 2;       // real-space augmentation: 0 LDS coefficients, 1 registers, 2 tiled
   int tune_rs_unroll = 4;       // real-space augmentation: points of a wavefront in flight (1, 2, 4)

@@ -53,21 +53,91 @@ __device__ __forceinline__ void stage_ip(cplx<T> *__restrict__ buf, const cplx<T
   __syncthreads();
 }
 
+// The stage list of an axis of length n as the host plan factors it
+// (radix 4 while it divides, then the primes in increasing order), with
+// each stage's p (product of the earlier radices) and its offset in the
+// axis twiddle table, which holds p (R - 1) twiddles then R roots per
+// stage.
+struct FftFactors {
+  int nst, r[12], p[12], twoff[12];
+};
+constexpr FftFactors fft_factors(int n) {
+  FftFactors f{};
+  int m = n;
+  while (m % 4 == 0 && f.nst < 12) {
+    f.r[f.nst++] = 4;
+    m /= 4;
+  }
+  for (int q = 2; m > 1 && q <= m && f.nst < 12; q++)
+    while (m % q == 0 && f.nst < 12) {
+      f.r[f.nst++] = q;
+      m /= q;
+    }
+  int p = 1, off = 0;
+  for (int s = 0; s < f.nst; s++) {
+    f.p[s] = p;
+    f.twoff[s] = off;
+    off += p * (f.r[s] - 1) + f.r[s];
+    p *= f.r[s];
+  }
+  return f;
+}
+
+// stage_ip for a stage known at compile time (radix R, p = P, TN = n / R
+// butterflies per band): every LDS and twiddle offset is a constant off one
+// base per butterfly, and the first stage (p = 1, all twiddles 1) skips
+// its twiddle products.
+template <typename T, int NBW, int R, int P, int TN, int NT>
+__device__ __forceinline__ void stage_fixed(cplx<T> *__restrict__ buf, const cplx<T> *__restrict__ tws) {
+  constexpr unsigned JS = NT / NBW, MB = (TN + JS - 1) / JS;
+  const unsigned b = threadIdx.x % NBW, j0 = threadIdx.x / NBW;
+  cplx<T> v[MB][R];
+  unsigned o[MB];
+#pragma unroll
+  for (unsigned q = 0; q < MB; q++) {
+    const unsigned j = j0 + q * JS;
+    if (TN % JS == 0 || q + 1 < MB || j < (unsigned)TN) {
+      const unsigned k = j % P;
+      o[q] = (j / P) * P * R + k;
+      v[q][0] = buf[j * NBW + b];
+#pragma unroll
+      for (int r = 1; r < R; r++) {
+        v[q][r] = buf[(j + r * TN) * NBW + b];
+        if constexpr (P > 1)
+          v[q][r] = v[q][r] * tws[k * (R - 1) + (r - 1)];
+      }
+      Dft<T, R>::run(v[q]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (unsigned q = 0; q < MB; q++) {
+    const unsigned j = j0 + q * JS;
+    if (TN % JS == 0 || q + 1 < MB || j < (unsigned)TN) {
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        buf[(o[q] + r * P) * NBW + b] = v[q][r];
+    }
+  }
+  __syncthreads();
+}
+
+// All stages of a line of compile-time length N (stage S onwards).
+template <typename T, int NBW, int N, int NT, int S = 0>
+__device__ __forceinline__ void stages_fixed(cplx<T> *__restrict__ buf, const cplx<T> *__restrict__ twl) {
+  constexpr FftFactors f = fft_factors(N);
+  if constexpr (S < f.nst) {
+    stage_fixed<T, NBW, f.r[S], f.p[S], N / f.r[S], NT>(buf, twl + f.twoff[S]);
+    stages_fixed<T, NBW, N, NT, S + 1>(buf, twl);
+  }
+}
+
 // Copy the twiddle/root table of one axis into LDS (visible after the
 // caller's next barrier).
 template <typename T, int NT = 256>
 __device__ __forceinline__ void stage_twiddles(cplx<T> *__restrict__ dst, const cplx<T> *__restrict__ tw, int len) {
-  for (int i0 = threadIdx.x; i0 < len; i0 += 4 * NT) {
-    cplx<T> v[4];
-#pragma unroll
-    for (int u = 0; u < 4; u++)
-      if (i0 + u * NT < len)
-        v[u] = tw[i0 + u * NT];
-#pragma unroll
-    for (int u = 0; u < 4; u++)
-      if (i0 + u * NT < len)
-        dst[i0 + u * NT] = v[u];
-  }
+  for (int i = threadIdx.x; i < len; i += NT)
+    dst[i] = tw[i];
 }
 
 // loads per thread issued together when a pass stages its line in LDS
@@ -80,6 +150,12 @@ constexpr int FFT_LB = 8;
 // which would spill.
 template <typename T, int MAXB, bool COMP> struct FftWaves {
   static constexpr int value = (!COMP && sizeof(T) == 4 && MAXB <= 4) ? 8 : 1;
+};
+// The same for a pass of compile-time length N (0: the generic pass): FP32
+// lines up to 192 at 8 wavefronts per SIMD, longer ones at 4 (up to seven
+// butterflies per thread in a radix-2 stage of 216).
+template <typename T, int MAXB, int N> struct PassWaves {
+  static constexpr int value = N == 0 ? FftWaves<T, MAXB, false>::value : sizeof(T) == 4 ? (N <= 192 ? 8 : 4) : 1;
 };
 
 // All stages of a line in LDS (radices up to RMAX; the caller guarantees

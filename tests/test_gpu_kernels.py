@@ -521,6 +521,38 @@ def test_xmask_writes_only_covered_points(ctx, precision):
     plan.close()
 
 
+@pytest.mark.parametrize("fftg", [(108, 60, 216), (192, 108, 60)])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-9), ("fp32", 1e-5)])
+def test_fixed_length_passes(ctx, fftg, precision, tol):
+    """The FFT passes compiled for the axis lengths 60, 108, 192 and 216
+    (fft_fixed = 1, the default) give the projections of the generic
+    passes (fft_fixed = 0) exactly (up to the sign of zeros, which == does
+    not see), and the oracle's."""
+    from pawpyseed_amd.core import batched as B
+    wl = synthetic.tiny_config(encut=150.0, fftg=fftg, nband=6, kpts=[[0.25, -0.25, 0.1]], ls=(0, 1, 2),
+                               alphas=(1.5, 3.0, 1.5))
+    Gs = wl.ensure_gvecs()
+    Cs = wl.coefficients("S", nband=6)
+    labels = np.zeros(len(wl.frac_S), np.int32)
+    T = orc.site_tables(wl.lattice, wl.fftg, wl.frac_S, [wl.species], labels)
+    want = orc.setup_projections(wl.lattice, wl.fftg, wl.kpts, Gs, Cs, T)
+    Cd, Gd, gofs, nw = device_inputs(Gs, Cs)
+    plan = B.SpherePlan(ctx, wl.lattice, wl.fftg, wl.frac_S, labels, [wl.species], precision)
+    outs = []
+    for fx in (0, 1):
+        ctx.set_param("fft_fixed", fx)
+        out = torch.zeros((1, 6, plan.ptot), dtype=torch.complex128, device="cuda")
+        B.setup_projections(ctx, plan, wl.kpts, gofs, Gd, Cd, out, precision)
+        torch.cuda.synchronize()
+        outs.append(out.cpu().numpy())
+    ctx.set_param("fft_fixed", 1)
+    err = normwise(outs[1], want)
+    print("fixed-length passes", fftg, precision, err, flush=True)
+    assert np.array_equal(outs[0], outs[1])
+    assert err < tol
+    plan.close()
+
+
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-9), ("fp32", 1e-5)])
 def test_projections_g_set_with_holes(ctx, precision, tol):
     """A plane-wave list that is not a cutoff sphere (a random subset of
