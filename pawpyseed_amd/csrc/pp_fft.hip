@@ -380,8 +380,39 @@ __device__ __forceinline__ void fft_pass_ip_tile(cplx<T> *__restrict__ g, const 
   // the line's loads are issued in batches of FFT_LB per thread before any
   // is written to LDS, so that many are in flight at once (a load-then-store
   // loop waits for each load in turn); FP32 lines move two bands (16 bytes)
-  // per access
-  if constexpr (sizeof(T) == 4 && NBW % 2 == 0) {
+  // per access.  Compile-time lengths: a thread's elements are EST apart on
+  // the axis, so their addresses step by a uniform stride from one base.
+  constexpr int PK = (sizeof(T) == 4 && NBW % 2 == 0) ? 2 : 1;  // complex values per access
+  constexpr int EST = PK * NT / NBW;
+  constexpr bool FXLS = N > 0 && (PK * NT) % NBW == 0;
+  if constexpr (FXLS) {
+    constexpr int TOTA = N * NBW / PK, EPT = (TOTA + NT - 1) / NT;
+    const int t = threadIdx.x, e0 = PK * t / NBW, bb = PK * t % NBW;
+    const cplx<T> *p = g + base + (long)e0 * es + bb;
+    const long step = (long)EST * es;
+    cplx<T> va[EPT][PK];
+#pragma unroll
+    for (int u = 0; u < EPT; u++) {
+#pragma unroll
+      for (int c = 0; c < PK; c++)
+        va[u][c] = mk<T>(0, 0);
+      const int e = e0 + u * EST;
+      const int se = e < N - N / 2 ? e : e - N;
+      if ((TOTA % NT == 0 || t + u * NT < TOTA) && se >= lo && se <= hi) {
+        if constexpr (PK == 2)
+          load2(reinterpret_cast<const cplx<float> *>(p + u * step), streaming > 1, va[u][0], va[u][1]);
+        else
+          va[u][0] = streaming > 1 ? load_nt(p + u * step) : p[u * step];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < EPT; u++)
+      if (TOTA % NT == 0 || t + u * NT < TOTA) {
+#pragma unroll
+        for (int c = 0; c < PK; c++)
+          buf[PK * (t + u * NT) + c] = va[u][c];
+      }
+  } else if constexpr (sizeof(T) == 4 && NBW % 2 == 0) {
     const int tot2 = tot / 2;
     for (int t0 = threadIdx.x; t0 < tot2; t0 += NT * FFT_LB) {
       cplx<T> va[FFT_LB], vb[FFT_LB];
@@ -428,6 +459,35 @@ __device__ __forceinline__ void fft_pass_ip_tile(cplx<T> *__restrict__ g, const 
   else
     stages_ip<T, NBW, MAXB, NT, COMP>(buf, twl, pl, n);
   const unsigned *xm = (AXIS == 0 && bd.xm) ? bd.xm + (long)line * bd.xmw : nullptr;
+  if constexpr (FXLS) {
+    constexpr int TOTA = N * NBW / PK, EPT = (TOTA + NT - 1) / NT;
+    const int t = threadIdx.x, e0 = PK * t / NBW, bb = PK * t % NBW;
+    cplx<T> *p = g + base + (long)e0 * es + bb;
+    const long step = (long)EST * es;
+    cplx<T> va[EPT][PK];
+#pragma unroll
+    for (int u = 0; u < EPT; u++)
+      if (TOTA % NT == 0 || t + u * NT < TOTA) {
+#pragma unroll
+        for (int c = 0; c < PK; c++)
+          va[u][c] = buf[PK * (t + u * NT) + c];
+      }
+#pragma unroll
+    for (int u = 0; u < EPT; u++) {
+      const int e = e0 + u * EST;
+      if (!(TOTA % NT == 0 || t + u * NT < TOTA))
+        continue;
+      if (xm && !((xm[e >> 5] >> (e & 31)) & 1u))
+        continue;  // no projector sphere reads this point
+      if constexpr (PK == 2)
+        store2(reinterpret_cast<cplx<float> *>(p + u * step), streaming != 0, va[u][0], va[u][1]);
+      else if (streaming)
+        store_nt(p + u * step, va[u][0]);
+      else
+        p[u * step] = va[u][0];
+    }
+    return;
+  }
   if constexpr (sizeof(T) == 4 && NBW % 2 == 0) {
     const int tot2 = tot / 2;
     for (int t0 = threadIdx.x; t0 < tot2; t0 += NT * FFT_LB) {
