@@ -258,7 +258,7 @@ int pp_ctx_set_tuning(pp_ctx *ctx, int fft_bands, int fft_tile, int fft_prune,
  * XCD-contiguous line ranges), "fft_fixed" (1, default: FFT passes along
  * axes of length 60, 108, 192 or 216 run kernels compiled for that length,
  * with every index and offset a constant; 0: the generic in-place passes;
- * results equal up to the sign of zeros), "fft_nt" (FFT passes: 0 cached, 1
+ * the two agree to rounding), "fft_nt" (FFT passes: 0 cached, 1
  * non-temporal stores, 2 non-temporal loads and stores, default 2; 3:
  * cached z pass, streamed stores in the y pass, streamed x pass),
  * "sphere_nt" (1, default: sphere-sum gathers with non-temporal loads),

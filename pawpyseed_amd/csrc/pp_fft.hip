@@ -548,7 +548,7 @@ __global__ __launch_bounds__(NT, (FftWaves<T, MAXB, COMP>::value)) void k_fft_pa
 // fft_pass_ip_tile); FP32 kernels at 8 wavefronts per SIMD as
 // k_fft_pass_ip's.
 template <typename T, int NBW, int AXIS, int N, int NT>
-__global__ __launch_bounds__(NT, (PassWaves<T, 4, N>::value)) void k_fft_pass_fx(
+__global__ __launch_bounds__(NT, (PassWaves<T, 4, N, NBW>::value)) void k_fft_pass_fx(
     cplx<T> *__restrict__ g, AxisPlan pl, const cplx<T> *__restrict__ tw, long es, long s1, long s2, int n2,
     int nslices, SphereBounds bd, int xcd_per, int nblk, int streaming) {
   const int blk = xcd_per > 0 ? (int)(blockIdx.x % 8) * xcd_per + (int)(blockIdx.x / 8) : (int)blockIdx.x;
@@ -563,7 +563,7 @@ __global__ __launch_bounds__(NT, (PassWaves<T, 4, N>::value)) void k_fft_pass_fx
 // (7 rather than 8 waves per SIMD where the single-chunk pass takes 8: the
 // descriptor's pointers and counts cost a few registers, and 64 would spill)
 template <typename T, int NBW, int AXIS, int MAXB, int NT, int N = 0>
-__global__ __launch_bounds__(NT, (PassWaves<T, MAXB, N>::value == 8 ? 7 : PassWaves<T, MAXB, N>::value))
+__global__ __launch_bounds__(NT, (PassWaves<T, MAXB, N, NBW>::value == 8 ? 7 : PassWaves<T, MAXB, N, NBW>::value))
 void k_fft_pass_ip_items(
     const ChunkItem *__restrict__ items, AxisPlan pl, const cplx<T> *__restrict__ tw, long es, long s1, long s2,
     int n2, int nslices, int streaming) {
@@ -803,6 +803,8 @@ static bool fixed_length(int n) {
 }
 // The compile-time pass serves an axis when its length has one and the
 // host plan's stages and twiddle offsets are the ones it was compiled with.
+// Its results agree with the generic pass's to rounding (the compiler is
+// free to contract the complex products differently in the two).
 static bool fixed_plan_matches(const AxisPlan &ap) {
   if (!fixed_length(ap.n))
     return false;
@@ -889,7 +891,8 @@ static int launch_pass(pp_ctx *ctx, cplx<T> *grid, const AxisPlan &ap, const cpl
       launched = true;
     }
     // the compile-time pass for the default tile width (16 FP32, 8 FP64 bands)
-    if constexpr (NBW == (sizeof(T) == 4 ? 16 : 8)) {
+    // and twice it
+    if constexpr (NBW == (sizeof(T) == 4 ? 16 : 8) || NBW == (sizeof(T) == 4 ? 32 : 16)) {
       if (!launched && ip && nt == 256 && ctx->tune_fft_fixed && fixed_plan_matches(ap)) {
         size_t lds = ((size_t)ap.n * NBW + ap.twlen) * sizeof(cplx<T>);
 #define PP_FX(NN)                                                                                               \
@@ -1003,7 +1006,7 @@ static int launch_pass_items(pp_ctx *ctx, const ChunkItem *items, int nitems, lo
     hipEvent_t ev;
     tbegin(ctx, cls, &ev);
     bool done = false;
-    if constexpr (NBW == (sizeof(T) == 4 ? 16 : 8)) {
+    if constexpr (NBW == (sizeof(T) == 4 ? 16 : 8) || NBW == (sizeof(T) == 4 ? 32 : 16)) {
       if (nt == 256 && ctx->tune_fft_fixed && fixed_plan_matches(ap)) {
 #define PP_FXI(NN)                                                                                              \
   case NN:                                                                                                      \

@@ -152,10 +152,12 @@ template <typename T, int MAXB, bool COMP> struct FftWaves {
   static constexpr int value = (!COMP && sizeof(T) == 4 && MAXB <= 4) ? 8 : 1;
 };
 // The same for a pass of compile-time length N (0: the generic pass): FP32
-// lines up to 192 at 8 wavefronts per SIMD, longer ones at 4 (up to seven
-// butterflies per thread in a radix-2 stage of 216).
-template <typename T, int MAXB, int N> struct PassWaves {
-  static constexpr int value = N == 0 ? FftWaves<T, MAXB, false>::value : sizeof(T) == 4 ? (N <= 192 ? 8 : 4) : 1;
+// 16-band tiles of lines up to 192 at 8 wavefronts per SIMD; longer lines
+// (up to seven butterflies per thread in a radix-2 stage of 216) and wider
+// tiles at 4.
+template <typename T, int MAXB, int N, int NBW> struct PassWaves {
+  static constexpr int value = N == 0 ? FftWaves<T, MAXB, false>::value
+                               : sizeof(T) == 4 ? (N <= 192 && NBW <= 16 ? 8 : 4) : 1;
 };
 
 // All stages of a line in LDS (radices up to RMAX; the caller guarantees
