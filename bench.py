@@ -945,9 +945,13 @@ def bench_cfg3(args, torch, B):
         # algorithmic bytes of the setup's kernel classes over all items of a step
         pb = 8 if p != "fp64" else 16
         alg = {"fft_x": 0.0, "fft_y": 0.0, "fft_z": 0.0, "sphere_sum": 0.0, "scatter": 0.0}
+        # with fft_xmask the x pass writes only the points a structure's
+        # projector spheres hold
+        xmask_on = bool((tuned or {}).get("fft_xmask", ctx.get_param("fft_xmask")))
+        covered = {s: plans[s].coverage() for s in range(1, 4)} if xmask_on else {}
         for k in range(nk):
-            bx, by, bz = pass_bytes(Gs[k], wl.fftg, 1, pb)
             for s in range(1, 4):
+                bx, by, bz = pass_bytes(Gs[k], wl.fftg, 1, pb, covered.get(s))
                 nb = int(wl.nbands[s][k])
                 alg["fft_x"] += bx * nb
                 alg["fft_y"] += by * nb
