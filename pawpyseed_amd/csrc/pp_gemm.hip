@@ -144,6 +144,9 @@ __global__ __launch_bounds__(256) void k_zgemm_mfma64(int M, int N, int kchunk, 
     if (kb + 16 < k1)
       fetch(kb + 16);
     const int sub = lane & 15, kq = lane >> 4;
+    // B'' = (b_im, -b_re): the neighbour k ^ 1 with its sign bit flipped for
+    // odd k -- one LDS read and no divergent branch
+    const unsigned long long sgn = (unsigned long long)(kq & 1) << 63;
 #pragma unroll
     for (int kk = 0; kk < 32; kk += 4) {
       const int kr = kk + kq;
@@ -153,7 +156,7 @@ __global__ __launch_bounds__(256) void k_zgemm_mfma64(int M, int N, int kchunk, 
       for (int nj = 0; nj < 2; nj++) {
         const int col = wn * 32 + nj * 16 + sub;
         const double b = Bs[col][kr];
-        const double b2 = (kq & 1) ? -Bs[col][kr - 1] : Bs[col][kr + 1];
+        const double b2 = __builtin_bit_cast(double, __builtin_bit_cast(unsigned long long, Bs[col][kr ^ 1]) ^ sgn);
         cr[0][nj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b, cr[0][nj], 0, 0, 0);
         ci[0][nj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b2, ci[0][nj], 0, 0, 0);
         cr[1][nj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b, cr[1][nj], 0, 0, 0);

@@ -134,6 +134,10 @@ __global__ __launch_bounds__(256) void k_cgemm_f32(int M, int N, int kchunk, int
       load8<VEC>(B, ldb, tn * 64 + lrow, N, kb + 16 + part * 4, k1, vb);
     }
     const int sub = lane & 15, kq = lane >> 4;
+    // B'' = (b_im, -b_re): the neighbour k ^ 1 (k even for re, odd for im)
+    // with its sign bit flipped for odd k -- one LDS read and no branch
+    // (a conditional choice of the two reads compiles to a divergent branch)
+    const unsigned sgn = (unsigned)(kq & 1) << 31;
 #pragma unroll
     for (int kk = 0; kk < 32; kk += 4) {
       const int kr = kk + kq;
@@ -143,7 +147,7 @@ __global__ __launch_bounds__(256) void k_cgemm_f32(int M, int N, int kchunk, int
       for (int nj = 0; nj < 2; nj++) {
         const int col = wn * 32 + nj * 16 + sub;
         const float b = Bs[col][kr];
-        const float b2 = (kq & 1) ? -Bs[col][kr - 1] : Bs[col][kr + 1];
+        const float b2 = __builtin_bit_cast(float, __builtin_bit_cast(unsigned, Bs[col][kr ^ 1]) ^ sgn);
         cr[0][nj] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b, cr[0][nj], 0, 0, 0);
         ci[0][nj] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b2, ci[0][nj], 0, 0, 0);
         cr[1][nj] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b, cr[1][nj], 0, 0, 0);
