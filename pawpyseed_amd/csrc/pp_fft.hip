@@ -376,7 +376,6 @@ __device__ __forceinline__ void fft_pass_ip_tile(cplx<T> *__restrict__ g, const 
   const long base = (long)l1 * s1 + (long)l2 * s2 + (long)slice * NBW;
   const int tot = n * NBW;
   cplx<T> *twl = buf + tot;
-  stage_twiddles<T, NT>(twl, tw, pl.twlen);
   // the line's loads are issued in batches of FFT_LB per thread before any
   // is written to LDS, so that many are in flight at once (a load-then-store
   // loop waits for each load in turn); FP32 lines move two bands (16 bytes)
@@ -385,6 +384,23 @@ __device__ __forceinline__ void fft_pass_ip_tile(cplx<T> *__restrict__ g, const 
   constexpr int PK = (sizeof(T) == 4 && NBW % 2 == 0) ? 2 : 1;  // complex values per access
   constexpr int EST = PK * NT / NBW;
   constexpr bool FXLS = N > 0 && (PK * NT) % NBW == 0;
+  constexpr int FX_MW = N > 0 ? (N + 31) / 32 : 1;  // coverage words of an x line
+  unsigned mw[FX_MW];
+  // compile-time lengths: the first 2 NT twiddles (all of them for the
+  // compiled lengths) are read into registers ahead of the line's loads and
+  // written to LDS after them, so that the two wait on one memory latency
+  // rather than one after the other
+  cplx<T> twv[2];
+  if constexpr (FXLS) {
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+      const int i = threadIdx.x + r * NT;
+      twv[r] = i < pl.twlen ? tw[i] : mk<T>(0, 0);
+    }
+    for (int i = threadIdx.x + 2 * NT; i < pl.twlen; i += NT)
+      twl[i] = tw[i];
+  } else
+    stage_twiddles<T, NT>(twl, tw, pl.twlen);
   if constexpr (FXLS) {
     constexpr int TOTA = N * NBW / PK, EPT = (TOTA + NT - 1) / NT;
     const int t = threadIdx.x, e0 = PK * t / NBW, bb = PK * t % NBW;
@@ -406,12 +422,25 @@ __device__ __forceinline__ void fft_pass_ip_tile(cplx<T> *__restrict__ g, const 
       }
     }
 #pragma unroll
+    for (int r = 0; r < 2; r++)
+      if (threadIdx.x + r * NT < pl.twlen)
+        twl[threadIdx.x + r * NT] = twv[r];
+#pragma unroll
     for (int u = 0; u < EPT; u++)
       if (TOTA % NT == 0 || t + u * NT < TOTA) {
 #pragma unroll
         for (int c = 0; c < PK; c++)
           buf[PK * (t + u * NT) + c] = va[u][c];
       }
+    // the x line's coverage words, read here so that their loads complete
+    // during the stages and the masked stores below test registers (a load
+    // of the word beside each store waited for it store by store)
+    if constexpr (AXIS == 0) {
+      const unsigned *xm0 = bd.xm ? bd.xm + (long)line * bd.xmw : nullptr;
+#pragma unroll
+      for (int q = 0; q < FX_MW; q++)
+        mw[q] = (xm0 && q < bd.xmw) ? xm0[q] : ~0u;
+    }
   } else if constexpr (sizeof(T) == 4 && NBW % 2 == 0) {
     const int tot2 = tot / 2;
     for (int t0 = threadIdx.x; t0 < tot2; t0 += NT * FFT_LB) {
@@ -477,8 +506,14 @@ __device__ __forceinline__ void fft_pass_ip_tile(cplx<T> *__restrict__ g, const 
       const int e = e0 + u * EST;
       if (!(TOTA % NT == 0 || t + u * NT < TOTA))
         continue;
-      if (xm && !((xm[e >> 5] >> (e & 31)) & 1u))
-        continue;  // no projector sphere reads this point
+      if constexpr (AXIS == 0) {
+        unsigned w = mw[0];
+#pragma unroll
+        for (int q = 1; q < FX_MW; q++)
+          w = (e >> 5) == q ? mw[q] : w;
+        if (!((w >> (e & 31)) & 1u))
+          continue;  // no projector sphere reads this point
+      }
       if constexpr (PK == 2)
         store2(reinterpret_cast<cplx<float> *>(p + u * step), streaming != 0, va[u][0], va[u][1]);
       else if (streaming)
