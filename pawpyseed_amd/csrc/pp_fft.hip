@@ -162,24 +162,40 @@ __global__ __launch_bounds__(256) void k_gidx(int n0, int n1, int n2, int nw, co
 }
 
 // Scatter a 32-coefficient x NB-band tile through LDS: coalesced reads
-// along G of each band row, contiguous NB-band writes per grid point.
+// along G of each band row, contiguous NB-band writes per grid point.  A
+// thread's 8 coefficient loads, and then its 8 grid-index loads, are issued
+// together before any is used (a loop that uses each load before issuing
+// the next waits for them one after another).
 template <typename T>
 __device__ __forceinline__ void scatter_tile(cplx<T> *__restrict__ g, int NB, const int *__restrict__ gidx, int nw,
                                              const cplx<float> *__restrict__ C, long ldc, int nbc, int g0) {
   __shared__ cplx<T> tile[64][33];
-  for (int e = threadIdx.x; e < 64 * 32; e += 256) {
-    int b = e >> 5, gg = e & 31;
-    cplx<T> v = mk<T>(0, 0);
+  constexpr int EPT = 64 * 32 / 256;
+  cplx<T> v[EPT];
+#pragma unroll
+  for (int u = 0; u < EPT; u++) {
+    const int e = threadIdx.x + u * 256, b = e >> 5, gg = e & 31;
+    v[u] = mk<T>(0, 0);
     if (b < nbc && g0 + gg < nw)
-      v = cvt<T>(C[(long)b * ldc + g0 + gg]);
-    tile[b][gg] = v;
+      v[u] = cvt<T>(C[(long)b * ldc + g0 + gg]);
+  }
+#pragma unroll
+  for (int u = 0; u < EPT; u++) {
+    const int e = threadIdx.x + u * 256;
+    tile[e >> 5][e & 31] = v[u];
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < 64 * 32; e += 256) {
-    int gg = e >> 6, b = e & 63;
-    if (b < nbc && b < NB && g0 + gg < nw)
-      g[(long)gidx[g0 + gg] * NB + b] = tile[b][gg];
+  const int b = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int gi[EPT];
+#pragma unroll
+  for (int u = 0; u < EPT; u++) {
+    const int gg = w + 4 * u;
+    gi[u] = g0 + gg < nw ? gidx[g0 + gg] : -1;
   }
+#pragma unroll
+  for (int u = 0; u < EPT; u++)
+    if (b < nbc && b < NB && gi[u] >= 0)
+      g[(long)gi[u] * NB + b] = tile[b][w + 4 * u];
 }
 template <typename T>
 __global__ __launch_bounds__(256) void k_scatter(cplx<T> *__restrict__ g, int NB, const int *__restrict__ gidx,
