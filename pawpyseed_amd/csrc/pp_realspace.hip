@@ -355,8 +355,23 @@ __global__ __launch_bounds__(256) void k_rs_augment_t(cplx<T> *__restrict__ grid
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const bool act = lane < nbc;
   const cplx<double> *rows = tab + tab_off[s] + (long)begin * P;
-  for (int t = threadIdx.x; t < cnt * P; t += 256)
-    Ts[t] = rows[t];
+  // the table tile goes from global memory straight to LDS
+  // (global_load_lds_dwordx4: lane l of a wavefront writes its 16-byte
+  // entry at the wave-uniform LDS address + 16 l, masked-off lanes write
+  // nothing), so all of a work-group's tile loads are in flight at once and
+  // hold no registers; a copy through registers waited on each load in turn
+  // (the kernel's registers leave no room to batch them)
+  for (int i = wv; i * 64 < cnt * P; i += 4) {
+    const int t = i * 64 + lane;
+    if (t < cnt * P)
+      __builtin_amdgcn_global_load_lds((const void *)(rows + t),
+                                       (__attribute__((address_space(3))) void *)(Ts + i * 64), 16, 0, 0);
+  }
+  // the site's projections of this lane's band, loaded with the tile
+  cplx<double> cf[RP];
+#pragma unroll
+  for (int p = 0; p < RP; p++)
+    cf[p] = (p < P && act) ? proj[(long)lane * ldp + g.proj_off + p] : mk<double>(0.0, 0.0);
   for (int t = threadIdx.x; t < cnt; t += 256) {
     const int4 pt = pts[p0 + begin + t];
     Ix[t] = pt.x;
@@ -367,10 +382,9 @@ __global__ __launch_bounds__(256) void k_rs_augment_t(cplx<T> *__restrict__ grid
     sincospi(-2.0 * kt, &sn, &cs);
     Ph[t] = mk<double>(cs * iscl, sn * iscl);
   }
-  cplx<double> cf[RP];
-#pragma unroll
-  for (int p = 0; p < RP; p++)
-    cf[p] = (p < P && act) ? proj[(long)lane * ldp + g.proj_off + p] : mk<double>(0.0, 0.0);
+  // the direct-to-LDS loads count as outstanding vector loads: wait for all
+  // of them before the barrier that publishes the tile
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   for (int q0 = wv * U; q0 < cnt; q0 += 4 * U) {
     cplx<T> v[U];
