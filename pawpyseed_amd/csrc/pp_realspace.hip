@@ -472,7 +472,16 @@ __global__ __launch_bounds__(256) void k_rs_density(const cplx<T> *__restrict__ 
   const int lane = threadIdx.x & 63;
   const long w0 = ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * ppw;
   const double wl = lane < nbc ? W.w[lane] : 0.0;
-  constexpr int U = 4;  // points whose loads are in flight together
+  constexpr int U = 8;  // points whose loads are in flight together and whose sums share one reduction
+  // The U band sums of a batch are reduced over the 64 lanes together: the
+  // exchanges across lane bits 5, 4 and 3 each halve the values a lane holds
+  // (it keeps the points of its half and adds the partner's partial sums of
+  // them), those across bits 2, 1, 0 finish one sum; 10 exchanges for 8
+  // points instead of 6 per point.  Lane l then holds the sum of point
+  // q(l) = 4 b5 + 2 b4 + b3 of the batch, and lane 8 q(l) of each group of 8
+  // updates that point's density.
+  const int qmine = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
+  const bool h5 = (lane >> 5) & 1, h4 = (lane >> 4) & 1, h3 = (lane >> 3) & 1;
   for (int q0 = 0; q0 < ppw; q0 += U) {
     cplx<T> a[U];
 #pragma unroll
@@ -480,18 +489,33 @@ __global__ __launch_bounds__(256) void k_rs_density(const cplx<T> *__restrict__ 
       const long pt = w0 + q0 + u;
       a[u] = (q0 + u < ppw && pt < npts && lane < nbc) ? grid[pt * NB + lane] : mk<T>((T)0, (T)0);
     }
+    // the density value this lane updates, read with the batch
+    const long ptm = w0 + q0 + qmine;
+    const bool owner = (lane & 7) == 0 && q0 + qmine < ppw && ptm < npts;
+    const double dold = owner ? dens[ptm] : 0.0;
+    double v[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      const long pt = w0 + q0 + u;
-      if (q0 + u >= ppw || pt >= npts)
-        break;  // uniform over the wavefront
       const cplx<double> c = cvt<double>(a[u]);
-      double v = wl * (c.x * c.x + c.y * c.y);
-      for (int o = 32; o > 0; o >>= 1)
-        v += __shfl_xor(v, o, 64);
-      if (lane == 0)
-        dens[pt] += scl2 * v;
+      v[u] = wl * (c.x * c.x + c.y * c.y);
     }
+    double v4[4], v2[2];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const double send = h5 ? v[j] : v[j + 4], keep = h5 ? v[j + 4] : v[j];
+      v4[j] = keep + __shfl_xor(send, 32, 64);
+    }
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      const double send = h4 ? v4[j] : v4[j + 2], keep = h4 ? v4[j + 2] : v4[j];
+      v2[j] = keep + __shfl_xor(send, 16, 64);
+    }
+    double v1 = (h3 ? v2[1] : v2[0]) + __shfl_xor(h3 ? v2[0] : v2[1], 8, 64);
+    v1 += __shfl_xor(v1, 4, 64);
+    v1 += __shfl_xor(v1, 2, 64);
+    v1 += __shfl_xor(v1, 1, 64);
+    if (owner)
+      dens[ptm] = dold + scl2 * v1;
   }
 }
 
