@@ -469,10 +469,14 @@ __device__ __forceinline__ void sphere_reduce_one(const cplx<double> *__restrict
     return;
   cplx<double> c = mk<double>(0, 0), sn = mk<double>(0, 0);
   const cplx<double> *ps = partial + ((long)s * nparts * NB + b) * PMAX;
+  // the partner slot of m > 0 (m = 0 reads its own slot again and leaves
+  // the sum unused), loaded without a branch and unrolled, so that several
+  // parts' loads are in flight together; the sums keep their order
+  const int jo = m > 0 ? j - 2 * m : j;
+#pragma unroll 4
   for (int q = 0; q < nparts; q++) {
     c = c + ps[(long)q * NB * PMAX + j];
-    if (m > 0)
-      sn = sn + ps[(long)q * NB * PMAX + j - 2 * m];
+    sn = sn + ps[(long)q * NB * PMAX + jo];
   }
   cplx<double> *o = out + (long)b * ptot + proj_off[s];
   if (m == 0) {
